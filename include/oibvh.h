@@ -1,0 +1,184 @@
+/*
+ * oibvh.h -- C ABI of the MI355X (gfx950) oibvh collision library
+ * (liboibvh_hip.so).
+ *
+ * Drop-in boundary for the reference's OibvhTree (include/cuda/oibvhTree.cuh:41-174,
+ * src/cuda/oibvhTree.cu) and Scene (include/cuda/scene.cuh:26-182, src/cuda/scene.cu)
+ * classes. The reference exposes plain C++ classes with void methods; here every
+ * entry point returns an oibvh_status and nothing throws across the ABI. Handles
+ * are opaque; the library copies caller data into device memory it owns unless a
+ * *_BORROW flag says otherwise. All device work of a context runs on the
+ * context's stream, in call order.
+ */
+#ifndef OIBVH_H
+#define OIBVH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(_WIN32)
+#define OIBVH_API __declspec(dllexport)
+#else
+#define OIBVH_API __attribute__((visibility("default")))
+#endif
+
+typedef enum {
+    OIBVH_OK = 0,
+    OIBVH_ERR_INVALID = 1,    /* bad argument (null handle, zero faces, index out of range, ...) */
+    OIBVH_ERR_NOT_BUILT = 2,  /* tree used before build (the reference asserts, scene.cu:97) */
+    OIBVH_ERR_HIP = 3,        /* a HIP runtime call failed; see oibvh_last_error */
+    OIBVH_ERR_CAPACITY = 4,   /* caller buffer too small; the required size is reported */
+    OIBVH_ERR_OVERFLOW = 5,   /* device work queue exceeded its maximum size */
+    OIBVH_ERR_TIMEOUT = 6,    /* traversal exceeded its time limit */
+    OIBVH_ERR_NO_DEVICE = 7   /* no usable gfx950 device */
+} oibvh_status;
+
+typedef struct oibvh_ctx oibvh_ctx;
+typedef struct oibvh_tree oibvh_tree;
+typedef struct oibvh_scene oibvh_scene;
+
+/* pointer kinds for positions / indices */
+#define OIBVH_PTR_HOST 0u          /* host memory, copied */
+#define OIBVH_PTR_DEVICE 1u        /* device memory, copied device-to-device */
+#define OIBVH_PTR_DEVICE_BORROW 2u /* device memory used in place until replaced; caller keeps it alive */
+
+/* detect flags */
+#define OIBVH_DETECT_SELF 1u          /* also collide every mesh with itself (pairs sharing a vertex skipped) */
+#define OIBVH_DETECT_FP64_FALLBACK 2u /* FP32 SAT with forward error filter, FP64 re-check of uncertain axes */
+#define OIBVH_DETECT_ASYNC 4u         /* enqueue only; results are checked at the next get/sync call */
+
+/* ---------------------------------------------------------------- context */
+
+/* Context on HIP device `device` with stream `stream` (a hipStream_t; NULL =
+ * the library creates one at default priority). No reference counterpart: the
+ * reference uses cudaSetDevice and the default stream (scene.cu:229). */
+OIBVH_API oibvh_status oibvh_ctx_create(int device, void* stream, oibvh_ctx** out);
+OIBVH_API void oibvh_ctx_destroy(oibvh_ctx* ctx);
+/* text of the last error on this context ("" if none) */
+OIBVH_API const char* oibvh_last_error(const oibvh_ctx* ctx);
+/* wait for all work queued on the context's stream */
+OIBVH_API oibvh_status oibvh_ctx_synchronize(oibvh_ctx* ctx);
+/* the context's hipStream_t */
+OIBVH_API void* oibvh_ctx_stream(oibvh_ctx* ctx);
+
+/* ---------------------------------------------------------------- trees */
+
+/* OibvhTree(std::shared_ptr<Mesh>) (oibvhTree.cuh:50, oibvhTree.cu:9-16 with
+ * setup() :157-191): a tree over V vertices (xyz, 3 floats each) and F faces
+ * (idx, 3 vertex indices each). Nothing is built yet. */
+OIBVH_API oibvh_status oibvh_tree_create(oibvh_ctx* ctx, const float* xyz, uint32_t V, const uint32_t* idx,
+                                         uint32_t F, uint32_t ptr_kind, oibvh_tree** out);
+
+/* OibvhTree(std::shared_ptr<OibvhTree> other, std::shared_ptr<Mesh>)
+ * (oibvhTree.cuh:57, oibvhTree.cu:17-33): a tree over a mesh with other's
+ * topology that reuses other's sorted faces and build state; new vertex
+ * positions xyz (V must equal other's). Call refit before use. */
+OIBVH_API oibvh_status oibvh_tree_create_like(oibvh_tree* other, const float* xyz, uint32_t V, uint32_t ptr_kind,
+                                              oibvh_tree** out);
+
+OIBVH_API void oibvh_tree_destroy(oibvh_tree* tree);
+
+/* OibvhTree::build() (oibvhTree.cu:237-388): Morton30 codes of the leaf-box
+ * centroids in the vertex AABB, stable radix sort, bottom-up AABB reduce. Uses
+ * the tree's current positions. */
+OIBVH_API oibvh_status oibvh_tree_build(oibvh_tree* tree);
+
+/* OibvhTree::refit() (oibvhTree.cu:193-235): recompute every AABB over the
+ * fixed sorted face order. xyz == NULL refits the current positions, else the
+ * positions are replaced first (ptr_kind as in oibvh_tree_create). */
+OIBVH_API oibvh_status oibvh_tree_refit(oibvh_tree* tree, const float* xyz, uint32_t ptr_kind);
+
+/* OibvhTree::getDepth() (oibvhTree.cuh:87, oibvhTree.cu:46-49): ilog2(node count) */
+OIBVH_API oibvh_status oibvh_tree_get_depth(const oibvh_tree* tree, uint32_t* depth);
+/* OibvhTree::getPrimCount() (oibvhTree.cuh:93, oibvhTree.cu:51-54) */
+OIBVH_API oibvh_status oibvh_tree_get_prim_count(const oibvh_tree* tree, uint32_t* count);
+/* number of tree nodes, oibvh_get_size (oibvh.cuh:56-59) */
+OIBVH_API oibvh_status oibvh_tree_get_node_count(const oibvh_tree* tree, uint32_t* count);
+OIBVH_API oibvh_status oibvh_tree_get_vertex_count(const oibvh_tree* tree, uint32_t* count);
+
+/* The tree's AABBs, 6 floats per node (min xyz, max xyz) in real (level-major)
+ * order -- the content of OibvhTree::m_aabbTree (oibvhTree.cuh:127) after
+ * build/refit. cap = capacity of out in nodes; *n receives the node count.
+ * Synchronises the stream. */
+OIBVH_API oibvh_status oibvh_tree_get_nodes(oibvh_tree* tree, float* out, uint64_t cap, uint32_t* n);
+
+/* Faces in leaf order, 3 indices each -- OibvhTree::m_faces after build
+ * (oibvhTree.cu:377). cap in faces. Synchronises. */
+OIBVH_API oibvh_status oibvh_tree_get_sorted_faces(oibvh_tree* tree, uint32_t* out, uint64_t cap);
+
+/* Leaf position -> original face id (no reference accessor: the reference drops
+ * the permutation). cap in faces. Synchronises. */
+OIBVH_API oibvh_status oibvh_tree_get_perm(oibvh_tree* tree, uint32_t* out, uint64_t cap);
+
+/* Current vertex positions (V*3 floats). Synchronises. */
+OIBVH_API oibvh_status oibvh_tree_get_positions(oibvh_tree* tree, float* out, uint64_t cap);
+
+/* Device pointer of the tree's current positions (V*3 floats), for callers
+ * that update vertices in place on the context's stream. */
+OIBVH_API oibvh_status oibvh_tree_positions_device(oibvh_tree* tree, float** dptr);
+
+/* ---------------------------------------------------------------- scenes */
+
+/* Scene() (scene.cuh:32, scene.cu:12-34) */
+OIBVH_API oibvh_status oibvh_scene_create(oibvh_ctx* ctx, oibvh_scene** out);
+OIBVH_API void oibvh_scene_destroy(oibvh_scene* scene);
+
+/* Scene::addOibvhTree (scene.cuh:44, scene.cu:95-129): the tree must be built
+ * (OIBVH_ERR_NOT_BUILT otherwise). Mesh ids in results are the insertion order. */
+OIBVH_API oibvh_status oibvh_scene_add_tree(oibvh_scene* scene, oibvh_tree* tree);
+
+/* Scene::detectCollision(DeviceType, entryLevel, expandLevels) (scene.cuh:53-55,
+ * scene.cu:157-185, 225-446): broad phase over every mesh pair i < j (and, with
+ * OIBVH_DETECT_SELF, every mesh with itself) and narrow phase. entryLevel sets
+ * the initial BVTT front (scene.cu:192-223); expandLevels the levels each side
+ * descends per step (values above 3 act as 3). Neither changes the result.
+ * Synchronous unless OIBVH_DETECT_ASYNC; retries internally with larger
+ * buffers when the hit list or work queue overflows. */
+OIBVH_API oibvh_status oibvh_scene_detect(oibvh_scene* scene, uint32_t entry_level, uint32_t expand_levels,
+                                          uint32_t flags);
+
+/* Scene::getIntTriPairCount (scene.cuh:61, scene.cu:50-53) */
+OIBVH_API oibvh_status oibvh_scene_get_pair_count(oibvh_scene* scene, uint32_t* count);
+
+/* Colliding triangle pairs, 4 uint32 each: meshA, meshB (meshA < meshB, or equal
+ * in self mode), original face id in A, original face id in B; sorted
+ * lexicographically. The reference keeps them in Scene::m_intTriPairs
+ * (scene.cuh:133) with sorted-order face ids and has no getter. cap in pairs;
+ * *n receives the pair count (OIBVH_ERR_CAPACITY if it exceeds cap). */
+OIBVH_API oibvh_status oibvh_scene_get_pairs(oibvh_scene* scene, uint32_t* out, uint64_t cap, uint32_t* n);
+
+/* Device pointer of the unsorted hit list (uint4 per pair, as above) and its
+ * length, valid until the next detect. Synchronises. */
+OIBVH_API oibvh_status oibvh_scene_hits_device(oibvh_scene* scene, void** dptr, uint32_t* n);
+
+typedef struct {
+    uint64_t bvtt_tests;    /* BVTT node pairs tested */
+    uint64_t candidates;    /* leaf pairs sent to the narrow phase */
+    uint64_t fp64_rechecks; /* SAT axis comparisons re-evaluated in FP64 */
+    uint64_t adjacent;      /* self-mode leaf pairs skipped for sharing a vertex */
+    uint32_t pairs;         /* colliding pairs */
+    uint32_t retries;       /* detect re-runs after buffer growth */
+} oibvh_detect_stats;
+
+/* counters of the last detect (no reference counterpart; the reference prints
+ * counts, scene.cu:318-321, 417-421). Synchronises. */
+OIBVH_API oibvh_status oibvh_scene_get_stats(oibvh_scene* scene, oibvh_detect_stats* stats);
+
+/* Tuning knobs (no reference counterpart; none changes results):
+ *   reduce_leaves_per_thread  1, 2, 4 or 8 (0 = automatic): subtree size of the
+ *                             build/refit reduce is 256 * this many leaves
+ *   traverse_blocks           persistent traversal grid (0 = CUs x occupancy)
+ *   queue_capacity            initial work-ring entries (power of two, 0 = default 2^22)
+ *   hits_capacity             initial hit-list entries (0 = default 2^20)
+ * Both buffers grow on overflow (the detect is re-run). */
+OIBVH_API oibvh_status oibvh_ctx_set_tuning(oibvh_ctx* ctx, uint32_t reduce_leaves_per_thread,
+                                            uint32_t traverse_blocks, uint32_t queue_capacity,
+                                            uint32_t hits_capacity);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OIBVH_H */

@@ -1,0 +1,230 @@
+// oibvh.hpp -- C++ facade with the reference's class interface over the C ABI.
+//
+// OibvhTree mirrors include/cuda/oibvhTree.cuh:41-174 and Scene mirrors
+// include/cuda/scene.cuh:26-182 (minus the OpenGL draw paths). Differences a
+// maintainer switching over should know:
+//   * failures throw oibvh::Error (the reference asserts or does nothing);
+//   * refit() reads the positions of the Mesh it was constructed with, as the
+//     reference does (oibvhTree.cu:196-199);
+//   * getIntTriPairs() returns the colliding pairs (the reference keeps them
+//     private), with original face ids, sorted;
+//   * one process-wide context (device + stream) per GPU replaces
+//     cudaSetDevice + the default stream.
+#pragma once
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "oibvh.h"
+
+namespace oibvh {
+
+class Error : public std::runtime_error {
+public:
+    Error(oibvh_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+    oibvh_status status;
+};
+
+// DeviceType (scene.cuh:12-24)
+enum class DeviceType { CPU = -1, GPU0, GPU1, GPU2, GPU3, GPU4, GPU5, GPU6, GPU7, GPU8 };
+
+// The parts of the reference Mesh (mesh.h:171-187) the path reads: vertex
+// positions (3 floats each) and triangle indices (3 per face).
+struct Mesh {
+    std::vector<float> positions;
+    std::vector<uint32_t> indices;
+    uint32_t vertexCount() const { return (uint32_t)(positions.size() / 3); }
+    uint32_t faceCount() const { return (uint32_t)(indices.size() / 3); }
+};
+
+// int_tri_pair_node_t (utils.h:57-61) with original face ids
+struct IntTriPair {
+    uint32_t bvhIndex[2];
+    uint32_t triIndex[2];
+};
+
+class Context {
+public:
+    explicit Context(int device) : device_(device)
+    {
+        oibvh_status s = oibvh_ctx_create(device, nullptr, &h_);
+        if (s != OIBVH_OK) throw Error(s, "oibvh_ctx_create(" + std::to_string(device) + ") failed");
+    }
+    ~Context() { oibvh_ctx_destroy(h_); }
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+    oibvh_ctx* handle() const { return h_; }
+    int device() const { return device_; }
+    void check(oibvh_status s, const char* what) const
+    {
+        if (s != OIBVH_OK) throw Error(s, std::string(what) + ": " + oibvh_last_error(h_));
+    }
+    static std::shared_ptr<Context> get(int device)
+    {
+        static std::mutex mu;
+        static std::map<int, std::weak_ptr<Context>> reg;
+        std::lock_guard<std::mutex> lk(mu);
+        std::shared_ptr<Context> c = reg[device].lock();
+        if (!c) {
+            c = std::make_shared<Context>(device);
+            reg[device] = c;
+        }
+        return c;
+    }
+
+private:
+    int device_;
+    oibvh_ctx* h_ = nullptr;
+};
+
+class OibvhTree {
+public:
+    OibvhTree() = delete;
+
+    // OibvhTree(std::shared_ptr<Mesh>) (oibvhTree.cuh:50); `device` picks the GPU
+    explicit OibvhTree(const std::shared_ptr<Mesh>& mesh, int device = 0)
+        : ctx_(Context::get(device)), mesh_(mesh)
+    {
+        ctx_->check(oibvh_tree_create(ctx_->handle(), mesh->positions.data(), mesh->vertexCount(),
+                                      mesh->indices.data(), mesh->faceCount(), OIBVH_PTR_HOST, &h_),
+                    "OibvhTree");
+    }
+
+    // OibvhTree(std::shared_ptr<OibvhTree> other, std::shared_ptr<Mesh>) (oibvhTree.cuh:57)
+    OibvhTree(const std::shared_ptr<OibvhTree>& other, const std::shared_ptr<Mesh>& mesh)
+        : ctx_(other->ctx_), mesh_(mesh)
+    {
+        ctx_->check(oibvh_tree_create_like(other->h_, mesh->positions.data(), mesh->vertexCount(), OIBVH_PTR_HOST,
+                                           &h_),
+                    "OibvhTree(copy)");
+    }
+
+    ~OibvhTree() { oibvh_tree_destroy(h_); }
+    OibvhTree(const OibvhTree&) = delete;
+    OibvhTree& operator=(const OibvhTree&) = delete;
+
+    // build() (oibvhTree.cu:237-388)
+    void build() { ctx_->check(oibvh_tree_build(h_), "OibvhTree::build"); }
+
+    // refit() (oibvhTree.cu:193-235): takes the mesh's current positions
+    void refit()
+    {
+        ctx_->check(oibvh_tree_refit(h_, mesh_->positions.data(), OIBVH_PTR_HOST), "OibvhTree::refit");
+    }
+
+    unsigned int getDepth() const
+    {
+        uint32_t d = 0;
+        ctx_->check(oibvh_tree_get_depth(h_, &d), "getDepth");
+        return d;
+    }
+
+    unsigned int getPrimCount() const
+    {
+        uint32_t n = 0;
+        ctx_->check(oibvh_tree_get_prim_count(h_, &n), "getPrimCount");
+        return n;
+    }
+
+    // m_aabbTree: 6 floats per node, real order
+    std::vector<float> aabbTree() const
+    {
+        uint32_t n = 0;
+        ctx_->check(oibvh_tree_get_node_count(h_, &n), "node count");
+        std::vector<float> out(6ull * n);
+        ctx_->check(oibvh_tree_get_nodes(h_, out.data(), n, &n), "aabbTree");
+        return out;
+    }
+
+    // m_faces after build: faces in leaf order
+    std::vector<uint32_t> sortedFaces() const
+    {
+        std::vector<uint32_t> out(3ull * getPrimCount());
+        ctx_->check(oibvh_tree_get_sorted_faces(h_, out.data(), getPrimCount()), "sortedFaces");
+        return out;
+    }
+
+    // leaf position -> original face id
+    std::vector<uint32_t> sortedTriIndex() const
+    {
+        std::vector<uint32_t> out(getPrimCount());
+        ctx_->check(oibvh_tree_get_perm(h_, out.data(), out.size()), "sortedTriIndex");
+        return out;
+    }
+
+    oibvh_tree* handle() const { return h_; }
+    const std::shared_ptr<Context>& context() const { return ctx_; }
+
+private:
+    std::shared_ptr<Context> ctx_;
+    std::shared_ptr<Mesh> mesh_;
+    oibvh_tree* h_ = nullptr;
+};
+
+class Scene {
+public:
+    // Scene() (scene.cuh:32); `device` picks the GPU
+    explicit Scene(int device = 0) : ctx_(Context::get(device))
+    {
+        ctx_->check(oibvh_scene_create(ctx_->handle(), &h_), "Scene");
+    }
+    ~Scene() { oibvh_scene_destroy(h_); }
+    Scene(const Scene&) = delete;
+    Scene& operator=(const Scene&) = delete;
+
+    // addOibvhTree (scene.cuh:44); the tree must be built
+    void addOibvhTree(std::shared_ptr<OibvhTree> tree)
+    {
+        ctx_->check(oibvh_scene_add_tree(h_, tree->handle()), "Scene::addOibvhTree");
+        trees_.push_back(std::move(tree));
+    }
+
+    // detectCollision (scene.cuh:53-55); DeviceType::CPU is the reference's
+    // empty TODO (scene.cu:187-190) and is rejected here
+    void detectCollision(const DeviceType deviceType = DeviceType::GPU0, const unsigned int entryLevel = 0,
+                         const unsigned int expandLevels = 1, uint32_t flags = 0)
+    {
+        if (deviceType == DeviceType::CPU || (int)deviceType != ctx_->device())
+            throw Error(OIBVH_ERR_INVALID, "detectCollision: scene lives on GPU" + std::to_string(ctx_->device()));
+        ctx_->check(oibvh_scene_detect(h_, entryLevel, expandLevels, flags), "Scene::detectCollision");
+    }
+
+    // getIntTriPairCount (scene.cuh:61)
+    unsigned int getIntTriPairCount() const
+    {
+        uint32_t n = 0;
+        ctx_->check(oibvh_scene_get_pair_count(h_, &n), "getIntTriPairCount");
+        return n;
+    }
+
+    std::vector<IntTriPair> getIntTriPairs() const
+    {
+        const uint32_t n = getIntTriPairCount();
+        std::vector<IntTriPair> out(n);
+        uint32_t got = 0;
+        if (n)
+            ctx_->check(oibvh_scene_get_pairs(h_, reinterpret_cast<uint32_t*>(out.data()), n, &got),
+                        "getIntTriPairs");
+        out.resize(got);
+        return out;
+    }
+
+    oibvh_detect_stats stats() const
+    {
+        oibvh_detect_stats s{};
+        ctx_->check(oibvh_scene_get_stats(h_, &s), "stats");
+        return s;
+    }
+
+private:
+    std::shared_ptr<Context> ctx_;
+    oibvh_scene* h_ = nullptr;
+    std::vector<std::shared_ptr<OibvhTree>> trees_;
+};
+
+}  // namespace oibvh

@@ -1,0 +1,264 @@
+"""Python mirror of the reference's OibvhTree / Scene interface over the C ABI.
+
+Names and argument meanings follow include/cuda/oibvhTree.cuh:41-174 and
+include/cuda/scene.cuh:26-182; errors raise OibvhError instead of asserting.
+Mesh data are numpy arrays: xyz float32 (V, 3), faces uint32 (F, 3).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _check(ctx_h, st: int, what: str) -> None:
+    if st != L.OK:
+        msg = L.load().oibvh_last_error(ctx_h) if ctx_h else b""
+        raise L.OibvhError(st, f"{what}: {msg.decode() if msg else ''}")
+
+
+def _ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+class Context:
+    """Device + stream that every tree and scene of this context uses."""
+
+    def __init__(self, device: int = 0):
+        self._lib = L.load()
+        h = C.c_void_p()
+        st = self._lib.oibvh_ctx_create(device, None, C.byref(h))
+        if st != L.OK:
+            raise L.OibvhError(st, f"oibvh_ctx_create(device={device})")
+        self.h = h
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self._lib.oibvh_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def stream(self) -> int:
+        return self._lib.oibvh_ctx_stream(self.h) or 0
+
+    def synchronize(self) -> None:
+        _check(self.h, self._lib.oibvh_ctx_synchronize(self.h), "synchronize")
+
+    def set_tuning(self, leaves_per_thread: int = 0, traverse_blocks: int = 0, queue_capacity: int = 0,
+                   hits_capacity: int = 0) -> None:
+        _check(self.h, self._lib.oibvh_ctx_set_tuning(self.h, leaves_per_thread, traverse_blocks, queue_capacity,
+                                                      hits_capacity), "set_tuning")
+
+
+class DeviceBuffer:
+    """Plain hipMalloc'd buffer (harness plumbing: device-resident inputs)."""
+
+    def __init__(self, nbytes: int):
+        H = L.hip()
+        p = C.c_void_p()
+        L.hip_check(H.hipMalloc(C.byref(p), max(1, nbytes)), "hipMalloc")
+        self.ptr = p
+        self.nbytes = nbytes
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        L.hip_check(L.hip().hipMemcpy(b.ptr, _ptr(a), a.nbytes, 1), "hipMemcpy H2D")
+        return b
+
+    def upload(self, a: np.ndarray, offset: int = 0) -> None:
+        a = np.ascontiguousarray(a)
+        L.hip_check(L.hip().hipMemcpy(C.c_void_p(self.ptr.value + offset), _ptr(a), a.nbytes, 1), "hipMemcpy H2D")
+
+    def at(self, offset: int) -> int:
+        return self.ptr.value + offset
+
+    def free(self) -> None:
+        if getattr(self, "ptr", None) and self.ptr.value:
+            L.hip().hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        self.free()
+
+
+class OibvhTree:
+    """OibvhTree (oibvhTree.cuh:41-174)."""
+
+    def __init__(self, ctx: Context, xyz=None, faces=None, *, _handle=None, _V=0, _F=0):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        if _handle is not None:
+            self.h, self.V, self.F = _handle, _V, _F
+            return
+        xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+        faces = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 3)
+        h = C.c_void_p()
+        _check(ctx.h, self._lib.oibvh_tree_create(ctx.h, _ptr(xyz), xyz.shape[0], _ptr(faces), faces.shape[0],
+                                                  L.PTR_HOST, C.byref(h)), "oibvh_tree_create")
+        self.h, self.V, self.F = h, xyz.shape[0], faces.shape[0]
+
+    @classmethod
+    def like(cls, other: "OibvhTree", xyz) -> "OibvhTree":
+        """OibvhTree(shared_ptr<OibvhTree> other, shared_ptr<Mesh>) (oibvhTree.cu:17-33)."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+        h = C.c_void_p()
+        _check(other.ctx.h, other._lib.oibvh_tree_create_like(other.h, _ptr(xyz), xyz.shape[0], L.PTR_HOST,
+                                                              C.byref(h)), "oibvh_tree_create_like")
+        return cls(other.ctx, _handle=h, _V=other.V, _F=other.F)
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self._lib.oibvh_tree_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def build(self) -> None:
+        _check(self.ctx.h, self._lib.oibvh_tree_build(self.h), "build")
+
+    def refit(self, xyz=None) -> None:
+        """refit() with the current positions, or with new host positions."""
+        if xyz is None:
+            _check(self.ctx.h, self._lib.oibvh_tree_refit(self.h, None, L.PTR_HOST), "refit")
+        else:
+            xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+            if xyz.shape[0] != self.V:
+                raise ValueError("vertex count changed")
+            _check(self.ctx.h, self._lib.oibvh_tree_refit(self.h, _ptr(xyz), L.PTR_HOST), "refit")
+
+    def refit_device(self, dptr: int, borrow: bool = True) -> None:
+        kind = L.PTR_DEVICE_BORROW if borrow else L.PTR_DEVICE
+        _check(self.ctx.h, self._lib.oibvh_tree_refit(self.h, C.c_void_p(dptr), kind), "refit")
+
+    def _u32(self, fn) -> int:
+        v = C.c_uint32()
+        _check(self.ctx.h, fn(self.h, C.byref(v)), fn.__name__)
+        return v.value
+
+    def getDepth(self) -> int:
+        return self._u32(self._lib.oibvh_tree_get_depth)
+
+    def getPrimCount(self) -> int:
+        return self._u32(self._lib.oibvh_tree_get_prim_count)
+
+    @property
+    def node_count(self) -> int:
+        return self._u32(self._lib.oibvh_tree_get_node_count)
+
+    def nodes(self) -> np.ndarray:
+        n = self.node_count
+        out = np.empty((n, 6), np.float32)
+        got = C.c_uint32()
+        _check(self.ctx.h, self._lib.oibvh_tree_get_nodes(self.h, out.ctypes.data_as(L._f32p), n, C.byref(got)),
+               "get_nodes")
+        return out
+
+    def sorted_faces(self) -> np.ndarray:
+        out = np.empty((self.F, 3), np.uint32)
+        _check(self.ctx.h, self._lib.oibvh_tree_get_sorted_faces(self.h, out.ctypes.data_as(L._u32p), self.F),
+               "get_sorted_faces")
+        return out
+
+    def perm(self) -> np.ndarray:
+        out = np.empty(self.F, np.uint32)
+        _check(self.ctx.h, self._lib.oibvh_tree_get_perm(self.h, out.ctypes.data_as(L._u32p), self.F), "get_perm")
+        return out
+
+    def positions(self) -> np.ndarray:
+        out = np.empty((self.V, 3), np.float32)
+        _check(self.ctx.h, self._lib.oibvh_tree_get_positions(self.h, out.ctypes.data_as(L._f32p), 3 * self.V),
+               "get_positions")
+        return out
+
+    def positions_device(self) -> int:
+        p = C.c_void_p()
+        _check(self.ctx.h, self._lib.oibvh_tree_positions_device(self.h, C.byref(p)), "positions_device")
+        return p.value
+
+
+class Scene:
+    """Scene (scene.cuh:26-182)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        h = C.c_void_p()
+        _check(ctx.h, self._lib.oibvh_scene_create(ctx.h, C.byref(h)), "oibvh_scene_create")
+        self.h = h
+        self.trees: list[OibvhTree] = []
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self._lib.oibvh_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def addOibvhTree(self, tree: OibvhTree) -> None:
+        _check(self.ctx.h, self._lib.oibvh_scene_add_tree(self.h, tree.h), "addOibvhTree")
+        self.trees.append(tree)
+
+    def detectCollision(self, entry_level: int = 0, expand_levels: int = 1, *, self_collide: bool = False,
+                        fp64_fallback: bool = False, asynchronous: bool = False) -> None:
+        flags = ((L.DETECT_SELF if self_collide else 0) | (L.DETECT_FP64_FALLBACK if fp64_fallback else 0)
+                 | (L.DETECT_ASYNC if asynchronous else 0))
+        _check(self.ctx.h, self._lib.oibvh_scene_detect(self.h, entry_level, expand_levels, flags), "detectCollision")
+
+    def getIntTriPairCount(self) -> int:
+        v = C.c_uint32()
+        _check(self.ctx.h, self._lib.oibvh_scene_get_pair_count(self.h, C.byref(v)), "getIntTriPairCount")
+        return v.value
+
+    def getIntTriPairs(self) -> np.ndarray:
+        """(n, 4) uint32 rows (meshA, meshB, origTriA, origTriB), sorted."""
+        n = self.getIntTriPairCount()
+        out = np.empty((max(n, 1), 4), np.uint32)
+        got = C.c_uint32()
+        _check(self.ctx.h, self._lib.oibvh_scene_get_pairs(self.h, out.ctypes.data_as(L._u32p), max(n, 1),
+                                                           C.byref(got)), "get_pairs")
+        return out[: got.value]
+
+    def stats(self) -> dict:
+        s = L.DetectStats()
+        _check(self.ctx.h, self._lib.oibvh_scene_get_stats(self.h, C.byref(s)), "get_stats")
+        return {f: getattr(s, f) for f, _ in L.DetectStats._fields_}
+
+
+class Timer:
+    """HIP events recorded on the library's stream."""
+
+    def __init__(self, ctx: Context):
+        H = L.hip()
+        self.H = H
+        self.stream = C.c_void_p(ctx.stream)
+        self.a, self.b = C.c_void_p(), C.c_void_p()
+        L.hip_check(H.hipEventCreate(C.byref(self.a)), "hipEventCreate")
+        L.hip_check(H.hipEventCreate(C.byref(self.b)), "hipEventCreate")
+
+    def start(self) -> None:
+        L.hip_check(self.H.hipEventRecord(self.a, self.stream), "hipEventRecord")
+
+    def stop(self) -> None:
+        L.hip_check(self.H.hipEventRecord(self.b, self.stream), "hipEventRecord")
+
+    def ms(self) -> float:
+        L.hip_check(self.H.hipEventSynchronize(self.b), "hipEventSynchronize")
+        v = C.c_float()
+        L.hip_check(self.H.hipEventElapsedTime(C.byref(v), self.a, self.b), "hipEventElapsedTime")
+        return v.value
+
+    def __del__(self):
+        try:
+            self.H.hipEventDestroy(self.a)
+            self.H.hipEventDestroy(self.b)
+        except Exception:
+            pass

@@ -1,0 +1,595 @@
+// oibvh_build.hip -- gfx950 kernels for oibvh tree build and refit.
+//
+// Build (OibvhTree::build, oibvhTree.cu:237-388):
+//   k_mesh_box     vertex AABB (the Morton quantisation box, Mesh::setupAABB
+//                  mesh.cpp:91-98); also clears the sort's scratch counters
+//   k_morton       leaf AABB centroid -> 30-bit Morton key per face
+//                  (calculate_aabb_and_morton_kernel, oibvh.cu:42-70) and the
+//                  four 8-bit digit histograms of the keys
+//   k_sort_pass x4 onesweep LSD radix pass with decoupled look-back; stable,
+//                  so the permutation equals thrust::stable_sort_by_key's
+//                  (oibvhTree.cu:287-297)
+//   k_reduce       leaves in sorted order + bottom-up AABB reduce, one subtree
+//                  per workgroup and the last workgroup finishing the apex
+//                  (oibvh_tree_construction_kernel2, oibvh.cu:156-219)
+// Refit (OibvhTree::refit, oibvhTree.cu:193-235) is k_reduce alone over the
+// already sorted faces (calculate_aabb_kernel + oibvh_tree_construction_kernel,
+// oibvh.cu:24-40, 80-154).
+//
+// Compiled with -ffp-contract=off. min/max are compare-selects with glm's tie
+// rules: min(a,b) = b < a ? b : a, max(a,b) = a < b ? b : a.
+#include <hip/hip_runtime.h>
+
+#include "oibvh_common.h"
+
+#define OI_BLOCK 256
+
+__device__ __forceinline__ float gmin(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float gmax(float a, float b) { return (a < b) ? b : a; }
+
+__device__ __forceinline__ uint32_t ld_acq_relaxed(const uint32_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_relaxed(uint32_t* p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------------
+// block-wide exclusive scan of one uint32 per thread (256 threads, wave64)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wsum, uint32_t& total)
+{
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(x, o, 64);
+        if (lane >= o) x += t;
+    }
+    if (lane == 63) s_wsum[wave] = x;
+    __syncthreads();
+    uint32_t w0 = s_wsum[0], w1 = s_wsum[1], w2 = s_wsum[2], w3 = s_wsum[3];
+    uint32_t before = (wave > 0 ? w0 : 0u) + (wave > 1 ? w1 : 0u) + (wave > 2 ? w2 : 0u);
+    total = w0 + w1 + w2 + w3;
+    __syncthreads();
+    return before + x - v;
+}
+
+// ---------------------------------------------------------------------------
+// k_mesh_box: vertex AABB with the sequential fold's exact result.
+// The fold acc = glm::min(v, acc) keeps the later value on ties, so partial
+// results combine as lo(L, R) = L < R ? L : R and hi(L, R) = R < L ? L : R in
+// index order; +-inf are the identities, the default box (+-FLT_MAX) is folded
+// in first, as aabb_box_t's initial value (utils.h:17-18).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float lo_c(float l, float r) { return (l < r) ? l : r; }
+__device__ __forceinline__ float hi_c(float l, float r) { return (r < l) ? l : r; }
+
+__global__ void __launch_bounds__(OI_BLOCK)
+k_mesh_box(const float* __restrict__ xyz, uint32_t V, uint32_t chunk, float* __restrict__ partials,
+           uint32_t* __restrict__ counter, float* __restrict__ box_out, uint32_t* __restrict__ clear_u32,
+           uint32_t clear_n)
+{
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = blockIdx.x * OI_BLOCK + tid; i < clear_n; i += gridDim.x * OI_BLOCK) clear_u32[i] = 0u;
+
+    __shared__ float s_v[6][OI_BLOCK];
+    __shared__ uint32_t s_last;
+    const float INF = __builtin_inff();
+    float mn[3] = {INF, INF, INF}, mx[3] = {-INF, -INF, -INF};
+    const uint32_t b0 = blockIdx.x * chunk;
+    const uint32_t b1 = min(V, b0 + chunk);
+    const uint32_t sub = (chunk + OI_BLOCK - 1) / OI_BLOCK;
+    const uint32_t s0 = b0 + tid * sub;
+    const uint32_t s1 = min(b1, s0 + sub);
+    for (uint32_t v = s0; v < s1; v++) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            float x = xyz[3u * v + k];
+            mn[k] = lo_c(mn[k], x);
+            mx[k] = hi_c(mx[k], x);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        s_v[k][tid] = mn[k];
+        s_v[3 + k][tid] = mx[k];
+    }
+    __syncthreads();
+    for (uint32_t s = 1; s < OI_BLOCK; s <<= 1) {
+        if ((tid & (2u * s - 1u)) == 0) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                s_v[k][tid] = lo_c(s_v[k][tid], s_v[k][tid + s]);
+                s_v[3 + k][tid] = hi_c(s_v[3 + k][tid], s_v[3 + k][tid + s]);
+            }
+        }
+        __syncthreads();
+    }
+    if (tid < 6) partials[6u * blockIdx.x + tid] = s_v[tid][0];
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t old = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = (old == gridDim.x - 1u);
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    // last block: fold the partials in block order
+    const uint32_t nb = gridDim.x;
+    const uint32_t psub = (nb + OI_BLOCK - 1) / OI_BLOCK;
+    float pm[3] = {INF, INF, INF}, pM[3] = {-INF, -INF, -INF};
+    for (uint32_t b = tid * psub; b < min(nb, tid * psub + psub); b++) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            pm[k] = lo_c(pm[k], partials[6u * b + k]);
+            pM[k] = hi_c(pM[k], partials[6u * b + 3 + k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        s_v[k][tid] = pm[k];
+        s_v[3 + k][tid] = pM[k];
+    }
+    __syncthreads();
+    for (uint32_t s = 1; s < OI_BLOCK; s <<= 1) {
+        if ((tid & (2u * s - 1u)) == 0) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                s_v[k][tid] = lo_c(s_v[k][tid], s_v[k][tid + s]);
+                s_v[3 + k][tid] = hi_c(s_v[3 + k][tid], s_v[3 + k][tid + s]);
+            }
+        }
+        __syncthreads();
+    }
+    if (tid < 3) {
+        box_out[tid] = lo_c(__FLT_MAX__, s_v[tid][0]);
+        box_out[3 + tid] = hi_c(-__FLT_MAX__, s_v[3 + tid][0]);
+    }
+    if (tid == 0) *counter = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// k_morton: Morton30 of each face's leaf-box centroid (oibvh.cu:6-22, 42-70)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t spread10(uint32_t x)
+{
+    x &= 0x3ffu;
+    x = (x | (x << 16)) & 0x30000ffu;
+    x = (x | (x << 8)) & 0x300f00fu;
+    x = (x | (x << 4)) & 0x30c30c3u;
+    x = (x | (x << 2)) & 0x9249249u;
+    return x;
+}
+
+// thrust::min(thrust::max(p * 1024, 0), 1023) -> unsigned; NaN -> 0
+__device__ __forceinline__ uint32_t quantise10(float p)
+{
+    float q = p * 1024.0f;
+    q = (q < 0.0f) ? 0.0f : q;
+    q = (1023.0f < q) ? 1023.0f : q;
+    return (q != q) ? 0u : (uint32_t)q;
+}
+
+__global__ void __launch_bounds__(OI_BLOCK)
+k_morton(const uint32_t* __restrict__ faces, const float* __restrict__ xyz, uint32_t F,
+         const float* __restrict__ box, uint32_t* __restrict__ keys, uint32_t* __restrict__ ghist)
+{
+    __shared__ uint32_t s_h[4][256];
+    const uint32_t tid = threadIdx.x;
+#pragma unroll
+    for (int d = 0; d < 4; d++) s_h[d][tid] = 0u;
+    __syncthreads();
+    const float bmn[3] = {box[0], box[1], box[2]};
+    const float bmx[3] = {box[3], box[4], box[5]};
+    for (uint32_t i = blockIdx.x * OI_BLOCK + tid; i < F; i += gridDim.x * OI_BLOCK) {
+        const uint32_t f0 = faces[3u * i], f1 = faces[3u * i + 1], f2 = faces[3u * i + 2];
+        uint32_t q[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const float a = xyz[3u * f0 + k], b = xyz[3u * f1 + k], c = xyz[3u * f2 + k];
+            const float lo = gmin(gmin(a, b), c);
+            const float hi = gmax(gmax(a, b), c);
+            const float cen = (lo + hi) * 0.5f;
+            const float off = cen - bmn[k];
+            const float len = bmx[k] - bmn[k];
+            q[k] = quantise10(off / len);
+        }
+        const uint32_t key = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+        keys[i] = key;
+#pragma unroll
+        for (int d = 0; d < 4; d++) atomicAdd(&s_h[d][(key >> (8 * d)) & 0xffu], 1u);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+        uint32_t c = s_h[d][tid];
+        if (c) atomicAdd(&ghist[256 * d + tid], c);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_sort_pass: one stable 8-bit LSD pass, onesweep style.
+// A tile is 4 waves x 16 rounds x 64 lanes = 4096 keys in input order (wave w
+// owns keys [1024w, 1024w + 1024) of the tile, round j lane l holds key
+// 1024w + 64j + l). Keys are ranked per wave with 8 ballots per round
+// (stable: earlier rounds and lower lanes come first), staged through LDS in
+// tile-local sorted order and written out in runs. Tiles take their id from an
+// atomic counter, so every tile a look-back waits on is already running.
+// Look-back status word: bits 31:30 = 1 aggregate / 2 inclusive prefix,
+// bits 29:0 = count.
+// ---------------------------------------------------------------------------
+#define RS_ITEMS 16
+#define RS_TILE (OI_BLOCK * RS_ITEMS)
+#define LB_AGG (1u << 30)
+#define LB_PRE (2u << 30)
+#define LB_CNT 0x3fffffffu
+
+__global__ void __launch_bounds__(OI_BLOCK)
+k_sort_pass(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
+            uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, uint32_t n, uint32_t shift,
+            const uint32_t* __restrict__ ghist, uint32_t* lookback, uint32_t* lookback_next,
+            uint32_t* tile_counter)
+{
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_whist[4][256];
+    __shared__ uint32_t s_lstart[256];
+    __shared__ uint32_t s_gbase[256];
+    __shared__ uint32_t s_wsum[4];
+    __shared__ uint32_t s_keys[RS_TILE];
+    __shared__ uint32_t s_vals[RS_TILE];
+
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    if (tid == 0) s_tile = __hip_atomic_fetch_add(tile_counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int w = 0; w < 4; w++) s_whist[w][tid] = 0u;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t base = tile * RS_TILE;
+    if (lookback_next) lookback_next[tile * 256u + tid] = 0u;
+
+    // exclusive prefix of the global digit histogram
+    uint32_t gtot;
+    const uint32_t gexcl = block_excl_scan(ghist[tid], s_wsum, gtot);
+
+    uint32_t k[RS_ITEMS], v[RS_ITEMS], rk[RS_ITEMS];
+    const uint32_t wbase = base + wave * (64u * RS_ITEMS);
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; j++) {
+        const uint32_t idx = wbase + 64u * j + lane;
+        const bool ok = idx < n;
+        k[j] = ok ? keys_in[idx] : 0u;
+        v[j] = ok ? (vals_in ? vals_in[idx] : idx) : 0u;
+    }
+    const unsigned long long lt = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; j++) {
+        const uint32_t idx = wbase + 64u * j + lane;
+        const bool ok = idx < n;
+        const uint32_t d = (k[j] >> shift) & 0xffu;
+        unsigned long long peers = __ballot(ok);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const bool bit = (d >> b) & 1u;
+            const unsigned long long m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t prev = s_whist[wave][d];
+        const uint32_t cnt = (uint32_t)__popcll(peers);
+        rk[j] = prev + (uint32_t)__popcll(peers & lt);
+        const bool leader = ok && ((peers & lt) == 0ull);
+        if (leader) s_whist[wave][d] = prev + cnt;
+    }
+    __syncthreads();
+    // per-wave exclusive offsets and tile totals (thread tid owns digit tid)
+    uint32_t tot = 0;
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        const uint32_t c = s_whist[w][tid];
+        s_whist[w][tid] = tot;
+        tot += c;
+    }
+    // publish this tile's aggregate (or inclusive prefix for tile 0)
+    uint32_t* my = lookback + tile * 256u + tid;
+    if (tile == 0)
+        st_relaxed(my, LB_PRE | tot);
+    else
+        st_relaxed(my, LB_AGG | tot);
+    uint32_t ltot;
+    const uint32_t lstart = block_excl_scan(tot, s_wsum, ltot);
+    s_lstart[tid] = lstart;
+    // decoupled look-back over preceding tiles for digit tid
+    uint32_t excl = 0;
+    if (tile > 0) {
+        int32_t i = (int32_t)tile - 1;
+        while (i >= 0) {
+            const uint32_t s = ld_acq_relaxed(lookback + (uint32_t)i * 256u + tid);
+            const uint32_t fl = s & ~LB_CNT;
+            if (fl == 0u) {
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            excl += s & LB_CNT;
+            if (fl == LB_PRE) break;
+            i--;
+        }
+        st_relaxed(my, LB_PRE | (excl + tot));
+    }
+    s_gbase[tid] = gexcl + excl;
+    __syncthreads();
+    // stage in tile-local sorted order
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; j++) {
+        const uint32_t idx = wbase + 64u * j + lane;
+        if (idx < n) {
+            const uint32_t d = (k[j] >> shift) & 0xffu;
+            const uint32_t lp = s_lstart[d] + s_whist[wave][d] + rk[j];
+            s_keys[lp] = k[j];
+            s_vals[lp] = v[j];
+        }
+    }
+    __syncthreads();
+    const uint32_t cnt_tile = (n - base < RS_TILE) ? (n - base) : RS_TILE;
+    for (uint32_t i = tid; i < cnt_tile; i += OI_BLOCK) {
+        const uint32_t key = s_keys[i];
+        const uint32_t d = (key >> shift) & 0xffu;
+        const uint32_t g = s_gbase[d] + (i - s_lstart[d]);
+        if (keys_out) keys_out[g] = key;
+        vals_out[g] = s_vals[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_reduce: leaves + bottom-up reduce.
+// Stage 1: workgroup w owns the subtree of height h = min(L, 8 + log2 M)
+// rooted at level L - h, position w (its 2^h leaves). Each thread computes M
+// consecutive leaves and folds them log2(M) levels in registers; the 256
+// results climb the remaining levels through LDS (ping-pong buffers, one
+// barrier per level). Every real node is written once, at its real index.
+// Stage 2: the last workgroup to finish (device-scope counter) reads the
+// stage-1 roots and climbs to the root in chunks of up to 2^11 nodes.
+// A parent is real iff its left child is; with no real right child the
+// parent equals the left child (oibvh.cu:139-146).
+// ---------------------------------------------------------------------------
+struct Box6 {
+    float v[6];
+};
+
+__device__ __forceinline__ void box_merge_into(Box6& a, const Box6& r)
+{
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        a.v[k] = gmin(a.v[k], r.v[k]);
+        a.v[3 + k] = gmax(a.v[3 + k], r.v[3 + k]);
+    }
+}
+
+__device__ __forceinline__ void store_node(float* nodes, uint32_t r, const Box6& b)
+{
+    float2* p = reinterpret_cast<float2*>(nodes + 6u * r);
+    p[0] = make_float2(b.v[0], b.v[1]);
+    p[1] = make_float2(b.v[2], b.v[3]);
+    p[2] = make_float2(b.v[4], b.v[5]);
+}
+
+__device__ __forceinline__ Box6 load_node(const float* nodes, uint32_t r)
+{
+    const float2* p = reinterpret_cast<const float2*>(nodes + 6u * r);
+    const float2 a = p[0], b = p[1], c = p[2];
+    Box6 o;
+    o.v[0] = a.x;
+    o.v[1] = a.y;
+    o.v[2] = b.x;
+    o.v[3] = b.y;
+    o.v[4] = c.x;
+    o.v[5] = c.y;
+    return o;
+}
+
+struct ReduceArgs {
+    const float* pos;
+    const uint32_t* faces;  // sorted faces (refit) or original faces (build)
+    const uint32_t* perm;   // build: leaf p takes face perm[p]; refit: nullptr
+    uint32_t* sfaces_out;   // build: sorted faces written here; refit: nullptr
+    float* nodes;
+    uint32_t* counter;      // zero between launches (self-resetting)
+    uint32_t F, L, vl;
+};
+
+struct LeafLoad {
+    const ReduceArgs& a;
+    uint32_t leaf_start;
+    __device__ __forceinline__ Box6 operator()(uint32_t p) const
+    {
+        const uint32_t fi = a.perm ? a.perm[p] : p;
+        const uint32_t f0 = a.faces[3u * fi], f1 = a.faces[3u * fi + 1], f2 = a.faces[3u * fi + 2];
+        if (a.sfaces_out) {
+            a.sfaces_out[3u * p] = f0;
+            a.sfaces_out[3u * p + 1] = f1;
+            a.sfaces_out[3u * p + 2] = f2;
+        }
+        Box6 b;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const float x0 = a.pos[3u * f0 + k], x1 = a.pos[3u * f1 + k], x2 = a.pos[3u * f2 + k];
+            b.v[k] = gmin(gmin(x0, x1), x2);
+            b.v[3 + k] = gmax(gmax(x0, x1), x2);
+        }
+        store_node(a.nodes, leaf_start + p, b);
+        return b;
+    }
+};
+
+struct NodeLoad {
+    const float* nodes;
+    uint32_t lvl_start;
+    __device__ __forceinline__ Box6 operator()(uint32_t p) const { return load_node(nodes, lvl_start + p); }
+};
+
+// Reduce the subtree whose children level is lc, h levels up, root position w
+// at level lc - h. Requires 2^h <= 256 * M. Returns with every node of the
+// subtree written; ends with a barrier.
+template <int M, class Load>
+__device__ void reduce_subtree(const Load& load, float* nodes, uint32_t L, uint32_t vl, uint32_t lc, uint32_t h,
+                               uint32_t w, float (*sA)[OI_BLOCK], float (*sB)[OI_BLOCK])
+{
+    constexpr uint32_t LGM = (M == 1) ? 0 : (M == 2) ? 1 : (M == 4) ? 2 : 3;
+    const uint32_t tid = threadIdx.x;
+    const bool act = (tid * M) < (1u << h);
+    const uint32_t c0 = (w << h) + tid * M;
+    Box6 b[M];
+    uint32_t ncnt = oi_lvl_cnt(lc, L, vl);
+#pragma unroll
+    for (int j = 0; j < M; j++)
+        if (act && c0 + j < ncnt) b[j] = load(c0 + j);
+    const uint32_t hl = (h < LGM) ? h : LGM;
+    uint32_t l = lc;
+#pragma unroll
+    for (uint32_t s = 1; s <= LGM; s++) {
+        if (s <= hl) {
+            const uint32_t cc = ncnt;  // real count of the child level
+            l -= 1;
+            ncnt = oi_lvl_cnt(l, L, vl);
+            const uint32_t st = oi_lvl_start(l, L, vl);
+#pragma unroll
+            for (uint32_t j = 0; j < (uint32_t)(M >> s); j++) {
+                const uint32_t q = (c0 >> s) + j;
+                if (act && 2u * q < cc) {
+                    Box6 x = b[2 * j];
+                    if (2u * q + 1u < cc) box_merge_into(x, b[2 * j + 1]);
+                    b[j] = x;
+                    store_node(nodes, st + q, x);
+                }
+            }
+        }
+    }
+    const uint32_t hs = h - hl;
+    if (hs == 0) {
+        __syncthreads();
+        return;
+    }
+    // LDS climb: thread t holds node t of level l within the subtree
+    if (act) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) sA[k][tid] = b[0].v[k];
+    }
+    __syncthreads();
+    float(*cur)[OI_BLOCK] = sA;
+    float(*nxt)[OI_BLOCK] = sB;
+    for (uint32_t s = 1; s <= hs; s++) {
+        const uint32_t cc = ncnt;
+        l -= 1;
+        ncnt = oi_lvl_cnt(l, L, vl);
+        const uint32_t width = 1u << (hs - s);
+        const uint32_t q = (w << (hs - s)) + tid;
+        if (tid < width && 2u * q < cc) {
+            Box6 x, y;
+#pragma unroll
+            for (int k = 0; k < 6; k++) x.v[k] = cur[k][2 * tid];
+            if (2u * q + 1u < cc) {
+#pragma unroll
+                for (int k = 0; k < 6; k++) y.v[k] = cur[k][2 * tid + 1];
+                box_merge_into(x, y);
+            }
+#pragma unroll
+            for (int k = 0; k < 6; k++) nxt[k][tid] = x.v[k];
+            store_node(nodes, oi_lvl_start(l, L, vl) + q, x);
+        }
+        __syncthreads();
+        float(*t)[OI_BLOCK] = cur;
+        cur = nxt;
+        nxt = t;
+    }
+}
+
+template <int M>
+__global__ void __launch_bounds__(OI_BLOCK) k_reduce(ReduceArgs a)
+{
+    __shared__ float sA[6][OI_BLOCK];
+    __shared__ float sB[6][OI_BLOCK];
+    __shared__ uint32_t s_last;
+    constexpr uint32_t LGM = (M == 1) ? 0 : (M == 2) ? 1 : (M == 4) ? 2 : 3;
+    const uint32_t L = a.L, vl = a.vl;
+    const uint32_t h = (L < 8 + LGM) ? L : 8 + LGM;
+    const uint32_t w = blockIdx.x;
+    LeafLoad ll{a, oi_lvl_start(L, L, vl)};
+    reduce_subtree<M>(ll, a.nodes, L, vl, L, h, w, sA, sB);
+    uint32_t lc = L - h;
+    if (lc == 0) return;
+    // the stage-1 root of this workgroup was stored by thread 0
+    __threadfence();
+    if (threadIdx.x == 0) {
+        const uint32_t old =
+            __hip_atomic_fetch_add(a.counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = (old == gridDim.x - 1u);
+        if (s_last) *a.counter = 0u;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    while (lc > 0) {
+        const uint32_t h2 = (lc < 11u) ? lc : 11u;
+        const uint32_t nchunk = oi_lvl_cnt(lc - h2, L, vl);
+        NodeLoad nl{a.nodes, oi_lvl_start(lc, L, vl)};
+        for (uint32_t c = 0; c < nchunk; c++) reduce_subtree<8>(nl, a.nodes, L, vl, lc, h2, c, sA, sB);
+        lc -= h2;
+        __threadfence_block();
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers (called from oibvh_host.cpp)
+// ---------------------------------------------------------------------------
+extern "C" {
+
+hipError_t oi_launch_mesh_box(hipStream_t st, const float* xyz, uint32_t V, float* partials, uint32_t nblocks,
+                              uint32_t* counter, float* box_out, uint32_t* clear_u32, uint32_t clear_n)
+{
+    const uint32_t chunk = (V + nblocks - 1) / nblocks;
+    hipLaunchKernelGGL(k_mesh_box, dim3(nblocks), dim3(OI_BLOCK), 0, st, xyz, V, chunk, partials, counter, box_out,
+                       clear_u32, clear_n);
+    return hipGetLastError();
+}
+
+hipError_t oi_launch_morton(hipStream_t st, const uint32_t* faces, const float* xyz, uint32_t F, const float* box,
+                            uint32_t* keys, uint32_t* ghist, uint32_t nblocks)
+{
+    hipLaunchKernelGGL(k_morton, dim3(nblocks), dim3(OI_BLOCK), 0, st, faces, xyz, F, box, keys, ghist);
+    return hipGetLastError();
+}
+
+uint32_t oi_sort_tile() { return RS_TILE; }
+
+hipError_t oi_launch_sort_pass(hipStream_t st, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+                               uint32_t* vals_out, uint32_t n, uint32_t shift, const uint32_t* ghist,
+                               uint32_t* lookback, uint32_t* lookback_next, uint32_t* tile_counter)
+{
+    const uint32_t tiles = (n + RS_TILE - 1) / RS_TILE;
+    hipLaunchKernelGGL(k_sort_pass, dim3(tiles), dim3(OI_BLOCK), 0, st, keys_in, vals_in, keys_out, vals_out, n,
+                       shift, ghist, lookback, lookback_next, tile_counter);
+    return hipGetLastError();
+}
+
+// M = leaves per thread in stage 1 (1, 2, 4 or 8)
+hipError_t oi_launch_reduce(hipStream_t st, int M, const float* pos, const uint32_t* faces, const uint32_t* perm,
+                            uint32_t* sfaces_out, float* nodes, uint32_t* counter, uint32_t F, uint32_t L,
+                            uint32_t vl)
+{
+    ReduceArgs a{pos, faces, perm, sfaces_out, nodes, counter, F, L, vl};
+    uint32_t lgm = (M == 1) ? 0 : (M == 2) ? 1 : (M == 4) ? 2 : 3;
+    uint32_t h = (L < 8 + lgm) ? L : 8 + lgm;
+    uint32_t grid = oi_lvl_cnt(L - h, L, vl);
+    switch (M) {
+    case 1: hipLaunchKernelGGL(k_reduce<1>, dim3(grid), dim3(OI_BLOCK), 0, st, a); break;
+    case 2: hipLaunchKernelGGL(k_reduce<2>, dim3(grid), dim3(OI_BLOCK), 0, st, a); break;
+    case 4: hipLaunchKernelGGL(k_reduce<4>, dim3(grid), dim3(OI_BLOCK), 0, st, a); break;
+    default: hipLaunchKernelGGL(k_reduce<8>, dim3(grid), dim3(OI_BLOCK), 0, st, a); break;
+    }
+    return hipGetLastError();
+}
+
+}  // extern "C"

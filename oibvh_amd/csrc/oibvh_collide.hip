@@ -1,0 +1,816 @@
+// oibvh_collide.hip -- gfx950 broad + narrow phase for Scene::detectCollision.
+//
+// The reference relaunches traversal_kernel once per BVTT level with a host
+// round trip for the front size (scene.cu:272-311, collide.cu:76-246) and then
+// runs triangle_intersect_kernel over the candidate list (collide.cu:248-300).
+// Here one persistent launch does both:
+//   * each workgroup keeps a BVTT stack in LDS and pops 256 entries per step
+//     (one per thread): AABB overlap test (collide.cu:12-17), leaf x leaf ->
+//     candidate, otherwise both sides are expanded `expand` levels (a leaf
+//     stands for itself, collide.cu:165-176) and the children are pushed back
+//     with a block prefix sum;
+//   * a global ring queue takes spills and donations to idle workgroups
+//     (work stealing); a `pending` counter of live entries decides termination;
+//   * candidates are compacted into an LDS buffer and tested 256 at a time
+//     with the 17-axis SAT (utils.cpp:71-169 / cuda_intersect_tritri.h:350-434),
+//     hits appended with one atomic per batch as (meshA, meshB, origTriA,
+//     origTriB) -- original face ids, mapped through the build permutation.
+// Self mode (no reference counterpart) also walks each mesh against itself:
+// a node paired with itself expands to the unordered pairs of its children and
+// leaf pairs sharing a vertex index are dropped before the narrow phase.
+//
+// BVTT entries are two 32-bit implicit indices in the scene's concatenated
+// implicit space (mesh m spans [ibase_m, ibase_m + 2^(L_m+1) - 1)).
+#include <hip/hip_runtime.h>
+
+#include "oibvh_common.h"
+
+#define TB 256
+#define STACK_CAP 3072
+#define CAND_CAP 512
+#define MESH_LDS 16
+
+typedef unsigned long long u64;
+
+__device__ __forceinline__ u64 ld64(const u64* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ void st64(u64* p, u64 v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ int ld32i(const int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ uint32_t ld32(const uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+// Reserve `need` ring slots; fails only when the ring would exceed capacity
+// (entries reserved by pushers minus entries reserved by poppers).
+__device__ bool ring_reserve(oi_trav_ctl* ctl, uint32_t cap, uint32_t need, u64& base)
+{
+    for (;;) {
+        const u64 t = ld64(&ctl->tail);
+        const u64 h = ld64(&ctl->head);
+        if (t - h + need > cap) return false;
+        u64 expected = t;
+        if (__hip_atomic_compare_exchange_strong(&ctl->tail, &expected, t + need, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            base = t;
+            return true;
+        }
+    }
+}
+
+// Slot hand-over. A reserved slot is written / emptied promptly by its owner,
+// so these waits end; the wall-clock limit only guards against a fault.
+__device__ __forceinline__ bool slot_put(u64* slot, u64 v, oi_trav_ctl* ctl, u64 t0, u64 limit)
+{
+    while (ld64(slot) != OI_EMPTY_SLOT) {
+        __builtin_amdgcn_s_sleep(1);
+        if (wall_clock64() - t0 > limit) {
+            __hip_atomic_fetch_or(&ctl->error, OI_ERR_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+    }
+    st64(slot, v);
+    return true;
+}
+
+__device__ __forceinline__ bool slot_take(u64* slot, u64& v, oi_trav_ctl* ctl, u64 t0, u64 limit)
+{
+    while ((v = ld64(slot)) == OI_EMPTY_SLOT) {
+        __builtin_amdgcn_s_sleep(1);
+        if (wall_clock64() - t0 > limit) {
+            __hip_atomic_fetch_or(&ctl->error, OI_ERR_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+    }
+    st64(slot, OI_EMPTY_SLOT);
+    return true;
+}
+
+__device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t* s_wsum, uint32_t& total)
+{
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(x, o, 64);
+        if (lane >= o) x += t;
+    }
+    if (lane == 63) s_wsum[wave] = x;
+    __syncthreads();
+    uint32_t w0 = s_wsum[0], w1 = s_wsum[1], w2 = s_wsum[2], w3 = s_wsum[3];
+    uint32_t before = (wave > 0 ? w0 : 0u) + (wave > 1 ? w1 : 0u) + (wave > 2 ? w2 : 0u);
+    total = w0 + w1 + w2 + w3;
+    __syncthreads();
+    return before + x - v;
+}
+
+// ---------------------------------------------------------------------------
+// SAT, FP32 (utils.cpp:71-169; glm cross/dot op order, no contraction)
+// ---------------------------------------------------------------------------
+struct v3f {
+    float x, y, z;
+};
+struct v3d {
+    double x, y, z;
+};
+
+template <class V>
+__device__ __forceinline__ V vsub(V a, V b)
+{
+    return V{a.x - b.x, a.y - b.y, a.z - b.z};
+}
+template <class V>
+__device__ __forceinline__ V vcross(V a, V b)
+{
+    return V{a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+}
+template <class T, class V>
+__device__ __forceinline__ T vdot(V a, V b)
+{
+    const T x = a.x * b.x, y = a.y * b.y, z = a.z * b.z;
+    return (x + y) + z;
+}
+
+// the 17 axes in the reference's test order (utils.cpp:146-168)
+enum { AX_N1, AX_M1, AX_E11, AX_E12, AX_E13, AX_E21, AX_E22, AX_E23, AX_E31, AX_E32, AX_E33,
+       AX_G1, AX_G2, AX_G3, AX_H1, AX_H2, AX_H3 };
+
+template <class T, class V>
+struct SatFrame {
+    V p2, p3, q1, q2, q3;
+    V e1, e2, e3, f1, f2, f3;
+    V n1, m1;
+    __device__ __forceinline__ void init(const V* P, const V* Q)
+    {
+        const V p1 = V{0, 0, 0};
+        p2 = vsub(P[1], P[0]);
+        p3 = vsub(P[2], P[0]);
+        q1 = vsub(Q[0], P[0]);
+        q2 = vsub(Q[1], P[0]);
+        q3 = vsub(Q[2], P[0]);
+        e1 = vsub(p2, p1);
+        e2 = vsub(p3, p2);
+        e3 = vsub(p1, p3);
+        f1 = vsub(q2, q1);
+        f2 = vsub(q3, q2);
+        f3 = vsub(q1, q3);
+        n1 = vcross(e1, e2);
+        m1 = vcross(f1, f2);
+    }
+    __device__ __forceinline__ V axis(int k) const
+    {
+        switch (k) {
+        case AX_N1: return n1;
+        case AX_M1: return m1;
+        case AX_E11: return vcross(e1, f1);
+        case AX_E12: return vcross(e1, f2);
+        case AX_E13: return vcross(e1, f3);
+        case AX_E21: return vcross(e2, f1);
+        case AX_E22: return vcross(e2, f2);
+        case AX_E23: return vcross(e2, f3);
+        case AX_E31: return vcross(e3, f1);
+        case AX_E32: return vcross(e3, f2);
+        case AX_E33: return vcross(e3, f3);
+        case AX_G1: return vcross(e1, n1);
+        case AX_G2: return vcross(e2, n1);
+        case AX_G3: return vcross(e3, n1);
+        case AX_H1: return vcross(f1, m1);
+        case AX_H2: return vcross(f2, m1);
+        default: return vcross(f3, m1);
+        }
+    }
+};
+
+__device__ __forceinline__ float fmx(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ float fmn(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ double fmx(double a, double b) { return fmax(a, b); }
+__device__ __forceinline__ double fmn(double a, double b) { return fmin(a, b); }
+
+// project6 (utils.cpp:71-95): 1 if the axis does NOT separate
+template <class T, class V>
+__device__ __forceinline__ bool project6(const SatFrame<T, V>& s, V ax)
+{
+    const V p1 = V{0, 0, 0};
+    const T P1 = vdot<T>(ax, p1), P2 = vdot<T>(ax, s.p2), P3 = vdot<T>(ax, s.p3);
+    const T Q1 = vdot<T>(ax, s.q1), Q2 = vdot<T>(ax, s.q2), Q3 = vdot<T>(ax, s.q3);
+    const T mx1 = fmx(fmx(P1, P2), P3), mn1 = fmn(fmn(P1, P2), P3);
+    const T mx2 = fmx(fmx(Q1, Q2), Q3), mn2 = fmn(fmn(Q1, Q2), Q3);
+    if (mn1 > mx2) return false;
+    if (mn2 > mx1) return false;
+    return true;
+}
+
+__device__ bool sat_f32(const v3f* P, const v3f* Q)
+{
+    SatFrame<float, v3f> s;
+    s.init(P, Q);
+    for (int k = 0; k < 17; k++)
+        if (!project6<float, v3f>(s, s.axis(k))) return false;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// FP32 filter with FP64 re-check (precision form).
+// Every FP32 quantity z carries a magnitude S(z) >= |z|, |exact z| built by
+// the same expression over |operands| (sums of magnitudes for +/-, products
+// for *). A value at rounding depth c has |z - exact z| <= c * 2^-24 * S(z)
+// (+ O(2^-48)); the deepest SAT projection (a g/h axis) has c = 14. A
+// comparison mn1 > mx2 is decided in FP32 only when |mn1 - mx2| exceeds
+// T = 32 * 2^-24 * (S_P + S_Q) + 2^-100, which covers the FP32 error of both
+// sides, the FP64 error of the oracle's own evaluation and the rounding of T;
+// then the FP32 decision equals the exact one and the FP64 one. Otherwise the
+// axis is evaluated again in double exactly as the FP64 oracle evaluates it.
+// Non-finite magnitudes never pass the test, so they also go to FP64.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ v3f vabs(v3f a) { return v3f{fabsf(a.x), fabsf(a.y), fabsf(a.z)}; }
+__device__ __forceinline__ v3f vaddm(v3f a, v3f b) { return v3f{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ v3f vcrossm(v3f a, v3f b)
+{
+    return v3f{a.y * b.z + b.y * a.z, a.z * b.x + b.z * a.x, a.x * b.y + b.x * a.y};
+}
+__device__ __forceinline__ float vdotm(v3f a, v3f b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+
+struct SatMag {
+    v3f p2, p3, q1, q2, q3, e1, e2, e3, f1, f2, f3, n1, m1;
+    __device__ __forceinline__ void init(const v3f* P, const v3f* Q)
+    {
+        const v3f a0 = vabs(P[0]);
+        p2 = vaddm(vabs(P[1]), a0);
+        p3 = vaddm(vabs(P[2]), a0);
+        q1 = vaddm(vabs(Q[0]), a0);
+        q2 = vaddm(vabs(Q[1]), a0);
+        q3 = vaddm(vabs(Q[2]), a0);
+        e1 = p2;
+        e2 = vaddm(p3, p2);
+        e3 = p3;
+        f1 = vaddm(q2, q1);
+        f2 = vaddm(q3, q2);
+        f3 = vaddm(q1, q3);
+        n1 = vcrossm(e1, e2);
+        m1 = vcrossm(f1, f2);
+    }
+    __device__ __forceinline__ v3f axis(int k) const
+    {
+        switch (k) {
+        case AX_N1: return n1;
+        case AX_M1: return m1;
+        case AX_E11: return vcrossm(e1, f1);
+        case AX_E12: return vcrossm(e1, f2);
+        case AX_E13: return vcrossm(e1, f3);
+        case AX_E21: return vcrossm(e2, f1);
+        case AX_E22: return vcrossm(e2, f2);
+        case AX_E23: return vcrossm(e2, f3);
+        case AX_E31: return vcrossm(e3, f1);
+        case AX_E32: return vcrossm(e3, f2);
+        case AX_E33: return vcrossm(e3, f3);
+        case AX_G1: return vcrossm(e1, n1);
+        case AX_G2: return vcrossm(e2, n1);
+        case AX_G3: return vcrossm(e3, n1);
+        case AX_H1: return vcrossm(f1, m1);
+        case AX_H2: return vcrossm(f2, m1);
+        default: return vcrossm(f3, m1);
+        }
+    }
+};
+
+// returns 1 = not separated, 0 = separated; counts FP64 re-evaluations
+__device__ bool sat_filtered(const v3f* P, const v3f* Q, uint32_t& n_fp64)
+{
+    SatFrame<float, v3f> s;
+    s.init(P, Q);
+    SatMag m;
+    m.init(P, Q);
+    bool have_d = false;
+    SatFrame<double, v3d> sd;
+    for (int k = 0; k < 17; k++) {
+        const v3f ax = s.axis(k);
+        const v3f am = m.axis(k);
+        const v3f p1 = v3f{0, 0, 0};
+        const float P1 = vdot<float>(ax, p1), P2 = vdot<float>(ax, s.p2), P3 = vdot<float>(ax, s.p3);
+        const float Q1 = vdot<float>(ax, s.q1), Q2 = vdot<float>(ax, s.q2), Q3 = vdot<float>(ax, s.q3);
+        const float mx1 = fmaxf(fmaxf(P1, P2), P3), mn1 = fminf(fminf(P1, P2), P3);
+        const float mx2 = fmaxf(fmaxf(Q1, Q2), Q3), mn2 = fminf(fminf(Q1, Q2), Q3);
+        const float SP = fmaxf(vdotm(am, m.p2), vdotm(am, m.p3));
+        const float SQ = fmaxf(fmaxf(vdotm(am, m.q1), vdotm(am, m.q2)), vdotm(am, m.q3));
+        const float T = 32.0f * 5.9604645e-8f * (SP + SQ) + 7.888609e-31f;
+        const bool finite = (T < __builtin_inff());
+        // separation on this axis: mn1 > mx2 or mn2 > mx1
+        const bool sep1_sure = finite && (mn1 - mx2 > T);
+        const bool nsep1_sure = finite && (mx2 - mn1 > T);
+        const bool sep2_sure = finite && (mn2 - mx1 > T);
+        const bool nsep2_sure = finite && (mx1 - mn2 > T);
+        if (sep1_sure || sep2_sure) return false;
+        if (nsep1_sure && nsep2_sure) continue;
+        // uncertain: evaluate this axis exactly like the FP64 oracle
+        if (!have_d) {
+            v3d Pd[3], Qd[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                Pd[i] = v3d{(double)P[i].x, (double)P[i].y, (double)P[i].z};
+                Qd[i] = v3d{(double)Q[i].x, (double)Q[i].y, (double)Q[i].z};
+            }
+            sd.init(Pd, Qd);
+            have_d = true;
+        }
+        n_fp64++;
+        if (!project6<double, v3d>(sd, sd.axis(k))) return false;
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// traversal
+// ---------------------------------------------------------------------------
+struct TravArgs {
+    const oi_mesh_desc* meshes;
+    uint32_t n_meshes;
+    u64* ring;
+    uint32_t ring_mask;  // capacity - 1 (capacity is a power of two)
+    oi_trav_ctl* ctl;
+    uint4* hits;
+    uint32_t hits_cap;
+    uint32_t expand;
+    uint32_t flags;
+    u64 timeout_ticks;
+};
+
+struct MeshRef {
+    const float* nodes;
+    const uint32_t* sfaces;
+    const float* pos;
+    const uint32_t* perm;
+    uint32_t L, vl, ibase;
+};
+
+__device__ __forceinline__ uint32_t find_mesh(const oi_mesh_desc* md, const uint32_t* s_ibase, uint32_t n,
+                                              uint32_t g)
+{
+    uint32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        const uint32_t b = (n <= MESH_LDS) ? s_ibase[mid] : md[mid].ibase;
+        if (b <= g)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ MeshRef mesh_ref(const oi_mesh_desc* md, uint32_t m)
+{
+    const oi_mesh_desc& d = md[m];
+    return MeshRef{d.nodes, d.sfaces, d.pos, d.perm, d.L, d.vl, d.ibase};
+}
+
+__device__ __forceinline__ bool overlap6(const float* a, const float* b)
+{
+    return (a[0] <= b[3] && a[3] >= b[0]) && (a[1] <= b[4] && a[4] >= b[1]) && (a[2] <= b[5] && a[5] >= b[2]);
+}
+
+__device__ __forceinline__ void load_box(const float* nodes, uint32_t r, float* o)
+{
+    const float2* p = reinterpret_cast<const float2*>(nodes + 6u * r);
+    const float2 a = p[0], b = p[1], c = p[2];
+    o[0] = a.x;
+    o[1] = a.y;
+    o[2] = b.x;
+    o[3] = b.y;
+    o[4] = c.x;
+    o[5] = c.y;
+}
+
+// Expansion of one side: the descendants `d` levels below implicit node i
+// (d = min(expand, L - level)); start = 2^d (i + 1) - 1, count clipped at
+// the level's right-most real node (collide.cu:165-176).
+struct Side {
+    uint32_t start, count;
+};
+
+__device__ __forceinline__ Side expand_side(uint32_t i, uint32_t lvl, uint32_t L, uint32_t vl, uint32_t e)
+{
+    if (lvl == L) return Side{i, 1u};
+    const uint32_t d = (L - lvl < e) ? (L - lvl) : e;
+    const uint32_t nl = lvl + d;
+    const uint32_t st = ((i + 1u) << d) - 1u;
+    const uint32_t last = oi_lvl_last_implicit(nl, L, vl);
+    const uint32_t c = last - st + 1u;
+    return Side{st, c < (1u << d) ? c : (1u << d)};
+}
+
+// k-th unordered pair (x <= y) of n items, row-major: (0,0),(0,1)..(0,n-1),(1,1)..
+__device__ __forceinline__ void tri_pair(uint32_t k, uint32_t n, uint32_t& x, uint32_t& y)
+{
+    x = 0;
+    uint32_t row = n;
+    while (k >= row) {
+        k -= row;
+        x++;
+        row--;
+    }
+    y = x + k;
+}
+
+template <bool FP64>
+__device__ __forceinline__ void narrow_one(const TravArgs& A, const uint32_t* s_ibase, uint2 c, bool valid,
+                                           bool& hit, uint4& out, uint32_t& n_fp64, uint32_t& n_adj)
+{
+    hit = false;
+    if (!valid) return;
+    const uint32_t ma = find_mesh(A.meshes, s_ibase, A.n_meshes, c.x);
+    const uint32_t mb = find_mesh(A.meshes, s_ibase, A.n_meshes, c.y);
+    const MeshRef ra = mesh_ref(A.meshes, ma), rb = mesh_ref(A.meshes, mb);
+    uint32_t sa = c.x - ra.ibase - ((1u << ra.L) - 1u);
+    uint32_t sb = c.y - rb.ibase - ((1u << rb.L) - 1u);
+    uint32_t fa[3], fb[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        fa[k] = ra.sfaces[3u * sa + k];
+        fb[k] = rb.sfaces[3u * sb + k];
+    }
+    if (ma == mb) {
+        bool adj = false;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) adj |= (fa[i] == fb[j]);
+        if (adj) {
+            n_adj++;
+            return;
+        }
+    }
+    uint32_t oa = ra.perm[sa], ob = rb.perm[sb];
+    const float* pa = ra.pos;
+    const float* pb = rb.pos;
+    if (ma == mb && oa > ob) {
+        // inside one mesh P is the face with the smaller original id
+        uint32_t t = oa;
+        oa = ob;
+        ob = t;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            uint32_t u = fa[k];
+            fa[k] = fb[k];
+            fb[k] = u;
+        }
+    }
+    v3f P[3], Q[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        P[k] = v3f{pa[3u * fa[k]], pa[3u * fa[k] + 1], pa[3u * fa[k] + 2]};
+        Q[k] = v3f{pb[3u * fb[k]], pb[3u * fb[k] + 1], pb[3u * fb[k] + 2]};
+    }
+    const bool h = FP64 ? sat_filtered(P, Q, n_fp64) : sat_f32(P, Q);
+    if (h) {
+        hit = true;
+        out = make_uint4(ma, mb, oa, ob);
+    }
+}
+
+template <bool FP64>
+__global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
+{
+    __shared__ uint2 s_stack[STACK_CAP];
+    __shared__ uint2 s_cand[CAND_CAP];
+    __shared__ uint32_t s_ibase[MESH_LDS];
+    __shared__ uint32_t s_wsum[4];
+    __shared__ uint32_t s_n, s_cn, s_pop_n, s_flag;
+    __shared__ u64 s_pop_h, s_push_base;
+
+    const uint32_t tid = threadIdx.x;
+    oi_trav_ctl* ctl = A.ctl;
+    if (tid < MESH_LDS && tid < A.n_meshes) s_ibase[tid] = A.meshes[tid].ibase;
+    if (tid == 0) {
+        s_n = 0;
+        s_cn = 0;
+    }
+    __syncthreads();
+
+    u64 n_tests = 0, n_cands = 0;
+    uint32_t n_fp64 = 0, n_adj = 0;
+    bool idle = false;
+    const u64 t0 = wall_clock64();
+    const uint32_t cap = A.ring_mask + 1u;
+
+    for (;;) {
+        // ---- refill from the ring when the local stack is short
+        if (tid == 0) {
+            s_pop_n = 0;
+            s_flag = 0;
+            if (s_n < TB) {
+                uint32_t want = TB - s_n;
+                for (int attempt = 0; attempt < 8; attempt++) {
+                    const u64 h = ld64(&ctl->head);
+                    const u64 t = ld64(&ctl->tail);
+                    if (h >= t) break;
+                    const u64 avail = t - h;
+                    uint32_t take = (avail < want) ? (uint32_t)avail : want;
+                    if (take > 64 && s_n == 0 && avail < 4096) take = 64;
+                    u64 expected = h;
+                    if (__hip_atomic_compare_exchange_strong(&ctl->head, &expected, h + take, __ATOMIC_RELAXED,
+                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        s_pop_h = h;
+                        s_pop_n = take;
+                        break;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        const uint32_t pn = s_pop_n;
+        if (pn) {
+            const u64 h = s_pop_h;
+            for (uint32_t i = tid; i < pn; i += TB) {
+                u64* slot = A.ring + ((h + i) & A.ring_mask);
+                u64 v = 0;
+                slot_take(slot, v, ctl, t0, A.timeout_ticks);
+                s_stack[s_n + i] = make_uint2((uint32_t)v, (uint32_t)(v >> 32));
+            }
+            __syncthreads();
+            if (tid == 0) s_n += pn;
+            __syncthreads();
+        }
+        const uint32_t n = s_n;
+        if (n == 0) {
+            // ---- idle: finished when nothing is pending anywhere
+            if (tid == 0) {
+                if (!idle) __hip_atomic_fetch_add(&ctl->idle, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const int pend = ld32i(&ctl->pending);
+                const uint32_t err = ld32(&ctl->error);
+                uint32_t f = 0;
+                if (pend <= 0 || err) f = 1;
+                else if (wall_clock64() - t0 > A.timeout_ticks) {
+                    __hip_atomic_fetch_or(&ctl->error, OI_ERR_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    f = 1;
+                }
+                s_flag = f;
+            }
+            idle = true;
+            __syncthreads();
+            if (s_flag) break;
+            __builtin_amdgcn_s_sleep(8);
+            continue;
+        }
+        if (idle) {
+            if (tid == 0) __hip_atomic_fetch_sub(&ctl->idle, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            idle = false;
+        }
+
+        // ---- take one batch from the top of the stack
+        const uint32_t take = (n < TB) ? n : TB;
+        const bool have = tid < take;
+        uint2 e = have ? s_stack[n - take + tid] : make_uint2(0, 0);
+        __syncthreads();
+        const uint32_t nb = n - take;
+
+        // ---- test
+        uint32_t nchild = 0;
+        bool cand = false;
+        Side sa = {0, 0}, sb = {0, 0};
+        bool selfnode = false;
+        uint32_t baseA = 0, baseB = 0;
+        if (have) {
+            const uint32_t ma = find_mesh(A.meshes, s_ibase, A.n_meshes, e.x);
+            const uint32_t mb = find_mesh(A.meshes, s_ibase, A.n_meshes, e.y);
+            const MeshRef ra = mesh_ref(A.meshes, ma), rb = mesh_ref(A.meshes, mb);
+            baseA = ra.ibase;
+            baseB = rb.ibase;
+            const uint32_t ia = e.x - ra.ibase, ib = e.y - rb.ibase;
+            const uint32_t la = oi_ilog2(ia + 1u), lb = oi_ilog2(ib + 1u);
+            selfnode = (e.x == e.y);
+            bool ov = true;
+            if (!selfnode) {
+                float ba[6], bb[6];
+                load_box(ra.nodes, ia - oi_virt_above(la, ra.L, ra.vl), ba);
+                load_box(rb.nodes, ib - oi_virt_above(lb, rb.L, rb.vl), bb);
+                ov = overlap6(ba, bb);
+            }
+            if (ov) {
+                const bool leafA = (la == ra.L), leafB = (lb == rb.L);
+                if (leafA && leafB) {
+                    cand = !selfnode;
+                } else {
+                    sa = expand_side(ia, la, ra.L, ra.vl, A.expand);
+                    sb = expand_side(ib, lb, rb.L, rb.vl, A.expand);
+                    nchild = selfnode ? sa.count * (sa.count + 1u) / 2u : sa.count * sb.count;
+                }
+            }
+        }
+        n_tests += have ? 1u : 0u;
+
+        // ---- place children: block prefix sum; spill what does not fit in LDS
+        uint32_t T;
+        const uint32_t off = block_scan(nchild, s_wsum, T);
+        const uint32_t room = STACK_CAP - nb;
+        const uint32_t spill = (T > room) ? (T - room) : 0u;
+        if (tid == 0) {
+            s_flag = 0;
+            // count the children before any of them becomes visible to others
+            if (T) __hip_atomic_fetch_add(&ctl->pending, (int)T, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (spill) {
+                u64 b = 0;
+                if (!ring_reserve(ctl, cap, spill, b)) {
+                    __hip_atomic_fetch_or(&ctl->error, OI_ERR_QUEUE_OVERFLOW, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+                    s_flag = 1;
+                }
+                s_push_base = b;
+            }
+        }
+        __syncthreads();
+        if (s_flag) break;
+        for (uint32_t c = 0; c < nchild; c++) {
+            uint32_t x, y;
+            if (selfnode) {
+                tri_pair(c, sa.count, x, y);
+                x = sa.start + x;
+                y = sa.start + y;
+            } else {
+                x = sa.start + c / sb.count;
+                y = sb.start + c % sb.count;
+            }
+            const uint2 ce = make_uint2(baseA + x, baseB + y);
+            const uint32_t k = off + c;
+            if (k < spill) {
+                u64* slot = A.ring + ((s_push_base + k) & A.ring_mask);
+                slot_put(slot, (u64)ce.x | ((u64)ce.y << 32), ctl, t0, A.timeout_ticks);
+            } else {
+                s_stack[nb + k - spill] = ce;
+            }
+        }
+        // ---- candidates into the LDS buffer
+        uint32_t CT;
+        const uint32_t coff = block_scan(cand ? 1u : 0u, s_wsum, CT);
+        if (cand) s_cand[s_cn + coff] = e;
+        n_cands += cand ? 1u : 0u;
+        __syncthreads();
+        if (tid == 0) {
+            s_n = nb + T - spill;
+            s_cn += CT;
+            if (take) __hip_atomic_fetch_sub(&ctl->pending, (int)take, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+
+        // ---- narrow phase on a full batch of candidates
+        if (s_cn >= TB) {
+            const uint32_t cn = s_cn;
+            uint2 c = s_cand[cn - TB + tid];
+            bool hit;
+            uint4 out;
+            narrow_one<FP64>(A, s_ibase, c, true, hit, out, n_fp64, n_adj);
+            uint32_t HT;
+            const uint32_t hoff = block_scan(hit ? 1u : 0u, s_wsum, HT);
+            if (tid == 0) {
+                s_cn = cn - TB;
+                s_pop_h = HT ? __hip_atomic_fetch_add(&ctl->n_hits, HT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                             : 0;
+            }
+            __syncthreads();
+            if (hit) {
+                const uint32_t pos = (uint32_t)s_pop_h + hoff;
+                if (pos < A.hits_cap) A.hits[pos] = out;
+            }
+            __syncthreads();
+        }
+
+        // ---- donate the bottom half of a deep stack when others are idle
+        if (tid == 0) {
+            s_flag = 0;
+            if (s_n >= 2 * TB && ld32(&ctl->idle) > 0u) {
+                const uint32_t D = s_n / 2;
+                u64 b = 0;
+                if (ring_reserve(ctl, cap, D, b)) {
+                    s_push_base = b;
+                    s_flag = D;
+                }
+            }
+        }
+        __syncthreads();
+        if (s_flag) {
+            const uint32_t D = s_flag, m = s_n;
+            for (uint32_t i = tid; i < D; i += TB) {
+                const uint2 ce = s_stack[i];
+                u64* slot = A.ring + ((s_push_base + i) & A.ring_mask);
+                slot_put(slot, (u64)ce.x | ((u64)ce.y << 32), ctl, t0, A.timeout_ticks);
+            }
+            // shift the rest down chunk by chunk: chunk r's destination lies
+            // below every later chunk's source
+            __syncthreads();
+            for (uint32_t r = 0; D + r * TB < m; r++) {
+                const uint32_t i = D + r * TB + tid;
+                uint2 x = make_uint2(0, 0);
+                if (i < m) x = s_stack[i];
+                __syncthreads();
+                if (i < m) s_stack[i - D] = x;
+                __syncthreads();
+            }
+            if (tid == 0) s_n = m - D;
+            __syncthreads();
+        }
+    }
+
+    // ---- drain the remaining candidates
+    __syncthreads();
+    const uint32_t cn = s_cn;
+    if (cn) {
+        bool hit = false;
+        uint4 out;
+        uint2 c = (tid < cn) ? s_cand[tid] : make_uint2(0, 0);
+        narrow_one<FP64>(A, s_ibase, c, tid < cn, hit, out, n_fp64, n_adj);
+        uint32_t HT;
+        const uint32_t hoff = block_scan(hit ? 1u : 0u, s_wsum, HT);
+        if (tid == 0)
+            s_pop_h = HT ? __hip_atomic_fetch_add(&ctl->n_hits, HT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+        __syncthreads();
+        if (hit) {
+            const uint32_t pos = (uint32_t)s_pop_h + hoff;
+            if (pos < A.hits_cap) A.hits[pos] = out;
+        }
+    }
+    // ---- statistics (one atomic per workgroup per counter)
+    uint32_t t_;
+    const uint32_t f64tot = block_scan(n_fp64, s_wsum, t_);
+    (void)f64tot;
+    if (t_) {
+        if (tid == 0) __hip_atomic_fetch_add(&ctl->fp64, (u64)t_, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    uint32_t a_;
+    block_scan(n_adj, s_wsum, a_);
+    if (tid == 0) {
+        if (a_) __hip_atomic_fetch_add(&ctl->adj, (u64)a_, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    uint32_t te, ce_;
+    block_scan((uint32_t)n_tests, s_wsum, te);
+    block_scan((uint32_t)n_cands, s_wsum, ce_);
+    if (tid == 0) {
+        __hip_atomic_fetch_add(&ctl->tests, (u64)te, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(&ctl->cands, (u64)ce_, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(&ctl->exited, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// seed: reset the counters and place the initial front in the ring
+__global__ void k_seed(u64* ring, const uint2* front, uint32_t n0, oi_trav_ctl* ctl)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n0) ring[i] = (u64)front[i].x | ((u64)front[i].y << 32);
+    if (i == 0) {
+        ctl->head = 0;
+        ctl->tail = n0;
+        ctl->pending = (int)n0;
+        ctl->n_hits = 0;
+        ctl->error = 0;
+        ctl->idle = 0;
+        ctl->tests = 0;
+        ctl->cands = 0;
+        ctl->fp64 = 0;
+        ctl->adj = 0;
+        ctl->exited = 0;
+    }
+}
+
+__global__ void k_fill64(u64* p, uint32_t n, u64 v)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+extern "C" {
+
+hipError_t oi_launch_fill64(hipStream_t st, unsigned long long* p, uint32_t n, unsigned long long v)
+{
+    uint32_t g = (n + 255) / 256;
+    if (g > 4096) g = 4096;
+    if (g == 0) g = 1;
+    hipLaunchKernelGGL(k_fill64, dim3(g), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t oi_launch_seed(hipStream_t st, unsigned long long* ring, const uint2* front, uint32_t n0,
+                          oi_trav_ctl* ctl)
+{
+    uint32_t g = (n0 + 255) / 256;
+    if (g == 0) g = 1;
+    hipLaunchKernelGGL(k_seed, dim3(g), dim3(256), 0, st, ring, front, n0, ctl);
+    return hipGetLastError();
+}
+
+hipError_t oi_traverse_occupancy(int* blocks_per_cu)
+{
+    int a = 0, b = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_traverse<false>, TB, 0);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_traverse<true>, TB, 0);
+    *blocks_per_cu = a < b ? a : b;
+    return e;
+}
+
+hipError_t oi_launch_traverse(hipStream_t st, uint32_t grid, const oi_mesh_desc* meshes, uint32_t n_meshes,
+                              unsigned long long* ring, uint32_t ring_cap, oi_trav_ctl* ctl,
+                              uint4* hits, uint32_t hits_cap, uint32_t expand, uint32_t flags,
+                              unsigned long long timeout_ticks)
+{
+    TravArgs a{meshes, n_meshes, ring, ring_cap - 1u, ctl, hits, hits_cap, expand, flags,
+               timeout_ticks};
+    if (flags & OI_DETECT_FP64_FALLBACK)
+        hipLaunchKernelGGL(k_traverse<true>, dim3(grid), dim3(TB), 0, st, a);
+    else
+        hipLaunchKernelGGL(k_traverse<false>, dim3(grid), dim3(TB), 0, st, a);
+    return hipGetLastError();
+}
+
+}  // extern "C"

@@ -1,0 +1,808 @@
+// oibvh_host.cpp -- C ABI implementation (include/oibvh.h): contexts, trees,
+// scenes. Host orchestration of the reference's OibvhTree (oibvhTree.cu) and
+// Scene (scene.cu): device buffers are allocated once per object and sized
+// from the inputs (the reference uses fixed 10M-element buffers,
+// oibvhTree.cu:11-13, scene.cu:15-21), every launch goes to the context's
+// stream and only result accessors synchronise.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "oibvh.h"
+#include "oibvh_common.h"
+
+extern "C" {
+hipError_t oi_launch_mesh_box(hipStream_t, const float*, uint32_t, float*, uint32_t, uint32_t*, float*, uint32_t*,
+                              uint32_t);
+hipError_t oi_launch_morton(hipStream_t, const uint32_t*, const float*, uint32_t, const float*, uint32_t*,
+                            uint32_t*, uint32_t);
+uint32_t oi_sort_tile();
+hipError_t oi_launch_sort_pass(hipStream_t, const uint32_t*, const uint32_t*, uint32_t*, uint32_t*, uint32_t,
+                               uint32_t, const uint32_t*, uint32_t*, uint32_t*, uint32_t*);
+hipError_t oi_launch_reduce(hipStream_t, int, const float*, const uint32_t*, const uint32_t*, uint32_t*, float*,
+                            uint32_t*, uint32_t, uint32_t, uint32_t);
+hipError_t oi_launch_fill64(hipStream_t, unsigned long long*, uint32_t, unsigned long long);
+hipError_t oi_launch_seed(hipStream_t, unsigned long long*, const uint2*, uint32_t, oi_trav_ctl*);
+hipError_t oi_traverse_occupancy(int*);
+hipError_t oi_launch_traverse(hipStream_t, uint32_t, const oi_mesh_desc*, uint32_t, unsigned long long*, uint32_t,
+                              oi_trav_ctl*, uint4*, uint32_t, uint32_t, uint32_t, unsigned long long);
+}
+
+// ---------------------------------------------------------------------------
+struct oibvh_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+    int cus = 0;
+    int trav_occ = 1;
+    uint64_t clock_khz = 100000;
+    uint32_t leaves_per_thread = 0;
+    uint32_t trav_blocks = 0;
+    uint32_t queue_cap = 1u << 22;
+    uint32_t hits_cap = 1u << 20;
+};
+
+struct oibvh_tree {
+    oibvh_ctx* ctx = nullptr;
+    uint32_t F = 0, V = 0, N = 0, L = 0, vl = 0;
+    float* pos_own = nullptr;   // owned positions
+    const float* pos = nullptr; // current positions (owned or borrowed)
+    uint32_t* faces = nullptr;  // original order
+    uint32_t* sfaces = nullptr; // leaf order
+    uint32_t* perm = nullptr;   // leaf -> original
+    float* nodes = nullptr;
+    // build scratch (allocated at the first build)
+    uint32_t* keys = nullptr;
+    uint32_t* keys_alt = nullptr;
+    uint32_t* vals_tmp = nullptr;
+    uint32_t* sort_scratch = nullptr;
+    uint32_t tiles = 0;
+    float* box = nullptr;        // 6 floats + partials
+    uint32_t* counters = nullptr; // [0] mesh box, [1] reduce
+    bool built = false;
+};
+
+struct oibvh_scene {
+    oibvh_ctx* ctx = nullptr;
+    std::vector<oibvh_tree*> trees;
+    oi_mesh_desc* d_desc = nullptr;
+    uint32_t desc_cap = 0;
+    uint2* d_front = nullptr;
+    uint32_t front_cap = 0;
+    uint32_t n_front = 0;
+    std::vector<uint2> h_front;
+    std::vector<oi_mesh_desc> h_desc;  // host copies stay alive until the next detect
+    uint32_t front_key[3] = {~0u, ~0u, ~0u}; // entry, flags&SELF, #trees
+    unsigned long long* ring = nullptr;
+    uint32_t ring_cap = 0;
+    bool ring_dirty = true;
+    oi_trav_ctl* ctl = nullptr;
+    oi_trav_ctl* h_ctl = nullptr; // pinned
+    uint4* hits = nullptr;
+    uint32_t hits_cap = 0;
+    // last detect
+    uint32_t last_entry = 0, last_expand = 1, last_flags = 0;
+    bool unchecked = false;
+    bool have_result = false;
+    uint32_t n_pairs = 0;
+    oibvh_detect_stats stats{};
+};
+
+// ---------------------------------------------------------------------------
+static oibvh_status set_err(oibvh_ctx* ctx, oibvh_status st, const char* fmt, ...)
+{
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return st;
+}
+
+#define HIPCHK(ctx, expr)                                                                          \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return set_err((ctx), OIBVH_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));   \
+    } while (0)
+
+template <class T>
+static hipError_t dalloc(T** p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    return hipMalloc((void**)p, n * sizeof(T));
+}
+
+template <class T>
+static void dfree(T*& p)
+{
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+static hipMemcpyKind kind_of(uint32_t ptr_kind)
+{
+    return ptr_kind == OIBVH_PTR_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+}
+
+// ---------------------------------------------------------------- context
+extern "C" oibvh_status oibvh_ctx_create(int device, void* stream, oibvh_ctx** out)
+{
+    if (!out) return OIBVH_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return OIBVH_ERR_NO_DEVICE;
+    oibvh_ctx* c = new (std::nothrow) oibvh_ctx;
+    if (!c) return OIBVH_ERR_INVALID;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) {
+        delete c;
+        return OIBVH_ERR_NO_DEVICE;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        delete c;
+        return OIBVH_ERR_NO_DEVICE;
+    }
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        delete c;
+        return OIBVH_ERR_NO_DEVICE;
+    }
+    c->cus = prop.multiProcessorCount;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+        c->clock_khz = (uint64_t)khz;
+    int occ = 0;
+    if (oi_traverse_occupancy(&occ) == hipSuccess && occ > 0) c->trav_occ = occ;
+    if (stream) {
+        c->stream = (hipStream_t)stream;
+    } else {
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete c;
+            return OIBVH_ERR_HIP;
+        }
+        c->own_stream = true;
+    }
+    *out = c;
+    return OIBVH_OK;
+}
+
+extern "C" void oibvh_ctx_destroy(oibvh_ctx* ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->own_stream) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+}
+
+extern "C" const char* oibvh_last_error(const oibvh_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+extern "C" oibvh_status oibvh_ctx_synchronize(oibvh_ctx* ctx)
+{
+    if (!ctx) return OIBVH_ERR_INVALID;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return OIBVH_OK;
+}
+
+extern "C" void* oibvh_ctx_stream(oibvh_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+extern "C" oibvh_status oibvh_ctx_set_tuning(oibvh_ctx* ctx, uint32_t lpt, uint32_t blocks, uint32_t qcap,
+                                            uint32_t hcap)
+{
+    if (!ctx) return OIBVH_ERR_INVALID;
+    if (lpt != 0 && lpt != 1 && lpt != 2 && lpt != 4 && lpt != 8)
+        return set_err(ctx, OIBVH_ERR_INVALID, "reduce_leaves_per_thread must be 0, 1, 2, 4 or 8");
+    if (qcap && (qcap & (qcap - 1)))
+        return set_err(ctx, OIBVH_ERR_INVALID, "queue_capacity must be a power of two");
+    ctx->leaves_per_thread = lpt;
+    ctx->trav_blocks = blocks;
+    if (qcap) ctx->queue_cap = qcap;
+    if (hcap) ctx->hits_cap = hcap;
+    return OIBVH_OK;
+}
+
+// ---------------------------------------------------------------- trees
+static void tree_free(oibvh_tree* t)
+{
+    dfree(t->pos_own);
+    dfree(t->faces);
+    dfree(t->sfaces);
+    dfree(t->perm);
+    dfree(t->nodes);
+    dfree(t->keys);
+    dfree(t->keys_alt);
+    dfree(t->vals_tmp);
+    dfree(t->sort_scratch);
+    dfree(t->box);
+    dfree(t->counters);
+}
+
+static oibvh_status tree_alloc_common(oibvh_tree* t)
+{
+    oibvh_ctx* c = t->ctx;
+    t->L = oi_ilog2(oi_np2(t->F));
+    t->vl = oi_np2(t->F) - t->F;
+    t->N = oi_tree_size(t->F);
+    HIPCHK(c, dalloc(&t->pos_own, 3ull * t->V));
+    HIPCHK(c, dalloc(&t->faces, 3ull * t->F));
+    HIPCHK(c, dalloc(&t->sfaces, 3ull * t->F));
+    HIPCHK(c, dalloc(&t->perm, t->F));
+    HIPCHK(c, dalloc(&t->nodes, 6ull * t->N));
+    HIPCHK(c, dalloc(&t->counters, 4));
+    HIPCHK(c, hipMemsetAsync(t->counters, 0, 4 * sizeof(uint32_t), c->stream));
+    t->pos = t->pos_own;
+    return OIBVH_OK;
+}
+
+static oibvh_status set_positions(oibvh_tree* t, const float* xyz, uint32_t ptr_kind)
+{
+    oibvh_ctx* c = t->ctx;
+    if (ptr_kind == OIBVH_PTR_DEVICE_BORROW) {
+        t->pos = xyz;
+        return OIBVH_OK;
+    }
+    HIPCHK(c, hipMemcpyAsync(t->pos_own, xyz, 12ull * t->V, kind_of(ptr_kind), c->stream));
+    t->pos = t->pos_own;
+    return OIBVH_OK;
+}
+
+extern "C" oibvh_status oibvh_tree_create(oibvh_ctx* ctx, const float* xyz, uint32_t V, const uint32_t* idx,
+                                         uint32_t F, uint32_t ptr_kind, oibvh_tree** out)
+{
+    if (!out) return OIBVH_ERR_INVALID;
+    *out = nullptr;
+    if (!ctx || !xyz || !idx || V == 0 || F == 0 || ptr_kind > OIBVH_PTR_DEVICE_BORROW)
+        return set_err(ctx, OIBVH_ERR_INVALID, "oibvh_tree_create: invalid argument");
+    if (F > (1u << 24))
+        return set_err(ctx, OIBVH_ERR_INVALID, "oibvh_tree_create: at most 2^24 faces per tree");
+    if (ptr_kind == OIBVH_PTR_HOST) {
+        for (uint64_t i = 0; i < 3ull * F; i++)
+            if (idx[i] >= V) return set_err(ctx, OIBVH_ERR_INVALID, "face index %u out of range", idx[i]);
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    oibvh_tree* t = new (std::nothrow) oibvh_tree;
+    if (!t) return OIBVH_ERR_INVALID;
+    t->ctx = ctx;
+    t->F = F;
+    t->V = V;
+    oibvh_status st = tree_alloc_common(t);
+    if (st == OIBVH_OK) {
+        hipError_t e = hipMemcpyAsync(t->faces, idx, 12ull * F,
+                                      ptr_kind == OIBVH_PTR_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice,
+                                      ctx->stream);
+        if (e != hipSuccess) st = set_err(ctx, OIBVH_ERR_HIP, "face upload: %s", hipGetErrorString(e));
+    }
+    if (st == OIBVH_OK) st = set_positions(t, xyz, ptr_kind);
+    if (st == OIBVH_OK && ptr_kind == OIBVH_PTR_HOST) {
+        // host inputs may be freed by the caller right after return
+        hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) st = set_err(ctx, OIBVH_ERR_HIP, "upload: %s", hipGetErrorString(e));
+    }
+    if (st != OIBVH_OK) {
+        tree_free(t);
+        delete t;
+        return st;
+    }
+    *out = t;
+    return OIBVH_OK;
+}
+
+extern "C" oibvh_status oibvh_tree_create_like(oibvh_tree* other, const float* xyz, uint32_t V, uint32_t ptr_kind,
+                                              oibvh_tree** out)
+{
+    if (!out) return OIBVH_ERR_INVALID;
+    *out = nullptr;
+    if (!other) return OIBVH_ERR_INVALID;
+    oibvh_ctx* ctx = other->ctx;
+    if (!xyz || V != other->V || ptr_kind > OIBVH_PTR_DEVICE_BORROW)
+        return set_err(ctx, OIBVH_ERR_INVALID, "oibvh_tree_create_like: invalid argument");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    oibvh_tree* t = new (std::nothrow) oibvh_tree;
+    if (!t) return OIBVH_ERR_INVALID;
+    t->ctx = ctx;
+    t->F = other->F;
+    t->V = V;
+    oibvh_status st = tree_alloc_common(t);
+    hipStream_t s = ctx->stream;
+    if (st == OIBVH_OK) {
+        hipError_t e = hipMemcpyAsync(t->faces, other->faces, 12ull * t->F, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(t->sfaces, other->sfaces, 12ull * t->F, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(t->perm, other->perm, 4ull * t->F, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(t->nodes, other->nodes, 24ull * t->N, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) st = set_err(ctx, OIBVH_ERR_HIP, "create_like copy: %s", hipGetErrorString(e));
+    }
+    if (st == OIBVH_OK) st = set_positions(t, xyz, ptr_kind);
+    if (st == OIBVH_OK && ptr_kind == OIBVH_PTR_HOST) {
+        hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = set_err(ctx, OIBVH_ERR_HIP, "upload: %s", hipGetErrorString(e));
+    }
+    if (st != OIBVH_OK) {
+        tree_free(t);
+        delete t;
+        return st;
+    }
+    t->built = other->built;
+    *out = t;
+    return OIBVH_OK;
+}
+
+extern "C" void oibvh_tree_destroy(oibvh_tree* t)
+{
+    if (!t) return;
+    (void)hipSetDevice(t->ctx->device);
+    (void)hipStreamSynchronize(t->ctx->stream);
+    tree_free(t);
+    delete t;
+}
+
+static int pick_lpt(const oibvh_tree* t)
+{
+    uint32_t lpt = t->ctx->leaves_per_thread;
+    if (lpt) return (int)lpt;
+    // aim for >= ~2 workgroups per CU in the reduce
+    const uint64_t target = 2ull * (uint64_t)(t->ctx->cus > 0 ? t->ctx->cus : 256) * 256ull;
+    uint32_t m = 1;
+    while (m < 8 && (uint64_t)t->F / (2ull * m) >= target) m *= 2;
+    return (int)m;
+}
+
+static oibvh_status launch_reduce(oibvh_tree* t, bool build)
+{
+    oibvh_ctx* c = t->ctx;
+    HIPCHK(c, oi_launch_reduce(c->stream, pick_lpt(t), t->pos, build ? t->faces : t->sfaces,
+                               build ? t->perm : nullptr, build ? t->sfaces : nullptr, t->nodes, t->counters + 1,
+                               t->F, t->L, t->vl));
+    return OIBVH_OK;
+}
+
+extern "C" oibvh_status oibvh_tree_build(oibvh_tree* t)
+{
+    if (!t) return OIBVH_ERR_INVALID;
+    oibvh_ctx* c = t->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    const uint32_t F = t->F;
+    const uint32_t tile = oi_sort_tile();
+    const uint32_t tiles = (F + tile - 1) / tile;
+    const uint32_t box_blocks = std::max(1u, std::min(256u, (t->V + 4095u) / 4096u));
+    if (!t->keys) {
+        HIPCHK(c, dalloc(&t->keys, F));
+        HIPCHK(c, dalloc(&t->keys_alt, F));
+        HIPCHK(c, dalloc(&t->vals_tmp, F));
+        t->tiles = tiles;
+        // ghist[1024] | tile counters[8] | lookback0[tiles*256] | lookback1[tiles*256]
+        HIPCHK(c, dalloc(&t->sort_scratch, 1032ull + 512ull * tiles));
+        HIPCHK(c, dalloc(&t->box, 6ull + 6ull * 256));
+    }
+    uint32_t* ghist = t->sort_scratch;
+    uint32_t* tctr = t->sort_scratch + 1024;
+    uint32_t* lb0 = t->sort_scratch + 1032;
+    uint32_t* lb1 = lb0 + 256ull * tiles;
+    hipStream_t s = c->stream;
+    HIPCHK(c, oi_launch_mesh_box(s, t->pos, t->V, t->box + 6, box_blocks, t->counters, t->box, t->sort_scratch,
+                                 1032u + 256u * tiles));
+    const uint32_t mblocks = std::max(1u, std::min((F + 255u) / 256u, (uint32_t)std::max(1, c->cus) * 8u));
+    HIPCHK(c, oi_launch_morton(s, t->faces, t->pos, F, t->box, t->keys, ghist, mblocks));
+    HIPCHK(c, oi_launch_sort_pass(s, t->keys, nullptr, t->keys_alt, t->vals_tmp, F, 0, ghist, lb0, lb1, tctr + 0));
+    HIPCHK(c, oi_launch_sort_pass(s, t->keys_alt, t->vals_tmp, t->keys, t->perm, F, 8, ghist + 256, lb1, lb0,
+                                  tctr + 1));
+    HIPCHK(c, oi_launch_sort_pass(s, t->keys, t->perm, t->keys_alt, t->vals_tmp, F, 16, ghist + 512, lb0, lb1,
+                                  tctr + 2));
+    HIPCHK(c, oi_launch_sort_pass(s, t->keys_alt, t->vals_tmp, nullptr, t->perm, F, 24, ghist + 768, lb1, nullptr,
+                                  tctr + 3));
+    oibvh_status st = launch_reduce(t, true);
+    if (st != OIBVH_OK) return st;
+    t->built = true;
+    return OIBVH_OK;
+}
+
+extern "C" oibvh_status oibvh_tree_refit(oibvh_tree* t, const float* xyz, uint32_t ptr_kind)
+{
+    if (!t) return OIBVH_ERR_INVALID;
+    oibvh_ctx* c = t->ctx;
+    if (ptr_kind > OIBVH_PTR_DEVICE_BORROW) return set_err(c, OIBVH_ERR_INVALID, "bad ptr_kind");
+    if (!t->built) return set_err(c, OIBVH_ERR_NOT_BUILT, "refit before build");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (xyz) {
+        oibvh_status st = set_positions(t, xyz, ptr_kind);
+        if (st != OIBVH_OK) return st;
+    }
+    oibvh_status st = launch_reduce(t, false);
+    if (st == OIBVH_OK && xyz && ptr_kind == OIBVH_PTR_HOST) HIPCHK(c, hipStreamSynchronize(c->stream));
+    return st;
+}
+
+extern "C" oibvh_status oibvh_tree_get_depth(const oibvh_tree* t, uint32_t* d)
+{
+    if (!t || !d) return OIBVH_ERR_INVALID;
+    *d = oi_ilog2(t->N);  // getDepth: ilog2(m_aabbTree.size()), oibvhTree.cu:46-49
+    return OIBVH_OK;
+}
+
+extern "C" oibvh_status oibvh_tree_get_prim_count(const oibvh_tree* t, uint32_t* n)
+{
+    if (!t || !n) return OIBVH_ERR_INVALID;
+    *n = t->F;
+    return OIBVH_OK;
+}
+
+extern "C" oibvh_status oibvh_tree_get_node_count(const oibvh_tree* t, uint32_t* n)
+{
+    if (!t || !n) return OIBVH_ERR_INVALID;
+    *n = t->N;
+    return OIBVH_OK;
+}
+
+extern "C" oibvh_status oibvh_tree_get_vertex_count(const oibvh_tree* t, uint32_t* n)
+{
+    if (!t || !n) return OIBVH_ERR_INVALID;
+    *n = t->V;
+    return OIBVH_OK;
+}
+
+static oibvh_status readback(oibvh_tree* t, void* dst, const void* src, size_t bytes)
+{
+    oibvh_ctx* c = t->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return OIBVH_OK;
+}
+
+extern "C" oibvh_status oibvh_tree_get_nodes(oibvh_tree* t, float* out, uint64_t cap, uint32_t* n)
+{
+    if (!t) return OIBVH_ERR_INVALID;
+    if (n) *n = t->N;
+    if (!t->built) return set_err(t->ctx, OIBVH_ERR_NOT_BUILT, "tree not built");
+    if (!out || cap < t->N) return set_err(t->ctx, OIBVH_ERR_CAPACITY, "need %u nodes", t->N);
+    return readback(t, out, t->nodes, 24ull * t->N);
+}
+
+extern "C" oibvh_status oibvh_tree_get_sorted_faces(oibvh_tree* t, uint32_t* out, uint64_t cap)
+{
+    if (!t) return OIBVH_ERR_INVALID;
+    if (!t->built) return set_err(t->ctx, OIBVH_ERR_NOT_BUILT, "tree not built");
+    if (!out || cap < t->F) return set_err(t->ctx, OIBVH_ERR_CAPACITY, "need %u faces", t->F);
+    return readback(t, out, t->sfaces, 12ull * t->F);
+}
+
+extern "C" oibvh_status oibvh_tree_get_perm(oibvh_tree* t, uint32_t* out, uint64_t cap)
+{
+    if (!t) return OIBVH_ERR_INVALID;
+    if (!t->built) return set_err(t->ctx, OIBVH_ERR_NOT_BUILT, "tree not built");
+    if (!out || cap < t->F) return set_err(t->ctx, OIBVH_ERR_CAPACITY, "need %u faces", t->F);
+    return readback(t, out, t->perm, 4ull * t->F);
+}
+
+extern "C" oibvh_status oibvh_tree_get_positions(oibvh_tree* t, float* out, uint64_t cap)
+{
+    if (!t) return OIBVH_ERR_INVALID;
+    if (!out || cap < 3ull * t->V) return set_err(t->ctx, OIBVH_ERR_CAPACITY, "need %u floats", 3 * t->V);
+    return readback(t, out, t->pos, 12ull * t->V);
+}
+
+extern "C" oibvh_status oibvh_tree_positions_device(oibvh_tree* t, float** dptr)
+{
+    if (!t || !dptr) return OIBVH_ERR_INVALID;
+    *dptr = const_cast<float*>(t->pos);
+    return OIBVH_OK;
+}
+
+// ---------------------------------------------------------------- scenes
+extern "C" oibvh_status oibvh_scene_create(oibvh_ctx* ctx, oibvh_scene** out)
+{
+    if (!out) return OIBVH_ERR_INVALID;
+    *out = nullptr;
+    if (!ctx) return OIBVH_ERR_INVALID;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    oibvh_scene* s = new (std::nothrow) oibvh_scene;
+    if (!s) return OIBVH_ERR_INVALID;
+    s->ctx = ctx;
+    hipError_t e = dalloc(&s->ctl, 1);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_ctl, sizeof(oi_trav_ctl), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        dfree(s->ctl);
+        delete s;
+        return set_err(ctx, OIBVH_ERR_HIP, "scene alloc: %s", hipGetErrorString(e));
+    }
+    std::memset(s->h_ctl, 0, sizeof(oi_trav_ctl));
+    *out = s;
+    return OIBVH_OK;
+}
+
+extern "C" void oibvh_scene_destroy(oibvh_scene* s)
+{
+    if (!s) return;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    dfree(s->d_desc);
+    dfree(s->d_front);
+    dfree(s->ring);
+    dfree(s->ctl);
+    dfree(s->hits);
+    if (s->h_ctl) (void)hipHostFree(s->h_ctl);
+    delete s;
+}
+
+extern "C" oibvh_status oibvh_scene_add_tree(oibvh_scene* s, oibvh_tree* t)
+{
+    if (!s || !t) return OIBVH_ERR_INVALID;
+    if (t->ctx != s->ctx) return set_err(s->ctx, OIBVH_ERR_INVALID, "tree from another context");
+    if (!t->built) return set_err(s->ctx, OIBVH_ERR_NOT_BUILT, "addOibvhTree: tree not built");
+    s->trees.push_back(t);
+    s->have_result = false;
+    return OIBVH_OK;
+}
+
+// initial BVTT front, Scene::expandBvttNodes (scene.cu:192-223): real nodes at
+// level min(L, entry) of mesh i x those of mesh j, i < j; in self mode also the
+// unordered pairs (x <= y) of each mesh's entry-level nodes.
+static void make_front(oibvh_scene* s, uint32_t entry, bool self)
+{
+    s->h_front.clear();
+    const uint32_t n = (uint32_t)s->trees.size();
+    std::vector<uint32_t> ibase(n);
+    uint64_t acc = 0;
+    for (uint32_t m = 0; m < n; m++) {
+        ibase[m] = (uint32_t)acc;
+        acc += (2ull << s->trees[m]->L) - 1ull;
+    }
+    auto lvl_range = [&](uint32_t m, uint32_t& a, uint32_t& b) {
+        const oibvh_tree* t = s->trees[m];
+        const uint32_t l = std::min(t->L, entry);
+        a = (1u << l) - 1u;
+        b = oi_lvl_last_implicit(l, t->L, t->vl);
+    };
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t a0, a1;
+        lvl_range(i, a0, a1);
+        if (self)
+            for (uint32_t x = a0; x <= a1; x++)
+                for (uint32_t y = x; y <= a1; y++) s->h_front.push_back(make_uint2(ibase[i] + x, ibase[i] + y));
+        for (uint32_t j = i + 1; j < n; j++) {
+            uint32_t b0, b1;
+            lvl_range(j, b0, b1);
+            for (uint32_t x = a0; x <= a1; x++)
+                for (uint32_t y = b0; y <= b1; y++) s->h_front.push_back(make_uint2(ibase[i] + x, ibase[j] + y));
+        }
+    }
+}
+
+static oibvh_status ensure_ring(oibvh_scene* s, uint32_t cap)
+{
+    oibvh_ctx* c = s->ctx;
+    if (s->ring_cap < cap) {
+        dfree(s->ring);
+        s->ring_cap = 0;
+        HIPCHK(c, dalloc(&s->ring, cap));
+        s->ring_cap = cap;
+        s->ring_dirty = true;
+    }
+    if (s->ring_dirty) {
+        HIPCHK(c, oi_launch_fill64(c->stream, s->ring, s->ring_cap, OI_EMPTY_SLOT));
+        s->ring_dirty = false;
+    }
+    return OIBVH_OK;
+}
+
+static oibvh_status ensure_hits(oibvh_scene* s, uint32_t cap)
+{
+    oibvh_ctx* c = s->ctx;
+    if (s->hits_cap < cap) {
+        dfree(s->hits);
+        s->hits_cap = 0;
+        HIPCHK(c, dalloc(&s->hits, cap));
+        s->hits_cap = cap;
+    }
+    return OIBVH_OK;
+}
+
+static oibvh_status enqueue_detect(oibvh_scene* s)
+{
+    oibvh_ctx* c = s->ctx;
+    hipStream_t st = c->stream;
+    const uint32_t n = (uint32_t)s->trees.size();
+    // mesh table (positions may be borrowed and change between frames)
+    std::vector<oi_mesh_desc>& desc = s->h_desc;
+    desc.resize(n);
+    uint64_t acc = 0;
+    for (uint32_t m = 0; m < n; m++) {
+        const oibvh_tree* t = s->trees[m];
+        oi_mesh_desc& d = desc[m];
+        std::memset(&d, 0, sizeof d);
+        d.nodes = t->nodes;
+        d.sfaces = t->sfaces;
+        d.pos = t->pos;
+        d.perm = t->perm;
+        d.F = t->F;
+        d.L = t->L;
+        d.vl = t->vl;
+        d.N = t->N;
+        d.ibase = (uint32_t)acc;
+        acc += (2ull << t->L) - 1ull;
+    }
+    if (acc >= 0xFFFFFFFFull) return set_err(c, OIBVH_ERR_INVALID, "scene too large for 32-bit BVTT indices");
+    if (s->desc_cap < n) {
+        dfree(s->d_desc);
+        s->desc_cap = 0;
+        HIPCHK(c, dalloc(&s->d_desc, n));
+        s->desc_cap = n;
+    }
+    HIPCHK(c, hipMemcpyAsync(s->d_desc, desc.data(), n * sizeof(oi_mesh_desc), hipMemcpyHostToDevice, st));
+    const bool self = (s->last_flags & OIBVH_DETECT_SELF) != 0;
+    const uint32_t key[3] = {s->last_entry, self ? 1u : 0u, n};
+    if (std::memcmp(key, s->front_key, sizeof key) != 0) {
+        make_front(s, s->last_entry, self);
+        s->n_front = (uint32_t)s->h_front.size();
+        if (s->front_cap < s->n_front) {
+            dfree(s->d_front);
+            s->front_cap = 0;
+            HIPCHK(c, dalloc(&s->d_front, s->n_front));
+            s->front_cap = s->n_front;
+        }
+        if (s->n_front)
+            HIPCHK(c, hipMemcpyAsync(s->d_front, s->h_front.data(), s->n_front * sizeof(uint2), hipMemcpyHostToDevice,
+                                     st));
+        std::memcpy(s->front_key, key, sizeof key);
+    }
+    uint32_t ring_need = std::max(std::max(c->queue_cap, s->ring_cap), oi_np2(std::max(1u, s->n_front) * 2u));
+    oibvh_status r = ensure_ring(s, ring_need);
+    if (r != OIBVH_OK) return r;
+    if (!s->hits) {
+        r = ensure_hits(s, c->hits_cap);
+        if (r != OIBVH_OK) return r;
+    }
+    HIPCHK(c, oi_launch_seed(st, s->ring, s->d_front, s->n_front, s->ctl));
+    uint32_t grid = c->trav_blocks ? c->trav_blocks : (uint32_t)std::max(1, c->cus) * (uint32_t)c->trav_occ;
+    const unsigned long long timeout = 30ull * 1000ull * c->clock_khz;  // 30 s
+    HIPCHK(c, oi_launch_traverse(st, grid, s->d_desc, n, s->ring, s->ring_cap, s->ctl, s->hits, s->hits_cap,
+                                 std::min(3u, std::max(1u, s->last_expand)), s->last_flags & 3u, timeout));
+    s->unchecked = true;
+    s->have_result = true;
+    return OIBVH_OK;
+}
+
+// wait for the last detect, grow buffers and re-run on overflow
+static oibvh_status check_detect(oibvh_scene* s)
+{
+    oibvh_ctx* c = s->ctx;
+    if (!s->unchecked) return OIBVH_OK;
+    uint32_t retries = 0;
+    for (;;) {
+        HIPCHK(c, hipMemcpyAsync(s->h_ctl, s->ctl, sizeof(oi_trav_ctl), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        const oi_trav_ctl& k = *s->h_ctl;
+        if (k.error & OI_ERR_TIMEOUT) {
+            s->ring_dirty = true;
+            s->unchecked = false;
+            s->have_result = false;
+            return set_err(c, OIBVH_ERR_TIMEOUT, "traversal timed out");
+        }
+        bool rerun = false;
+        if (k.error & OI_ERR_QUEUE_OVERFLOW) {
+            s->ring_dirty = true;
+            if (s->ring_cap >= (1u << 30)) {
+                s->unchecked = false;
+                s->have_result = false;
+                return set_err(c, OIBVH_ERR_OVERFLOW, "work queue overflow at %u entries", s->ring_cap);
+            }
+            oibvh_status r = ensure_ring(s, s->ring_cap * 2u);
+            if (r != OIBVH_OK) return r;
+            rerun = true;
+        } else if (k.n_hits > s->hits_cap) {
+            oibvh_status r = ensure_hits(s, oi_np2(k.n_hits + (k.n_hits >> 3)));
+            if (r != OIBVH_OK) return r;
+            rerun = true;
+        }
+        if (!rerun) {
+            s->n_pairs = k.n_hits;
+            s->stats.bvtt_tests = k.tests;
+            s->stats.candidates = k.cands;
+            s->stats.fp64_rechecks = k.fp64;
+            s->stats.adjacent = k.adj;
+            s->stats.pairs = k.n_hits;
+            s->stats.retries = retries;
+            s->unchecked = false;
+            return OIBVH_OK;
+        }
+        retries++;
+        if (retries > 8) return set_err(c, OIBVH_ERR_OVERFLOW, "detect: too many retries");
+        oibvh_status r = enqueue_detect(s);
+        if (r != OIBVH_OK) return r;
+    }
+}
+
+extern "C" oibvh_status oibvh_scene_detect(oibvh_scene* s, uint32_t entry, uint32_t expand, uint32_t flags)
+{
+    if (!s) return OIBVH_ERR_INVALID;
+    oibvh_ctx* c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    for (oibvh_tree* t : s->trees)
+        if (!t->built) return set_err(c, OIBVH_ERR_NOT_BUILT, "detect: tree not built");
+    if (s->unchecked) {
+        oibvh_status r = check_detect(s);
+        if (r != OIBVH_OK) return r;
+    }
+    s->last_entry = entry;
+    s->last_expand = expand;
+    s->last_flags = flags;
+    s->n_pairs = 0;
+    s->stats = oibvh_detect_stats{};
+    if (s->trees.empty() || (s->trees.size() == 1 && !(flags & OIBVH_DETECT_SELF))) {
+        s->have_result = true;
+        s->unchecked = false;
+        return OIBVH_OK;
+    }
+    oibvh_status r = enqueue_detect(s);
+    if (r != OIBVH_OK) return r;
+    if (flags & OIBVH_DETECT_ASYNC) return OIBVH_OK;
+    return check_detect(s);
+}
+
+extern "C" oibvh_status oibvh_scene_get_pair_count(oibvh_scene* s, uint32_t* count)
+{
+    if (!s || !count) return OIBVH_ERR_INVALID;
+    oibvh_status r = check_detect(s);
+    if (r != OIBVH_OK) return r;
+    *count = s->n_pairs;
+    return OIBVH_OK;
+}
+
+extern "C" oibvh_status oibvh_scene_get_stats(oibvh_scene* s, oibvh_detect_stats* st)
+{
+    if (!s || !st) return OIBVH_ERR_INVALID;
+    oibvh_status r = check_detect(s);
+    if (r != OIBVH_OK) return r;
+    *st = s->stats;
+    return OIBVH_OK;
+}
+
+extern "C" oibvh_status oibvh_scene_hits_device(oibvh_scene* s, void** dptr, uint32_t* n)
+{
+    if (!s || !dptr || !n) return OIBVH_ERR_INVALID;
+    oibvh_status r = check_detect(s);
+    if (r != OIBVH_OK) return r;
+    *dptr = s->hits;
+    *n = s->n_pairs;
+    return OIBVH_OK;
+}
+
+extern "C" oibvh_status oibvh_scene_get_pairs(oibvh_scene* s, uint32_t* out, uint64_t cap, uint32_t* n)
+{
+    if (!s) return OIBVH_ERR_INVALID;
+    oibvh_ctx* c = s->ctx;
+    oibvh_status r = check_detect(s);
+    if (r != OIBVH_OK) return r;
+    if (n) *n = s->n_pairs;
+    if (s->n_pairs == 0) return OIBVH_OK;
+    if (!out || cap < s->n_pairs) return set_err(c, OIBVH_ERR_CAPACITY, "need %u pairs", s->n_pairs);
+    std::vector<uint4> h(s->n_pairs);
+    HIPCHK(c, hipMemcpyAsync(h.data(), s->hits, s->n_pairs * sizeof(uint4), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::sort(h.begin(), h.end(), [](const uint4& a, const uint4& b) {
+        if (a.x != b.x) return a.x < b.x;
+        if (a.y != b.y) return a.y < b.y;
+        if (a.z != b.z) return a.z < b.z;
+        return a.w < b.w;
+    });
+    for (uint32_t i = 0; i < s->n_pairs; i++) {
+        out[4ull * i] = h[i].x;
+        out[4ull * i + 1] = h[i].y;
+        out[4ull * i + 2] = h[i].z;
+        out[4ull * i + 3] = h[i].w;
+    }
+    return OIBVH_OK;
+}

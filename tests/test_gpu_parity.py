@@ -1,0 +1,355 @@
+"""GPU parity: the HIP path through the C ABI against the oracle.
+
+Bar: bit-exact tree AABBs, identical leaf permutation and sorted faces, and
+identical sorted colliding-pair sets (integer outputs, exact)."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from oibvh_amd import meshgen as M
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def api():
+    from oibvh_amd import api as A
+
+    return A
+
+
+def gpu_tree(api, ctx, xyz, f):
+    t = api.OibvhTree(ctx, xyz, f)
+    t.build()
+    return t
+
+
+def check_tree(orc, t, xyz, f, order=None):
+    o = orc.Tree(xyz, f, order="morton" if order is None else order)
+    assert np.array_equal(t.perm(), o.order), "leaf permutation differs"
+    assert np.array_equal(t.sorted_faces(), f[o.order]), "sorted faces differ"
+    a, b = t.nodes(), o.nodes()
+    assert a.shape == b.shape
+    bad = np.nonzero((bits(a) != bits(b)).any(axis=1))[0]
+    assert bad.size == 0, f"{bad.size} nodes differ, first at real index {bad[:5]}"
+    return o
+
+
+def soup(F, seed):
+    r = M.Rng(seed)
+    V = F + 2
+    xyz = r.uniform(V * 3, -1.0, 1.0).reshape(V, 3).astype(np.float32)
+    f = r.integers(F * 3, 0, V - 1).reshape(F, 3).astype(np.uint32)
+    return xyz, f
+
+
+# --------------------------------------------------------------------- build
+@pytest.mark.parametrize("F", [1, 2, 3, 4, 5, 7, 8, 9, 100, 255, 256, 257, 1000, 2047, 2048, 2049, 4096, 4097,
+                               65537, 300001])
+def test_build_soup_bit_exact(api, ctx, orc, F):
+    say(f"build soup F={F}")
+    xyz, f = soup(F, 1000 + F)
+    check_tree(orc, gpu_tree(api, ctx, xyz, f), xyz, f)
+
+
+@pytest.mark.parametrize("lpt", [1, 2, 4, 8])
+def test_build_subtree_sizes(api, ctx, orc, lpt):
+    say(f"build leaves_per_thread={lpt}")
+    ctx.set_tuning(leaves_per_thread=lpt)
+    try:
+        for xyz, f in [M.icosphere(5, 1.0, 0.02, 3), M.uv_sphere(300, 151), soup(12345, 7)]:
+            check_tree(orc, gpu_tree(api, ctx, xyz, f), xyz, f)
+    finally:
+        ctx.set_tuning(leaves_per_thread=0)
+
+
+def test_build_flat_axis_morton_nan_policy(api, ctx, orc):
+    """A plane has a zero-extent axis: offset/length is 0/0 = NaN there, which
+    both sides quantise to 0."""
+    say("build flat plane")
+    xyz, f = M.plane(50)
+    check_tree(orc, gpu_tree(api, ctx, xyz, f), xyz, f)
+
+
+def test_build_cfg1_full_size(api, ctx, orc):
+    say("build two 81,920-face icospheres")
+    for xyz, f in M.interpenetrating_icospheres(subdiv=6):
+        check_tree(orc, gpu_tree(api, ctx, xyz, f), xyz, f)
+
+
+def test_build_is_deterministic(api, ctx):
+    say("build determinism")
+    xyz, f = M.icosphere(6, 1.0, 0.02, 1)
+    t = gpu_tree(api, ctx, xyz, f)
+    a = t.nodes().copy()
+    p = t.perm().copy()
+    for _ in range(3):
+        t.build()
+        assert np.array_equal(bits(t.nodes()), bits(a)) and np.array_equal(t.perm(), p)
+
+
+# --------------------------------------------------------------------- refit
+def test_refit_after_deformation(api, ctx, orc):
+    say("refit deforming uv sphere")
+    meshes, waves = M.deforming_uv_pair(200, 101)
+    xyz, f = meshes[0]
+    t = gpu_tree(api, ctx, xyz, f)
+    perm = t.perm()
+    for frame in (1, 7, 33):
+        x = waves[0].frame(frame)
+        t.refit(x)
+        o = orc.Tree(x, f, order=perm)
+        assert np.array_equal(bits(t.nodes()), bits(o.nodes()))
+
+
+def test_refit_from_device_pointer(api, ctx, orc):
+    say("refit from borrowed device positions")
+    meshes, waves = M.deforming_uv_pair(120, 61)
+    xyz, f = meshes[1]
+    t = gpu_tree(api, ctx, xyz, f)
+    perm = t.perm()
+    x = waves[1].frame(5)
+    buf = api.DeviceBuffer.from_array(x)
+    t.refit_device(buf.at(0), borrow=True)
+    assert np.array_equal(bits(t.nodes()), bits(orc.Tree(x, f, order=perm).nodes()))
+    assert np.array_equal(bits(t.positions()), bits(x))
+    t.refit(xyz)  # back to owned memory before the buffer goes away
+    ctx.synchronize()
+    buf.free()
+
+
+def test_create_like_then_refit(api, ctx, orc):
+    """main.cpp:127-151: tree 2 copy-constructed from tree 1 over a
+    translated mesh with the same faces, then refit."""
+    say("copy-constructed tree")
+    xyz, f = M.icosphere(4, 1.0, 0.02, 2)
+    t1 = gpu_tree(api, ctx, xyz, f)
+    x2 = (xyz + np.float32([1.0, 0.0, 0.0])).astype(np.float32)
+    t2 = api.OibvhTree.like(t1, x2)
+    t2.refit()
+    o = orc.Tree(x2, f, order=t1.perm())
+    assert np.array_equal(bits(t2.nodes()), bits(o.nodes()))
+    assert t2.getPrimCount() == f.shape[0] and t2.getDepth() == t1.getDepth()
+
+
+# --------------------------------------------------------------------- detect
+def gpu_pairs(api, ctx, meshes, entry=0, expand=1, self_collide=False, fp64=False):
+    trees = [gpu_tree(api, ctx, x, f) for x, f in meshes]
+    sc = api.Scene(ctx)
+    for t in trees:
+        sc.addOibvhTree(t)
+    sc.detectCollision(entry, expand, self_collide=self_collide, fp64_fallback=fp64)
+    return sc.getIntTriPairs(), sc.stats(), trees
+
+
+def test_detect_golden_scenes(api, ctx):
+    g = np.load(os.path.join(GOLD, "scenes.npz"))
+    for name in g["names"]:
+        say(f"golden scene {name}")
+        n = int(g[f"{name}_n"][0])
+        self_c = bool(g[f"{name}_self"][0])
+        meshes = [(g[f"{name}_xyz{k}"], g[f"{name}_faces{k}"]) for k in range(n)]
+        p, _, trees = gpu_pairs(api, ctx, meshes, 0, 1, self_collide=self_c)
+        for k, t in enumerate(trees):
+            assert np.array_equal(t.perm(), g[f"{name}_perm{k}"])
+            assert np.array_equal(bits(t.nodes()), bits(g[f"{name}_nodes{k}"]))
+        assert np.array_equal(p, g[f"{name}_pairs"]), name
+        p64, _, _ = gpu_pairs(api, ctx, meshes, 0, 1, self_collide=self_c, fp64=True)
+        assert np.array_equal(p64, g[f"{name}_pairs64"]), name
+
+
+def test_detect_reference_sat_vectors(api, ctx, orc):
+    """The 8,000 SAT cases whose outcome the reference's own triangleIntersect
+    fixed (tests/golden/sat_ref.npz), as two meshes: every overlapping face
+    pair is tested, and the diagonal pairs (k, k) must match the reference."""
+    say("reference SAT vectors on the GPU")
+    g = np.load(os.path.join(GOLD, "sat_ref.npz"))
+    P, Q, ref, f64 = g["P"].reshape(-1, 3), g["Q"].reshape(-1, 3), g["ref"], g["fp64"]
+    n = ref.size
+    fa = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    meshes = [(P, fa), (Q, fa)]
+    p, _, _ = gpu_pairs(api, ctx, meshes, 2, 2)
+    want = orc.detect([orc.Tree(P, fa), orc.Tree(Q, fa)])
+    assert np.array_equal(p, want)
+    diag = np.zeros(n, np.uint8)
+    d = p[p[:, 2] == p[:, 3]]
+    diag[d[:, 2]] = 1
+    # a diagonal pair whose leaf boxes do not overlap never reaches the SAT;
+    # every such pair is also a SAT miss in the reference
+    assert np.all(diag <= ref) and np.array_equal(diag[ref == 1], ref[ref == 1])
+    p64, st, _ = gpu_pairs(api, ctx, meshes, 2, 2, fp64=True)
+    want64 = orc.detect([orc.Tree(P, fa), orc.Tree(Q, fa)], fp64=True)
+    assert np.array_equal(p64, want64)
+    d64 = np.zeros(n, np.uint8)
+    d64[p64[p64[:, 2] == p64[:, 3]][:, 2]] = 1
+    assert np.array_equal(d64[f64 == 1], f64[f64 == 1])
+    assert st["fp64_rechecks"] > 0
+
+
+def test_detect_cfg1_full_size(api, ctx, orc):
+    say("detect cfg1 (2 x 81,920 faces)")
+    meshes = M.interpenetrating_icospheres(subdiv=6)
+    want = orc.detect([orc.Tree(x, f) for x, f in meshes])
+    for entry, expand in [(4, 3), (0, 1), (2, 2)]:
+        p, st, _ = gpu_pairs(api, ctx, meshes, entry, expand)
+        assert np.array_equal(p, want), (entry, expand)
+        assert st["pairs"] == len(want)
+
+
+@pytest.mark.parametrize("entry", [0, 1, 3, 6, 30])
+@pytest.mark.parametrize("expand", [1, 2, 3, 5])
+def test_detect_entry_expand_invariance(api, ctx, orc, entry, expand):
+    say(f"detect entry={entry} expand={expand}")
+    meshes = M.interpenetrating_icospheres(subdiv=4, seed=21, offset=1.3)
+    want = orc.detect([orc.Tree(x, f) for x, f in meshes])
+    p, _, _ = gpu_pairs(api, ctx, meshes, entry, expand)
+    assert np.array_equal(p, want)
+
+
+def test_detect_many_meshes(api, ctx, orc):
+    say("detect 6 meshes")
+    r = M.Rng(99)
+    meshes = []
+    for k in range(6):
+        c = r.uniform(3, 0.0, 1.2)
+        if k % 2:
+            meshes.append(M.icosphere(3, 0.5, 0.02, 50 + k, center=tuple(c)))
+        else:
+            x, f = M.box(8, 0.4, noise=0.01, seed=60 + k)
+            meshes.append(((x + c.astype(np.float32)).astype(np.float32), f))
+    want = orc.detect([orc.Tree(x, f) for x, f in meshes])
+    assert len(want) > 0
+    p, _, _ = gpu_pairs(api, ctx, meshes, 1, 2)
+    assert np.array_equal(p, want)
+
+
+def test_detect_separated_and_identical(api, ctx, orc):
+    say("detect separated / identical meshes")
+    a = M.icosphere(3, 1.0, 0.0, 0)
+    b = M.icosphere(3, 1.0, 0.0, 0, center=(5.0, 0.0, 0.0))
+    p, _, _ = gpu_pairs(api, ctx, [a, b])
+    assert p.shape == (0, 4)
+    p, _, _ = gpu_pairs(api, ctx, [a, a])
+    assert np.array_equal(p, orc.detect([orc.Tree(*a), orc.Tree(*a)]))
+    assert len(p) > a[1].shape[0]  # every face hits itself and its neighbours
+
+
+def test_detect_self_collision(api, ctx, orc):
+    say("self collision cloth")
+    x, f = M.cloth(96, amplitude=0.2, wavelength=0.2, seed=4)
+    want = orc.detect([orc.Tree(x, f)], self_collide=True)
+    assert len(want) > 0
+    p, st, _ = gpu_pairs(api, ctx, [(x, f)], 0, 1, self_collide=True)
+    assert np.array_equal(p, want)
+    assert st["adjacent"] > 0
+    p, _, _ = gpu_pairs(api, ctx, [(x, f)], 3, 3, self_collide=True)
+    assert np.array_equal(p, want)
+
+
+def test_detect_overflow_retry(api, ctx, orc):
+    say("detect with tiny queue and hit buffers")
+    meshes = M.interpenetrating_icospheres(subdiv=5, seed=31, offset=1.2)
+    want = orc.detect([orc.Tree(x, f) for x, f in meshes])
+    ctx2 = api.Context(0)
+    ctx2.set_tuning(queue_capacity=256, hits_capacity=16)
+    p, st, _ = gpu_pairs(api, ctx2, meshes, 0, 3)
+    assert np.array_equal(p, want)
+    assert st["retries"] > 0
+    ctx2.close()
+
+
+def test_detect_repeated_frames(api, ctx, orc):
+    """refit + detect over several frames of the deforming pair, async detect."""
+    say("deforming pair frames")
+    meshes, waves = M.deforming_uv_pair(160, 81)
+    trees = [gpu_tree(api, ctx, x, f) for x, f in meshes]
+    sc = api.Scene(ctx)
+    for t in trees:
+        sc.addOibvhTree(t)
+    perms = [t.perm() for t in trees]
+    for frame in (0, 3, 9):
+        xs = [w.frame(frame) for w in waves]
+        for t, x in zip(trees, xs):
+            t.refit(x)
+        sc.detectCollision(4, 3, asynchronous=True)
+        got = sc.getIntTriPairs()
+        want = orc.detect([orc.Tree(x, m[1], order=p) for x, m, p in zip(xs, meshes, perms)])
+        assert np.array_equal(got, want), frame
+
+
+def test_fp64_fallback_coplanar_stress(api, ctx, orc):
+    say("precision stress: near-coplanar planes")
+    x, f = M.plane(40)
+    r = M.Rng(5)
+    x2 = x.copy()
+    x2[:, 2] = r.uniform(x.shape[0], -1e-6, 1e-6).astype(np.float32)
+    x2[:, :2] += np.float32(0.013)
+    meshes = [(x, f), (x2.astype(np.float32), f)]
+    want64 = orc.detect([orc.Tree(*m) for m in meshes], fp64=True)
+    p64, st, _ = gpu_pairs(api, ctx, meshes, 0, 1, fp64=True)
+    assert np.array_equal(p64, want64)
+    assert st["fp64_rechecks"] > 0
+    want32 = orc.detect([orc.Tree(*m) for m in meshes])
+    p32, _, _ = gpu_pairs(api, ctx, meshes, 0, 1)
+    assert np.array_equal(p32, want32)
+
+
+def test_cpp_facade_sequence(ctx, orc, tmp_path):
+    say("C++ facade program")
+    exe = os.path.join(ROOT, "tests", "cpp", "facade_test")
+    assert os.path.exists(exe), "facade_test not built"
+    xyz, f = M.icosphere(4, 1.0, 0.02, 8)
+    x2 = (xyz + np.float32([1.0, 0.0, 0.0])).astype(np.float32)
+    inp = tmp_path / "in.bin"
+    out = tmp_path / "out.bin"
+    with open(inp, "wb") as fh:
+        np.array([xyz.shape[0], f.shape[0]], np.uint32).tofile(fh)
+        xyz.tofile(fh)
+        x2.tofile(fh)
+        f.tofile(fh)
+    r = subprocess.run([exe, str(inp), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(out, np.uint32)
+    N = int(raw[0])
+    o = 1
+    n1 = raw[o:o + 6 * N].view(np.float32).reshape(N, 6)
+    o += 6 * N
+    n2 = raw[o:o + 6 * N].view(np.float32).reshape(N, 6)
+    o += 6 * N
+    F = f.shape[0]
+    perm = raw[o:o + F]
+    o += F
+    H = int(raw[o])
+    o += 1
+    pairs = raw[o:o + 4 * H].reshape(H, 4)
+    o += 4 * H
+    depth, prims = int(raw[o]), int(raw[o + 1])
+    t1 = orc.Tree(xyz, f, order="morton")
+    t2 = orc.Tree(x2, f, order=t1.order)
+    assert np.array_equal(perm, t1.order)
+    assert np.array_equal(bits(n1), bits(t1.nodes())) and np.array_equal(bits(n2), bits(t2.nodes()))
+    assert np.array_equal(pairs, orc.detect([t1, t2]))
+    assert prims == F and depth == int(np.log2(N))
+
+
+def test_smoke_entry():
+    say("smoke()")
+    sys.path.insert(0, ROOT)
+    import __graft_entry__
+
+    __graft_entry__.smoke()
