@@ -48,7 +48,9 @@ typedef struct oibvh_scene oibvh_scene;
 /* detect flags */
 #define OIBVH_DETECT_SELF 1u          /* also collide every mesh with itself (pairs sharing a vertex skipped) */
 #define OIBVH_DETECT_FP64_FALLBACK 2u /* FP32 SAT with forward error filter, FP64 re-check of uncertain axes */
-#define OIBVH_DETECT_ASYNC 4u         /* enqueue only; results are checked at the next get/sync call */
+#define OIBVH_DETECT_ASYNC 4u         /* enqueue only: no host wait. Results are checked at the next
+                                         get/stats call or synchronous detect; a further ASYNC detect
+                                         replaces an unchecked one, whose errors go to async_errors */
 
 /* ---------------------------------------------------------------- context */
 
@@ -90,6 +92,12 @@ OIBVH_API oibvh_status oibvh_tree_build(oibvh_tree* tree);
  * fixed sorted face order. xyz == NULL refits the current positions, else the
  * positions are replaced first (ptr_kind as in oibvh_tree_create). */
 OIBVH_API oibvh_status oibvh_tree_refit(oibvh_tree* tree, const float* xyz, uint32_t ptr_kind);
+
+/* Replace the vertex positions without refitting: the counterpart of the
+ * reference's Mesh::transform / rotate / translate (mesh.cpp:170-214), whose
+ * new m_vertices the next refit() uploads (oibvhTree.cu:196-199). The next
+ * build or refit uses them. */
+OIBVH_API oibvh_status oibvh_tree_set_positions(oibvh_tree* tree, const float* xyz, uint32_t ptr_kind);
 
 /* OibvhTree::getDepth() (oibvhTree.cuh:87, oibvhTree.cu:46-49): ilog2(node count) */
 OIBVH_API oibvh_status oibvh_tree_get_depth(const oibvh_tree* tree, uint32_t* depth);
@@ -161,6 +169,9 @@ typedef struct {
     uint64_t adjacent;      /* self-mode leaf pairs skipped for sharing a vertex */
     uint32_t pairs;         /* colliding pairs */
     uint32_t retries;       /* detect re-runs after buffer growth */
+    uint32_t async_errors;  /* error bits of earlier OIBVH_DETECT_ASYNC detects whose results were
+                               never checked (their pair lists were incomplete); 0 when all were clean */
+    uint32_t reserved;
 } oibvh_detect_stats;
 
 /* counters of the last detect (no reference counterpart; the reference prints

@@ -32,7 +32,8 @@ _f32p, _u32p = C.POINTER(C.c_float), C.POINTER(C.c_uint32)
 
 class DetectStats(C.Structure):
     _fields_ = [("bvtt_tests", C.c_uint64), ("candidates", C.c_uint64), ("fp64_rechecks", C.c_uint64),
-                ("adjacent", C.c_uint64), ("pairs", C.c_uint32), ("retries", C.c_uint32)]
+                ("adjacent", C.c_uint64), ("pairs", C.c_uint32), ("retries", C.c_uint32),
+                ("async_errors", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 API = [
@@ -47,6 +48,7 @@ API = [
     ("oibvh_tree_destroy", None, [_vp]),
     ("oibvh_tree_build", _i, [_vp]),
     ("oibvh_tree_refit", _i, [_vp, _vp, _u32]),
+    ("oibvh_tree_set_positions", _i, [_vp, _vp, _u32]),
     ("oibvh_tree_get_depth", _i, [_vp, _u32p]),
     ("oibvh_tree_get_prim_count", _i, [_vp, _u32p]),
     ("oibvh_tree_get_node_count", _i, [_vp, _u32p]),

@@ -134,6 +134,16 @@ class OibvhTree:
                 raise ValueError("vertex count changed")
             _check(self.ctx.h, self._lib.oibvh_tree_refit(self.h, _ptr(xyz), L.PTR_HOST), "refit")
 
+    def set_positions_device(self, dptr: int, borrow: bool = True) -> None:
+        kind = L.PTR_DEVICE_BORROW if borrow else L.PTR_DEVICE
+        _check(self.ctx.h, self._lib.oibvh_tree_set_positions(self.h, C.c_void_p(dptr), kind), "set_positions")
+
+    def set_positions(self, xyz) -> None:
+        xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+        if xyz.shape[0] != self.V:
+            raise ValueError("vertex count changed")
+        _check(self.ctx.h, self._lib.oibvh_tree_set_positions(self.h, _ptr(xyz), L.PTR_HOST), "set_positions")
+
     def refit_device(self, dptr: int, borrow: bool = True) -> None:
         kind = L.PTR_DEVICE_BORROW if borrow else L.PTR_DEVICE
         _check(self.ctx.h, self._lib.oibvh_tree_refit(self.h, C.c_void_p(dptr), kind), "refit")

@@ -54,31 +54,60 @@ __device__ bool ring_reserve(oi_trav_ctl* ctl, uint32_t cap, uint32_t need, u64&
     }
 }
 
-// Slot hand-over. A reserved slot is written / emptied promptly by its owner,
-// so these waits end; the wall-clock limit only guards against a fault.
-__device__ __forceinline__ bool slot_put(u64* slot, u64 v, oi_trav_ctl* ctl, u64 t0, u64 limit)
+// Ring slots. Ring index i maps to slot i & (cap - 1) and to lap
+// (i >> log2 cap) & 15. A slot moves through EMPTY(lap) -> FULL(lap) ->
+// EMPTY(lap + 1): the pusher of index i waits for EMPTY(lap(i)) and the popper
+// of index i waits for FULL(lap(i)), so neither can act on a value of another
+// lap when the ring wraps. Entries are two 30-bit implicit indices (the host
+// keeps a scene's implicit space below 2^30 - 1); the lap tag lives in bits
+// 31:30 and 63:62, and EMPTY is the all-ones index pair.
+#define RING_IDX 0x3fffffffu
+__device__ __forceinline__ u64 ring_pack(uint32_t a, uint32_t b, uint32_t lap)
 {
-    while (ld64(slot) != OI_EMPTY_SLOT) {
+    return (u64)(a | ((lap & 3u) << 30)) | ((u64)(b | ((lap >> 2) << 30)) << 32);
+}
+__device__ __forceinline__ u64 ring_empty(uint32_t lap) { return ring_pack(RING_IDX, RING_IDX, lap & 15u); }
+__device__ __forceinline__ uint32_t ring_lap_of(u64 v) { return ((uint32_t)v >> 30) | (((uint32_t)(v >> 32) >> 30) << 2); }
+
+// A reserved slot is filled / emptied promptly by its owner, so these waits
+// end; the wall-clock limit only guards against a fault. On expiry they flag
+// OI_ERR_TIMEOUT and report failure, and the caller drops the entry.
+__device__ __forceinline__ bool slot_put(u64* ring, uint32_t shift, u64 idx, uint2 e, oi_trav_ctl* ctl, u64 t0,
+                                         u64 limit)
+{
+    const uint32_t mask = (1u << shift) - 1u;
+    const uint32_t lap = (uint32_t)(idx >> shift) & 15u;
+    u64* slot = ring + (idx & mask);
+    const u64 want = ring_empty(lap);
+    while (ld64(slot) != want) {
         __builtin_amdgcn_s_sleep(1);
         if (wall_clock64() - t0 > limit) {
             __hip_atomic_fetch_or(&ctl->error, OI_ERR_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return false;
         }
     }
-    st64(slot, v);
+    st64(slot, ring_pack(e.x, e.y, lap));
     return true;
 }
 
-__device__ __forceinline__ bool slot_take(u64* slot, u64& v, oi_trav_ctl* ctl, u64 t0, u64 limit)
+__device__ __forceinline__ bool slot_take(u64* ring, uint32_t shift, u64 idx, uint2& e, oi_trav_ctl* ctl, u64 t0,
+                                          u64 limit)
 {
-    while ((v = ld64(slot)) == OI_EMPTY_SLOT) {
+    const uint32_t mask = (1u << shift) - 1u;
+    const uint32_t lap = (uint32_t)(idx >> shift) & 15u;
+    u64* slot = ring + (idx & mask);
+    u64 v;
+    for (;;) {
+        v = ld64(slot);
+        if (ring_lap_of(v) == lap && ((uint32_t)v & RING_IDX) != RING_IDX) break;
         __builtin_amdgcn_s_sleep(1);
         if (wall_clock64() - t0 > limit) {
             __hip_atomic_fetch_or(&ctl->error, OI_ERR_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return false;
         }
     }
-    st64(slot, OI_EMPTY_SLOT);
+    e = make_uint2((uint32_t)v & RING_IDX, (uint32_t)(v >> 32) & RING_IDX);
+    st64(slot, ring_empty(lap + 1u));
     return true;
 }
 
@@ -317,17 +346,20 @@ __device__ bool sat_filtered(const v3f* P, const v3f* Q, uint32_t& n_fp64)
 // ---------------------------------------------------------------------------
 // traversal
 // ---------------------------------------------------------------------------
+#define INLINE_MESHES 4
 struct TravArgs {
-    const oi_mesh_desc* meshes;
+    const oi_mesh_desc* meshes;  // mesh table in device memory (scenes of more than INLINE_MESHES meshes)
     uint32_t n_meshes;
+    oi_mesh_desc inl[INLINE_MESHES];  // small scenes: the table travels in the kernel arguments
     u64* ring;
-    uint32_t ring_mask;  // capacity - 1 (capacity is a power of two)
+    uint32_t ring_shift;  // log2 of the ring capacity
     oi_trav_ctl* ctl;
     uint4* hits;
     uint32_t hits_cap;
     uint32_t expand;
     uint32_t flags;
     u64 timeout_ticks;
+    uint32_t implicit_end;  // end of the scene's implicit index space
 };
 
 struct MeshRef {
@@ -353,9 +385,10 @@ __device__ __forceinline__ uint32_t find_mesh(const oi_mesh_desc* md, const uint
     return lo;
 }
 
-__device__ __forceinline__ MeshRef mesh_ref(const oi_mesh_desc* md, uint32_t m)
+// mesh m's descriptor: LDS copy for the first MESH_LDS meshes, else device memory
+__device__ __forceinline__ MeshRef mesh_ref(const oi_mesh_desc* md, const oi_mesh_desc* s_desc, uint32_t m)
 {
-    const oi_mesh_desc& d = md[m];
+    const oi_mesh_desc& d = (m < MESH_LDS) ? s_desc[m] : md[m];
     return MeshRef{d.nodes, d.sfaces, d.pos, d.perm, d.L, d.vl, d.ibase};
 }
 
@@ -408,14 +441,15 @@ __device__ __forceinline__ void tri_pair(uint32_t k, uint32_t n, uint32_t& x, ui
 }
 
 template <bool FP64>
-__device__ __forceinline__ void narrow_one(const TravArgs& A, const uint32_t* s_ibase, uint2 c, bool valid,
+__device__ __forceinline__ void narrow_one(const TravArgs& A, const uint32_t* s_ibase, const oi_mesh_desc* s_desc,
+                                           uint2 c, bool valid,
                                            bool& hit, uint4& out, uint32_t& n_fp64, uint32_t& n_adj)
 {
     hit = false;
     if (!valid) return;
     const uint32_t ma = find_mesh(A.meshes, s_ibase, A.n_meshes, c.x);
     const uint32_t mb = find_mesh(A.meshes, s_ibase, A.n_meshes, c.y);
-    const MeshRef ra = mesh_ref(A.meshes, ma), rb = mesh_ref(A.meshes, mb);
+    const MeshRef ra = mesh_ref(A.meshes, s_desc, ma), rb = mesh_ref(A.meshes, s_desc, mb);
     uint32_t sa = c.x - ra.ibase - ((1u << ra.L) - 1u);
     uint32_t sb = c.y - rb.ibase - ((1u << rb.L) - 1u);
     uint32_t fa[3], fb[3];
@@ -469,13 +503,26 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
     __shared__ uint2 s_stack[STACK_CAP];
     __shared__ uint2 s_cand[CAND_CAP];
     __shared__ uint32_t s_ibase[MESH_LDS];
+    __shared__ oi_mesh_desc s_desc[MESH_LDS];
     __shared__ uint32_t s_wsum[4];
     __shared__ uint32_t s_n, s_cn, s_pop_n, s_flag;
     __shared__ u64 s_pop_h, s_push_base;
 
     const uint32_t tid = threadIdx.x;
     oi_trav_ctl* ctl = A.ctl;
-    if (tid < MESH_LDS && tid < A.n_meshes) s_ibase[tid] = A.meshes[tid].ibase;
+    if (tid < MESH_LDS && tid < A.n_meshes) {
+        if (A.n_meshes <= INLINE_MESHES) {
+            // constant indices keep the argument block out of scratch memory
+            if (tid == 0) s_desc[0] = A.inl[0];
+            if (tid == 1) s_desc[1] = A.inl[1];
+            if (tid == 2) s_desc[2] = A.inl[2];
+            if (tid == 3) s_desc[3] = A.inl[3];
+        } else {
+            s_desc[tid] = A.meshes[tid];
+        }
+    }
+    __syncthreads();
+    if (tid < MESH_LDS && tid < A.n_meshes) s_ibase[tid] = s_desc[tid].ibase;
     if (tid == 0) {
         s_n = 0;
         s_cn = 0;
@@ -486,7 +533,7 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
     uint32_t n_fp64 = 0, n_adj = 0;
     bool idle = false;
     const u64 t0 = wall_clock64();
-    const uint32_t cap = A.ring_mask + 1u;
+    const uint32_t cap = 1u << A.ring_shift;
 
     for (;;) {
         // ---- refill from the ring when the local stack is short
@@ -517,10 +564,10 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
         if (pn) {
             const u64 h = s_pop_h;
             for (uint32_t i = tid; i < pn; i += TB) {
-                u64* slot = A.ring + ((h + i) & A.ring_mask);
-                u64 v = 0;
-                slot_take(slot, v, ctl, t0, A.timeout_ticks);
-                s_stack[s_n + i] = make_uint2((uint32_t)v, (uint32_t)(v >> 32));
+                uint2 v;
+                if (!slot_take(A.ring, A.ring_shift, h + i, v, ctl, t0, A.timeout_ticks))
+                    v = make_uint2(OI_BAD_ENTRY, OI_BAD_ENTRY);
+                s_stack[s_n + i] = v;
             }
             __syncthreads();
             if (tid == 0) s_n += pn;
@@ -552,9 +599,13 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
             idle = false;
         }
 
-        // ---- take one batch from the top of the stack
-        const uint32_t take = (n < TB) ? n : TB;
-        const bool have = tid < take;
+        // ---- take one batch from the top of the stack; near a full stack take
+        // only as many entries as the stack can absorb the children of
+        const uint32_t fan = 1u << (2u * A.expand);  // worst case children per entry
+        uint32_t take = (n < TB) ? n : TB;
+        const uint32_t fit = (STACK_CAP - n) / (fan - 1u);
+        if (take > fit) take = (fit > 64u) ? fit : (take < 64u ? take : 64u);
+        bool have = tid < take;
         uint2 e = have ? s_stack[n - take + tid] : make_uint2(0, 0);
         __syncthreads();
         const uint32_t nb = n - take;
@@ -565,10 +616,15 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
         Side sa = {0, 0}, sb = {0, 0};
         bool selfnode = false;
         uint32_t baseA = 0, baseB = 0;
+        if (have && (e.x >= A.implicit_end || e.y >= A.implicit_end)) {
+            // an entry dropped after a failed hand-over: never dereference it
+            __hip_atomic_fetch_or(&ctl->error, OI_ERR_CORRUPT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            have = false;
+        }
         if (have) {
             const uint32_t ma = find_mesh(A.meshes, s_ibase, A.n_meshes, e.x);
             const uint32_t mb = find_mesh(A.meshes, s_ibase, A.n_meshes, e.y);
-            const MeshRef ra = mesh_ref(A.meshes, ma), rb = mesh_ref(A.meshes, mb);
+            const MeshRef ra = mesh_ref(A.meshes, s_desc, ma), rb = mesh_ref(A.meshes, s_desc, mb);
             baseA = ra.ibase;
             baseB = rb.ibase;
             const uint32_t ia = e.x - ra.ibase, ib = e.y - rb.ibase;
@@ -628,8 +684,7 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
             const uint2 ce = make_uint2(baseA + x, baseB + y);
             const uint32_t k = off + c;
             if (k < spill) {
-                u64* slot = A.ring + ((s_push_base + k) & A.ring_mask);
-                slot_put(slot, (u64)ce.x | ((u64)ce.y << 32), ctl, t0, A.timeout_ticks);
+                slot_put(A.ring, A.ring_shift, s_push_base + k, ce, ctl, t0, A.timeout_ticks);
             } else {
                 s_stack[nb + k - spill] = ce;
             }
@@ -653,7 +708,7 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
             uint2 c = s_cand[cn - TB + tid];
             bool hit;
             uint4 out;
-            narrow_one<FP64>(A, s_ibase, c, true, hit, out, n_fp64, n_adj);
+            narrow_one<FP64>(A, s_ibase, s_desc, c, true, hit, out, n_fp64, n_adj);
             uint32_t HT;
             const uint32_t hoff = block_scan(hit ? 1u : 0u, s_wsum, HT);
             if (tid == 0) {
@@ -664,7 +719,10 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
             __syncthreads();
             if (hit) {
                 const uint32_t pos = (uint32_t)s_pop_h + hoff;
-                if (pos < A.hits_cap) A.hits[pos] = out;
+                if (pos < A.hits_cap)
+                    A.hits[pos] = out;
+                else
+                    __hip_atomic_fetch_or(&ctl->error, OI_ERR_HITS_OVERFLOW, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             __syncthreads();
         }
@@ -685,9 +743,7 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
         if (s_flag) {
             const uint32_t D = s_flag, m = s_n;
             for (uint32_t i = tid; i < D; i += TB) {
-                const uint2 ce = s_stack[i];
-                u64* slot = A.ring + ((s_push_base + i) & A.ring_mask);
-                slot_put(slot, (u64)ce.x | ((u64)ce.y << 32), ctl, t0, A.timeout_ticks);
+                slot_put(A.ring, A.ring_shift, s_push_base + i, s_stack[i], ctl, t0, A.timeout_ticks);
             }
             // shift the rest down chunk by chunk: chunk r's destination lies
             // below every later chunk's source
@@ -712,7 +768,7 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
         bool hit = false;
         uint4 out;
         uint2 c = (tid < cn) ? s_cand[tid] : make_uint2(0, 0);
-        narrow_one<FP64>(A, s_ibase, c, tid < cn, hit, out, n_fp64, n_adj);
+        narrow_one<FP64>(A, s_ibase, s_desc, c, tid < cn, hit, out, n_fp64, n_adj);
         uint32_t HT;
         const uint32_t hoff = block_scan(hit ? 1u : 0u, s_wsum, HT);
         if (tid == 0)
@@ -720,9 +776,13 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
         __syncthreads();
         if (hit) {
             const uint32_t pos = (uint32_t)s_pop_h + hoff;
-            if (pos < A.hits_cap) A.hits[pos] = out;
+            if (pos < A.hits_cap)
+                A.hits[pos] = out;
+            else
+                __hip_atomic_fetch_or(&ctl->sticky, OI_ERR_HITS_OVERFLOW, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
+
     // ---- statistics (one atomic per workgroup per counter)
     uint32_t t_;
     const uint32_t f64tot = block_scan(n_fp64, s_wsum, t_);
@@ -745,14 +805,24 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
     }
 }
 
-// seed: reset the counters and place the initial front in the ring
-__global__ void k_seed(u64* ring, const uint2* front, uint32_t n0, oi_trav_ctl* ctl)
+// seed: place the initial front at the ring's current head (after a clean
+// traversal head == tail and every slot waits for its next lap) and reset the
+// counters. Only block 0 / thread 0 writes the counters; every thread reads
+// `head`, which no one writes during this kernel.
+__global__ void k_seed(u64* ring, uint32_t shift, const uint2* front, uint32_t n0, oi_trav_ctl* ctl,
+                       uint32_t reset_sticky)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n0) ring[i] = (u64)front[i].x | ((u64)front[i].y << 32);
+    const u64 base = ctl->head;
+    if (i < n0) {
+        const u64 idx = base + i;
+        ring[idx & ((1u << shift) - 1u)] = ring_pack(front[i].x, front[i].y, (uint32_t)(idx >> shift) & 15u);
+    }
     if (i == 0) {
-        ctl->head = 0;
-        ctl->tail = n0;
+        // errors of the previous traversal move to `sticky` unless the caller
+        // checked it synchronously
+        ctl->sticky = reset_sticky ? 0u : (ctl->sticky | ctl->error);
+        ctl->tail = base + n0;
         ctl->pending = (int)n0;
         ctl->n_hits = 0;
         ctl->error = 0;
@@ -762,6 +832,18 @@ __global__ void k_seed(u64* ring, const uint2* front, uint32_t n0, oi_trav_ctl* 
         ctl->fp64 = 0;
         ctl->adj = 0;
         ctl->exited = 0;
+    }
+}
+
+// ring reset (new ring, or after an aborted traversal): every slot EMPTY(0),
+// head = tail = 0
+__global__ void k_ring_reset(u64* ring, uint32_t n, oi_trav_ctl* ctl)
+{
+    const u64 e0 = ring_empty(0);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) ring[i] = e0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->head = 0;
+        ctl->tail = 0;
     }
 }
 
@@ -781,12 +863,20 @@ hipError_t oi_launch_fill64(hipStream_t st, unsigned long long* p, uint32_t n, u
     return hipGetLastError();
 }
 
-hipError_t oi_launch_seed(hipStream_t st, unsigned long long* ring, const uint2* front, uint32_t n0,
-                          oi_trav_ctl* ctl)
+hipError_t oi_launch_ring_reset(hipStream_t st, unsigned long long* ring, uint32_t cap, oi_trav_ctl* ctl)
+{
+    uint32_t g = (cap + 255) / 256;
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(k_ring_reset, dim3(g), dim3(256), 0, st, ring, cap, ctl);
+    return hipGetLastError();
+}
+
+hipError_t oi_launch_seed(hipStream_t st, unsigned long long* ring, uint32_t cap, const uint2* front, uint32_t n0,
+                          oi_trav_ctl* ctl, uint32_t reset_sticky)
 {
     uint32_t g = (n0 + 255) / 256;
     if (g == 0) g = 1;
-    hipLaunchKernelGGL(k_seed, dim3(g), dim3(256), 0, st, ring, front, n0, ctl);
+    hipLaunchKernelGGL(k_seed, dim3(g), dim3(256), 0, st, ring, oi_ilog2(cap), front, n0, ctl, reset_sticky);
     return hipGetLastError();
 }
 
@@ -799,13 +889,25 @@ hipError_t oi_traverse_occupancy(int* blocks_per_cu)
     return e;
 }
 
-hipError_t oi_launch_traverse(hipStream_t st, uint32_t grid, const oi_mesh_desc* meshes, uint32_t n_meshes,
+hipError_t oi_launch_traverse(hipStream_t st, uint32_t grid, const oi_mesh_desc* meshes, const oi_mesh_desc* host_meshes,
+                              uint32_t n_meshes,
                               unsigned long long* ring, uint32_t ring_cap, oi_trav_ctl* ctl,
                               uint4* hits, uint32_t hits_cap, uint32_t expand, uint32_t flags,
-                              unsigned long long timeout_ticks)
+                              unsigned long long timeout_ticks, uint32_t implicit_end)
 {
-    TravArgs a{meshes, n_meshes, ring, ring_cap - 1u, ctl, hits, hits_cap, expand, flags,
-               timeout_ticks};
+    TravArgs a{};
+    a.meshes = meshes;
+    a.n_meshes = n_meshes;
+    for (uint32_t m = 0; m < n_meshes && m < INLINE_MESHES; m++) a.inl[m] = host_meshes[m];
+    a.ring = ring;
+    a.ring_shift = oi_ilog2(ring_cap);
+    a.ctl = ctl;
+    a.hits = hits;
+    a.hits_cap = hits_cap;
+    a.expand = expand;
+    a.flags = flags;
+    a.timeout_ticks = timeout_ticks;
+    a.implicit_end = implicit_end;
     if (flags & OI_DETECT_FP64_FALLBACK)
         hipLaunchKernelGGL(k_traverse<true>, dim3(grid), dim3(TB), 0, st, a);
     else

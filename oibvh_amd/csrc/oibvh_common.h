@@ -93,13 +93,18 @@ struct oi_trav_ctl {
     unsigned long long fp64;  // SAT axis comparisons re-evaluated in double
     unsigned long long adj;   // leaf pairs dropped as adjacent (self mode)
     uint32_t exited;          // workgroups that left the loop
-    uint32_t pad;
+    uint32_t sticky;          // OI_ERR_* bits of earlier asynchronous traversals not checked since
 };
 
 #define OI_ERR_QUEUE_OVERFLOW 1u
 #define OI_ERR_TIMEOUT 2u
+#define OI_ERR_CORRUPT 4u
+#define OI_ERR_HITS_OVERFLOW 8u
 
-#define OI_EMPTY_SLOT 0xFFFFFFFFFFFFFFFFull
+// implicit index pair that marks a dropped BVTT entry (never dereferenced)
+#define OI_BAD_ENTRY 0xFFFFFFFFu
+// a scene's concatenated implicit index space must stay below this bound
+#define OI_IMPLICIT_LIMIT 0x3fffffffu
 
 // detect flags (mirror include/oibvh.h)
 #define OI_DETECT_SELF 1u

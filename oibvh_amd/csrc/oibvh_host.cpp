@@ -28,10 +28,13 @@ hipError_t oi_launch_sort_pass(hipStream_t, const uint32_t*, const uint32_t*, ui
 hipError_t oi_launch_reduce(hipStream_t, int, const float*, const uint32_t*, const uint32_t*, uint32_t*, float*,
                             uint32_t*, uint32_t, uint32_t, uint32_t);
 hipError_t oi_launch_fill64(hipStream_t, unsigned long long*, uint32_t, unsigned long long);
-hipError_t oi_launch_seed(hipStream_t, unsigned long long*, const uint2*, uint32_t, oi_trav_ctl*);
+hipError_t oi_launch_ring_reset(hipStream_t, unsigned long long*, uint32_t, oi_trav_ctl*);
+hipError_t oi_launch_seed(hipStream_t, unsigned long long*, uint32_t, const uint2*, uint32_t, oi_trav_ctl*,
+                          uint32_t);
 hipError_t oi_traverse_occupancy(int*);
-hipError_t oi_launch_traverse(hipStream_t, uint32_t, const oi_mesh_desc*, uint32_t, unsigned long long*, uint32_t,
-                              oi_trav_ctl*, uint4*, uint32_t, uint32_t, uint32_t, unsigned long long);
+hipError_t oi_launch_traverse(hipStream_t, uint32_t, const oi_mesh_desc*, const oi_mesh_desc*, uint32_t,
+                              unsigned long long*, uint32_t,
+                              oi_trav_ctl*, uint4*, uint32_t, uint32_t, uint32_t, unsigned long long, uint32_t);
 }
 
 // ---------------------------------------------------------------------------
@@ -79,6 +82,7 @@ struct oibvh_scene {
     uint32_t n_front = 0;
     std::vector<uint2> h_front;
     std::vector<oi_mesh_desc> h_desc;  // host copies stay alive until the next detect
+    uint32_t implicit_end = 0;
     uint32_t front_key[3] = {~0u, ~0u, ~0u}; // entry, flags&SELF, #trees
     unsigned long long* ring = nullptr;
     uint32_t ring_cap = 0;
@@ -425,6 +429,17 @@ extern "C" oibvh_status oibvh_tree_refit(oibvh_tree* t, const float* xyz, uint32
     return st;
 }
 
+extern "C" oibvh_status oibvh_tree_set_positions(oibvh_tree* t, const float* xyz, uint32_t ptr_kind)
+{
+    if (!t || !xyz) return OIBVH_ERR_INVALID;
+    oibvh_ctx* c = t->ctx;
+    if (ptr_kind > OIBVH_PTR_DEVICE_BORROW) return set_err(c, OIBVH_ERR_INVALID, "bad ptr_kind");
+    HIPCHK(c, hipSetDevice(c->device));
+    oibvh_status st = set_positions(t, xyz, ptr_kind);
+    if (st == OIBVH_OK && ptr_kind == OIBVH_PTR_HOST) HIPCHK(c, hipStreamSynchronize(c->stream));
+    return st;
+}
+
 extern "C" oibvh_status oibvh_tree_get_depth(const oibvh_tree* t, uint32_t* d)
 {
     if (!t || !d) return OIBVH_ERR_INVALID;
@@ -592,7 +607,8 @@ static oibvh_status ensure_ring(oibvh_scene* s, uint32_t cap)
         s->ring_dirty = true;
     }
     if (s->ring_dirty) {
-        HIPCHK(c, oi_launch_fill64(c->stream, s->ring, s->ring_cap, OI_EMPTY_SLOT));
+        // every slot to its first lap, head = tail = 0
+        HIPCHK(c, oi_launch_ring_reset(c->stream, s->ring, s->ring_cap, s->ctl));
         s->ring_dirty = false;
     }
     return OIBVH_OK;
@@ -610,7 +626,7 @@ static oibvh_status ensure_hits(oibvh_scene* s, uint32_t cap)
     return OIBVH_OK;
 }
 
-static oibvh_status enqueue_detect(oibvh_scene* s)
+static oibvh_status enqueue_detect(oibvh_scene* s, bool reset_sticky)
 {
     oibvh_ctx* c = s->ctx;
     hipStream_t st = c->stream;
@@ -634,14 +650,19 @@ static oibvh_status enqueue_detect(oibvh_scene* s)
         d.ibase = (uint32_t)acc;
         acc += (2ull << t->L) - 1ull;
     }
-    if (acc >= 0xFFFFFFFFull) return set_err(c, OIBVH_ERR_INVALID, "scene too large for 32-bit BVTT indices");
-    if (s->desc_cap < n) {
-        dfree(s->d_desc);
-        s->desc_cap = 0;
-        HIPCHK(c, dalloc(&s->d_desc, n));
-        s->desc_cap = n;
+    if (acc > OI_IMPLICIT_LIMIT)
+        return set_err(c, OIBVH_ERR_INVALID, "scene too large: %llu implicit BVTT indices (limit %u)",
+                       (unsigned long long)acc, OI_IMPLICIT_LIMIT);
+    s->implicit_end = (uint32_t)acc;
+    if (n > 4) {  // small scenes pass the table in the kernel arguments
+        if (s->desc_cap < n) {
+            dfree(s->d_desc);
+            s->desc_cap = 0;
+            HIPCHK(c, dalloc(&s->d_desc, n));
+            s->desc_cap = n;
+        }
+        HIPCHK(c, hipMemcpyAsync(s->d_desc, desc.data(), n * sizeof(oi_mesh_desc), hipMemcpyHostToDevice, st));
     }
-    HIPCHK(c, hipMemcpyAsync(s->d_desc, desc.data(), n * sizeof(oi_mesh_desc), hipMemcpyHostToDevice, st));
     const bool self = (s->last_flags & OIBVH_DETECT_SELF) != 0;
     const uint32_t key[3] = {s->last_entry, self ? 1u : 0u, n};
     if (std::memcmp(key, s->front_key, sizeof key) != 0) {
@@ -665,11 +686,13 @@ static oibvh_status enqueue_detect(oibvh_scene* s)
         r = ensure_hits(s, c->hits_cap);
         if (r != OIBVH_OK) return r;
     }
-    HIPCHK(c, oi_launch_seed(st, s->ring, s->d_front, s->n_front, s->ctl));
+    HIPCHK(c, oi_launch_seed(st, s->ring, s->ring_cap, s->d_front, s->n_front, s->ctl, reset_sticky ? 1u : 0u));
     uint32_t grid = c->trav_blocks ? c->trav_blocks : (uint32_t)std::max(1, c->cus) * (uint32_t)c->trav_occ;
     const unsigned long long timeout = 30ull * 1000ull * c->clock_khz;  // 30 s
-    HIPCHK(c, oi_launch_traverse(st, grid, s->d_desc, n, s->ring, s->ring_cap, s->ctl, s->hits, s->hits_cap,
-                                 std::min(3u, std::max(1u, s->last_expand)), s->last_flags & 3u, timeout));
+    HIPCHK(c, oi_launch_traverse(st, grid, n > 4 ? s->d_desc : nullptr, desc.data(), n, s->ring, s->ring_cap,
+                                 s->ctl, s->hits, s->hits_cap,
+                                 std::min(3u, std::max(1u, s->last_expand)), s->last_flags & 3u, timeout,
+                                 s->implicit_end));
     s->unchecked = true;
     s->have_result = true;
     return OIBVH_OK;
@@ -680,29 +703,30 @@ static oibvh_status check_detect(oibvh_scene* s)
 {
     oibvh_ctx* c = s->ctx;
     if (!s->unchecked) return OIBVH_OK;
-    uint32_t retries = 0;
+    uint32_t retries = 0, earlier = 0;
     for (;;) {
         HIPCHK(c, hipMemcpyAsync(s->h_ctl, s->ctl, sizeof(oi_trav_ctl), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         const oi_trav_ctl& k = *s->h_ctl;
-        if (k.error & OI_ERR_TIMEOUT) {
+        earlier |= k.sticky;  // errors of unchecked asynchronous detects before this one
+        if (k.error & (OI_ERR_TIMEOUT | OI_ERR_CORRUPT)) {
             s->ring_dirty = true;
             s->unchecked = false;
             s->have_result = false;
-            return set_err(c, OIBVH_ERR_TIMEOUT, "traversal timed out");
+            return set_err(c, OIBVH_ERR_TIMEOUT, "traversal timed out (error bits 0x%x)", k.error);
         }
         bool rerun = false;
         if (k.error & OI_ERR_QUEUE_OVERFLOW) {
             s->ring_dirty = true;
-            if (s->ring_cap >= (1u << 30)) {
+            if (s->ring_cap >= (1u << 28)) {
                 s->unchecked = false;
                 s->have_result = false;
                 return set_err(c, OIBVH_ERR_OVERFLOW, "work queue overflow at %u entries", s->ring_cap);
             }
-            oibvh_status r = ensure_ring(s, s->ring_cap * 2u);
+            oibvh_status r = ensure_ring(s, std::min(1u << 28, s->ring_cap * 8u));
             if (r != OIBVH_OK) return r;
             rerun = true;
-        } else if (k.n_hits > s->hits_cap) {
+        } else if (k.n_hits > s->hits_cap || (k.error & OI_ERR_HITS_OVERFLOW)) {
             oibvh_status r = ensure_hits(s, oi_np2(k.n_hits + (k.n_hits >> 3)));
             if (r != OIBVH_OK) return r;
             rerun = true;
@@ -715,12 +739,13 @@ static oibvh_status check_detect(oibvh_scene* s)
             s->stats.adjacent = k.adj;
             s->stats.pairs = k.n_hits;
             s->stats.retries = retries;
+            s->stats.async_errors = earlier;
             s->unchecked = false;
             return OIBVH_OK;
         }
         retries++;
-        if (retries > 8) return set_err(c, OIBVH_ERR_OVERFLOW, "detect: too many retries");
-        oibvh_status r = enqueue_detect(s);
+        if (retries > 16) return set_err(c, OIBVH_ERR_OVERFLOW, "detect: too many retries");
+        oibvh_status r = enqueue_detect(s, true);
         if (r != OIBVH_OK) return r;
     }
 }
@@ -732,7 +757,8 @@ extern "C" oibvh_status oibvh_scene_detect(oibvh_scene* s, uint32_t entry, uint3
     HIPCHK(c, hipSetDevice(c->device));
     for (oibvh_tree* t : s->trees)
         if (!t->built) return set_err(c, OIBVH_ERR_NOT_BUILT, "detect: tree not built");
-    if (s->unchecked) {
+    const bool async = (flags & OIBVH_DETECT_ASYNC) != 0;
+    if (s->unchecked && !async) {
         oibvh_status r = check_detect(s);
         if (r != OIBVH_OK) return r;
     }
@@ -746,9 +772,11 @@ extern "C" oibvh_status oibvh_scene_detect(oibvh_scene* s, uint32_t entry, uint3
         s->unchecked = false;
         return OIBVH_OK;
     }
-    oibvh_status r = enqueue_detect(s);
+    // an asynchronous detect keeps the error history of unchecked earlier
+    // ones (reported as async_errors); a synchronous one starts afresh
+    oibvh_status r = enqueue_detect(s, !async);
     if (r != OIBVH_OK) return r;
-    if (flags & OIBVH_DETECT_ASYNC) return OIBVH_OK;
+    if (async) return OIBVH_OK;
     return check_detect(s);
 }
 
