@@ -58,6 +58,8 @@ typedef struct oibvh_scene oibvh_scene;
  * the library creates one at default priority). No reference counterpart: the
  * reference uses cudaSetDevice and the default stream (scene.cu:229). */
 OIBVH_API oibvh_status oibvh_ctx_create(int device, void* stream, oibvh_ctx** out);
+/* Trees and scenes keep their context alive: destroying a context that still
+ * has live objects defers the release until the last of them is destroyed. */
 OIBVH_API void oibvh_ctx_destroy(oibvh_ctx* ctx);
 /* text of the last error on this context ("" if none) */
 OIBVH_API const char* oibvh_last_error(const oibvh_ctx* ctx);

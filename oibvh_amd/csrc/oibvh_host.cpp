@@ -50,7 +50,26 @@ struct oibvh_ctx {
     uint32_t trav_blocks = 0;
     uint32_t queue_cap = 1u << 22;
     uint32_t hits_cap = 1u << 20;
+    int live = 0;                // trees and scenes of this context not yet destroyed
+    bool destroy_pending = false; // oibvh_ctx_destroy called while objects were live
 };
+
+static void ctx_free(oibvh_ctx* ctx)
+{
+    (void)hipSetDevice(ctx->device);
+    if (ctx->own_stream) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+}
+
+// a context outlives its trees and scenes: destroying it while they are live
+// only marks it, and the last object to go frees it
+static void ctx_release(oibvh_ctx* ctx)
+{
+    if (--ctx->live == 0 && ctx->destroy_pending) ctx_free(ctx);
+}
 
 struct oibvh_tree {
     oibvh_ctx* ctx = nullptr;
@@ -185,12 +204,11 @@ extern "C" oibvh_status oibvh_ctx_create(int device, void* stream, oibvh_ctx** o
 extern "C" void oibvh_ctx_destroy(oibvh_ctx* ctx)
 {
     if (!ctx) return;
-    (void)hipSetDevice(ctx->device);
-    if (ctx->own_stream) {
-        (void)hipStreamSynchronize(ctx->stream);
-        (void)hipStreamDestroy(ctx->stream);
+    if (ctx->live > 0) {
+        ctx->destroy_pending = true;
+        return;
     }
-    delete ctx;
+    ctx_free(ctx);
 }
 
 extern "C" const char* oibvh_last_error(const oibvh_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -301,6 +319,7 @@ extern "C" oibvh_status oibvh_tree_create(oibvh_ctx* ctx, const float* xyz, uint
         delete t;
         return st;
     }
+    ctx->live++;
     *out = t;
     return OIBVH_OK;
 }
@@ -340,6 +359,7 @@ extern "C" oibvh_status oibvh_tree_create_like(oibvh_tree* other, const float* x
         return st;
     }
     t->built = other->built;
+    ctx->live++;
     *out = t;
     return OIBVH_OK;
 }
@@ -347,10 +367,12 @@ extern "C" oibvh_status oibvh_tree_create_like(oibvh_tree* other, const float* x
 extern "C" void oibvh_tree_destroy(oibvh_tree* t)
 {
     if (!t) return;
-    (void)hipSetDevice(t->ctx->device);
-    (void)hipStreamSynchronize(t->ctx->stream);
+    oibvh_ctx* c = t->ctx;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
     tree_free(t);
     delete t;
+    ctx_release(c);
 }
 
 static int pick_lpt(const oibvh_tree* t)
@@ -534,6 +556,7 @@ extern "C" oibvh_status oibvh_scene_create(oibvh_ctx* ctx, oibvh_scene** out)
         return set_err(ctx, OIBVH_ERR_HIP, "scene alloc: %s", hipGetErrorString(e));
     }
     std::memset(s->h_ctl, 0, sizeof(oi_trav_ctl));
+    ctx->live++;
     *out = s;
     return OIBVH_OK;
 }
@@ -549,7 +572,9 @@ extern "C" void oibvh_scene_destroy(oibvh_scene* s)
     dfree(s->ctl);
     dfree(s->hits);
     if (s->h_ctl) (void)hipHostFree(s->h_ctl);
+    oibvh_ctx* c = s->ctx;
     delete s;
+    ctx_release(c);
 }
 
 extern "C" oibvh_status oibvh_scene_add_tree(oibvh_scene* s, oibvh_tree* t)
