@@ -180,16 +180,21 @@ typedef struct {
  * counts, scene.cu:318-321, 417-421). Synchronises. */
 OIBVH_API oibvh_status oibvh_scene_get_stats(oibvh_scene* scene, oibvh_detect_stats* stats);
 
-/* Tuning knobs (no reference counterpart; none changes results):
- *   reduce_leaves_per_thread  1, 2, 4 or 8 (0 = automatic): subtree size of the
- *                             build/refit reduce is 256 * this many leaves
- *   traverse_blocks           persistent traversal grid (0 = CUs x occupancy)
- *   queue_capacity            initial work-ring entries (power of two, 0 = default 2^22)
- *   hits_capacity             initial hit-list entries (0 = default 2^20)
- * Both buffers grow on overflow (the detect is re-run). */
-OIBVH_API oibvh_status oibvh_ctx_set_tuning(oibvh_ctx* ctx, uint32_t reduce_leaves_per_thread,
-                                            uint32_t traverse_blocks, uint32_t queue_capacity,
-                                            uint32_t hits_capacity);
+/* Tuning knobs (no reference counterpart; none changes results). Zero fields
+ * select the defaults. */
+typedef struct {
+    uint32_t reduce_leaves_per_thread; /* 1, 2, 4 or 8: the build/refit reduce gives each workgroup a
+                                          subtree of 256 x this many leaves (0 = by mesh size) */
+    uint32_t traverse_blocks;          /* persistent traversal grid (0 = CUs x resident workgroups) */
+    uint32_t traverse_expand;          /* levels each side descends per BVTT step, 1..3 (0 = the
+                                          detect call's expandLevels, clamped to 1..3) */
+    uint32_t queue_capacity;           /* initial work-ring entries, a power of two (0 = 2^22) */
+    uint32_t hits_capacity;            /* initial hit-list entries (0 = 2^20) */
+} oibvh_tuning;
+
+/* Both buffers grow on overflow (the detect is re-run). */
+OIBVH_API oibvh_status oibvh_ctx_set_tuning(oibvh_ctx* ctx, const oibvh_tuning* tuning);
+OIBVH_API oibvh_status oibvh_ctx_get_tuning(const oibvh_ctx* ctx, oibvh_tuning* tuning);
 
 #ifdef __cplusplus
 }

@@ -30,6 +30,11 @@ _vp, _u32, _u64, _i = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
 _f32p, _u32p = C.POINTER(C.c_float), C.POINTER(C.c_uint32)
 
 
+class Tuning(C.Structure):
+    _fields_ = [("reduce_leaves_per_thread", C.c_uint32), ("traverse_blocks", C.c_uint32),
+                ("traverse_expand", C.c_uint32), ("queue_capacity", C.c_uint32), ("hits_capacity", C.c_uint32)]
+
+
 class DetectStats(C.Structure):
     _fields_ = [("bvtt_tests", C.c_uint64), ("candidates", C.c_uint64), ("fp64_rechecks", C.c_uint64),
                 ("adjacent", C.c_uint64), ("pairs", C.c_uint32), ("retries", C.c_uint32),
@@ -42,7 +47,8 @@ API = [
     ("oibvh_last_error", C.c_char_p, [_vp]),
     ("oibvh_ctx_synchronize", _i, [_vp]),
     ("oibvh_ctx_stream", _vp, [_vp]),
-    ("oibvh_ctx_set_tuning", _i, [_vp, _u32, _u32, _u32, _u32]),
+    ("oibvh_ctx_set_tuning", _i, [_vp, C.POINTER(Tuning)]),
+    ("oibvh_ctx_get_tuning", _i, [_vp, C.POINTER(Tuning)]),
     ("oibvh_tree_create", _i, [_vp, _vp, _u32, _vp, _u32, _u32, C.POINTER(_vp)]),
     ("oibvh_tree_create_like", _i, [_vp, _vp, _u32, _u32, C.POINTER(_vp)]),
     ("oibvh_tree_destroy", None, [_vp]),

@@ -49,10 +49,21 @@ class Context:
     def synchronize(self) -> None:
         _check(self.h, self._lib.oibvh_ctx_synchronize(self.h), "synchronize")
 
-    def set_tuning(self, leaves_per_thread: int = 0, traverse_blocks: int = 0, queue_capacity: int = 0,
-                   hits_capacity: int = 0) -> None:
-        _check(self.h, self._lib.oibvh_ctx_set_tuning(self.h, leaves_per_thread, traverse_blocks, queue_capacity,
-                                                      hits_capacity), "set_tuning")
+    def set_tuning(self, **kw) -> None:
+        """Set tuning knobs (oibvh_tuning field names); unspecified ones keep
+        their current values."""
+        t = self.get_tuning()
+        for k, v in kw.items():
+            if k not in t:
+                raise KeyError(k)
+            t[k] = v
+        s = L.Tuning(**t)
+        _check(self.h, self._lib.oibvh_ctx_set_tuning(self.h, C.byref(s)), "set_tuning")
+
+    def get_tuning(self) -> dict:
+        s = L.Tuning()
+        _check(self.h, self._lib.oibvh_ctx_get_tuning(self.h, C.byref(s)), "get_tuning")
+        return {f: getattr(s, f) for f, _ in L.Tuning._fields_}
 
 
 class DeviceBuffer:

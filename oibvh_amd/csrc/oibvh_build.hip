@@ -1,16 +1,18 @@
 // oibvh_build.hip -- gfx950 kernels for oibvh tree build and refit.
 //
 // Build (OibvhTree::build, oibvhTree.cu:237-388):
-//   k_mesh_box     vertex AABB (the Morton quantisation box, Mesh::setupAABB
-//                  mesh.cpp:91-98); also clears the sort's scratch counters
-//   k_morton       leaf AABB centroid -> 30-bit Morton key per face
+//   k_mesh_box     per-block vertex AABBs (the Morton quantisation box,
+//                  Mesh::setupAABB mesh.cpp:91-98); also clears the sort's
+//                  scratch counters
+//   k_morton       folds the block boxes, then leaf AABB centroid -> 30-bit
+//                  Morton key per face
 //                  (calculate_aabb_and_morton_kernel, oibvh.cu:42-70) and the
 //                  four 8-bit digit histograms of the keys
 //   k_sort_pass x4 onesweep LSD radix pass with decoupled look-back; stable,
 //                  so the permutation equals thrust::stable_sort_by_key's
 //                  (oibvhTree.cu:287-297)
 //   k_reduce       leaves in sorted order + bottom-up AABB reduce, one subtree
-//                  per workgroup and the last workgroup finishing the apex
+//                  per workgroup; k_reduce_apex climbs the rest in one workgroup
 //                  (oibvh_tree_construction_kernel2, oibvh.cu:156-219)
 // Refit (OibvhTree::refit, oibvhTree.cu:193-235) is k_reduce alone over the
 // already sorted faces (calculate_aabb_kernel + oibvh_tree_construction_kernel,
@@ -69,14 +71,12 @@ __device__ __forceinline__ float hi_c(float l, float r) { return (r < l) ? l : r
 
 __global__ void __launch_bounds__(OI_BLOCK)
 k_mesh_box(const float* __restrict__ xyz, uint32_t V, uint32_t chunk, float* __restrict__ partials,
-           uint32_t* __restrict__ counter, float* __restrict__ box_out, uint32_t* __restrict__ clear_u32,
-           uint32_t clear_n)
+           uint32_t* __restrict__ clear_u32, uint32_t clear_n)
 {
     const uint32_t tid = threadIdx.x;
     for (uint32_t i = blockIdx.x * OI_BLOCK + tid; i < clear_n; i += gridDim.x * OI_BLOCK) clear_u32[i] = 0u;
 
     __shared__ float s_v[6][OI_BLOCK];
-    __shared__ uint32_t s_last;
     const float INF = __builtin_inff();
     float mn[3] = {INF, INF, INF}, mx[3] = {-INF, -INF, -INF};
     const uint32_t b0 = blockIdx.x * chunk;
@@ -109,30 +109,19 @@ k_mesh_box(const float* __restrict__ xyz, uint32_t V, uint32_t chunk, float* __r
         __syncthreads();
     }
     if (tid < 6) partials[6u * blockIdx.x + tid] = s_v[tid][0];
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) {
-        uint32_t old = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = (old == gridDim.x - 1u);
-    }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    // last block: fold the partials in block order
-    const uint32_t nb = gridDim.x;
-    const uint32_t psub = (nb + OI_BLOCK - 1) / OI_BLOCK;
-    float pm[3] = {INF, INF, INF}, pM[3] = {-INF, -INF, -INF};
-    for (uint32_t b = tid * psub; b < min(nb, tid * psub + psub); b++) {
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            pm[k] = lo_c(pm[k], partials[6u * b + k]);
-            pM[k] = hi_c(pM[k], partials[6u * b + 3 + k]);
-        }
-    }
+}
+
+// Fold the per-block partial boxes (at most OI_BLOCK of them) in block order and
+// then with the default box (+-FLT_MAX) in front; every thread gets the result.
+__device__ void fold_partials(const float* __restrict__ partials, uint32_t nparts, float (*s_v)[OI_BLOCK],
+                              float* bmn, float* bmx)
+{
+    const uint32_t tid = threadIdx.x;
+    const float INF = __builtin_inff();
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        s_v[k][tid] = pm[k];
-        s_v[3 + k][tid] = pM[k];
+        s_v[k][tid] = (tid < nparts) ? partials[6u * tid + k] : INF;
+        s_v[3 + k][tid] = (tid < nparts) ? partials[6u * tid + 3 + k] : -INF;
     }
     __syncthreads();
     for (uint32_t s = 1; s < OI_BLOCK; s <<= 1) {
@@ -145,11 +134,12 @@ k_mesh_box(const float* __restrict__ xyz, uint32_t V, uint32_t chunk, float* __r
         }
         __syncthreads();
     }
-    if (tid < 3) {
-        box_out[tid] = lo_c(__FLT_MAX__, s_v[tid][0]);
-        box_out[3 + tid] = hi_c(-__FLT_MAX__, s_v[3 + tid][0]);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        bmn[k] = lo_c(__FLT_MAX__, s_v[k][0]);
+        bmx[k] = hi_c(-__FLT_MAX__, s_v[3 + k][0]);
     }
-    if (tid == 0) *counter = 0u;
+    __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
@@ -176,15 +166,20 @@ __device__ __forceinline__ uint32_t quantise10(float p)
 
 __global__ void __launch_bounds__(OI_BLOCK)
 k_morton(const uint32_t* __restrict__ faces, const float* __restrict__ xyz, uint32_t F,
-         const float* __restrict__ box, uint32_t* __restrict__ keys, uint32_t* __restrict__ ghist)
+         const float* __restrict__ partials, uint32_t nparts, float* __restrict__ box_out,
+         uint32_t* __restrict__ keys, uint32_t* __restrict__ ghist)
 {
     __shared__ uint32_t s_h[4][256];
+    __shared__ float s_v[6][OI_BLOCK];
     const uint32_t tid = threadIdx.x;
 #pragma unroll
     for (int d = 0; d < 4; d++) s_h[d][tid] = 0u;
-    __syncthreads();
-    const float bmn[3] = {box[0], box[1], box[2]};
-    const float bmx[3] = {box[3], box[4], box[5]};
+    float bmn[3], bmx[3];
+    fold_partials(partials, nparts, s_v, bmn, bmx);
+    if (blockIdx.x == 0 && tid < 3) {
+        box_out[tid] = bmn[tid];
+        box_out[3 + tid] = bmx[tid];
+    }
     for (uint32_t i = blockIdx.x * OI_BLOCK + tid; i < F; i += gridDim.x * OI_BLOCK) {
         const uint32_t f0 = faces[3u * i], f1 = faces[3u * i + 1], f2 = faces[3u * i + 2];
         uint32_t q[3];
@@ -395,7 +390,6 @@ struct ReduceArgs {
     const uint32_t* perm;   // build: leaf p takes face perm[p]; refit: nullptr
     uint32_t* sfaces_out;   // build: sorted faces written here; refit: nullptr
     float* nodes;
-    uint32_t* counter;      // zero between launches (self-resetting)
     uint32_t F, L, vl;
 };
 
@@ -510,26 +504,22 @@ __global__ void __launch_bounds__(OI_BLOCK) k_reduce(ReduceArgs a)
 {
     __shared__ float sA[6][OI_BLOCK];
     __shared__ float sB[6][OI_BLOCK];
-    __shared__ uint32_t s_last;
     constexpr uint32_t LGM = (M == 1) ? 0 : (M == 2) ? 1 : (M == 4) ? 2 : 3;
     const uint32_t L = a.L, vl = a.vl;
     const uint32_t h = (L < 8 + LGM) ? L : 8 + LGM;
-    const uint32_t w = blockIdx.x;
     LeafLoad ll{a, oi_lvl_start(L, L, vl)};
-    reduce_subtree<M>(ll, a.nodes, L, vl, L, h, w, sA, sB);
-    uint32_t lc = L - h;
-    if (lc == 0) return;
-    // the stage-1 root of this workgroup was stored by thread 0
-    __threadfence();
-    if (threadIdx.x == 0) {
-        const uint32_t old =
-            __hip_atomic_fetch_add(a.counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = (old == gridDim.x - 1u);
-        if (s_last) *a.counter = 0u;
-    }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
+    reduce_subtree<M>(ll, a.nodes, L, vl, L, h, blockIdx.x, sA, sB);
+}
+
+// Stage 2, one workgroup: the levels above the stage-1 subtree roots (level
+// lc), in chunks of up to 2^11 nodes per climb. A separate launch, so the
+// roots written by every stage-1 workgroup are visible without device-scope
+// fences.
+__global__ void __launch_bounds__(OI_BLOCK) k_reduce_apex(ReduceArgs a, uint32_t lc)
+{
+    __shared__ float sA[6][OI_BLOCK];
+    __shared__ float sB[6][OI_BLOCK];
+    const uint32_t L = a.L, vl = a.vl;
     while (lc > 0) {
         const uint32_t h2 = (lc < 11u) ? lc : 11u;
         const uint32_t nchunk = oi_lvl_cnt(lc - h2, L, vl);
@@ -547,18 +537,20 @@ __global__ void __launch_bounds__(OI_BLOCK) k_reduce(ReduceArgs a)
 extern "C" {
 
 hipError_t oi_launch_mesh_box(hipStream_t st, const float* xyz, uint32_t V, float* partials, uint32_t nblocks,
-                              uint32_t* counter, float* box_out, uint32_t* clear_u32, uint32_t clear_n)
+                              uint32_t* clear_u32, uint32_t clear_n)
 {
     const uint32_t chunk = (V + nblocks - 1) / nblocks;
-    hipLaunchKernelGGL(k_mesh_box, dim3(nblocks), dim3(OI_BLOCK), 0, st, xyz, V, chunk, partials, counter, box_out,
-                       clear_u32, clear_n);
+    hipLaunchKernelGGL(k_mesh_box, dim3(nblocks), dim3(OI_BLOCK), 0, st, xyz, V, chunk, partials, clear_u32,
+                       clear_n);
     return hipGetLastError();
 }
 
-hipError_t oi_launch_morton(hipStream_t st, const uint32_t* faces, const float* xyz, uint32_t F, const float* box,
-                            uint32_t* keys, uint32_t* ghist, uint32_t nblocks)
+hipError_t oi_launch_morton(hipStream_t st, const uint32_t* faces, const float* xyz, uint32_t F,
+                            const float* partials, uint32_t nparts, float* box_out, uint32_t* keys, uint32_t* ghist,
+                            uint32_t nblocks)
 {
-    hipLaunchKernelGGL(k_morton, dim3(nblocks), dim3(OI_BLOCK), 0, st, faces, xyz, F, box, keys, ghist);
+    hipLaunchKernelGGL(k_morton, dim3(nblocks), dim3(OI_BLOCK), 0, st, faces, xyz, F, partials, nparts, box_out,
+                       keys, ghist);
     return hipGetLastError();
 }
 
@@ -576,10 +568,9 @@ hipError_t oi_launch_sort_pass(hipStream_t st, const uint32_t* keys_in, const ui
 
 // M = leaves per thread in stage 1 (1, 2, 4 or 8)
 hipError_t oi_launch_reduce(hipStream_t st, int M, const float* pos, const uint32_t* faces, const uint32_t* perm,
-                            uint32_t* sfaces_out, float* nodes, uint32_t* counter, uint32_t F, uint32_t L,
-                            uint32_t vl)
+                            uint32_t* sfaces_out, float* nodes, uint32_t F, uint32_t L, uint32_t vl)
 {
-    ReduceArgs a{pos, faces, perm, sfaces_out, nodes, counter, F, L, vl};
+    ReduceArgs a{pos, faces, perm, sfaces_out, nodes, F, L, vl};
     uint32_t lgm = (M == 1) ? 0 : (M == 2) ? 1 : (M == 4) ? 2 : 3;
     uint32_t h = (L < 8 + lgm) ? L : 8 + lgm;
     uint32_t grid = oi_lvl_cnt(L - h, L, vl);
@@ -589,6 +580,7 @@ hipError_t oi_launch_reduce(hipStream_t st, int M, const float* pos, const uint3
     case 4: hipLaunchKernelGGL(k_reduce<4>, dim3(grid), dim3(OI_BLOCK), 0, st, a); break;
     default: hipLaunchKernelGGL(k_reduce<8>, dim3(grid), dim3(OI_BLOCK), 0, st, a); break;
     }
+    if (L > h) hipLaunchKernelGGL(k_reduce_apex, dim3(1), dim3(OI_BLOCK), 0, st, a, L - h);
     return hipGetLastError();
 }
 

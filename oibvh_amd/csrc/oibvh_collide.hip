@@ -535,6 +535,31 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
     const u64 t0 = wall_clock64();
     const uint32_t cap = 1u << A.ring_shift;
 
+    const uint32_t fan = 1u << (2u * A.expand);  // worst-case children per entry
+    uint32_t backoff = 1, step = 0;
+
+    // Move the bottom D entries of the LDS stack to the ring (the oldest
+    // entries: the largest subtrees, the most useful work for another
+    // workgroup) and shift the rest down. Called by every thread with
+    // uniform D; s_push_base holds the reserved ring index.
+    auto move_bottom = [&](uint32_t D) {
+        const uint32_t m = s_n;
+        for (uint32_t i = tid; i < D; i += TB)
+            if (!slot_put(A.ring, A.ring_shift, s_push_base + i, s_stack[i], ctl, t0, A.timeout_ticks)) break;
+        __syncthreads();
+        // chunk r's destination lies below every later chunk's source
+        for (uint32_t r = 0; D + r * TB < m; r++) {
+            const uint32_t i = D + r * TB + tid;
+            uint2 x = make_uint2(0, 0);
+            if (i < m) x = s_stack[i];
+            __syncthreads();
+            if (i < m) s_stack[i - D] = x;
+            __syncthreads();
+        }
+        if (tid == 0) s_n = m - D;
+        __syncthreads();
+    };
+
     for (;;) {
         // ---- refill from the ring when the local stack is short
         if (tid == 0) {
@@ -542,7 +567,7 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
             s_flag = 0;
             if (s_n < TB) {
                 uint32_t want = TB - s_n;
-                for (int attempt = 0; attempt < 8; attempt++) {
+                for (int attempt = 0; attempt < 2; attempt++) {
                     const u64 h = ld64(&ctl->head);
                     const u64 t = ld64(&ctl->tail);
                     if (h >= t) break;
@@ -591,20 +616,38 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
             idle = true;
             __syncthreads();
             if (s_flag) break;
-            __builtin_amdgcn_s_sleep(8);
+            // exponential back-off keeps idle workgroups off the shared counters
+            for (uint32_t b = 0; b < backoff; b++) __builtin_amdgcn_s_sleep(32);
+            if (backoff < 32) backoff <<= 1;
             continue;
         }
+        backoff = 1;
         if (idle) {
             if (tid == 0) __hip_atomic_fetch_sub(&ctl->idle, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             idle = false;
         }
 
-        // ---- take one batch from the top of the stack; near a full stack take
-        // only as many entries as the stack can absorb the children of
-        const uint32_t fan = 1u << (2u * A.expand);  // worst case children per entry
-        uint32_t take = (n < TB) ? n : TB;
+        // ---- a stack too full to expand even one entry gives half to the ring
+        if (STACK_CAP - n < fan - 1u) {
+            if (tid == 0) {
+                u64 b = 0;
+                s_flag = ring_reserve(ctl, cap, n / 2, b) ? 0u : 1u;
+                if (s_flag)
+                    __hip_atomic_fetch_or(&ctl->error, OI_ERR_QUEUE_OVERFLOW, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+                s_push_base = b;
+            }
+            __syncthreads();
+            if (s_flag) break;
+            move_bottom(n / 2);
+            continue;
+        }
+
+        // ---- take as many entries from the top as the stack can absorb the
+        // children of (take * fan <= free slots + take), at most one per thread
         const uint32_t fit = (STACK_CAP - n) / (fan - 1u);
-        if (take > fit) take = (fit > 64u) ? fit : (take < 64u ? take : 64u);
+        uint32_t take = (n < TB) ? n : TB;
+        if (take > fit) take = fit;
         bool have = tid < take;
         uint2 e = have ? s_stack[n - take + tid] : make_uint2(0, 0);
         __syncthreads();
@@ -650,27 +693,11 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
         }
         n_tests += have ? 1u : 0u;
 
-        // ---- place children: block prefix sum; spill what does not fit in LDS
+        // ---- children onto the stack (block prefix sum; they always fit)
         uint32_t T;
         const uint32_t off = block_scan(nchild, s_wsum, T);
-        const uint32_t room = STACK_CAP - nb;
-        const uint32_t spill = (T > room) ? (T - room) : 0u;
-        if (tid == 0) {
-            s_flag = 0;
-            // count the children before any of them becomes visible to others
-            if (T) __hip_atomic_fetch_add(&ctl->pending, (int)T, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (spill) {
-                u64 b = 0;
-                if (!ring_reserve(ctl, cap, spill, b)) {
-                    __hip_atomic_fetch_or(&ctl->error, OI_ERR_QUEUE_OVERFLOW, __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
-                    s_flag = 1;
-                }
-                s_push_base = b;
-            }
-        }
-        __syncthreads();
-        if (s_flag) break;
+        if (tid == 0 && T)
+            __hip_atomic_fetch_add(&ctl->pending, (int)T, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         for (uint32_t c = 0; c < nchild; c++) {
             uint32_t x, y;
             if (selfnode) {
@@ -681,13 +708,7 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
                 x = sa.start + c / sb.count;
                 y = sb.start + c % sb.count;
             }
-            const uint2 ce = make_uint2(baseA + x, baseB + y);
-            const uint32_t k = off + c;
-            if (k < spill) {
-                slot_put(A.ring, A.ring_shift, s_push_base + k, ce, ctl, t0, A.timeout_ticks);
-            } else {
-                s_stack[nb + k - spill] = ce;
-            }
+            s_stack[nb + off + c] = make_uint2(baseA + x, baseB + y);
         }
         // ---- candidates into the LDS buffer
         uint32_t CT;
@@ -696,7 +717,7 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
         n_cands += cand ? 1u : 0u;
         __syncthreads();
         if (tid == 0) {
-            s_n = nb + T - spill;
+            s_n = nb + T;
             s_cn += CT;
             if (take) __hip_atomic_fetch_sub(&ctl->pending, (int)take, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
@@ -727,37 +748,23 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
             __syncthreads();
         }
 
-        // ---- donate the bottom half of a deep stack when others are idle
-        if (tid == 0) {
-            s_flag = 0;
-            if (s_n >= 2 * TB && ld32(&ctl->idle) > 0u) {
-                const uint32_t D = s_n / 2;
-                u64 b = 0;
-                if (ring_reserve(ctl, cap, D, b)) {
-                    s_push_base = b;
-                    s_flag = D;
+        // ---- every 4th step: donate the bottom half of a deep stack when
+        // other workgroups are idle
+        step++;
+        if ((step & 3u) == 0u) {
+            if (tid == 0) {
+                s_flag = 0;
+                if (s_n >= 2 * TB && ld32(&ctl->idle) > 0u) {
+                    const uint32_t D = s_n / 2;
+                    u64 b = 0;
+                    if (ring_reserve(ctl, cap, D, b)) {
+                        s_push_base = b;
+                        s_flag = D;
+                    }
                 }
             }
-        }
-        __syncthreads();
-        if (s_flag) {
-            const uint32_t D = s_flag, m = s_n;
-            for (uint32_t i = tid; i < D; i += TB) {
-                slot_put(A.ring, A.ring_shift, s_push_base + i, s_stack[i], ctl, t0, A.timeout_ticks);
-            }
-            // shift the rest down chunk by chunk: chunk r's destination lies
-            // below every later chunk's source
             __syncthreads();
-            for (uint32_t r = 0; D + r * TB < m; r++) {
-                const uint32_t i = D + r * TB + tid;
-                uint2 x = make_uint2(0, 0);
-                if (i < m) x = s_stack[i];
-                __syncthreads();
-                if (i < m) s_stack[i - D] = x;
-                __syncthreads();
-            }
-            if (tid == 0) s_n = m - D;
-            __syncthreads();
+            if (s_flag) move_bottom(s_flag);
         }
     }
 
@@ -880,12 +887,11 @@ hipError_t oi_launch_seed(hipStream_t st, unsigned long long* ring, uint32_t cap
     return hipGetLastError();
 }
 
+// resident workgroups per CU of the plain [0] and FP64-filter [1] traversal
 hipError_t oi_traverse_occupancy(int* blocks_per_cu)
 {
-    int a = 0, b = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_traverse<false>, TB, 0);
-    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_traverse<true>, TB, 0);
-    *blocks_per_cu = a < b ? a : b;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu[0], k_traverse<false>, TB, 0);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu[1], k_traverse<true>, TB, 0);
     return e;
 }
 

@@ -18,15 +18,14 @@
 #include "oibvh_common.h"
 
 extern "C" {
-hipError_t oi_launch_mesh_box(hipStream_t, const float*, uint32_t, float*, uint32_t, uint32_t*, float*, uint32_t*,
-                              uint32_t);
-hipError_t oi_launch_morton(hipStream_t, const uint32_t*, const float*, uint32_t, const float*, uint32_t*,
-                            uint32_t*, uint32_t);
+hipError_t oi_launch_mesh_box(hipStream_t, const float*, uint32_t, float*, uint32_t, uint32_t*, uint32_t);
+hipError_t oi_launch_morton(hipStream_t, const uint32_t*, const float*, uint32_t, const float*, uint32_t, float*,
+                            uint32_t*, uint32_t*, uint32_t);
 uint32_t oi_sort_tile();
 hipError_t oi_launch_sort_pass(hipStream_t, const uint32_t*, const uint32_t*, uint32_t*, uint32_t*, uint32_t,
                                uint32_t, const uint32_t*, uint32_t*, uint32_t*, uint32_t*);
 hipError_t oi_launch_reduce(hipStream_t, int, const float*, const uint32_t*, const uint32_t*, uint32_t*, float*,
-                            uint32_t*, uint32_t, uint32_t, uint32_t);
+                            uint32_t, uint32_t, uint32_t);
 hipError_t oi_launch_fill64(hipStream_t, unsigned long long*, uint32_t, unsigned long long);
 hipError_t oi_launch_ring_reset(hipStream_t, unsigned long long*, uint32_t, oi_trav_ctl*);
 hipError_t oi_launch_seed(hipStream_t, unsigned long long*, uint32_t, const uint2*, uint32_t, oi_trav_ctl*,
@@ -44,10 +43,11 @@ struct oibvh_ctx {
     bool own_stream = false;
     std::string err;
     int cus = 0;
-    int trav_occ = 1;
+    int trav_occ[2] = {1, 1};  // traversal workgroups per CU: plain, FP64 filter
     uint64_t clock_khz = 100000;
     uint32_t leaves_per_thread = 0;
     uint32_t trav_blocks = 0;
+    uint32_t trav_expand = 0;
     uint32_t queue_cap = 1u << 22;
     uint32_t hits_cap = 1u << 20;
     int live = 0;                // trees and scenes of this context not yet destroyed
@@ -86,8 +86,7 @@ struct oibvh_tree {
     uint32_t* vals_tmp = nullptr;
     uint32_t* sort_scratch = nullptr;
     uint32_t tiles = 0;
-    float* box = nullptr;        // 6 floats + partials
-    uint32_t* counters = nullptr; // [0] mesh box, [1] reduce
+    float* box = nullptr;        // 6 floats + per-block partial boxes
     bool built = false;
 };
 
@@ -186,8 +185,11 @@ extern "C" oibvh_status oibvh_ctx_create(int device, void* stream, oibvh_ctx** o
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
         c->clock_khz = (uint64_t)khz;
-    int occ = 0;
-    if (oi_traverse_occupancy(&occ) == hipSuccess && occ > 0) c->trav_occ = occ;
+    int occ[2] = {0, 0};
+    if (oi_traverse_occupancy(occ) == hipSuccess) {
+        if (occ[0] > 0) c->trav_occ[0] = occ[0];
+        if (occ[1] > 0) c->trav_occ[1] = occ[1];
+    }
     if (stream) {
         c->stream = (hipStream_t)stream;
     } else {
@@ -222,18 +224,31 @@ extern "C" oibvh_status oibvh_ctx_synchronize(oibvh_ctx* ctx)
 
 extern "C" void* oibvh_ctx_stream(oibvh_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
-extern "C" oibvh_status oibvh_ctx_set_tuning(oibvh_ctx* ctx, uint32_t lpt, uint32_t blocks, uint32_t qcap,
-                                            uint32_t hcap)
+extern "C" oibvh_status oibvh_ctx_set_tuning(oibvh_ctx* ctx, const oibvh_tuning* t)
 {
-    if (!ctx) return OIBVH_ERR_INVALID;
+    if (!ctx || !t) return OIBVH_ERR_INVALID;
+    const uint32_t lpt = t->reduce_leaves_per_thread;
     if (lpt != 0 && lpt != 1 && lpt != 2 && lpt != 4 && lpt != 8)
         return set_err(ctx, OIBVH_ERR_INVALID, "reduce_leaves_per_thread must be 0, 1, 2, 4 or 8");
-    if (qcap && (qcap & (qcap - 1)))
+    if (t->queue_capacity && (t->queue_capacity & (t->queue_capacity - 1)))
         return set_err(ctx, OIBVH_ERR_INVALID, "queue_capacity must be a power of two");
+    if (t->traverse_expand > 3) return set_err(ctx, OIBVH_ERR_INVALID, "traverse_expand must be 0..3");
     ctx->leaves_per_thread = lpt;
-    ctx->trav_blocks = blocks;
-    if (qcap) ctx->queue_cap = qcap;
-    if (hcap) ctx->hits_cap = hcap;
+    ctx->trav_blocks = t->traverse_blocks;
+    ctx->trav_expand = t->traverse_expand;
+    ctx->queue_cap = t->queue_capacity ? t->queue_capacity : (1u << 22);
+    ctx->hits_cap = t->hits_capacity ? t->hits_capacity : (1u << 20);
+    return OIBVH_OK;
+}
+
+extern "C" oibvh_status oibvh_ctx_get_tuning(const oibvh_ctx* ctx, oibvh_tuning* t)
+{
+    if (!ctx || !t) return OIBVH_ERR_INVALID;
+    t->reduce_leaves_per_thread = ctx->leaves_per_thread;
+    t->traverse_blocks = ctx->trav_blocks;
+    t->traverse_expand = ctx->trav_expand;
+    t->queue_capacity = ctx->queue_cap;
+    t->hits_capacity = ctx->hits_cap;
     return OIBVH_OK;
 }
 
@@ -250,7 +265,6 @@ static void tree_free(oibvh_tree* t)
     dfree(t->vals_tmp);
     dfree(t->sort_scratch);
     dfree(t->box);
-    dfree(t->counters);
 }
 
 static oibvh_status tree_alloc_common(oibvh_tree* t)
@@ -264,8 +278,6 @@ static oibvh_status tree_alloc_common(oibvh_tree* t)
     HIPCHK(c, dalloc(&t->sfaces, 3ull * t->F));
     HIPCHK(c, dalloc(&t->perm, t->F));
     HIPCHK(c, dalloc(&t->nodes, 6ull * t->N));
-    HIPCHK(c, dalloc(&t->counters, 4));
-    HIPCHK(c, hipMemsetAsync(t->counters, 0, 4 * sizeof(uint32_t), c->stream));
     t->pos = t->pos_own;
     return OIBVH_OK;
 }
@@ -390,8 +402,8 @@ static oibvh_status launch_reduce(oibvh_tree* t, bool build)
 {
     oibvh_ctx* c = t->ctx;
     HIPCHK(c, oi_launch_reduce(c->stream, pick_lpt(t), t->pos, build ? t->faces : t->sfaces,
-                               build ? t->perm : nullptr, build ? t->sfaces : nullptr, t->nodes, t->counters + 1,
-                               t->F, t->L, t->vl));
+                               build ? t->perm : nullptr, build ? t->sfaces : nullptr, t->nodes, t->F, t->L,
+                               t->vl));
     return OIBVH_OK;
 }
 
@@ -418,10 +430,9 @@ extern "C" oibvh_status oibvh_tree_build(oibvh_tree* t)
     uint32_t* lb0 = t->sort_scratch + 1032;
     uint32_t* lb1 = lb0 + 256ull * tiles;
     hipStream_t s = c->stream;
-    HIPCHK(c, oi_launch_mesh_box(s, t->pos, t->V, t->box + 6, box_blocks, t->counters, t->box, t->sort_scratch,
-                                 1032u + 256u * tiles));
+    HIPCHK(c, oi_launch_mesh_box(s, t->pos, t->V, t->box + 6, box_blocks, t->sort_scratch, 1032u + 256u * tiles));
     const uint32_t mblocks = std::max(1u, std::min((F + 255u) / 256u, (uint32_t)std::max(1, c->cus) * 8u));
-    HIPCHK(c, oi_launch_morton(s, t->faces, t->pos, F, t->box, t->keys, ghist, mblocks));
+    HIPCHK(c, oi_launch_morton(s, t->faces, t->pos, F, t->box + 6, box_blocks, t->box, t->keys, ghist, mblocks));
     HIPCHK(c, oi_launch_sort_pass(s, t->keys, nullptr, t->keys_alt, t->vals_tmp, F, 0, ghist, lb0, lb1, tctr + 0));
     HIPCHK(c, oi_launch_sort_pass(s, t->keys_alt, t->vals_tmp, t->keys, t->perm, F, 8, ghist + 256, lb1, lb0,
                                   tctr + 1));
@@ -712,11 +723,14 @@ static oibvh_status enqueue_detect(oibvh_scene* s, bool reset_sticky)
         if (r != OIBVH_OK) return r;
     }
     HIPCHK(c, oi_launch_seed(st, s->ring, s->ring_cap, s->d_front, s->n_front, s->ctl, reset_sticky ? 1u : 0u));
-    uint32_t grid = c->trav_blocks ? c->trav_blocks : (uint32_t)std::max(1, c->cus) * (uint32_t)c->trav_occ;
+    const int variant = (s->last_flags & OIBVH_DETECT_FP64_FALLBACK) ? 1 : 0;
+    uint32_t grid = c->trav_blocks ? c->trav_blocks
+                                   : (uint32_t)std::max(1, c->cus) * (uint32_t)c->trav_occ[variant];
     const unsigned long long timeout = 30ull * 1000ull * c->clock_khz;  // 30 s
     HIPCHK(c, oi_launch_traverse(st, grid, n > 4 ? s->d_desc : nullptr, desc.data(), n, s->ring, s->ring_cap,
                                  s->ctl, s->hits, s->hits_cap,
-                                 std::min(3u, std::max(1u, s->last_expand)), s->last_flags & 3u, timeout,
+                                 c->trav_expand ? c->trav_expand : std::min(3u, std::max(1u, s->last_expand)),
+                                 s->last_flags & 3u, timeout,
                                  s->implicit_end));
     s->unchecked = true;
     s->have_result = true;

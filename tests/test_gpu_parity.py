@@ -70,12 +70,12 @@ def test_build_soup_bit_exact(api, ctx, orc, F):
 @pytest.mark.parametrize("lpt", [1, 2, 4, 8])
 def test_build_subtree_sizes(api, ctx, orc, lpt):
     say(f"build leaves_per_thread={lpt}")
-    ctx.set_tuning(leaves_per_thread=lpt)
+    ctx.set_tuning(reduce_leaves_per_thread=lpt)
     try:
         for xyz, f in [M.icosphere(5, 1.0, 0.02, 3), M.uv_sphere(300, 151), soup(12345, 7)]:
             check_tree(orc, gpu_tree(api, ctx, xyz, f), xyz, f)
     finally:
-        ctx.set_tuning(leaves_per_thread=0)
+        ctx.set_tuning(reduce_leaves_per_thread=0)
 
 
 def test_build_flat_axis_morton_nan_policy(api, ctx, orc):
