@@ -10,7 +10,9 @@
 //     stands for itself, collide.cu:165-176) and the children are pushed back
 //     with a block prefix sum;
 //   * a global ring queue takes spills and donations to idle workgroups
-//     (work stealing); a `pending` counter of live entries decides termination;
+//     (work stealing); a `pending` counter (ring entries + workgroups holding
+//     entries, updated only when work moves through the ring or a workgroup
+//     changes between busy and idle) lets idle workgroups stop;
 //   * candidates are compacted into an LDS buffer and tested 256 at a time
 //     with the 17-axis SAT (utils.cpp:71-169 / cuda_intersect_tritri.h:350-434),
 //     hits appended with one atomic per batch as (meshA, meshB, origTriA,
@@ -531,7 +533,7 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
 
     u64 n_tests = 0, n_cands = 0;
     uint32_t n_fp64 = 0, n_adj = 0;
-    bool idle = false;
+    bool idle = true;  // every workgroup starts without work (the seed counts them all idle)
     const u64 t0 = wall_clock64();
     const uint32_t cap = 1u << A.ring_shift;
 
@@ -561,25 +563,32 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
     };
 
     for (;;) {
-        // ---- refill from the ring when the local stack is short
+        // ---- refill from the ring when the local stack is short. `pending`
+        // counts ring entries plus busy workgroups: a workgroup that turns
+        // busy adds itself before it takes entries out of the ring, so the
+        // count never drops to zero while work exists (a stale low read only
+        // lets an idle workgroup leave early, which loses no work: no
+        // workgroup leaves while it holds entries or sees a non-empty ring).
         if (tid == 0) {
             s_pop_n = 0;
             s_flag = 0;
             if (s_n < TB) {
                 uint32_t want = TB - s_n;
-                for (int attempt = 0; attempt < 2; attempt++) {
-                    const u64 h = ld64(&ctl->head);
-                    const u64 t = ld64(&ctl->tail);
-                    if (h >= t) break;
+                const u64 h = ld64(&ctl->head);
+                const u64 t = ld64(&ctl->tail);
+                if (h < t) {
                     const u64 avail = t - h;
                     uint32_t take = (avail < want) ? (uint32_t)avail : want;
                     if (take > 64 && s_n == 0 && avail < 4096) take = 64;
+                    if (idle) __hip_atomic_fetch_add(&ctl->pending, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     u64 expected = h;
                     if (__hip_atomic_compare_exchange_strong(&ctl->head, &expected, h + take, __ATOMIC_RELAXED,
                                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                         s_pop_h = h;
                         s_pop_n = take;
-                        break;
+                        __hip_atomic_fetch_sub(&ctl->pending, (int)take, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    } else if (idle) {
+                        __hip_atomic_fetch_sub(&ctl->pending, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     }
                 }
             }
@@ -602,7 +611,11 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
         if (n == 0) {
             // ---- idle: finished when nothing is pending anywhere
             if (tid == 0) {
-                if (!idle) __hip_atomic_fetch_add(&ctl->idle, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (!idle) {
+                    // busy -> idle
+                    __hip_atomic_fetch_sub(&ctl->pending, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_fetch_add(&ctl->idle, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
                 const int pend = ld32i(&ctl->pending);
                 const uint32_t err = ld32(&ctl->error);
                 uint32_t f = 0;
@@ -623,6 +636,7 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
         }
         backoff = 1;
         if (idle) {
+            // idle -> busy (its +1 on `pending` preceded the pop)
             if (tid == 0) __hip_atomic_fetch_sub(&ctl->idle, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             idle = false;
         }
@@ -631,6 +645,7 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
         if (STACK_CAP - n < fan - 1u) {
             if (tid == 0) {
                 u64 b = 0;
+                __hip_atomic_fetch_add(&ctl->pending, (int)(n / 2), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 s_flag = ring_reserve(ctl, cap, n / 2, b) ? 0u : 1u;
                 if (s_flag)
                     __hip_atomic_fetch_or(&ctl->error, OI_ERR_QUEUE_OVERFLOW, __ATOMIC_RELAXED,
@@ -696,8 +711,6 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
         // ---- children onto the stack (block prefix sum; they always fit)
         uint32_t T;
         const uint32_t off = block_scan(nchild, s_wsum, T);
-        if (tid == 0 && T)
-            __hip_atomic_fetch_add(&ctl->pending, (int)T, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         for (uint32_t c = 0; c < nchild; c++) {
             uint32_t x, y;
             if (selfnode) {
@@ -719,7 +732,6 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
         if (tid == 0) {
             s_n = nb + T;
             s_cn += CT;
-            if (take) __hip_atomic_fetch_sub(&ctl->pending, (int)take, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         __syncthreads();
 
@@ -757,9 +769,12 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
                 if (s_n >= 2 * TB && ld32(&ctl->idle) > 0u) {
                     const uint32_t D = s_n / 2;
                     u64 b = 0;
+                    __hip_atomic_fetch_add(&ctl->pending, (int)D, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     if (ring_reserve(ctl, cap, D, b)) {
                         s_push_base = b;
                         s_flag = D;
+                    } else {
+                        __hip_atomic_fetch_sub(&ctl->pending, (int)D, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     }
                 }
             }
@@ -817,7 +832,7 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
 // counters. Only block 0 / thread 0 writes the counters; every thread reads
 // `head`, which no one writes during this kernel.
 __global__ void k_seed(u64* ring, uint32_t shift, const uint2* front, uint32_t n0, oi_trav_ctl* ctl,
-                       uint32_t reset_sticky)
+                       uint32_t reset_sticky, uint32_t n_wg)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const u64 base = ctl->head;
@@ -833,7 +848,7 @@ __global__ void k_seed(u64* ring, uint32_t shift, const uint2* front, uint32_t n
         ctl->pending = (int)n0;
         ctl->n_hits = 0;
         ctl->error = 0;
-        ctl->idle = 0;
+        ctl->idle = n_wg;
         ctl->tests = 0;
         ctl->cands = 0;
         ctl->fp64 = 0;
@@ -879,11 +894,11 @@ hipError_t oi_launch_ring_reset(hipStream_t st, unsigned long long* ring, uint32
 }
 
 hipError_t oi_launch_seed(hipStream_t st, unsigned long long* ring, uint32_t cap, const uint2* front, uint32_t n0,
-                          oi_trav_ctl* ctl, uint32_t reset_sticky)
+                          oi_trav_ctl* ctl, uint32_t reset_sticky, uint32_t n_wg)
 {
     uint32_t g = (n0 + 255) / 256;
     if (g == 0) g = 1;
-    hipLaunchKernelGGL(k_seed, dim3(g), dim3(256), 0, st, ring, oi_ilog2(cap), front, n0, ctl, reset_sticky);
+    hipLaunchKernelGGL(k_seed, dim3(g), dim3(256), 0, st, ring, oi_ilog2(cap), front, n0, ctl, reset_sticky, n_wg);
     return hipGetLastError();
 }
 

@@ -80,20 +80,27 @@ struct oi_mesh_desc {
     uint32_t pad0, pad1, pad2;
 };
 
-// Traversal counters, zeroed by the seed kernel before each detect.
-struct oi_trav_ctl {
+// Traversal counters, reset by the seed kernel before each detect. The
+// counters every workgroup touches live on separate 128-byte lines.
+struct __attribute__((aligned(128))) oi_trav_ctl {
     unsigned long long head;  // ring entries popped (reserved)
+    char pad0[120];
     unsigned long long tail;  // ring entries pushed (reserved)
-    int pending;              // BVTT entries alive anywhere
-    uint32_t n_hits;          // hits found (may exceed capacity)
-    uint32_t error;           // OI_ERR_* bits
+    char pad1[120];
+    int pending;              // ring entries + workgroups holding BVTT entries
+    char pad2[124];
     uint32_t idle;            // workgroups currently without work
+    char pad3[124];
+    uint32_t n_hits;          // hits found (may exceed capacity)
+    char pad4[124];
+    uint32_t error;           // OI_ERR_* bits
+    uint32_t sticky;          // OI_ERR_* bits of earlier asynchronous traversals not checked since
+    uint32_t exited;          // workgroups that left the loop
+    uint32_t pad5;
     unsigned long long tests; // BVTT entries tested
     unsigned long long cands; // leaf pairs sent to the narrow phase
     unsigned long long fp64;  // SAT axis comparisons re-evaluated in double
     unsigned long long adj;   // leaf pairs dropped as adjacent (self mode)
-    uint32_t exited;          // workgroups that left the loop
-    uint32_t sticky;          // OI_ERR_* bits of earlier asynchronous traversals not checked since
 };
 
 #define OI_ERR_QUEUE_OVERFLOW 1u

@@ -29,7 +29,7 @@ hipError_t oi_launch_reduce(hipStream_t, int, const float*, const uint32_t*, con
 hipError_t oi_launch_fill64(hipStream_t, unsigned long long*, uint32_t, unsigned long long);
 hipError_t oi_launch_ring_reset(hipStream_t, unsigned long long*, uint32_t, oi_trav_ctl*);
 hipError_t oi_launch_seed(hipStream_t, unsigned long long*, uint32_t, const uint2*, uint32_t, oi_trav_ctl*,
-                          uint32_t);
+                          uint32_t, uint32_t);
 hipError_t oi_traverse_occupancy(int*);
 hipError_t oi_launch_traverse(hipStream_t, uint32_t, const oi_mesh_desc*, const oi_mesh_desc*, uint32_t,
                               unsigned long long*, uint32_t,
@@ -722,10 +722,10 @@ static oibvh_status enqueue_detect(oibvh_scene* s, bool reset_sticky)
         r = ensure_hits(s, c->hits_cap);
         if (r != OIBVH_OK) return r;
     }
-    HIPCHK(c, oi_launch_seed(st, s->ring, s->ring_cap, s->d_front, s->n_front, s->ctl, reset_sticky ? 1u : 0u));
     const int variant = (s->last_flags & OIBVH_DETECT_FP64_FALLBACK) ? 1 : 0;
     uint32_t grid = c->trav_blocks ? c->trav_blocks
                                    : (uint32_t)std::max(1, c->cus) * (uint32_t)c->trav_occ[variant];
+    HIPCHK(c, oi_launch_seed(st, s->ring, s->ring_cap, s->d_front, s->n_front, s->ctl, reset_sticky ? 1u : 0u, grid));
     const unsigned long long timeout = 30ull * 1000ull * c->clock_khz;  // 30 s
     HIPCHK(c, oi_launch_traverse(st, grid, n > 4 ? s->d_desc : nullptr, desc.data(), n, s->ring, s->ring_cap,
                                  s->ctl, s->hits, s->hits_cap,
