@@ -190,11 +190,20 @@ typedef struct {
                                           detect call's expandLevels, clamped to 1..3) */
     uint32_t queue_capacity;           /* initial work-ring entries, a power of two (0 = 2^22) */
     uint32_t hits_capacity;            /* initial hit-list entries (0 = 2^20) */
+    uint32_t trace;                    /* 1 = record a per-workgroup traversal trace */
 } oibvh_tuning;
 
 /* Both buffers grow on overflow (the detect is re-run). */
 OIBVH_API oibvh_status oibvh_ctx_set_tuning(oibvh_ctx* ctx, const oibvh_tuning* tuning);
 OIBVH_API oibvh_status oibvh_ctx_get_tuning(const oibvh_ctx* ctx, oibvh_tuning* tuning);
+
+/* Per-workgroup trace of the last detect (tuning.trace = 1): 16 uint64 per
+ * workgroup -- steps, entries tested, ring pops, entries popped, moves to the
+ * ring, entries moved, idle iterations, narrow batches, then wall-clock ticks
+ * spent stepping, idle, moving, in the narrow phase and refilling, ticks to
+ * first work, ticks to exit, deepest stack. *n_wg receives the workgroup count
+ * (0 when tracing was off). Diagnostics only; no reference counterpart. */
+OIBVH_API oibvh_status oibvh_scene_get_trace(oibvh_scene* scene, uint64_t* out, uint64_t cap, uint32_t* n_wg);
 
 #ifdef __cplusplus
 }

@@ -362,6 +362,7 @@ struct TravArgs {
     uint32_t flags;
     u64 timeout_ticks;
     uint32_t implicit_end;  // end of the scene's implicit index space
+    unsigned long long* trace;  // OI_TRACE_WORDS per workgroup, or nullptr
 };
 
 struct MeshRef {
@@ -535,6 +536,20 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
     uint32_t n_fp64 = 0, n_adj = 0;
     bool idle = true;  // every workgroup starts without work (the seed counts them all idle)
     const u64 t0 = wall_clock64();
+    // trace counters (thread 0; only written out when A.trace is set)
+    u64 tr[OI_TRACE_WORDS];
+#pragma unroll
+    for (int k = 0; k < OI_TRACE_WORDS; k++) tr[k] = 0;
+    const bool tracing = A.trace != nullptr;
+    u64 tmark = t0;
+#define TRACE_T(slot)                                       \
+    if (tracing && tid == 0) {                              \
+        const u64 now_ = wall_clock64();                    \
+        tr[slot] += now_ - tmark;                           \
+        tmark = now_;                                       \
+    }
+#define TRACE_N(slot, v) \
+    if (tracing && tid == 0) tr[slot] += (v);
     const uint32_t cap = 1u << A.ring_shift;
 
     const uint32_t fan = 1u << (2u * A.expand);  // worst-case children per entry
@@ -596,6 +611,8 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
         __syncthreads();
         const uint32_t pn = s_pop_n;
         if (pn) {
+            TRACE_N(OI_TR_POPS, 1);
+            TRACE_N(OI_TR_POPPED, pn);
             const u64 h = s_pop_h;
             for (uint32_t i = tid; i < pn; i += TB) {
                 uint2 v;
@@ -607,6 +624,7 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
             if (tid == 0) s_n += pn;
             __syncthreads();
         }
+        TRACE_T(idle ? OI_TR_T_IDLE : OI_TR_T_REFILL);
         const uint32_t n = s_n;
         if (n == 0) {
             // ---- idle: finished when nothing is pending anywhere
@@ -628,6 +646,7 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
             }
             idle = true;
             __syncthreads();
+            TRACE_N(OI_TR_IDLE_LOOPS, 1);
             if (s_flag) break;
             // exponential back-off keeps idle workgroups off the shared counters
             for (uint32_t b = 0; b < backoff; b++) __builtin_amdgcn_s_sleep(32);
@@ -638,8 +657,10 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
         if (idle) {
             // idle -> busy (its +1 on `pending` preceded the pop)
             if (tid == 0) __hip_atomic_fetch_sub(&ctl->idle, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (tracing && tid == 0 && tr[OI_TR_T_FIRST] == 0) tr[OI_TR_T_FIRST] = wall_clock64() - t0;
             idle = false;
         }
+        if (tracing && tid == 0 && n > tr[OI_TR_MAX_STACK]) tr[OI_TR_MAX_STACK] = n;
 
         // ---- a stack too full to expand even one entry gives half to the ring
         if (STACK_CAP - n < fan - 1u) {
@@ -655,6 +676,9 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
             __syncthreads();
             if (s_flag) break;
             move_bottom(n / 2);
+            TRACE_N(OI_TR_MOVES, 1);
+            TRACE_N(OI_TR_MOVED, n / 2);
+            TRACE_T(OI_TR_T_MOVE);
             continue;
         }
 
@@ -734,6 +758,9 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
             s_cn += CT;
         }
         __syncthreads();
+        TRACE_N(OI_TR_STEPS, 1);
+        TRACE_N(OI_TR_TESTS, take);
+        TRACE_T(OI_TR_T_STEP);
 
         // ---- narrow phase on a full batch of candidates
         if (s_cn >= TB) {
@@ -758,6 +785,8 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
                     __hip_atomic_fetch_or(&ctl->error, OI_ERR_HITS_OVERFLOW, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             __syncthreads();
+            TRACE_N(OI_TR_NARROW, 1);
+            TRACE_T(OI_TR_T_NARROW);
         }
 
         // ---- every 4th step: donate the bottom half of a deep stack when
@@ -779,7 +808,12 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
                 }
             }
             __syncthreads();
-            if (s_flag) move_bottom(s_flag);
+            if (s_flag) {
+                TRACE_N(OI_TR_MOVES, 1);
+                TRACE_N(OI_TR_MOVED, s_flag);
+                move_bottom(s_flag);
+                TRACE_T(OI_TR_T_MOVE);
+            }
         }
     }
 
@@ -805,6 +839,12 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
         }
     }
 
+    if (tracing && tid == 0) {
+        tr[OI_TR_T_EXIT] = wall_clock64() - t0;
+        for (int k = 0; k < OI_TRACE_WORDS; k++) A.trace[(u64)blockIdx.x * OI_TRACE_WORDS + k] = tr[k];
+    }
+#undef TRACE_T
+#undef TRACE_N
     // ---- statistics (one atomic per workgroup per counter)
     uint32_t t_;
     const uint32_t f64tot = block_scan(n_fp64, s_wsum, t_);
@@ -914,7 +954,8 @@ hipError_t oi_launch_traverse(hipStream_t st, uint32_t grid, const oi_mesh_desc*
                               uint32_t n_meshes,
                               unsigned long long* ring, uint32_t ring_cap, oi_trav_ctl* ctl,
                               uint4* hits, uint32_t hits_cap, uint32_t expand, uint32_t flags,
-                              unsigned long long timeout_ticks, uint32_t implicit_end)
+                              unsigned long long timeout_ticks, uint32_t implicit_end,
+                              unsigned long long* trace)
 {
     TravArgs a{};
     a.meshes = meshes;
@@ -929,6 +970,7 @@ hipError_t oi_launch_traverse(hipStream_t st, uint32_t grid, const oi_mesh_desc*
     a.flags = flags;
     a.timeout_ticks = timeout_ticks;
     a.implicit_end = implicit_end;
+    a.trace = trace;
     if (flags & OI_DETECT_FP64_FALLBACK)
         hipLaunchKernelGGL(k_traverse<true>, dim3(grid), dim3(TB), 0, st, a);
     else

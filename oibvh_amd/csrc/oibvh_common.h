@@ -103,6 +103,27 @@ struct __attribute__((aligned(128))) oi_trav_ctl {
     unsigned long long adj;   // leaf pairs dropped as adjacent (self mode)
 };
 
+// per-workgroup traversal trace (optional): OI_TRACE_WORDS uint64 per workgroup
+#define OI_TRACE_WORDS 16
+enum {
+    OI_TR_STEPS,       // BVTT steps taken
+    OI_TR_TESTS,       // entries tested
+    OI_TR_POPS,        // successful pops from the ring
+    OI_TR_POPPED,      // entries popped
+    OI_TR_MOVES,       // stack-bottom moves to the ring (donations and full stacks)
+    OI_TR_MOVED,       // entries moved to the ring
+    OI_TR_IDLE_LOOPS,  // iterations without work
+    OI_TR_NARROW,      // narrow-phase batches
+    OI_TR_T_STEP,      // wall-clock ticks in BVTT steps (test + expand)
+    OI_TR_T_IDLE,      // ticks idle
+    OI_TR_T_MOVE,      // ticks moving entries to the ring
+    OI_TR_T_NARROW,    // ticks in narrow-phase batches
+    OI_TR_T_REFILL,    // ticks popping from the ring
+    OI_TR_T_FIRST,     // ticks from kernel start to first work
+    OI_TR_T_EXIT,      // ticks from kernel start to exit
+    OI_TR_MAX_STACK    // deepest LDS stack
+};
+
 #define OI_ERR_QUEUE_OVERFLOW 1u
 #define OI_ERR_TIMEOUT 2u
 #define OI_ERR_CORRUPT 4u

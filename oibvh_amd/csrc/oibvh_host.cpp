@@ -33,7 +33,8 @@ hipError_t oi_launch_seed(hipStream_t, unsigned long long*, uint32_t, const uint
 hipError_t oi_traverse_occupancy(int*);
 hipError_t oi_launch_traverse(hipStream_t, uint32_t, const oi_mesh_desc*, const oi_mesh_desc*, uint32_t,
                               unsigned long long*, uint32_t,
-                              oi_trav_ctl*, uint4*, uint32_t, uint32_t, uint32_t, unsigned long long, uint32_t);
+                              oi_trav_ctl*, uint4*, uint32_t, uint32_t, uint32_t, unsigned long long, uint32_t,
+                              unsigned long long*);
 }
 
 // ---------------------------------------------------------------------------
@@ -48,6 +49,7 @@ struct oibvh_ctx {
     uint32_t leaves_per_thread = 0;
     uint32_t trav_blocks = 0;
     uint32_t trav_expand = 0;
+    uint32_t trace = 0;
     uint32_t queue_cap = 1u << 22;
     uint32_t hits_cap = 1u << 20;
     int live = 0;                // trees and scenes of this context not yet destroyed
@@ -101,6 +103,7 @@ struct oibvh_scene {
     std::vector<uint2> h_front;
     std::vector<oi_mesh_desc> h_desc;  // host copies stay alive until the next detect
     uint32_t implicit_end = 0;
+    uint32_t last_grid = 0;
     uint32_t front_key[3] = {~0u, ~0u, ~0u}; // entry, flags&SELF, #trees
     unsigned long long* ring = nullptr;
     uint32_t ring_cap = 0;
@@ -109,6 +112,8 @@ struct oibvh_scene {
     oi_trav_ctl* h_ctl = nullptr; // pinned
     uint4* hits = nullptr;
     uint32_t hits_cap = 0;
+    unsigned long long* trace = nullptr;  // OI_TRACE_WORDS per traversal workgroup
+    uint32_t trace_wg = 0;
     // last detect
     uint32_t last_entry = 0, last_expand = 1, last_flags = 0;
     bool unchecked = false;
@@ -238,6 +243,7 @@ extern "C" oibvh_status oibvh_ctx_set_tuning(oibvh_ctx* ctx, const oibvh_tuning*
     ctx->trav_expand = t->traverse_expand;
     ctx->queue_cap = t->queue_capacity ? t->queue_capacity : (1u << 22);
     ctx->hits_cap = t->hits_capacity ? t->hits_capacity : (1u << 20);
+    ctx->trace = t->trace ? 1u : 0u;
     return OIBVH_OK;
 }
 
@@ -249,6 +255,7 @@ extern "C" oibvh_status oibvh_ctx_get_tuning(const oibvh_ctx* ctx, oibvh_tuning*
     t->traverse_expand = ctx->trav_expand;
     t->queue_capacity = ctx->queue_cap;
     t->hits_capacity = ctx->hits_cap;
+    t->trace = ctx->trace;
     return OIBVH_OK;
 }
 
@@ -582,6 +589,7 @@ extern "C" void oibvh_scene_destroy(oibvh_scene* s)
     dfree(s->ring);
     dfree(s->ctl);
     dfree(s->hits);
+    dfree(s->trace);
     if (s->h_ctl) (void)hipHostFree(s->h_ctl);
     oibvh_ctx* c = s->ctx;
     delete s;
@@ -726,12 +734,23 @@ static oibvh_status enqueue_detect(oibvh_scene* s, bool reset_sticky)
     uint32_t grid = c->trav_blocks ? c->trav_blocks
                                    : (uint32_t)std::max(1, c->cus) * (uint32_t)c->trav_occ[variant];
     HIPCHK(c, oi_launch_seed(st, s->ring, s->ring_cap, s->d_front, s->n_front, s->ctl, reset_sticky ? 1u : 0u, grid));
+    unsigned long long* trace = nullptr;
+    if (c->trace) {
+        if (s->trace_wg < grid) {
+            dfree(s->trace);
+            s->trace_wg = 0;
+            HIPCHK(c, dalloc(&s->trace, (size_t)grid * OI_TRACE_WORDS));
+            s->trace_wg = grid;
+        }
+        trace = s->trace;
+    }
+    s->last_grid = grid;
     const unsigned long long timeout = 30ull * 1000ull * c->clock_khz;  // 30 s
     HIPCHK(c, oi_launch_traverse(st, grid, n > 4 ? s->d_desc : nullptr, desc.data(), n, s->ring, s->ring_cap,
                                  s->ctl, s->hits, s->hits_cap,
                                  c->trav_expand ? c->trav_expand : std::min(3u, std::max(1u, s->last_expand)),
                                  s->last_flags & 3u, timeout,
-                                 s->implicit_end));
+                                 s->implicit_end, trace));
     s->unchecked = true;
     s->have_result = true;
     return OIBVH_OK;
@@ -871,5 +890,20 @@ extern "C" oibvh_status oibvh_scene_get_pairs(oibvh_scene* s, uint32_t* out, uin
         out[4ull * i + 2] = h[i].z;
         out[4ull * i + 3] = h[i].w;
     }
+    return OIBVH_OK;
+}
+
+extern "C" oibvh_status oibvh_scene_get_trace(oibvh_scene* s, uint64_t* out, uint64_t cap, uint32_t* n_wg)
+{
+    if (!s || !n_wg) return OIBVH_ERR_INVALID;
+    oibvh_ctx* c = s->ctx;
+    oibvh_status r = check_detect(s);
+    if (r != OIBVH_OK) return r;
+    *n_wg = s->trace ? s->last_grid : 0;
+    if (!s->trace) return OIBVH_OK;
+    const uint64_t words = (uint64_t)s->last_grid * OI_TRACE_WORDS;
+    if (!out || cap < words) return set_err(c, OIBVH_ERR_CAPACITY, "need %llu words", (unsigned long long)words);
+    HIPCHK(c, hipMemcpyAsync(out, s->trace, words * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return OIBVH_OK;
 }
