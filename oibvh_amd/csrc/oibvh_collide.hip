@@ -39,18 +39,19 @@ __device__ __forceinline__ void st64(u64* p, u64 v) { __hip_atomic_store(p, v, _
 __device__ __forceinline__ int ld32i(const int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 __device__ __forceinline__ uint32_t ld32(const uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
-// Reserve `need` ring slots; fails only when the ring would exceed capacity
-// (entries reserved by pushers minus entries reserved by poppers).
-__device__ bool ring_reserve(oi_trav_ctl* ctl, uint32_t cap, uint32_t need, u64& base)
+// Reserve `need` ring slots. A detect never wraps the ring: every index it
+// reserves lies in [base, base + cap), so each slot it writes was emptied by an
+// earlier detect (or the reset) and pushes need not wait. Fails when this
+// detect would exceed the capacity; the host then grows the ring and re-runs.
+__device__ bool ring_reserve(oi_trav_ctl* ctl, u64 base, uint32_t cap, uint32_t need, u64& out)
 {
     for (;;) {
         const u64 t = ld64(&ctl->tail);
-        const u64 h = ld64(&ctl->head);
-        if (t - h + need > cap) return false;
+        if (t + need - base > cap) return false;
         u64 expected = t;
         if (__hip_atomic_compare_exchange_strong(&ctl->tail, &expected, t + need, __ATOMIC_RELAXED,
                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-            base = t;
+            out = t;
             return true;
         }
     }
@@ -71,27 +72,18 @@ __device__ __forceinline__ u64 ring_pack(uint32_t a, uint32_t b, uint32_t lap)
 __device__ __forceinline__ u64 ring_empty(uint32_t lap) { return ring_pack(RING_IDX, RING_IDX, lap & 15u); }
 __device__ __forceinline__ uint32_t ring_lap_of(u64 v) { return ((uint32_t)v >> 30) | (((uint32_t)(v >> 32) >> 30) << 2); }
 
-// A reserved slot is filled / emptied promptly by its owner, so these waits
-// end; the wall-clock limit only guards against a fault. On expiry they flag
-// OI_ERR_TIMEOUT and report failure, and the caller drops the entry.
-__device__ __forceinline__ bool slot_put(u64* ring, uint32_t shift, u64 idx, uint2 e, oi_trav_ctl* ctl, u64 t0,
-                                         u64 limit)
+// Push: the slot of a reserved index is free (see ring_reserve), so a plain
+// atomic store publishes the entry.
+__device__ __forceinline__ void slot_put(u64* ring, uint32_t shift, u64 idx, uint2 e)
 {
     const uint32_t mask = (1u << shift) - 1u;
     const uint32_t lap = (uint32_t)(idx >> shift) & 15u;
-    u64* slot = ring + (idx & mask);
-    const u64 want = ring_empty(lap);
-    while (ld64(slot) != want) {
-        __builtin_amdgcn_s_sleep(1);
-        if (wall_clock64() - t0 > limit) {
-            __hip_atomic_fetch_or(&ctl->error, OI_ERR_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return false;
-        }
-    }
-    st64(slot, ring_pack(e.x, e.y, lap));
-    return true;
+    st64(ring + (idx & mask), ring_pack(e.x, e.y, lap));
 }
 
+// Pop: a reserved index is written promptly by its pusher, so the wait ends;
+// the wall-clock limit only guards against a fault. On expiry it flags
+// OI_ERR_TIMEOUT and reports failure, and the caller drops the entry.
 __device__ __forceinline__ bool slot_take(u64* ring, uint32_t shift, u64 idx, uint2& e, oi_trav_ctl* ctl, u64 t0,
                                           u64 limit)
 {
@@ -508,7 +500,7 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
     __shared__ uint32_t s_ibase[MESH_LDS];
     __shared__ oi_mesh_desc s_desc[MESH_LDS];
     __shared__ uint32_t s_wsum[4];
-    __shared__ uint32_t s_n, s_cn, s_pop_n, s_flag;
+    __shared__ uint32_t s_n, s_cn, s_pop_n, s_flag, s_idle_hint;
     __shared__ u64 s_pop_h, s_push_base;
 
     const uint32_t tid = threadIdx.x;
@@ -529,6 +521,7 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
     if (tid == 0) {
         s_n = 0;
         s_cn = 0;
+        s_idle_hint = 1;
     }
     __syncthreads();
 
@@ -551,9 +544,10 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
 #define TRACE_N(slot, v) \
     if (tracing && tid == 0) tr[slot] += (v);
     const uint32_t cap = 1u << A.ring_shift;
+    const u64 ring_base = ld64(&ctl->base);
 
     const uint32_t fan = 1u << (2u * A.expand);  // worst-case children per entry
-    uint32_t backoff = 1, step = 0;
+    uint32_t backoff = 1;
 
     // Move the bottom D entries of the LDS stack to the ring (the oldest
     // entries: the largest subtrees, the most useful work for another
@@ -562,7 +556,7 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
     auto move_bottom = [&](uint32_t D) {
         const uint32_t m = s_n;
         for (uint32_t i = tid; i < D; i += TB)
-            if (!slot_put(A.ring, A.ring_shift, s_push_base + i, s_stack[i], ctl, t0, A.timeout_ticks)) break;
+            slot_put(A.ring, A.ring_shift, s_push_base + i, s_stack[i]);
         __syncthreads();
         // chunk r's destination lies below every later chunk's source
         for (uint32_t r = 0; D + r * TB < m; r++) {
@@ -594,7 +588,12 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
                 if (h < t) {
                     const u64 avail = t - h;
                     uint32_t take = (avail < want) ? (uint32_t)avail : want;
-                    if (take > 64 && s_n == 0 && avail < 4096) take = 64;
+                    // an idle workgroup takes its share of a short ring
+                    if (s_n == 0) {
+                        const uint32_t share = (uint32_t)(avail / (u64)(s_idle_hint > 0 ? s_idle_hint : 1u));
+                        const uint32_t lim = share < 32u ? 32u : share;
+                        if (take > lim) take = lim;
+                    }
                     if (idle) __hip_atomic_fetch_add(&ctl->pending, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     u64 expected = h;
                     if (__hip_atomic_compare_exchange_strong(&ctl->head, &expected, h + take, __ATOMIC_RELAXED,
@@ -636,6 +635,7 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
                 }
                 const int pend = ld32i(&ctl->pending);
                 const uint32_t err = ld32(&ctl->error);
+                s_idle_hint = ld32(&ctl->idle);
                 uint32_t f = 0;
                 if (pend <= 0 || err) f = 1;
                 else if (wall_clock64() - t0 > A.timeout_ticks) {
@@ -650,7 +650,7 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
             if (s_flag) break;
             // exponential back-off keeps idle workgroups off the shared counters
             for (uint32_t b = 0; b < backoff; b++) __builtin_amdgcn_s_sleep(32);
-            if (backoff < 32) backoff <<= 1;
+            if (backoff < 4) backoff <<= 1;
             continue;
         }
         backoff = 1;
@@ -667,7 +667,7 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
             if (tid == 0) {
                 u64 b = 0;
                 __hip_atomic_fetch_add(&ctl->pending, (int)(n / 2), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                s_flag = ring_reserve(ctl, cap, n / 2, b) ? 0u : 1u;
+                s_flag = ring_reserve(ctl, ring_base, cap, n / 2, b) ? 0u : 1u;
                 if (s_flag)
                     __hip_atomic_fetch_or(&ctl->error, OI_ERR_QUEUE_OVERFLOW, __ATOMIC_RELAXED,
                                           __HIP_MEMORY_SCOPE_AGENT);
@@ -681,6 +681,10 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
             TRACE_T(OI_TR_T_MOVE);
             continue;
         }
+
+        // idle count of the other workgroups, loaded now, used after the step
+        uint32_t idle_now = 0;
+        if (tid == 0) idle_now = ld32(&ctl->idle);
 
         // ---- take as many entries from the top as the stack can absorb the
         // children of (take * fan <= free slots + take), at most one per thread
@@ -789,53 +793,29 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
             TRACE_T(OI_TR_T_NARROW);
         }
 
-        // ---- every 4th step: donate the bottom half of a deep stack when
-        // other workgroups are idle
-        step++;
-        if ((step & 3u) == 0u) {
-            if (tid == 0) {
-                s_flag = 0;
-                if (s_n >= 2 * TB && ld32(&ctl->idle) > 0u) {
-                    const uint32_t D = s_n / 2;
-                    u64 b = 0;
-                    __hip_atomic_fetch_add(&ctl->pending, (int)D, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    if (ring_reserve(ctl, cap, D, b)) {
-                        s_push_base = b;
-                        s_flag = D;
-                    } else {
-                        __hip_atomic_fetch_sub(&ctl->pending, (int)D, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    }
+        // ---- while other workgroups are idle, everything beyond the next
+        // batch goes to the ring (oldest entries: the largest subtrees)
+        if (tid == 0) {
+            s_flag = 0;
+            s_idle_hint = idle_now;
+            if (idle_now > 0u && s_n > TB) {
+                const uint32_t D = s_n - TB;
+                u64 b = 0;
+                __hip_atomic_fetch_add(&ctl->pending, (int)D, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (ring_reserve(ctl, ring_base, cap, D, b)) {
+                    s_push_base = b;
+                    s_flag = D;
+                } else {
+                    __hip_atomic_fetch_sub(&ctl->pending, (int)D, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
             }
-            __syncthreads();
-            if (s_flag) {
-                TRACE_N(OI_TR_MOVES, 1);
-                TRACE_N(OI_TR_MOVED, s_flag);
-                move_bottom(s_flag);
-                TRACE_T(OI_TR_T_MOVE);
-            }
         }
-    }
-
-    // ---- drain the remaining candidates
-    __syncthreads();
-    const uint32_t cn = s_cn;
-    if (cn) {
-        bool hit = false;
-        uint4 out;
-        uint2 c = (tid < cn) ? s_cand[tid] : make_uint2(0, 0);
-        narrow_one<FP64>(A, s_ibase, s_desc, c, tid < cn, hit, out, n_fp64, n_adj);
-        uint32_t HT;
-        const uint32_t hoff = block_scan(hit ? 1u : 0u, s_wsum, HT);
-        if (tid == 0)
-            s_pop_h = HT ? __hip_atomic_fetch_add(&ctl->n_hits, HT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
         __syncthreads();
-        if (hit) {
-            const uint32_t pos = (uint32_t)s_pop_h + hoff;
-            if (pos < A.hits_cap)
-                A.hits[pos] = out;
-            else
-                __hip_atomic_fetch_or(&ctl->sticky, OI_ERR_HITS_OVERFLOW, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (s_flag) {
+            TRACE_N(OI_TR_MOVES, 1);
+            TRACE_N(OI_TR_MOVED, s_flag);
+            move_bottom(s_flag);
+            TRACE_T(OI_TR_T_MOVE);
         }
     }
 
@@ -884,6 +864,7 @@ __global__ void k_seed(u64* ring, uint32_t shift, const uint2* front, uint32_t n
         // errors of the previous traversal move to `sticky` unless the caller
         // checked it synchronously
         ctl->sticky = reset_sticky ? 0u : (ctl->sticky | ctl->error);
+        ctl->base = base;
         ctl->tail = base + n0;
         ctl->pending = (int)n0;
         ctl->n_hits = 0;
@@ -906,6 +887,7 @@ __global__ void k_ring_reset(u64* ring, uint32_t n, oi_trav_ctl* ctl)
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ctl->head = 0;
         ctl->tail = 0;
+        ctl->base = 0;
     }
 }
 

@@ -86,7 +86,8 @@ struct __attribute__((aligned(128))) oi_trav_ctl {
     unsigned long long head;  // ring entries popped (reserved)
     char pad0[120];
     unsigned long long tail;  // ring entries pushed (reserved)
-    char pad1[120];
+    unsigned long long base;  // ring index where this detect started (the ring never wraps within one)
+    char pad1[112];
     int pending;              // ring entries + workgroups holding BVTT entries
     char pad2[124];
     uint32_t idle;            // workgroups currently without work
