@@ -819,6 +819,28 @@ __global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
         }
     }
 
+    // ---- drain the remaining candidates (fewer than one batch)
+    __syncthreads();
+    const uint32_t cn = s_cn;
+    if (cn) {
+        bool hit = false;
+        uint4 out;
+        uint2 c = (tid < cn) ? s_cand[tid] : make_uint2(0, 0);
+        narrow_one<FP64>(A, s_ibase, s_desc, c, tid < cn, hit, out, n_fp64, n_adj);
+        uint32_t HT;
+        const uint32_t hoff = block_scan(hit ? 1u : 0u, s_wsum, HT);
+        if (tid == 0)
+            s_pop_h = HT ? __hip_atomic_fetch_add(&ctl->n_hits, HT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+        __syncthreads();
+        if (hit) {
+            const uint32_t pos = (uint32_t)s_pop_h + hoff;
+            if (pos < A.hits_cap)
+                A.hits[pos] = out;
+            else
+                __hip_atomic_fetch_or(&ctl->error, OI_ERR_HITS_OVERFLOW, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+
     if (tracing && tid == 0) {
         tr[OI_TR_T_EXIT] = wall_clock64() - t0;
         for (int k = 0; k < OI_TRACE_WORDS; k++) A.trace[(u64)blockIdx.x * OI_TRACE_WORDS + k] = tr[k];
