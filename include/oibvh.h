@@ -31,7 +31,7 @@ typedef enum {
     OIBVH_ERR_NOT_BUILT = 2,  /* tree used before build (the reference asserts, scene.cu:97) */
     OIBVH_ERR_HIP = 3,        /* a HIP runtime call failed; see oibvh_last_error */
     OIBVH_ERR_CAPACITY = 4,   /* caller buffer too small; the required size is reported */
-    OIBVH_ERR_OVERFLOW = 5,   /* device work queue exceeded its maximum size */
+    OIBVH_ERR_OVERFLOW = 5,   /* a detect buffer would exceed its maximum size */
     OIBVH_ERR_TIMEOUT = 6,    /* traversal exceeded its time limit */
     OIBVH_ERR_NO_DEVICE = 7   /* no usable gfx950 device */
 } oibvh_status;
@@ -185,25 +185,22 @@ OIBVH_API oibvh_status oibvh_scene_get_stats(oibvh_scene* scene, oibvh_detect_st
 typedef struct {
     uint32_t reduce_leaves_per_thread; /* 1, 2, 4 or 8: the build/refit reduce gives each workgroup a
                                           subtree of 256 x this many leaves (0 = by mesh size) */
-    uint32_t traverse_blocks;          /* persistent traversal grid (0 = CUs x resident workgroups) */
+    uint32_t traverse_blocks;          /* workgroups per BVTT-level and narrow-phase launch (0 = 4 per CU) */
     uint32_t traverse_expand;          /* levels each side descends per BVTT step, 1..3 (0 = the
                                           detect call's expandLevels, clamped to 1..3) */
-    uint32_t queue_capacity;           /* initial work-ring entries, a power of two (0 = 2^22) */
+    uint32_t front_capacity;           /* initial BVTT front entries per buffer, a power of two (0 = 2^22) */
     uint32_t hits_capacity;            /* initial hit-list entries (0 = 2^20) */
-    uint32_t trace;                    /* 1 = record a per-workgroup traversal trace */
+    uint32_t candidate_capacity;       /* initial candidate leaf-pair entries (0 = 2^20) */
 } oibvh_tuning;
 
-/* Both buffers grow on overflow (the detect is re-run). */
+/* The detect buffers grow on overflow (the detect is re-run). */
 OIBVH_API oibvh_status oibvh_ctx_set_tuning(oibvh_ctx* ctx, const oibvh_tuning* tuning);
 OIBVH_API oibvh_status oibvh_ctx_get_tuning(const oibvh_ctx* ctx, oibvh_tuning* tuning);
 
-/* Per-workgroup trace of the last detect (tuning.trace = 1): 16 uint64 per
- * workgroup -- steps, entries tested, ring pops, entries popped, moves to the
- * ring, entries moved, idle iterations, narrow batches, then wall-clock ticks
- * spent stepping, idle, moving, in the narrow phase and refilling, ticks to
- * first work, ticks to exit, deepest stack. *n_wg receives the workgroup count
- * (0 when tracing was off). Diagnostics only; no reference counterpart. */
-OIBVH_API oibvh_status oibvh_scene_get_trace(oibvh_scene* scene, uint64_t* out, uint64_t cap, uint32_t* n_wg);
+/* BVTT front length of each level of the last detect (level 0 = the initial
+ * front); *n receives the number of levels. Diagnostics; the reference prints
+ * these (scene.cu:274-276). */
+OIBVH_API oibvh_status oibvh_scene_get_level_counts(oibvh_scene* scene, uint32_t* out, uint64_t cap, uint32_t* n);
 
 #ifdef __cplusplus
 }

@@ -32,8 +32,8 @@ _f32p, _u32p = C.POINTER(C.c_float), C.POINTER(C.c_uint32)
 
 class Tuning(C.Structure):
     _fields_ = [("reduce_leaves_per_thread", C.c_uint32), ("traverse_blocks", C.c_uint32),
-                ("traverse_expand", C.c_uint32), ("queue_capacity", C.c_uint32), ("hits_capacity", C.c_uint32),
-                ("trace", C.c_uint32)]
+                ("traverse_expand", C.c_uint32), ("front_capacity", C.c_uint32), ("hits_capacity", C.c_uint32),
+                ("candidate_capacity", C.c_uint32)]
 
 
 class DetectStats(C.Structure):
@@ -73,7 +73,7 @@ API = [
     ("oibvh_scene_get_pairs", _i, [_vp, _u32p, _u64, _u32p]),
     ("oibvh_scene_hits_device", _i, [_vp, C.POINTER(_vp), _u32p]),
     ("oibvh_scene_get_stats", _i, [_vp, C.POINTER(DetectStats)]),
-    ("oibvh_scene_get_trace", _i, [_vp, C.POINTER(C.c_uint64), _u64, _u32p]),
+    ("oibvh_scene_get_level_counts", _i, [_vp, _u32p, _u64, _u32p]),
 ]
 
 _lib = None

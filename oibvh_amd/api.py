@@ -248,21 +248,13 @@ class Scene:
                                                            C.byref(got)), "get_pairs")
         return out[: got.value]
 
-    TRACE_FIELDS = ["steps", "tests", "pops", "popped", "moves", "moved", "idle_loops", "narrow", "t_step",
-                    "t_idle", "t_move", "t_narrow", "t_refill", "t_first", "t_exit", "max_stack"]
-
-    def trace(self):
-        """Per-workgroup trace of the last detect (needs tuning trace=1):
-        (n_wg, 16) uint64 array, columns TRACE_FIELDS; None when off."""
+    def level_counts(self) -> np.ndarray:
+        """BVTT front length per level of the last detect."""
+        out = np.zeros(64, np.uint32)
         n = C.c_uint32()
-        st = self._lib.oibvh_scene_get_trace(self.h, None, 0, C.byref(n))
-        if n.value == 0:
-            _check(self.ctx.h, st, "get_trace")
-            return None
-        out = np.zeros((n.value, 16), np.uint64)
-        _check(self.ctx.h, self._lib.oibvh_scene_get_trace(self.h, out.ctypes.data_as(C.POINTER(C.c_uint64)),
-                                                           out.size, C.byref(n)), "get_trace")
-        return out
+        _check(self.ctx.h, self._lib.oibvh_scene_get_level_counts(self.h, out.ctypes.data_as(L._u32p), out.size,
+                                                                  C.byref(n)), "get_level_counts")
+        return out[: n.value].copy()
 
     def stats(self) -> dict:
         s = L.DetectStats()

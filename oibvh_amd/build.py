@@ -33,7 +33,7 @@ def _compile(src: str) -> str:
     if src.endswith(".hip"):
         cmd = [HIPCC, "-c", *COMMON, *DEVICE, path, "-o", obj]
     else:
-        cmd = [HIPCC, "-c", "-O2", *COMMON, path, "-o", obj]
+        cmd = [HIPCC, "-c", "-O2", f"--offload-arch={ARCH}", *COMMON, path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

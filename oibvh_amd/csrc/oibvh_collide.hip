@@ -3,23 +3,20 @@
 // The reference relaunches traversal_kernel once per BVTT level with a host
 // round trip for the front size (scene.cu:272-311, collide.cu:76-246) and then
 // runs triangle_intersect_kernel over the candidate list (collide.cu:248-300).
-// Here one persistent launch does both:
-//   * each workgroup keeps a BVTT stack in LDS and pops 256 entries per step
-//     (one per thread): AABB overlap test (collide.cu:12-17), leaf x leaf ->
-//     candidate, otherwise both sides are expanded `expand` levels (a leaf
-//     stands for itself, collide.cu:165-176) and the children are pushed back
-//     with a block prefix sum;
-//   * a global ring queue takes spills and donations to idle workgroups
-//     (work stealing); a `pending` counter (ring entries + workgroups holding
-//     entries, updated only when work moves through the ring or a workgroup
-//     changes between busy and idle) lets idle workgroups stop;
-//   * candidates are compacted into an LDS buffer and tested 256 at a time
-//     with the 17-axis SAT (utils.cpp:71-169 / cuda_intersect_tritri.h:350-434),
-//     hits appended with one atomic per batch as (meshA, meshB, origTriA,
-//     origTriB) -- original face ids, mapped through the build permutation.
+// Here the levels are launched back to back with no host round trip:
+//   k_bvtt_level  one BVTT level, grid-stride over the current front (its
+//                 length is read from device memory): AABB overlap test
+//                 (collide.cu:12-17); leaf x leaf -> candidate list, otherwise
+//                 both sides descend `expand` levels (a leaf stands for itself,
+//                 collide.cu:165-176) and the children go to the next front.
+//                 Children and candidates are compacted with a block prefix sum
+//                 and one device-scope fetch_add per block;
+//   k_narrow      17-axis SAT (utils.cpp:71-169 / cuda_intersect_tritri.h:
+//                 350-434) over the candidate list, hits appended as
+//                 (meshA, meshB, origTriA, origTriB) in original face ids.
 // Self mode (no reference counterpart) also walks each mesh against itself:
-// a node paired with itself expands to the unordered pairs of its children and
-// leaf pairs sharing a vertex index are dropped before the narrow phase.
+// a node paired with itself expands to the unordered pairs of its children;
+// leaf pairs sharing a vertex index are dropped before the SAT.
 //
 // BVTT entries are two 32-bit implicit indices in the scene's concatenated
 // implicit space (mesh m spans [ibase_m, ibase_m + 2^(L_m+1) - 1)).
@@ -28,82 +25,10 @@
 #include "oibvh_common.h"
 
 #define TB 256
-#define STACK_CAP 3072
-#define CAND_CAP 512
 #define MESH_LDS 16
+#define INLINE_MESHES 4
 
 typedef unsigned long long u64;
-
-__device__ __forceinline__ u64 ld64(const u64* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-__device__ __forceinline__ void st64(u64* p, u64 v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-__device__ __forceinline__ int ld32i(const int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-__device__ __forceinline__ uint32_t ld32(const uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-
-// Reserve `need` ring slots. A detect never wraps the ring: every index it
-// reserves lies in [base, base + cap), so each slot it writes was emptied by an
-// earlier detect (or the reset) and pushes need not wait. Fails when this
-// detect would exceed the capacity; the host then grows the ring and re-runs.
-__device__ bool ring_reserve(oi_trav_ctl* ctl, u64 base, uint32_t cap, uint32_t need, u64& out)
-{
-    for (;;) {
-        const u64 t = ld64(&ctl->tail);
-        if (t + need - base > cap) return false;
-        u64 expected = t;
-        if (__hip_atomic_compare_exchange_strong(&ctl->tail, &expected, t + need, __ATOMIC_RELAXED,
-                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-            out = t;
-            return true;
-        }
-    }
-}
-
-// Ring slots. Ring index i maps to slot i & (cap - 1) and to lap
-// (i >> log2 cap) & 15. A slot moves through EMPTY(lap) -> FULL(lap) ->
-// EMPTY(lap + 1): the pusher of index i waits for EMPTY(lap(i)) and the popper
-// of index i waits for FULL(lap(i)), so neither can act on a value of another
-// lap when the ring wraps. Entries are two 30-bit implicit indices (the host
-// keeps a scene's implicit space below 2^30 - 1); the lap tag lives in bits
-// 31:30 and 63:62, and EMPTY is the all-ones index pair.
-#define RING_IDX 0x3fffffffu
-__device__ __forceinline__ u64 ring_pack(uint32_t a, uint32_t b, uint32_t lap)
-{
-    return (u64)(a | ((lap & 3u) << 30)) | ((u64)(b | ((lap >> 2) << 30)) << 32);
-}
-__device__ __forceinline__ u64 ring_empty(uint32_t lap) { return ring_pack(RING_IDX, RING_IDX, lap & 15u); }
-__device__ __forceinline__ uint32_t ring_lap_of(u64 v) { return ((uint32_t)v >> 30) | (((uint32_t)(v >> 32) >> 30) << 2); }
-
-// Push: the slot of a reserved index is free (see ring_reserve), so a plain
-// atomic store publishes the entry.
-__device__ __forceinline__ void slot_put(u64* ring, uint32_t shift, u64 idx, uint2 e)
-{
-    const uint32_t mask = (1u << shift) - 1u;
-    const uint32_t lap = (uint32_t)(idx >> shift) & 15u;
-    st64(ring + (idx & mask), ring_pack(e.x, e.y, lap));
-}
-
-// Pop: a reserved index is written promptly by its pusher, so the wait ends;
-// the wall-clock limit only guards against a fault. On expiry it flags
-// OI_ERR_TIMEOUT and reports failure, and the caller drops the entry.
-__device__ __forceinline__ bool slot_take(u64* ring, uint32_t shift, u64 idx, uint2& e, oi_trav_ctl* ctl, u64 t0,
-                                          u64 limit)
-{
-    const uint32_t mask = (1u << shift) - 1u;
-    const uint32_t lap = (uint32_t)(idx >> shift) & 15u;
-    u64* slot = ring + (idx & mask);
-    u64 v;
-    for (;;) {
-        v = ld64(slot);
-        if (ring_lap_of(v) == lap && ((uint32_t)v & RING_IDX) != RING_IDX) break;
-        __builtin_amdgcn_s_sleep(1);
-        if (wall_clock64() - t0 > limit) {
-            __hip_atomic_fetch_or(&ctl->error, OI_ERR_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return false;
-        }
-    }
-    e = make_uint2((uint32_t)v & RING_IDX, (uint32_t)(v >> 32) & RING_IDX);
-    st64(slot, ring_empty(lap + 1u));
-    return true;
-}
 
 __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t* s_wsum, uint32_t& total)
 {
@@ -121,6 +46,15 @@ __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t* s_wsum, uin
     total = w0 + w1 + w2 + w3;
     __syncthreads();
     return before + x - v;
+}
+
+__device__ __forceinline__ uint32_t add_u32(uint32_t* p, uint32_t v)
+{
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void or_u32(uint32_t* p, uint32_t v)
+{
+    __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------------------
@@ -338,23 +272,12 @@ __device__ bool sat_filtered(const v3f* P, const v3f* Q, uint32_t& n_fp64)
 }
 
 // ---------------------------------------------------------------------------
-// traversal
+// scene tables
 // ---------------------------------------------------------------------------
-#define INLINE_MESHES 4
-struct TravArgs {
-    const oi_mesh_desc* meshes;  // mesh table in device memory (scenes of more than INLINE_MESHES meshes)
+struct MeshTab {
+    const oi_mesh_desc* meshes;       // device table (scenes of more than INLINE_MESHES meshes)
     uint32_t n_meshes;
     oi_mesh_desc inl[INLINE_MESHES];  // small scenes: the table travels in the kernel arguments
-    u64* ring;
-    uint32_t ring_shift;  // log2 of the ring capacity
-    oi_trav_ctl* ctl;
-    uint4* hits;
-    uint32_t hits_cap;
-    uint32_t expand;
-    uint32_t flags;
-    u64 timeout_ticks;
-    uint32_t implicit_end;  // end of the scene's implicit index space
-    unsigned long long* trace;  // OI_TRACE_WORDS per workgroup, or nullptr
 };
 
 struct MeshRef {
@@ -365,13 +288,32 @@ struct MeshRef {
     uint32_t L, vl, ibase;
 };
 
-__device__ __forceinline__ uint32_t find_mesh(const oi_mesh_desc* md, const uint32_t* s_ibase, uint32_t n,
-                                              uint32_t g)
+// copy the first MESH_LDS mesh descriptors to LDS
+__device__ __forceinline__ void load_mesh_tab(const MeshTab& T, oi_mesh_desc* s_desc, uint32_t* s_ibase)
 {
-    uint32_t lo = 0, hi = n - 1;
+    const uint32_t tid = threadIdx.x;
+    if (tid < MESH_LDS && tid < T.n_meshes) {
+        if (T.n_meshes <= INLINE_MESHES) {
+            // constant indices keep the argument block out of scratch memory
+            if (tid == 0) s_desc[0] = T.inl[0];
+            if (tid == 1) s_desc[1] = T.inl[1];
+            if (tid == 2) s_desc[2] = T.inl[2];
+            if (tid == 3) s_desc[3] = T.inl[3];
+        } else {
+            s_desc[tid] = T.meshes[tid];
+        }
+    }
+    __syncthreads();
+    if (tid < MESH_LDS && tid < T.n_meshes) s_ibase[tid] = s_desc[tid].ibase;
+    __syncthreads();
+}
+
+__device__ __forceinline__ uint32_t find_mesh(const MeshTab& T, const uint32_t* s_ibase, uint32_t g)
+{
+    uint32_t lo = 0, hi = T.n_meshes - 1;
     while (lo < hi) {
         const uint32_t mid = (lo + hi + 1) >> 1;
-        const uint32_t b = (n <= MESH_LDS) ? s_ibase[mid] : md[mid].ibase;
+        const uint32_t b = (mid < MESH_LDS) ? s_ibase[mid] : T.meshes[mid].ibase;
         if (b <= g)
             lo = mid;
         else
@@ -380,10 +322,9 @@ __device__ __forceinline__ uint32_t find_mesh(const oi_mesh_desc* md, const uint
     return lo;
 }
 
-// mesh m's descriptor: LDS copy for the first MESH_LDS meshes, else device memory
-__device__ __forceinline__ MeshRef mesh_ref(const oi_mesh_desc* md, const oi_mesh_desc* s_desc, uint32_t m)
+__device__ __forceinline__ MeshRef mesh_ref(const MeshTab& T, const oi_mesh_desc* s_desc, uint32_t m)
 {
-    const oi_mesh_desc& d = (m < MESH_LDS) ? s_desc[m] : md[m];
+    const oi_mesh_desc& d = (m < MESH_LDS) ? s_desc[m] : T.meshes[m];
     return MeshRef{d.nodes, d.sfaces, d.pos, d.perm, d.L, d.vl, d.ibase};
 }
 
@@ -435,18 +376,131 @@ __device__ __forceinline__ void tri_pair(uint32_t k, uint32_t n, uint32_t& x, ui
     y = x + k;
 }
 
+// ---------------------------------------------------------------------------
+// k_bvtt_level: one level of the BVTT walk
+// ---------------------------------------------------------------------------
+struct LevelArgs {
+    MeshTab tab;
+    const uint2* front_in;
+    const uint32_t* count_in;   // length of front_in (written by the previous level / the seed)
+    uint2* front_out;
+    uint32_t* count_out;        // zeroed by the seed
+    uint32_t cap_out;
+    uint2* cands;
+    uint32_t* cand_count;
+    uint32_t cand_cap;
+    uint32_t* error;
+    uint32_t expand;
+    uint32_t implicit_end;
+};
+
+__global__ void __launch_bounds__(TB) k_bvtt_level(LevelArgs A)
+{
+    __shared__ oi_mesh_desc s_desc[MESH_LDS];
+    __shared__ uint32_t s_ibase[MESH_LDS];
+    __shared__ uint32_t s_wsum[4];
+    __shared__ uint32_t s_obase, s_cbase;
+    const uint32_t n_in = *A.count_in;
+    if (blockIdx.x * TB >= n_in) return;
+    load_mesh_tab(A.tab, s_desc, s_ibase);
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t base = blockIdx.x * TB; base < n_in; base += gridDim.x * TB) {
+        const uint32_t i = base + tid;
+        bool have = i < n_in;
+        uint2 e = have ? A.front_in[i] : make_uint2(0, 0);
+        if (have && (e.x >= A.implicit_end || e.y >= A.implicit_end)) {
+            or_u32(A.error, OI_ERR_CORRUPT);  // never dereference a bad entry
+            have = false;
+        }
+        uint32_t nchild = 0;
+        bool cand = false, selfnode = false;
+        Side sa = {0, 0}, sb = {0, 0};
+        uint32_t baseA = 0, baseB = 0;
+        if (have) {
+            const uint32_t ma = find_mesh(A.tab, s_ibase, e.x);
+            const uint32_t mb = find_mesh(A.tab, s_ibase, e.y);
+            const MeshRef ra = mesh_ref(A.tab, s_desc, ma), rb = mesh_ref(A.tab, s_desc, mb);
+            baseA = ra.ibase;
+            baseB = rb.ibase;
+            const uint32_t ia = e.x - ra.ibase, ib = e.y - rb.ibase;
+            const uint32_t la = oi_ilog2(ia + 1u), lb = oi_ilog2(ib + 1u);
+            selfnode = (e.x == e.y);
+            bool ov = true;
+            if (!selfnode) {
+                float ba[6], bb[6];
+                load_box(ra.nodes, ia - oi_virt_above(la, ra.L, ra.vl), ba);
+                load_box(rb.nodes, ib - oi_virt_above(lb, rb.L, rb.vl), bb);
+                ov = overlap6(ba, bb);
+            }
+            if (ov) {
+                if (la == ra.L && lb == rb.L) {
+                    cand = !selfnode;
+                } else {
+                    sa = expand_side(ia, la, ra.L, ra.vl, A.expand);
+                    sb = expand_side(ib, lb, rb.L, rb.vl, A.expand);
+                    nchild = selfnode ? sa.count * (sa.count + 1u) / 2u : sa.count * sb.count;
+                }
+            }
+        }
+        uint32_t T, C;
+        const uint32_t off = block_scan(nchild, s_wsum, T);
+        const uint32_t coff = block_scan(cand ? 1u : 0u, s_wsum, C);
+        if (tid == 0) {
+            s_obase = T ? add_u32(A.count_out, T) : 0u;
+            s_cbase = C ? add_u32(A.cand_count, C) : 0u;
+        }
+        __syncthreads();
+        const uint32_t ob = s_obase + off;
+        if (nchild && ob + nchild > A.cap_out) or_u32(A.error, OI_ERR_QUEUE_OVERFLOW);
+        for (uint32_t c = 0; c < nchild; c++) {
+            uint32_t x, y;
+            if (selfnode) {
+                tri_pair(c, sa.count, x, y);
+                x = sa.start + x;
+                y = sa.start + y;
+            } else {
+                x = sa.start + c / sb.count;
+                y = sb.start + c % sb.count;
+            }
+            if (ob + c < A.cap_out) A.front_out[ob + c] = make_uint2(baseA + x, baseB + y);
+        }
+        if (cand) {
+            const uint32_t cp = s_cbase + coff;
+            if (cp < A.cand_cap)
+                A.cands[cp] = e;
+            else
+                or_u32(A.error, OI_ERR_CAND_OVERFLOW);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_narrow: SAT over the candidate list
+// ---------------------------------------------------------------------------
+struct NarrowArgs {
+    MeshTab tab;
+    const uint2* cands;
+    const uint32_t* cand_count;
+    uint32_t cand_cap;
+    uint4* hits;
+    uint32_t* n_hits;
+    uint32_t hits_cap;
+    uint32_t* error;
+    unsigned long long* fp64;   // SAT axis comparisons re-evaluated in double
+    unsigned long long* adj;    // self-mode leaf pairs dropped as adjacent
+};
+
 template <bool FP64>
-__device__ __forceinline__ void narrow_one(const TravArgs& A, const uint32_t* s_ibase, const oi_mesh_desc* s_desc,
-                                           uint2 c, bool valid,
-                                           bool& hit, uint4& out, uint32_t& n_fp64, uint32_t& n_adj)
+__device__ __forceinline__ void narrow_one(const MeshTab& T, const uint32_t* s_ibase, const oi_mesh_desc* s_desc,
+                                           uint2 c, bool& hit, uint4& out, uint32_t& n_fp64, uint32_t& n_adj)
 {
     hit = false;
-    if (!valid) return;
-    const uint32_t ma = find_mesh(A.meshes, s_ibase, A.n_meshes, c.x);
-    const uint32_t mb = find_mesh(A.meshes, s_ibase, A.n_meshes, c.y);
-    const MeshRef ra = mesh_ref(A.meshes, s_desc, ma), rb = mesh_ref(A.meshes, s_desc, mb);
-    uint32_t sa = c.x - ra.ibase - ((1u << ra.L) - 1u);
-    uint32_t sb = c.y - rb.ibase - ((1u << rb.L) - 1u);
+    const uint32_t ma = find_mesh(T, s_ibase, c.x);
+    const uint32_t mb = find_mesh(T, s_ibase, c.y);
+    const MeshRef ra = mesh_ref(T, s_desc, ma), rb = mesh_ref(T, s_desc, mb);
+    const uint32_t sa = c.x - ra.ibase - ((1u << ra.L) - 1u);
+    const uint32_t sb = c.y - rb.ibase - ((1u << rb.L) - 1u);
     uint32_t fa[3], fb[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -493,492 +547,131 @@ __device__ __forceinline__ void narrow_one(const TravArgs& A, const uint32_t* s_
 }
 
 template <bool FP64>
-__global__ void __launch_bounds__(TB) k_traverse(TravArgs A)
+__global__ void __launch_bounds__(TB) k_narrow(NarrowArgs A)
 {
-    __shared__ uint2 s_stack[STACK_CAP];
-    __shared__ uint2 s_cand[CAND_CAP];
-    __shared__ uint32_t s_ibase[MESH_LDS];
     __shared__ oi_mesh_desc s_desc[MESH_LDS];
+    __shared__ uint32_t s_ibase[MESH_LDS];
     __shared__ uint32_t s_wsum[4];
-    __shared__ uint32_t s_n, s_cn, s_pop_n, s_flag, s_idle_hint;
-    __shared__ u64 s_pop_h, s_push_base;
-
+    __shared__ uint32_t s_hbase;
+    uint32_t n = *A.cand_count;
+    if (n > A.cand_cap) n = A.cand_cap;
+    if (blockIdx.x * TB >= n) return;
+    load_mesh_tab(A.tab, s_desc, s_ibase);
     const uint32_t tid = threadIdx.x;
-    oi_trav_ctl* ctl = A.ctl;
-    if (tid < MESH_LDS && tid < A.n_meshes) {
-        if (A.n_meshes <= INLINE_MESHES) {
-            // constant indices keep the argument block out of scratch memory
-            if (tid == 0) s_desc[0] = A.inl[0];
-            if (tid == 1) s_desc[1] = A.inl[1];
-            if (tid == 2) s_desc[2] = A.inl[2];
-            if (tid == 3) s_desc[3] = A.inl[3];
-        } else {
-            s_desc[tid] = A.meshes[tid];
-        }
-    }
-    __syncthreads();
-    if (tid < MESH_LDS && tid < A.n_meshes) s_ibase[tid] = s_desc[tid].ibase;
-    if (tid == 0) {
-        s_n = 0;
-        s_cn = 0;
-        s_idle_hint = 1;
-    }
-    __syncthreads();
-
-    u64 n_tests = 0, n_cands = 0;
     uint32_t n_fp64 = 0, n_adj = 0;
-    bool idle = true;  // every workgroup starts without work (the seed counts them all idle)
-    const u64 t0 = wall_clock64();
-    // trace counters (thread 0; only written out when A.trace is set)
-    u64 tr[OI_TRACE_WORDS];
-#pragma unroll
-    for (int k = 0; k < OI_TRACE_WORDS; k++) tr[k] = 0;
-    const bool tracing = A.trace != nullptr;
-    u64 tmark = t0;
-#define TRACE_T(slot)                                       \
-    if (tracing && tid == 0) {                              \
-        const u64 now_ = wall_clock64();                    \
-        tr[slot] += now_ - tmark;                           \
-        tmark = now_;                                       \
-    }
-#define TRACE_N(slot, v) \
-    if (tracing && tid == 0) tr[slot] += (v);
-    const uint32_t cap = 1u << A.ring_shift;
-    const u64 ring_base = ld64(&ctl->base);
-
-    const uint32_t fan = 1u << (2u * A.expand);  // worst-case children per entry
-    uint32_t backoff = 1;
-
-    // Move the bottom D entries of the LDS stack to the ring (the oldest
-    // entries: the largest subtrees, the most useful work for another
-    // workgroup) and shift the rest down. Called by every thread with
-    // uniform D; s_push_base holds the reserved ring index.
-    auto move_bottom = [&](uint32_t D) {
-        const uint32_t m = s_n;
-        for (uint32_t i = tid; i < D; i += TB)
-            slot_put(A.ring, A.ring_shift, s_push_base + i, s_stack[i]);
-        __syncthreads();
-        // chunk r's destination lies below every later chunk's source
-        for (uint32_t r = 0; D + r * TB < m; r++) {
-            const uint32_t i = D + r * TB + tid;
-            uint2 x = make_uint2(0, 0);
-            if (i < m) x = s_stack[i];
-            __syncthreads();
-            if (i < m) s_stack[i - D] = x;
-            __syncthreads();
-        }
-        if (tid == 0) s_n = m - D;
-        __syncthreads();
-    };
-
-    for (;;) {
-        // ---- refill from the ring when the local stack is short. `pending`
-        // counts ring entries plus busy workgroups: a workgroup that turns
-        // busy adds itself before it takes entries out of the ring, so the
-        // count never drops to zero while work exists (a stale low read only
-        // lets an idle workgroup leave early, which loses no work: no
-        // workgroup leaves while it holds entries or sees a non-empty ring).
-        if (tid == 0) {
-            s_pop_n = 0;
-            s_flag = 0;
-            if (s_n < TB) {
-                uint32_t want = TB - s_n;
-                const u64 h = ld64(&ctl->head);
-                const u64 t = ld64(&ctl->tail);
-                if (h < t) {
-                    const u64 avail = t - h;
-                    uint32_t take = (avail < want) ? (uint32_t)avail : want;
-                    // an idle workgroup takes its share of a short ring
-                    if (s_n == 0) {
-                        const uint32_t share = (uint32_t)(avail / (u64)(s_idle_hint > 0 ? s_idle_hint : 1u));
-                        const uint32_t lim = share < 32u ? 32u : share;
-                        if (take > lim) take = lim;
-                    }
-                    if (idle) __hip_atomic_fetch_add(&ctl->pending, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    u64 expected = h;
-                    if (__hip_atomic_compare_exchange_strong(&ctl->head, &expected, h + take, __ATOMIC_RELAXED,
-                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                        s_pop_h = h;
-                        s_pop_n = take;
-                        __hip_atomic_fetch_sub(&ctl->pending, (int)take, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    } else if (idle) {
-                        __hip_atomic_fetch_sub(&ctl->pending, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        const uint32_t pn = s_pop_n;
-        if (pn) {
-            TRACE_N(OI_TR_POPS, 1);
-            TRACE_N(OI_TR_POPPED, pn);
-            const u64 h = s_pop_h;
-            for (uint32_t i = tid; i < pn; i += TB) {
-                uint2 v;
-                if (!slot_take(A.ring, A.ring_shift, h + i, v, ctl, t0, A.timeout_ticks))
-                    v = make_uint2(OI_BAD_ENTRY, OI_BAD_ENTRY);
-                s_stack[s_n + i] = v;
-            }
-            __syncthreads();
-            if (tid == 0) s_n += pn;
-            __syncthreads();
-        }
-        TRACE_T(idle ? OI_TR_T_IDLE : OI_TR_T_REFILL);
-        const uint32_t n = s_n;
-        if (n == 0) {
-            // ---- idle: finished when nothing is pending anywhere
-            if (tid == 0) {
-                if (!idle) {
-                    // busy -> idle
-                    __hip_atomic_fetch_sub(&ctl->pending, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    __hip_atomic_fetch_add(&ctl->idle, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
-                const int pend = ld32i(&ctl->pending);
-                const uint32_t err = ld32(&ctl->error);
-                s_idle_hint = ld32(&ctl->idle);
-                uint32_t f = 0;
-                if (pend <= 0 || err) f = 1;
-                else if (wall_clock64() - t0 > A.timeout_ticks) {
-                    __hip_atomic_fetch_or(&ctl->error, OI_ERR_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    f = 1;
-                }
-                s_flag = f;
-            }
-            idle = true;
-            __syncthreads();
-            TRACE_N(OI_TR_IDLE_LOOPS, 1);
-            if (s_flag) break;
-            // exponential back-off keeps idle workgroups off the shared counters
-            for (uint32_t b = 0; b < backoff; b++) __builtin_amdgcn_s_sleep(32);
-            if (backoff < 4) backoff <<= 1;
-            continue;
-        }
-        backoff = 1;
-        if (idle) {
-            // idle -> busy (its +1 on `pending` preceded the pop)
-            if (tid == 0) __hip_atomic_fetch_sub(&ctl->idle, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (tracing && tid == 0 && tr[OI_TR_T_FIRST] == 0) tr[OI_TR_T_FIRST] = wall_clock64() - t0;
-            idle = false;
-        }
-        if (tracing && tid == 0 && n > tr[OI_TR_MAX_STACK]) tr[OI_TR_MAX_STACK] = n;
-
-        // ---- a stack too full to expand even one entry gives half to the ring
-        if (STACK_CAP - n < fan - 1u) {
-            if (tid == 0) {
-                u64 b = 0;
-                __hip_atomic_fetch_add(&ctl->pending, (int)(n / 2), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                s_flag = ring_reserve(ctl, ring_base, cap, n / 2, b) ? 0u : 1u;
-                if (s_flag)
-                    __hip_atomic_fetch_or(&ctl->error, OI_ERR_QUEUE_OVERFLOW, __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
-                s_push_base = b;
-            }
-            __syncthreads();
-            if (s_flag) break;
-            move_bottom(n / 2);
-            TRACE_N(OI_TR_MOVES, 1);
-            TRACE_N(OI_TR_MOVED, n / 2);
-            TRACE_T(OI_TR_T_MOVE);
-            continue;
-        }
-
-        // idle count of the other workgroups, loaded now, used after the step
-        uint32_t idle_now = 0;
-        if (tid == 0) idle_now = ld32(&ctl->idle);
-
-        // ---- take as many entries from the top as the stack can absorb the
-        // children of (take * fan <= free slots + take), at most one per thread
-        const uint32_t fit = (STACK_CAP - n) / (fan - 1u);
-        uint32_t take = (n < TB) ? n : TB;
-        if (take > fit) take = fit;
-        bool have = tid < take;
-        uint2 e = have ? s_stack[n - take + tid] : make_uint2(0, 0);
-        __syncthreads();
-        const uint32_t nb = n - take;
-
-        // ---- test
-        uint32_t nchild = 0;
-        bool cand = false;
-        Side sa = {0, 0}, sb = {0, 0};
-        bool selfnode = false;
-        uint32_t baseA = 0, baseB = 0;
-        if (have && (e.x >= A.implicit_end || e.y >= A.implicit_end)) {
-            // an entry dropped after a failed hand-over: never dereference it
-            __hip_atomic_fetch_or(&ctl->error, OI_ERR_CORRUPT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            have = false;
-        }
-        if (have) {
-            const uint32_t ma = find_mesh(A.meshes, s_ibase, A.n_meshes, e.x);
-            const uint32_t mb = find_mesh(A.meshes, s_ibase, A.n_meshes, e.y);
-            const MeshRef ra = mesh_ref(A.meshes, s_desc, ma), rb = mesh_ref(A.meshes, s_desc, mb);
-            baseA = ra.ibase;
-            baseB = rb.ibase;
-            const uint32_t ia = e.x - ra.ibase, ib = e.y - rb.ibase;
-            const uint32_t la = oi_ilog2(ia + 1u), lb = oi_ilog2(ib + 1u);
-            selfnode = (e.x == e.y);
-            bool ov = true;
-            if (!selfnode) {
-                float ba[6], bb[6];
-                load_box(ra.nodes, ia - oi_virt_above(la, ra.L, ra.vl), ba);
-                load_box(rb.nodes, ib - oi_virt_above(lb, rb.L, rb.vl), bb);
-                ov = overlap6(ba, bb);
-            }
-            if (ov) {
-                const bool leafA = (la == ra.L), leafB = (lb == rb.L);
-                if (leafA && leafB) {
-                    cand = !selfnode;
-                } else {
-                    sa = expand_side(ia, la, ra.L, ra.vl, A.expand);
-                    sb = expand_side(ib, lb, rb.L, rb.vl, A.expand);
-                    nchild = selfnode ? sa.count * (sa.count + 1u) / 2u : sa.count * sb.count;
-                }
-            }
-        }
-        n_tests += have ? 1u : 0u;
-
-        // ---- children onto the stack (block prefix sum; they always fit)
-        uint32_t T;
-        const uint32_t off = block_scan(nchild, s_wsum, T);
-        for (uint32_t c = 0; c < nchild; c++) {
-            uint32_t x, y;
-            if (selfnode) {
-                tri_pair(c, sa.count, x, y);
-                x = sa.start + x;
-                y = sa.start + y;
-            } else {
-                x = sa.start + c / sb.count;
-                y = sb.start + c % sb.count;
-            }
-            s_stack[nb + off + c] = make_uint2(baseA + x, baseB + y);
-        }
-        // ---- candidates into the LDS buffer
-        uint32_t CT;
-        const uint32_t coff = block_scan(cand ? 1u : 0u, s_wsum, CT);
-        if (cand) s_cand[s_cn + coff] = e;
-        n_cands += cand ? 1u : 0u;
-        __syncthreads();
-        if (tid == 0) {
-            s_n = nb + T;
-            s_cn += CT;
-        }
-        __syncthreads();
-        TRACE_N(OI_TR_STEPS, 1);
-        TRACE_N(OI_TR_TESTS, take);
-        TRACE_T(OI_TR_T_STEP);
-
-        // ---- narrow phase on a full batch of candidates
-        if (s_cn >= TB) {
-            const uint32_t cn = s_cn;
-            uint2 c = s_cand[cn - TB + tid];
-            bool hit;
-            uint4 out;
-            narrow_one<FP64>(A, s_ibase, s_desc, c, true, hit, out, n_fp64, n_adj);
-            uint32_t HT;
-            const uint32_t hoff = block_scan(hit ? 1u : 0u, s_wsum, HT);
-            if (tid == 0) {
-                s_cn = cn - TB;
-                s_pop_h = HT ? __hip_atomic_fetch_add(&ctl->n_hits, HT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                             : 0;
-            }
-            __syncthreads();
-            if (hit) {
-                const uint32_t pos = (uint32_t)s_pop_h + hoff;
-                if (pos < A.hits_cap)
-                    A.hits[pos] = out;
-                else
-                    __hip_atomic_fetch_or(&ctl->error, OI_ERR_HITS_OVERFLOW, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            __syncthreads();
-            TRACE_N(OI_TR_NARROW, 1);
-            TRACE_T(OI_TR_T_NARROW);
-        }
-
-        // ---- while other workgroups are idle, everything beyond the next
-        // batch goes to the ring (oldest entries: the largest subtrees)
-        if (tid == 0) {
-            s_flag = 0;
-            s_idle_hint = idle_now;
-            if (idle_now > 0u && s_n > TB) {
-                const uint32_t D = s_n - TB;
-                u64 b = 0;
-                __hip_atomic_fetch_add(&ctl->pending, (int)D, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (ring_reserve(ctl, ring_base, cap, D, b)) {
-                    s_push_base = b;
-                    s_flag = D;
-                } else {
-                    __hip_atomic_fetch_sub(&ctl->pending, (int)D, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
-            }
-        }
-        __syncthreads();
-        if (s_flag) {
-            TRACE_N(OI_TR_MOVES, 1);
-            TRACE_N(OI_TR_MOVED, s_flag);
-            move_bottom(s_flag);
-            TRACE_T(OI_TR_T_MOVE);
-        }
-    }
-
-    // ---- drain the remaining candidates (fewer than one batch)
-    __syncthreads();
-    const uint32_t cn = s_cn;
-    if (cn) {
+    for (uint32_t base = blockIdx.x * TB; base < n; base += gridDim.x * TB) {
+        const uint32_t i = base + tid;
         bool hit = false;
         uint4 out;
-        uint2 c = (tid < cn) ? s_cand[tid] : make_uint2(0, 0);
-        narrow_one<FP64>(A, s_ibase, s_desc, c, tid < cn, hit, out, n_fp64, n_adj);
-        uint32_t HT;
-        const uint32_t hoff = block_scan(hit ? 1u : 0u, s_wsum, HT);
-        if (tid == 0)
-            s_pop_h = HT ? __hip_atomic_fetch_add(&ctl->n_hits, HT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+        if (i < n) narrow_one<FP64>(A.tab, s_ibase, s_desc, A.cands[i], hit, out, n_fp64, n_adj);
+        uint32_t H;
+        const uint32_t hoff = block_scan(hit ? 1u : 0u, s_wsum, H);
+        if (tid == 0) s_hbase = H ? add_u32(A.n_hits, H) : 0u;
         __syncthreads();
         if (hit) {
-            const uint32_t pos = (uint32_t)s_pop_h + hoff;
+            const uint32_t pos = s_hbase + hoff;
             if (pos < A.hits_cap)
                 A.hits[pos] = out;
             else
-                __hip_atomic_fetch_or(&ctl->error, OI_ERR_HITS_OVERFLOW, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                or_u32(A.error, OI_ERR_HITS_OVERFLOW);
         }
+        __syncthreads();
     }
-
-    if (tracing && tid == 0) {
-        tr[OI_TR_T_EXIT] = wall_clock64() - t0;
-        for (int k = 0; k < OI_TRACE_WORDS; k++) A.trace[(u64)blockIdx.x * OI_TRACE_WORDS + k] = tr[k];
-    }
-#undef TRACE_T
-#undef TRACE_N
-    // ---- statistics (one atomic per workgroup per counter)
-    uint32_t t_;
-    const uint32_t f64tot = block_scan(n_fp64, s_wsum, t_);
-    (void)f64tot;
-    if (t_) {
-        if (tid == 0) __hip_atomic_fetch_add(&ctl->fp64, (u64)t_, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    uint32_t a_;
-    block_scan(n_adj, s_wsum, a_);
+    uint32_t f, a;
+    block_scan(n_fp64, s_wsum, f);
+    block_scan(n_adj, s_wsum, a);
     if (tid == 0) {
-        if (a_) __hip_atomic_fetch_add(&ctl->adj, (u64)a_, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    uint32_t te, ce_;
-    block_scan((uint32_t)n_tests, s_wsum, te);
-    block_scan((uint32_t)n_cands, s_wsum, ce_);
-    if (tid == 0) {
-        __hip_atomic_fetch_add(&ctl->tests, (u64)te, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_add(&ctl->cands, (u64)ce_, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_add(&ctl->exited, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (f) __hip_atomic_fetch_add(A.fp64, (u64)f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (a) __hip_atomic_fetch_add(A.adj, (u64)a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
-// seed: place the initial front at the ring's current head (after a clean
-// traversal head == tail and every slot waits for its next lap) and reset the
-// counters. Only block 0 / thread 0 writes the counters; every thread reads
-// `head`, which no one writes during this kernel.
-__global__ void k_seed(u64* ring, uint32_t shift, const uint2* front, uint32_t n0, oi_trav_ctl* ctl,
-                       uint32_t reset_sticky, uint32_t n_wg)
+// seed: the initial front, zeroed level counts and result counters
+__global__ void k_seed(uint2* front0, const uint2* init, uint32_t n0, uint32_t* counts, uint32_t n_counts,
+                       oi_detect_ctl* ctl, uint32_t reset_sticky)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const u64 base = ctl->head;
-    if (i < n0) {
-        const u64 idx = base + i;
-        ring[idx & ((1u << shift) - 1u)] = ring_pack(front[i].x, front[i].y, (uint32_t)(idx >> shift) & 15u);
-    }
+    if (i < n0) front0[i] = init[i];
+    if (i < n_counts) counts[i] = (i == 0) ? n0 : 0u;
     if (i == 0) {
-        // errors of the previous traversal move to `sticky` unless the caller
+        // errors of the previous detect move to `sticky` unless the caller
         // checked it synchronously
         ctl->sticky = reset_sticky ? 0u : (ctl->sticky | ctl->error);
-        ctl->base = base;
-        ctl->tail = base + n0;
-        ctl->pending = (int)n0;
-        ctl->n_hits = 0;
         ctl->error = 0;
-        ctl->idle = n_wg;
-        ctl->tests = 0;
+        ctl->n_hits = 0;
         ctl->cands = 0;
         ctl->fp64 = 0;
         ctl->adj = 0;
-        ctl->exited = 0;
     }
 }
 
-// ring reset (new ring, or after an aborted traversal): every slot EMPTY(0),
-// head = tail = 0
-__global__ void k_ring_reset(u64* ring, uint32_t n, oi_trav_ctl* ctl)
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static MeshTab make_tab(const oi_mesh_desc* d_meshes, const oi_mesh_desc* h_meshes, uint32_t n)
 {
-    const u64 e0 = ring_empty(0);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) ring[i] = e0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ctl->head = 0;
-        ctl->tail = 0;
-        ctl->base = 0;
-    }
-}
-
-__global__ void k_fill64(u64* p, uint32_t n, u64 v)
-{
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+    MeshTab t{};
+    t.meshes = d_meshes;
+    t.n_meshes = n;
+    for (uint32_t m = 0; m < n && m < INLINE_MESHES; m++) t.inl[m] = h_meshes[m];
+    return t;
 }
 
 extern "C" {
 
-hipError_t oi_launch_fill64(hipStream_t st, unsigned long long* p, uint32_t n, unsigned long long v)
+hipError_t oi_launch_seed(hipStream_t st, uint2* front0, const uint2* init, uint32_t n0, uint32_t* counts,
+                          uint32_t n_counts, oi_detect_ctl* ctl, uint32_t reset_sticky)
 {
-    uint32_t g = (n + 255) / 256;
-    if (g > 4096) g = 4096;
-    if (g == 0) g = 1;
-    hipLaunchKernelGGL(k_fill64, dim3(g), dim3(256), 0, st, p, n, v);
+    const uint32_t n = n0 > n_counts ? n0 : n_counts;
+    const uint32_t g = (n + 255) / 256;
+    hipLaunchKernelGGL(k_seed, dim3(g ? g : 1), dim3(256), 0, st, front0, init, n0, counts, n_counts, ctl,
+                       reset_sticky);
     return hipGetLastError();
 }
 
-hipError_t oi_launch_ring_reset(hipStream_t st, unsigned long long* ring, uint32_t cap, oi_trav_ctl* ctl)
+hipError_t oi_launch_bvtt_level(hipStream_t st, uint32_t grid, const oi_mesh_desc* d_meshes,
+                                const oi_mesh_desc* h_meshes, uint32_t n_meshes, const uint2* front_in,
+                                const uint32_t* count_in, uint2* front_out, uint32_t* count_out, uint32_t cap_out,
+                                uint2* cands, uint32_t* cand_count, uint32_t cand_cap, uint32_t* error,
+                                uint32_t expand, uint32_t implicit_end)
 {
-    uint32_t g = (cap + 255) / 256;
-    if (g > 4096) g = 4096;
-    hipLaunchKernelGGL(k_ring_reset, dim3(g), dim3(256), 0, st, ring, cap, ctl);
-    return hipGetLastError();
-}
-
-hipError_t oi_launch_seed(hipStream_t st, unsigned long long* ring, uint32_t cap, const uint2* front, uint32_t n0,
-                          oi_trav_ctl* ctl, uint32_t reset_sticky, uint32_t n_wg)
-{
-    uint32_t g = (n0 + 255) / 256;
-    if (g == 0) g = 1;
-    hipLaunchKernelGGL(k_seed, dim3(g), dim3(256), 0, st, ring, oi_ilog2(cap), front, n0, ctl, reset_sticky, n_wg);
-    return hipGetLastError();
-}
-
-// resident workgroups per CU of the plain [0] and FP64-filter [1] traversal
-hipError_t oi_traverse_occupancy(int* blocks_per_cu)
-{
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu[0], k_traverse<false>, TB, 0);
-    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu[1], k_traverse<true>, TB, 0);
-    return e;
-}
-
-hipError_t oi_launch_traverse(hipStream_t st, uint32_t grid, const oi_mesh_desc* meshes, const oi_mesh_desc* host_meshes,
-                              uint32_t n_meshes,
-                              unsigned long long* ring, uint32_t ring_cap, oi_trav_ctl* ctl,
-                              uint4* hits, uint32_t hits_cap, uint32_t expand, uint32_t flags,
-                              unsigned long long timeout_ticks, uint32_t implicit_end,
-                              unsigned long long* trace)
-{
-    TravArgs a{};
-    a.meshes = meshes;
-    a.n_meshes = n_meshes;
-    for (uint32_t m = 0; m < n_meshes && m < INLINE_MESHES; m++) a.inl[m] = host_meshes[m];
-    a.ring = ring;
-    a.ring_shift = oi_ilog2(ring_cap);
-    a.ctl = ctl;
-    a.hits = hits;
-    a.hits_cap = hits_cap;
+    LevelArgs a{};
+    a.tab = make_tab(d_meshes, h_meshes, n_meshes);
+    a.front_in = front_in;
+    a.count_in = count_in;
+    a.front_out = front_out;
+    a.count_out = count_out;
+    a.cap_out = cap_out;
+    a.cands = cands;
+    a.cand_count = cand_count;
+    a.cand_cap = cand_cap;
+    a.error = error;
     a.expand = expand;
-    a.flags = flags;
-    a.timeout_ticks = timeout_ticks;
     a.implicit_end = implicit_end;
-    a.trace = trace;
-    if (flags & OI_DETECT_FP64_FALLBACK)
-        hipLaunchKernelGGL(k_traverse<true>, dim3(grid), dim3(TB), 0, st, a);
+    hipLaunchKernelGGL(k_bvtt_level, dim3(grid), dim3(TB), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t oi_launch_narrow(hipStream_t st, uint32_t grid, const oi_mesh_desc* d_meshes,
+                            const oi_mesh_desc* h_meshes, uint32_t n_meshes, const uint2* cands,
+                            const uint32_t* cand_count, uint32_t cand_cap, uint4* hits, uint32_t* n_hits,
+                            uint32_t hits_cap, oi_detect_ctl* ctl, int fp64)
+{
+    NarrowArgs a{};
+    a.tab = make_tab(d_meshes, h_meshes, n_meshes);
+    a.cands = cands;
+    a.cand_count = cand_count;
+    a.cand_cap = cand_cap;
+    a.hits = hits;
+    a.n_hits = n_hits;
+    a.hits_cap = hits_cap;
+    a.error = &ctl->error;
+    a.fp64 = &ctl->fp64;
+    a.adj = &ctl->adj;
+    if (fp64)
+        hipLaunchKernelGGL(k_narrow<true>, dim3(grid), dim3(TB), 0, st, a);
     else
-        hipLaunchKernelGGL(k_traverse<false>, dim3(grid), dim3(TB), 0, st, a);
+        hipLaunchKernelGGL(k_narrow<false>, dim3(grid), dim3(TB), 0, st, a);
     return hipGetLastError();
 }
 

@@ -80,60 +80,24 @@ struct oi_mesh_desc {
     uint32_t pad0, pad1, pad2;
 };
 
-// Traversal counters, reset by the seed kernel before each detect. The
-// counters every workgroup touches live on separate 128-byte lines.
-struct __attribute__((aligned(128))) oi_trav_ctl {
-    unsigned long long head;  // ring entries popped (reserved)
-    char pad0[120];
-    unsigned long long tail;  // ring entries pushed (reserved)
-    unsigned long long base;  // ring index where this detect started (the ring never wraps within one)
-    char pad1[112];
-    int pending;              // ring entries + workgroups holding BVTT entries
-    char pad2[124];
-    uint32_t idle;            // workgroups currently without work
-    char pad3[124];
-    uint32_t n_hits;          // hits found (may exceed capacity)
-    char pad4[124];
-    uint32_t error;           // OI_ERR_* bits
-    uint32_t sticky;          // OI_ERR_* bits of earlier asynchronous traversals not checked since
-    uint32_t exited;          // workgroups that left the loop
-    uint32_t pad5;
-    unsigned long long tests; // BVTT entries tested
-    unsigned long long cands; // leaf pairs sent to the narrow phase
-    unsigned long long fp64;  // SAT axis comparisons re-evaluated in double
-    unsigned long long adj;   // leaf pairs dropped as adjacent (self mode)
+// Detect counters, reset by the seed kernel before each detect.
+struct oi_detect_ctl {
+    uint32_t n_hits;           // hits found (may exceed capacity)
+    uint32_t cands;            // candidate leaf pairs (may exceed capacity)
+    uint32_t error;            // OI_ERR_* bits of this detect
+    uint32_t sticky;           // OI_ERR_* bits of earlier asynchronous detects not checked since
+    unsigned long long fp64;   // SAT axis comparisons re-evaluated in double
+    unsigned long long adj;    // leaf pairs dropped as adjacent (self mode)
 };
 
-// per-workgroup traversal trace (optional): OI_TRACE_WORDS uint64 per workgroup
-#define OI_TRACE_WORDS 16
-enum {
-    OI_TR_STEPS,       // BVTT steps taken
-    OI_TR_TESTS,       // entries tested
-    OI_TR_POPS,        // successful pops from the ring
-    OI_TR_POPPED,      // entries popped
-    OI_TR_MOVES,       // stack-bottom moves to the ring (donations and full stacks)
-    OI_TR_MOVED,       // entries moved to the ring
-    OI_TR_IDLE_LOOPS,  // iterations without work
-    OI_TR_NARROW,      // narrow-phase batches
-    OI_TR_T_STEP,      // wall-clock ticks in BVTT steps (test + expand)
-    OI_TR_T_IDLE,      // ticks idle
-    OI_TR_T_MOVE,      // ticks moving entries to the ring
-    OI_TR_T_NARROW,    // ticks in narrow-phase batches
-    OI_TR_T_REFILL,    // ticks popping from the ring
-    OI_TR_T_FIRST,     // ticks from kernel start to first work
-    OI_TR_T_EXIT,      // ticks from kernel start to exit
-    OI_TR_MAX_STACK    // deepest LDS stack
-};
-
-#define OI_ERR_QUEUE_OVERFLOW 1u
+#define OI_ERR_QUEUE_OVERFLOW 1u  // a BVTT front exceeded its buffer
 #define OI_ERR_TIMEOUT 2u
 #define OI_ERR_CORRUPT 4u
 #define OI_ERR_HITS_OVERFLOW 8u
+#define OI_ERR_CAND_OVERFLOW 16u
 
-// implicit index pair that marks a dropped BVTT entry (never dereferenced)
-#define OI_BAD_ENTRY 0xFFFFFFFFu
 // a scene's concatenated implicit index space must stay below this bound
-#define OI_IMPLICIT_LIMIT 0x3fffffffu
+#define OI_IMPLICIT_LIMIT 0xfffffffeu
 
 // detect flags (mirror include/oibvh.h)
 #define OI_DETECT_SELF 1u

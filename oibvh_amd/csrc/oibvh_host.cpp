@@ -26,15 +26,12 @@ hipError_t oi_launch_sort_pass(hipStream_t, const uint32_t*, const uint32_t*, ui
                                uint32_t, const uint32_t*, uint32_t*, uint32_t*, uint32_t*);
 hipError_t oi_launch_reduce(hipStream_t, int, const float*, const uint32_t*, const uint32_t*, uint32_t*, float*,
                             uint32_t, uint32_t, uint32_t);
-hipError_t oi_launch_fill64(hipStream_t, unsigned long long*, uint32_t, unsigned long long);
-hipError_t oi_launch_ring_reset(hipStream_t, unsigned long long*, uint32_t, oi_trav_ctl*);
-hipError_t oi_launch_seed(hipStream_t, unsigned long long*, uint32_t, const uint2*, uint32_t, oi_trav_ctl*,
-                          uint32_t, uint32_t);
-hipError_t oi_traverse_occupancy(int*);
-hipError_t oi_launch_traverse(hipStream_t, uint32_t, const oi_mesh_desc*, const oi_mesh_desc*, uint32_t,
-                              unsigned long long*, uint32_t,
-                              oi_trav_ctl*, uint4*, uint32_t, uint32_t, uint32_t, unsigned long long, uint32_t,
-                              unsigned long long*);
+hipError_t oi_launch_seed(hipStream_t, uint2*, const uint2*, uint32_t, uint32_t*, uint32_t, oi_detect_ctl*, uint32_t);
+hipError_t oi_launch_bvtt_level(hipStream_t, uint32_t, const oi_mesh_desc*, const oi_mesh_desc*, uint32_t,
+                                const uint2*, const uint32_t*, uint2*, uint32_t*, uint32_t, uint2*, uint32_t*,
+                                uint32_t, uint32_t*, uint32_t, uint32_t);
+hipError_t oi_launch_narrow(hipStream_t, uint32_t, const oi_mesh_desc*, const oi_mesh_desc*, uint32_t, const uint2*,
+                            const uint32_t*, uint32_t, uint4*, uint32_t*, uint32_t, oi_detect_ctl*, int);
 }
 
 // ---------------------------------------------------------------------------
@@ -44,14 +41,13 @@ struct oibvh_ctx {
     bool own_stream = false;
     std::string err;
     int cus = 0;
-    int trav_occ[2] = {1, 1};  // traversal workgroups per CU: plain, FP64 filter
     uint64_t clock_khz = 100000;
     uint32_t leaves_per_thread = 0;
     uint32_t trav_blocks = 0;
     uint32_t trav_expand = 0;
-    uint32_t trace = 0;
     uint32_t queue_cap = 1u << 22;
     uint32_t hits_cap = 1u << 20;
+    uint32_t cand_cap = 1u << 20;
     int live = 0;                // trees and scenes of this context not yet destroyed
     bool destroy_pending = false; // oibvh_ctx_destroy called while objects were live
 };
@@ -97,23 +93,25 @@ struct oibvh_scene {
     std::vector<oibvh_tree*> trees;
     oi_mesh_desc* d_desc = nullptr;
     uint32_t desc_cap = 0;
-    uint2* d_front = nullptr;
-    uint32_t front_cap = 0;
-    uint32_t n_front = 0;
-    std::vector<uint2> h_front;
     std::vector<oi_mesh_desc> h_desc;  // host copies stay alive until the next detect
     uint32_t implicit_end = 0;
-    uint32_t last_grid = 0;
+    uint2* d_init = nullptr;           // initial BVTT front
+    uint32_t init_cap = 0;
+    uint32_t n_front = 0;
+    std::vector<uint2> h_front;
     uint32_t front_key[3] = {~0u, ~0u, ~0u}; // entry, flags&SELF, #trees
-    unsigned long long* ring = nullptr;
-    uint32_t ring_cap = 0;
-    bool ring_dirty = true;
-    oi_trav_ctl* ctl = nullptr;
-    oi_trav_ctl* h_ctl = nullptr; // pinned
+    uint2* front[2] = {nullptr, nullptr};     // ping-pong BVTT fronts
+    uint32_t front_cap = 0;
+    uint32_t* counts = nullptr;        // front length per level
+    uint32_t counts_cap = 0;
+    uint32_t n_levels = 0;
+    uint2* cands = nullptr;
+    uint32_t cand_cap = 0;
     uint4* hits = nullptr;
     uint32_t hits_cap = 0;
-    unsigned long long* trace = nullptr;  // OI_TRACE_WORDS per traversal workgroup
-    uint32_t trace_wg = 0;
+    oi_detect_ctl* ctl = nullptr;
+    oi_detect_ctl* h_ctl = nullptr;    // pinned
+    std::vector<uint32_t> h_counts;
     // last detect
     uint32_t last_entry = 0, last_expand = 1, last_flags = 0;
     bool unchecked = false;
@@ -190,11 +188,6 @@ extern "C" oibvh_status oibvh_ctx_create(int device, void* stream, oibvh_ctx** o
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
         c->clock_khz = (uint64_t)khz;
-    int occ[2] = {0, 0};
-    if (oi_traverse_occupancy(occ) == hipSuccess) {
-        if (occ[0] > 0) c->trav_occ[0] = occ[0];
-        if (occ[1] > 0) c->trav_occ[1] = occ[1];
-    }
     if (stream) {
         c->stream = (hipStream_t)stream;
     } else {
@@ -235,15 +228,15 @@ extern "C" oibvh_status oibvh_ctx_set_tuning(oibvh_ctx* ctx, const oibvh_tuning*
     const uint32_t lpt = t->reduce_leaves_per_thread;
     if (lpt != 0 && lpt != 1 && lpt != 2 && lpt != 4 && lpt != 8)
         return set_err(ctx, OIBVH_ERR_INVALID, "reduce_leaves_per_thread must be 0, 1, 2, 4 or 8");
-    if (t->queue_capacity && (t->queue_capacity & (t->queue_capacity - 1)))
-        return set_err(ctx, OIBVH_ERR_INVALID, "queue_capacity must be a power of two");
+    if (t->front_capacity && (t->front_capacity & (t->front_capacity - 1)))
+        return set_err(ctx, OIBVH_ERR_INVALID, "front_capacity must be a power of two");
     if (t->traverse_expand > 3) return set_err(ctx, OIBVH_ERR_INVALID, "traverse_expand must be 0..3");
     ctx->leaves_per_thread = lpt;
     ctx->trav_blocks = t->traverse_blocks;
     ctx->trav_expand = t->traverse_expand;
-    ctx->queue_cap = t->queue_capacity ? t->queue_capacity : (1u << 22);
+    ctx->queue_cap = t->front_capacity ? t->front_capacity : (1u << 22);
+    ctx->cand_cap = t->candidate_capacity ? t->candidate_capacity : (1u << 20);
     ctx->hits_cap = t->hits_capacity ? t->hits_capacity : (1u << 20);
-    ctx->trace = t->trace ? 1u : 0u;
     return OIBVH_OK;
 }
 
@@ -253,9 +246,9 @@ extern "C" oibvh_status oibvh_ctx_get_tuning(const oibvh_ctx* ctx, oibvh_tuning*
     t->reduce_leaves_per_thread = ctx->leaves_per_thread;
     t->traverse_blocks = ctx->trav_blocks;
     t->traverse_expand = ctx->trav_expand;
-    t->queue_capacity = ctx->queue_cap;
+    t->front_capacity = ctx->queue_cap;
+    t->candidate_capacity = ctx->cand_cap;
     t->hits_capacity = ctx->hits_cap;
-    t->trace = ctx->trace;
     return OIBVH_OK;
 }
 
@@ -567,13 +560,14 @@ extern "C" oibvh_status oibvh_scene_create(oibvh_ctx* ctx, oibvh_scene** out)
     if (!s) return OIBVH_ERR_INVALID;
     s->ctx = ctx;
     hipError_t e = dalloc(&s->ctl, 1);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_ctl, sizeof(oi_trav_ctl), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMemsetAsync(s->ctl, 0, sizeof(oi_detect_ctl), ctx->stream);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_ctl, sizeof(oi_detect_ctl), hipHostMallocDefault);
     if (e != hipSuccess) {
         dfree(s->ctl);
         delete s;
         return set_err(ctx, OIBVH_ERR_HIP, "scene alloc: %s", hipGetErrorString(e));
     }
-    std::memset(s->h_ctl, 0, sizeof(oi_trav_ctl));
+    std::memset(s->h_ctl, 0, sizeof(oi_detect_ctl));
     ctx->live++;
     *out = s;
     return OIBVH_OK;
@@ -585,11 +579,13 @@ extern "C" void oibvh_scene_destroy(oibvh_scene* s)
     (void)hipSetDevice(s->ctx->device);
     (void)hipStreamSynchronize(s->ctx->stream);
     dfree(s->d_desc);
-    dfree(s->d_front);
-    dfree(s->ring);
-    dfree(s->ctl);
+    dfree(s->d_init);
+    dfree(s->front[0]);
+    dfree(s->front[1]);
+    dfree(s->counts);
+    dfree(s->cands);
     dfree(s->hits);
-    dfree(s->trace);
+    dfree(s->ctl);
     if (s->h_ctl) (void)hipHostFree(s->h_ctl);
     oibvh_ctx* c = s->ctx;
     delete s;
@@ -640,34 +636,21 @@ static void make_front(oibvh_scene* s, uint32_t entry, bool self)
     }
 }
 
-static oibvh_status ensure_ring(oibvh_scene* s, uint32_t cap)
+template <class T>
+static oibvh_status grow(oibvh_ctx* c, T*& p, uint32_t& cap, uint64_t need)
 {
-    oibvh_ctx* c = s->ctx;
-    if (s->ring_cap < cap) {
-        dfree(s->ring);
-        s->ring_cap = 0;
-        HIPCHK(c, dalloc(&s->ring, cap));
-        s->ring_cap = cap;
-        s->ring_dirty = true;
-    }
-    if (s->ring_dirty) {
-        // every slot to its first lap, head = tail = 0
-        HIPCHK(c, oi_launch_ring_reset(c->stream, s->ring, s->ring_cap, s->ctl));
-        s->ring_dirty = false;
-    }
+    if (cap >= need) return OIBVH_OK;
+    if (need > 0x7fffffffull) return set_err(c, OIBVH_ERR_OVERFLOW, "detect buffer would exceed 2^31 entries");
+    dfree(p);
+    cap = 0;
+    HIPCHK(c, dalloc(&p, need));
+    cap = (uint32_t)need;
     return OIBVH_OK;
 }
 
-static oibvh_status ensure_hits(oibvh_scene* s, uint32_t cap)
+static uint32_t grid_for(const oibvh_ctx* c)
 {
-    oibvh_ctx* c = s->ctx;
-    if (s->hits_cap < cap) {
-        dfree(s->hits);
-        s->hits_cap = 0;
-        HIPCHK(c, dalloc(&s->hits, cap));
-        s->hits_cap = cap;
-    }
-    return OIBVH_OK;
+    return c->trav_blocks ? c->trav_blocks : (uint32_t)std::max(1, c->cus) * 4u;
 }
 
 static oibvh_status enqueue_detect(oibvh_scene* s, bool reset_sticky)
@@ -679,6 +662,8 @@ static oibvh_status enqueue_detect(oibvh_scene* s, bool reset_sticky)
     std::vector<oi_mesh_desc>& desc = s->h_desc;
     desc.resize(n);
     uint64_t acc = 0;
+    uint32_t depth = 0;
+    const uint32_t expand = c->trav_expand ? c->trav_expand : std::min(3u, std::max(1u, s->last_expand));
     for (uint32_t m = 0; m < n; m++) {
         const oibvh_tree* t = s->trees[m];
         oi_mesh_desc& d = desc[m];
@@ -693,6 +678,7 @@ static oibvh_status enqueue_detect(oibvh_scene* s, bool reset_sticky)
         d.N = t->N;
         d.ibase = (uint32_t)acc;
         acc += (2ull << t->L) - 1ull;
+        depth = std::max(depth, t->L - std::min(t->L, s->last_entry));
     }
     if (acc > OI_IMPLICIT_LIMIT)
         return set_err(c, OIBVH_ERR_INVALID, "scene too large: %llu implicit BVTT indices (limit %u)",
@@ -712,45 +698,41 @@ static oibvh_status enqueue_detect(oibvh_scene* s, bool reset_sticky)
     if (std::memcmp(key, s->front_key, sizeof key) != 0) {
         make_front(s, s->last_entry, self);
         s->n_front = (uint32_t)s->h_front.size();
-        if (s->front_cap < s->n_front) {
-            dfree(s->d_front);
-            s->front_cap = 0;
-            HIPCHK(c, dalloc(&s->d_front, s->n_front));
-            s->front_cap = s->n_front;
-        }
+        oibvh_status r = grow(c, s->d_init, s->init_cap, std::max(1u, s->n_front));
+        if (r != OIBVH_OK) return r;
         if (s->n_front)
-            HIPCHK(c, hipMemcpyAsync(s->d_front, s->h_front.data(), s->n_front * sizeof(uint2), hipMemcpyHostToDevice,
+            HIPCHK(c, hipMemcpyAsync(s->d_init, s->h_front.data(), s->n_front * sizeof(uint2), hipMemcpyHostToDevice,
                                      st));
         std::memcpy(s->front_key, key, sizeof key);
     }
-    uint32_t ring_need = std::max(std::max(c->queue_cap, s->ring_cap), oi_np2(std::max(1u, s->n_front) * 2u));
-    oibvh_status r = ensure_ring(s, ring_need);
-    if (r != OIBVH_OK) return r;
-    if (!s->hits) {
-        r = ensure_hits(s, c->hits_cap);
-        if (r != OIBVH_OK) return r;
+    // every BVTT step takes each non-leaf side `expand` levels down; one more
+    // step tests the leaf pairs
+    s->n_levels = (depth + expand - 1) / expand + 1;
+    oibvh_status r;
+    const uint32_t fcap = std::max(c->queue_cap, oi_np2(std::max(1u, s->n_front)));
+    if (s->front_cap < fcap) {
+        dfree(s->front[0]);
+        dfree(s->front[1]);
+        s->front_cap = 0;
+        HIPCHK(c, dalloc(&s->front[0], fcap));
+        HIPCHK(c, dalloc(&s->front[1], fcap));
+        s->front_cap = fcap;
     }
-    const int variant = (s->last_flags & OIBVH_DETECT_FP64_FALLBACK) ? 1 : 0;
-    uint32_t grid = c->trav_blocks ? c->trav_blocks
-                                   : (uint32_t)std::max(1, c->cus) * (uint32_t)c->trav_occ[variant];
-    HIPCHK(c, oi_launch_seed(st, s->ring, s->ring_cap, s->d_front, s->n_front, s->ctl, reset_sticky ? 1u : 0u, grid));
-    unsigned long long* trace = nullptr;
-    if (c->trace) {
-        if (s->trace_wg < grid) {
-            dfree(s->trace);
-            s->trace_wg = 0;
-            HIPCHK(c, dalloc(&s->trace, (size_t)grid * OI_TRACE_WORDS));
-            s->trace_wg = grid;
-        }
-        trace = s->trace;
+    if ((r = grow(c, s->counts, s->counts_cap, s->n_levels + 1)) != OIBVH_OK) return r;
+    if ((r = grow(c, s->cands, s->cand_cap, c->cand_cap)) != OIBVH_OK) return r;
+    if ((r = grow(c, s->hits, s->hits_cap, c->hits_cap)) != OIBVH_OK) return r;
+    HIPCHK(c, oi_launch_seed(st, s->front[0], s->d_init, s->n_front, s->counts, s->n_levels + 1, s->ctl,
+                             reset_sticky ? 1u : 0u));
+    const uint32_t grid = grid_for(c);
+    const oi_mesh_desc* dd = n > 4 ? s->d_desc : nullptr;
+    for (uint32_t k = 0; k < s->n_levels; k++) {
+        HIPCHK(c, oi_launch_bvtt_level(st, grid, dd, desc.data(), n, s->front[k & 1], s->counts + k,
+                                       s->front[(k + 1) & 1], s->counts + k + 1, s->front_cap, s->cands,
+                                       &s->ctl->cands, s->cand_cap, &s->ctl->error, expand, s->implicit_end));
     }
-    s->last_grid = grid;
-    const unsigned long long timeout = 30ull * 1000ull * c->clock_khz;  // 30 s
-    HIPCHK(c, oi_launch_traverse(st, grid, n > 4 ? s->d_desc : nullptr, desc.data(), n, s->ring, s->ring_cap,
-                                 s->ctl, s->hits, s->hits_cap,
-                                 c->trav_expand ? c->trav_expand : std::min(3u, std::max(1u, s->last_expand)),
-                                 s->last_flags & 3u, timeout,
-                                 s->implicit_end, trace));
+    HIPCHK(c, oi_launch_narrow(st, grid, dd, desc.data(), n, s->cands, &s->ctl->cands, s->cand_cap, s->hits,
+                               &s->ctl->n_hits, s->hits_cap, s->ctl,
+                               (s->last_flags & OIBVH_DETECT_FP64_FALLBACK) ? 1 : 0));
     s->unchecked = true;
     s->have_result = true;
     return OIBVH_OK;
@@ -763,35 +745,52 @@ static oibvh_status check_detect(oibvh_scene* s)
     if (!s->unchecked) return OIBVH_OK;
     uint32_t retries = 0, earlier = 0;
     for (;;) {
-        HIPCHK(c, hipMemcpyAsync(s->h_ctl, s->ctl, sizeof(oi_trav_ctl), hipMemcpyDeviceToHost, c->stream));
+        s->h_counts.resize(s->n_levels + 1);
+        HIPCHK(c, hipMemcpyAsync(s->h_ctl, s->ctl, sizeof(oi_detect_ctl), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(s->h_counts.data(), s->counts, (s->n_levels + 1) * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        const oi_trav_ctl& k = *s->h_ctl;
+        const oi_detect_ctl& k = *s->h_ctl;
         earlier |= k.sticky;  // errors of unchecked asynchronous detects before this one
-        if (k.error & (OI_ERR_TIMEOUT | OI_ERR_CORRUPT)) {
-            s->ring_dirty = true;
+        if (k.error & OI_ERR_CORRUPT) {
             s->unchecked = false;
             s->have_result = false;
-            return set_err(c, OIBVH_ERR_TIMEOUT, "traversal timed out (error bits 0x%x)", k.error);
+            return set_err(c, OIBVH_ERR_HIP, "detect: corrupt BVTT entry (error bits 0x%x)", k.error);
         }
         bool rerun = false;
+        oibvh_status r = OIBVH_OK;
+        uint32_t maxfront = 0;
+        for (uint32_t v : s->h_counts) maxfront = std::max(maxfront, v);
         if (k.error & OI_ERR_QUEUE_OVERFLOW) {
-            s->ring_dirty = true;
-            if (s->ring_cap >= (1u << 28)) {
+            const uint64_t need = oi_np2(std::max(maxfront, s->front_cap * 4u));
+            if (need > (1u << 28)) {
                 s->unchecked = false;
                 s->have_result = false;
-                return set_err(c, OIBVH_ERR_OVERFLOW, "work queue overflow at %u entries", s->ring_cap);
+                return set_err(c, OIBVH_ERR_OVERFLOW, "BVTT front exceeds %u entries", 1u << 28);
             }
-            oibvh_status r = ensure_ring(s, std::min(1u << 28, s->ring_cap * 8u));
-            if (r != OIBVH_OK) return r;
+            dfree(s->front[0]);
+            dfree(s->front[1]);
+            s->front_cap = 0;
+            HIPCHK(c, dalloc(&s->front[0], need));
+            HIPCHK(c, dalloc(&s->front[1], need));
+            s->front_cap = (uint32_t)need;
             rerun = true;
-        } else if (k.n_hits > s->hits_cap || (k.error & OI_ERR_HITS_OVERFLOW)) {
-            oibvh_status r = ensure_hits(s, oi_np2(k.n_hits + (k.n_hits >> 3)));
-            if (r != OIBVH_OK) return r;
+        }
+        if ((k.error & OI_ERR_CAND_OVERFLOW) || k.cands > s->cand_cap) {
+            if ((r = grow(c, s->cands, s->cand_cap, std::max<uint64_t>(k.cands, 2ull * s->cand_cap))) != OIBVH_OK)
+                return r;
+            rerun = true;
+        }
+        if ((k.error & OI_ERR_HITS_OVERFLOW) || k.n_hits > s->hits_cap) {
+            const uint64_t need = std::max<uint64_t>((uint64_t)k.n_hits + (k.n_hits >> 3), 2ull * s->hits_cap);
+            if ((r = grow(c, s->hits, s->hits_cap, need)) != OIBVH_OK) return r;
             rerun = true;
         }
         if (!rerun) {
+            uint64_t tests = 0;
+            for (uint32_t l = 0; l < s->n_levels; l++) tests += s->h_counts[l];
             s->n_pairs = k.n_hits;
-            s->stats.bvtt_tests = k.tests;
+            s->stats.bvtt_tests = tests;
             s->stats.candidates = k.cands;
             s->stats.fp64_rechecks = k.fp64;
             s->stats.adjacent = k.adj;
@@ -803,8 +802,7 @@ static oibvh_status check_detect(oibvh_scene* s)
         }
         retries++;
         if (retries > 16) return set_err(c, OIBVH_ERR_OVERFLOW, "detect: too many retries");
-        oibvh_status r = enqueue_detect(s, true);
-        if (r != OIBVH_OK) return r;
+        if ((r = enqueue_detect(s, true)) != OIBVH_OK) return r;
     }
 }
 
@@ -893,17 +891,14 @@ extern "C" oibvh_status oibvh_scene_get_pairs(oibvh_scene* s, uint32_t* out, uin
     return OIBVH_OK;
 }
 
-extern "C" oibvh_status oibvh_scene_get_trace(oibvh_scene* s, uint64_t* out, uint64_t cap, uint32_t* n_wg)
+extern "C" oibvh_status oibvh_scene_get_level_counts(oibvh_scene* s, uint32_t* out, uint64_t cap, uint32_t* n)
 {
-    if (!s || !n_wg) return OIBVH_ERR_INVALID;
-    oibvh_ctx* c = s->ctx;
+    if (!s || !n) return OIBVH_ERR_INVALID;
     oibvh_status r = check_detect(s);
     if (r != OIBVH_OK) return r;
-    *n_wg = s->trace ? s->last_grid : 0;
-    if (!s->trace) return OIBVH_OK;
-    const uint64_t words = (uint64_t)s->last_grid * OI_TRACE_WORDS;
-    if (!out || cap < words) return set_err(c, OIBVH_ERR_CAPACITY, "need %llu words", (unsigned long long)words);
-    HIPCHK(c, hipMemcpyAsync(out, s->trace, words * 8, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *n = s->have_result ? s->n_levels : 0;
+    if (*n == 0) return OIBVH_OK;
+    if (!out || cap < *n) return set_err(s->ctx, OIBVH_ERR_CAPACITY, "need %u entries", *n);
+    std::memcpy(out, s->h_counts.data(), *n * sizeof(uint32_t));
     return OIBVH_OK;
 }

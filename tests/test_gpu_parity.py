@@ -266,7 +266,7 @@ def test_detect_overflow_retry(api, ctx, orc):
     meshes = M.interpenetrating_icospheres(subdiv=5, seed=31, offset=1.2)
     want = orc.detect([orc.Tree(x, f) for x, f in meshes])
     ctx2 = api.Context(0)
-    ctx2.set_tuning(queue_capacity=256, hits_capacity=16)
+    ctx2.set_tuning(front_capacity=256, hits_capacity=16, candidate_capacity=64)
     p, st, _ = gpu_pairs(api, ctx2, meshes, 0, 3)
     assert np.array_equal(p, want)
     assert st["retries"] > 0
