@@ -400,7 +400,10 @@ __global__ void __launch_bounds__(TB) k_bvtt_level(LevelArgs A)
     __shared__ uint32_t s_ibase[MESH_LDS];
     __shared__ uint32_t s_wsum[4];
     __shared__ uint32_t s_obase, s_cbase;
-    const uint32_t n_in = *A.count_in;
+    // after an overflow the count keeps growing past the buffer (writes were
+    // dropped and the detect will be re-run): read only what exists
+    uint32_t n_in = *A.count_in;
+    if (n_in > A.cap_out) n_in = A.cap_out;  // both fronts have the same capacity
     if (blockIdx.x * TB >= n_in) return;
     load_mesh_tab(A.tab, s_desc, s_ibase);
     const uint32_t tid = threadIdx.x;
