@@ -40,8 +40,10 @@ def parse():
     p.add_argument("--steps", type=int, default=200)
     p.add_argument("--warmup", type=int, default=10)
     p.add_argument("--workload", default="deforming-uv-pair", choices=["deforming-uv-pair", "icosphere-pair"])
-    p.add_argument("--entry-level", type=int, default=4)
-    p.add_argument("--expand-levels", type=int, default=3)
+    p.add_argument("--entry-level", type=int, default=None,
+                   help="BVTT entry level; default: picked by the autotuner during warm-up")
+    p.add_argument("--expand-levels", type=int, default=None,
+                   help="levels descended per BVTT step; default: picked by the autotuner")
     p.add_argument("--rebuild-every", type=int, default=50)
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -166,13 +168,36 @@ def main():
                     ev_refit.stop()
         if timed:
             ev_detect.start()
-        scene.detectCollision(args.entry_level, args.expand_levels, asynchronous=True)
+        scene.detectCollision(shape[0], shape[1], asynchronous=True)
         if timed:
             ev_detect.stop()
 
-    # warm-up: synchronous detects size the device buffers
+    shape = [args.entry_level if args.entry_level is not None else 4,
+             args.expand_levels if args.expand_levels is not None else 1]
+    # warm-up: synchronous detects size the device buffers; then the autotuner
+    # picks entry level, levels per BVTT step, workgroups per launch and the
+    # reduce subtree size on the live scene (none of them changes results)
     for fi in range(W):
         step(fi, False)
+        scene.getIntTriPairCount()
+    tuned = None
+    if args.entry_level is None or args.expand_levels is None:
+        from oibvh_amd import autotune
+
+        def tune_frame(i):
+            f = (W + i) % len(frames[0])
+            for m, t in enumerate(trees):
+                t.set_positions_device(bufs[m].at(f * V * 12))
+                t.refit()
+
+        tuned = autotune.tune(ctx, scene, tune_frame)
+        if args.entry_level is None:
+            args.entry_level = tuned["entry_level"]
+        if args.expand_levels is None:
+            args.expand_levels = tuned["expand_levels"]
+        ctx.set_tuning(traverse_expand=args.expand_levels)
+        shape[:] = [args.entry_level, args.expand_levels]
+        scene.detectCollision(args.entry_level, args.expand_levels)
         scene.getIntTriPairCount()
     timer = api.Timer(ctx)
     ctx.synchronize()
@@ -198,9 +223,9 @@ def main():
     detect_bytes = sum(24 * n + 12 * V + 12 * F for F, n in zip(Fm, N)) + 16 * H_pairs
     share = {"refit": refit_ms * ev_refit.i, "detect": detect_ms * ev_detect.i, "build": build_ms * ev_build.i}
     kernels = {
-        "refit": {"kernel": "k_reduce", "avg_ms": refit_ms, "launches": ev_refit.i,
+        "refit": {"kernel": "k_reduce (+ k_reduce_apex above 2^11 leaves per workgroup)", "avg_ms": refit_ms, "launches": ev_refit.i,
                   "alg_bytes": refit_bytes, "gbps": refit_bytes / (refit_ms * 1e-3) / 1e9 if refit_ms else None},
-        "detect": {"kernel": "k_seed + k_traverse", "avg_ms": detect_ms, "launches": ev_detect.i,
+        "detect": {"kernel": "k_seed, k_bvtt_level x levels, k_narrow", "avg_ms": detect_ms, "launches": ev_detect.i,
                    "alg_bytes": detect_bytes,
                    "gbps": detect_bytes / (detect_ms * 1e-3) / 1e9 if detect_ms else None},
         "build": {"kernels": "k_mesh_box, k_morton, 4 x k_sort_pass, k_reduce", "avg_ms": build_ms,
@@ -269,7 +294,10 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": workload, "form": "plain", "triangles_per_frame": tri_per_frame,
-                   "entry_level": args.entry_level, "expand_levels": args.expand_levels},
+                   "entry_level": args.entry_level, "expand_levels": args.expand_levels,
+                   "tuning": ctx.get_tuning()},
+        "autotune": tuned,
+        "bvtt_levels_last_frame": scene.level_counts().tolist(),
         "mtri_per_s": tri_per_frame * fps / 1e6,
         "pair_diffs": diffs,
         "pairs_last_frame": H_pairs,

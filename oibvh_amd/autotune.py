@@ -1,0 +1,57 @@
+"""Autotuner for the knobs that change speed but never results.
+
+For a live scene it times short runs of refit + detect over a small grid of
+settings -- the reduce subtree size (leaves per thread), the BVTT entry
+level, the levels descended per BVTT step and the workgroups per launch --
+with HIP events on the context's stream, and keeps the fastest. The caller
+provides `frame(i)`, which refits the scene's trees for trial i (it may move
+vertices); the tuner adds the detect.
+"""
+from __future__ import annotations
+
+import itertools
+
+from . import api
+
+
+def _time(ctx: api.Context, scene: api.Scene, frame, entry: int, expand: int, reps: int) -> float:
+    timer = api.Timer(ctx)
+    frame(0)
+    scene.detectCollision(entry, expand)
+    scene.getIntTriPairCount()  # sizes buffers, checks errors
+    ctx.synchronize()
+    timer.start()
+    for i in range(reps):
+        frame(i + 1)
+        scene.detectCollision(entry, expand, asynchronous=True)
+    timer.stop()
+    ms = timer.ms() / reps
+    scene.getIntTriPairCount()
+    return ms
+
+
+def tune(ctx: api.Context, scene: api.Scene, frame, *, entries=(4, 6, 8), expands=(1, 2, 3),
+         blocks=(0, 2048), leaves=(0, 1, 2, 4), reps: int = 5) -> dict:
+    """Return {'entry_level', 'expand_levels', 'traverse_blocks',
+    'reduce_leaves_per_thread', 'ms'} of the fastest setting and apply the
+    tuning knobs to ctx."""
+    best = None
+    # reduce subtree size first (refit only depends on it), then traversal shape
+    base = ctx.get_tuning()
+    lpt_best, lpt_ms = 0, None
+    for lpt in leaves:
+        ctx.set_tuning(reduce_leaves_per_thread=lpt)
+        ms = _time(ctx, scene, frame, entries[0], expands[0], reps)
+        if lpt_ms is None or ms < lpt_ms:
+            lpt_best, lpt_ms = lpt, ms
+    ctx.set_tuning(reduce_leaves_per_thread=lpt_best)
+    for entry, expand, nb in itertools.product(entries, expands, blocks):
+        ctx.set_tuning(traverse_blocks=nb, traverse_expand=expand)
+        ms = _time(ctx, scene, frame, entry, expand, reps)
+        if best is None or ms < best["ms"]:
+            best = {"entry_level": entry, "expand_levels": expand, "traverse_blocks": nb,
+                    "reduce_leaves_per_thread": lpt_best, "ms": ms}
+    ctx.set_tuning(**{**{k: base[k] for k in base},
+                      "reduce_leaves_per_thread": best["reduce_leaves_per_thread"],
+                      "traverse_blocks": best["traverse_blocks"], "traverse_expand": best["expand_levels"]})
+    return best
