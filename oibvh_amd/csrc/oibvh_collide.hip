@@ -3,14 +3,17 @@
 // The reference relaunches traversal_kernel once per BVTT level with a host
 // round trip for the front size (scene.cu:272-311, collide.cu:76-246) and then
 // runs triangle_intersect_kernel over the candidate list (collide.cu:248-300).
-// Here the levels are launched back to back with no host round trip:
-//   k_bvtt_level  one BVTT level, grid-stride over the current front (its
-//                 length is read from device memory): AABB overlap test
-//                 (collide.cu:12-17); leaf x leaf -> candidate list, otherwise
-//                 both sides descend `expand` levels (a leaf stands for itself,
-//                 collide.cu:165-176) and the children go to the next front.
-//                 Children and candidates are compacted with a block prefix sum
-//                 and one device-scope fetch_add per block;
+// Here the steps are launched back to back with no host round trip:
+//   k_seed        zeroes the level counts and detect counters;
+//   k_bvtt_step   one BVTT step, grid-stride over the current front (its
+//                 length is read from device memory): both sides of an
+//                 overlapping pair descend up to E levels (a leaf stands for
+//                 itself, collide.cu:165-176) and every pair of the two
+//                 descendant runs is tested for AABB overlap (collide.cu:
+//                 12-17); overlapping leaf pairs go to the candidate list,
+//                 other overlapping pairs to the next front. Outputs are
+//                 compacted with a block prefix sum and one device-scope
+//                 fetch_add per block;
 //   k_narrow      17-axis SAT (utils.cpp:71-169 / cuda_intersect_tritri.h:
 //                 350-434) over the candidate list, hits appended as
 //                 (meshA, meshB, origTriA, origTriB) in original face ids.
@@ -363,26 +366,27 @@ __device__ __forceinline__ Side expand_side(uint32_t i, uint32_t lvl, uint32_t L
     return Side{st, c < (1u << d) ? c : (1u << d)};
 }
 
-// k-th unordered pair (x <= y) of n items, row-major: (0,0),(0,1)..(0,n-1),(1,1)..
-__device__ __forceinline__ void tri_pair(uint32_t k, uint32_t n, uint32_t& x, uint32_t& y)
-{
-    x = 0;
-    uint32_t row = n;
-    while (k >= row) {
-        k -= row;
-        x++;
-        row--;
-    }
-    y = x + k;
-}
-
 // ---------------------------------------------------------------------------
-// k_bvtt_level: one level of the BVTT walk
+// k_bvtt_step: one step of the BVTT walk, testing on expansion
+//
+// A front entry is a node pair whose boxes are known to overlap (or, in self
+// mode, a node paired with itself). A step takes both sides up to E levels
+// down (a leaf stands for itself; collide.cu:165-176) and tests every pair of
+// the two descendant runs, which are contiguous in the real node order, so
+// each side is one coalesced run of at most 2^E boxes. Overlapping pairs go
+// to the next front, or to the candidate list when both sides are leaves.
+// Only overlapping pairs are ever written, and parents contain their children
+// exactly (compare-select merges), so the candidate set equals that of a walk
+// which tests every level (simpleCollide.cpp:88-143) for finite coordinates.
+// The first step also tests the entry pairs themselves (the initial front is
+// every pair of entry-level nodes, scene.cu:192-223).
 // ---------------------------------------------------------------------------
-struct LevelArgs {
+struct StepArgs {
     MeshTab tab;
     const uint2* front_in;
-    const uint32_t* count_in;   // length of front_in (written by the previous level / the seed)
+    const uint32_t* count_in;   // length of front_in; null: n_in
+    uint32_t n_in;
+    uint32_t cap_in;            // entries front_in can hold
     uint2* front_out;
     uint32_t* count_out;        // zeroed by the seed
     uint32_t cap_out;
@@ -390,33 +394,44 @@ struct LevelArgs {
     uint32_t* cand_count;
     uint32_t cand_cap;
     uint32_t* error;
-    uint32_t expand;
+    unsigned long long* tests;  // box pair tests (statistics)
     uint32_t implicit_end;
 };
 
-__global__ void __launch_bounds__(TB) k_bvtt_level(LevelArgs A)
+template <int E>
+__device__ __forceinline__ void load_run(const float* nodes, uint32_t r0, uint32_t n, float (&b)[1 << E][6])
 {
+#pragma unroll
+    for (int k = 0; k < (1 << E); k++)
+        if ((uint32_t)k < n) load_box(nodes, r0 + k, b[k]);
+}
+
+template <int E, bool FIRST>
+__global__ void __launch_bounds__(TB) k_bvtt_step(StepArgs A)
+{
+    constexpr int K = 1 << E;
     __shared__ oi_mesh_desc s_desc[MESH_LDS];
     __shared__ uint32_t s_ibase[MESH_LDS];
     __shared__ uint32_t s_wsum[4];
     __shared__ uint32_t s_obase, s_cbase;
     // after an overflow the count keeps growing past the buffer (writes were
     // dropped and the detect will be re-run): read only what exists
-    uint32_t n_in = *A.count_in;
-    if (n_in > A.cap_out) n_in = A.cap_out;  // both fronts have the same capacity
+    uint32_t n_in = A.count_in ? *A.count_in : A.n_in;
+    if (n_in > A.cap_in) n_in = A.cap_in;
     if (blockIdx.x * TB >= n_in) return;
     load_mesh_tab(A.tab, s_desc, s_ibase);
     const uint32_t tid = threadIdx.x;
+    uint32_t n_tests = 0;
     for (uint32_t base = blockIdx.x * TB; base < n_in; base += gridDim.x * TB) {
         const uint32_t i = base + tid;
         bool have = i < n_in;
-        uint2 e = have ? A.front_in[i] : make_uint2(0, 0);
+        const uint2 e = have ? A.front_in[i] : make_uint2(0, 0);
         if (have && (e.x >= A.implicit_end || e.y >= A.implicit_end)) {
             or_u32(A.error, OI_ERR_CORRUPT);  // never dereference a bad entry
             have = false;
         }
-        uint32_t nchild = 0;
-        bool cand = false, selfnode = false;
+        u64 mask = 0;          // bit (x << E) + y: pair (start_a + x, start_b + y) kept
+        bool leafout = false;  // kept pairs are leaf pairs (candidates)
         Side sa = {0, 0}, sb = {0, 0};
         uint32_t baseA = 0, baseB = 0;
         if (have) {
@@ -427,55 +442,76 @@ __global__ void __launch_bounds__(TB) k_bvtt_level(LevelArgs A)
             baseB = rb.ibase;
             const uint32_t ia = e.x - ra.ibase, ib = e.y - rb.ibase;
             const uint32_t la = oi_ilog2(ia + 1u), lb = oi_ilog2(ib + 1u);
-            selfnode = (e.x == e.y);
-            bool ov = true;
-            if (!selfnode) {
+            const bool selfnode = (e.x == e.y);
+            const bool leafleaf = (la == ra.L && lb == rb.L);
+            bool go = !(selfnode && leafleaf);  // a triangle against itself
+            if (FIRST && go && !selfnode && !leafleaf) {
                 float ba[6], bb[6];
                 load_box(ra.nodes, ia - oi_virt_above(la, ra.L, ra.vl), ba);
                 load_box(rb.nodes, ib - oi_virt_above(lb, rb.L, rb.vl), bb);
-                ov = overlap6(ba, bb);
+                go = overlap6(ba, bb);
+                n_tests++;
             }
-            if (ov) {
-                if (la == ra.L && lb == rb.L) {
-                    cand = !selfnode;
-                } else {
-                    sa = expand_side(ia, la, ra.L, ra.vl, A.expand);
-                    sb = expand_side(ib, lb, rb.L, rb.vl, A.expand);
-                    nchild = selfnode ? sa.count * (sa.count + 1u) / 2u : sa.count * sb.count;
+            if (go) {
+                sa = expand_side(ia, la, ra.L, ra.vl, E);
+                sb = selfnode ? sa : expand_side(ib, lb, rb.L, rb.vl, E);
+                const uint32_t ca = oi_ilog2(sa.start + 1u), cb = oi_ilog2(sb.start + 1u);
+                leafout = (ca == ra.L && cb == rb.L);
+                float bxa[K][6], bxb[K][6];
+                load_run<E>(ra.nodes, sa.start - oi_virt_above(ca, ra.L, ra.vl), sa.count, bxa);
+                if (!selfnode) load_run<E>(rb.nodes, sb.start - oi_virt_above(cb, rb.L, rb.vl), sb.count, bxb);
+#pragma unroll
+                for (int x = 0; x < K; x++) {
+#pragma unroll
+                    for (int y = 0; y < K; y++) {
+                        if ((uint32_t)x >= sa.count || (uint32_t)y >= sb.count) continue;
+                        bool keep;
+                        if (selfnode) {
+                            // unordered child pairs; a child with itself needs
+                            // no test and is no leaf pair
+                            if (y < x) continue;
+                            if (y == x) {
+                                keep = !leafout;
+                            } else {
+                                keep = overlap6(bxa[x], bxa[y]);
+                                n_tests++;
+                            }
+                        } else {
+                            keep = overlap6(bxa[x], bxb[y]);
+                            n_tests++;
+                        }
+                        if (keep) mask |= 1ull << (x * K + y);
+                    }
                 }
             }
         }
+        const uint32_t nout = (uint32_t)__popcll(mask);
         uint32_t T, C;
-        const uint32_t off = block_scan(nchild, s_wsum, T);
-        const uint32_t coff = block_scan(cand ? 1u : 0u, s_wsum, C);
+        const uint32_t off = block_scan(leafout ? 0u : nout, s_wsum, T);
+        const uint32_t coff = block_scan(leafout ? nout : 0u, s_wsum, C);
         if (tid == 0) {
             s_obase = T ? add_u32(A.count_out, T) : 0u;
             s_cbase = C ? add_u32(A.cand_count, C) : 0u;
         }
         __syncthreads();
-        const uint32_t ob = s_obase + off;
-        if (nchild && ob + nchild > A.cap_out) or_u32(A.error, OI_ERR_QUEUE_OVERFLOW);
-        for (uint32_t c = 0; c < nchild; c++) {
-            uint32_t x, y;
-            if (selfnode) {
-                tri_pair(c, sa.count, x, y);
-                x = sa.start + x;
-                y = sa.start + y;
-            } else {
-                x = sa.start + c / sb.count;
-                y = sb.start + c % sb.count;
+        if (nout) {
+            uint2* dst = leafout ? A.cands : A.front_out;
+            const uint32_t cap = leafout ? A.cand_cap : A.cap_out;
+            uint32_t o = leafout ? s_cbase + coff : s_obase + off;
+            if (o + nout > cap) or_u32(A.error, leafout ? OI_ERR_CAND_OVERFLOW : OI_ERR_QUEUE_OVERFLOW);
+            u64 m = mask;
+            while (m) {
+                const uint32_t k = (uint32_t)__ffsll((long long)m) - 1u;
+                m &= m - 1ull;
+                if (o < cap) dst[o] = make_uint2(baseA + sa.start + (k >> E), baseB + sb.start + (k & (K - 1)));
+                o++;
             }
-            if (ob + c < A.cap_out) A.front_out[ob + c] = make_uint2(baseA + x, baseB + y);
-        }
-        if (cand) {
-            const uint32_t cp = s_cbase + coff;
-            if (cp < A.cand_cap)
-                A.cands[cp] = e;
-            else
-                or_u32(A.error, OI_ERR_CAND_OVERFLOW);
         }
         __syncthreads();
     }
+    uint32_t nt;
+    block_scan(n_tests, s_wsum, nt);
+    if (tid == 0 && nt) __hip_atomic_fetch_add(A.tests, (u64)nt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------------------
@@ -589,12 +625,10 @@ __global__ void __launch_bounds__(TB) k_narrow(NarrowArgs A)
     }
 }
 
-// seed: the initial front, zeroed level counts and result counters
-__global__ void k_seed(uint2* front0, const uint2* init, uint32_t n0, uint32_t* counts, uint32_t n_counts,
-                       oi_detect_ctl* ctl, uint32_t reset_sticky)
+// seed: zeroed level counts and result counters (counts[0] = initial front)
+__global__ void k_seed(uint32_t n0, uint32_t* counts, uint32_t n_counts, oi_detect_ctl* ctl, uint32_t reset_sticky)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n0) front0[i] = init[i];
+    const uint32_t i = threadIdx.x;
     if (i < n_counts) counts[i] = (i == 0) ? n0 : 0u;
     if (i == 0) {
         // errors of the previous detect move to `sticky` unless the caller
@@ -605,6 +639,7 @@ __global__ void k_seed(uint2* front0, const uint2* init, uint32_t n0, uint32_t* 
         ctl->cands = 0;
         ctl->fp64 = 0;
         ctl->adj = 0;
+        ctl->tests = 0;
     }
 }
 
@@ -622,36 +657,49 @@ static MeshTab make_tab(const oi_mesh_desc* d_meshes, const oi_mesh_desc* h_mesh
 
 extern "C" {
 
-hipError_t oi_launch_seed(hipStream_t st, uint2* front0, const uint2* init, uint32_t n0, uint32_t* counts,
-                          uint32_t n_counts, oi_detect_ctl* ctl, uint32_t reset_sticky)
+hipError_t oi_launch_seed(hipStream_t st, uint32_t n0, uint32_t* counts, uint32_t n_counts, oi_detect_ctl* ctl,
+                          uint32_t reset_sticky)
 {
-    const uint32_t n = n0 > n_counts ? n0 : n_counts;
-    const uint32_t g = (n + 255) / 256;
-    hipLaunchKernelGGL(k_seed, dim3(g ? g : 1), dim3(256), 0, st, front0, init, n0, counts, n_counts, ctl,
-                       reset_sticky);
+    if (n_counts > 256) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_seed, dim3(1), dim3(256), 0, st, n0, counts, n_counts, ctl, reset_sticky);
     return hipGetLastError();
 }
 
-hipError_t oi_launch_bvtt_level(hipStream_t st, uint32_t grid, const oi_mesh_desc* d_meshes,
-                                const oi_mesh_desc* h_meshes, uint32_t n_meshes, const uint2* front_in,
-                                const uint32_t* count_in, uint2* front_out, uint32_t* count_out, uint32_t cap_out,
-                                uint2* cands, uint32_t* cand_count, uint32_t cand_cap, uint32_t* error,
-                                uint32_t expand, uint32_t implicit_end)
+// one BVTT step; front_in's length is *count_in, or n_in when count_in is null
+hipError_t oi_launch_bvtt_step(hipStream_t st, uint32_t grid, const oi_mesh_desc* d_meshes,
+                               const oi_mesh_desc* h_meshes, uint32_t n_meshes, int first, const uint2* front_in,
+                               const uint32_t* count_in, uint32_t n_in, uint32_t cap_in, uint2* front_out,
+                               uint32_t* count_out, uint32_t cap_out, uint2* cands, uint32_t* cand_count,
+                               uint32_t cand_cap, oi_detect_ctl* ctl, uint32_t expand, uint32_t implicit_end)
 {
-    LevelArgs a{};
+    StepArgs a{};
     a.tab = make_tab(d_meshes, h_meshes, n_meshes);
     a.front_in = front_in;
     a.count_in = count_in;
+    a.n_in = n_in;
+    a.cap_in = cap_in;
     a.front_out = front_out;
     a.count_out = count_out;
     a.cap_out = cap_out;
     a.cands = cands;
     a.cand_count = cand_count;
     a.cand_cap = cand_cap;
-    a.error = error;
-    a.expand = expand;
+    a.error = &ctl->error;
+    a.tests = &ctl->tests;
     a.implicit_end = implicit_end;
-    hipLaunchKernelGGL(k_bvtt_level, dim3(grid), dim3(TB), 0, st, a);
+    const dim3 g(grid), b(TB);
+#define OI_STEP(EE)                                                       \
+    if (first)                                                            \
+        hipLaunchKernelGGL((k_bvtt_step<EE, true>), g, b, 0, st, a);      \
+    else                                                                  \
+        hipLaunchKernelGGL((k_bvtt_step<EE, false>), g, b, 0, st, a);
+    switch (expand) {
+    case 1: OI_STEP(1) break;
+    case 2: OI_STEP(2) break;
+    case 3: OI_STEP(3) break;
+    default: return hipErrorInvalidValue;
+    }
+#undef OI_STEP
     return hipGetLastError();
 }
 

@@ -88,6 +88,7 @@ struct oi_detect_ctl {
     uint32_t sticky;           // OI_ERR_* bits of earlier asynchronous detects not checked since
     unsigned long long fp64;   // SAT axis comparisons re-evaluated in double
     unsigned long long adj;    // leaf pairs dropped as adjacent (self mode)
+    unsigned long long tests;  // BVTT box pair tests
 };
 
 #define OI_ERR_QUEUE_OVERFLOW 1u  // a BVTT front exceeded its buffer

@@ -26,10 +26,10 @@ hipError_t oi_launch_sort_pass(hipStream_t, const uint32_t*, const uint32_t*, ui
                                uint32_t, const uint32_t*, uint32_t*, uint32_t*, uint32_t*);
 hipError_t oi_launch_reduce(hipStream_t, int, const float*, const uint32_t*, const uint32_t*, uint32_t*, float*,
                             uint32_t, uint32_t, uint32_t);
-hipError_t oi_launch_seed(hipStream_t, uint2*, const uint2*, uint32_t, uint32_t*, uint32_t, oi_detect_ctl*, uint32_t);
-hipError_t oi_launch_bvtt_level(hipStream_t, uint32_t, const oi_mesh_desc*, const oi_mesh_desc*, uint32_t,
-                                const uint2*, const uint32_t*, uint2*, uint32_t*, uint32_t, uint2*, uint32_t*,
-                                uint32_t, uint32_t*, uint32_t, uint32_t);
+hipError_t oi_launch_seed(hipStream_t, uint32_t, uint32_t*, uint32_t, oi_detect_ctl*, uint32_t);
+hipError_t oi_launch_bvtt_step(hipStream_t, uint32_t, const oi_mesh_desc*, const oi_mesh_desc*, uint32_t, int,
+                               const uint2*, const uint32_t*, uint32_t, uint32_t, uint2*, uint32_t*, uint32_t, uint2*,
+                               uint32_t*, uint32_t, oi_detect_ctl*, uint32_t, uint32_t);
 hipError_t oi_launch_narrow(hipStream_t, uint32_t, const oi_mesh_desc*, const oi_mesh_desc*, uint32_t, const uint2*,
                             const uint32_t*, uint32_t, uint4*, uint32_t*, uint32_t, oi_detect_ctl*, int);
 }
@@ -705,9 +705,10 @@ static oibvh_status enqueue_detect(oibvh_scene* s, bool reset_sticky)
                                      st));
         std::memcpy(s->front_key, key, sizeof key);
     }
-    // every BVTT step takes each non-leaf side `expand` levels down; one more
-    // step tests the leaf pairs
-    s->n_levels = (depth + expand - 1) / expand + 1;
+    // every BVTT step takes each non-leaf side up to `expand` levels down and
+    // tests the pairs it reaches: the walk ends at the leaves after
+    // ceil(depth / expand) steps (one step tests entry-level leaf pairs)
+    s->n_levels = std::max(1u, (depth + expand - 1) / expand);
     oibvh_status r;
     const uint32_t fcap = std::max(c->queue_cap, oi_np2(std::max(1u, s->n_front)));
     if (s->front_cap < fcap) {
@@ -721,14 +722,17 @@ static oibvh_status enqueue_detect(oibvh_scene* s, bool reset_sticky)
     if ((r = grow(c, s->counts, s->counts_cap, s->n_levels + 1)) != OIBVH_OK) return r;
     if ((r = grow(c, s->cands, s->cand_cap, c->cand_cap)) != OIBVH_OK) return r;
     if ((r = grow(c, s->hits, s->hits_cap, c->hits_cap)) != OIBVH_OK) return r;
-    HIPCHK(c, oi_launch_seed(st, s->front[0], s->d_init, s->n_front, s->counts, s->n_levels + 1, s->ctl,
-                             reset_sticky ? 1u : 0u));
+    if (s->n_levels + 1 > 256) return set_err(c, OIBVH_ERR_INVALID, "detect: %u BVTT steps", s->n_levels);
+    HIPCHK(c, oi_launch_seed(st, s->n_front, s->counts, s->n_levels + 1, s->ctl, reset_sticky ? 1u : 0u));
     const uint32_t grid = grid_for(c);
     const oi_mesh_desc* dd = n > 4 ? s->d_desc : nullptr;
     for (uint32_t k = 0; k < s->n_levels; k++) {
-        HIPCHK(c, oi_launch_bvtt_level(st, grid, dd, desc.data(), n, s->front[k & 1], s->counts + k,
-                                       s->front[(k + 1) & 1], s->counts + k + 1, s->front_cap, s->cands,
-                                       &s->ctl->cands, s->cand_cap, &s->ctl->error, expand, s->implicit_end));
+        // step 0 reads the initial front in place; then the fronts alternate
+        const uint2* in = k ? s->front[(k - 1) & 1] : s->d_init;
+        HIPCHK(c, oi_launch_bvtt_step(st, grid, dd, desc.data(), n, k == 0, in, k ? s->counts + k : nullptr,
+                                      s->n_front, k ? s->front_cap : s->n_front, s->front[k & 1],
+                                      s->counts + k + 1, s->front_cap, s->cands, &s->ctl->cands, s->cand_cap,
+                                      s->ctl, expand, s->implicit_end));
     }
     HIPCHK(c, oi_launch_narrow(st, grid, dd, desc.data(), n, s->cands, &s->ctl->cands, s->cand_cap, s->hits,
                                &s->ctl->n_hits, s->hits_cap, s->ctl,
@@ -752,6 +756,13 @@ static oibvh_status check_detect(oibvh_scene* s)
         HIPCHK(c, hipStreamSynchronize(c->stream));
         const oi_detect_ctl& k = *s->h_ctl;
         earlier |= k.sticky;  // errors of unchecked asynchronous detects before this one
+        if (s->h_counts[s->n_levels] != 0) {
+            // the last step reaches the leaves on every side: it writes no front
+            s->unchecked = false;
+            s->have_result = false;
+            return set_err(c, OIBVH_ERR_HIP, "detect: %u BVTT pairs left after the last step",
+                           s->h_counts[s->n_levels]);
+        }
         if (k.error & OI_ERR_CORRUPT) {
             s->unchecked = false;
             s->have_result = false;
@@ -787,10 +798,8 @@ static oibvh_status check_detect(oibvh_scene* s)
             rerun = true;
         }
         if (!rerun) {
-            uint64_t tests = 0;
-            for (uint32_t l = 0; l < s->n_levels; l++) tests += s->h_counts[l];
             s->n_pairs = k.n_hits;
-            s->stats.bvtt_tests = tests;
+            s->stats.bvtt_tests = k.tests;
             s->stats.candidates = k.cands;
             s->stats.fp64_rechecks = k.fp64;
             s->stats.adjacent = k.adj;
