@@ -101,6 +101,15 @@ OIBVH_API oibvh_status oibvh_tree_refit(oibvh_tree* tree, const float* xyz, uint
  * build or refit uses them. */
 OIBVH_API oibvh_status oibvh_tree_set_positions(oibvh_tree* tree, const float* xyz, uint32_t ptr_kind);
 
+/* Mesh::transform(glm::mat4) (mesh.cpp:189-214, transform.cu:35-41) applied on
+ * the fly: every later build, refit and detect of this tree sees each vertex
+ * as mat4 * (x, y, z, 1), evaluated in glm's order for mat4 * vec4
+ * ((m0 x + m1 y) + (m2 z + m3), columns m0..m3, no fused multiply-add).
+ * mat4: 16 floats, column-major like glm::mat4 (host memory, read before the
+ * call returns); NULL restores the identity. The stored positions are not
+ * changed. Refit (or build) afterwards. */
+OIBVH_API oibvh_status oibvh_tree_set_transform(oibvh_tree* tree, const float* mat4);
+
 /* OibvhTree::getDepth() (oibvhTree.cuh:87, oibvhTree.cu:46-49): ilog2(node count) */
 OIBVH_API oibvh_status oibvh_tree_get_depth(const oibvh_tree* tree, uint32_t* depth);
 /* OibvhTree::getPrimCount() (oibvhTree.cuh:93, oibvhTree.cu:51-54) */

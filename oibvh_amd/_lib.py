@@ -56,6 +56,7 @@ API = [
     ("oibvh_tree_build", _i, [_vp]),
     ("oibvh_tree_refit", _i, [_vp, _vp, _u32]),
     ("oibvh_tree_set_positions", _i, [_vp, _vp, _u32]),
+    ("oibvh_tree_set_transform", _i, [_vp, _f32p]),
     ("oibvh_tree_get_depth", _i, [_vp, _u32p]),
     ("oibvh_tree_get_prim_count", _i, [_vp, _u32p]),
     ("oibvh_tree_get_node_count", _i, [_vp, _u32p]),

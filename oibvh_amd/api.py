@@ -155,6 +155,17 @@ class OibvhTree:
             raise ValueError("vertex count changed")
         _check(self.ctx.h, self._lib.oibvh_tree_set_positions(self.h, _ptr(xyz), L.PTR_HOST), "set_positions")
 
+    def set_transform(self, mat4) -> None:
+        """Mesh::transform: every later build, refit and detect sees each
+        vertex as mat4 @ (x, y, z, 1) (glm evaluation order). mat4 is a 4x4
+        matrix in the usual row/column sense (stored column-major like
+        glm::mat4 when passed down); None restores the identity."""
+        if mat4 is None:
+            _check(self.ctx.h, self._lib.oibvh_tree_set_transform(self.h, None), "set_transform")
+            return
+        m = np.ascontiguousarray(np.asarray(mat4, np.float32).reshape(4, 4).T)  # column-major
+        _check(self.ctx.h, self._lib.oibvh_tree_set_transform(self.h, m.ctypes.data_as(L._f32p)), "set_transform")
+
     def refit_device(self, dptr: int, borrow: bool = True) -> None:
         kind = L.PTR_DEVICE_BORROW if borrow else L.PTR_DEVICE
         _check(self.ctx.h, self._lib.oibvh_tree_refit(self.h, C.c_void_p(dptr), kind), "refit")

@@ -18,6 +18,12 @@
 // already sorted faces (calculate_aabb_kernel + oibvh_tree_construction_kernel,
 // oibvh.cu:24-40, 80-154).
 //
+// Every kernel is table-driven: a BuildTab names the meshes (oi_build_desc)
+// and maps each workgroup to a (mesh, chunk) work item, so one launch per
+// stage serves one tree or a batch of hundreds of trees. Vertex positions go
+// through the mesh's model matrix when it has one (Mesh::transform,
+// mesh.cpp:189-214), evaluated on the fly wherever a position is read.
+//
 // Compiled with -ffp-contract=off. min/max are compare-selects with glm's tie
 // rules: min(a,b) = b < a ? b : a, max(a,b) = a < b ? b : a.
 #include <hip/hip_runtime.h>
@@ -25,6 +31,7 @@
 #include "oibvh_common.h"
 
 #define OI_BLOCK 256
+#define OI_MORTON_CHUNK 2048
 
 __device__ __forceinline__ float gmin(float a, float b) { return (b < a) ? b : a; }
 __device__ __forceinline__ float gmax(float a, float b) { return (a < b) ? b : a; }
@@ -60,36 +67,77 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wsum
 }
 
 // ---------------------------------------------------------------------------
+// mesh tables
+// ---------------------------------------------------------------------------
+struct BuildTab {
+    const oi_build_desc* d;  // device table; null: `one` is mesh 0
+    const uint2* work;       // workgroup -> (mesh, chunk); null: (0, blockIdx.x)
+    oi_build_desc one;
+    __device__ __forceinline__ uint2 item() const { return work ? work[blockIdx.x] : make_uint2(0u, blockIdx.x); }
+    __device__ __forceinline__ oi_build_desc mesh(uint32_t m) const { return d ? d[m] : one; }
+};
+
+// a mesh's model matrix, held in registers
+struct Xf {
+    float m[16];
+    bool on;
+    __device__ __forceinline__ void load(const float* xf)
+    {
+        on = xf != nullptr;
+#pragma unroll
+        for (int k = 0; k < 16; k++) m[k] = on ? xf[k] : 0.0f;
+    }
+    __device__ __forceinline__ void vertex(const float* pos, uint32_t v, float* o) const
+    {
+        const float x = pos[3u * v], y = pos[3u * v + 1], z = pos[3u * v + 2];
+        if (on) {
+            oi_xform(m, x, y, z, o);
+        } else {
+            o[0] = x;
+            o[1] = y;
+            o[2] = z;
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
 // k_mesh_box: vertex AABB with the sequential fold's exact result.
 // The fold acc = glm::min(v, acc) keeps the later value on ties, so partial
 // results combine as lo(L, R) = L < R ? L : R and hi(L, R) = R < L ? L : R in
 // index order; +-inf are the identities, the default box (+-FLT_MAX) is folded
 // in first, as aabb_box_t's initial value (utils.h:17-18).
+// Work item (m, c): vertices [c * vchunk, (c + 1) * vchunk) of mesh m, partial
+// box c of the mesh.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float lo_c(float l, float r) { return (l < r) ? l : r; }
 __device__ __forceinline__ float hi_c(float l, float r) { return (r < l) ? l : r; }
 
 __global__ void __launch_bounds__(OI_BLOCK)
-k_mesh_box(const float* __restrict__ xyz, uint32_t V, uint32_t chunk, float* __restrict__ partials,
-           uint32_t* __restrict__ clear_u32, uint32_t clear_n)
+k_mesh_box(BuildTab T, float* __restrict__ partials, uint32_t* __restrict__ clear_u32, uint32_t clear_n)
 {
     const uint32_t tid = threadIdx.x;
     for (uint32_t i = blockIdx.x * OI_BLOCK + tid; i < clear_n; i += gridDim.x * OI_BLOCK) clear_u32[i] = 0u;
 
     __shared__ float s_v[6][OI_BLOCK];
+    const uint2 it = T.item();
+    const oi_build_desc D = T.mesh(it.x);
+    Xf X;
+    X.load(D.xf);
     const float INF = __builtin_inff();
     float mn[3] = {INF, INF, INF}, mx[3] = {-INF, -INF, -INF};
-    const uint32_t b0 = blockIdx.x * chunk;
-    const uint32_t b1 = min(V, b0 + chunk);
+    const uint32_t chunk = D.vchunk;
+    const uint32_t b0 = it.y * chunk;
+    const uint32_t b1 = min(D.V, b0 + chunk);
     const uint32_t sub = (chunk + OI_BLOCK - 1) / OI_BLOCK;
     const uint32_t s0 = b0 + tid * sub;
     const uint32_t s1 = min(b1, s0 + sub);
     for (uint32_t v = s0; v < s1; v++) {
+        float p[3];
+        X.vertex(D.pos, v, p);
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            float x = xyz[3u * v + k];
-            mn[k] = lo_c(mn[k], x);
-            mx[k] = hi_c(mx[k], x);
+            mn[k] = lo_c(mn[k], p[k]);
+            mx[k] = hi_c(mx[k], p[k]);
         }
     }
 #pragma unroll
@@ -108,7 +156,7 @@ k_mesh_box(const float* __restrict__ xyz, uint32_t V, uint32_t chunk, float* __r
         }
         __syncthreads();
     }
-    if (tid < 6) partials[6u * blockIdx.x + tid] = s_v[tid][0];
+    if (tid < 6) partials[6u * (D.pbase + it.y) + tid] = s_v[tid][0];
 }
 
 // Fold the per-block partial boxes (at most OI_BLOCK of them) in block order and
@@ -164,37 +212,50 @@ __device__ __forceinline__ uint32_t quantise10(float p)
     return (q != q) ? 0u : (uint32_t)q;
 }
 
+// Work item (m, c): faces [c * OI_MORTON_CHUNK, (c + 1) * OI_MORTON_CHUNK) of
+// mesh m. Keys go to keys[fbase + face]; the four 8-bit digit histograms of
+// all keys of the launch accumulate in ghist; fmesh (if not null) receives
+// the mesh id of each face (fmesh[fbase + face] = m).
 __global__ void __launch_bounds__(OI_BLOCK)
-k_morton(const uint32_t* __restrict__ faces, const float* __restrict__ xyz, uint32_t F,
-         const float* __restrict__ partials, uint32_t nparts, float* __restrict__ box_out,
-         uint32_t* __restrict__ keys, uint32_t* __restrict__ ghist)
+k_morton(BuildTab T, const float* __restrict__ partials, uint32_t* __restrict__ keys, uint32_t* __restrict__ ghist,
+         uint32_t* __restrict__ fmesh)
 {
     __shared__ uint32_t s_h[4][256];
     __shared__ float s_v[6][OI_BLOCK];
     const uint32_t tid = threadIdx.x;
+    const uint2 it = T.item();
+    const oi_build_desc D = T.mesh(it.x);
+    Xf X;
+    X.load(D.xf);
 #pragma unroll
     for (int d = 0; d < 4; d++) s_h[d][tid] = 0u;
     float bmn[3], bmx[3];
-    fold_partials(partials, nparts, s_v, bmn, bmx);
-    if (blockIdx.x == 0 && tid < 3) {
-        box_out[tid] = bmn[tid];
-        box_out[3 + tid] = bmx[tid];
+    fold_partials(partials + 6u * D.pbase, D.nparts, s_v, bmn, bmx);
+    if (it.y == 0 && tid < 3) {
+        D.box[tid] = bmn[tid];
+        D.box[3 + tid] = bmx[tid];
     }
-    for (uint32_t i = blockIdx.x * OI_BLOCK + tid; i < F; i += gridDim.x * OI_BLOCK) {
-        const uint32_t f0 = faces[3u * i], f1 = faces[3u * i + 1], f2 = faces[3u * i + 2];
+    const uint32_t f0 = it.y * OI_MORTON_CHUNK;
+    const uint32_t f1 = min(D.F, f0 + OI_MORTON_CHUNK);
+    for (uint32_t i = f0 + tid; i < f1; i += OI_BLOCK) {
+        const uint32_t v0 = D.faces[3u * i], v1 = D.faces[3u * i + 1], v2 = D.faces[3u * i + 2];
+        float a[3], b[3], c[3];
+        X.vertex(D.pos, v0, a);
+        X.vertex(D.pos, v1, b);
+        X.vertex(D.pos, v2, c);
         uint32_t q[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            const float a = xyz[3u * f0 + k], b = xyz[3u * f1 + k], c = xyz[3u * f2 + k];
-            const float lo = gmin(gmin(a, b), c);
-            const float hi = gmax(gmax(a, b), c);
+            const float lo = gmin(gmin(a[k], b[k]), c[k]);
+            const float hi = gmax(gmax(a[k], b[k]), c[k]);
             const float cen = (lo + hi) * 0.5f;
             const float off = cen - bmn[k];
             const float len = bmx[k] - bmn[k];
             q[k] = quantise10(off / len);
         }
         const uint32_t key = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
-        keys[i] = key;
+        keys[D.fbase + i] = key;
+        if (fmesh) fmesh[D.fbase + i] = it.x;
 #pragma unroll
         for (int d = 0; d < 4; d++) atomicAdd(&s_h[d][(key >> (8 * d)) & 0xffu], 1u);
     }
@@ -215,7 +276,9 @@ k_morton(const uint32_t* __restrict__ faces, const float* __restrict__ xyz, uint
 // tile-local sorted order and written out in runs. Tiles take their id from an
 // atomic counter, so every tile a look-back waits on is already running.
 // Look-back status word: bits 31:30 = 1 aggregate / 2 inclusive prefix,
-// bits 29:0 = count.
+// bits 29:0 = count. With key_of_val the key of a value v is key_of_val[v]
+// (a batch's mesh-id passes: a stable partition of the Morton-sorted faces
+// of all meshes back into their meshes).
 // ---------------------------------------------------------------------------
 #define RS_ITEMS 16
 #define RS_TILE (OI_BLOCK * RS_ITEMS)
@@ -227,7 +290,7 @@ __global__ void __launch_bounds__(OI_BLOCK)
 k_sort_pass(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
             uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, uint32_t n, uint32_t shift,
             const uint32_t* __restrict__ ghist, uint32_t* lookback, uint32_t* lookback_next,
-            uint32_t* tile_counter)
+            uint32_t* tile_counter, const uint32_t* __restrict__ key_of_val)
 {
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_whist[4][256];
@@ -256,8 +319,8 @@ k_sort_pass(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ v
     for (int j = 0; j < RS_ITEMS; j++) {
         const uint32_t idx = wbase + 64u * j + lane;
         const bool ok = idx < n;
-        k[j] = ok ? keys_in[idx] : 0u;
         v[j] = ok ? (vals_in ? vals_in[idx] : idx) : 0u;
+        k[j] = ok ? (key_of_val ? key_of_val[v[j]] : keys_in[idx]) : 0u;
     }
     const unsigned long long lt = (1ull << lane) - 1ull;
 #pragma unroll
@@ -339,12 +402,13 @@ k_sort_pass(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ v
 
 // ---------------------------------------------------------------------------
 // k_reduce: leaves + bottom-up reduce.
-// Stage 1: workgroup w owns the subtree of height h = min(L, 8 + log2 M)
-// rooted at level L - h, position w (its 2^h leaves). Each thread computes M
-// consecutive leaves and folds them log2(M) levels in registers; the 256
-// results climb the remaining levels through LDS (ping-pong buffers, one
-// barrier per level). Every real node is written once, at its real index.
-// Stage 2: the last workgroup to finish (device-scope counter) reads the
+// Stage 1: work item (m, w) owns the subtree of mesh m of height
+// h = min(L, 8 + log2 M) rooted at level L - h, position w (its 2^h leaves).
+// Each thread computes M consecutive leaves and folds them log2(M) levels in
+// registers; the 256 results climb the remaining levels through LDS
+// (ping-pong buffers, one barrier per level). Every real node is written
+// once, at its real index.
+// Stage 2 (k_reduce_apex, a second launch): one workgroup per mesh reads the
 // stage-1 roots and climbs to the root in chunks of up to 2^11 nodes.
 // A parent is real iff its left child is; with no real right child the
 // parent equals the left child (oibvh.cu:139-146).
@@ -385,35 +449,46 @@ __device__ __forceinline__ Box6 load_node(const float* nodes, uint32_t r)
 }
 
 struct ReduceArgs {
-    const float* pos;
-    const uint32_t* faces;  // sorted faces (refit) or original faces (build)
-    const uint32_t* perm;   // build: leaf p takes face perm[p]; refit: nullptr
-    uint32_t* sfaces_out;   // build: sorted faces written here; refit: nullptr
-    float* nodes;
-    uint32_t F, L, vl;
+    BuildTab T;
+    const uint32_t* vals;  // build: Morton-sorted face ids (fbase + face); null: refit over sfaces
+    uint32_t write_perm;   // build: also write perm (faces local to the mesh)
 };
 
 struct LeafLoad {
-    const ReduceArgs& a;
+    const oi_build_desc& D;
+    const Xf& X;
+    const uint32_t* vals;
+    uint32_t write_perm;
     uint32_t leaf_start;
     __device__ __forceinline__ Box6 operator()(uint32_t p) const
     {
-        const uint32_t fi = a.perm ? a.perm[p] : p;
-        const uint32_t f0 = a.faces[3u * fi], f1 = a.faces[3u * fi + 1], f2 = a.faces[3u * fi + 2];
-        if (a.sfaces_out) {
-            a.sfaces_out[3u * p] = f0;
-            a.sfaces_out[3u * p + 1] = f1;
-            a.sfaces_out[3u * p + 2] = f2;
+        uint32_t f0, f1, f2;
+        if (vals) {
+            const uint32_t fi = vals[D.fbase + p] - D.fbase;
+            f0 = D.faces[3u * fi];
+            f1 = D.faces[3u * fi + 1];
+            f2 = D.faces[3u * fi + 2];
+            D.sfaces[3u * p] = f0;
+            D.sfaces[3u * p + 1] = f1;
+            D.sfaces[3u * p + 2] = f2;
+            if (write_perm) D.perm[p] = fi;
+        } else {
+            f0 = D.sfaces[3u * p];
+            f1 = D.sfaces[3u * p + 1];
+            f2 = D.sfaces[3u * p + 2];
         }
-        Box6 b;
+        float a[3], b[3], c[3];
+        X.vertex(D.pos, f0, a);
+        X.vertex(D.pos, f1, b);
+        X.vertex(D.pos, f2, c);
+        Box6 r;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            const float x0 = a.pos[3u * f0 + k], x1 = a.pos[3u * f1 + k], x2 = a.pos[3u * f2 + k];
-            b.v[k] = gmin(gmin(x0, x1), x2);
-            b.v[3 + k] = gmax(gmax(x0, x1), x2);
+            r.v[k] = gmin(gmin(a[k], b[k]), c[k]);
+            r.v[3 + k] = gmax(gmax(a[k], b[k]), c[k]);
         }
-        store_node(a.nodes, leaf_start + p, b);
-        return b;
+        store_node(D.nodes, leaf_start + p, r);
+        return r;
     }
 };
 
@@ -505,26 +580,34 @@ __global__ void __launch_bounds__(OI_BLOCK) k_reduce(ReduceArgs a)
     __shared__ float sA[6][OI_BLOCK];
     __shared__ float sB[6][OI_BLOCK];
     constexpr uint32_t LGM = (M == 1) ? 0 : (M == 2) ? 1 : (M == 4) ? 2 : 3;
-    const uint32_t L = a.L, vl = a.vl;
+    const uint2 it = a.T.item();
+    const oi_build_desc D = a.T.mesh(it.x);
+    Xf X;
+    X.load(D.xf);
+    const uint32_t L = D.L, vl = D.vl;
     const uint32_t h = (L < 8 + LGM) ? L : 8 + LGM;
-    LeafLoad ll{a, oi_lvl_start(L, L, vl)};
-    reduce_subtree<M>(ll, a.nodes, L, vl, L, h, blockIdx.x, sA, sB);
+    LeafLoad ll{D, X, a.vals, a.write_perm, oi_lvl_start(L, L, vl)};
+    reduce_subtree<M>(ll, D.nodes, L, vl, L, h, it.y, sA, sB);
 }
 
-// Stage 2, one workgroup: the levels above the stage-1 subtree roots (level
-// lc), in chunks of up to 2^11 nodes per climb. A separate launch, so the
-// roots written by every stage-1 workgroup are visible without device-scope
-// fences.
-__global__ void __launch_bounds__(OI_BLOCK) k_reduce_apex(ReduceArgs a, uint32_t lc)
+// Stage 2, one workgroup per mesh: the levels above the stage-1 subtree roots
+// (level lc = L - h), in chunks of up to 2^11 nodes per climb. A separate
+// launch, so the roots written by every stage-1 workgroup are visible without
+// device-scope fences.
+__global__ void __launch_bounds__(OI_BLOCK) k_reduce_apex(ReduceArgs a, uint32_t lgm)
 {
     __shared__ float sA[6][OI_BLOCK];
     __shared__ float sB[6][OI_BLOCK];
-    const uint32_t L = a.L, vl = a.vl;
+    const uint2 it = a.T.item();
+    const oi_build_desc D = a.T.mesh(it.x);
+    const uint32_t L = D.L, vl = D.vl;
+    const uint32_t h = (L < 8 + lgm) ? L : 8 + lgm;
+    uint32_t lc = L - h;
     while (lc > 0) {
         const uint32_t h2 = (lc < 11u) ? lc : 11u;
         const uint32_t nchunk = oi_lvl_cnt(lc - h2, L, vl);
-        NodeLoad nl{a.nodes, oi_lvl_start(lc, L, vl)};
-        for (uint32_t c = 0; c < nchunk; c++) reduce_subtree<8>(nl, a.nodes, L, vl, lc, h2, c, sA, sB);
+        NodeLoad nl{D.nodes, oi_lvl_start(lc, L, vl)};
+        for (uint32_t c = 0; c < nchunk; c++) reduce_subtree<8>(nl, D.nodes, L, vl, lc, h2, c, sA, sB);
         lc -= h2;
         __threadfence_block();
         __syncthreads();
@@ -534,23 +617,55 @@ __global__ void __launch_bounds__(OI_BLOCK) k_reduce_apex(ReduceArgs a, uint32_t
 // ---------------------------------------------------------------------------
 // host-side launchers (called from oibvh_host.cpp)
 // ---------------------------------------------------------------------------
+// a tree's model matrix, written in stream order (the host copy may go away)
+struct Mat16 {
+    float m[16];
+};
+__global__ void k_set_xf(float* dst, Mat16 m)
+{
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < 16; k++) dst[k] = m.m[k];
+    }
+}
+
+static BuildTab make_btab(const oi_build_desc* d, const uint2* work, const oi_build_desc* one)
+{
+    BuildTab t{};
+    t.d = d;
+    t.work = work;
+    if (one) t.one = *one;
+    return t;
+}
+
 extern "C" {
 
-hipError_t oi_launch_mesh_box(hipStream_t st, const float* xyz, uint32_t V, float* partials, uint32_t nblocks,
-                              uint32_t* clear_u32, uint32_t clear_n)
+// Every launcher takes the mesh table as (d, work, one): a device descriptor
+// table and work list, or d = work = null and `one` for a single mesh whose
+// workgroup b takes chunk b.
+hipError_t oi_launch_mesh_box(hipStream_t st, const oi_build_desc* d, const uint2* work, const oi_build_desc* one,
+                              uint32_t nblocks, float* partials, uint32_t* clear_u32, uint32_t clear_n)
 {
-    const uint32_t chunk = (V + nblocks - 1) / nblocks;
-    hipLaunchKernelGGL(k_mesh_box, dim3(nblocks), dim3(OI_BLOCK), 0, st, xyz, V, chunk, partials, clear_u32,
-                       clear_n);
+    hipLaunchKernelGGL(k_mesh_box, dim3(nblocks), dim3(OI_BLOCK), 0, st, make_btab(d, work, one), partials,
+                       clear_u32, clear_n);
     return hipGetLastError();
 }
 
-hipError_t oi_launch_morton(hipStream_t st, const uint32_t* faces, const float* xyz, uint32_t F,
-                            const float* partials, uint32_t nparts, float* box_out, uint32_t* keys, uint32_t* ghist,
-                            uint32_t nblocks)
+uint32_t oi_morton_chunk() { return OI_MORTON_CHUNK; }
+
+hipError_t oi_launch_set_xf(hipStream_t st, float* dst, const float* mat4)
 {
-    hipLaunchKernelGGL(k_morton, dim3(nblocks), dim3(OI_BLOCK), 0, st, faces, xyz, F, partials, nparts, box_out,
-                       keys, ghist);
+    Mat16 m;
+    for (int k = 0; k < 16; k++) m.m[k] = mat4[k];
+    hipLaunchKernelGGL(k_set_xf, dim3(1), dim3(64), 0, st, dst, m);
+    return hipGetLastError();
+}
+
+hipError_t oi_launch_morton(hipStream_t st, const oi_build_desc* d, const uint2* work, const oi_build_desc* one,
+                            uint32_t nblocks, const float* partials, uint32_t* keys, uint32_t* ghist, uint32_t* fmesh)
+{
+    hipLaunchKernelGGL(k_morton, dim3(nblocks), dim3(OI_BLOCK), 0, st, make_btab(d, work, one), partials, keys,
+                       ghist, fmesh);
     return hipGetLastError();
 }
 
@@ -558,29 +673,43 @@ uint32_t oi_sort_tile() { return RS_TILE; }
 
 hipError_t oi_launch_sort_pass(hipStream_t st, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
                                uint32_t* vals_out, uint32_t n, uint32_t shift, const uint32_t* ghist,
-                               uint32_t* lookback, uint32_t* lookback_next, uint32_t* tile_counter)
+                               uint32_t* lookback, uint32_t* lookback_next, uint32_t* tile_counter,
+                               const uint32_t* key_of_val)
 {
     const uint32_t tiles = (n + RS_TILE - 1) / RS_TILE;
     hipLaunchKernelGGL(k_sort_pass, dim3(tiles), dim3(OI_BLOCK), 0, st, keys_in, vals_in, keys_out, vals_out, n,
-                       shift, ghist, lookback, lookback_next, tile_counter);
+                       shift, ghist, lookback, lookback_next, tile_counter, key_of_val);
     return hipGetLastError();
 }
 
-// M = leaves per thread in stage 1 (1, 2, 4 or 8)
-hipError_t oi_launch_reduce(hipStream_t st, int M, const float* pos, const uint32_t* faces, const uint32_t* perm,
-                            uint32_t* sfaces_out, float* nodes, uint32_t F, uint32_t L, uint32_t vl)
+// subtree height of stage 1 for M leaves per thread
+uint32_t oi_reduce_height(int M, uint32_t L)
 {
-    ReduceArgs a{pos, faces, perm, sfaces_out, nodes, F, L, vl};
-    uint32_t lgm = (M == 1) ? 0 : (M == 2) ? 1 : (M == 4) ? 2 : 3;
-    uint32_t h = (L < 8 + lgm) ? L : 8 + lgm;
-    uint32_t grid = oi_lvl_cnt(L - h, L, vl);
+    const uint32_t lgm = (M == 1) ? 0 : (M == 2) ? 1 : (M == 4) ? 2 : 3;
+    return (L < 8 + lgm) ? L : 8 + lgm;
+}
+
+// stage 1 (M = leaves per thread: 1, 2, 4 or 8); vals: build, null: refit
+hipError_t oi_launch_reduce(hipStream_t st, int M, const oi_build_desc* d, const uint2* work,
+                            const oi_build_desc* one, uint32_t nblocks, const uint32_t* vals, uint32_t write_perm)
+{
+    ReduceArgs a{make_btab(d, work, one), vals, write_perm};
     switch (M) {
-    case 1: hipLaunchKernelGGL(k_reduce<1>, dim3(grid), dim3(OI_BLOCK), 0, st, a); break;
-    case 2: hipLaunchKernelGGL(k_reduce<2>, dim3(grid), dim3(OI_BLOCK), 0, st, a); break;
-    case 4: hipLaunchKernelGGL(k_reduce<4>, dim3(grid), dim3(OI_BLOCK), 0, st, a); break;
-    default: hipLaunchKernelGGL(k_reduce<8>, dim3(grid), dim3(OI_BLOCK), 0, st, a); break;
+    case 1: hipLaunchKernelGGL(k_reduce<1>, dim3(nblocks), dim3(OI_BLOCK), 0, st, a); break;
+    case 2: hipLaunchKernelGGL(k_reduce<2>, dim3(nblocks), dim3(OI_BLOCK), 0, st, a); break;
+    case 4: hipLaunchKernelGGL(k_reduce<4>, dim3(nblocks), dim3(OI_BLOCK), 0, st, a); break;
+    default: hipLaunchKernelGGL(k_reduce<8>, dim3(nblocks), dim3(OI_BLOCK), 0, st, a); break;
     }
-    if (L > h) hipLaunchKernelGGL(k_reduce_apex, dim3(1), dim3(OI_BLOCK), 0, st, a, L - h);
+    return hipGetLastError();
+}
+
+// stage 2, one workgroup per work item (mesh, 0) whose L exceeds the stage-1 height
+hipError_t oi_launch_reduce_apex(hipStream_t st, int M, const oi_build_desc* d, const uint2* work,
+                                 const oi_build_desc* one, uint32_t nblocks)
+{
+    ReduceArgs a{make_btab(d, work, one), nullptr, 0};
+    const uint32_t lgm = (M == 1) ? 0 : (M == 2) ? 1 : (M == 4) ? 2 : 3;
+    hipLaunchKernelGGL(k_reduce_apex, dim3(nblocks), dim3(OI_BLOCK), 0, st, a, lgm);
     return hipGetLastError();
 }
 

@@ -288,6 +288,7 @@ struct MeshRef {
     const uint32_t* sfaces;
     const float* pos;
     const uint32_t* perm;
+    const float* xf;
     uint32_t L, vl, ibase;
 };
 
@@ -328,7 +329,47 @@ __device__ __forceinline__ uint32_t find_mesh(const MeshTab& T, const uint32_t* 
 __device__ __forceinline__ MeshRef mesh_ref(const MeshTab& T, const oi_mesh_desc* s_desc, uint32_t m)
 {
     const oi_mesh_desc& d = (m < MESH_LDS) ? s_desc[m] : T.meshes[m];
-    return MeshRef{d.nodes, d.sfaces, d.pos, d.perm, d.L, d.vl, d.ibase};
+    return MeshRef{d.nodes, d.sfaces, d.pos, d.perm, d.xf, d.L, d.vl, d.ibase};
+}
+
+// A batch's mesh table: every mesh's implicit range starts at a multiple of
+// 2^shift and is padded to one, so the mesh of global implicit index g is
+// chunk_mesh[g >> shift] (one lookup instead of a search over the meshes).
+struct ChunkTab {
+    const oi_mesh_desc* meshes;
+    const uint32_t* chunk_mesh;
+    uint32_t shift;
+};
+
+// uniform access to both table kinds: per-workgroup set-up, then lookups
+__device__ __forceinline__ void tab_init(const MeshTab& T, oi_mesh_desc* s_desc, uint32_t* s_ibase)
+{
+    load_mesh_tab(T, s_desc, s_ibase);
+}
+__device__ __forceinline__ void tab_init(const ChunkTab&, oi_mesh_desc*, uint32_t*) {}
+
+__device__ __forceinline__ MeshRef tab_ref(const MeshTab& T, const oi_mesh_desc* s_desc, const uint32_t* s_ibase,
+                                           uint32_t g, uint32_t& m)
+{
+    m = find_mesh(T, s_ibase, g);
+    return mesh_ref(T, s_desc, m);
+}
+__device__ __forceinline__ MeshRef tab_ref(const ChunkTab& T, const oi_mesh_desc*, const uint32_t*, uint32_t g,
+                                           uint32_t& m)
+{
+    m = T.chunk_mesh[g >> T.shift];
+    const oi_mesh_desc& d = T.meshes[m];
+    return MeshRef{d.nodes, d.sfaces, d.pos, d.perm, d.xf, d.L, d.vl, d.ibase};
+}
+
+// a vertex through the mesh's model matrix, if any
+__device__ __forceinline__ v3f vertex(const float* pos, const float* xf, uint32_t v)
+{
+    const float x = pos[3u * v], y = pos[3u * v + 1], z = pos[3u * v + 2];
+    if (!xf) return v3f{x, y, z};
+    float o[3];
+    oi_xform(xf, x, y, z, o);
+    return v3f{o[0], o[1], o[2]};
 }
 
 __device__ __forceinline__ bool overlap6(const float* a, const float* b)
@@ -381,8 +422,9 @@ __device__ __forceinline__ Side expand_side(uint32_t i, uint32_t lvl, uint32_t L
 // The first step also tests the entry pairs themselves (the initial front is
 // every pair of entry-level nodes, scene.cu:192-223).
 // ---------------------------------------------------------------------------
+template <class Tab>
 struct StepArgs {
-    MeshTab tab;
+    Tab tab;
     const uint2* front_in;
     const uint32_t* count_in;   // length of front_in; null: n_in
     uint32_t n_in;
@@ -406,8 +448,8 @@ __device__ __forceinline__ void load_run(const float* nodes, uint32_t r0, uint32
         if ((uint32_t)k < n) load_box(nodes, r0 + k, b[k]);
 }
 
-template <int E, bool FIRST>
-__global__ void __launch_bounds__(TB) k_bvtt_step(StepArgs A)
+template <int E, bool FIRST, class Tab>
+__global__ void __launch_bounds__(TB) k_bvtt_step(StepArgs<Tab> A)
 {
     constexpr int K = 1 << E;
     __shared__ oi_mesh_desc s_desc[MESH_LDS];
@@ -419,7 +461,7 @@ __global__ void __launch_bounds__(TB) k_bvtt_step(StepArgs A)
     uint32_t n_in = A.count_in ? *A.count_in : A.n_in;
     if (n_in > A.cap_in) n_in = A.cap_in;
     if (blockIdx.x * TB >= n_in) return;
-    load_mesh_tab(A.tab, s_desc, s_ibase);
+    tab_init(A.tab, s_desc, s_ibase);
     const uint32_t tid = threadIdx.x;
     uint32_t n_tests = 0;
     for (uint32_t base = blockIdx.x * TB; base < n_in; base += gridDim.x * TB) {
@@ -435,9 +477,9 @@ __global__ void __launch_bounds__(TB) k_bvtt_step(StepArgs A)
         Side sa = {0, 0}, sb = {0, 0};
         uint32_t baseA = 0, baseB = 0;
         if (have) {
-            const uint32_t ma = find_mesh(A.tab, s_ibase, e.x);
-            const uint32_t mb = find_mesh(A.tab, s_ibase, e.y);
-            const MeshRef ra = mesh_ref(A.tab, s_desc, ma), rb = mesh_ref(A.tab, s_desc, mb);
+            uint32_t ma, mb;
+            const MeshRef ra = tab_ref(A.tab, s_desc, s_ibase, e.x, ma);
+            const MeshRef rb = tab_ref(A.tab, s_desc, s_ibase, e.y, mb);
             baseA = ra.ibase;
             baseB = rb.ibase;
             const uint32_t ia = e.x - ra.ibase, ib = e.y - rb.ibase;
@@ -517,8 +559,9 @@ __global__ void __launch_bounds__(TB) k_bvtt_step(StepArgs A)
 // ---------------------------------------------------------------------------
 // k_narrow: SAT over the candidate list
 // ---------------------------------------------------------------------------
+template <class Tab>
 struct NarrowArgs {
-    MeshTab tab;
+    Tab tab;
     const uint2* cands;
     const uint32_t* cand_count;
     uint32_t cand_cap;
@@ -530,14 +573,14 @@ struct NarrowArgs {
     unsigned long long* adj;    // self-mode leaf pairs dropped as adjacent
 };
 
-template <bool FP64>
-__device__ __forceinline__ void narrow_one(const MeshTab& T, const uint32_t* s_ibase, const oi_mesh_desc* s_desc,
+template <bool FP64, class Tab>
+__device__ __forceinline__ void narrow_one(const Tab& T, const uint32_t* s_ibase, const oi_mesh_desc* s_desc,
                                            uint2 c, bool& hit, uint4& out, uint32_t& n_fp64, uint32_t& n_adj)
 {
     hit = false;
-    const uint32_t ma = find_mesh(T, s_ibase, c.x);
-    const uint32_t mb = find_mesh(T, s_ibase, c.y);
-    const MeshRef ra = mesh_ref(T, s_desc, ma), rb = mesh_ref(T, s_desc, mb);
+    uint32_t ma, mb;
+    const MeshRef ra = tab_ref(T, s_desc, s_ibase, c.x, ma);
+    const MeshRef rb = tab_ref(T, s_desc, s_ibase, c.y, mb);
     const uint32_t sa = c.x - ra.ibase - ((1u << ra.L) - 1u);
     const uint32_t sb = c.y - rb.ibase - ((1u << rb.L) - 1u);
     uint32_t fa[3], fb[3];
@@ -575,8 +618,8 @@ __device__ __forceinline__ void narrow_one(const MeshTab& T, const uint32_t* s_i
     v3f P[3], Q[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        P[k] = v3f{pa[3u * fa[k]], pa[3u * fa[k] + 1], pa[3u * fa[k] + 2]};
-        Q[k] = v3f{pb[3u * fb[k]], pb[3u * fb[k] + 1], pb[3u * fb[k] + 2]};
+        P[k] = vertex(pa, ra.xf, fa[k]);
+        Q[k] = vertex(pb, rb.xf, fb[k]);
     }
     const bool h = FP64 ? sat_filtered(P, Q, n_fp64) : sat_f32(P, Q);
     if (h) {
@@ -585,8 +628,8 @@ __device__ __forceinline__ void narrow_one(const MeshTab& T, const uint32_t* s_i
     }
 }
 
-template <bool FP64>
-__global__ void __launch_bounds__(TB) k_narrow(NarrowArgs A)
+template <bool FP64, class Tab>
+__global__ void __launch_bounds__(TB) k_narrow(NarrowArgs<Tab> A)
 {
     __shared__ oi_mesh_desc s_desc[MESH_LDS];
     __shared__ uint32_t s_ibase[MESH_LDS];
@@ -595,14 +638,14 @@ __global__ void __launch_bounds__(TB) k_narrow(NarrowArgs A)
     uint32_t n = *A.cand_count;
     if (n > A.cand_cap) n = A.cand_cap;
     if (blockIdx.x * TB >= n) return;
-    load_mesh_tab(A.tab, s_desc, s_ibase);
+    tab_init(A.tab, s_desc, s_ibase);
     const uint32_t tid = threadIdx.x;
     uint32_t n_fp64 = 0, n_adj = 0;
     for (uint32_t base = blockIdx.x * TB; base < n; base += gridDim.x * TB) {
         const uint32_t i = base + tid;
         bool hit = false;
         uint4 out;
-        if (i < n) narrow_one<FP64>(A.tab, s_ibase, s_desc, A.cands[i], hit, out, n_fp64, n_adj);
+        if (i < n) narrow_one<FP64, Tab>(A.tab, s_ibase, s_desc, A.cands[i], hit, out, n_fp64, n_adj);
         uint32_t H;
         const uint32_t hoff = block_scan(hit ? 1u : 0u, s_wsum, H);
         if (tid == 0) s_hbase = H ? add_u32(A.n_hits, H) : 0u;
@@ -655,25 +698,14 @@ static MeshTab make_tab(const oi_mesh_desc* d_meshes, const oi_mesh_desc* h_mesh
     return t;
 }
 
-extern "C" {
-
-hipError_t oi_launch_seed(hipStream_t st, uint32_t n0, uint32_t* counts, uint32_t n_counts, oi_detect_ctl* ctl,
-                          uint32_t reset_sticky)
+template <class Tab>
+static hipError_t launch_step(hipStream_t st, uint32_t grid, const Tab& tab, int first, const uint2* front_in,
+                              const uint32_t* count_in, uint32_t n_in, uint32_t cap_in, uint2* front_out,
+                              uint32_t* count_out, uint32_t cap_out, uint2* cands, uint32_t* cand_count,
+                              uint32_t cand_cap, oi_detect_ctl* ctl, uint32_t expand, uint32_t implicit_end)
 {
-    if (n_counts > 256) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_seed, dim3(1), dim3(256), 0, st, n0, counts, n_counts, ctl, reset_sticky);
-    return hipGetLastError();
-}
-
-// one BVTT step; front_in's length is *count_in, or n_in when count_in is null
-hipError_t oi_launch_bvtt_step(hipStream_t st, uint32_t grid, const oi_mesh_desc* d_meshes,
-                               const oi_mesh_desc* h_meshes, uint32_t n_meshes, int first, const uint2* front_in,
-                               const uint32_t* count_in, uint32_t n_in, uint32_t cap_in, uint2* front_out,
-                               uint32_t* count_out, uint32_t cap_out, uint2* cands, uint32_t* cand_count,
-                               uint32_t cand_cap, oi_detect_ctl* ctl, uint32_t expand, uint32_t implicit_end)
-{
-    StepArgs a{};
-    a.tab = make_tab(d_meshes, h_meshes, n_meshes);
+    StepArgs<Tab> a{};
+    a.tab = tab;
     a.front_in = front_in;
     a.count_in = count_in;
     a.n_in = n_in;
@@ -688,11 +720,11 @@ hipError_t oi_launch_bvtt_step(hipStream_t st, uint32_t grid, const oi_mesh_desc
     a.tests = &ctl->tests;
     a.implicit_end = implicit_end;
     const dim3 g(grid), b(TB);
-#define OI_STEP(EE)                                                       \
-    if (first)                                                            \
-        hipLaunchKernelGGL((k_bvtt_step<EE, true>), g, b, 0, st, a);      \
-    else                                                                  \
-        hipLaunchKernelGGL((k_bvtt_step<EE, false>), g, b, 0, st, a);
+#define OI_STEP(EE)                                                        \
+    if (first)                                                             \
+        hipLaunchKernelGGL((k_bvtt_step<EE, true, Tab>), g, b, 0, st, a);  \
+    else                                                                   \
+        hipLaunchKernelGGL((k_bvtt_step<EE, false, Tab>), g, b, 0, st, a);
     switch (expand) {
     case 1: OI_STEP(1) break;
     case 2: OI_STEP(2) break;
@@ -703,13 +735,13 @@ hipError_t oi_launch_bvtt_step(hipStream_t st, uint32_t grid, const oi_mesh_desc
     return hipGetLastError();
 }
 
-hipError_t oi_launch_narrow(hipStream_t st, uint32_t grid, const oi_mesh_desc* d_meshes,
-                            const oi_mesh_desc* h_meshes, uint32_t n_meshes, const uint2* cands,
-                            const uint32_t* cand_count, uint32_t cand_cap, uint4* hits, uint32_t* n_hits,
-                            uint32_t hits_cap, oi_detect_ctl* ctl, int fp64)
+template <class Tab>
+static hipError_t launch_narrow(hipStream_t st, uint32_t grid, const Tab& tab, const uint2* cands,
+                                const uint32_t* cand_count, uint32_t cand_cap, uint4* hits, uint32_t* n_hits,
+                                uint32_t hits_cap, oi_detect_ctl* ctl, int fp64)
 {
-    NarrowArgs a{};
-    a.tab = make_tab(d_meshes, h_meshes, n_meshes);
+    NarrowArgs<Tab> a{};
+    a.tab = tab;
     a.cands = cands;
     a.cand_count = cand_count;
     a.cand_cap = cand_cap;
@@ -720,10 +752,47 @@ hipError_t oi_launch_narrow(hipStream_t st, uint32_t grid, const oi_mesh_desc* d
     a.fp64 = &ctl->fp64;
     a.adj = &ctl->adj;
     if (fp64)
-        hipLaunchKernelGGL(k_narrow<true>, dim3(grid), dim3(TB), 0, st, a);
+        hipLaunchKernelGGL((k_narrow<true, Tab>), dim3(grid), dim3(TB), 0, st, a);
     else
-        hipLaunchKernelGGL(k_narrow<false>, dim3(grid), dim3(TB), 0, st, a);
+        hipLaunchKernelGGL((k_narrow<false, Tab>), dim3(grid), dim3(TB), 0, st, a);
     return hipGetLastError();
+}
+
+extern "C" {
+
+hipError_t oi_launch_seed(hipStream_t st, uint32_t n0, uint32_t* counts, uint32_t n_counts, oi_detect_ctl* ctl,
+                          uint32_t reset_sticky)
+{
+    if (n_counts > 256) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_seed, dim3(1), dim3(256), 0, st, n0, counts, n_counts, ctl, reset_sticky);
+    return hipGetLastError();
+}
+
+// one BVTT step; front_in's length is *count_in, or n_in when count_in is null
+hipError_t oi_launch_bvtt_step(hipStream_t st, uint32_t grid, const oi_tab_args* t, int first,
+                               const uint2* front_in, const uint32_t* count_in, uint32_t n_in, uint32_t cap_in,
+                               uint2* front_out, uint32_t* count_out, uint32_t cap_out, uint2* cands,
+                               uint32_t* cand_count, uint32_t cand_cap, oi_detect_ctl* ctl, uint32_t expand,
+                               uint32_t implicit_end)
+{
+    if (t->chunk_mesh)
+        return launch_step(st, grid, ChunkTab{t->d_meshes, t->chunk_mesh, t->shift}, first, front_in, count_in, n_in,
+                           cap_in, front_out, count_out, cap_out, cands, cand_count, cand_cap, ctl, expand,
+                           implicit_end);
+    return launch_step(st, grid, make_tab(t->d_meshes, t->h_meshes, t->n_meshes), first, front_in, count_in, n_in,
+                       cap_in, front_out, count_out, cap_out, cands, cand_count, cand_cap, ctl, expand,
+                       implicit_end);
+}
+
+hipError_t oi_launch_narrow(hipStream_t st, uint32_t grid, const oi_tab_args* t, const uint2* cands,
+                            const uint32_t* cand_count, uint32_t cand_cap, uint4* hits, uint32_t* n_hits,
+                            uint32_t hits_cap, oi_detect_ctl* ctl, int fp64)
+{
+    if (t->chunk_mesh)
+        return launch_narrow(st, grid, ChunkTab{t->d_meshes, t->chunk_mesh, t->shift}, cands, cand_count, cand_cap,
+                             hits, n_hits, hits_cap, ctl, fp64);
+    return launch_narrow(st, grid, make_tab(t->d_meshes, t->h_meshes, t->n_meshes), cands, cand_count, cand_cap,
+                         hits, n_hits, hits_cap, ctl, fp64);
 }
 
 }  // extern "C"

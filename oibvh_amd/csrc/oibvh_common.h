@@ -68,6 +68,37 @@ OI_HD uint32_t oi_lvl_last_implicit(uint32_t l, uint32_t L, uint32_t vl)
 // device descriptors
 // ---------------------------------------------------------------------------
 
+// p' = M * vec4(p, 1) with glm's evaluation order for mat4 * vec4
+// (type_mat4x4.inl:561-573): (m[0] x + m[1] y) + (m[2] z + m[3] * 1), m[c] the
+// c-th column, stored column-major like glm::mat4 (xf[4c + r]). This is the
+// vertex transform of Mesh::transform (mesh.cpp:189-214, transform.cu:35-41).
+OI_HD void oi_xform(const float* xf, float x, float y, float z, float* o)
+{
+    for (int r = 0; r < 3; r++) {
+        const float a = xf[r] * x;
+        const float b = xf[4 + r] * y;
+        const float c = xf[8 + r] * z;
+        const float d = xf[12 + r] * 1.0f;
+        o[r] = (a + b) + (c + d);
+    }
+}
+
+// One mesh as the build and refit kernels see it.
+struct oi_build_desc {
+    const float* pos;        // V*3 vertex positions
+    const float* xf;         // model matrix applied to pos (oi_xform), or null
+    const uint32_t* faces;   // F*3 original faces (vertex ids local to the mesh)
+    uint32_t* sfaces;        // F*3 faces in leaf order (written by the build, read by the refit)
+    uint32_t* perm;          // F: leaf -> original face id
+    float* nodes;            // N*6 floats, real order
+    float* box;              // 6 floats: the Morton quantisation box (written by the build)
+    uint32_t F, V, L, vl;
+    uint32_t fbase;          // offset in the concatenated per-face sort arrays
+    uint32_t pbase;          // first mesh-box partial of this mesh
+    uint32_t nparts;         // mesh-box partials (1..256)
+    uint32_t vchunk;         // vertices per mesh-box partial
+};
+
 // One mesh as the traversal sees it. `ibase` is the mesh's offset in the
 // scene's concatenated implicit-index space (each mesh spans 2^(L+1) - 1).
 struct oi_mesh_desc {
@@ -75,9 +106,20 @@ struct oi_mesh_desc {
     const uint32_t* sfaces;  // F*3, faces in leaf (Morton) order
     const float* pos;        // V*3 vertex positions
     const uint32_t* perm;    // F: leaf position -> original face id
+    const float* xf;         // model matrix applied to pos, or null
     uint32_t F, L, vl, N;
     uint32_t ibase;
-    uint32_t pad0, pad1, pad2;
+    uint32_t pad0;
+};
+
+// A mesh table is either a scene's (d_meshes / h_meshes, n_meshes) or, when
+// chunk_mesh is not null, a batch's (d_meshes indexed through chunk_mesh).
+struct oi_tab_args {
+    const oi_mesh_desc* d_meshes;
+    const oi_mesh_desc* h_meshes;
+    uint32_t n_meshes;
+    const uint32_t* chunk_mesh;
+    uint32_t shift;
 };
 
 // Detect counters, reset by the seed kernel before each detect.

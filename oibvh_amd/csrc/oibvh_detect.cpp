@@ -1,0 +1,214 @@
+// oibvh_detect.cpp -- the BVTT detector shared by scenes and batches: buffer
+// management, the launch sequence of one query and the overflow re-run loop
+// (Scene::detectCollisionOnGPU, scene.cu:225-446, which instead copies the
+// front size to the host after every level and uses fixed 10M-entry buffers).
+#include "oibvh_internal.h"
+
+template <class T>
+static oibvh_status grow(oibvh_ctx* c, T*& p, uint32_t& cap, uint64_t need, uint64_t& gen)
+{
+    if (cap >= need) return OIBVH_OK;
+    if (need > 0x7fffffffull) return oi_set_err(c, OIBVH_ERR_OVERFLOW, "detect buffer would exceed 2^31 entries");
+    dfree(p);
+    cap = 0;
+    gen++;
+    HIPCHK(c, dalloc(&p, need));
+    cap = (uint32_t)need;
+    return OIBVH_OK;
+}
+
+oibvh_status oi_det_init(oi_detector* d, oibvh_ctx* c)
+{
+    d->ctx = c;
+    hipError_t e = dalloc(&d->ctl, 1);
+    if (e == hipSuccess) e = hipMemsetAsync(d->ctl, 0, sizeof(oi_detect_ctl), c->stream);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&d->h_ctl, sizeof(oi_detect_ctl), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        oi_det_free(d);
+        return oi_set_err(c, OIBVH_ERR_HIP, "detector alloc: %s", hipGetErrorString(e));
+    }
+    std::memset(d->h_ctl, 0, sizeof(oi_detect_ctl));
+    return OIBVH_OK;
+}
+
+void oi_det_free(oi_detector* d)
+{
+    dfree(d->d_init);
+    dfree(d->front[0]);
+    dfree(d->front[1]);
+    dfree(d->counts);
+    dfree(d->cands);
+    dfree(d->hits);
+    dfree(d->ctl);
+    if (d->h_ctl) (void)hipHostFree(d->h_ctl);
+    d->h_ctl = nullptr;
+}
+
+oibvh_status oi_det_set_front(oi_detector* d, const std::vector<uint2>& front)
+{
+    oibvh_ctx* c = d->ctx;
+    d->n_front = (uint32_t)front.size();
+    oibvh_status r = grow(c, d->d_init, d->init_cap, std::max(1u, d->n_front), d->generation);
+    if (r != OIBVH_OK) return r;
+    if (d->n_front)
+        HIPCHK(c, hipMemcpyAsync(d->d_init, front.data(), d->n_front * sizeof(uint2), hipMemcpyHostToDevice,
+                                 c->stream));
+    return OIBVH_OK;
+}
+
+oibvh_status oi_det_prepare(oi_detector* d)
+{
+    oibvh_ctx* c = d->ctx;
+    // every BVTT step takes each non-leaf side up to `expand` levels down and
+    // tests the pairs it reaches: the walk ends at the leaves after
+    // ceil(depth / expand) steps (one step tests entry-level leaf pairs)
+    d->n_levels = std::max(1u, (d->depth + d->expand - 1) / d->expand);
+    if (d->n_levels + 1 > 256) return oi_set_err(c, OIBVH_ERR_INVALID, "detect: %u BVTT steps", d->n_levels);
+    const uint32_t fcap = std::max(c->queue_cap, oi_np2(std::max(1u, d->n_front)));
+    if (d->front_cap < fcap) {
+        dfree(d->front[0]);
+        dfree(d->front[1]);
+        d->front_cap = 0;
+        d->generation++;
+        HIPCHK(c, dalloc(&d->front[0], fcap));
+        HIPCHK(c, dalloc(&d->front[1], fcap));
+        d->front_cap = fcap;
+    }
+    oibvh_status r;
+    if ((r = grow(c, d->counts, d->counts_cap, 256, d->generation)) != OIBVH_OK) return r;
+    if ((r = grow(c, d->cands, d->cand_cap, c->cand_cap, d->generation)) != OIBVH_OK) return r;
+    if ((r = grow(c, d->hits, d->hits_cap, c->hits_cap, d->generation)) != OIBVH_OK) return r;
+    return OIBVH_OK;
+}
+
+oibvh_status oi_det_enqueue(oi_detector* d, bool reset_sticky)
+{
+    oibvh_ctx* c = d->ctx;
+    hipStream_t st = c->stream;
+    HIPCHK(c, oi_launch_seed(st, d->n_front, d->counts, d->n_levels + 1, d->ctl, reset_sticky ? 1u : 0u));
+    const uint32_t grid = oi_grid_for(c);
+    for (uint32_t k = 0; k < d->n_levels; k++) {
+        // step 0 reads the initial front in place; then the fronts alternate
+        const uint2* in = k ? d->front[(k - 1) & 1] : d->d_init;
+        HIPCHK(c, oi_launch_bvtt_step(st, grid, &d->tab, k == 0, in, k ? d->counts + k : nullptr, d->n_front,
+                                      k ? d->front_cap : d->n_front, d->front[k & 1], d->counts + k + 1,
+                                      d->front_cap, d->cands, &d->ctl->cands, d->cand_cap, d->ctl, d->expand,
+                                      d->implicit_end));
+    }
+    HIPCHK(c, oi_launch_narrow(st, grid, &d->tab, d->cands, &d->ctl->cands, d->cand_cap, d->hits, &d->ctl->n_hits,
+                               d->hits_cap, d->ctl, (d->flags & OIBVH_DETECT_FP64_FALLBACK) ? 1 : 0));
+    d->unchecked = true;
+    d->have_result = true;
+    return OIBVH_OK;
+}
+
+oibvh_status oi_det_check(oi_detector* d)
+{
+    oibvh_ctx* c = d->ctx;
+    if (!d->unchecked) return OIBVH_OK;
+    uint32_t retries = 0, earlier = 0;
+    for (;;) {
+        d->h_counts.resize(d->n_levels + 1);
+        HIPCHK(c, hipMemcpyAsync(d->h_ctl, d->ctl, sizeof(oi_detect_ctl), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(d->h_counts.data(), d->counts, (d->n_levels + 1) * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        const oi_detect_ctl& k = *d->h_ctl;
+        earlier |= k.sticky;  // errors of unchecked asynchronous detects before this one
+        if (d->h_counts[d->n_levels] != 0) {
+            // the last step reaches the leaves on every side: it writes no front
+            d->unchecked = false;
+            d->have_result = false;
+            return oi_set_err(c, OIBVH_ERR_HIP, "detect: %u BVTT pairs left after the last step",
+                              d->h_counts[d->n_levels]);
+        }
+        if (k.error & OI_ERR_CORRUPT) {
+            d->unchecked = false;
+            d->have_result = false;
+            return oi_set_err(c, OIBVH_ERR_HIP, "detect: corrupt BVTT entry (error bits 0x%x)", k.error);
+        }
+        bool rerun = false;
+        oibvh_status r = OIBVH_OK;
+        uint32_t maxfront = 0;
+        for (uint32_t v : d->h_counts) maxfront = std::max(maxfront, v);
+        if (k.error & OI_ERR_QUEUE_OVERFLOW) {
+            const uint64_t need = oi_np2(std::max(maxfront, d->front_cap * 4u));
+            if (need > (1u << 28)) {
+                d->unchecked = false;
+                d->have_result = false;
+                return oi_set_err(c, OIBVH_ERR_OVERFLOW, "BVTT front exceeds %u entries", 1u << 28);
+            }
+            dfree(d->front[0]);
+            dfree(d->front[1]);
+            d->front_cap = 0;
+            d->generation++;
+            HIPCHK(c, dalloc(&d->front[0], need));
+            HIPCHK(c, dalloc(&d->front[1], need));
+            d->front_cap = (uint32_t)need;
+            rerun = true;
+        }
+        if ((k.error & OI_ERR_CAND_OVERFLOW) || k.cands > d->cand_cap) {
+            r = grow(c, d->cands, d->cand_cap, std::max<uint64_t>(k.cands, 2ull * d->cand_cap), d->generation);
+            if (r != OIBVH_OK) return r;
+            rerun = true;
+        }
+        if ((k.error & OI_ERR_HITS_OVERFLOW) || k.n_hits > d->hits_cap) {
+            const uint64_t need = std::max<uint64_t>((uint64_t)k.n_hits + (k.n_hits >> 3), 2ull * d->hits_cap);
+            if ((r = grow(c, d->hits, d->hits_cap, need, d->generation)) != OIBVH_OK) return r;
+            rerun = true;
+        }
+        if (!rerun) {
+            d->n_pairs = k.n_hits;
+            d->stats.bvtt_tests = k.tests;
+            d->stats.candidates = k.cands;
+            d->stats.fp64_rechecks = k.fp64;
+            d->stats.adjacent = k.adj;
+            d->stats.pairs = k.n_hits;
+            d->stats.retries = retries;
+            d->stats.async_errors = earlier;
+            d->unchecked = false;
+            return OIBVH_OK;
+        }
+        retries++;
+        if (retries > 16) return oi_set_err(c, OIBVH_ERR_OVERFLOW, "detect: too many retries");
+        if ((r = oi_det_prepare(d)) != OIBVH_OK) return r;
+        if ((r = oi_det_enqueue(d, true)) != OIBVH_OK) return r;
+    }
+}
+
+oibvh_status oi_det_get_pairs(oi_detector* d, uint32_t* out, uint64_t cap, uint32_t* n)
+{
+    oibvh_ctx* c = d->ctx;
+    oibvh_status r = oi_det_check(d);
+    if (r != OIBVH_OK) return r;
+    if (n) *n = d->n_pairs;
+    if (d->n_pairs == 0) return OIBVH_OK;
+    if (!out || cap < d->n_pairs) return oi_set_err(c, OIBVH_ERR_CAPACITY, "need %u pairs", d->n_pairs);
+    std::vector<uint4> h(d->n_pairs);
+    HIPCHK(c, hipMemcpyAsync(h.data(), d->hits, d->n_pairs * sizeof(uint4), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::sort(h.begin(), h.end(), [](const uint4& a, const uint4& b) {
+        if (a.x != b.x) return a.x < b.x;
+        if (a.y != b.y) return a.y < b.y;
+        if (a.z != b.z) return a.z < b.z;
+        return a.w < b.w;
+    });
+    for (uint32_t i = 0; i < d->n_pairs; i++) {
+        out[4ull * i] = h[i].x;
+        out[4ull * i + 1] = h[i].y;
+        out[4ull * i + 2] = h[i].z;
+        out[4ull * i + 3] = h[i].w;
+    }
+    return OIBVH_OK;
+}
+
+oibvh_status oi_det_level_counts(oi_detector* d, uint32_t* out, uint64_t cap, uint32_t* n)
+{
+    oibvh_status r = oi_det_check(d);
+    if (r != OIBVH_OK) return r;
+    *n = d->have_result ? d->n_levels : 0;
+    if (*n == 0) return OIBVH_OK;
+    if (!out || cap < *n) return oi_set_err(d->ctx, OIBVH_ERR_CAPACITY, "need %u entries", *n);
+    std::memcpy(out, d->h_counts.data(), *n * sizeof(uint32_t));
+    return OIBVH_OK;
+}

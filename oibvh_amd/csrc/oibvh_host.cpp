@@ -4,53 +4,9 @@
 // from the inputs (the reference uses fixed 10M-element buffers,
 // oibvhTree.cu:11-13, scene.cu:15-21), every launch goes to the context's
 // stream and only result accessors synchronise.
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <cstdarg>
-#include <cstdio>
-#include <cstring>
 #include <new>
-#include <string>
-#include <vector>
 
-#include "oibvh.h"
-#include "oibvh_common.h"
-
-extern "C" {
-hipError_t oi_launch_mesh_box(hipStream_t, const float*, uint32_t, float*, uint32_t, uint32_t*, uint32_t);
-hipError_t oi_launch_morton(hipStream_t, const uint32_t*, const float*, uint32_t, const float*, uint32_t, float*,
-                            uint32_t*, uint32_t*, uint32_t);
-uint32_t oi_sort_tile();
-hipError_t oi_launch_sort_pass(hipStream_t, const uint32_t*, const uint32_t*, uint32_t*, uint32_t*, uint32_t,
-                               uint32_t, const uint32_t*, uint32_t*, uint32_t*, uint32_t*);
-hipError_t oi_launch_reduce(hipStream_t, int, const float*, const uint32_t*, const uint32_t*, uint32_t*, float*,
-                            uint32_t, uint32_t, uint32_t);
-hipError_t oi_launch_seed(hipStream_t, uint32_t, uint32_t*, uint32_t, oi_detect_ctl*, uint32_t);
-hipError_t oi_launch_bvtt_step(hipStream_t, uint32_t, const oi_mesh_desc*, const oi_mesh_desc*, uint32_t, int,
-                               const uint2*, const uint32_t*, uint32_t, uint32_t, uint2*, uint32_t*, uint32_t, uint2*,
-                               uint32_t*, uint32_t, oi_detect_ctl*, uint32_t, uint32_t);
-hipError_t oi_launch_narrow(hipStream_t, uint32_t, const oi_mesh_desc*, const oi_mesh_desc*, uint32_t, const uint2*,
-                            const uint32_t*, uint32_t, uint4*, uint32_t*, uint32_t, oi_detect_ctl*, int);
-}
-
-// ---------------------------------------------------------------------------
-struct oibvh_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    std::string err;
-    int cus = 0;
-    uint64_t clock_khz = 100000;
-    uint32_t leaves_per_thread = 0;
-    uint32_t trav_blocks = 0;
-    uint32_t trav_expand = 0;
-    uint32_t queue_cap = 1u << 22;
-    uint32_t hits_cap = 1u << 20;
-    uint32_t cand_cap = 1u << 20;
-    int live = 0;                // trees and scenes of this context not yet destroyed
-    bool destroy_pending = false; // oibvh_ctx_destroy called while objects were live
-};
+#include "oibvh_internal.h"
 
 static void ctx_free(oibvh_ctx* ctx)
 {
@@ -62,13 +18,27 @@ static void ctx_free(oibvh_ctx* ctx)
     delete ctx;
 }
 
-// a context outlives its trees and scenes: destroying it while they are live
-// only marks it, and the last object to go frees it
-static void ctx_release(oibvh_ctx* ctx)
+// a context outlives its trees, scenes and batches: destroying it while they
+// are live only marks it, and the last object to go frees it
+void oi_ctx_release(oibvh_ctx* ctx)
 {
     if (--ctx->live == 0 && ctx->destroy_pending) ctx_free(ctx);
 }
 
+oibvh_status oi_set_err(oibvh_ctx* ctx, oibvh_status st, const char* fmt, ...)
+{
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return st;
+}
+
+// ---------------------------------------------------------------------------
 struct oibvh_tree {
     oibvh_ctx* ctx = nullptr;
     uint32_t F = 0, V = 0, N = 0, L = 0, vl = 0;
@@ -85,6 +55,8 @@ struct oibvh_tree {
     uint32_t* sort_scratch = nullptr;
     uint32_t tiles = 0;
     float* box = nullptr;        // 6 floats + per-block partial boxes
+    float* xf = nullptr;         // model matrix (16 floats, device)
+    bool has_xf = false;
     bool built = false;
 };
 
@@ -94,72 +66,11 @@ struct oibvh_scene {
     oi_mesh_desc* d_desc = nullptr;
     uint32_t desc_cap = 0;
     std::vector<oi_mesh_desc> h_desc;  // host copies stay alive until the next detect
-    uint32_t implicit_end = 0;
-    uint2* d_init = nullptr;           // initial BVTT front
-    uint32_t init_cap = 0;
-    uint32_t n_front = 0;
     std::vector<uint2> h_front;
     uint32_t front_key[3] = {~0u, ~0u, ~0u}; // entry, flags&SELF, #trees
-    uint2* front[2] = {nullptr, nullptr};     // ping-pong BVTT fronts
-    uint32_t front_cap = 0;
-    uint32_t* counts = nullptr;        // front length per level
-    uint32_t counts_cap = 0;
-    uint32_t n_levels = 0;
-    uint2* cands = nullptr;
-    uint32_t cand_cap = 0;
-    uint4* hits = nullptr;
-    uint32_t hits_cap = 0;
-    oi_detect_ctl* ctl = nullptr;
-    oi_detect_ctl* h_ctl = nullptr;    // pinned
-    std::vector<uint32_t> h_counts;
-    // last detect
+    oi_detector det;
     uint32_t last_entry = 0, last_expand = 1, last_flags = 0;
-    bool unchecked = false;
-    bool have_result = false;
-    uint32_t n_pairs = 0;
-    oibvh_detect_stats stats{};
 };
-
-// ---------------------------------------------------------------------------
-static oibvh_status set_err(oibvh_ctx* ctx, oibvh_status st, const char* fmt, ...)
-{
-    if (ctx) {
-        char buf[512];
-        va_list ap;
-        va_start(ap, fmt);
-        vsnprintf(buf, sizeof buf, fmt, ap);
-        va_end(ap);
-        ctx->err = buf;
-    }
-    return st;
-}
-
-#define HIPCHK(ctx, expr)                                                                          \
-    do {                                                                                           \
-        hipError_t e_ = (expr);                                                                    \
-        if (e_ != hipSuccess)                                                                      \
-            return set_err((ctx), OIBVH_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));   \
-    } while (0)
-
-template <class T>
-static hipError_t dalloc(T** p, size_t n)
-{
-    *p = nullptr;
-    if (n == 0) n = 1;
-    return hipMalloc((void**)p, n * sizeof(T));
-}
-
-template <class T>
-static void dfree(T*& p)
-{
-    if (p) (void)hipFree((void*)p);
-    p = nullptr;
-}
-
-static hipMemcpyKind kind_of(uint32_t ptr_kind)
-{
-    return ptr_kind == OIBVH_PTR_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
-}
 
 // ---------------------------------------------------------------- context
 extern "C" oibvh_status oibvh_ctx_create(int device, void* stream, oibvh_ctx** out)
@@ -227,10 +138,10 @@ extern "C" oibvh_status oibvh_ctx_set_tuning(oibvh_ctx* ctx, const oibvh_tuning*
     if (!ctx || !t) return OIBVH_ERR_INVALID;
     const uint32_t lpt = t->reduce_leaves_per_thread;
     if (lpt != 0 && lpt != 1 && lpt != 2 && lpt != 4 && lpt != 8)
-        return set_err(ctx, OIBVH_ERR_INVALID, "reduce_leaves_per_thread must be 0, 1, 2, 4 or 8");
+        return oi_set_err(ctx, OIBVH_ERR_INVALID, "reduce_leaves_per_thread must be 0, 1, 2, 4 or 8");
     if (t->front_capacity && (t->front_capacity & (t->front_capacity - 1)))
-        return set_err(ctx, OIBVH_ERR_INVALID, "front_capacity must be a power of two");
-    if (t->traverse_expand > 3) return set_err(ctx, OIBVH_ERR_INVALID, "traverse_expand must be 0..3");
+        return oi_set_err(ctx, OIBVH_ERR_INVALID, "front_capacity must be a power of two");
+    if (t->traverse_expand > 3) return oi_set_err(ctx, OIBVH_ERR_INVALID, "traverse_expand must be 0..3");
     ctx->leaves_per_thread = lpt;
     ctx->trav_blocks = t->traverse_blocks;
     ctx->trav_expand = t->traverse_expand;
@@ -265,6 +176,7 @@ static void tree_free(oibvh_tree* t)
     dfree(t->vals_tmp);
     dfree(t->sort_scratch);
     dfree(t->box);
+    dfree(t->xf);
 }
 
 static oibvh_status tree_alloc_common(oibvh_tree* t)
@@ -278,8 +190,34 @@ static oibvh_status tree_alloc_common(oibvh_tree* t)
     HIPCHK(c, dalloc(&t->sfaces, 3ull * t->F));
     HIPCHK(c, dalloc(&t->perm, t->F));
     HIPCHK(c, dalloc(&t->nodes, 6ull * t->N));
+    HIPCHK(c, dalloc(&t->box, 6ull + 6ull * 256));
+    HIPCHK(c, dalloc(&t->xf, 16));
     t->pos = t->pos_own;
     return OIBVH_OK;
+}
+
+static uint32_t box_blocks(const oibvh_tree* t) { return std::max(1u, std::min(256u, (t->V + 4095u) / 4096u)); }
+
+// the tree as a one-mesh build table
+static oi_build_desc tree_desc(const oibvh_tree* t)
+{
+    oi_build_desc d{};
+    d.pos = t->pos;
+    d.xf = t->has_xf ? t->xf : nullptr;
+    d.faces = t->faces;
+    d.sfaces = t->sfaces;
+    d.perm = t->perm;
+    d.nodes = t->nodes;
+    d.box = t->box;
+    d.F = t->F;
+    d.V = t->V;
+    d.L = t->L;
+    d.vl = t->vl;
+    d.fbase = 0;
+    d.pbase = 0;
+    d.nparts = box_blocks(t);
+    d.vchunk = (t->V + d.nparts - 1) / d.nparts;
+    return d;
 }
 
 static oibvh_status set_positions(oibvh_tree* t, const float* xyz, uint32_t ptr_kind)
@@ -300,12 +238,12 @@ extern "C" oibvh_status oibvh_tree_create(oibvh_ctx* ctx, const float* xyz, uint
     if (!out) return OIBVH_ERR_INVALID;
     *out = nullptr;
     if (!ctx || !xyz || !idx || V == 0 || F == 0 || ptr_kind > OIBVH_PTR_DEVICE_BORROW)
-        return set_err(ctx, OIBVH_ERR_INVALID, "oibvh_tree_create: invalid argument");
+        return oi_set_err(ctx, OIBVH_ERR_INVALID, "oibvh_tree_create: invalid argument");
     if (F > (1u << 24))
-        return set_err(ctx, OIBVH_ERR_INVALID, "oibvh_tree_create: at most 2^24 faces per tree");
+        return oi_set_err(ctx, OIBVH_ERR_INVALID, "oibvh_tree_create: at most 2^24 faces per tree");
     if (ptr_kind == OIBVH_PTR_HOST) {
         for (uint64_t i = 0; i < 3ull * F; i++)
-            if (idx[i] >= V) return set_err(ctx, OIBVH_ERR_INVALID, "face index %u out of range", idx[i]);
+            if (idx[i] >= V) return oi_set_err(ctx, OIBVH_ERR_INVALID, "face index %u out of range", idx[i]);
     }
     HIPCHK(ctx, hipSetDevice(ctx->device));
     oibvh_tree* t = new (std::nothrow) oibvh_tree;
@@ -318,13 +256,13 @@ extern "C" oibvh_status oibvh_tree_create(oibvh_ctx* ctx, const float* xyz, uint
         hipError_t e = hipMemcpyAsync(t->faces, idx, 12ull * F,
                                       ptr_kind == OIBVH_PTR_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice,
                                       ctx->stream);
-        if (e != hipSuccess) st = set_err(ctx, OIBVH_ERR_HIP, "face upload: %s", hipGetErrorString(e));
+        if (e != hipSuccess) st = oi_set_err(ctx, OIBVH_ERR_HIP, "face upload: %s", hipGetErrorString(e));
     }
     if (st == OIBVH_OK) st = set_positions(t, xyz, ptr_kind);
     if (st == OIBVH_OK && ptr_kind == OIBVH_PTR_HOST) {
         // host inputs may be freed by the caller right after return
         hipError_t e = hipStreamSynchronize(ctx->stream);
-        if (e != hipSuccess) st = set_err(ctx, OIBVH_ERR_HIP, "upload: %s", hipGetErrorString(e));
+        if (e != hipSuccess) st = oi_set_err(ctx, OIBVH_ERR_HIP, "upload: %s", hipGetErrorString(e));
     }
     if (st != OIBVH_OK) {
         tree_free(t);
@@ -344,7 +282,7 @@ extern "C" oibvh_status oibvh_tree_create_like(oibvh_tree* other, const float* x
     if (!other) return OIBVH_ERR_INVALID;
     oibvh_ctx* ctx = other->ctx;
     if (!xyz || V != other->V || ptr_kind > OIBVH_PTR_DEVICE_BORROW)
-        return set_err(ctx, OIBVH_ERR_INVALID, "oibvh_tree_create_like: invalid argument");
+        return oi_set_err(ctx, OIBVH_ERR_INVALID, "oibvh_tree_create_like: invalid argument");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     oibvh_tree* t = new (std::nothrow) oibvh_tree;
     if (!t) return OIBVH_ERR_INVALID;
@@ -358,12 +296,12 @@ extern "C" oibvh_status oibvh_tree_create_like(oibvh_tree* other, const float* x
         if (e == hipSuccess) e = hipMemcpyAsync(t->sfaces, other->sfaces, 12ull * t->F, hipMemcpyDeviceToDevice, s);
         if (e == hipSuccess) e = hipMemcpyAsync(t->perm, other->perm, 4ull * t->F, hipMemcpyDeviceToDevice, s);
         if (e == hipSuccess) e = hipMemcpyAsync(t->nodes, other->nodes, 24ull * t->N, hipMemcpyDeviceToDevice, s);
-        if (e != hipSuccess) st = set_err(ctx, OIBVH_ERR_HIP, "create_like copy: %s", hipGetErrorString(e));
+        if (e != hipSuccess) st = oi_set_err(ctx, OIBVH_ERR_HIP, "create_like copy: %s", hipGetErrorString(e));
     }
     if (st == OIBVH_OK) st = set_positions(t, xyz, ptr_kind);
     if (st == OIBVH_OK && ptr_kind == OIBVH_PTR_HOST) {
         hipError_t e = hipStreamSynchronize(s);
-        if (e != hipSuccess) st = set_err(ctx, OIBVH_ERR_HIP, "upload: %s", hipGetErrorString(e));
+        if (e != hipSuccess) st = oi_set_err(ctx, OIBVH_ERR_HIP, "upload: %s", hipGetErrorString(e));
     }
     if (st != OIBVH_OK) {
         tree_free(t);
@@ -384,7 +322,7 @@ extern "C" void oibvh_tree_destroy(oibvh_tree* t)
     (void)hipStreamSynchronize(c->stream);
     tree_free(t);
     delete t;
-    ctx_release(c);
+    oi_ctx_release(c);
 }
 
 static int pick_lpt(const oibvh_tree* t)
@@ -401,9 +339,12 @@ static int pick_lpt(const oibvh_tree* t)
 static oibvh_status launch_reduce(oibvh_tree* t, bool build)
 {
     oibvh_ctx* c = t->ctx;
-    HIPCHK(c, oi_launch_reduce(c->stream, pick_lpt(t), t->pos, build ? t->faces : t->sfaces,
-                               build ? t->perm : nullptr, build ? t->sfaces : nullptr, t->nodes, t->F, t->L,
-                               t->vl));
+    const int M = pick_lpt(t);
+    const uint32_t h = oi_reduce_height(M, t->L);
+    const oi_build_desc d = tree_desc(t);
+    HIPCHK(c, oi_launch_reduce(c->stream, M, nullptr, nullptr, &d, oi_lvl_cnt(t->L - h, t->L, t->vl),
+                               build ? t->perm : nullptr, 0));
+    if (t->L > h) HIPCHK(c, oi_launch_reduce_apex(c->stream, M, nullptr, nullptr, &d, 1));
     return OIBVH_OK;
 }
 
@@ -415,7 +356,6 @@ extern "C" oibvh_status oibvh_tree_build(oibvh_tree* t)
     const uint32_t F = t->F;
     const uint32_t tile = oi_sort_tile();
     const uint32_t tiles = (F + tile - 1) / tile;
-    const uint32_t box_blocks = std::max(1u, std::min(256u, (t->V + 4095u) / 4096u));
     if (!t->keys) {
         HIPCHK(c, dalloc(&t->keys, F));
         HIPCHK(c, dalloc(&t->keys_alt, F));
@@ -423,26 +363,43 @@ extern "C" oibvh_status oibvh_tree_build(oibvh_tree* t)
         t->tiles = tiles;
         // ghist[1024] | tile counters[8] | lookback0[tiles*256] | lookback1[tiles*256]
         HIPCHK(c, dalloc(&t->sort_scratch, 1032ull + 512ull * tiles));
-        HIPCHK(c, dalloc(&t->box, 6ull + 6ull * 256));
     }
     uint32_t* ghist = t->sort_scratch;
     uint32_t* tctr = t->sort_scratch + 1024;
     uint32_t* lb0 = t->sort_scratch + 1032;
     uint32_t* lb1 = lb0 + 256ull * tiles;
     hipStream_t s = c->stream;
-    HIPCHK(c, oi_launch_mesh_box(s, t->pos, t->V, t->box + 6, box_blocks, t->sort_scratch, 1032u + 256u * tiles));
-    const uint32_t mblocks = std::max(1u, std::min((F + 255u) / 256u, (uint32_t)std::max(1, c->cus) * 8u));
-    HIPCHK(c, oi_launch_morton(s, t->faces, t->pos, F, t->box + 6, box_blocks, t->box, t->keys, ghist, mblocks));
-    HIPCHK(c, oi_launch_sort_pass(s, t->keys, nullptr, t->keys_alt, t->vals_tmp, F, 0, ghist, lb0, lb1, tctr + 0));
+    const oi_build_desc d = tree_desc(t);
+    HIPCHK(c, oi_launch_mesh_box(s, nullptr, nullptr, &d, d.nparts, t->box + 6, t->sort_scratch, 1032u + 256u * tiles));
+    const uint32_t mblocks = (F + oi_morton_chunk() - 1) / oi_morton_chunk();
+    HIPCHK(c, oi_launch_morton(s, nullptr, nullptr, &d, mblocks, t->box + 6, t->keys, ghist, nullptr));
+    HIPCHK(c, oi_launch_sort_pass(s, t->keys, nullptr, t->keys_alt, t->vals_tmp, F, 0, ghist, lb0, lb1, tctr + 0,
+                                  nullptr));
     HIPCHK(c, oi_launch_sort_pass(s, t->keys_alt, t->vals_tmp, t->keys, t->perm, F, 8, ghist + 256, lb1, lb0,
-                                  tctr + 1));
+                                  tctr + 1, nullptr));
     HIPCHK(c, oi_launch_sort_pass(s, t->keys, t->perm, t->keys_alt, t->vals_tmp, F, 16, ghist + 512, lb0, lb1,
-                                  tctr + 2));
+                                  tctr + 2, nullptr));
     HIPCHK(c, oi_launch_sort_pass(s, t->keys_alt, t->vals_tmp, nullptr, t->perm, F, 24, ghist + 768, lb1, nullptr,
-                                  tctr + 3));
+                                  tctr + 3, nullptr));
     oibvh_status st = launch_reduce(t, true);
     if (st != OIBVH_OK) return st;
     t->built = true;
+    return OIBVH_OK;
+}
+
+// Mesh::transform (mesh.cpp:189-214): every later build, refit and detect
+// sees positions M * (x, y, z, 1); null restores the identity
+extern "C" oibvh_status oibvh_tree_set_transform(oibvh_tree* t, const float* mat4)
+{
+    if (!t) return OIBVH_ERR_INVALID;
+    oibvh_ctx* c = t->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (!mat4) {
+        t->has_xf = false;
+        return OIBVH_OK;
+    }
+    HIPCHK(c, oi_launch_set_xf(c->stream, t->xf, mat4));
+    t->has_xf = true;
     return OIBVH_OK;
 }
 
@@ -450,8 +407,8 @@ extern "C" oibvh_status oibvh_tree_refit(oibvh_tree* t, const float* xyz, uint32
 {
     if (!t) return OIBVH_ERR_INVALID;
     oibvh_ctx* c = t->ctx;
-    if (ptr_kind > OIBVH_PTR_DEVICE_BORROW) return set_err(c, OIBVH_ERR_INVALID, "bad ptr_kind");
-    if (!t->built) return set_err(c, OIBVH_ERR_NOT_BUILT, "refit before build");
+    if (ptr_kind > OIBVH_PTR_DEVICE_BORROW) return oi_set_err(c, OIBVH_ERR_INVALID, "bad ptr_kind");
+    if (!t->built) return oi_set_err(c, OIBVH_ERR_NOT_BUILT, "refit before build");
     HIPCHK(c, hipSetDevice(c->device));
     if (xyz) {
         oibvh_status st = set_positions(t, xyz, ptr_kind);
@@ -466,7 +423,7 @@ extern "C" oibvh_status oibvh_tree_set_positions(oibvh_tree* t, const float* xyz
 {
     if (!t || !xyz) return OIBVH_ERR_INVALID;
     oibvh_ctx* c = t->ctx;
-    if (ptr_kind > OIBVH_PTR_DEVICE_BORROW) return set_err(c, OIBVH_ERR_INVALID, "bad ptr_kind");
+    if (ptr_kind > OIBVH_PTR_DEVICE_BORROW) return oi_set_err(c, OIBVH_ERR_INVALID, "bad ptr_kind");
     HIPCHK(c, hipSetDevice(c->device));
     oibvh_status st = set_positions(t, xyz, ptr_kind);
     if (st == OIBVH_OK && ptr_kind == OIBVH_PTR_HOST) HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -514,31 +471,31 @@ extern "C" oibvh_status oibvh_tree_get_nodes(oibvh_tree* t, float* out, uint64_t
 {
     if (!t) return OIBVH_ERR_INVALID;
     if (n) *n = t->N;
-    if (!t->built) return set_err(t->ctx, OIBVH_ERR_NOT_BUILT, "tree not built");
-    if (!out || cap < t->N) return set_err(t->ctx, OIBVH_ERR_CAPACITY, "need %u nodes", t->N);
+    if (!t->built) return oi_set_err(t->ctx, OIBVH_ERR_NOT_BUILT, "tree not built");
+    if (!out || cap < t->N) return oi_set_err(t->ctx, OIBVH_ERR_CAPACITY, "need %u nodes", t->N);
     return readback(t, out, t->nodes, 24ull * t->N);
 }
 
 extern "C" oibvh_status oibvh_tree_get_sorted_faces(oibvh_tree* t, uint32_t* out, uint64_t cap)
 {
     if (!t) return OIBVH_ERR_INVALID;
-    if (!t->built) return set_err(t->ctx, OIBVH_ERR_NOT_BUILT, "tree not built");
-    if (!out || cap < t->F) return set_err(t->ctx, OIBVH_ERR_CAPACITY, "need %u faces", t->F);
+    if (!t->built) return oi_set_err(t->ctx, OIBVH_ERR_NOT_BUILT, "tree not built");
+    if (!out || cap < t->F) return oi_set_err(t->ctx, OIBVH_ERR_CAPACITY, "need %u faces", t->F);
     return readback(t, out, t->sfaces, 12ull * t->F);
 }
 
 extern "C" oibvh_status oibvh_tree_get_perm(oibvh_tree* t, uint32_t* out, uint64_t cap)
 {
     if (!t) return OIBVH_ERR_INVALID;
-    if (!t->built) return set_err(t->ctx, OIBVH_ERR_NOT_BUILT, "tree not built");
-    if (!out || cap < t->F) return set_err(t->ctx, OIBVH_ERR_CAPACITY, "need %u faces", t->F);
+    if (!t->built) return oi_set_err(t->ctx, OIBVH_ERR_NOT_BUILT, "tree not built");
+    if (!out || cap < t->F) return oi_set_err(t->ctx, OIBVH_ERR_CAPACITY, "need %u faces", t->F);
     return readback(t, out, t->perm, 4ull * t->F);
 }
 
 extern "C" oibvh_status oibvh_tree_get_positions(oibvh_tree* t, float* out, uint64_t cap)
 {
     if (!t) return OIBVH_ERR_INVALID;
-    if (!out || cap < 3ull * t->V) return set_err(t->ctx, OIBVH_ERR_CAPACITY, "need %u floats", 3 * t->V);
+    if (!out || cap < 3ull * t->V) return oi_set_err(t->ctx, OIBVH_ERR_CAPACITY, "need %u floats", 3 * t->V);
     return readback(t, out, t->pos, 12ull * t->V);
 }
 
@@ -559,15 +516,11 @@ extern "C" oibvh_status oibvh_scene_create(oibvh_ctx* ctx, oibvh_scene** out)
     oibvh_scene* s = new (std::nothrow) oibvh_scene;
     if (!s) return OIBVH_ERR_INVALID;
     s->ctx = ctx;
-    hipError_t e = dalloc(&s->ctl, 1);
-    if (e == hipSuccess) e = hipMemsetAsync(s->ctl, 0, sizeof(oi_detect_ctl), ctx->stream);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_ctl, sizeof(oi_detect_ctl), hipHostMallocDefault);
-    if (e != hipSuccess) {
-        dfree(s->ctl);
+    oibvh_status r = oi_det_init(&s->det, ctx);
+    if (r != OIBVH_OK) {
         delete s;
-        return set_err(ctx, OIBVH_ERR_HIP, "scene alloc: %s", hipGetErrorString(e));
+        return r;
     }
-    std::memset(s->h_ctl, 0, sizeof(oi_detect_ctl));
     ctx->live++;
     *out = s;
     return OIBVH_OK;
@@ -579,26 +532,19 @@ extern "C" void oibvh_scene_destroy(oibvh_scene* s)
     (void)hipSetDevice(s->ctx->device);
     (void)hipStreamSynchronize(s->ctx->stream);
     dfree(s->d_desc);
-    dfree(s->d_init);
-    dfree(s->front[0]);
-    dfree(s->front[1]);
-    dfree(s->counts);
-    dfree(s->cands);
-    dfree(s->hits);
-    dfree(s->ctl);
-    if (s->h_ctl) (void)hipHostFree(s->h_ctl);
+    oi_det_free(&s->det);
     oibvh_ctx* c = s->ctx;
     delete s;
-    ctx_release(c);
+    oi_ctx_release(c);
 }
 
 extern "C" oibvh_status oibvh_scene_add_tree(oibvh_scene* s, oibvh_tree* t)
 {
     if (!s || !t) return OIBVH_ERR_INVALID;
-    if (t->ctx != s->ctx) return set_err(s->ctx, OIBVH_ERR_INVALID, "tree from another context");
-    if (!t->built) return set_err(s->ctx, OIBVH_ERR_NOT_BUILT, "addOibvhTree: tree not built");
+    if (t->ctx != s->ctx) return oi_set_err(s->ctx, OIBVH_ERR_INVALID, "tree from another context");
+    if (!t->built) return oi_set_err(s->ctx, OIBVH_ERR_NOT_BUILT, "addOibvhTree: tree not built");
     s->trees.push_back(t);
-    s->have_result = false;
+    s->det.have_result = false;
     return OIBVH_OK;
 }
 
@@ -636,54 +582,36 @@ static void make_front(oibvh_scene* s, uint32_t entry, bool self)
     }
 }
 
-template <class T>
-static oibvh_status grow(oibvh_ctx* c, T*& p, uint32_t& cap, uint64_t need)
-{
-    if (cap >= need) return OIBVH_OK;
-    if (need > 0x7fffffffull) return set_err(c, OIBVH_ERR_OVERFLOW, "detect buffer would exceed 2^31 entries");
-    dfree(p);
-    cap = 0;
-    HIPCHK(c, dalloc(&p, need));
-    cap = (uint32_t)need;
-    return OIBVH_OK;
-}
-
-static uint32_t grid_for(const oibvh_ctx* c)
-{
-    return c->trav_blocks ? c->trav_blocks : (uint32_t)std::max(1, c->cus) * 4u;
-}
-
 static oibvh_status enqueue_detect(oibvh_scene* s, bool reset_sticky)
 {
     oibvh_ctx* c = s->ctx;
-    hipStream_t st = c->stream;
+    oi_detector& d = s->det;
     const uint32_t n = (uint32_t)s->trees.size();
     // mesh table (positions may be borrowed and change between frames)
     std::vector<oi_mesh_desc>& desc = s->h_desc;
     desc.resize(n);
     uint64_t acc = 0;
     uint32_t depth = 0;
-    const uint32_t expand = c->trav_expand ? c->trav_expand : std::min(3u, std::max(1u, s->last_expand));
     for (uint32_t m = 0; m < n; m++) {
         const oibvh_tree* t = s->trees[m];
-        oi_mesh_desc& d = desc[m];
-        std::memset(&d, 0, sizeof d);
-        d.nodes = t->nodes;
-        d.sfaces = t->sfaces;
-        d.pos = t->pos;
-        d.perm = t->perm;
-        d.F = t->F;
-        d.L = t->L;
-        d.vl = t->vl;
-        d.N = t->N;
-        d.ibase = (uint32_t)acc;
+        oi_mesh_desc& md = desc[m];
+        std::memset(&md, 0, sizeof md);
+        md.nodes = t->nodes;
+        md.sfaces = t->sfaces;
+        md.pos = t->pos;
+        md.perm = t->perm;
+        md.xf = t->has_xf ? t->xf : nullptr;
+        md.F = t->F;
+        md.L = t->L;
+        md.vl = t->vl;
+        md.N = t->N;
+        md.ibase = (uint32_t)acc;
         acc += (2ull << t->L) - 1ull;
         depth = std::max(depth, t->L - std::min(t->L, s->last_entry));
     }
     if (acc > OI_IMPLICIT_LIMIT)
-        return set_err(c, OIBVH_ERR_INVALID, "scene too large: %llu implicit BVTT indices (limit %u)",
-                       (unsigned long long)acc, OI_IMPLICIT_LIMIT);
-    s->implicit_end = (uint32_t)acc;
+        return oi_set_err(c, OIBVH_ERR_INVALID, "scene too large: %llu implicit BVTT indices (limit %u)",
+                          (unsigned long long)acc, OI_IMPLICIT_LIMIT);
     if (n > 4) {  // small scenes pass the table in the kernel arguments
         if (s->desc_cap < n) {
             dfree(s->d_desc);
@@ -691,128 +619,25 @@ static oibvh_status enqueue_detect(oibvh_scene* s, bool reset_sticky)
             HIPCHK(c, dalloc(&s->d_desc, n));
             s->desc_cap = n;
         }
-        HIPCHK(c, hipMemcpyAsync(s->d_desc, desc.data(), n * sizeof(oi_mesh_desc), hipMemcpyHostToDevice, st));
+        HIPCHK(c, hipMemcpyAsync(s->d_desc, desc.data(), n * sizeof(oi_mesh_desc), hipMemcpyHostToDevice,
+                                 c->stream));
     }
     const bool self = (s->last_flags & OIBVH_DETECT_SELF) != 0;
     const uint32_t key[3] = {s->last_entry, self ? 1u : 0u, n};
     if (std::memcmp(key, s->front_key, sizeof key) != 0) {
         make_front(s, s->last_entry, self);
-        s->n_front = (uint32_t)s->h_front.size();
-        oibvh_status r = grow(c, s->d_init, s->init_cap, std::max(1u, s->n_front));
+        oibvh_status r = oi_det_set_front(&d, s->h_front);
         if (r != OIBVH_OK) return r;
-        if (s->n_front)
-            HIPCHK(c, hipMemcpyAsync(s->d_init, s->h_front.data(), s->n_front * sizeof(uint2), hipMemcpyHostToDevice,
-                                     st));
         std::memcpy(s->front_key, key, sizeof key);
     }
-    // every BVTT step takes each non-leaf side up to `expand` levels down and
-    // tests the pairs it reaches: the walk ends at the leaves after
-    // ceil(depth / expand) steps (one step tests entry-level leaf pairs)
-    s->n_levels = std::max(1u, (depth + expand - 1) / expand);
-    oibvh_status r;
-    const uint32_t fcap = std::max(c->queue_cap, oi_np2(std::max(1u, s->n_front)));
-    if (s->front_cap < fcap) {
-        dfree(s->front[0]);
-        dfree(s->front[1]);
-        s->front_cap = 0;
-        HIPCHK(c, dalloc(&s->front[0], fcap));
-        HIPCHK(c, dalloc(&s->front[1], fcap));
-        s->front_cap = fcap;
-    }
-    if ((r = grow(c, s->counts, s->counts_cap, s->n_levels + 1)) != OIBVH_OK) return r;
-    if ((r = grow(c, s->cands, s->cand_cap, c->cand_cap)) != OIBVH_OK) return r;
-    if ((r = grow(c, s->hits, s->hits_cap, c->hits_cap)) != OIBVH_OK) return r;
-    if (s->n_levels + 1 > 256) return set_err(c, OIBVH_ERR_INVALID, "detect: %u BVTT steps", s->n_levels);
-    HIPCHK(c, oi_launch_seed(st, s->n_front, s->counts, s->n_levels + 1, s->ctl, reset_sticky ? 1u : 0u));
-    const uint32_t grid = grid_for(c);
-    const oi_mesh_desc* dd = n > 4 ? s->d_desc : nullptr;
-    for (uint32_t k = 0; k < s->n_levels; k++) {
-        // step 0 reads the initial front in place; then the fronts alternate
-        const uint2* in = k ? s->front[(k - 1) & 1] : s->d_init;
-        HIPCHK(c, oi_launch_bvtt_step(st, grid, dd, desc.data(), n, k == 0, in, k ? s->counts + k : nullptr,
-                                      s->n_front, k ? s->front_cap : s->n_front, s->front[k & 1],
-                                      s->counts + k + 1, s->front_cap, s->cands, &s->ctl->cands, s->cand_cap,
-                                      s->ctl, expand, s->implicit_end));
-    }
-    HIPCHK(c, oi_launch_narrow(st, grid, dd, desc.data(), n, s->cands, &s->ctl->cands, s->cand_cap, s->hits,
-                               &s->ctl->n_hits, s->hits_cap, s->ctl,
-                               (s->last_flags & OIBVH_DETECT_FP64_FALLBACK) ? 1 : 0));
-    s->unchecked = true;
-    s->have_result = true;
-    return OIBVH_OK;
-}
-
-// wait for the last detect, grow buffers and re-run on overflow
-static oibvh_status check_detect(oibvh_scene* s)
-{
-    oibvh_ctx* c = s->ctx;
-    if (!s->unchecked) return OIBVH_OK;
-    uint32_t retries = 0, earlier = 0;
-    for (;;) {
-        s->h_counts.resize(s->n_levels + 1);
-        HIPCHK(c, hipMemcpyAsync(s->h_ctl, s->ctl, sizeof(oi_detect_ctl), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipMemcpyAsync(s->h_counts.data(), s->counts, (s->n_levels + 1) * sizeof(uint32_t),
-                                 hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        const oi_detect_ctl& k = *s->h_ctl;
-        earlier |= k.sticky;  // errors of unchecked asynchronous detects before this one
-        if (s->h_counts[s->n_levels] != 0) {
-            // the last step reaches the leaves on every side: it writes no front
-            s->unchecked = false;
-            s->have_result = false;
-            return set_err(c, OIBVH_ERR_HIP, "detect: %u BVTT pairs left after the last step",
-                           s->h_counts[s->n_levels]);
-        }
-        if (k.error & OI_ERR_CORRUPT) {
-            s->unchecked = false;
-            s->have_result = false;
-            return set_err(c, OIBVH_ERR_HIP, "detect: corrupt BVTT entry (error bits 0x%x)", k.error);
-        }
-        bool rerun = false;
-        oibvh_status r = OIBVH_OK;
-        uint32_t maxfront = 0;
-        for (uint32_t v : s->h_counts) maxfront = std::max(maxfront, v);
-        if (k.error & OI_ERR_QUEUE_OVERFLOW) {
-            const uint64_t need = oi_np2(std::max(maxfront, s->front_cap * 4u));
-            if (need > (1u << 28)) {
-                s->unchecked = false;
-                s->have_result = false;
-                return set_err(c, OIBVH_ERR_OVERFLOW, "BVTT front exceeds %u entries", 1u << 28);
-            }
-            dfree(s->front[0]);
-            dfree(s->front[1]);
-            s->front_cap = 0;
-            HIPCHK(c, dalloc(&s->front[0], need));
-            HIPCHK(c, dalloc(&s->front[1], need));
-            s->front_cap = (uint32_t)need;
-            rerun = true;
-        }
-        if ((k.error & OI_ERR_CAND_OVERFLOW) || k.cands > s->cand_cap) {
-            if ((r = grow(c, s->cands, s->cand_cap, std::max<uint64_t>(k.cands, 2ull * s->cand_cap))) != OIBVH_OK)
-                return r;
-            rerun = true;
-        }
-        if ((k.error & OI_ERR_HITS_OVERFLOW) || k.n_hits > s->hits_cap) {
-            const uint64_t need = std::max<uint64_t>((uint64_t)k.n_hits + (k.n_hits >> 3), 2ull * s->hits_cap);
-            if ((r = grow(c, s->hits, s->hits_cap, need)) != OIBVH_OK) return r;
-            rerun = true;
-        }
-        if (!rerun) {
-            s->n_pairs = k.n_hits;
-            s->stats.bvtt_tests = k.tests;
-            s->stats.candidates = k.cands;
-            s->stats.fp64_rechecks = k.fp64;
-            s->stats.adjacent = k.adj;
-            s->stats.pairs = k.n_hits;
-            s->stats.retries = retries;
-            s->stats.async_errors = earlier;
-            s->unchecked = false;
-            return OIBVH_OK;
-        }
-        retries++;
-        if (retries > 16) return set_err(c, OIBVH_ERR_OVERFLOW, "detect: too many retries");
-        if ((r = enqueue_detect(s, true)) != OIBVH_OK) return r;
-    }
+    d.tab = oi_tab_args{n > 4 ? s->d_desc : nullptr, desc.data(), n, nullptr, 0};
+    d.depth = depth;
+    d.expand = c->trav_expand ? c->trav_expand : std::min(3u, std::max(1u, s->last_expand));
+    d.flags = s->last_flags;
+    d.implicit_end = (uint32_t)acc;
+    oibvh_status r = oi_det_prepare(&d);
+    if (r != OIBVH_OK) return r;
+    return oi_det_enqueue(&d, reset_sticky);
 }
 
 extern "C" oibvh_status oibvh_scene_detect(oibvh_scene* s, uint32_t entry, uint32_t expand, uint32_t flags)
@@ -821,20 +646,22 @@ extern "C" oibvh_status oibvh_scene_detect(oibvh_scene* s, uint32_t entry, uint3
     oibvh_ctx* c = s->ctx;
     HIPCHK(c, hipSetDevice(c->device));
     for (oibvh_tree* t : s->trees)
-        if (!t->built) return set_err(c, OIBVH_ERR_NOT_BUILT, "detect: tree not built");
+        if (!t->built) return oi_set_err(c, OIBVH_ERR_NOT_BUILT, "detect: tree not built");
     const bool async = (flags & OIBVH_DETECT_ASYNC) != 0;
-    if (s->unchecked && !async) {
-        oibvh_status r = check_detect(s);
+    oi_detector& d = s->det;
+    if (d.unchecked && !async) {
+        oibvh_status r = oi_det_check(&d);
         if (r != OIBVH_OK) return r;
     }
     s->last_entry = entry;
     s->last_expand = expand;
     s->last_flags = flags;
-    s->n_pairs = 0;
-    s->stats = oibvh_detect_stats{};
+    d.n_pairs = 0;
+    d.stats = oibvh_detect_stats{};
     if (s->trees.empty() || (s->trees.size() == 1 && !(flags & OIBVH_DETECT_SELF))) {
-        s->have_result = true;
-        s->unchecked = false;
+        d.have_result = true;
+        d.unchecked = false;
+        d.n_levels = 0;
         return OIBVH_OK;
     }
     // an asynchronous detect keeps the error history of unchecked earlier
@@ -842,72 +669,45 @@ extern "C" oibvh_status oibvh_scene_detect(oibvh_scene* s, uint32_t entry, uint3
     oibvh_status r = enqueue_detect(s, !async);
     if (r != OIBVH_OK) return r;
     if (async) return OIBVH_OK;
-    return check_detect(s);
+    return oi_det_check(&d);
 }
 
 extern "C" oibvh_status oibvh_scene_get_pair_count(oibvh_scene* s, uint32_t* count)
 {
     if (!s || !count) return OIBVH_ERR_INVALID;
-    oibvh_status r = check_detect(s);
+    oibvh_status r = oi_det_check(&s->det);
     if (r != OIBVH_OK) return r;
-    *count = s->n_pairs;
+    *count = s->det.n_pairs;
     return OIBVH_OK;
 }
 
 extern "C" oibvh_status oibvh_scene_get_stats(oibvh_scene* s, oibvh_detect_stats* st)
 {
     if (!s || !st) return OIBVH_ERR_INVALID;
-    oibvh_status r = check_detect(s);
+    oibvh_status r = oi_det_check(&s->det);
     if (r != OIBVH_OK) return r;
-    *st = s->stats;
+    *st = s->det.stats;
     return OIBVH_OK;
 }
 
 extern "C" oibvh_status oibvh_scene_hits_device(oibvh_scene* s, void** dptr, uint32_t* n)
 {
     if (!s || !dptr || !n) return OIBVH_ERR_INVALID;
-    oibvh_status r = check_detect(s);
+    oibvh_status r = oi_det_check(&s->det);
     if (r != OIBVH_OK) return r;
-    *dptr = s->hits;
-    *n = s->n_pairs;
+    *dptr = s->det.hits;
+    *n = s->det.n_pairs;
     return OIBVH_OK;
 }
 
 extern "C" oibvh_status oibvh_scene_get_pairs(oibvh_scene* s, uint32_t* out, uint64_t cap, uint32_t* n)
 {
     if (!s) return OIBVH_ERR_INVALID;
-    oibvh_ctx* c = s->ctx;
-    oibvh_status r = check_detect(s);
-    if (r != OIBVH_OK) return r;
-    if (n) *n = s->n_pairs;
-    if (s->n_pairs == 0) return OIBVH_OK;
-    if (!out || cap < s->n_pairs) return set_err(c, OIBVH_ERR_CAPACITY, "need %u pairs", s->n_pairs);
-    std::vector<uint4> h(s->n_pairs);
-    HIPCHK(c, hipMemcpyAsync(h.data(), s->hits, s->n_pairs * sizeof(uint4), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    std::sort(h.begin(), h.end(), [](const uint4& a, const uint4& b) {
-        if (a.x != b.x) return a.x < b.x;
-        if (a.y != b.y) return a.y < b.y;
-        if (a.z != b.z) return a.z < b.z;
-        return a.w < b.w;
-    });
-    for (uint32_t i = 0; i < s->n_pairs; i++) {
-        out[4ull * i] = h[i].x;
-        out[4ull * i + 1] = h[i].y;
-        out[4ull * i + 2] = h[i].z;
-        out[4ull * i + 3] = h[i].w;
-    }
-    return OIBVH_OK;
+    return oi_det_get_pairs(&s->det, out, cap, n);
 }
 
 extern "C" oibvh_status oibvh_scene_get_level_counts(oibvh_scene* s, uint32_t* out, uint64_t cap, uint32_t* n)
 {
     if (!s || !n) return OIBVH_ERR_INVALID;
-    oibvh_status r = check_detect(s);
-    if (r != OIBVH_OK) return r;
-    *n = s->have_result ? s->n_levels : 0;
-    if (*n == 0) return OIBVH_OK;
-    if (!out || cap < *n) return set_err(s->ctx, OIBVH_ERR_CAPACITY, "need %u entries", *n);
-    std::memcpy(out, s->h_counts.data(), *n * sizeof(uint32_t));
-    return OIBVH_OK;
+    return oi_det_level_counts(&s->det, out, cap, n);
 }

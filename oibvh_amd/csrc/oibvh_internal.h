@@ -1,0 +1,145 @@
+// oibvh_internal.h -- host-side internals shared by the C ABI implementation
+// files: the context, error handling, device allocation helpers, the kernel
+// launchers and the BVTT detector used by scenes and batches.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "oibvh.h"
+#include "oibvh_common.h"
+
+extern "C" {
+hipError_t oi_launch_mesh_box(hipStream_t, const oi_build_desc*, const uint2*, const oi_build_desc*, uint32_t, float*,
+                              uint32_t*, uint32_t);
+uint32_t oi_morton_chunk();
+hipError_t oi_launch_morton(hipStream_t, const oi_build_desc*, const uint2*, const oi_build_desc*, uint32_t,
+                            const float*, uint32_t*, uint32_t*, uint32_t*);
+uint32_t oi_sort_tile();
+hipError_t oi_launch_sort_pass(hipStream_t, const uint32_t*, const uint32_t*, uint32_t*, uint32_t*, uint32_t,
+                               uint32_t, const uint32_t*, uint32_t*, uint32_t*, uint32_t*, const uint32_t*);
+uint32_t oi_reduce_height(int, uint32_t);
+hipError_t oi_launch_reduce(hipStream_t, int, const oi_build_desc*, const uint2*, const oi_build_desc*, uint32_t,
+                            const uint32_t*, uint32_t);
+hipError_t oi_launch_reduce_apex(hipStream_t, int, const oi_build_desc*, const uint2*, const oi_build_desc*,
+                                 uint32_t);
+hipError_t oi_launch_set_xf(hipStream_t, float*, const float*);
+hipError_t oi_launch_seed(hipStream_t, uint32_t, uint32_t*, uint32_t, oi_detect_ctl*, uint32_t);
+hipError_t oi_launch_bvtt_step(hipStream_t, uint32_t, const oi_tab_args*, int, const uint2*, const uint32_t*,
+                               uint32_t, uint32_t, uint2*, uint32_t*, uint32_t, uint2*, uint32_t*, uint32_t,
+                               oi_detect_ctl*, uint32_t, uint32_t);
+hipError_t oi_launch_narrow(hipStream_t, uint32_t, const oi_tab_args*, const uint2*, const uint32_t*, uint32_t,
+                            uint4*, uint32_t*, uint32_t, oi_detect_ctl*, int);
+}
+
+// ---------------------------------------------------------------------------
+struct oibvh_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+    int cus = 0;
+    uint64_t clock_khz = 100000;
+    uint32_t leaves_per_thread = 0;
+    uint32_t trav_blocks = 0;
+    uint32_t trav_expand = 0;
+    uint32_t queue_cap = 1u << 22;
+    uint32_t hits_cap = 1u << 20;
+    uint32_t cand_cap = 1u << 20;
+    int live = 0;                 // trees, scenes and batches of this context not yet destroyed
+    bool destroy_pending = false; // oibvh_ctx_destroy called while objects were live
+};
+
+// drop one reference held by a tree, scene or batch (frees a context whose
+// destroy was deferred)
+void oi_ctx_release(oibvh_ctx* ctx);
+
+oibvh_status oi_set_err(oibvh_ctx* ctx, oibvh_status st, const char* fmt, ...);
+
+#define HIPCHK(ctx, expr)                                                                           \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            return oi_set_err((ctx), OIBVH_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+static inline hipError_t dalloc(T** p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    return hipMalloc((void**)p, n * sizeof(T));
+}
+
+template <class T>
+static inline void dfree(T*& p)
+{
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+static inline hipMemcpyKind kind_of(uint32_t ptr_kind)
+{
+    return ptr_kind == OIBVH_PTR_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+}
+
+// workgroups per BVTT / narrow-phase launch
+static inline uint32_t oi_grid_for(const oibvh_ctx* c)
+{
+    return c->trav_blocks ? c->trav_blocks : (uint32_t)std::max(1, c->cus) * 4u;
+}
+
+// ---------------------------------------------------------------------------
+// BVTT detector: the device buffers and launch sequence of one collision
+// query (Scene::detectCollisionOnGPU, scene.cu:225-446), shared by scenes and
+// batches. The owner provides the mesh table, the initial front and the
+// depth; enqueue only launches kernels (it can be captured in a graph),
+// prepare sizes the buffers beforehand, check waits, grows what overflowed
+// and runs the query again.
+// ---------------------------------------------------------------------------
+struct oi_detector {
+    oibvh_ctx* ctx = nullptr;
+    uint2* d_init = nullptr;  // initial BVTT front
+    uint32_t init_cap = 0;
+    uint32_t n_front = 0;
+    uint2* front[2] = {nullptr, nullptr};  // ping-pong BVTT fronts
+    uint32_t front_cap = 0;
+    uint32_t* counts = nullptr;  // front length per step (+1)
+    uint32_t counts_cap = 0;
+    uint32_t n_levels = 0;
+    uint2* cands = nullptr;
+    uint32_t cand_cap = 0;
+    uint4* hits = nullptr;
+    uint32_t hits_cap = 0;
+    oi_detect_ctl* ctl = nullptr;
+    oi_detect_ctl* h_ctl = nullptr;  // pinned
+    std::vector<uint32_t> h_counts;
+    // the query (kept for re-runs)
+    oi_tab_args tab{};
+    uint32_t depth = 0, expand = 1, flags = 0, implicit_end = 0;
+    bool unchecked = false;
+    bool have_result = false;
+    uint32_t n_pairs = 0;
+    oibvh_detect_stats stats{};
+    uint64_t generation = 0;  // bumped whenever a device buffer is replaced
+};
+
+oibvh_status oi_det_init(oi_detector* d, oibvh_ctx* c);
+void oi_det_free(oi_detector* d);
+// upload the initial front (the vector must stay unchanged until the stream
+// has consumed it)
+oibvh_status oi_det_set_front(oi_detector* d, const std::vector<uint2>& front);
+// size every buffer for the current query (allocates; not capturable)
+oibvh_status oi_det_prepare(oi_detector* d);
+// launch seed, steps and narrow phase (no allocation, no synchronisation)
+oibvh_status oi_det_enqueue(oi_detector* d, bool reset_sticky);
+// wait; on overflow grow the buffers and run the query again
+oibvh_status oi_det_check(oi_detector* d);
+// sorted hits (4 words each)
+oibvh_status oi_det_get_pairs(oi_detector* d, uint32_t* out, uint64_t cap, uint32_t* n);
+oibvh_status oi_det_level_counts(oi_detector* d, uint32_t* out, uint64_t cap, uint32_t* n);

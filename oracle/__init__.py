@@ -58,6 +58,7 @@ def lib():
             ("orc_real_to_implicit", u32, [u32, u32, u32]),
             ("orc_box_overlap", C.c_int, [f32p, f32p]),
             ("orc_mesh_box", None, [f32p, u32, f32p]),
+            ("orc_xform", None, [f32p, f32p, u32, f32p]),
             ("orc_quantise10", u32, [C.c_float]),
             ("orc_morton30", u32, [C.c_float, C.c_float, C.c_float]),
             ("orc_leaf_keys", None, [u32p, f32p, u32, f32p, u32p, f32p]),
@@ -97,6 +98,7 @@ def load_ref():
         R.ref_overlap.argtypes = [f32p, f32p]
         R.ref_overlap.restype = C.c_int
         R.ref_mesh_box.argtypes = [f32p, C.c_uint32, f32p]
+        R.ref_xform.argtypes = [f32p, f32p, C.c_uint32, f32p]
         _ref = R
     return _ref
 
@@ -128,6 +130,17 @@ def implicit_to_real(i: int, L: int, vl: int) -> int:
 
 def real_to_implicit(r: int, L: int, vl: int) -> int:
     return lib().orc_real_to_implicit(r, L, vl)
+
+
+def xform(mat4, xyz) -> np.ndarray:
+    """Mesh::transform's vertex map: mat4 @ (x, y, z, 1) in glm's evaluation
+    order, FP32 without contraction. mat4 is a 4x4 matrix (row, column)."""
+    m = _f32(np.asarray(mat4, np.float32).reshape(4, 4).T)  # column-major
+    x = _f32(xyz).reshape(-1, 3)
+    out = np.empty_like(x)
+    lib().orc_xform(m.ctypes.data_as(C.POINTER(C.c_float)), x.ctypes.data_as(C.POINTER(C.c_float)), x.shape[0],
+                    out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out
 
 
 def quantise10(p: float) -> int:

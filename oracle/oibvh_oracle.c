@@ -132,6 +132,26 @@ EXPORT void orc_mesh_box(const float* xyz, uint32_t V, float* out)
 }
 
 /* ------------------------------------------------------------------------ */
+/* Mesh::transform (mesh.cpp:189-214; transform_vec4_kernel, transform.cu:35-41): */
+/* p' = M * vec4(p, 1) in glm's mat4 * vec4 order (type_mat4x4.inl:561-573):   */
+/* (m[0] x + m[1] y) + (m[2] z + m[3] w), m[c] the c-th column, w = 1.          */
+/* m is column-major (glm::mat4 memory layout).                                 */
+/* ------------------------------------------------------------------------ */
+EXPORT void orc_xform(const float* m, const float* xyz, uint32_t n, float* out)
+{
+    for (uint32_t i = 0; i < n; i++) {
+        const float x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2], w = 1.0f;
+        for (int r = 0; r < 3; r++) {
+            const float mul0 = m[r] * x, mul1 = m[4 + r] * y;
+            const float add0 = mul0 + mul1;
+            const float mul2 = m[8 + r] * z, mul3 = m[12 + r] * w;
+            const float add1 = mul2 + mul3;
+            out[3 * i + r] = add0 + add1;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* Morton30: strech_by_3 / morton3D oibvh.cu:6-22, leaf kernel oibvh.cu:42-70 */
 /* ------------------------------------------------------------------------ */
 static inline uint32_t spread10(uint32_t x)

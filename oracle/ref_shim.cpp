@@ -77,4 +77,19 @@ void ref_mesh_box(const float* xyz, uint32_t V, float* out)
     }
 }
 
+// glm::mat4 * glm::vec4(p, 1) as transform_vec4_kernel applies it
+// (transform.cu:35-41); m is 16 floats, column-major (glm::make_mat4 order)
+void ref_xform(const float* m, const float* xyz, uint32_t n, float* out)
+{
+    glm::mat4 M;
+    for (int c = 0; c < 4; c++)
+        for (int r = 0; r < 4; r++) M[c][r] = m[4 * c + r];
+    for (uint32_t i = 0; i < n; i++) {
+        const glm::vec4 p = M * glm::vec4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 1.0f);
+        out[3 * i] = p.x;
+        out[3 * i + 1] = p.y;
+        out[3 * i + 2] = p.z;
+    }
+}
+
 }  // extern "C"

@@ -12,8 +12,10 @@ only:
                  header, and Mesh::setupAABB's fold (mesh.cpp:91-98)
   scenes.npz     small seeded scenes with the oracle's trees (Morton order)
                  and sorted colliding pairs, inputs included
+  xform_ref.npz  model matrices, points and glm's mat4 * vec4(p, 1) as the
+                 reference's transform kernel applies it (transform.cu:35-41)
 
-usage: python tests/golden/make_golden.py
+usage: python tests/golden/make_golden.py [sat aabb scenes xform]
 """
 from __future__ import annotations
 
@@ -173,11 +175,42 @@ def make_scenes():
     np.savez_compressed(os.path.join(OUT, "scenes.npz"), **arrays)
 
 
+def make_xform(ref, n_mat: int = 64, n_pts: int = 256, seed: int = 0x7F0):
+    """Rigid motions with scale, shear and translation; points of mixed scale."""
+    r = M.Rng(seed)
+    mats = []
+    for k in range(n_mat):
+        R = M.random_rotation(r)
+        S = np.diag(r.uniform(3, 0.25, 4.0)) if k % 3 == 0 else np.eye(3)
+        A = np.eye(4)
+        A[:3, :3] = R @ S
+        if k % 5 == 1:
+            A[0, 1] += r.uniform(1, -0.5, 0.5)[0]  # shear
+        A[:3, 3] = r.uniform(3, -10.0, 10.0)
+        mats.append(A.astype(np.float32))
+    pts = r.uniform(n_pts * 3, -1.0, 1.0).reshape(n_pts, 3)
+    pts[: n_pts // 4] *= 1e3
+    pts[n_pts // 4: n_pts // 2] *= 1e-3
+    pts = pts.astype(np.float32)
+    out = np.empty((n_mat, n_pts, 3), np.float32)
+    colmajor = np.stack([np.ascontiguousarray(m.T).reshape(16) for m in mats])
+    for k in range(n_mat):
+        ref.ref_xform(_fp(np.ascontiguousarray(colmajor[k])), _fp(pts), n_pts, _fp(out[k]))
+    np.savez_compressed(os.path.join(OUT, "xform_ref.npz"), M=np.stack(mats), P=pts, out=out)
+    print(f"xform_ref.npz: {n_mat} matrices x {n_pts} points")
+
+
 if __name__ == "__main__":
     oracle.build()
     ref = oracle.load_ref()
     if ref is None:
         sys.exit("oracle/_ref/libref.so is not built (the reference tree must be mounted)")
-    make_sat(ref)
-    make_aabb(ref)
-    make_scenes()
+    which = set(sys.argv[1:]) or {"sat", "aabb", "scenes", "xform"}
+    if "sat" in which:
+        make_sat(ref)
+    if "aabb" in which:
+        make_aabb(ref)
+    if "scenes" in which:
+        make_scenes()
+    if "xform" in which:
+        make_xform(ref)

@@ -133,6 +133,52 @@ def test_refit_from_device_pointer(api, ctx, orc):
     buf.free()
 
 
+def _motion(seed, k):
+    r = M.Rng(seed * 97 + k)
+    A = np.eye(4, dtype=np.float32)
+    A[:3, :3] = M.random_rotation(r)
+    A[:3, 3] = r.uniform(3, -0.5, 0.5)
+    return A
+
+
+def test_transform_build_refit_detect(api, ctx, orc):
+    """Mesh::transform applied on the fly: build, refit and detect see
+    mat4 @ (p, 1) in glm's order, bit for bit what the oracle computes from
+    the transformed positions."""
+    say("model matrices: build, refit, detect")
+    (x0, f0), (x1, f1) = M.interpenetrating_icospheres(subdiv=4, offset=0.0)
+    t0, t1 = api.OibvhTree(ctx, x0, f0), api.OibvhTree(ctx, x1, f1)
+    A0, A1 = _motion(1, 0), _motion(2, 0)
+    t0.set_transform(A0)
+    t1.set_transform(A1)
+    t0.build()
+    t1.build()
+    y0, y1 = orc.xform(A0, x0), orc.xform(A1, x1)
+    check_tree(orc, t0, y0, f0)
+    check_tree(orc, t1, y1, f1)
+    sc = api.Scene(ctx)
+    sc.addOibvhTree(t0)
+    sc.addOibvhTree(t1)
+    for k in range(1, 4):
+        A0, A1 = _motion(1, k), _motion(2, k)
+        t0.set_transform(A0)
+        t1.set_transform(A1)
+        t0.refit()
+        t1.refit()
+        y0, y1 = orc.xform(A0, x0), orc.xform(A1, x1)
+        o0 = orc.Tree(y0, f0, order=t0.perm())
+        o1 = orc.Tree(y1, f1, order=t1.perm())
+        assert np.array_equal(bits(t0.nodes()), bits(o0.nodes()))
+        assert np.array_equal(bits(t1.nodes()), bits(o1.nodes()))
+        sc.detectCollision(2, 2)
+        want = orc.detect([o0, o1])
+        assert len(want) > 0
+        assert np.array_equal(sc.getIntTriPairs(), want)
+    t0.set_transform(None)
+    t0.refit()
+    assert np.array_equal(bits(t0.nodes()), bits(orc.Tree(x0, f0, order=t0.perm()).nodes()))
+
+
 def test_create_like_then_refit(api, ctx, orc):
     """main.cpp:127-151: tree 2 copy-constructed from tree 1 over a
     translated mesh with the same faces, then refit."""

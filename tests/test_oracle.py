@@ -250,6 +250,57 @@ def test_aabb_matches_reference_golden(orc):
     assert checked > 1000
 
 
+# ----------------------------------------------------------------- transform
+def test_xform_matches_reference_golden(orc):
+    """glm's own mat4 * vec4 (via the reference's vendored glm) on seeded
+    matrices and points, bit for bit."""
+    g = np.load(os.path.join(GOLD, "xform_ref.npz"))
+    for k in range(g["M"].shape[0]):
+        assert np.array_equal(bits(orc.xform(g["M"][k], g["P"])), bits(g["out"][k]))
+
+
+def test_xform_matches_reference_live(orc):
+    ref = orc.load_ref()
+    if ref is None:
+        pytest.skip("oracle/_ref not built (reference tree not mounted)")
+    r = M.Rng(0x7F1)
+    fp = C.POINTER(C.c_float)
+    for k in range(16):
+        A = np.eye(4, dtype=np.float32)
+        A[:3, :] = r.uniform(12, -3.0, 3.0).reshape(3, 4)
+        P = r.uniform(3 * 500, -50.0, 50.0).reshape(500, 3).astype(np.float32)
+        want = np.empty_like(P)
+        cm = np.ascontiguousarray(A.T).reshape(16)
+        ref.ref_xform(cm.ctypes.data_as(fp), P.ctypes.data_as(fp), 500, want.ctypes.data_as(fp))
+        assert np.array_equal(bits(orc.xform(A, P)), bits(want))
+
+
+def test_xform_closed_forms(orc):
+    """Translation and axis permutation are exact; the grouping
+    (m0 x + m1 y) + (m2 z + m3) is pinned by a case where the other
+    summation orders round differently, and the column-major layout by an
+    asymmetric matrix."""
+    P = np.array([[1.5, -2.0, 0.25], [3.0, 4.0, -5.0]], np.float32)
+    T = np.eye(4, dtype=np.float32)
+    T[:3, 3] = [10.0, -20.0, 0.5]
+    assert np.array_equal(orc.xform(T, P), P + T[:3, 3])
+    S = np.zeros((4, 4), np.float32)
+    S[0, 1] = S[1, 2] = S[2, 0] = S[3, 3] = 1.0  # (x, y, z) -> (y, z, x)
+    assert np.array_equal(orc.xform(S, P), P[:, [1, 2, 0]])
+    # row (1, 1, 1, e) on p = (1, e, e), e = 2^-24: glm's grouping
+    # (1 + e) + (e + e) = 1 + 2^-23, a left-to-right sum rounds to 1
+    e = np.float32(2.0 ** -24)
+    A = np.zeros((4, 4), np.float32)
+    A[0, :] = [1.0, 1.0, 1.0, 1.0]
+    A[3, 3] = 1.0
+    p = np.array([[1.0, e, e]], np.float32)
+    A[0, 3] = e
+    got = orc.xform(A, p)[0, 0]
+    glm = (np.float32(1.0) + e) + (e + e)            # = 1 + 2^-23
+    ltr = ((np.float32(1.0) + e) + e) + e            # = 1
+    assert glm != ltr and got == glm
+
+
 # ----------------------------------------------------------------- traversal
 def _scene(orc, meshes, order):
     return [orc.Tree(x, f, order=order) for x, f in meshes]
