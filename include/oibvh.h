@@ -39,6 +39,7 @@ typedef enum {
 typedef struct oibvh_ctx oibvh_ctx;
 typedef struct oibvh_tree oibvh_tree;
 typedef struct oibvh_scene oibvh_scene;
+typedef struct oibvh_batch oibvh_batch;
 
 /* pointer kinds for positions / indices */
 #define OIBVH_PTR_HOST 0u          /* host memory, copied */
@@ -210,6 +211,68 @@ OIBVH_API oibvh_status oibvh_ctx_get_tuning(const oibvh_ctx* ctx, oibvh_tuning* 
  * front); *n receives the number of levels. Diagnostics; the reference prints
  * these (scene.cu:274-276). */
 OIBVH_API oibvh_status oibvh_scene_get_level_counts(oibvh_scene* scene, uint32_t* out, uint64_t cap, uint32_t* n);
+
+/* ---------------------------------------------------------------- batches
+ * No reference counterpart. Many independent scenes in one object: every
+ * stage (build, refit, each BVTT step, narrow phase) is one launch over all
+ * meshes of all scenes. Meshes of one scene are tested against each other
+ * (and each against itself with OIBVH_DETECT_SELF), meshes of different
+ * scenes never. Per mesh, results equal those of an oibvh_tree over the same
+ * (transformed) vertices, and per scene those of an oibvh_scene over its
+ * meshes in batch order. */
+
+OIBVH_API oibvh_status oibvh_batch_create(oibvh_ctx* ctx, oibvh_batch** out);
+OIBVH_API void oibvh_batch_destroy(oibvh_batch* batch);
+
+/* Add a mesh to scene `scene` (any number; scenes need not be contiguous):
+ * V vertices xyz and F faces idx in host memory, copied. *index receives the
+ * mesh's batch index (insertion order). Only before the first build. */
+OIBVH_API oibvh_status oibvh_batch_add_mesh(oibvh_batch* batch, uint32_t scene, const float* xyz, uint32_t V,
+                                            const uint32_t* idx, uint32_t F, uint32_t* index);
+
+/* Add a rigid instance of mesh `source` to scene `scene`: it shares the
+ * source's vertex positions and faces and has its own model matrix, tree and
+ * leaf order. Only before the first build. */
+OIBVH_API oibvh_status oibvh_batch_add_instance(oibvh_batch* batch, uint32_t scene, uint32_t source,
+                                                uint32_t* index);
+
+OIBVH_API oibvh_status oibvh_batch_mesh_count(const oibvh_batch* batch, uint32_t* n);
+
+/* Model matrices of all meshes: mesh_count x 16 floats, column-major like
+ * glm::mat4, applied on the fly as oibvh_tree_set_transform does. Host memory
+ * is read before return; device memory (OIBVH_PTR_DEVICE) is copied in stream
+ * order, so a caller can keep every frame's matrices in HBM. */
+OIBVH_API oibvh_status oibvh_batch_set_transforms(oibvh_batch* batch, const float* mats, uint32_t ptr_kind);
+
+/* Device address of a mesh's vertex positions inside the batch (V*3 floats;
+ * shared with its instances). Callers may rewrite them in stream order
+ * (deforming meshes); the next refit or frame uses them. */
+OIBVH_API oibvh_status oibvh_batch_positions_device(oibvh_batch* batch, uint32_t mesh, float** dptr);
+
+/* Build every tree (OibvhTree::build per mesh, oibvhTree.cu:237-388). */
+OIBVH_API oibvh_status oibvh_batch_build(oibvh_batch* batch);
+/* Refit every tree (OibvhTree::refit per mesh, oibvhTree.cu:193-235). */
+OIBVH_API oibvh_status oibvh_batch_refit(oibvh_batch* batch);
+/* Scene::detectCollision for every scene (scene.cu:225-446); flags as for
+ * oibvh_scene_detect. */
+OIBVH_API oibvh_status oibvh_batch_detect(oibvh_batch* batch, uint32_t entry_level, uint32_t expand_levels,
+                                          uint32_t flags);
+/* One frame, refit + detect, replayed from a hipGraph captured at the first
+ * call and again whenever the parameters, the tuning or a device buffer
+ * change. With OIBVH_DETECT_ASYNC it returns after the launch. */
+OIBVH_API oibvh_status oibvh_batch_frame(oibvh_batch* batch, uint32_t entry_level, uint32_t expand_levels,
+                                         uint32_t flags);
+
+OIBVH_API oibvh_status oibvh_batch_get_pair_count(oibvh_batch* batch, uint32_t* count);
+/* Sorted (meshA, meshB, origTriA, origTriB) with batch mesh indices,
+ * meshA <= meshB, 4 words each; cap in pairs. */
+OIBVH_API oibvh_status oibvh_batch_get_pairs(oibvh_batch* batch, uint32_t* out, uint64_t cap, uint32_t* n);
+OIBVH_API oibvh_status oibvh_batch_get_stats(oibvh_batch* batch, oibvh_detect_stats* stats);
+OIBVH_API oibvh_status oibvh_batch_get_level_counts(oibvh_batch* batch, uint32_t* out, uint64_t cap, uint32_t* n);
+/* One mesh's tree nodes (as oibvh_tree_get_nodes) and leaf permutation. */
+OIBVH_API oibvh_status oibvh_batch_get_nodes(oibvh_batch* batch, uint32_t mesh, float* out, uint64_t cap,
+                                             uint32_t* n);
+OIBVH_API oibvh_status oibvh_batch_get_perm(oibvh_batch* batch, uint32_t mesh, uint32_t* out, uint64_t cap);
 
 #ifdef __cplusplus
 }

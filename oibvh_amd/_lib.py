@@ -75,6 +75,23 @@ API = [
     ("oibvh_scene_hits_device", _i, [_vp, C.POINTER(_vp), _u32p]),
     ("oibvh_scene_get_stats", _i, [_vp, C.POINTER(DetectStats)]),
     ("oibvh_scene_get_level_counts", _i, [_vp, _u32p, _u64, _u32p]),
+    ("oibvh_batch_create", _i, [_vp, C.POINTER(_vp)]),
+    ("oibvh_batch_destroy", None, [_vp]),
+    ("oibvh_batch_add_mesh", _i, [_vp, _u32, _vp, _u32, _vp, _u32, _u32p]),
+    ("oibvh_batch_add_instance", _i, [_vp, _u32, _u32, _u32p]),
+    ("oibvh_batch_mesh_count", _i, [_vp, _u32p]),
+    ("oibvh_batch_set_transforms", _i, [_vp, _vp, _u32]),
+    ("oibvh_batch_positions_device", _i, [_vp, _u32, C.POINTER(_vp)]),
+    ("oibvh_batch_build", _i, [_vp]),
+    ("oibvh_batch_refit", _i, [_vp]),
+    ("oibvh_batch_detect", _i, [_vp, _u32, _u32, _u32]),
+    ("oibvh_batch_frame", _i, [_vp, _u32, _u32, _u32]),
+    ("oibvh_batch_get_pair_count", _i, [_vp, _u32p]),
+    ("oibvh_batch_get_pairs", _i, [_vp, _u32p, _u64, _u32p]),
+    ("oibvh_batch_get_stats", _i, [_vp, C.POINTER(DetectStats)]),
+    ("oibvh_batch_get_level_counts", _i, [_vp, _u32p, _u64, _u32p]),
+    ("oibvh_batch_get_nodes", _i, [_vp, _u32, _f32p, _u64, _u32p]),
+    ("oibvh_batch_get_perm", _i, [_vp, _u32, _u32p, _u64]),
 ]
 
 _lib = None

@@ -273,6 +273,131 @@ class Scene:
         return {f: getattr(s, f) for f, _ in L.DetectStats._fields_}
 
 
+def _flags(self_collide: bool, fp64_fallback: bool, asynchronous: bool) -> int:
+    return ((L.DETECT_SELF if self_collide else 0) | (L.DETECT_FP64_FALLBACK if fp64_fallback else 0)
+            | (L.DETECT_ASYNC if asynchronous else 0))
+
+
+class Batch:
+    """Many independent scenes built, refit and queried together
+    (oibvh_batch_*). Meshes of one scene collide with each other, never with
+    other scenes' meshes."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        h = C.c_void_p()
+        _check(ctx.h, self._lib.oibvh_batch_create(ctx.h, C.byref(h)), "oibvh_batch_create")
+        self.h = h
+        self.scene_of: list[int] = []
+        self.faces_of: list[int] = []
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self._lib.oibvh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def add_mesh(self, scene: int, xyz, faces) -> int:
+        x = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        f = np.ascontiguousarray(faces, np.uint32).reshape(-1, 3)
+        idx = C.c_uint32()
+        _check(self.ctx.h, self._lib.oibvh_batch_add_mesh(self.h, scene, x.ctypes.data, x.shape[0], f.ctypes.data,
+                                                          f.shape[0], C.byref(idx)), "add_mesh")
+        self.scene_of.append(scene)
+        self.faces_of.append(f.shape[0])
+        return idx.value
+
+    def add_instance(self, scene: int, source: int) -> int:
+        idx = C.c_uint32()
+        _check(self.ctx.h, self._lib.oibvh_batch_add_instance(self.h, scene, source, C.byref(idx)), "add_instance")
+        self.scene_of.append(scene)
+        self.faces_of.append(self.faces_of[source])
+        return idx.value
+
+    @property
+    def mesh_count(self) -> int:
+        return len(self.scene_of)
+
+    def set_transforms(self, mats) -> None:
+        """mats: (n, 4, 4) matrices (row, column); stored column-major."""
+        m = np.ascontiguousarray(np.asarray(mats, np.float32).reshape(-1, 4, 4).transpose(0, 2, 1))
+        _check(self.ctx.h, self._lib.oibvh_batch_set_transforms(self.h, m.ctypes.data, L.PTR_HOST), "set_transforms")
+
+    def set_transforms_device(self, dptr: int) -> None:
+        """n x 16 column-major floats already in device memory."""
+        _check(self.ctx.h, self._lib.oibvh_batch_set_transforms(self.h, dptr, L.PTR_DEVICE), "set_transforms")
+
+    def positions_device(self, mesh: int) -> int:
+        p = C.c_void_p()
+        _check(self.ctx.h, self._lib.oibvh_batch_positions_device(self.h, mesh, C.byref(p)), "positions_device")
+        return p.value
+
+    def build(self) -> None:
+        _check(self.ctx.h, self._lib.oibvh_batch_build(self.h), "batch build")
+
+    def refit(self) -> None:
+        _check(self.ctx.h, self._lib.oibvh_batch_refit(self.h), "batch refit")
+
+    def detect(self, entry_level: int = 0, expand_levels: int = 1, *, self_collide: bool = False,
+               fp64_fallback: bool = False, asynchronous: bool = False) -> None:
+        _check(self.ctx.h, self._lib.oibvh_batch_detect(self.h, entry_level, expand_levels,
+                                                        _flags(self_collide, fp64_fallback, asynchronous)),
+               "batch detect")
+
+    def frame(self, entry_level: int = 0, expand_levels: int = 1, *, self_collide: bool = False,
+              fp64_fallback: bool = False, asynchronous: bool = False) -> None:
+        """refit + detect replayed from a captured hipGraph."""
+        _check(self.ctx.h, self._lib.oibvh_batch_frame(self.h, entry_level, expand_levels,
+                                                       _flags(self_collide, fp64_fallback, asynchronous)),
+               "batch frame")
+
+    def pair_count(self) -> int:
+        v = C.c_uint32()
+        _check(self.ctx.h, self._lib.oibvh_batch_get_pair_count(self.h, C.byref(v)), "batch pair count")
+        return v.value
+
+    def pairs(self) -> np.ndarray:
+        """(n, 4) uint32 rows (meshA, meshB, origTriA, origTriB), batch mesh
+        indices, sorted."""
+        n = self.pair_count()
+        out = np.empty((max(n, 1), 4), np.uint32)
+        got = C.c_uint32()
+        _check(self.ctx.h, self._lib.oibvh_batch_get_pairs(self.h, out.ctypes.data_as(L._u32p), max(n, 1),
+                                                           C.byref(got)), "batch pairs")
+        return out[: got.value]
+
+    def stats(self) -> dict:
+        s = L.DetectStats()
+        _check(self.ctx.h, self._lib.oibvh_batch_get_stats(self.h, C.byref(s)), "batch stats")
+        return {f: getattr(s, f) for f, _ in L.DetectStats._fields_}
+
+    def level_counts(self) -> np.ndarray:
+        out = np.zeros(256, np.uint32)
+        n = C.c_uint32()
+        _check(self.ctx.h, self._lib.oibvh_batch_get_level_counts(self.h, out.ctypes.data_as(L._u32p), out.size,
+                                                                  C.byref(n)), "batch level counts")
+        return out[: n.value].copy()
+
+    def nodes(self, mesh: int) -> np.ndarray:
+        F = self.faces_of[mesh]
+        N = 2 * F + bin((1 << (F - 1).bit_length()) - F).count("1") - 1
+        out = np.empty((N, 6), np.float32)
+        n = C.c_uint32()
+        _check(self.ctx.h, self._lib.oibvh_batch_get_nodes(self.h, mesh, out.ctypes.data_as(L._f32p), N,
+                                                           C.byref(n)), "batch nodes")
+        return out[: n.value]
+
+    def perm(self, mesh: int) -> np.ndarray:
+        F = self.faces_of[mesh]
+        out = np.empty(F, np.uint32)
+        _check(self.ctx.h, self._lib.oibvh_batch_get_perm(self.h, mesh, out.ctypes.data_as(L._u32p), F),
+               "batch perm")
+        return out
+
+
 class Timer:
     """HIP events recorded on the library's stream."""
 

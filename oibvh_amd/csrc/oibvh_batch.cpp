@@ -1,0 +1,668 @@
+// oibvh_batch.cpp -- many independent scenes built, refit and queried
+// together (no reference counterpart; the reference handles one scene of a
+// few meshes, scene.cu:225-446).
+//
+// Every mesh of every scene lives in shared concatenated buffers, and every
+// stage is one launch over all meshes: the build and refit kernels walk a
+// (mesh, chunk) work table, the Morton sort sorts all faces at once (four
+// 8-bit Morton passes, then 8-bit passes over the mesh id that partition the
+// faces back into their meshes, stably), and the BVTT steps and the narrow
+// phase consume one front that holds the pairs of all scenes. Meshes of a
+// scene are tested against each other (and, in self mode, each against
+// itself); meshes of different scenes never meet: the initial front only
+// pairs meshes of one scene. A frame (refit + detect) can be replayed from a
+// captured hipGraph.
+#include <new>
+
+#include "oibvh_internal.h"
+
+#define OI_CHUNK_SHIFT 12u  // granularity of a batch's implicit index space
+
+struct oibvh_batch {
+    oibvh_ctx* ctx = nullptr;
+    // meshes as added (host staging until the first build)
+    struct Mesh {
+        uint32_t scene, V, F;
+        uint32_t src;  // mesh whose vertices and faces this one shares (itself if owned)
+        uint32_t L, vl, N;
+        uint64_t vbase, fbase, nbase, ibase;  // offsets in the concatenated buffers
+        uint32_t pbase, nparts, vchunk;
+    };
+    std::vector<Mesh> meshes;
+    std::vector<std::vector<float>> h_xyz;      // per owned mesh (freed after upload)
+    std::vector<std::vector<uint32_t>> h_idx;
+    bool committed = false, built = false;
+    uint64_t Vt = 0, Ft = 0, Nt = 0, It = 0, Pt = 0;
+    uint32_t n_scenes = 0;
+    // device
+    float* pos = nullptr;
+    uint32_t* faces = nullptr;
+    uint32_t* sfaces = nullptr;
+    uint32_t* perm = nullptr;
+    float* nodes = nullptr;
+    float* boxes = nullptr;
+    float* partials = nullptr;
+    float* xf = nullptr;
+    bool has_xf = false;
+    uint32_t *keys = nullptr, *keys_alt = nullptr, *vals = nullptr, *vals_tmp = nullptr, *fmesh = nullptr;
+    uint32_t* sort_scratch = nullptr;  // ghist[1024] | tile counters[8] | lookback0 | lookback1
+    uint32_t* mesh_hist = nullptr;     // digit histograms of the mesh ids, 256 per pass
+    uint32_t tiles = 0, mesh_passes = 0;
+    oi_build_desc* d_bdesc = nullptr;
+    std::vector<oi_build_desc> h_bdesc;
+    uint2* w_box = nullptr;
+    uint32_t n_box = 0;
+    uint2* w_morton = nullptr;
+    uint32_t n_morton = 0;
+    uint2* w_reduce = nullptr;
+    uint32_t n_reduce = 0;
+    uint2* w_apex = nullptr;
+    uint32_t n_apex = 0;
+    int reduce_M = 0;  // M the reduce tables were made for
+    // traversal
+    oi_mesh_desc* d_mdesc = nullptr;
+    std::vector<oi_mesh_desc> h_mdesc;
+    uint32_t* chunk_mesh = nullptr;
+    std::vector<uint2> h_front;
+    uint32_t front_key[2] = {~0u, ~0u};  // entry, self
+    oi_detector det;
+    // captured frame
+    hipGraphExec_t gexec = nullptr;
+    uint64_t graph_key[6] = {0, 0, 0, 0, 0, 0};
+    uint32_t desc_gen = 0;
+};
+
+static oibvh_status batch_err(oibvh_batch* b, oibvh_status st, const char* what)
+{
+    return oi_set_err(b->ctx, st, "%s", what);
+}
+
+extern "C" oibvh_status oibvh_batch_create(oibvh_ctx* ctx, oibvh_batch** out)
+{
+    if (!out) return OIBVH_ERR_INVALID;
+    *out = nullptr;
+    if (!ctx) return OIBVH_ERR_INVALID;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    oibvh_batch* b = new (std::nothrow) oibvh_batch;
+    if (!b) return OIBVH_ERR_INVALID;
+    b->ctx = ctx;
+    oibvh_status r = oi_det_init(&b->det, ctx);
+    if (r != OIBVH_OK) {
+        delete b;
+        return r;
+    }
+    ctx->live++;
+    *out = b;
+    return OIBVH_OK;
+}
+
+static void batch_free_device(oibvh_batch* b)
+{
+    dfree(b->pos);
+    dfree(b->faces);
+    dfree(b->sfaces);
+    dfree(b->perm);
+    dfree(b->nodes);
+    dfree(b->boxes);
+    dfree(b->partials);
+    dfree(b->xf);
+    dfree(b->keys);
+    dfree(b->keys_alt);
+    dfree(b->vals);
+    dfree(b->vals_tmp);
+    dfree(b->fmesh);
+    dfree(b->sort_scratch);
+    dfree(b->mesh_hist);
+    dfree(b->d_bdesc);
+    dfree(b->w_box);
+    dfree(b->w_morton);
+    dfree(b->w_reduce);
+    dfree(b->w_apex);
+    dfree(b->d_mdesc);
+    dfree(b->chunk_mesh);
+}
+
+extern "C" void oibvh_batch_destroy(oibvh_batch* b)
+{
+    if (!b) return;
+    oibvh_ctx* c = b->ctx;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    batch_free_device(b);
+    oi_det_free(&b->det);
+    delete b;
+    oi_ctx_release(c);
+}
+
+extern "C" oibvh_status oibvh_batch_add_mesh(oibvh_batch* b, uint32_t scene, const float* xyz, uint32_t V,
+                                             const uint32_t* idx, uint32_t F, uint32_t* index)
+{
+    if (!b) return OIBVH_ERR_INVALID;
+    if (b->committed) return batch_err(b, OIBVH_ERR_INVALID, "oibvh_batch_add_mesh: batch already built");
+    if (!xyz || !idx || V == 0 || F == 0) return batch_err(b, OIBVH_ERR_INVALID, "oibvh_batch_add_mesh: invalid argument");
+    if (F > (1u << 24)) return batch_err(b, OIBVH_ERR_INVALID, "oibvh_batch_add_mesh: at most 2^24 faces per mesh");
+    for (uint64_t i = 0; i < 3ull * F; i++)
+        if (idx[i] >= V) return oi_set_err(b->ctx, OIBVH_ERR_INVALID, "face index %u out of range", idx[i]);
+    oibvh_batch::Mesh m{};
+    m.scene = scene;
+    m.V = V;
+    m.F = F;
+    m.src = (uint32_t)b->meshes.size();
+    b->meshes.push_back(m);
+    b->h_xyz.emplace_back(xyz, xyz + 3ull * V);
+    b->h_idx.emplace_back(idx, idx + 3ull * F);
+    if (index) *index = m.src;
+    return OIBVH_OK;
+}
+
+extern "C" oibvh_status oibvh_batch_add_instance(oibvh_batch* b, uint32_t scene, uint32_t source, uint32_t* index)
+{
+    if (!b) return OIBVH_ERR_INVALID;
+    if (b->committed) return batch_err(b, OIBVH_ERR_INVALID, "oibvh_batch_add_instance: batch already built");
+    if (source >= b->meshes.size()) return batch_err(b, OIBVH_ERR_INVALID, "oibvh_batch_add_instance: bad source");
+    oibvh_batch::Mesh m = b->meshes[source];
+    m.scene = scene;
+    m.src = b->meshes[source].src;
+    b->meshes.push_back(m);
+    b->h_xyz.emplace_back();
+    b->h_idx.emplace_back();
+    if (index) *index = (uint32_t)b->meshes.size() - 1;
+    return OIBVH_OK;
+}
+
+extern "C" oibvh_status oibvh_batch_mesh_count(const oibvh_batch* b, uint32_t* n)
+{
+    if (!b || !n) return OIBVH_ERR_INVALID;
+    *n = (uint32_t)b->meshes.size();
+    return OIBVH_OK;
+}
+
+static int batch_lpt(const oibvh_batch* b)
+{
+    if (b->ctx->leaves_per_thread) return (int)b->ctx->leaves_per_thread;
+    // aim for >= ~2 workgroups per CU in the reduce
+    const uint64_t target = 2ull * (uint64_t)(b->ctx->cus > 0 ? b->ctx->cus : 256) * 256ull;
+    uint32_t m = 1;
+    while (m < 8 && b->Ft / (2ull * m) >= target) m *= 2;
+    return (int)m;
+}
+
+template <class T>
+static oibvh_status upload(oibvh_ctx* c, T*& dst, const std::vector<T>& src)
+{
+    HIPCHK(c, dalloc(&dst, src.size()));
+    if (!src.empty()) HIPCHK(c, hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return OIBVH_OK;
+}
+
+static void fill_descs(oibvh_batch* b)
+{
+    const uint32_t n = (uint32_t)b->meshes.size();
+    b->h_bdesc.assign(n, oi_build_desc{});
+    b->h_mdesc.assign(n, oi_mesh_desc{});
+    for (uint32_t i = 0; i < n; i++) {
+        const oibvh_batch::Mesh& m = b->meshes[i];
+        const oibvh_batch::Mesh& s = b->meshes[m.src];
+        oi_build_desc& d = b->h_bdesc[i];
+        d.pos = b->pos + 3ull * s.vbase;
+        d.xf = b->has_xf ? b->xf + 16ull * i : nullptr;
+        d.faces = b->faces + 3ull * s.fbase;
+        d.sfaces = b->sfaces + 3ull * m.fbase;
+        d.perm = b->perm + m.fbase;
+        d.nodes = b->nodes + 6ull * m.nbase;
+        d.box = b->boxes + 6ull * i;
+        d.F = m.F;
+        d.V = m.V;
+        d.L = m.L;
+        d.vl = m.vl;
+        d.fbase = (uint32_t)m.fbase;
+        d.pbase = m.pbase;
+        d.nparts = m.nparts;
+        d.vchunk = m.vchunk;
+        oi_mesh_desc& t = b->h_mdesc[i];
+        t.nodes = d.nodes;
+        t.sfaces = d.sfaces;
+        t.pos = d.pos;
+        t.perm = d.perm;
+        t.xf = d.xf;
+        t.F = m.F;
+        t.L = m.L;
+        t.vl = m.vl;
+        t.N = m.N;
+        t.ibase = (uint32_t)m.ibase;
+    }
+}
+
+static oibvh_status upload_descs(oibvh_batch* b)
+{
+    oibvh_ctx* c = b->ctx;
+    fill_descs(b);
+    HIPCHK(c, hipMemcpy(b->d_bdesc, b->h_bdesc.data(), b->h_bdesc.size() * sizeof(oi_build_desc),
+                        hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(b->d_mdesc, b->h_mdesc.data(), b->h_mdesc.size() * sizeof(oi_mesh_desc),
+                        hipMemcpyHostToDevice));
+    b->desc_gen++;
+    return OIBVH_OK;
+}
+
+// reduce work tables for M leaves per thread
+static oibvh_status make_reduce_tables(oibvh_batch* b, int M)
+{
+    oibvh_ctx* c = b->ctx;
+    std::vector<uint2> wr, wa;
+    for (uint32_t i = 0; i < b->meshes.size(); i++) {
+        const oibvh_batch::Mesh& m = b->meshes[i];
+        const uint32_t h = oi_reduce_height(M, m.L);
+        const uint32_t nb = oi_lvl_cnt(m.L - h, m.L, m.vl);
+        for (uint32_t w = 0; w < nb; w++) wr.push_back(make_uint2(i, w));
+        if (m.L > h) wa.push_back(make_uint2(i, 0));
+    }
+    dfree(b->w_reduce);
+    dfree(b->w_apex);
+    oibvh_status r;
+    if ((r = upload(c, b->w_reduce, wr)) != OIBVH_OK) return r;
+    if ((r = upload(c, b->w_apex, wa)) != OIBVH_OK) return r;
+    b->n_reduce = (uint32_t)wr.size();
+    b->n_apex = (uint32_t)wa.size();
+    b->reduce_M = M;
+    return OIBVH_OK;
+}
+
+static oibvh_status commit(oibvh_batch* b)
+{
+    if (b->committed) return OIBVH_OK;
+    oibvh_ctx* c = b->ctx;
+    const uint32_t n = (uint32_t)b->meshes.size();
+    if (n == 0) return batch_err(b, OIBVH_ERR_INVALID, "batch has no meshes");
+    // layout
+    uint64_t V = 0, F = 0, N = 0, I = 0, P = 0;
+    uint32_t scenes = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        oibvh_batch::Mesh& m = b->meshes[i];
+        m.L = oi_ilog2(oi_np2(m.F));
+        m.vl = oi_np2(m.F) - m.F;
+        m.N = oi_tree_size(m.F);
+        if (m.src == i) {
+            m.vbase = V;
+            V += m.V;
+        } else {
+            m.vbase = b->meshes[m.src].vbase;
+        }
+        m.fbase = F;
+        F += m.F;
+        m.nbase = N;
+        N += m.N;
+        m.ibase = I;
+        const uint64_t span = (2ull << m.L) - 1ull;
+        I += (span + (1ull << OI_CHUNK_SHIFT) - 1) >> OI_CHUNK_SHIFT << OI_CHUNK_SHIFT;
+        m.nparts = std::max(1u, std::min(256u, (m.V + 4095u) / 4096u));
+        m.vchunk = (m.V + m.nparts - 1) / m.nparts;
+        m.pbase = (uint32_t)P;
+        P += m.nparts;
+        scenes = std::max(scenes, m.scene + 1);
+    }
+    if (F >= (1ull << 30)) return batch_err(b, OIBVH_ERR_INVALID, "batch exceeds 2^30 faces");
+    if (I > OI_IMPLICIT_LIMIT)
+        return oi_set_err(c, OIBVH_ERR_INVALID, "batch too large: %llu implicit BVTT indices (limit %u)",
+                          (unsigned long long)I, OI_IMPLICIT_LIMIT);
+    b->Vt = V;
+    b->Ft = F;
+    b->Nt = N;
+    b->It = I;
+    b->Pt = P;
+    b->n_scenes = scenes;
+    HIPCHK(c, hipSetDevice(c->device));
+    // geometry
+    HIPCHK(c, dalloc(&b->pos, 3 * V));
+    HIPCHK(c, dalloc(&b->faces, 3 * F));
+    for (uint32_t i = 0; i < n; i++) {
+        const oibvh_batch::Mesh& m = b->meshes[i];
+        if (m.src != i) continue;
+        HIPCHK(c, hipMemcpy(b->pos + 3 * m.vbase, b->h_xyz[i].data(), 12ull * m.V, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(b->faces + 3 * m.fbase, b->h_idx[i].data(), 12ull * m.F, hipMemcpyHostToDevice));
+        std::vector<float>().swap(b->h_xyz[i]);
+        std::vector<uint32_t>().swap(b->h_idx[i]);
+    }
+    HIPCHK(c, dalloc(&b->sfaces, 3 * F));
+    HIPCHK(c, dalloc(&b->perm, F));
+    HIPCHK(c, dalloc(&b->nodes, 6 * N));
+    HIPCHK(c, dalloc(&b->boxes, 6ull * n));
+    HIPCHK(c, dalloc(&b->partials, 6 * P));
+    // sort
+    HIPCHK(c, dalloc(&b->keys, F));
+    HIPCHK(c, dalloc(&b->keys_alt, F));
+    HIPCHK(c, dalloc(&b->vals, F));
+    HIPCHK(c, dalloc(&b->vals_tmp, F));
+    b->mesh_passes = n <= 1 ? 0 : n <= 256 ? 1 : n <= 65536 ? 2 : 3;
+    if (b->mesh_passes) HIPCHK(c, dalloc(&b->fmesh, F));
+    b->tiles = (uint32_t)((F + oi_sort_tile() - 1) / oi_sort_tile());
+    HIPCHK(c, dalloc(&b->sort_scratch, 1032ull + 512ull * b->tiles));
+    {
+        std::vector<uint32_t> mh(768, 0u);
+        for (uint32_t i = 0; i < n; i++)
+            for (uint32_t d = 0; d < 3; d++) mh[256 * d + ((i >> (8 * d)) & 0xffu)] += b->meshes[i].F;
+        oibvh_status r = upload(c, b->mesh_hist, mh);
+        if (r != OIBVH_OK) return r;
+    }
+    // work tables
+    std::vector<uint2> wb, wm;
+    for (uint32_t i = 0; i < n; i++) {
+        const oibvh_batch::Mesh& m = b->meshes[i];
+        for (uint32_t k = 0; k < m.nparts; k++) wb.push_back(make_uint2(i, k));
+        const uint32_t mc = (m.F + oi_morton_chunk() - 1) / oi_morton_chunk();
+        for (uint32_t k = 0; k < mc; k++) wm.push_back(make_uint2(i, k));
+    }
+    oibvh_status r;
+    if ((r = upload(c, b->w_box, wb)) != OIBVH_OK) return r;
+    if ((r = upload(c, b->w_morton, wm)) != OIBVH_OK) return r;
+    b->n_box = (uint32_t)wb.size();
+    b->n_morton = (uint32_t)wm.size();
+    // traversal: mesh of every 2^OI_CHUNK_SHIFT implicit indices
+    {
+        std::vector<uint32_t> cm(std::max<uint64_t>(1, I >> OI_CHUNK_SHIFT), 0u);
+        for (uint32_t i = 0; i < n; i++) {
+            const oibvh_batch::Mesh& m = b->meshes[i];
+            const uint64_t span = (2ull << m.L) - 1ull;
+            for (uint64_t k = m.ibase >> OI_CHUNK_SHIFT; k < (m.ibase + span + (1ull << OI_CHUNK_SHIFT) - 1) >> OI_CHUNK_SHIFT; k++)
+                cm[k] = i;
+        }
+        if ((r = upload(c, b->chunk_mesh, cm)) != OIBVH_OK) return r;
+    }
+    HIPCHK(c, dalloc(&b->d_bdesc, n));
+    HIPCHK(c, dalloc(&b->d_mdesc, n));
+    if ((r = upload_descs(b)) != OIBVH_OK) return r;
+    b->committed = true;
+    return OIBVH_OK;
+}
+
+extern "C" oibvh_status oibvh_batch_set_transforms(oibvh_batch* b, const float* mats, uint32_t ptr_kind)
+{
+    if (!b || !mats || ptr_kind > OIBVH_PTR_DEVICE_BORROW) return OIBVH_ERR_INVALID;
+    oibvh_ctx* c = b->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    oibvh_status r = commit(b);
+    if (r != OIBVH_OK) return r;
+    const size_t bytes = 64ull * b->meshes.size();
+    if (!b->xf) {
+        HIPCHK(c, dalloc(&b->xf, 16ull * b->meshes.size()));
+        b->has_xf = true;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if ((r = upload_descs(b)) != OIBVH_OK) return r;
+    }
+    if (ptr_kind == OIBVH_PTR_HOST) {
+        // host memory: copied before return
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpy(b->xf, mats, bytes, hipMemcpyHostToDevice));
+    } else {
+        HIPCHK(c, hipMemcpyAsync(b->xf, mats, bytes, hipMemcpyDeviceToDevice, c->stream));
+    }
+    return OIBVH_OK;
+}
+
+extern "C" oibvh_status oibvh_batch_positions_device(oibvh_batch* b, uint32_t mesh, float** dptr)
+{
+    if (!b || !dptr || mesh >= b->meshes.size()) return OIBVH_ERR_INVALID;
+    HIPCHK(b->ctx, hipSetDevice(b->ctx->device));
+    oibvh_status r = commit(b);
+    if (r != OIBVH_OK) return r;
+    *dptr = b->pos + 3ull * b->meshes[b->meshes[mesh].src].vbase;
+    return OIBVH_OK;
+}
+
+static oibvh_status enqueue_refit(oibvh_batch* b)
+{
+    oibvh_ctx* c = b->ctx;
+    const int M = b->reduce_M;
+    HIPCHK(c, oi_launch_reduce(c->stream, M, b->d_bdesc, b->w_reduce, nullptr, b->n_reduce, nullptr, 0));
+    if (b->n_apex) HIPCHK(c, oi_launch_reduce_apex(c->stream, M, b->d_bdesc, b->w_apex, nullptr, b->n_apex));
+    return OIBVH_OK;
+}
+
+static oibvh_status ensure_reduce_tables(oibvh_batch* b)
+{
+    const int M = batch_lpt(b);
+    if (M != b->reduce_M) {
+        HIPCHK(b->ctx, hipStreamSynchronize(b->ctx->stream));
+        return make_reduce_tables(b, M);
+    }
+    return OIBVH_OK;
+}
+
+extern "C" oibvh_status oibvh_batch_build(oibvh_batch* b)
+{
+    if (!b) return OIBVH_ERR_INVALID;
+    oibvh_ctx* c = b->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    oibvh_status r = commit(b);
+    if (r != OIBVH_OK) return r;
+    if ((r = ensure_reduce_tables(b)) != OIBVH_OK) return r;
+    hipStream_t s = c->stream;
+    const uint32_t F = (uint32_t)b->Ft;
+    uint32_t* ghist = b->sort_scratch;
+    uint32_t* tctr = b->sort_scratch + 1024;
+    uint32_t* lb[2] = {b->sort_scratch + 1032, b->sort_scratch + 1032 + 256ull * b->tiles};
+    HIPCHK(c, oi_launch_mesh_box(s, b->d_bdesc, b->w_box, nullptr, b->n_box, b->partials, b->sort_scratch,
+                                 1032u + 256u * b->tiles));
+    HIPCHK(c, oi_launch_morton(s, b->d_bdesc, b->w_morton, nullptr, b->n_morton, b->partials, b->keys, ghist,
+                               b->fmesh));
+    // four Morton passes, then the mesh-id passes; lookback buffers alternate
+    const uint32_t passes = 4 + b->mesh_passes;
+    uint32_t* kin = b->keys;
+    uint32_t* kout = b->keys_alt;
+    const uint32_t* vin = nullptr;
+    uint32_t* vout = b->vals_tmp;
+    for (uint32_t p = 0; p < passes; p++) {
+        const bool morton = p < 4;
+        const bool last = (p + 1 == passes);
+        const uint32_t* hist = morton ? ghist + 256 * p : b->mesh_hist + 256 * (p - 4);
+        const uint32_t shift = morton ? 8 * p : 8 * (p - 4);
+        uint32_t* ko = (morton && p < 3) ? kout : nullptr;
+        HIPCHK(c, oi_launch_sort_pass(s, morton ? kin : nullptr, vin, ko, vout, F, shift, hist, lb[p & 1],
+                                      last ? nullptr : lb[(p + 1) & 1], tctr + p, morton ? nullptr : b->fmesh));
+        std::swap(kin, kout);
+        vin = vout;
+        vout = (vout == b->vals_tmp) ? b->vals : b->vals_tmp;
+    }
+    HIPCHK(c, oi_launch_reduce(s, b->reduce_M, b->d_bdesc, b->w_reduce, nullptr, b->n_reduce, vin, 1));
+    if (b->n_apex) HIPCHK(c, oi_launch_reduce_apex(s, b->reduce_M, b->d_bdesc, b->w_apex, nullptr, b->n_apex));
+    b->built = true;
+    return OIBVH_OK;
+}
+
+extern "C" oibvh_status oibvh_batch_refit(oibvh_batch* b)
+{
+    if (!b) return OIBVH_ERR_INVALID;
+    oibvh_ctx* c = b->ctx;
+    if (!b->built) return oi_set_err(c, OIBVH_ERR_NOT_BUILT, "batch refit before build");
+    HIPCHK(c, hipSetDevice(c->device));
+    oibvh_status r = ensure_reduce_tables(b);
+    if (r != OIBVH_OK) return r;
+    return enqueue_refit(b);
+}
+
+// initial front: for every scene, the entry-level node pairs of every mesh
+// pair i < j (Scene::expandBvttNodes, scene.cu:192-223), and in self mode
+// the unordered entry-level pairs of each mesh with itself
+static void make_front(oibvh_batch* b, uint32_t entry, bool self)
+{
+    b->h_front.clear();
+    const uint32_t n = (uint32_t)b->meshes.size();
+    std::vector<std::vector<uint32_t>> by_scene(b->n_scenes);
+    for (uint32_t i = 0; i < n; i++) by_scene[b->meshes[i].scene].push_back(i);
+    auto lvl_range = [&](uint32_t m, uint32_t& a, uint32_t& z) {
+        const oibvh_batch::Mesh& t = b->meshes[m];
+        const uint32_t l = std::min(t.L, entry);
+        a = (1u << l) - 1u;
+        z = oi_lvl_last_implicit(l, t.L, t.vl);
+    };
+    for (const std::vector<uint32_t>& ms : by_scene) {
+        for (size_t p = 0; p < ms.size(); p++) {
+            const uint32_t i = ms[p];
+            const uint32_t bi = (uint32_t)b->meshes[i].ibase;
+            uint32_t a0, a1;
+            lvl_range(i, a0, a1);
+            if (self)
+                for (uint32_t x = a0; x <= a1; x++)
+                    for (uint32_t y = x; y <= a1; y++) b->h_front.push_back(make_uint2(bi + x, bi + y));
+            for (size_t q = p + 1; q < ms.size(); q++) {
+                const uint32_t j = ms[q];
+                const uint32_t bj = (uint32_t)b->meshes[j].ibase;
+                uint32_t b0, b1;
+                lvl_range(j, b0, b1);
+                for (uint32_t x = a0; x <= a1; x++)
+                    for (uint32_t y = b0; y <= b1; y++) b->h_front.push_back(make_uint2(bi + x, bj + y));
+            }
+        }
+    }
+}
+
+// set the detector up for (entry, expand, flags): front, table, depth, buffers
+static oibvh_status prepare_detect(oibvh_batch* b, uint32_t entry, uint32_t expand, uint32_t flags)
+{
+    oibvh_ctx* c = b->ctx;
+    oi_detector& d = b->det;
+    const bool self = (flags & OIBVH_DETECT_SELF) != 0;
+    const uint32_t key[2] = {entry, self ? 1u : 0u};
+    if (std::memcmp(key, b->front_key, sizeof key) != 0) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));  // the previous front may still be in flight
+        make_front(b, entry, self);
+        oibvh_status r = oi_det_set_front(&d, b->h_front);
+        if (r != OIBVH_OK) return r;
+        std::memcpy(b->front_key, key, sizeof key);
+    }
+    uint32_t depth = 0;
+    for (const oibvh_batch::Mesh& m : b->meshes) depth = std::max(depth, m.L - std::min(m.L, entry));
+    d.tab = oi_tab_args{b->d_mdesc, b->h_mdesc.data(), (uint32_t)b->meshes.size(), b->chunk_mesh, OI_CHUNK_SHIFT};
+    d.depth = depth;
+    d.expand = c->trav_expand ? c->trav_expand : std::min(3u, std::max(1u, expand));
+    d.flags = flags;
+    d.implicit_end = (uint32_t)b->It;
+    return oi_det_prepare(&d);
+}
+
+extern "C" oibvh_status oibvh_batch_detect(oibvh_batch* b, uint32_t entry, uint32_t expand, uint32_t flags)
+{
+    if (!b) return OIBVH_ERR_INVALID;
+    oibvh_ctx* c = b->ctx;
+    if (!b->built) return oi_set_err(c, OIBVH_ERR_NOT_BUILT, "batch detect before build");
+    HIPCHK(c, hipSetDevice(c->device));
+    oi_detector& d = b->det;
+    const bool async = (flags & OIBVH_DETECT_ASYNC) != 0;
+    if (d.unchecked && !async) {
+        oibvh_status r = oi_det_check(&d);
+        if (r != OIBVH_OK) return r;
+    }
+    d.n_pairs = 0;
+    d.stats = oibvh_detect_stats{};
+    oibvh_status r = prepare_detect(b, entry, expand, flags);
+    if (r != OIBVH_OK) return r;
+    if ((r = oi_det_enqueue(&d, !async)) != OIBVH_OK) return r;
+    if (async) return OIBVH_OK;
+    return oi_det_check(&d);
+}
+
+// refit + detect replayed from a captured graph; the graph is captured again
+// whenever the query, the reduce shape or a buffer changes
+extern "C" oibvh_status oibvh_batch_frame(oibvh_batch* b, uint32_t entry, uint32_t expand, uint32_t flags)
+{
+    if (!b) return OIBVH_ERR_INVALID;
+    oibvh_ctx* c = b->ctx;
+    if (!b->built) return oi_set_err(c, OIBVH_ERR_NOT_BUILT, "batch frame before build");
+    HIPCHK(c, hipSetDevice(c->device));
+    oibvh_status r = ensure_reduce_tables(b);
+    if (r != OIBVH_OK) return r;
+    oi_detector& d = b->det;
+    d.n_pairs = 0;
+    d.stats = oibvh_detect_stats{};
+    if ((r = prepare_detect(b, entry, expand, flags & ~OIBVH_DETECT_ASYNC)) != OIBVH_OK) return r;
+    const uint64_t key[6] = {entry, d.expand, flags & ~OIBVH_DETECT_ASYNC, (uint64_t)b->reduce_M,
+                             d.generation, (uint64_t)b->desc_gen ^ ((uint64_t)oi_grid_for(c) << 32)};
+    if (!b->gexec || std::memcmp(key, b->graph_key, sizeof key) != 0) {
+        if (b->gexec) {
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            (void)hipGraphExecDestroy(b->gexec);
+            b->gexec = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        oibvh_status rr = enqueue_refit(b);
+        if (rr == OIBVH_OK) rr = oi_det_enqueue(&d, false);
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rr != OIBVH_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rr;
+        }
+        if (e != hipSuccess) return oi_set_err(c, OIBVH_ERR_HIP, "frame capture: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            b->gexec = nullptr;
+            return oi_set_err(c, OIBVH_ERR_HIP, "frame instantiate: %s", hipGetErrorString(e));
+        }
+        std::memcpy(b->graph_key, key, sizeof key);
+    }
+    HIPCHK(c, hipGraphLaunch(b->gexec, c->stream));
+    d.unchecked = true;
+    d.have_result = true;
+    if (flags & OIBVH_DETECT_ASYNC) return OIBVH_OK;
+    return oi_det_check(&d);
+}
+
+extern "C" oibvh_status oibvh_batch_get_pair_count(oibvh_batch* b, uint32_t* count)
+{
+    if (!b || !count) return OIBVH_ERR_INVALID;
+    oibvh_status r = oi_det_check(&b->det);
+    if (r != OIBVH_OK) return r;
+    *count = b->det.n_pairs;
+    return OIBVH_OK;
+}
+
+extern "C" oibvh_status oibvh_batch_get_pairs(oibvh_batch* b, uint32_t* out, uint64_t cap, uint32_t* n)
+{
+    if (!b) return OIBVH_ERR_INVALID;
+    return oi_det_get_pairs(&b->det, out, cap, n);
+}
+
+extern "C" oibvh_status oibvh_batch_get_stats(oibvh_batch* b, oibvh_detect_stats* st)
+{
+    if (!b || !st) return OIBVH_ERR_INVALID;
+    oibvh_status r = oi_det_check(&b->det);
+    if (r != OIBVH_OK) return r;
+    *st = b->det.stats;
+    return OIBVH_OK;
+}
+
+extern "C" oibvh_status oibvh_batch_get_level_counts(oibvh_batch* b, uint32_t* out, uint64_t cap, uint32_t* n)
+{
+    if (!b || !n) return OIBVH_ERR_INVALID;
+    return oi_det_level_counts(&b->det, out, cap, n);
+}
+
+static oibvh_status mesh_readback(oibvh_batch* b, uint32_t mesh, void* dst, const void* src, size_t bytes)
+{
+    oibvh_ctx* c = b->ctx;
+    if (mesh >= b->meshes.size()) return batch_err(b, OIBVH_ERR_INVALID, "bad mesh index");
+    if (!b->built) return oi_set_err(c, OIBVH_ERR_NOT_BUILT, "batch not built");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return OIBVH_OK;
+}
+
+extern "C" oibvh_status oibvh_batch_get_nodes(oibvh_batch* b, uint32_t mesh, float* out, uint64_t cap, uint32_t* n)
+{
+    if (!b || mesh >= b->meshes.size()) return OIBVH_ERR_INVALID;
+    const oibvh_batch::Mesh& m = b->meshes[mesh];
+    if (n) *n = oi_tree_size(m.F);
+    if (!out || cap < oi_tree_size(m.F)) return oi_set_err(b->ctx, OIBVH_ERR_CAPACITY, "need %u nodes", oi_tree_size(m.F));
+    return mesh_readback(b, mesh, out, b->nodes + 6ull * m.nbase, 24ull * m.N);
+}
+
+extern "C" oibvh_status oibvh_batch_get_perm(oibvh_batch* b, uint32_t mesh, uint32_t* out, uint64_t cap)
+{
+    if (!b || mesh >= b->meshes.size()) return OIBVH_ERR_INVALID;
+    const oibvh_batch::Mesh& m = b->meshes[mesh];
+    if (!out || cap < m.F) return oi_set_err(b->ctx, OIBVH_ERR_CAPACITY, "need %u faces", m.F);
+    return mesh_readback(b, mesh, out, b->perm + m.fbase, 4ull * m.F);
+}

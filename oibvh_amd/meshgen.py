@@ -275,3 +275,123 @@ def deforming_uv_pair(nlon: int = 1000, nstack: int = 501, offset: float = 0.9,
     b = uv_sphere(nlon, nstack, center=(offset, 0.0, 0.0))
     waves = [NormalWave(a[0], (0.0, 0.0, 0.0), seed), NormalWave(b[0], (offset, 0.0, 0.0), seed + 1)]
     return [a, b], waves
+
+
+def _axis_rotation(axis: np.ndarray, angle: float) -> np.ndarray:
+    """Rodrigues rotation (float64)."""
+    a = axis / math.sqrt(float((axis * axis).sum()))
+    K = np.array([[0.0, -a[2], a[1]], [a[2], 0.0, -a[0]], [-a[1], a[0], 0.0]])
+    return np.eye(3) + math.sin(angle) * K + (1.0 - math.cos(angle)) * (K @ K)
+
+
+class RigidScenes:
+    """Batched workload: `n_scenes` scenes of U{lo..hi} rigid meshes, noisy
+    icospheres or boxes, each with a triangle count drawn log-uniform in
+    [fmin, fmax] (icospheres come in 20 * 4^s triangles, s = 3..6, boxes in
+    12 n^2; a target an icosphere level cannot reach within its range becomes
+    a box). Meshes are instances of a small library of base shapes (kind,
+    size, noise variant) scaled to `radius`; every instance has its own rigid
+    motion: a random initial rotation, a translation uniform in the unit cube,
+    a spin about a random axis and a slow drift. transforms(f) gives the
+    (n, 4, 4) float32 model matrices of frame f."""
+
+    def __init__(self, n_scenes: int = 512, seed: int = CFG3_SEED, meshes=(2, 16), faces=(1000, 200000),
+                 radius: float = 0.25, variants: int = 2, noise: float = 0.02):
+        r = Rng(seed)
+        self.library: list[tuple[np.ndarray, np.ndarray]] = []
+        self.kinds: list[tuple] = []
+        index: dict[tuple, int] = {}
+        self.instances: list[tuple[int, int]] = []  # (scene, library index)
+        lo, hi = math.log(faces[0]), math.log(faces[1])
+        for s in range(n_scenes):
+            k = int(r.integers(1, meshes[0], meshes[1])[0])
+            for _ in range(k):
+                target = math.exp(float(r.uniform(1, lo, hi)[0]))
+                ico = r.u01(1)[0] < 0.5
+                variant = int(r.integers(1, 0, variants - 1)[0])
+                key = None
+                if ico:
+                    sub = int(round(math.log(target / 20.0) / math.log(4.0)))
+                    if 3 <= sub <= 6 and faces[0] <= 20 * 4 ** sub <= faces[1] * 2:
+                        key = ("ico", sub, variant)
+                if key is None:
+                    key = ("box", max(3, int(round(math.sqrt(target / 12.0)))), variant)
+                if key not in index:
+                    vs = seed * 131 + len(self.library) * 7 + 1
+                    if key[0] == "ico":
+                        xyz, f = icosphere(key[1], 1.0, noise, vs)
+                    else:
+                        xyz, f = box(key[1], 0.6, noise, vs)
+                    index[key] = len(self.library)
+                    self.library.append((xyz, f))
+                    self.kinds.append(key)
+                self.instances.append((s, index[key]))
+        n = len(self.instances)
+        self.n_scenes = n_scenes
+        self.radius = radius
+        self.R0 = np.stack([random_rotation(r) for _ in range(n)])
+        self.t0 = r.uniform(3 * n, 0.0, 1.0).reshape(n, 3)
+        self.axis = r.uniform(3 * n, -1.0, 1.0).reshape(n, 3)
+        self.omega = r.uniform(n, 0.005, 0.03)
+        self.vel = r.uniform(3 * n, -0.002, 0.002).reshape(n, 3)
+
+    @property
+    def triangles(self) -> int:
+        return int(sum(self.library[b][1].shape[0] for _, b in self.instances))
+
+    def transforms(self, frame: int) -> np.ndarray:
+        n = len(self.instances)
+        out = np.zeros((n, 4, 4), np.float64)
+        for i in range(n):
+            R = _axis_rotation(self.axis[i], self.omega[i] * frame) @ self.R0[i]
+            out[i, :3, :3] = R * self.radius
+            out[i, :3, 3] = self.t0[i] + self.vel[i] * frame
+            out[i, 3, 3] = 1.0
+        return out.astype(np.float32)
+
+
+def precision_scenes(n_scenes: int = 64, seed: int = CFG4_SEED, n_plane: int = 354, n_box: int = 144):
+    """Precision stress: scenes of two meshes (~250k triangles each at the
+    default sizes: planes 2 * 354^2 = 250,632, boxes 12 * 144^2 = 248,832)
+    placed coplanar or edge-touching with seeded perturbations of
+    10^U[-7,-5] of the unit extent, so that many SAT comparisons are
+    near-ties. Kinds cycle through: coplanar planes (B shifted in-plane and
+    lifted by +-eps), planes meeting along an edge (B stood up at x = 1 + eps),
+    boxes face to face (gap or overlap eps), boxes edge to edge."""
+    r = Rng(seed)
+    scenes = []
+    pa = plane(n_plane)
+    ba = box(n_box, 0.5)
+    for s in range(n_scenes):
+        kind = s % 4
+        eps = 10.0 ** float(r.uniform(1, -7.0, -5.0)[0]) * (1.0 if r.u01(1)[0] < 0.5 else -1.0)
+        if kind == 0:
+            x, f = pa
+            shift = r.uniform(2, 0.1, 0.5)
+            b = x.astype(np.float64) + np.array([shift[0], shift[1], eps])
+            scenes.append([pa, (b.astype(np.float32), f)])
+        elif kind == 1:
+            x, f = pa
+            p = x.astype(np.float64)
+            b = np.stack([1.0 + eps + p[:, 2], p[:, 1] + float(r.uniform(1, -0.3, 0.3)[0]), p[:, 0] - 0.5], 1)
+            scenes.append([pa, (b.astype(np.float32), f)])
+        elif kind == 2:
+            x, f = ba
+            p = x.astype(np.float64)
+            off = r.uniform(2, -0.4, 0.4)
+            b = p + np.array([1.0 + eps, off[0], off[1]])
+            scenes.append([ba, (b.astype(np.float32), f)])
+        else:
+            x, f = ba
+            p = x.astype(np.float64)
+            R = _axis_rotation(np.array([0.0, 0.0, 1.0]), math.pi / 4)
+            b = p @ R.T + np.array([0.5 + math.sqrt(0.5) + eps, float(r.uniform(1, -0.3, 0.3)[0]), 0.0])
+            scenes.append([ba, (b.astype(np.float32), f)])
+    return scenes
+
+
+def cloth_instances(n: int = 8, nv: int = 2048, seed: int = CFG5_SEED, amplitude: float = 0.2,
+                    wavelength: float = 0.05):
+    """Self-collision workload: n cloth sheets of nv x nv vertices, each with
+    its own seeded phases (the perturbation between instances)."""
+    return [cloth(nv, amplitude, wavelength, seed + i) for i in range(n)]

@@ -1,0 +1,258 @@
+"""GPU parity of the batched form (oibvh_batch_*): every mesh's tree equals
+the oracle's tree over the same transformed vertices, and every scene's pair
+set equals the oracle's detect over that scene's meshes (integer outputs:
+exact). Graph replays are checked replay by replay against eager runs."""
+import sys
+
+import numpy as np
+import pytest
+
+from oibvh_amd import meshgen as M
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def api():
+    from oibvh_amd import api as A
+
+    return A
+
+
+def small_rigid(n_scenes=6, seed=0x5EED, meshes=(2, 4), faces=(20, 6000)):
+    return M.RigidScenes(n_scenes=n_scenes, seed=seed, meshes=meshes, faces=faces, radius=0.3)
+
+
+def make_batch(api, ctx, rs, instances=True):
+    b = api.Batch(ctx)
+    first = {}
+    for scene, lib in rs.instances:
+        if instances and lib in first:
+            b.add_instance(scene, first[lib])
+        else:
+            xyz, f = rs.library[lib]
+            idx = b.add_mesh(scene, xyz, f)
+            first.setdefault(lib, idx)
+    return b
+
+
+def oracle_scene_pairs(orc, rs, mats, scene, fp64=False, self_collide=False, perms=None):
+    """Oracle detect over one scene's meshes (batch order); rows mapped to
+    batch mesh indices."""
+    ids = [i for i, (s, _) in enumerate(rs.instances) if s == scene]
+    trees = []
+    for i in ids:
+        xyz, f = rs.library[rs.instances[i][1]]
+        y = orc.xform(mats[i], xyz)
+        trees.append(orc.Tree(y, f, order="morton" if perms is None else perms[i]))
+    if not trees:
+        return np.zeros((0, 4), np.uint32)
+    p = orc.detect(trees, self_collide=self_collide, fp64=fp64)
+    if len(p) == 0:
+        return np.zeros((0, 4), np.uint32)
+    p = p.copy()
+    p[:, 0] = np.asarray(ids, np.uint32)[p[:, 0]]
+    p[:, 1] = np.asarray(ids, np.uint32)[p[:, 1]]
+    return p
+
+
+def all_pairs(orc, rs, mats, **kw):
+    rows = [oracle_scene_pairs(orc, rs, mats, s, **kw) for s in range(rs.n_scenes)]
+    rows = [r for r in rows if len(r)]
+    if not rows:
+        return np.zeros((0, 4), np.uint32)
+    p = np.concatenate(rows)
+    return p[np.lexsort((p[:, 3], p[:, 2], p[:, 1], p[:, 0]))]
+
+
+def test_batch_trees_and_pairs(api, ctx, orc):
+    rs = small_rigid()
+    say(f"batch: {len(rs.instances)} meshes in {rs.n_scenes} scenes, {rs.triangles} triangles")
+    b = make_batch(api, ctx, rs)
+    mats = rs.transforms(0)
+    b.set_transforms(mats)
+    b.build()
+    for i, (s, lib) in enumerate(rs.instances):
+        xyz, f = rs.library[lib]
+        o = orc.Tree(orc.xform(mats[i], xyz), f, order="morton")
+        assert np.array_equal(b.perm(i), o.order), f"mesh {i}: leaf permutation differs"
+        assert np.array_equal(bits(b.nodes(i)), bits(o.nodes())), f"mesh {i}: nodes differ"
+    for entry, expand in [(0, 1), (2, 2), (3, 3)]:
+        b.detect(entry, expand)
+        want = all_pairs(orc, rs, mats)
+        got = b.pairs()
+        assert len(want) > 0
+        assert np.array_equal(got, want), f"entry {entry} expand {expand}"
+    # refit after motion
+    for frame in (3, 9):
+        mats = rs.transforms(frame)
+        b.set_transforms(mats)
+        b.refit()
+        b.detect(2, 2)
+        perms = [b.perm(i) for i in range(b.mesh_count)]
+        want = all_pairs(orc, rs, mats, perms=perms)
+        assert np.array_equal(b.pairs(), want), f"frame {frame}"
+        for i in (0, b.mesh_count - 1):
+            xyz, f = rs.library[rs.instances[i][1]]
+            o = orc.Tree(orc.xform(mats[i], xyz), f, order=perms[i])
+            assert np.array_equal(bits(b.nodes(i)), bits(o.nodes()))
+    b.close()
+
+
+def test_batch_without_instances_equals_with(api, ctx, orc):
+    rs = small_rigid(n_scenes=4, seed=0x1234)
+    mats = rs.transforms(2)
+    out = []
+    for inst in (True, False):
+        b = make_batch(api, ctx, rs, instances=inst)
+        b.set_transforms(mats)
+        b.build()
+        b.detect(1, 2)
+        out.append(b.pairs())
+        b.close()
+    assert np.array_equal(out[0], out[1])
+    assert np.array_equal(out[0], all_pairs(orc, rs, mats))
+
+
+def test_batch_many_meshes_two_mesh_passes(api, ctx, orc):
+    """> 256 meshes: the mesh-id sort takes two 8-bit passes."""
+    rs = M.RigidScenes(n_scenes=90, seed=0x77, meshes=(2, 5), faces=(12, 400), radius=0.35)
+    assert len(rs.instances) > 256
+    say(f"batch: {len(rs.instances)} meshes")
+    b = make_batch(api, ctx, rs)
+    mats = rs.transforms(1)
+    b.set_transforms(mats)
+    b.build()
+    for i in range(0, len(rs.instances), 37):
+        xyz, f = rs.library[rs.instances[i][1]]
+        o = orc.Tree(orc.xform(mats[i], xyz), f, order="morton")
+        assert np.array_equal(b.perm(i), o.order)
+        assert np.array_equal(bits(b.nodes(i)), bits(o.nodes()))
+    b.detect(0, 2)
+    assert np.array_equal(b.pairs(), all_pairs(orc, rs, mats))
+    b.close()
+
+
+def test_batch_self_collision(api, ctx, orc):
+    say("batch: cloth self collision")
+    cl = [M.cloth(24 + 6 * i, 0.2, 0.25, seed=40 + i) for i in range(3)]
+    b = api.Batch(ctx)
+    for s, (x, f) in enumerate(cl):
+        b.add_mesh(s, x, f)
+    b.build()
+    b.detect(2, 2, self_collide=True)
+    got = b.pairs()
+    rows = []
+    for s, (x, f) in enumerate(cl):
+        p = orc.detect([orc.Tree(x, f, order="morton")], self_collide=True)
+        if len(p):
+            p = p.copy()
+            p[:, 0] = s
+            p[:, 1] = s
+            rows.append(p)
+    want = np.concatenate(rows)
+    want = want[np.lexsort((want[:, 3], want[:, 2], want[:, 1], want[:, 0]))]
+    assert len(want) > 0
+    assert np.array_equal(got, want)
+    assert b.stats()["adjacent"] > 0
+    b.close()
+
+
+def test_batch_precision_fp64_fallback(api, ctx, orc):
+    """Coplanar / edge-touching scenes: FP32 filter + FP64 re-check equals the
+    all-FP64 oracle exactly; the plain FP32 path equals the FP32 oracle."""
+    scenes = M.precision_scenes(n_scenes=8, n_plane=24, n_box=10)
+    b = api.Batch(ctx)
+    for s, ms in enumerate(scenes):
+        for x, f in ms:
+            b.add_mesh(s, x, f)
+    b.build()
+
+    def want(fp64):
+        rows = []
+        for s, ms in enumerate(scenes):
+            p = orc.detect([orc.Tree(x, f, order="morton") for x, f in ms], fp64=fp64)
+            if len(p):
+                p = p.copy()
+                p[:, 0] = 2 * s + p[:, 0]
+                p[:, 1] = 2 * s + p[:, 1]
+                rows.append(p)
+        w = np.concatenate(rows)
+        return w[np.lexsort((w[:, 3], w[:, 2], w[:, 1], w[:, 0]))]
+
+    b.detect(2, 2, fp64_fallback=True)
+    w64 = want(True)
+    assert np.array_equal(b.pairs(), w64)
+    assert b.stats()["fp64_rechecks"] > 0
+    b.detect(2, 2)
+    w32 = want(False)
+    assert np.array_equal(b.pairs(), w32)
+    say(f"precision: {len(w64)} FP64 pairs, {len(w32)} FP32 pairs")
+    b.close()
+
+
+def test_batch_frame_graph_replays(api, ctx, orc):
+    """refit + detect replayed from a captured graph: new transforms between
+    replays (in stream order), unrelated eager work and synchronisations in
+    between, every replay checked against an eager refit + detect."""
+    rs = small_rigid(n_scenes=5, seed=0xAB)
+    b = make_batch(api, ctx, rs)
+    ref = make_batch(api, ctx, rs)
+    b.set_transforms(rs.transforms(0))
+    ref.set_transforms(rs.transforms(0))
+    b.build()
+    ref.build()
+    side = api.Scene(ctx)
+    (x0, f0), (x1, f1) = M.interpenetrating_icospheres(subdiv=2, offset=1.0)
+    t0, t1 = api.OibvhTree(ctx, x0, f0), api.OibvhTree(ctx, x1, f1)
+    t0.build()
+    t1.build()
+    side.addOibvhTree(t0)
+    side.addOibvhTree(t1)
+    side_want = orc.detect([orc.Tree(x0, f0, order="morton"), orc.Tree(x1, f1, order="morton")])
+    for frame in range(1, 7):
+        mats = rs.transforms(frame)
+        b.set_transforms(mats)
+        b.frame(2, 2)
+        got = b.pairs()
+        ref.set_transforms(mats)
+        ref.refit()
+        ref.detect(2, 2)
+        assert np.array_equal(got, ref.pairs()), f"replay {frame}"
+        for i in (0, b.mesh_count // 2):
+            assert np.array_equal(bits(b.nodes(i)), bits(ref.nodes(i)))
+        side.detectCollision(1, 1)
+        assert np.array_equal(side.getIntTriPairs(), side_want)
+        ctx.synchronize()
+    perms = [b.perm(i) for i in range(b.mesh_count)]
+    assert np.array_equal(got, all_pairs(orc, rs, mats, perms=perms))
+    b.close()
+    ref.close()
+
+
+def test_batch_overflow_retry(api, orc):
+    say("batch: tiny detect buffers")
+    rs = small_rigid(n_scenes=4, seed=0x99)
+    ctx2 = api.Context(0)
+    ctx2.set_tuning(front_capacity=256, hits_capacity=16, candidate_capacity=64)
+    b = make_batch(api, ctx2, rs)
+    mats = rs.transforms(0)
+    b.set_transforms(mats)
+    b.build()
+    b.frame(1, 1)
+    got = b.pairs()
+    assert b.stats()["retries"] > 0
+    assert np.array_equal(got, all_pairs(orc, rs, mats))
+    b.frame(1, 1)  # captured again over the grown buffers
+    assert np.array_equal(b.pairs(), got)
+    b.close()
+    ctx2.close()
