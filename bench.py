@@ -14,6 +14,17 @@ that takes most of the frame, from algorithmic bytes per launch over that
 kernel's average duration (events around each launch in the timed region);
 `cpu_baseline` times the oracle (the CPU restatement of the reference CPU
 path, one thread) on the same workload.
+
+The same line carries the added forms, each on its own workload:
+  batched          512 scenes of 2-16 rigid meshes (noisy icospheres and
+                   boxes, 1k-200k triangles each, log-uniform): the batched
+                   FP32 + FP64-fallback frame (one hipGraph per frame) against
+                   the plain per-tree / per-scene API on the same frames
+  precision        64 scenes of two ~250k-triangle planes or boxes placed
+                   coplanar or edge-touching: FP64 re-check rate and pair
+                   differences against the FP64 oracle
+  self_collision   8 folded cloth sheets of 2048 x 2048 vertices: refit +
+                   self-BVTT query per frame
 """
 from __future__ import annotations
 
@@ -48,7 +59,20 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-verify", action="store_true")
+    p.add_argument("--sections", default="plain,batched,precision,self",
+                   help="comma list of: plain (the headline), batched, precision, self")
+    p.add_argument("--batched-frames", type=int, default=100)
+    p.add_argument("--plain-frames", type=int, default=5, help="frames of the plain form on the batched workload")
+    p.add_argument("--batched-scenes", type=int, default=512)
+    p.add_argument("--precision-frames", type=int, default=20)
+    p.add_argument("--self-frames", type=int, default=100)
+    p.add_argument("--cloth-nv", type=int, default=2048)
     return p.parse_args()
+
+
+def progress(msg: str) -> None:
+    """Progress to stderr (stdout carries only the JSON line)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
 def cpu_model():
@@ -94,12 +118,7 @@ class EventPairs:
             self.H.hipEventDestroy(e)
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this pipeline runs on a single "
-              "MI355X; multi-GPU sharding is described in DESIGN.md and not built)", file=sys.stderr)
-        sys.exit(2)
+def run_plain(args, ctx):
     from oibvh_amd import _lib as L
     from oibvh_amd import api
     from oibvh_amd import meshgen as M
@@ -127,7 +146,6 @@ def main():
         frames = [[m[0]] for m in meshes]
     t_gen = time.time() - t_gen
 
-    ctx = api.Context(0)
     H = L.hip()
     bufs = []
     for m in range(len(meshes)):
@@ -174,6 +192,7 @@ def main():
 
     shape = [args.entry_level if args.entry_level is not None else 4,
              args.expand_levels if args.expand_levels is not None else 1]
+    progress("plain pair: warm-up")
     # warm-up: synchronous detects size the device buffers; then the autotuner
     # picks entry level, levels per BVTT step, workgroups per launch and the
     # reduce subtree size on the live scene (none of them changes results)
@@ -237,6 +256,7 @@ def main():
                 "unit": "GB/s", "frac": (dk["gbps"] * 1e9 / HBM_PEAK) if dk["gbps"] else None,
                 "traffic": None, "alg_bytes_per_launch": dk["alg_bytes"], "avg_launch_ms": dk["avg_ms"]}
 
+    progress(f"plain pair: {fps:.1f} frames/s; parity and CPU baseline")
     # ---------------------------------------------------------------- parity
     diffs = None
     if not args.no_verify:
@@ -309,9 +329,382 @@ def main():
         "cpu_baseline": cpu,
         "host_gen_s": t_gen,
     }
-    print(json.dumps(out), flush=True)
     for e in (ev_refit, ev_build, ev_detect):
         e.close()
+    for t in trees:
+        t.close()
+    scene.close()
+    for b in bufs:
+        b.free()
+    return out
+
+def _colmajor(mats: np.ndarray) -> np.ndarray:
+    """(..., 4, 4) matrices -> column-major float32 (glm::mat4 layout)."""
+    return np.ascontiguousarray(np.swapaxes(np.asarray(mats, np.float32), -1, -2))
+
+
+def _pair_diff(a: np.ndarray, b: np.ndarray) -> int:
+    sa = set(map(tuple, np.asarray(a).tolist()))
+    sb = set(map(tuple, np.asarray(b).tolist()))
+    return len(sa ^ sb)
+
+
+def run_batched(args, ctx):
+    """Batched FP32 + FP64-fallback frames against the plain API on the same
+    512-scene workload."""
+    import ctypes as C
+
+    from oibvh_amd import _lib as L
+    from oibvh_amd import api, autotune
+    from oibvh_amd import meshgen as M
+
+    progress("batched: generating scenes")
+    t0 = time.time()
+    rs = M.RigidScenes(n_scenes=args.batched_scenes)
+    n = len(rs.instances)
+    nf = max(2, args.batched_frames)
+    mats = np.stack([_colmajor(rs.transforms(f)) for f in range(nf)])  # (nf, n, 4, 4) column-major
+    t_gen = time.time() - t0
+    xbuf = api.DeviceBuffer(mats.nbytes)
+    xbuf.upload(mats)
+    stride = n * 64
+
+    # ---------------- batched
+    b = api.Batch(ctx)
+    first = {}
+    for scene, lib in rs.instances:
+        if lib in first:
+            b.add_instance(scene, first[lib])
+        else:
+            xyz, f = rs.library[lib]
+            first[lib] = b.add_mesh(scene, xyz, f)
+    b.set_transforms_device(xbuf.at(0))
+    t1 = time.time()
+    b.build()
+    ctx.synchronize()
+    t_build = time.time() - t1
+
+    def stage(i):
+        b.set_transforms_device(xbuf.at((i % nf) * stride))
+
+    progress(f"batched: built {n} meshes in {t_build:.2f} s; autotuning")
+    tuned = autotune.tune_batch(ctx, b, stage, flags={"fp64_fallback": True})
+    progress(f"batched: tuned {tuned}")
+    entry, expand = tuned["entry_level"], tuned["expand_levels"]
+    # warm-up, then the timed frames: frame f = transforms of frame f, refit, detect
+    for i in range(3):
+        stage(i)
+        b.frame(entry, expand, fp64_fallback=True)
+    timer = api.Timer(ctx)
+    ctx.synchronize()
+    timer.start()
+    for f in range(nf):
+        stage(f)
+        b.frame(entry, expand, fp64_fallback=True, asynchronous=True)
+    timer.stop()
+    ms_b = timer.ms() / nf
+    st_b = b.stats()
+    pairs_b64 = b.pairs()
+    lc = b.level_counts().tolist()
+    # the last frame again in plain FP32 for the bit-exact comparison with the plain API
+    stage(nf - 1)
+    b.refit()
+    b.detect(entry, expand)
+    pairs_b32 = b.pairs()
+
+    progress(f"batched: {1000.0 / ms_b:.1f} frames/s; setting up the plain form")
+    # ---------------- plain: one tree per mesh, one scene per scene
+    lib = L.load()
+    t1 = time.time()
+    trees = []
+    for i, (scene, li) in enumerate(rs.instances):
+        xyz, f = rs.library[li]
+        t = api.OibvhTree(ctx, xyz, f)
+        trees.append(t)
+    scenes = [api.Scene(ctx) for _ in range(rs.n_scenes)]
+    fptr = C.POINTER(C.c_float)
+    for i, t in enumerate(trees):
+        lib.oibvh_tree_set_transform(t.h, mats[0, i].ctypes.data_as(fptr))
+        t.build()
+    for i, (scene, _) in enumerate(rs.instances):
+        scenes[scene].addOibvhTree(trees[i])
+    ctx.synchronize()
+    t_plain_setup = time.time() - t1
+    hs = [t.h for t in trees]
+    shs = [s.h for s in scenes if len(s.trees) > 1]
+    async_flags = L.DETECT_ASYNC
+
+    def plain_frame(f):
+        m = mats[f % nf]
+        for i, h in enumerate(hs):
+            lib.oibvh_tree_set_transform(h, m[i].ctypes.data_as(fptr))
+            lib.oibvh_tree_refit(h, None, 0)
+        for sh in shs:
+            lib.oibvh_scene_detect(sh, entry, expand, async_flags)
+
+    progress(f"plain: {len(trees)} trees built in {t_plain_setup:.1f} s")
+    plain_frame(0)
+    for s_ in scenes:
+        s_.getIntTriPairCount()
+    kp = max(1, args.plain_frames)
+    ctx.synchronize()
+    tw = time.time()
+    timer.start()
+    for f in range(nf - kp, nf):
+        plain_frame(f)
+    timer.stop()
+    ms_p_gpu = timer.ms() / kp
+    ms_p = (time.time() - tw) * 1000.0 / kp
+    rows = []
+    for si, s_ in enumerate(scenes):
+        if len(s_.trees) < 2:
+            continue
+        p = s_.getIntTriPairs()
+        if len(p):
+            ids = np.asarray([i for i, (sc, _) in enumerate(rs.instances) if sc == si], np.uint32)
+            p = p.copy()
+            p[:, 0] = ids[p[:, 0]]
+            p[:, 1] = ids[p[:, 1]]
+            rows.append(p)
+    pairs_plain = np.concatenate(rows) if rows else np.zeros((0, 4), np.uint32)
+    diff_plain = _pair_diff(pairs_b32, pairs_plain)
+
+    progress(f"plain: {1000.0 / ms_p:.2f} frames/s; oracle check")
+    # ---------------- FP64 oracle on a sample of scenes (last frame)
+    diffs64 = None
+    sample = []
+    if not args.no_verify:
+        import oracle
+
+        last = mats[nf - 1]
+        sample = list(range(0, rs.n_scenes, max(1, rs.n_scenes // 8)))[:8]
+        diffs64 = 0
+        for si in sample:
+            ids = [i for i, (sc, _) in enumerate(rs.instances) if sc == si]
+            ot = []
+            for i in ids:
+                xyz, f = rs.library[rs.instances[i][1]]
+                ot.append(oracle.Tree(oracle.xform(last[i].T, xyz), f, order=b.perm(i)))
+            want = oracle.detect(ot, fp64=True)
+            if len(want):
+                want = want.copy()
+                want[:, 0] = np.asarray(ids, np.uint32)[want[:, 0]]
+                want[:, 1] = np.asarray(ids, np.uint32)[want[:, 1]]
+            got = pairs_b64[np.isin(pairs_b64[:, 0], ids)] if len(pairs_b64) else pairs_b64
+            diffs64 += _pair_diff(got, want)
+
+    # overlap statistic: mesh pairs of a scene with at least one hit
+    hit_pairs = len(set(map(tuple, pairs_b64[:, :2].tolist()))) if len(pairs_b64) else 0
+    mesh_pairs = 0
+    per_scene = np.bincount([s for s, _ in rs.instances], minlength=rs.n_scenes)
+    mesh_pairs = int(sum(k * (k - 1) // 2 for k in per_scene))
+    tri = rs.triangles
+    res = {
+        "workload": (f"{rs.n_scenes} scenes, {n} rigid meshes (2-16 per scene; noisy icospheres and boxes, "
+                     f"log-uniform 1k-200k triangles each; {len(rs.library)} base shapes instanced), "
+                     f"{tri} triangles in all, radius 0.25, translations in the unit cube, spin + drift per "
+                     f"frame; refit + broad + narrow every frame, build at frame 0"),
+        "triangles": tri,
+        "meshes": n,
+        "frames": nf,
+        "batched_fps": 1000.0 / ms_b,
+        "batched_ms_per_frame": ms_b,
+        "batched_mtri_per_s": tri / (ms_b * 1e-3) / 1e6,
+        "batched_form": "FP32 + FP64 fallback, one hipGraph per frame (refit + detect), model matrices staged "
+                        "device-to-device from HBM before each frame",
+        "plain_fps": 1000.0 / ms_p,
+        "plain_ms_per_frame": ms_p,
+        "plain_gpu_ms_per_frame": ms_p_gpu,
+        "plain_frames": kp,
+        "plain_form": "one oibvh_tree per mesh (set_transform + refit) and one oibvh_scene per scene (asynchronous "
+                      "detect), FP32, driven through the C ABI from Python",
+        "speedup_batched_vs_plain": ms_p / ms_b,
+        "pairs_last_frame": int(len(pairs_b64)),
+        "pair_diffs_batched_fp32_vs_plain": diff_plain,
+        "pair_diffs_vs_fp64_oracle_sampled": diffs64,
+        "oracle_sample_scenes": sample,
+        "fp64_rechecks_last_frame": st_b["fp64_rechecks"],
+        "candidates_last_frame": st_b["candidates"],
+        "bvtt_tests_last_frame": st_b["bvtt_tests"],
+        "bvtt_levels_last_frame": lc,
+        "mesh_pairs_overlapping": hit_pairs,
+        "mesh_pairs_total": mesh_pairs,
+        "autotune": tuned,
+        "build_s": t_build,
+        "plain_setup_s": t_plain_setup,
+        "host_gen_s": t_gen,
+    }
+    for t in trees:
+        t.close()
+    for s_ in scenes:
+        s_.close()
+    b.close()
+    xbuf.free()
+    return res
+
+
+def run_precision(args, ctx):
+    """Coplanar / edge-touching scenes: FP32 filter + FP64 re-check."""
+    from oibvh_amd import api
+    from oibvh_amd import meshgen as M
+
+    progress("precision: generating scenes")
+    t0 = time.time()
+    scenes = M.precision_scenes()
+    t_gen = time.time() - t0
+    b = api.Batch(ctx)
+    for si, ms in enumerate(scenes):
+        for x, f in ms:
+            b.add_mesh(si, x, f)
+    b.build()
+    tri = int(sum(f.shape[0] for ms in scenes for _, f in ms))
+    entry, expand = 4, 2
+    b.frame(entry, expand, fp64_fallback=True)
+    timer = api.Timer(ctx)
+    k = max(1, args.precision_frames)
+    ctx.synchronize()
+    timer.start()
+    for _ in range(k):
+        b.frame(entry, expand, fp64_fallback=True, asynchronous=True)
+    timer.stop()
+    ms = timer.ms() / k
+    st = b.stats()
+    p64 = b.pairs()
+    b.detect(entry, expand)
+    p32 = b.pairs()
+    res = {"workload": (f"{len(scenes)} scenes of two meshes (planes 2*354^2 = 250,632 or boxes 12*144^2 = "
+                        f"248,832 triangles) coplanar or edge-touching, perturbations 10^U[-7,-5]; {tri} triangles"),
+           "triangles": tri, "frames": k, "fps": 1000.0 / ms, "ms_per_frame": ms,
+           "pairs_fp64_fallback": int(len(p64)), "pairs_fp32": int(len(p32)),
+           "candidates": st["candidates"], "fp64_axis_rechecks": st["fp64_rechecks"],
+           "fp64_recheck_rate_per_axis_test": st["fp64_rechecks"] / max(1, 17 * st["candidates"])}
+    progress(f"precision: {1000.0 / ms:.1f} frames/s; oracle check")
+    if not args.no_verify:
+        import oracle
+
+        sample = [0, 1, 2, 3, 4, 5, 6, 7]
+        d64 = miss = spur = 0
+        for si in sample:
+            ot = [oracle.Tree(x, f, order=b.perm(2 * si + j)) for j, (x, f) in enumerate(scenes[si])]
+            want = oracle.detect(ot, fp64=True)
+            if len(want):
+                want = want.copy()
+                want[:, :2] += 2 * si
+            sel64 = p64[(p64[:, 0] // 2) == si] if len(p64) else p64
+            sel32 = p32[(p32[:, 0] // 2) == si] if len(p32) else p32
+            d64 += _pair_diff(sel64, want)
+            a32 = set(map(tuple, sel32.tolist()))
+            a64 = set(map(tuple, want.tolist()))
+            miss += len(a64 - a32)
+            spur += len(a32 - a64)
+        res.update({"oracle_sample_scenes": sample, "pair_diffs_vs_fp64_oracle": d64,
+                    "fp32_path_missed_vs_fp64": miss, "fp32_path_spurious_vs_fp64": spur})
+    res["host_gen_s"] = t_gen
+    b.close()
+    return res
+
+
+def run_self(args, ctx):
+    """Folded cloth sheets: refit + self query per frame, 8 instances per launch."""
+    from oibvh_amd import api
+    from oibvh_amd import meshgen as M
+
+    nv = args.cloth_nv
+    progress("self: generating cloth frames")
+    t0 = time.time()
+    n_inst, n_distinct, dphase = 8, 8, 0.02
+    base = M.cloth_instances(n_inst, nv)
+    t_gen = time.time() - t0
+    V = nv * nv
+    # deformation frames: the height field's phase advances dphase per frame
+    bufs = []
+    for d in range(n_distinct):
+        buf = api.DeviceBuffer(n_inst * V * 12)
+        for i in range(n_inst):
+            buf.upload(M.cloth_positions(nv, 0.2, 0.05, M.CFG5_SEED + i, phase=dphase * d), i * V * 12)
+        bufs.append(buf)
+    t_gen = time.time() - t0
+    b = api.Batch(ctx)
+    for i, (x, f) in enumerate(base):
+        b.add_mesh(i, x, f)
+    b.build()
+
+    def stage(k):
+        buf = bufs[k % n_distinct]
+        for i in range(n_inst):
+            b.set_positions_device(i, buf.at(i * V * 12))
+
+    from oibvh_amd import autotune
+
+    progress("self: built; autotuning")
+    ctx.set_tuning(front_capacity=1 << 29)
+    tuned = autotune.tune_batch(ctx, b, stage, entries=(4, 6), expands=(2, 3), blocks=(0,), leaves=(0,),
+                                flags={"self_collide": True}, reps=2)
+    progress(f"self: tuned {tuned}")
+    entry, expand = tuned["entry_level"], tuned["expand_levels"]
+    stage(0)
+    b.frame(entry, expand, self_collide=True)
+    timer = api.Timer(ctx)
+    k = max(1, args.self_frames)
+    ctx.synchronize()
+    timer.start()
+    for j in range(k):
+        stage(j)
+        b.frame(entry, expand, self_collide=True, asynchronous=True)
+    timer.stop()
+    ms = timer.ms() / k
+    st = b.stats()
+    pairs = b.pairs()
+    tri = int(sum(f.shape[0] for _, f in base))
+    res = {"workload": (f"{n_inst} folded cloth sheets of {nv} x {nv} vertices ({base[0][1].shape[0]} triangles "
+                        f"each, {tri} in all), amplitude 0.2, wavelength 0.05, fold 1.5; the height field's phase "
+                        f"advances {dphase} rad per frame ({n_distinct} distinct frames cycled, resident in HBM); "
+                        f"refit + self-BVTT query (adjacent pairs excluded) per frame"),
+           "triangles": tri, "frames": k, "fps": 1000.0 / ms, "ms_per_frame": ms,
+           "mtri_per_s": tri / (ms * 1e-3) / 1e6, "pairs_last_frame": int(len(pairs)),
+           "adjacent_skipped": st["adjacent"], "candidates": st["candidates"], "bvtt_tests": st["bvtt_tests"],
+           "bvtt_levels_last_frame": b.level_counts().tolist(), "autotune": tuned, "host_gen_s": t_gen}
+    if not args.no_verify:
+        import oracle
+
+        # parity spot check at reduced size (the full sheet is covered by the
+        # GPU tests' smaller sheets and the batch's per-mesh equivalence)
+        x, f = M.cloth(128, 0.2, 0.05, M.CFG5_SEED)
+        c = api.Batch(ctx)
+        c.add_mesh(0, x, f)
+        c.build()
+        c.detect(2, 2, self_collide=True)
+        want = oracle.detect([oracle.Tree(x, f, order="morton")], self_collide=True)
+        res["spot_check"] = {"cloth_nv": 128, "pairs": int(len(want)), "pair_diffs": _pair_diff(c.pairs(), want)}
+        c.close()
+    b.close()
+    for buf in bufs:
+        buf.free()
+    return res
+
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this pipeline runs on a single "
+              "MI355X; multi-GPU sharding is described in DESIGN.md and not built)", file=sys.stderr)
+        sys.exit(2)
+    from oibvh_amd import api
+
+    sections = set(args.sections.split(","))
+    ctx = api.Context(0)
+    out = run_plain(args, ctx) if "plain" in sections else {"metric": METRIC, "value": None}
+    extra = [("batched", run_batched), ("precision", run_precision), ("self", run_self)]
+    for name, fn in extra:
+        if name in sections:
+            base = ctx.get_tuning()
+            t0 = time.time()
+            res = fn(args, ctx)
+            res["section_wall_s"] = time.time() - t0
+            out["self_collision" if name == "self" else name] = res
+            ctx.set_tuning(**base)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

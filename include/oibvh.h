@@ -249,6 +249,15 @@ OIBVH_API oibvh_status oibvh_batch_set_transforms(oibvh_batch* batch, const floa
  * (deforming meshes); the next refit or frame uses them. */
 OIBVH_API oibvh_status oibvh_batch_positions_device(oibvh_batch* batch, uint32_t mesh, float** dptr);
 
+/* New vertex positions for one mesh (V*3 floats), as oibvh_tree_set_positions:
+ * host memory is copied before return, OIBVH_PTR_DEVICE is copied in stream
+ * order into the batch's own buffer, OIBVH_PTR_DEVICE_BORROW makes the mesh
+ * read the caller's memory in place from then on (switched in stream order,
+ * so captured frames follow it). Instances of the mesh keep their source's
+ * vertices. The next build, refit or frame uses them. */
+OIBVH_API oibvh_status oibvh_batch_set_positions(oibvh_batch* batch, uint32_t mesh, const float* xyz,
+                                                 uint32_t ptr_kind);
+
 /* Build every tree (OibvhTree::build per mesh, oibvhTree.cu:237-388). */
 OIBVH_API oibvh_status oibvh_batch_build(oibvh_batch* batch);
 /* Refit every tree (OibvhTree::refit per mesh, oibvhTree.cu:193-235). */

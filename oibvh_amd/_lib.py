@@ -82,6 +82,7 @@ API = [
     ("oibvh_batch_mesh_count", _i, [_vp, _u32p]),
     ("oibvh_batch_set_transforms", _i, [_vp, _vp, _u32]),
     ("oibvh_batch_positions_device", _i, [_vp, _u32, C.POINTER(_vp)]),
+    ("oibvh_batch_set_positions", _i, [_vp, _u32, _vp, _u32]),
     ("oibvh_batch_build", _i, [_vp]),
     ("oibvh_batch_refit", _i, [_vp]),
     ("oibvh_batch_detect", _i, [_vp, _u32, _u32, _u32]),

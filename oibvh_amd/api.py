@@ -335,6 +335,15 @@ class Batch:
         _check(self.ctx.h, self._lib.oibvh_batch_positions_device(self.h, mesh, C.byref(p)), "positions_device")
         return p.value
 
+    def set_positions(self, mesh: int, xyz) -> None:
+        x = np.ascontiguousarray(xyz, np.float32)
+        _check(self.ctx.h, self._lib.oibvh_batch_set_positions(self.h, mesh, x.ctypes.data, L.PTR_HOST),
+               "batch set_positions")
+
+    def set_positions_device(self, mesh: int, dptr: int, borrow: bool = True) -> None:
+        kind = L.PTR_DEVICE_BORROW if borrow else L.PTR_DEVICE
+        _check(self.ctx.h, self._lib.oibvh_batch_set_positions(self.h, mesh, dptr, kind), "batch set_positions")
+
     def build(self) -> None:
         _check(self.ctx.h, self._lib.oibvh_batch_build(self.h), "batch build")
 

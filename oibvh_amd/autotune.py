@@ -55,3 +55,45 @@ def tune(ctx: api.Context, scene: api.Scene, frame, *, entries=(4, 6, 8), expand
                       "reduce_leaves_per_thread": best["reduce_leaves_per_thread"],
                       "traverse_blocks": best["traverse_blocks"], "traverse_expand": best["expand_levels"]})
     return best
+
+
+def tune_batch(ctx: api.Context, batch: api.Batch, before_frame, *, entries=(0, 2, 4), expands=(1, 2, 3),
+               blocks=(0, 4096), leaves=(0, 4, 8), flags=None, reps: int = 3) -> dict:
+    """Pick the reduce subtree size (leaves per thread), the BVTT entry level,
+    the levels per step and the workgroups per launch for a batch's captured
+    frame (refit + detect). before_frame(i) stages frame i's inputs (model
+    matrices or vertex pointers) in stream order."""
+    flags = flags or {}
+    base = ctx.get_tuning()
+    timer = api.Timer(ctx)
+
+    def run(entry, expand):
+        before_frame(0)
+        batch.frame(entry, expand, **flags)  # captures, sizes buffers
+        ctx.synchronize()
+        timer.start()
+        for i in range(reps):
+            before_frame(i + 1)
+            batch.frame(entry, expand, asynchronous=True, **flags)
+        timer.stop()
+        ms = timer.ms() / reps
+        batch.pair_count()
+        return ms
+
+    lpt_best, lpt_ms = 0, None
+    for lpt in leaves:
+        ctx.set_tuning(reduce_leaves_per_thread=lpt)
+        ms = run(entries[0], expands[0])
+        if lpt_ms is None or ms < lpt_ms:
+            lpt_best, lpt_ms = lpt, ms
+    ctx.set_tuning(reduce_leaves_per_thread=lpt_best)
+    best = None
+    for entry, expand, nb in itertools.product(entries, expands, blocks):
+        ctx.set_tuning(traverse_blocks=nb, traverse_expand=expand)
+        ms = run(entry, expand)
+        if best is None or ms < best["ms"]:
+            best = {"entry_level": entry, "expand_levels": expand, "traverse_blocks": nb,
+                    "reduce_leaves_per_thread": lpt_best, "ms": ms}
+    ctx.set_tuning(**{**base, "reduce_leaves_per_thread": best["reduce_leaves_per_thread"],
+                      "traverse_blocks": best["traverse_blocks"], "traverse_expand": best["expand_levels"]})
+    return best

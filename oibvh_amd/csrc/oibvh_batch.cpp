@@ -44,6 +44,7 @@ struct oibvh_batch {
     float* partials = nullptr;
     float* xf = nullptr;
     bool has_xf = false;
+    bool borrowed = false;  // some mesh reads caller-owned vertex memory
     uint32_t *keys = nullptr, *keys_alt = nullptr, *vals = nullptr, *vals_tmp = nullptr, *fmesh = nullptr;
     uint32_t* sort_scratch = nullptr;  // ghist[1024] | tile counters[8] | lookback0 | lookback1
     uint32_t* mesh_hist = nullptr;     // digit histograms of the mesh ids, 256 per pass
@@ -237,7 +238,15 @@ static void fill_descs(oibvh_batch* b)
 static oibvh_status upload_descs(oibvh_batch* b)
 {
     oibvh_ctx* c = b->ctx;
+    // keep borrowed vertex pointers across a refill
+    std::vector<const float*> keep;
+    for (const oi_build_desc& d : b->h_bdesc) keep.push_back(d.pos);
     fill_descs(b);
+    if (b->borrowed)
+        for (size_t i = 0; i < keep.size(); i++) {
+            b->h_bdesc[i].pos = keep[i];
+            b->h_mdesc[i].pos = keep[i];
+        }
     HIPCHK(c, hipMemcpy(b->d_bdesc, b->h_bdesc.data(), b->h_bdesc.size() * sizeof(oi_build_desc),
                         hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(b->d_mdesc, b->h_mdesc.data(), b->h_mdesc.size() * sizeof(oi_mesh_desc),
@@ -407,6 +416,38 @@ extern "C" oibvh_status oibvh_batch_positions_device(oibvh_batch* b, uint32_t me
     oibvh_status r = commit(b);
     if (r != OIBVH_OK) return r;
     *dptr = b->pos + 3ull * b->meshes[b->meshes[mesh].src].vbase;
+    return OIBVH_OK;
+}
+
+extern "C" oibvh_status oibvh_batch_set_positions(oibvh_batch* b, uint32_t mesh, const float* xyz, uint32_t ptr_kind)
+{
+    if (!b || !xyz || mesh >= b->meshes.size() || ptr_kind > OIBVH_PTR_DEVICE_BORROW) return OIBVH_ERR_INVALID;
+    oibvh_ctx* c = b->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    oibvh_status r = commit(b);
+    if (r != OIBVH_OK) return r;
+    const oibvh_batch::Mesh& m = b->meshes[mesh];
+    if (ptr_kind == OIBVH_PTR_DEVICE_BORROW) {
+        // the kernels read the descriptor tables at run time: update them in
+        // stream order (the host copies follow for later uploads)
+        HIPCHK(c, oi_launch_set_mesh_pos(c->stream, b->d_bdesc, b->d_mdesc, mesh, xyz));
+        b->h_bdesc[mesh].pos = xyz;
+        b->h_mdesc[mesh].pos = xyz;
+        b->borrowed = true;
+        return OIBVH_OK;
+    }
+    float* own = b->pos + 3ull * b->meshes[m.src].vbase;
+    if (b->h_bdesc[mesh].pos != own) {
+        HIPCHK(c, oi_launch_set_mesh_pos(c->stream, b->d_bdesc, b->d_mdesc, mesh, own));
+        b->h_bdesc[mesh].pos = own;
+        b->h_mdesc[mesh].pos = own;
+    }
+    if (ptr_kind == OIBVH_PTR_HOST) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpy(own, xyz, 12ull * m.V, hipMemcpyHostToDevice));
+    } else {
+        HIPCHK(c, hipMemcpyAsync(own, xyz, 12ull * m.V, hipMemcpyDeviceToDevice, c->stream));
+    }
     return OIBVH_OK;
 }
 

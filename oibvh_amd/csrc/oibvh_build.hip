@@ -629,6 +629,16 @@ __global__ void k_set_xf(float* dst, Mat16 m)
     }
 }
 
+// point mesh m of a batch at other vertex memory, in stream order (so that a
+// captured frame replays against it without capturing again)
+__global__ void k_set_mesh_pos(oi_build_desc* bd, oi_mesh_desc* md, uint32_t m, const float* pos)
+{
+    if (threadIdx.x == 0) {
+        bd[m].pos = pos;
+        md[m].pos = pos;
+    }
+}
+
 static BuildTab make_btab(const oi_build_desc* d, const uint2* work, const oi_build_desc* one)
 {
     BuildTab t{};
@@ -666,6 +676,12 @@ hipError_t oi_launch_morton(hipStream_t st, const oi_build_desc* d, const uint2*
 {
     hipLaunchKernelGGL(k_morton, dim3(nblocks), dim3(OI_BLOCK), 0, st, make_btab(d, work, one), partials, keys,
                        ghist, fmesh);
+    return hipGetLastError();
+}
+
+hipError_t oi_launch_set_mesh_pos(hipStream_t st, oi_build_desc* bd, oi_mesh_desc* md, uint32_t m, const float* pos)
+{
+    hipLaunchKernelGGL(k_set_mesh_pos, dim3(1), dim3(64), 0, st, bd, md, m, pos);
     return hipGetLastError();
 }
 

@@ -437,6 +437,7 @@ struct StepArgs {
     uint32_t cand_cap;
     uint32_t* error;
     unsigned long long* tests;  // box pair tests (statistics)
+    unsigned long long* adj;    // self-mode leaf pairs dropped as adjacent
     uint32_t implicit_end;
 };
 
@@ -446,6 +447,30 @@ __device__ __forceinline__ void load_run(const float* nodes, uint32_t r0, uint32
 #pragma unroll
     for (int k = 0; k < (1 << E); k++)
         if ((uint32_t)k < n) load_box(nodes, r0 + k, b[k]);
+}
+
+// faces of leaf run [start, start + n) (implicit indices at level L)
+template <int E>
+__device__ __forceinline__ void load_faces(const uint32_t* sfaces, uint32_t leaf0, uint32_t n,
+                                           uint32_t (&f)[1 << E][3])
+{
+#pragma unroll
+    for (int k = 0; k < (1 << E); k++)
+        if ((uint32_t)k < n) {
+            f[k][0] = sfaces[3u * (leaf0 + k)];
+            f[k][1] = sfaces[3u * (leaf0 + k) + 1];
+            f[k][2] = sfaces[3u * (leaf0 + k) + 2];
+        }
+}
+
+__device__ __forceinline__ bool share_vertex(const uint32_t* a, const uint32_t* b)
+{
+    bool s = false;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) s |= (a[i] == b[j]);
+    return s;
 }
 
 template <int E, bool FIRST, class Tab>
@@ -463,7 +488,7 @@ __global__ void __launch_bounds__(TB) k_bvtt_step(StepArgs<Tab> A)
     if (blockIdx.x * TB >= n_in) return;
     tab_init(A.tab, s_desc, s_ibase);
     const uint32_t tid = threadIdx.x;
-    uint32_t n_tests = 0;
+    uint32_t n_tests = 0, n_adj = 0;
     for (uint32_t base = blockIdx.x * TB; base < n_in; base += gridDim.x * TB) {
         const uint32_t i = base + tid;
         bool have = i < n_in;
@@ -525,6 +550,24 @@ __global__ void __launch_bounds__(TB) k_bvtt_step(StepArgs<Tab> A)
                         if (keep) mask |= 1ull << (x * K + y);
                     }
                 }
+                // inside one mesh (self mode) leaf pairs sharing a vertex are
+                // not collisions: drop them before they become candidates
+                if (leafout && ma == mb && mask) {
+                    uint32_t fa[K][3], fb[K][3];
+                    const uint32_t leaf0 = (1u << ra.L) - 1u;
+                    load_faces<E>(ra.sfaces, sa.start - leaf0, sa.count, fa);
+                    load_faces<E>(ra.sfaces, sb.start - leaf0, sb.count, fb);
+#pragma unroll
+                    for (int x = 0; x < K; x++)
+#pragma unroll
+                        for (int y = 0; y < K; y++) {
+                            const u64 bit = 1ull << (x * K + y);
+                            if ((mask & bit) && share_vertex(fa[x], fb[y])) {
+                                mask &= ~bit;
+                                n_adj++;
+                            }
+                        }
+                }
             }
         }
         const uint32_t nout = (uint32_t)__popcll(mask);
@@ -551,9 +594,11 @@ __global__ void __launch_bounds__(TB) k_bvtt_step(StepArgs<Tab> A)
         }
         __syncthreads();
     }
-    uint32_t nt;
+    uint32_t nt, na;
     block_scan(n_tests, s_wsum, nt);
+    block_scan(n_adj, s_wsum, na);
     if (tid == 0 && nt) __hip_atomic_fetch_add(A.tests, (u64)nt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0 && na) __hip_atomic_fetch_add(A.adj, (u64)na, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------------------
@@ -718,6 +763,7 @@ static hipError_t launch_step(hipStream_t st, uint32_t grid, const Tab& tab, int
     a.cand_cap = cand_cap;
     a.error = &ctl->error;
     a.tests = &ctl->tests;
+    a.adj = &ctl->adj;
     a.implicit_end = implicit_end;
     const dim3 g(grid), b(TB);
 #define OI_STEP(EE)                                                        \

@@ -132,11 +132,12 @@ oibvh_status oi_det_check(oi_detector* d)
         uint32_t maxfront = 0;
         for (uint32_t v : d->h_counts) maxfront = std::max(maxfront, v);
         if (k.error & OI_ERR_QUEUE_OVERFLOW) {
-            const uint64_t need = oi_np2(std::max(maxfront, d->front_cap * 4u));
-            if (need > (1u << 28)) {
+            uint64_t need = 1;
+            while (need < std::max<uint64_t>(maxfront, 4ull * d->front_cap)) need <<= 1;
+            if (need > (1u << 30)) {
                 d->unchecked = false;
                 d->have_result = false;
-                return oi_set_err(c, OIBVH_ERR_OVERFLOW, "BVTT front exceeds %u entries", 1u << 28);
+                return oi_set_err(c, OIBVH_ERR_OVERFLOW, "BVTT front exceeds %u entries", 1u << 30);
             }
             dfree(d->front[0]);
             dfree(d->front[1]);

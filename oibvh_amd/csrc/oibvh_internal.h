@@ -29,6 +29,7 @@ hipError_t oi_launch_reduce(hipStream_t, int, const oi_build_desc*, const uint2*
 hipError_t oi_launch_reduce_apex(hipStream_t, int, const oi_build_desc*, const uint2*, const oi_build_desc*,
                                  uint32_t);
 hipError_t oi_launch_set_xf(hipStream_t, float*, const float*);
+hipError_t oi_launch_set_mesh_pos(hipStream_t, oi_build_desc*, oi_mesh_desc*, uint32_t, const float*);
 hipError_t oi_launch_seed(hipStream_t, uint32_t, uint32_t*, uint32_t, oi_detect_ctl*, uint32_t);
 hipError_t oi_launch_bvtt_step(hipStream_t, uint32_t, const oi_tab_args*, int, const uint2*, const uint32_t*,
                                uint32_t, uint32_t, uint2*, uint32_t*, uint32_t, uint2*, uint32_t*, uint32_t,

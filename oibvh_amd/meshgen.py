@@ -190,23 +190,31 @@ def box(n: int, half: float = 0.5, noise: float = 0.0, seed: int = 0) -> tuple[n
     return np.concatenate(vs).astype(np.float32), np.concatenate(fs).astype(np.uint32)
 
 
-def cloth(nv: int, amplitude: float = 0.2, wavelength: float = 0.05, seed: int = 0,
-          fold: float = 1.5) -> tuple[np.ndarray, np.ndarray]:
-    """nv x nv vertex cloth over [0,1]^2, F = 2(nv-1)^2.
-
-    z = A sin(2 pi u / wl + a) sin(2 pi v / wl + b) is the height field; the
-    x coordinate is folded as x = u + fold * wl / (2 pi) * sin(2 pi u / wl + a),
-    which turns back on itself whenever fold > 1, so the sheet self-intersects.
-    Phases a, b come from the seed."""
+def cloth_positions(nv: int, amplitude: float = 0.2, wavelength: float = 0.05, seed: int = 0,
+                    fold: float = 1.5, phase: float = 0.0) -> np.ndarray:
+    """Vertex positions of cloth(); `phase` advances the height field's first
+    phase (a deformation over time)."""
     a, b = uniform(seed, 2, 0.0, 2.0 * np.pi)
+    a = a + phase
     t = np.arange(nv, dtype=np.float64) / (nv - 1)
     U, Vv = np.meshgrid(t, t, indexing="ij")
     w = 2.0 * np.pi / wavelength
     su = np.sin(w * U + a)
     x = U + fold / w * su
     z = amplitude * su * np.sin(w * Vv + b)
-    v = np.stack([x.ravel(), Vv.ravel(), z.ravel()], 1)
-    return v.astype(np.float32), _grid_faces(nv, nv).astype(np.uint32)
+    return np.stack([x.ravel(), Vv.ravel(), z.ravel()], 1).astype(np.float32)
+
+
+def cloth(nv: int, amplitude: float = 0.2, wavelength: float = 0.05, seed: int = 0,
+          fold: float = 1.5, phase: float = 0.0) -> tuple[np.ndarray, np.ndarray]:
+    """nv x nv vertex cloth over [0,1]^2, F = 2(nv-1)^2.
+
+    z = A sin(2 pi u / wl + a) sin(2 pi v / wl + b) is the height field; the
+    x coordinate is folded as x = u + fold * wl / (2 pi) * sin(2 pi u / wl + a),
+    which turns back on itself whenever fold > 1, so the sheet self-intersects.
+    Phases a, b come from the seed."""
+    v = cloth_positions(nv, amplitude, wavelength, seed, fold, phase)
+    return v, _grid_faces(nv, nv).astype(np.uint32)
 
 
 # --------------------------------------------------------------------------
@@ -355,9 +363,10 @@ def precision_scenes(n_scenes: int = 64, seed: int = CFG4_SEED, n_plane: int = 3
     default sizes: planes 2 * 354^2 = 250,632, boxes 12 * 144^2 = 248,832)
     placed coplanar or edge-touching with seeded perturbations of
     10^U[-7,-5] of the unit extent, so that many SAT comparisons are
-    near-ties. Kinds cycle through: coplanar planes (B shifted in-plane and
-    lifted by +-eps), planes meeting along an edge (B stood up at x = 1 + eps),
-    boxes face to face (gap or overlap eps), boxes edge to edge."""
+    near-ties. Kinds cycle through: coplanar planes (B shifted in-plane, every
+    vertex of B lifted by its own +-eps so that B's triangles straddle A's
+    plane), planes meeting along an edge (B stood up at x = 1 + eps), boxes
+    face to face (gap or overlap eps), boxes edge to edge."""
     r = Rng(seed)
     scenes = []
     pa = plane(n_plane)
@@ -368,7 +377,10 @@ def precision_scenes(n_scenes: int = 64, seed: int = CFG4_SEED, n_plane: int = 3
         if kind == 0:
             x, f = pa
             shift = r.uniform(2, 0.1, 0.5)
-            b = x.astype(np.float64) + np.array([shift[0], shift[1], eps])
+            nvx = x.shape[0]
+            jit = 10.0 ** r.uniform(nvx, -7.0, -5.0) * np.where(r.u01(nvx) < 0.5, -1.0, 1.0)
+            b = x.astype(np.float64) + np.array([shift[0], shift[1], 0.0])
+            b[:, 2] = jit
             scenes.append([pa, (b.astype(np.float32), f)])
         elif kind == 1:
             x, f = pa
