@@ -42,6 +42,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
+# HBM bytes per refit of one 1M-triangle tree (k_reduce + k_reduce_apex),
+# FETCH_SIZE + WRITE_SIZE from two rocprofv3 --pmc runs, scaled by the counters'
+# reading for a device-to-device copy of known size (profiles/refit_traffic_deforming_uv_pair.md)
+TRAFFIC_REFIT = None
 METRIC = "frames/s & Mtri/s (build+refit+broad+narrow), batched vs plain, 0 pair diffs"
 
 
@@ -67,6 +71,8 @@ def parse():
     p.add_argument("--precision-frames", type=int, default=20)
     p.add_argument("--self-frames", type=int, default=100)
     p.add_argument("--cloth-nv", type=int, default=2048)
+    p.add_argument("--copy-probe-mb", type=int, default=0,
+                   help="first copy this many MiB device to device (a kernel of known traffic, for counter scaling)")
     return p.parse_args()
 
 
@@ -250,13 +256,17 @@ def run_plain(args, ctx):
         "build": {"kernels": "k_mesh_box, k_morton, 4 x k_sort_pass, k_reduce", "avg_ms": build_ms,
                   "launches": ev_build.i},
     }
-    dom = "detect" if share["detect"] >= share["refit"] else "refit"
-    dk = kernels[dom]
-    roofline = {"bound": "hbm", "kernel": dk["kernel"], "achieved": dk["gbps"], "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": (dk["gbps"] * 1e9 / HBM_PEAK) if dk["gbps"] else None,
-                "traffic": None, "alg_bytes_per_launch": dk["alg_bytes"], "avg_launch_ms": dk["avg_ms"]}
+    # the roofline kernel is the refit (k_reduce, 2 launches per frame), the
+    # single kernel with the largest share of the frame; the events bracket
+    # one tree's refit launch pair (k_reduce + k_reduce_apex)
+    dk = kernels["refit"]
+    roofline = {"bound": "hbm", "kernel": "k_reduce (refit; events bracket k_reduce + k_reduce_apex of one tree)",
+                "achieved": dk["gbps"], "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": (dk["gbps"] * 1e9 / HBM_PEAK) if dk["gbps"] else None, "traffic": TRAFFIC_REFIT,
+                "alg_bytes_per_launch": dk["alg_bytes"], "avg_launch_ms": dk["avg_ms"],
+                "alg_bytes_unit": "12 V + 12 F + 24 N per tree (positions, sorted faces, nodes)"}
+    share_ms = {k: v for k, v in share.items()}
 
-    progress(f"plain pair: {fps:.1f} frames/s; parity and CPU baseline")
     # ---------------------------------------------------------------- parity
     diffs = None
     if not args.no_verify:
@@ -325,6 +335,7 @@ def run_plain(args, ctx):
         "bvtt_tests_last_frame": st["bvtt_tests"],
         "candidates_last_frame": st["candidates"],
         "kernels": kernels,
+        "time_share_ms": share_ms,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "host_gen_s": t_gen,
@@ -694,6 +705,16 @@ def main():
 
     sections = set(args.sections.split(","))
     ctx = api.Context(0)
+    if args.copy_probe_mb:
+        from oibvh_amd import _lib as L
+
+        nb = args.copy_probe_mb << 20
+        a, b = api.DeviceBuffer(nb), api.DeviceBuffer(nb)
+        H = L.hip()
+        for _ in range(3):
+            L.hip_check(H.hipMemcpy(b.at(0), a.at(0), nb, 3), "hipMemcpy D2D")
+        a.free()
+        b.free()
     out = run_plain(args, ctx) if "plain" in sections else {"metric": METRIC, "value": None}
     extra = [("batched", run_batched), ("precision", run_precision), ("self", run_self)]
     for name, fn in extra:

@@ -73,7 +73,11 @@ struct BuildTab {
     const oi_build_desc* d;  // device table; null: `one` is mesh 0
     const uint2* work;       // workgroup -> (mesh, chunk); null: (0, blockIdx.x)
     oi_build_desc one;
-    __device__ __forceinline__ uint2 item() const { return work ? work[blockIdx.x] : make_uint2(0u, blockIdx.x); }
+    __device__ __forceinline__ uint2 item() const
+    {
+        const uint32_t w = oi_xcd_remap(blockIdx.x, gridDim.x);
+        return work ? work[w] : make_uint2(0u, w);
+    }
     __device__ __forceinline__ oi_build_desc mesh(uint32_t m) const { return d ? d[m] : one; }
 };
 
@@ -405,9 +409,11 @@ k_sort_pass(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ v
 // Stage 1: work item (m, w) owns the subtree of mesh m of height
 // h = min(L, 8 + log2 M) rooted at level L - h, position w (its 2^h leaves).
 // Each thread computes M consecutive leaves and folds them log2(M) levels in
-// registers; the 256 results climb the remaining levels through LDS
-// (ping-pong buffers, one barrier per level). Every real node is written
-// once, at its real index.
+// registers; the 256 results climb the remaining levels through LDS. Every
+// level of the subtree is a contiguous run of real nodes: it is staged in LDS
+// and written out as whole cache lines from consecutive lanes (one thread
+// writing its own 24-byte nodes would leave lines partially written). Every
+// real node is written once, at its real index.
 // Stage 2 (k_reduce_apex, a second launch): one workgroup per mesh reads the
 // stage-1 roots and climbs to the root in chunks of up to 2^11 nodes.
 // A parent is real iff its left child is; with no real right child the
@@ -424,14 +430,6 @@ __device__ __forceinline__ void box_merge_into(Box6& a, const Box6& r)
         a.v[k] = gmin(a.v[k], r.v[k]);
         a.v[3 + k] = gmax(a.v[3 + k], r.v[3 + k]);
     }
-}
-
-__device__ __forceinline__ void store_node(float* nodes, uint32_t r, const Box6& b)
-{
-    float2* p = reinterpret_cast<float2*>(nodes + 6u * r);
-    p[0] = make_float2(b.v[0], b.v[1]);
-    p[1] = make_float2(b.v[2], b.v[3]);
-    p[2] = make_float2(b.v[4], b.v[5]);
 }
 
 __device__ __forceinline__ Box6 load_node(const float* nodes, uint32_t r)
@@ -459,7 +457,6 @@ struct LeafLoad {
     const Xf& X;
     const uint32_t* vals;
     uint32_t write_perm;
-    uint32_t leaf_start;
     __device__ __forceinline__ Box6 operator()(uint32_t p) const
     {
         uint32_t f0, f1, f2;
@@ -487,7 +484,6 @@ struct LeafLoad {
             r.v[k] = gmin(gmin(a[k], b[k]), c[k]);
             r.v[3 + k] = gmax(gmax(a[k], b[k]), c[k]);
         }
-        store_node(D.nodes, leaf_start + p, r);
         return r;
     }
 };
@@ -498,22 +494,61 @@ struct NodeLoad {
     __device__ __forceinline__ Box6 operator()(uint32_t p) const { return load_node(nodes, lvl_start + p); }
 };
 
-// Reduce the subtree whose children level is lc, h levels up, root position w
-// at level lc - h. Requires 2^h <= 256 * M. Returns with every node of the
-// subtree written; ends with a barrier.
+// LDS staging of one level's nodes in node layout (6 floats per node)
+__device__ __forceinline__ void put_box(float* stage, uint32_t i, const Box6& b)
+{
+    float2* d = reinterpret_cast<float2*>(stage + 6u * i);
+    d[0] = make_float2(b.v[0], b.v[1]);
+    d[1] = make_float2(b.v[2], b.v[3]);
+    d[2] = make_float2(b.v[4], b.v[5]);
+}
+__device__ __forceinline__ Box6 get_box(const float* stage, uint32_t i)
+{
+    const float2* d = reinterpret_cast<const float2*>(stage + 6u * i);
+    const float2 a = d[0], b = d[1], c = d[2];
+    Box6 o;
+    o.v[0] = a.x;
+    o.v[1] = a.y;
+    o.v[2] = b.x;
+    o.v[3] = b.y;
+    o.v[4] = c.x;
+    o.v[5] = c.y;
+    return o;
+}
+
+// Write `count` staged nodes to real index `first` onwards as one contiguous
+// run of float2 (whole cache lines from consecutive lanes); the block's
+// threads share the copy.
+__device__ __forceinline__ void copy_out(float* nodes, uint32_t first, uint32_t count, const float* stage)
+{
+    float2* dst = reinterpret_cast<float2*>(nodes + 6u * first);
+    const float2* src = reinterpret_cast<const float2*>(stage);
+    for (uint32_t i = threadIdx.x; i < 3u * count; i += OI_BLOCK) dst[i] = src[i];
+}
+
+// Reduce the subtree whose bottom level is lc, h levels up, root position w
+// at level lc - h. Requires 2^h <= 256 * M and stage >= 6 * 256 * M floats.
+// Every level of the subtree is staged in LDS and written out contiguously.
+// Returns with every node of the subtree written; ends with a barrier.
 template <int M, class Load>
 __device__ void reduce_subtree(const Load& load, float* nodes, uint32_t L, uint32_t vl, uint32_t lc, uint32_t h,
-                               uint32_t w, float (*sA)[OI_BLOCK], float (*sB)[OI_BLOCK])
+                               uint32_t w, float* stage)
 {
     constexpr uint32_t LGM = (M == 1) ? 0 : (M == 2) ? 1 : (M == 4) ? 2 : 3;
     const uint32_t tid = threadIdx.x;
     const bool act = (tid * M) < (1u << h);
     const uint32_t c0 = (w << h) + tid * M;
-    Box6 b[M];
     uint32_t ncnt = oi_lvl_cnt(lc, L, vl);
+    Box6 b[M];
 #pragma unroll
     for (int j = 0; j < M; j++)
-        if (act && c0 + j < ncnt) b[j] = load(c0 + j);
+        if (act && c0 + j < ncnt) {
+            b[j] = load(c0 + j);
+            put_box(stage, tid * M + j, b[j]);
+        }
+    __syncthreads();
+    copy_out(nodes, oi_lvl_start(lc, L, vl) + (w << h), min(1u << h, ncnt - (w << h)), stage);
+    __syncthreads();
     const uint32_t hl = (h < LGM) ? h : LGM;
     uint32_t l = lc;
 #pragma unroll
@@ -522,7 +557,6 @@ __device__ void reduce_subtree(const Load& load, float* nodes, uint32_t L, uint3
             const uint32_t cc = ncnt;  // real count of the child level
             l -= 1;
             ncnt = oi_lvl_cnt(l, L, vl);
-            const uint32_t st = oi_lvl_start(l, L, vl);
 #pragma unroll
             for (uint32_t j = 0; j < (uint32_t)(M >> s); j++) {
                 const uint32_t q = (c0 >> s) + j;
@@ -530,55 +564,42 @@ __device__ void reduce_subtree(const Load& load, float* nodes, uint32_t L, uint3
                     Box6 x = b[2 * j];
                     if (2u * q + 1u < cc) box_merge_into(x, b[2 * j + 1]);
                     b[j] = x;
-                    store_node(nodes, st + q, x);
+                    put_box(stage, tid * (uint32_t)(M >> s) + j, x);
                 }
             }
+            __syncthreads();
+            const uint32_t f = w << (h - s);
+            copy_out(nodes, oi_lvl_start(l, L, vl) + f, min(1u << (h - s), ncnt - f), stage);
+            __syncthreads();
         }
     }
     const uint32_t hs = h - hl;
-    if (hs == 0) {
-        __syncthreads();
-        return;
-    }
-    // LDS climb: thread t holds node t of level l within the subtree
-    if (act) {
-#pragma unroll
-        for (int k = 0; k < 6; k++) sA[k][tid] = b[0].v[k];
-    }
-    __syncthreads();
-    float(*cur)[OI_BLOCK] = sA;
-    float(*nxt)[OI_BLOCK] = sB;
+    // LDS climb: slot t holds node (w << hs) + t of level l
     for (uint32_t s = 1; s <= hs; s++) {
         const uint32_t cc = ncnt;
         l -= 1;
         ncnt = oi_lvl_cnt(l, L, vl);
         const uint32_t width = 1u << (hs - s);
-        const uint32_t q = (w << (hs - s)) + tid;
-        if (tid < width && 2u * q < cc) {
-            Box6 x, y;
-#pragma unroll
-            for (int k = 0; k < 6; k++) x.v[k] = cur[k][2 * tid];
-            if (2u * q + 1u < cc) {
-#pragma unroll
-                for (int k = 0; k < 6; k++) y.v[k] = cur[k][2 * tid + 1];
-                box_merge_into(x, y);
-            }
-#pragma unroll
-            for (int k = 0; k < 6; k++) nxt[k][tid] = x.v[k];
-            store_node(nodes, oi_lvl_start(l, L, vl) + q, x);
+        const uint32_t f = w << (hs - s);
+        const uint32_t q = f + tid;
+        const bool ok = tid < width && 2u * q < cc;
+        Box6 x;
+        if (ok) {
+            x = get_box(stage, 2u * tid);
+            if (2u * q + 1u < cc) box_merge_into(x, get_box(stage, 2u * tid + 1u));
         }
         __syncthreads();
-        float(*t)[OI_BLOCK] = cur;
-        cur = nxt;
-        nxt = t;
+        if (ok) put_box(stage, tid, x);
+        __syncthreads();
+        copy_out(nodes, oi_lvl_start(l, L, vl) + f, min(width, ncnt - f), stage);
+        __syncthreads();
     }
 }
 
 template <int M>
 __global__ void __launch_bounds__(OI_BLOCK) k_reduce(ReduceArgs a)
 {
-    __shared__ float sA[6][OI_BLOCK];
-    __shared__ float sB[6][OI_BLOCK];
+    __shared__ float stage[6 * OI_BLOCK * M];
     constexpr uint32_t LGM = (M == 1) ? 0 : (M == 2) ? 1 : (M == 4) ? 2 : 3;
     const uint2 it = a.T.item();
     const oi_build_desc D = a.T.mesh(it.x);
@@ -586,8 +607,8 @@ __global__ void __launch_bounds__(OI_BLOCK) k_reduce(ReduceArgs a)
     X.load(D.xf);
     const uint32_t L = D.L, vl = D.vl;
     const uint32_t h = (L < 8 + LGM) ? L : 8 + LGM;
-    LeafLoad ll{D, X, a.vals, a.write_perm, oi_lvl_start(L, L, vl)};
-    reduce_subtree<M>(ll, D.nodes, L, vl, L, h, it.y, sA, sB);
+    LeafLoad ll{D, X, a.vals, a.write_perm};
+    reduce_subtree<M>(ll, D.nodes, L, vl, L, h, it.y, stage);
 }
 
 // Stage 2, one workgroup per mesh: the levels above the stage-1 subtree roots
@@ -596,8 +617,7 @@ __global__ void __launch_bounds__(OI_BLOCK) k_reduce(ReduceArgs a)
 // device-scope fences.
 __global__ void __launch_bounds__(OI_BLOCK) k_reduce_apex(ReduceArgs a, uint32_t lgm)
 {
-    __shared__ float sA[6][OI_BLOCK];
-    __shared__ float sB[6][OI_BLOCK];
+    __shared__ float stage[6 * OI_BLOCK * 8];
     const uint2 it = a.T.item();
     const oi_build_desc D = a.T.mesh(it.x);
     const uint32_t L = D.L, vl = D.vl;
@@ -607,7 +627,7 @@ __global__ void __launch_bounds__(OI_BLOCK) k_reduce_apex(ReduceArgs a, uint32_t
         const uint32_t h2 = (lc < 11u) ? lc : 11u;
         const uint32_t nchunk = oi_lvl_cnt(lc - h2, L, vl);
         NodeLoad nl{D.nodes, oi_lvl_start(lc, L, vl)};
-        for (uint32_t c = 0; c < nchunk; c++) reduce_subtree<8>(nl, D.nodes, L, vl, lc, h2, c, sA, sB);
+        for (uint32_t c = 0; c < nchunk; c++) reduce_subtree<8>(nl, D.nodes, L, vl, lc, h2, c, stage);
         lc -= h2;
         __threadfence_block();
         __syncthreads();

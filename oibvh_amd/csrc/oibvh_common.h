@@ -64,6 +64,18 @@ OI_HD uint32_t oi_lvl_last_implicit(uint32_t l, uint32_t L, uint32_t vl)
     return ((1u << (l + 1u)) - 2u) - (vl >> (L - l));
 }
 
+// Workgroups are dealt to the 8 XCDs of an MI355X round-robin in launch
+// order (workgroup b runs on XCD b % 8), and each XCD has its own L2. This maps
+// workgroup b to a work index so that each XCD takes one contiguous range of
+// work indices: neighbouring Morton ranges, and the vertices they gather,
+// stay in one L2. A bijection on [0, n).
+OI_HD uint32_t oi_xcd_remap(uint32_t b, uint32_t n)
+{
+    const uint32_t q = n >> 3, r = n & 7u;
+    const uint32_t x = b & 7u, k = b >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
+
 // ---------------------------------------------------------------------------
 // device descriptors
 // ---------------------------------------------------------------------------
