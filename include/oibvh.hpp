@@ -149,6 +149,11 @@ public:
         return out;
     }
 
+    // Mesh::transform(glm::mat4) (mesh.cpp:189-214) applied on the fly: later
+    // build/refit/detect see mat4 * (p, 1). mat4 = 16 floats, column-major
+    // (glm::value_ptr of a glm::mat4); nullptr restores the identity.
+    void setTransform(const float* mat4) { ctx_->check(oibvh_tree_set_transform(h_, mat4), "setTransform"); }
+
     // leaf position -> original face id
     std::vector<uint32_t> sortedTriIndex() const
     {
@@ -225,6 +230,74 @@ private:
     std::shared_ptr<Context> ctx_;
     oibvh_scene* h_ = nullptr;
     std::vector<std::shared_ptr<OibvhTree>> trees_;
+};
+
+// Many independent scenes built, refit and queried together (no reference
+// counterpart): every stage is one launch over all meshes of all scenes.
+class Batch {
+public:
+    explicit Batch(int device = 0) : ctx_(Context::get(device))
+    {
+        ctx_->check(oibvh_batch_create(ctx_->handle(), &h_), "Batch");
+    }
+    ~Batch() { oibvh_batch_destroy(h_); }
+    Batch(const Batch&) = delete;
+    Batch& operator=(const Batch&) = delete;
+
+    // returns the mesh's batch index
+    uint32_t addMesh(uint32_t scene, const Mesh& mesh)
+    {
+        uint32_t idx = 0;
+        ctx_->check(oibvh_batch_add_mesh(h_, scene, mesh.positions.data(), mesh.vertexCount(), mesh.indices.data(),
+                                         mesh.faceCount(), &idx),
+                    "Batch::addMesh");
+        return idx;
+    }
+    uint32_t addInstance(uint32_t scene, uint32_t source)
+    {
+        uint32_t idx = 0;
+        ctx_->check(oibvh_batch_add_instance(h_, scene, source, &idx), "Batch::addInstance");
+        return idx;
+    }
+    // meshCount x 16 floats, column-major
+    void setTransforms(const float* mats, uint32_t ptrKind = OIBVH_PTR_HOST)
+    {
+        ctx_->check(oibvh_batch_set_transforms(h_, mats, ptrKind), "Batch::setTransforms");
+    }
+    void build() { ctx_->check(oibvh_batch_build(h_), "Batch::build"); }
+    void refit() { ctx_->check(oibvh_batch_refit(h_), "Batch::refit"); }
+    void detectCollision(unsigned int entryLevel = 0, unsigned int expandLevels = 1, uint32_t flags = 0)
+    {
+        ctx_->check(oibvh_batch_detect(h_, entryLevel, expandLevels, flags), "Batch::detectCollision");
+    }
+    // refit + detect replayed from a captured hipGraph
+    void frame(unsigned int entryLevel = 0, unsigned int expandLevels = 1, uint32_t flags = 0)
+    {
+        ctx_->check(oibvh_batch_frame(h_, entryLevel, expandLevels, flags), "Batch::frame");
+    }
+    unsigned int getIntTriPairCount() const
+    {
+        uint32_t n = 0;
+        ctx_->check(oibvh_batch_get_pair_count(h_, &n), "Batch::getIntTriPairCount");
+        return n;
+    }
+    // bvhIndex = batch mesh indices
+    std::vector<IntTriPair> getIntTriPairs() const
+    {
+        const uint32_t n = getIntTriPairCount();
+        std::vector<IntTriPair> out(n);
+        uint32_t got = 0;
+        if (n)
+            ctx_->check(oibvh_batch_get_pairs(h_, reinterpret_cast<uint32_t*>(out.data()), n, &got),
+                        "Batch::getIntTriPairs");
+        out.resize(got);
+        return out;
+    }
+    oibvh_batch* handle() const { return h_; }
+
+private:
+    std::shared_ptr<Context> ctx_;
+    oibvh_batch* h_ = nullptr;
 };
 
 }  // namespace oibvh
