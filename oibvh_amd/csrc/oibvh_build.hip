@@ -89,17 +89,18 @@ struct Xf {
     {
         on = xf != nullptr;
 #pragma unroll
-        for (int k = 0; k < 16; k++) m[k] = on ? xf[k] : 0.0f;
+        for (int k = 0; k < 16; k++) m[k] = on ? oi_g(xf)[k] : 0.0f;
     }
     __device__ __forceinline__ void vertex(const float* pos, uint32_t v, float* o) const
     {
-        const float x = pos[3u * v], y = pos[3u * v + 1], z = pos[3u * v + 2];
+        float p[3];
+        oi_ld3(pos, v, p);
         if (on) {
-            oi_xform(m, x, y, z, o);
+            oi_xform(m, p[0], p[1], p[2], o);
         } else {
-            o[0] = x;
-            o[1] = y;
-            o[2] = z;
+            o[0] = p[0];
+            o[1] = p[1];
+            o[2] = p[2];
         }
     }
 };
@@ -160,7 +161,7 @@ k_mesh_box(BuildTab T, float* __restrict__ partials, uint32_t* __restrict__ clea
         }
         __syncthreads();
     }
-    if (tid < 6) partials[6u * (D.pbase + it.y) + tid] = s_v[tid][0];
+    if (tid < 6) oi_gw(partials)[6u * (D.pbase + it.y) + tid] = s_v[tid][0];
 }
 
 // Fold the per-block partial boxes (at most OI_BLOCK of them) in block order and
@@ -236,17 +237,18 @@ k_morton(BuildTab T, const float* __restrict__ partials, uint32_t* __restrict__ 
     float bmn[3], bmx[3];
     fold_partials(partials + 6u * D.pbase, D.nparts, s_v, bmn, bmx);
     if (it.y == 0 && tid < 3) {
-        D.box[tid] = bmn[tid];
-        D.box[3 + tid] = bmx[tid];
+        oi_gw(D.box)[tid] = bmn[tid];
+        oi_gw(D.box)[3 + tid] = bmx[tid];
     }
     const uint32_t f0 = it.y * OI_MORTON_CHUNK;
     const uint32_t f1 = min(D.F, f0 + OI_MORTON_CHUNK);
     for (uint32_t i = f0 + tid; i < f1; i += OI_BLOCK) {
-        const uint32_t v0 = D.faces[3u * i], v1 = D.faces[3u * i + 1], v2 = D.faces[3u * i + 2];
+        uint32_t fv[3];
+        oi_ld3(D.faces, i, fv);
         float a[3], b[3], c[3];
-        X.vertex(D.pos, v0, a);
-        X.vertex(D.pos, v1, b);
-        X.vertex(D.pos, v2, c);
+        X.vertex(D.pos, fv[0], a);
+        X.vertex(D.pos, fv[1], b);
+        X.vertex(D.pos, fv[2], c);
         uint32_t q[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
@@ -258,8 +260,8 @@ k_morton(BuildTab T, const float* __restrict__ partials, uint32_t* __restrict__ 
             q[k] = quantise10(off / len);
         }
         const uint32_t key = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
-        keys[D.fbase + i] = key;
-        if (fmesh) fmesh[D.fbase + i] = it.x;
+        oi_gw(keys)[D.fbase + i] = key;
+        if (fmesh) oi_gw(fmesh)[D.fbase + i] = it.x;
 #pragma unroll
         for (int d = 0; d < 4; d++) atomicAdd(&s_h[d][(key >> (8 * d)) & 0xffu], 1u);
     }
@@ -434,15 +436,10 @@ __device__ __forceinline__ void box_merge_into(Box6& a, const Box6& r)
 
 __device__ __forceinline__ Box6 load_node(const float* nodes, uint32_t r)
 {
-    const float2* p = reinterpret_cast<const float2*>(nodes + 6u * r);
-    const float2 a = p[0], b = p[1], c = p[2];
+    const OI_GLOBAL float* p = oi_g(nodes) + 6ull * r;
     Box6 o;
-    o.v[0] = a.x;
-    o.v[1] = a.y;
-    o.v[2] = b.x;
-    o.v[3] = b.y;
-    o.v[4] = c.x;
-    o.v[5] = c.y;
+#pragma unroll
+    for (int k = 0; k < 6; k++) o.v[k] = p[k];
     return o;
 }
 
@@ -459,25 +456,22 @@ struct LeafLoad {
     uint32_t write_perm;
     __device__ __forceinline__ Box6 operator()(uint32_t p) const
     {
-        uint32_t f0, f1, f2;
+        uint32_t f[3];
         if (vals) {
-            const uint32_t fi = vals[D.fbase + p] - D.fbase;
-            f0 = D.faces[3u * fi];
-            f1 = D.faces[3u * fi + 1];
-            f2 = D.faces[3u * fi + 2];
-            D.sfaces[3u * p] = f0;
-            D.sfaces[3u * p + 1] = f1;
-            D.sfaces[3u * p + 2] = f2;
-            if (write_perm) D.perm[p] = fi;
+            const uint32_t fi = oi_g(vals)[D.fbase + p] - D.fbase;
+            oi_ld3(D.faces, fi, f);
+            OI_GLOBAL uint32_t* q = oi_gw(D.sfaces) + 3ull * p;
+            q[0] = f[0];
+            q[1] = f[1];
+            q[2] = f[2];
+            if (write_perm) oi_gw(D.perm)[p] = fi;
         } else {
-            f0 = D.sfaces[3u * p];
-            f1 = D.sfaces[3u * p + 1];
-            f2 = D.sfaces[3u * p + 2];
+            oi_ld3(D.sfaces, p, f);
         }
         float a[3], b[3], c[3];
-        X.vertex(D.pos, f0, a);
-        X.vertex(D.pos, f1, b);
-        X.vertex(D.pos, f2, c);
+        X.vertex(D.pos, f[0], a);
+        X.vertex(D.pos, f[1], b);
+        X.vertex(D.pos, f[2], c);
         Box6 r;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
@@ -521,20 +515,37 @@ __device__ __forceinline__ Box6 get_box(const float* stage, uint32_t i)
 // threads share the copy.
 __device__ __forceinline__ void copy_out(float* nodes, uint32_t first, uint32_t count, const float* stage)
 {
-    float2* dst = reinterpret_cast<float2*>(nodes + 6u * first);
+    OI_GLOBAL float* dst = oi_gw(nodes) + 6ull * first;
     const float2* src = reinterpret_cast<const float2*>(stage);
-    for (uint32_t i = threadIdx.x; i < 3u * count; i += OI_BLOCK) dst[i] = src[i];
+    for (uint32_t i = threadIdx.x; i < 3u * count; i += OI_BLOCK) {
+        const float2 v = src[i];
+        dst[2u * i] = v.x;
+        dst[2u * i + 1u] = v.y;
+    }
 }
 
+// LDS for one subtree reduce with M leaves per thread: two staging buffers
+// used in turn, 256 M and max(128 M, 256) nodes
+template <int M>
+struct StageSize {
+    static constexpr uint32_t n0 = 256u * M;
+    static constexpr uint32_t n1 = (128u * M > 256u) ? 128u * M : 256u;
+    static constexpr uint32_t floats = 6u * (n0 + n1);
+};
+
 // Reduce the subtree whose bottom level is lc, h levels up, root position w
-// at level lc - h. Requires 2^h <= 256 * M and stage >= 6 * 256 * M floats.
-// Every level of the subtree is staged in LDS and written out contiguously.
+// at level lc - h. Requires 2^h <= 256 * M and stage >= StageSize<M>::floats.
+// Each level is staged in LDS and written out contiguously; consecutive
+// levels use the two buffers in turn, so one barrier per level orders both
+// the next level's reads and the previous buffer's reuse.
 // Returns with every node of the subtree written; ends with a barrier.
 template <int M, class Load>
 __device__ void reduce_subtree(const Load& load, float* nodes, uint32_t L, uint32_t vl, uint32_t lc, uint32_t h,
                                uint32_t w, float* stage)
 {
     constexpr uint32_t LGM = (M == 1) ? 0 : (M == 2) ? 1 : (M == 4) ? 2 : 3;
+    float* const s0 = stage;
+    float* const s1 = stage + 6u * StageSize<M>::n0;
     const uint32_t tid = threadIdx.x;
     const bool act = (tid * M) < (1u << h);
     const uint32_t c0 = (w << h) + tid * M;
@@ -544,11 +555,11 @@ __device__ void reduce_subtree(const Load& load, float* nodes, uint32_t L, uint3
     for (int j = 0; j < M; j++)
         if (act && c0 + j < ncnt) {
             b[j] = load(c0 + j);
-            put_box(stage, tid * M + j, b[j]);
+            put_box(s0, tid * M + j, b[j]);
         }
     __syncthreads();
-    copy_out(nodes, oi_lvl_start(lc, L, vl) + (w << h), min(1u << h, ncnt - (w << h)), stage);
-    __syncthreads();
+    copy_out(nodes, oi_lvl_start(lc, L, vl) + (w << h), min(1u << h, ncnt - (w << h)), s0);
+    uint32_t cur = 0;  // buffer holding the last level
     const uint32_t hl = (h < LGM) ? h : LGM;
     uint32_t l = lc;
 #pragma unroll
@@ -557,6 +568,7 @@ __device__ void reduce_subtree(const Load& load, float* nodes, uint32_t L, uint3
             const uint32_t cc = ncnt;  // real count of the child level
             l -= 1;
             ncnt = oi_lvl_cnt(l, L, vl);
+            cur ^= 1u;
 #pragma unroll
             for (uint32_t j = 0; j < (uint32_t)(M >> s); j++) {
                 const uint32_t q = (c0 >> s) + j;
@@ -564,17 +576,16 @@ __device__ void reduce_subtree(const Load& load, float* nodes, uint32_t L, uint3
                     Box6 x = b[2 * j];
                     if (2u * q + 1u < cc) box_merge_into(x, b[2 * j + 1]);
                     b[j] = x;
-                    put_box(stage, tid * (uint32_t)(M >> s) + j, x);
+                    put_box(cur ? s1 : s0, tid * (uint32_t)(M >> s) + j, x);
                 }
             }
             __syncthreads();
             const uint32_t f = w << (h - s);
-            copy_out(nodes, oi_lvl_start(l, L, vl) + f, min(1u << (h - s), ncnt - f), stage);
-            __syncthreads();
+            copy_out(nodes, oi_lvl_start(l, L, vl) + f, min(1u << (h - s), ncnt - f), cur ? s1 : s0);
         }
     }
     const uint32_t hs = h - hl;
-    // LDS climb: slot t holds node (w << hs) + t of level l
+    // LDS climb: slot t of the current buffer holds node (w << hs) + t of level l
     for (uint32_t s = 1; s <= hs; s++) {
         const uint32_t cc = ncnt;
         l -= 1;
@@ -582,24 +593,24 @@ __device__ void reduce_subtree(const Load& load, float* nodes, uint32_t L, uint3
         const uint32_t width = 1u << (hs - s);
         const uint32_t f = w << (hs - s);
         const uint32_t q = f + tid;
-        const bool ok = tid < width && 2u * q < cc;
-        Box6 x;
-        if (ok) {
-            x = get_box(stage, 2u * tid);
-            if (2u * q + 1u < cc) box_merge_into(x, get_box(stage, 2u * tid + 1u));
+        const float* src = cur ? s1 : s0;
+        float* dst = cur ? s0 : s1;
+        if (tid < width && 2u * q < cc) {
+            Box6 x = get_box(src, 2u * tid);
+            if (2u * q + 1u < cc) box_merge_into(x, get_box(src, 2u * tid + 1u));
+            put_box(dst, tid, x);
         }
+        cur ^= 1u;
         __syncthreads();
-        if (ok) put_box(stage, tid, x);
-        __syncthreads();
-        copy_out(nodes, oi_lvl_start(l, L, vl) + f, min(width, ncnt - f), stage);
-        __syncthreads();
+        copy_out(nodes, oi_lvl_start(l, L, vl) + f, min(width, ncnt - f), dst);
     }
+    __syncthreads();
 }
 
 template <int M>
 __global__ void __launch_bounds__(OI_BLOCK) k_reduce(ReduceArgs a)
 {
-    __shared__ float stage[6 * OI_BLOCK * M];
+    __shared__ float stage[StageSize<M>::floats];
     constexpr uint32_t LGM = (M == 1) ? 0 : (M == 2) ? 1 : (M == 4) ? 2 : 3;
     const uint2 it = a.T.item();
     const oi_build_desc D = a.T.mesh(it.x);
@@ -612,22 +623,36 @@ __global__ void __launch_bounds__(OI_BLOCK) k_reduce(ReduceArgs a)
 }
 
 // Stage 2, one workgroup per mesh: the levels above the stage-1 subtree roots
-// (level lc = L - h), in chunks of up to 2^11 nodes per climb. A separate
-// launch, so the roots written by every stage-1 workgroup are visible without
-// device-scope fences.
+// (level lc = L - h), in chunks of up to 2^11 nodes per climb, with as many
+// nodes per thread as keep all 256 threads busy. A separate launch, so the
+// roots written by every stage-1 workgroup are visible without device-scope
+// fences.
+template <int M>
+__device__ __forceinline__ void apex_chunks(const oi_build_desc& D, uint32_t lc, uint32_t h2, float* stage)
+{
+    const uint32_t nchunk = oi_lvl_cnt(lc - h2, D.L, D.vl);
+    NodeLoad nl{D.nodes, oi_lvl_start(lc, D.L, D.vl)};
+    for (uint32_t c = 0; c < nchunk; c++) reduce_subtree<M>(nl, D.nodes, D.L, D.vl, lc, h2, c, stage);
+}
+
 __global__ void __launch_bounds__(OI_BLOCK) k_reduce_apex(ReduceArgs a, uint32_t lgm)
 {
-    __shared__ float stage[6 * OI_BLOCK * 8];
+    __shared__ float stage[StageSize<8>::floats];
     const uint2 it = a.T.item();
     const oi_build_desc D = a.T.mesh(it.x);
-    const uint32_t L = D.L, vl = D.vl;
+    const uint32_t L = D.L;
     const uint32_t h = (L < 8 + lgm) ? L : 8 + lgm;
     uint32_t lc = L - h;
     while (lc > 0) {
         const uint32_t h2 = (lc < 11u) ? lc : 11u;
-        const uint32_t nchunk = oi_lvl_cnt(lc - h2, L, vl);
-        NodeLoad nl{D.nodes, oi_lvl_start(lc, L, vl)};
-        for (uint32_t c = 0; c < nchunk; c++) reduce_subtree<8>(nl, D.nodes, L, vl, lc, h2, c, stage);
+        if (h2 <= 8)
+            apex_chunks<1>(D, lc, h2, stage);
+        else if (h2 == 9)
+            apex_chunks<2>(D, lc, h2, stage);
+        else if (h2 == 10)
+            apex_chunks<4>(D, lc, h2, stage);
+        else
+            apex_chunks<8>(D, lc, h2, stage);
         lc -= h2;
         __threadfence_block();
         __syncthreads();

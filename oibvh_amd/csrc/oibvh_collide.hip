@@ -365,10 +365,14 @@ __device__ __forceinline__ MeshRef tab_ref(const ChunkTab& T, const oi_mesh_desc
 // a vertex through the mesh's model matrix, if any
 __device__ __forceinline__ v3f vertex(const float* pos, const float* xf, uint32_t v)
 {
-    const float x = pos[3u * v], y = pos[3u * v + 1], z = pos[3u * v + 2];
-    if (!xf) return v3f{x, y, z};
+    float p[3];
+    oi_ld3(pos, v, p);
+    if (!xf) return v3f{p[0], p[1], p[2]};
+    float m[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) m[k] = oi_g(xf)[k];
     float o[3];
-    oi_xform(xf, x, y, z, o);
+    oi_xform(m, p[0], p[1], p[2], o);
     return v3f{o[0], o[1], o[2]};
 }
 
@@ -379,14 +383,9 @@ __device__ __forceinline__ bool overlap6(const float* a, const float* b)
 
 __device__ __forceinline__ void load_box(const float* nodes, uint32_t r, float* o)
 {
-    const float2* p = reinterpret_cast<const float2*>(nodes + 6u * r);
-    const float2 a = p[0], b = p[1], c = p[2];
-    o[0] = a.x;
-    o[1] = a.y;
-    o[2] = b.x;
-    o[3] = b.y;
-    o[4] = c.x;
-    o[5] = c.y;
+    const OI_GLOBAL float* p = oi_g(nodes) + 6ull * r;
+#pragma unroll
+    for (int k = 0; k < 6; k++) o[k] = p[k];
 }
 
 // Expansion of one side: the descendants `d` levels below implicit node i
@@ -456,11 +455,7 @@ __device__ __forceinline__ void load_faces(const uint32_t* sfaces, uint32_t leaf
 {
 #pragma unroll
     for (int k = 0; k < (1 << E); k++)
-        if ((uint32_t)k < n) {
-            f[k][0] = sfaces[3u * (leaf0 + k)];
-            f[k][1] = sfaces[3u * (leaf0 + k) + 1];
-            f[k][2] = sfaces[3u * (leaf0 + k) + 2];
-        }
+        if ((uint32_t)k < n) oi_ld3(sfaces, leaf0 + k, f[k]);
 }
 
 __device__ __forceinline__ bool share_vertex(const uint32_t* a, const uint32_t* b)
@@ -629,11 +624,8 @@ __device__ __forceinline__ void narrow_one(const Tab& T, const uint32_t* s_ibase
     const uint32_t sa = c.x - ra.ibase - ((1u << ra.L) - 1u);
     const uint32_t sb = c.y - rb.ibase - ((1u << rb.L) - 1u);
     uint32_t fa[3], fb[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        fa[k] = ra.sfaces[3u * sa + k];
-        fb[k] = rb.sfaces[3u * sb + k];
-    }
+    oi_ld3(ra.sfaces, sa, fa);
+    oi_ld3(rb.sfaces, sb, fb);
     if (ma == mb) {
         bool adj = false;
 #pragma unroll
@@ -645,7 +637,7 @@ __device__ __forceinline__ void narrow_one(const Tab& T, const uint32_t* s_ibase
             return;
         }
     }
-    uint32_t oa = ra.perm[sa], ob = rb.perm[sb];
+    uint32_t oa = oi_g(ra.perm)[sa], ob = oi_g(rb.perm)[sb];
     const float* pa = ra.pos;
     const float* pb = rb.pos;
     if (ma == mb && oa > ob) {

@@ -76,6 +76,39 @@ OI_HD uint32_t oi_xcd_remap(uint32_t b, uint32_t n)
     return x * q + (x < r ? x : r) + k;
 }
 
+#if defined(__HIPCC__) || defined(__HIP__)
+// Pointers read from descriptor tables are generic to the compiler, which
+// then emits flat memory instructions; these views make them global so that
+// loads and stores are global_* and adjacent words combine into one
+// multi-dword access (indices are 64-bit so adjacency is provable).
+#define OI_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const OI_GLOBAL T* oi_g(const T* p)
+{
+    return (const OI_GLOBAL T*)p;
+}
+template <class T>
+__device__ __forceinline__ OI_GLOBAL T* oi_gw(T* p)
+{
+    return (OI_GLOBAL T*)p;
+}
+// three consecutive 32-bit words at base + 3 i (a vertex or a face)
+__device__ __forceinline__ void oi_ld3(const float* base, uint32_t i, float* o)
+{
+    const OI_GLOBAL float* q = oi_g(base) + 3ull * i;
+    o[0] = q[0];
+    o[1] = q[1];
+    o[2] = q[2];
+}
+__device__ __forceinline__ void oi_ld3(const uint32_t* base, uint32_t i, uint32_t* o)
+{
+    const OI_GLOBAL uint32_t* q = oi_g(base) + 3ull * i;
+    o[0] = q[0];
+    o[1] = q[1];
+    o[2] = q[2];
+}
+#endif
+
 // ---------------------------------------------------------------------------
 // device descriptors
 // ---------------------------------------------------------------------------
