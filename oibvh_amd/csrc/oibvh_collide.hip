@@ -167,16 +167,19 @@ __device__ bool sat_f32(const v3f* P, const v3f* Q)
 
 // ---------------------------------------------------------------------------
 // FP32 filter with FP64 re-check (precision form).
-// Every FP32 quantity z carries a magnitude S(z) >= |z|, |exact z| built by
-// the same expression over |operands| (sums of magnitudes for +/-, products
-// for *). A value at rounding depth c has |z - exact z| <= c * 2^-24 * S(z)
-// (+ O(2^-48)); the deepest SAT projection (a g/h axis) has c = 14. A
+// Every FP32 quantity z carries a magnitude S(z) with |z - exact z| <=
+// c * 2^-24 * S(z) (+ O(2^-48)) at rounding depth c. The differences of the
+// exact FP32 inputs (p2 = P2 - P1 and so on) are the leaves: their only error
+// is their own rounding, so S = |fl(difference)| at depth 1; above them S is
+// built by the same expression over magnitudes (sums of magnitudes for +/-,
+// products for *). The deepest SAT projection (a g/h axis) has c = 13. A
 // comparison mn1 > mx2 is decided in FP32 only when |mn1 - mx2| exceeds
 // T = 32 * 2^-24 * (S_P + S_Q) + 2^-100, which covers the FP32 error of both
-// sides, the FP64 error of the oracle's own evaluation and the rounding of T;
-// then the FP32 decision equals the exact one and the FP64 one. Otherwise the
-// axis is evaluated again in double exactly as the FP64 oracle evaluates it.
-// Non-finite magnitudes never pass the test, so they also go to FP64.
+// sides, the FP64 error of the oracle's own evaluation, the rounding of the
+// magnitudes themselves and of T; then the FP32 decision equals the exact one
+// and the FP64 one. Otherwise the axis is evaluated again in double exactly
+// as the FP64 oracle evaluates it. Non-finite magnitudes never pass the test,
+// so they also go to FP64.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ v3f vabs(v3f a) { return v3f{fabsf(a.x), fabsf(a.y), fabsf(a.z)}; }
 __device__ __forceinline__ v3f vaddm(v3f a, v3f b) { return v3f{a.x + b.x, a.y + b.y, a.z + b.z}; }
@@ -188,14 +191,14 @@ __device__ __forceinline__ float vdotm(v3f a, v3f b) { return (a.x * b.x + a.y *
 
 struct SatMag {
     v3f p2, p3, q1, q2, q3, e1, e2, e3, f1, f2, f3, n1, m1;
-    __device__ __forceinline__ void init(const v3f* P, const v3f* Q)
+    // s: the FP32 frame of the same pair (its differences are the leaves)
+    __device__ __forceinline__ void init(const SatFrame<float, v3f>& s)
     {
-        const v3f a0 = vabs(P[0]);
-        p2 = vaddm(vabs(P[1]), a0);
-        p3 = vaddm(vabs(P[2]), a0);
-        q1 = vaddm(vabs(Q[0]), a0);
-        q2 = vaddm(vabs(Q[1]), a0);
-        q3 = vaddm(vabs(Q[2]), a0);
+        p2 = vabs(s.p2);
+        p3 = vabs(s.p3);
+        q1 = vabs(s.q1);
+        q2 = vabs(s.q2);
+        q3 = vabs(s.q3);
         e1 = p2;
         e2 = vaddm(p3, p2);
         e3 = p3;
@@ -235,7 +238,7 @@ __device__ bool sat_filtered(const v3f* P, const v3f* Q, uint32_t& n_fp64)
     SatFrame<float, v3f> s;
     s.init(P, Q);
     SatMag m;
-    m.init(P, Q);
+    m.init(s);
     bool have_d = false;
     SatFrame<double, v3d> sd;
     for (int k = 0; k < 17; k++) {
