@@ -207,6 +207,12 @@ typedef struct {
 OIBVH_API oibvh_status oibvh_ctx_set_tuning(oibvh_ctx* ctx, const oibvh_tuning* tuning);
 OIBVH_API oibvh_status oibvh_ctx_get_tuning(const oibvh_ctx* ctx, oibvh_tuning* tuning);
 
+/* Refit every tree of the scene (OibvhTree::refit on each, oibvhTree.cu:193-235)
+ * with one launch per stage for up to four trees instead of one per tree.
+ * Results are those of oibvh_tree_refit on every tree with its current
+ * positions and transform. */
+OIBVH_API oibvh_status oibvh_scene_refit(oibvh_scene* scene);
+
 /* BVTT front length of each level of the last detect (level 0 = the initial
  * front); *n receives the number of levels. Diagnostics; the reference prints
  * these (scene.cu:274-276). */

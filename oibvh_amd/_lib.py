@@ -75,6 +75,7 @@ API = [
     ("oibvh_scene_hits_device", _i, [_vp, C.POINTER(_vp), _u32p]),
     ("oibvh_scene_get_stats", _i, [_vp, C.POINTER(DetectStats)]),
     ("oibvh_scene_get_level_counts", _i, [_vp, _u32p, _u64, _u32p]),
+    ("oibvh_scene_refit", _i, [_vp]),
     ("oibvh_batch_create", _i, [_vp, C.POINTER(_vp)]),
     ("oibvh_batch_destroy", None, [_vp]),
     ("oibvh_batch_add_mesh", _i, [_vp, _u32, _vp, _u32, _vp, _u32, _u32p]),

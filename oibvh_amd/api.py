@@ -245,6 +245,10 @@ class Scene:
                  | (L.DETECT_ASYNC if asynchronous else 0))
         _check(self.ctx.h, self._lib.oibvh_scene_detect(self.h, entry_level, expand_levels, flags), "detectCollision")
 
+    def refit(self) -> None:
+        """Refit every tree of the scene in one launch per stage."""
+        _check(self.ctx.h, self._lib.oibvh_scene_refit(self.h), "scene refit")
+
     def getIntTriPairCount(self) -> int:
         v = C.c_uint32()
         _check(self.ctx.h, self._lib.oibvh_scene_get_pair_count(self.h, C.byref(v)), "getIntTriPairCount")

@@ -69,16 +69,37 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wsum
 // ---------------------------------------------------------------------------
 // mesh tables
 // ---------------------------------------------------------------------------
+// Meshes come from a device table with a work list, or (d = null) travel in
+// the kernel arguments: up to OI_INLINE_MESHES meshes, mesh m owning the
+// workgroups [boff[m], boff[m + 1]).
+#define OI_INLINE_MESHES 4
 struct BuildTab {
-    const oi_build_desc* d;  // device table; null: `one` is mesh 0
-    const uint2* work;       // workgroup -> (mesh, chunk); null: (0, blockIdx.x)
-    oi_build_desc one;
+    const oi_build_desc* d;  // device table, or null
+    const uint2* work;       // workgroup -> (mesh, chunk) for the device table
+    uint32_t n_inl;
+    uint32_t boff[OI_INLINE_MESHES + 1];
+    oi_build_desc inl[OI_INLINE_MESHES];
     __device__ __forceinline__ uint2 item() const
     {
         const uint32_t w = oi_xcd_remap(blockIdx.x, gridDim.x);
-        return work ? work[w] : make_uint2(0u, w);
+        if (work) return work[w];
+        uint32_t m = 0;
+#pragma unroll
+        for (uint32_t k = 1; k < OI_INLINE_MESHES; k++)
+            if (k < n_inl && w >= boff[k]) m = k;
+        return make_uint2(m, w - boff[m]);
     }
-    __device__ __forceinline__ oi_build_desc mesh(uint32_t m) const { return d ? d[m] : one; }
+    __device__ __forceinline__ oi_build_desc mesh(uint32_t m) const
+    {
+        if (d) return d[m];
+        // constant indices keep the argument block out of scratch memory
+        switch (m) {
+        case 0: return inl[0];
+        case 1: return inl[1];
+        case 2: return inl[2];
+        default: return inl[3];
+        }
+    }
 };
 
 // a mesh's model matrix, held in registers
@@ -689,7 +710,22 @@ static BuildTab make_btab(const oi_build_desc* d, const uint2* work, const oi_bu
     BuildTab t{};
     t.d = d;
     t.work = work;
-    if (one) t.one = *one;
+    t.n_inl = 1;
+    if (one) t.inl[0] = *one;
+    return t;
+}
+
+// n (<= OI_INLINE_MESHES) meshes in the kernel arguments; mesh m owns
+// workgroups [boff[m], boff[m + 1])
+static BuildTab make_btab_inline(const oi_build_desc* meshes, uint32_t n, const uint32_t* boff)
+{
+    BuildTab t{};
+    t.n_inl = n;
+    for (uint32_t m = 0; m < n && m < OI_INLINE_MESHES; m++) {
+        t.inl[m] = meshes[m];
+        t.boff[m] = boff[m];
+    }
+    t.boff[n < OI_INLINE_MESHES ? n : OI_INLINE_MESHES] = boff[n];
     return t;
 }
 
@@ -761,6 +797,37 @@ hipError_t oi_launch_reduce(hipStream_t st, int M, const oi_build_desc* d, const
     case 4: hipLaunchKernelGGL(k_reduce<4>, dim3(nblocks), dim3(OI_BLOCK), 0, st, a); break;
     default: hipLaunchKernelGGL(k_reduce<8>, dim3(nblocks), dim3(OI_BLOCK), 0, st, a); break;
     }
+    return hipGetLastError();
+}
+
+uint32_t oi_inline_meshes() { return OI_INLINE_MESHES; }
+
+// stage 1 over up to OI_INLINE_MESHES meshes passed by value; boff[n + 1]
+// gives each mesh's first workgroup
+hipError_t oi_launch_reduce_inline(hipStream_t st, int M, const oi_build_desc* meshes, uint32_t n,
+                                   const uint32_t* boff, const uint32_t* vals, uint32_t write_perm)
+{
+    if (n == 0 || n > OI_INLINE_MESHES) return hipErrorInvalidValue;
+    ReduceArgs a{make_btab_inline(meshes, n, boff), vals, write_perm};
+    const dim3 g(boff[n]), b(OI_BLOCK);
+    switch (M) {
+    case 1: hipLaunchKernelGGL(k_reduce<1>, g, b, 0, st, a); break;
+    case 2: hipLaunchKernelGGL(k_reduce<2>, g, b, 0, st, a); break;
+    case 4: hipLaunchKernelGGL(k_reduce<4>, g, b, 0, st, a); break;
+    default: hipLaunchKernelGGL(k_reduce<8>, g, b, 0, st, a); break;
+    }
+    return hipGetLastError();
+}
+
+// stage 2 for n inline meshes, one workgroup each
+hipError_t oi_launch_reduce_apex_inline(hipStream_t st, int M, const oi_build_desc* meshes, uint32_t n)
+{
+    if (n == 0 || n > OI_INLINE_MESHES) return hipErrorInvalidValue;
+    uint32_t boff[OI_INLINE_MESHES + 1];
+    for (uint32_t m = 0; m <= n; m++) boff[m] = m;
+    ReduceArgs a{make_btab_inline(meshes, n, boff), nullptr, 0};
+    const uint32_t lgm = (M == 1) ? 0 : (M == 2) ? 1 : (M == 4) ? 2 : 3;
+    hipLaunchKernelGGL(k_reduce_apex, dim3(n), dim3(OI_BLOCK), 0, st, a, lgm);
     return hipGetLastError();
 }
 

@@ -440,6 +440,9 @@ struct StepArgs {
     uint32_t* error;
     unsigned long long* tests;  // box pair tests (statistics)
     unsigned long long* adj;    // self-mode leaf pairs dropped as adjacent
+    uint4* hits;                // fused narrow phase: hits and their count
+    uint32_t* n_hits;
+    uint32_t hits_cap;
     uint32_t implicit_end;
 };
 
@@ -471,14 +474,40 @@ __device__ __forceinline__ bool share_vertex(const uint32_t* a, const uint32_t* 
     return s;
 }
 
-template <int E, bool FIRST, class Tab>
+template <bool FP64, class Tab>
+__device__ __forceinline__ void narrow_one(const Tab& T, const uint32_t* s_ibase, const oi_mesh_desc* s_desc,
+                                           uint2 c, bool& hit, uint4& out, uint32_t& n_fp64, uint32_t& n_adj);
+
+// position of the k-th set bit of m (k < popcount(m))
+__device__ __forceinline__ uint32_t select_bit(u64 m, uint32_t k)
+{
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t w = 32; w >= 1; w >>= 1) {
+        const uint32_t c = (uint32_t)__popcll(m & ((1ull << w) - 1ull));
+        if (k >= c) {
+            k -= c;
+            m >>= w;
+            pos += w;
+        }
+    }
+    return pos;
+}
+
+// FUSED: overlapping leaf pairs are tested with the FP32 SAT by the
+// workgroup that found them (cooperatively: candidate c of the workgroup is
+// found by a binary search over the threads' candidate offsets and a select
+// in the owner's mask) instead of going to the candidate list.
+template <int E, bool FIRST, bool FUSED, class Tab>
 __global__ void __launch_bounds__(TB) k_bvtt_step(StepArgs<Tab> A)
 {
     constexpr int K = 1 << E;
     __shared__ oi_mesh_desc s_desc[MESH_LDS];
     __shared__ uint32_t s_ibase[MESH_LDS];
     __shared__ uint32_t s_wsum[4];
-    __shared__ uint32_t s_obase, s_cbase;
+    __shared__ uint32_t s_obase, s_cbase, s_hbase;
+    __shared__ u64 s_cmask[FUSED ? TB : 1];
+    __shared__ uint32_t s_coff[FUSED ? TB : 1], s_ga[FUSED ? TB : 1], s_gb[FUSED ? TB : 1];
     // after an overflow the count keeps growing past the buffer (writes were
     // dropped and the detect will be re-run): read only what exists
     uint32_t n_in = A.count_in ? *A.count_in : A.n_in;
@@ -486,7 +515,7 @@ __global__ void __launch_bounds__(TB) k_bvtt_step(StepArgs<Tab> A)
     if (blockIdx.x * TB >= n_in) return;
     tab_init(A.tab, s_desc, s_ibase);
     const uint32_t tid = threadIdx.x;
-    uint32_t n_tests = 0, n_adj = 0;
+    uint32_t n_tests = 0, n_adj = 0, n_cands = 0, n_fp64 = 0;
     for (uint32_t base = blockIdx.x * TB; base < n_in; base += gridDim.x * TB) {
         const uint32_t i = base + tid;
         bool have = i < n_in;
@@ -574,10 +603,16 @@ __global__ void __launch_bounds__(TB) k_bvtt_step(StepArgs<Tab> A)
         const uint32_t coff = block_scan(leafout ? nout : 0u, s_wsum, C);
         if (tid == 0) {
             s_obase = T ? add_u32(A.count_out, T) : 0u;
-            s_cbase = C ? add_u32(A.cand_count, C) : 0u;
+            if (!FUSED) s_cbase = C ? add_u32(A.cand_count, C) : 0u;
+        }
+        if (FUSED) {
+            s_cmask[tid] = leafout ? mask : 0ull;
+            s_coff[tid] = coff;
+            s_ga[tid] = baseA + sa.start;
+            s_gb[tid] = baseB + sb.start;
         }
         __syncthreads();
-        if (nout) {
+        if (nout && !(FUSED && leafout)) {
             uint2* dst = leafout ? A.cands : A.front_out;
             const uint32_t cap = leafout ? A.cand_cap : A.cap_out;
             uint32_t o = leafout ? s_cbase + coff : s_obase + off;
@@ -590,6 +625,38 @@ __global__ void __launch_bounds__(TB) k_bvtt_step(StepArgs<Tab> A)
                 o++;
             }
         }
+        if (FUSED && C) {
+            if (tid == 0) n_cands += C;
+            for (uint32_t c0 = 0; c0 < C; c0 += TB) {
+                const uint32_t c = c0 + tid;
+                bool hit = false;
+                uint4 h = make_uint4(0, 0, 0, 0);
+                if (c < C) {
+                    uint32_t lo = 0, hi = TB - 1;  // last thread whose offset is <= c
+                    while (lo < hi) {
+                        const uint32_t mid = (lo + hi + 1u) >> 1;
+                        if (s_coff[mid] <= c)
+                            lo = mid;
+                        else
+                            hi = mid - 1u;
+                    }
+                    const uint32_t bit = select_bit(s_cmask[lo], c - s_coff[lo]);
+                    const uint2 pr = make_uint2(s_ga[lo] + (bit >> E), s_gb[lo] + (bit & (K - 1)));
+                    narrow_one<false, Tab>(A.tab, s_ibase, s_desc, pr, hit, h, n_fp64, n_adj);
+                }
+                uint32_t H;
+                const uint32_t hoff = block_scan(hit ? 1u : 0u, s_wsum, H);
+                if (tid == 0) s_hbase = H ? add_u32(A.n_hits, H) : 0u;
+                __syncthreads();
+                if (hit) {
+                    const uint32_t o = s_hbase + hoff;
+                    if (o < A.hits_cap)
+                        A.hits[o] = h;
+                    else
+                        or_u32(A.error, OI_ERR_HITS_OVERFLOW);
+                }
+            }
+        }
         __syncthreads();
     }
     uint32_t nt, na;
@@ -597,6 +664,7 @@ __global__ void __launch_bounds__(TB) k_bvtt_step(StepArgs<Tab> A)
     block_scan(n_adj, s_wsum, na);
     if (tid == 0 && nt) __hip_atomic_fetch_add(A.tests, (u64)nt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (tid == 0 && na) __hip_atomic_fetch_add(A.adj, (u64)na, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (FUSED && tid == 0 && n_cands) add_u32(A.cand_count, n_cands);
 }
 
 // ---------------------------------------------------------------------------
@@ -742,7 +810,8 @@ template <class Tab>
 static hipError_t launch_step(hipStream_t st, uint32_t grid, const Tab& tab, int first, const uint2* front_in,
                               const uint32_t* count_in, uint32_t n_in, uint32_t cap_in, uint2* front_out,
                               uint32_t* count_out, uint32_t cap_out, uint2* cands, uint32_t* cand_count,
-                              uint32_t cand_cap, oi_detect_ctl* ctl, uint32_t expand, uint32_t implicit_end)
+                              uint32_t cand_cap, uint4* hits, uint32_t hits_cap, oi_detect_ctl* ctl, uint32_t expand,
+                              uint32_t implicit_end, int fused)
 {
     StepArgs<Tab> a{};
     a.tab = tab;
@@ -759,13 +828,20 @@ static hipError_t launch_step(hipStream_t st, uint32_t grid, const Tab& tab, int
     a.error = &ctl->error;
     a.tests = &ctl->tests;
     a.adj = &ctl->adj;
+    a.hits = hits;
+    a.n_hits = &ctl->n_hits;
+    a.hits_cap = hits_cap;
     a.implicit_end = implicit_end;
     const dim3 g(grid), b(TB);
-#define OI_STEP(EE)                                                        \
-    if (first)                                                             \
-        hipLaunchKernelGGL((k_bvtt_step<EE, true, Tab>), g, b, 0, st, a);  \
-    else                                                                   \
-        hipLaunchKernelGGL((k_bvtt_step<EE, false, Tab>), g, b, 0, st, a);
+#define OI_STEP(EE)                                                                    \
+    if (first && fused)                                                                \
+        hipLaunchKernelGGL((k_bvtt_step<EE, true, true, Tab>), g, b, 0, st, a);        \
+    else if (first)                                                                    \
+        hipLaunchKernelGGL((k_bvtt_step<EE, true, false, Tab>), g, b, 0, st, a);       \
+    else if (fused)                                                                    \
+        hipLaunchKernelGGL((k_bvtt_step<EE, false, true, Tab>), g, b, 0, st, a);       \
+    else                                                                               \
+        hipLaunchKernelGGL((k_bvtt_step<EE, false, false, Tab>), g, b, 0, st, a);
     switch (expand) {
     case 1: OI_STEP(1) break;
     case 2: OI_STEP(2) break;
@@ -810,19 +886,21 @@ hipError_t oi_launch_seed(hipStream_t st, uint32_t n0, uint32_t* counts, uint32_
 }
 
 // one BVTT step; front_in's length is *count_in, or n_in when count_in is null
+// fused: overlapping leaf pairs are tested with the FP32 SAT inside the step
+// (hits appended directly) instead of going to the candidate list
 hipError_t oi_launch_bvtt_step(hipStream_t st, uint32_t grid, const oi_tab_args* t, int first,
                                const uint2* front_in, const uint32_t* count_in, uint32_t n_in, uint32_t cap_in,
                                uint2* front_out, uint32_t* count_out, uint32_t cap_out, uint2* cands,
-                               uint32_t* cand_count, uint32_t cand_cap, oi_detect_ctl* ctl, uint32_t expand,
-                               uint32_t implicit_end)
+                               uint32_t* cand_count, uint32_t cand_cap, uint4* hits, uint32_t hits_cap,
+                               oi_detect_ctl* ctl, uint32_t expand, uint32_t implicit_end, int fused)
 {
     if (t->chunk_mesh)
         return launch_step(st, grid, ChunkTab{t->d_meshes, t->chunk_mesh, t->shift}, first, front_in, count_in, n_in,
-                           cap_in, front_out, count_out, cap_out, cands, cand_count, cand_cap, ctl, expand,
-                           implicit_end);
+                           cap_in, front_out, count_out, cap_out, cands, cand_count, cand_cap, hits, hits_cap, ctl,
+                           expand, implicit_end, fused);
     return launch_step(st, grid, make_tab(t->d_meshes, t->h_meshes, t->n_meshes), first, front_in, count_in, n_in,
-                       cap_in, front_out, count_out, cap_out, cands, cand_count, cand_cap, ctl, expand,
-                       implicit_end);
+                       cap_in, front_out, count_out, cap_out, cands, cand_count, cand_cap, hits, hits_cap, ctl,
+                       expand, implicit_end, fused);
 }
 
 hipError_t oi_launch_narrow(hipStream_t st, uint32_t grid, const oi_tab_args* t, const uint2* cands,

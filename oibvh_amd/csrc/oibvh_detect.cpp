@@ -87,16 +87,21 @@ oibvh_status oi_det_enqueue(oi_detector* d, bool reset_sticky)
     hipStream_t st = c->stream;
     HIPCHK(c, oi_launch_seed(st, d->n_front, d->counts, d->n_levels + 1, d->ctl, reset_sticky ? 1u : 0u));
     const uint32_t grid = oi_grid_for(c);
+    // FP32 queries test leaf pairs inside the steps; the FP64 fallback keeps
+    // its own narrow pass (its registers would slow every step)
+    const bool fp64 = (d->flags & OIBVH_DETECT_FP64_FALLBACK) != 0;
+    d->fused = !fp64;
     for (uint32_t k = 0; k < d->n_levels; k++) {
         // step 0 reads the initial front in place; then the fronts alternate
         const uint2* in = k ? d->front[(k - 1) & 1] : d->d_init;
         HIPCHK(c, oi_launch_bvtt_step(st, grid, &d->tab, k == 0, in, k ? d->counts + k : nullptr, d->n_front,
                                       k ? d->front_cap : d->n_front, d->front[k & 1], d->counts + k + 1,
-                                      d->front_cap, d->cands, &d->ctl->cands, d->cand_cap, d->ctl, d->expand,
-                                      d->implicit_end));
+                                      d->front_cap, d->cands, &d->ctl->cands, d->cand_cap, d->hits, d->hits_cap,
+                                      d->ctl, d->expand, d->implicit_end, d->fused ? 1 : 0));
     }
-    HIPCHK(c, oi_launch_narrow(st, grid, &d->tab, d->cands, &d->ctl->cands, d->cand_cap, d->hits, &d->ctl->n_hits,
-                               d->hits_cap, d->ctl, (d->flags & OIBVH_DETECT_FP64_FALLBACK) ? 1 : 0));
+    if (!d->fused)
+        HIPCHK(c, oi_launch_narrow(st, grid, &d->tab, d->cands, &d->ctl->cands, d->cand_cap, d->hits,
+                                   &d->ctl->n_hits, d->hits_cap, d->ctl, 1));
     d->unchecked = true;
     d->have_result = true;
     return OIBVH_OK;
@@ -148,7 +153,7 @@ oibvh_status oi_det_check(oi_detector* d)
             d->front_cap = (uint32_t)need;
             rerun = true;
         }
-        if ((k.error & OI_ERR_CAND_OVERFLOW) || k.cands > d->cand_cap) {
+        if ((k.error & OI_ERR_CAND_OVERFLOW) || (!d->fused && k.cands > d->cand_cap)) {
             r = grow(c, d->cands, d->cand_cap, std::max<uint64_t>(k.cands, 2ull * d->cand_cap), d->generation);
             if (r != OIBVH_OK) return r;
             rerun = true;

@@ -672,6 +672,43 @@ extern "C" oibvh_status oibvh_scene_detect(oibvh_scene* s, uint32_t entry, uint3
     return oi_det_check(&d);
 }
 
+// Every tree of the scene refit at once: up to four trees per launch pair
+// (their descriptors travel in the kernel arguments), so the reduce fills the
+// GPU once instead of once per tree. Same results as oibvh_tree_refit on each.
+extern "C" oibvh_status oibvh_scene_refit(oibvh_scene* s)
+{
+    if (!s) return OIBVH_ERR_INVALID;
+    oibvh_ctx* c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    for (oibvh_tree* t : s->trees)
+        if (!t->built) return oi_set_err(c, OIBVH_ERR_NOT_BUILT, "scene refit: tree not built");
+    const uint32_t per = oi_inline_meshes();
+    for (size_t i0 = 0; i0 < s->trees.size(); i0 += per) {
+        const uint32_t n = (uint32_t)std::min<size_t>(per, s->trees.size() - i0);
+        uint64_t F = 0;
+        for (uint32_t k = 0; k < n; k++) F += s->trees[i0 + k]->F;
+        int M = (int)c->leaves_per_thread;
+        if (!M) {
+            const uint64_t target = 2ull * (uint64_t)(c->cus > 0 ? c->cus : 256) * 256ull;
+            M = 1;
+            while (M < 8 && F / (2ull * M) >= target) M *= 2;
+        }
+        oi_build_desc d[8], apex[8];
+        uint32_t boff[9], na = 0;
+        boff[0] = 0;
+        for (uint32_t k = 0; k < n; k++) {
+            const oibvh_tree* t = s->trees[i0 + k];
+            d[k] = tree_desc(t);
+            const uint32_t h = oi_reduce_height(M, t->L);
+            boff[k + 1] = boff[k] + oi_lvl_cnt(t->L - h, t->L, t->vl);
+            if (t->L > h) apex[na++] = d[k];
+        }
+        HIPCHK(c, oi_launch_reduce_inline(c->stream, M, d, n, boff, nullptr, 0));
+        if (na) HIPCHK(c, oi_launch_reduce_apex_inline(c->stream, M, apex, na));
+    }
+    return OIBVH_OK;
+}
+
 extern "C" oibvh_status oibvh_scene_get_pair_count(oibvh_scene* s, uint32_t* count)
 {
     if (!s || !count) return OIBVH_ERR_INVALID;

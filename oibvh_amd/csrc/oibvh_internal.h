@@ -29,11 +29,15 @@ hipError_t oi_launch_reduce(hipStream_t, int, const oi_build_desc*, const uint2*
 hipError_t oi_launch_reduce_apex(hipStream_t, int, const oi_build_desc*, const uint2*, const oi_build_desc*,
                                  uint32_t);
 hipError_t oi_launch_set_xf(hipStream_t, float*, const float*);
+uint32_t oi_inline_meshes();
+hipError_t oi_launch_reduce_inline(hipStream_t, int, const oi_build_desc*, uint32_t, const uint32_t*, const uint32_t*,
+                                   uint32_t);
+hipError_t oi_launch_reduce_apex_inline(hipStream_t, int, const oi_build_desc*, uint32_t);
 hipError_t oi_launch_set_mesh_pos(hipStream_t, oi_build_desc*, oi_mesh_desc*, uint32_t, const float*);
 hipError_t oi_launch_seed(hipStream_t, uint32_t, uint32_t*, uint32_t, oi_detect_ctl*, uint32_t);
 hipError_t oi_launch_bvtt_step(hipStream_t, uint32_t, const oi_tab_args*, int, const uint2*, const uint32_t*,
-                               uint32_t, uint32_t, uint2*, uint32_t*, uint32_t, uint2*, uint32_t*, uint32_t,
-                               oi_detect_ctl*, uint32_t, uint32_t);
+                               uint32_t, uint32_t, uint2*, uint32_t*, uint32_t, uint2*, uint32_t*, uint32_t, uint4*,
+                               uint32_t, oi_detect_ctl*, uint32_t, uint32_t, int);
 hipError_t oi_launch_narrow(hipStream_t, uint32_t, const oi_tab_args*, const uint2*, const uint32_t*, uint32_t,
                             uint4*, uint32_t*, uint32_t, oi_detect_ctl*, int);
 }
@@ -123,6 +127,7 @@ struct oi_detector {
     // the query (kept for re-runs)
     oi_tab_args tab{};
     uint32_t depth = 0, expand = 1, flags = 0, implicit_end = 0;
+    bool fused = false;  // narrow phase inside the BVTT steps (FP32 queries)
     bool unchecked = false;
     bool have_result = false;
     uint32_t n_pairs = 0;

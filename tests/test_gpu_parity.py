@@ -179,6 +179,37 @@ def test_transform_build_refit_detect(api, ctx, orc):
     assert np.array_equal(bits(t0.nodes()), bits(orc.Tree(x0, f0, order=t0.perm()).nodes()))
 
 
+@pytest.mark.parametrize("n", [1, 3, 6])
+def test_scene_refit_equals_tree_refits(api, ctx, orc, n):
+    """oibvh_scene_refit (up to four trees per launch) gives every tree the
+    nodes its own refit gives: mixed sizes (one without an apex stage), one
+    tree with a model matrix."""
+    say(f"scene refit over {n} trees")
+    shapes = [M.icosphere(4, 1.0, 0.02, 5), soup(3000, 11), M.uv_sphere(200, 101), soup(700, 12),
+              M.box(20, 0.5, 0.02, 3), M.icosphere(3, 1.0, 0.02, 6)][:n]
+    sc = api.Scene(ctx)
+    trees = []
+    for k, (x, f) in enumerate(shapes):
+        t = api.OibvhTree(ctx, x, f)
+        t.build()
+        sc.addOibvhTree(t)
+        trees.append(t)
+    if n > 1:
+        trees[1].set_transform(_motion(7, 1))
+    moved = []
+    for k, (x, f) in enumerate(shapes):
+        y = (x * np.float32(1.01) + np.float32(0.003 * k)).astype(np.float32)
+        trees[k].set_positions(y)
+        moved.append(y)
+    sc.refit()
+    for k, (x, f) in enumerate(shapes):
+        y = moved[k]
+        if n > 1 and k == 1:
+            y = orc.xform(_motion(7, 1), y)
+        o = orc.Tree(y, f, order=trees[k].perm())
+        assert np.array_equal(bits(trees[k].nodes()), bits(o.nodes())), f"tree {k}"
+
+
 def test_create_like_then_refit(api, ctx, orc):
     """main.cpp:127-151: tree 2 copy-constructed from tree 1 over a
     translated mesh with the same faces, then refit."""
