@@ -42,10 +42,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
-# HBM bytes per refit of one 1M-triangle tree (k_reduce + k_reduce_apex),
-# FETCH_SIZE + WRITE_SIZE from two rocprofv3 --pmc runs, scaled by the counters'
-# reading for a device-to-device copy of known size (profiles/refit_traffic_deforming_uv_pair.md)
-TRAFFIC_REFIT = None
+# HBM bytes of one scene refit of the two 1M-triangle trees (k_reduce), as
+# measured by two rocprofv3 counter runs of this bench (FETCH_SIZE, then
+# WRITE_SIZE, per dispatch), each scaled by the counter's reading for a 256 MiB
+# device-to-device copy (--copy-probe-mb 256): 37.0 MB fetched + 96.0 MB
+# written. Re-measure after changing the refit kernel.
+TRAFFIC_REFIT = 133.0e6
 METRIC = "frames/s & Mtri/s (build+refit+broad+narrow), batched vs plain, 0 pair diffs"
 
 
