@@ -111,6 +111,14 @@ OIBVH_API oibvh_status oibvh_tree_set_positions(oibvh_tree* tree, const float* x
  * changed. Refit (or build) afterwards. */
 OIBVH_API oibvh_status oibvh_tree_set_transform(oibvh_tree* tree, const float* mat4);
 
+/* Morton key width of later builds: 30 (the reference's morton3D, 10 bits per
+ * axis, oibvh.cu:6-22; the default) or 63 (21 bits per axis: the same
+ * centroid, box and quantisation with 2^21 cells, NaN -> 0). No reference
+ * counterpart: 63-bit keys change the leaf order (and so the tree) of meshes
+ * whose centroids share a 30-bit cell; the pair set of a detect does not
+ * depend on it. */
+OIBVH_API oibvh_status oibvh_tree_set_morton_bits(oibvh_tree* tree, uint32_t bits);
+
 /* OibvhTree::getDepth() (oibvhTree.cuh:87, oibvhTree.cu:46-49): ilog2(node count) */
 OIBVH_API oibvh_status oibvh_tree_get_depth(const oibvh_tree* tree, uint32_t* depth);
 /* OibvhTree::getPrimCount() (oibvhTree.cuh:93, oibvhTree.cu:51-54) */
@@ -249,6 +257,10 @@ OIBVH_API oibvh_status oibvh_batch_mesh_count(const oibvh_batch* batch, uint32_t
  * is read before return; device memory (OIBVH_PTR_DEVICE) is copied in stream
  * order, so a caller can keep every frame's matrices in HBM. */
 OIBVH_API oibvh_status oibvh_batch_set_transforms(oibvh_batch* batch, const float* mats, uint32_t ptr_kind);
+
+/* Morton key width (30 or 63) of the next build of every mesh, as
+ * oibvh_tree_set_morton_bits. */
+OIBVH_API oibvh_status oibvh_batch_set_morton_bits(oibvh_batch* batch, uint32_t bits);
 
 /* Device address of a mesh's vertex positions inside the batch (V*3 floats;
  * shared with its instances). Callers may rewrite them in stream order

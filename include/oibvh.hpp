@@ -153,6 +153,8 @@ public:
     // build/refit/detect see mat4 * (p, 1). mat4 = 16 floats, column-major
     // (glm::value_ptr of a glm::mat4); nullptr restores the identity.
     void setTransform(const float* mat4) { ctx_->check(oibvh_tree_set_transform(h_, mat4), "setTransform"); }
+    // Morton key width of later builds: 30 (the reference's) or 63
+    void setMortonBits(uint32_t bits) { ctx_->check(oibvh_tree_set_morton_bits(h_, bits), "setMortonBits"); }
 
     // leaf position -> original face id
     std::vector<uint32_t> sortedTriIndex() const
@@ -264,6 +266,7 @@ public:
     {
         ctx_->check(oibvh_batch_set_transforms(h_, mats, ptrKind), "Batch::setTransforms");
     }
+    void setMortonBits(uint32_t bits) { ctx_->check(oibvh_batch_set_morton_bits(h_, bits), "Batch::setMortonBits"); }
     void build() { ctx_->check(oibvh_batch_build(h_), "Batch::build"); }
     void refit() { ctx_->check(oibvh_batch_refit(h_), "Batch::refit"); }
     void detectCollision(unsigned int entryLevel = 0, unsigned int expandLevels = 1, uint32_t flags = 0)

@@ -57,6 +57,7 @@ API = [
     ("oibvh_tree_refit", _i, [_vp, _vp, _u32]),
     ("oibvh_tree_set_positions", _i, [_vp, _vp, _u32]),
     ("oibvh_tree_set_transform", _i, [_vp, _f32p]),
+    ("oibvh_tree_set_morton_bits", _i, [_vp, _u32]),
     ("oibvh_tree_get_depth", _i, [_vp, _u32p]),
     ("oibvh_tree_get_prim_count", _i, [_vp, _u32p]),
     ("oibvh_tree_get_node_count", _i, [_vp, _u32p]),
@@ -82,6 +83,7 @@ API = [
     ("oibvh_batch_add_instance", _i, [_vp, _u32, _u32, _u32p]),
     ("oibvh_batch_mesh_count", _i, [_vp, _u32p]),
     ("oibvh_batch_set_transforms", _i, [_vp, _vp, _u32]),
+    ("oibvh_batch_set_morton_bits", _i, [_vp, _u32]),
     ("oibvh_batch_positions_device", _i, [_vp, _u32, C.POINTER(_vp)]),
     ("oibvh_batch_set_positions", _i, [_vp, _u32, _vp, _u32]),
     ("oibvh_batch_build", _i, [_vp]),

@@ -166,6 +166,10 @@ class OibvhTree:
         m = np.ascontiguousarray(np.asarray(mat4, np.float32).reshape(4, 4).T)  # column-major
         _check(self.ctx.h, self._lib.oibvh_tree_set_transform(self.h, m.ctypes.data_as(L._f32p)), "set_transform")
 
+    def set_morton_bits(self, bits: int) -> None:
+        """Key width of later builds: 30 (morton3D, oibvh.cu:6-22) or 63."""
+        _check(self.ctx.h, self._lib.oibvh_tree_set_morton_bits(self.h, bits), "set_morton_bits")
+
     def refit_device(self, dptr: int, borrow: bool = True) -> None:
         kind = L.PTR_DEVICE_BORROW if borrow else L.PTR_DEVICE
         _check(self.ctx.h, self._lib.oibvh_tree_refit(self.h, C.c_void_p(dptr), kind), "refit")
@@ -329,6 +333,10 @@ class Batch:
         """mats: (n, 4, 4) matrices (row, column); stored column-major."""
         m = np.ascontiguousarray(np.asarray(mats, np.float32).reshape(-1, 4, 4).transpose(0, 2, 1))
         _check(self.ctx.h, self._lib.oibvh_batch_set_transforms(self.h, m.ctypes.data, L.PTR_HOST), "set_transforms")
+
+    def set_morton_bits(self, bits: int) -> None:
+        """Key width (30 or 63) of the next build of every mesh."""
+        _check(self.ctx.h, self._lib.oibvh_batch_set_morton_bits(self.h, bits), "set_morton_bits")
 
     def set_transforms_device(self, dptr: int) -> None:
         """n x 16 column-major floats already in device memory."""

@@ -46,7 +46,9 @@ struct oibvh_batch {
     bool has_xf = false;
     bool borrowed = false;  // some mesh reads caller-owned vertex memory
     uint32_t *keys = nullptr, *keys_alt = nullptr, *vals = nullptr, *vals_tmp = nullptr, *fmesh = nullptr;
-    uint32_t* sort_scratch = nullptr;  // ghist[1024] | tile counters[8] | lookback0 | lookback1
+    uint32_t* keys_hi = nullptr;       // high words of 63-bit keys
+    uint32_t morton_bits = 30;
+    uint32_t* sort_scratch = nullptr;  // histograms | tile counters | lookback0 | lookback1
     uint32_t* mesh_hist = nullptr;     // digit histograms of the mesh ids, 256 per pass
     uint32_t tiles = 0, mesh_passes = 0;
     oi_build_desc* d_bdesc = nullptr;
@@ -109,6 +111,7 @@ static void batch_free_device(oibvh_batch* b)
     dfree(b->xf);
     dfree(b->keys);
     dfree(b->keys_alt);
+    dfree(b->keys_hi);
     dfree(b->vals);
     dfree(b->vals_tmp);
     dfree(b->fmesh);
@@ -346,7 +349,7 @@ static oibvh_status commit(oibvh_batch* b)
     b->mesh_passes = n <= 1 ? 0 : n <= 256 ? 1 : n <= 65536 ? 2 : 3;
     if (b->mesh_passes) HIPCHK(c, dalloc(&b->fmesh, F));
     b->tiles = (uint32_t)((F + oi_sort_tile() - 1) / oi_sort_tile());
-    HIPCHK(c, dalloc(&b->sort_scratch, 1032ull + 512ull * b->tiles));
+    HIPCHK(c, dalloc(&b->sort_scratch, OI_SORT_HEAD + 512ull * b->tiles));
     {
         std::vector<uint32_t> mh(768, 0u);
         for (uint32_t i = 0; i < n; i++)
@@ -480,27 +483,34 @@ extern "C" oibvh_status oibvh_batch_build(oibvh_batch* b)
     if ((r = ensure_reduce_tables(b)) != OIBVH_OK) return r;
     hipStream_t s = c->stream;
     const uint32_t F = (uint32_t)b->Ft;
+    const bool m63 = b->morton_bits == 63;
+    if (m63 && !b->keys_hi) HIPCHK(c, dalloc(&b->keys_hi, F));
     uint32_t* ghist = b->sort_scratch;
-    uint32_t* tctr = b->sort_scratch + 1024;
-    uint32_t* lb[2] = {b->sort_scratch + 1032, b->sort_scratch + 1032 + 256ull * b->tiles};
+    uint32_t* tctr = b->sort_scratch + OI_SORT_HIST;
+    uint32_t* lb[2] = {b->sort_scratch + OI_SORT_HEAD, b->sort_scratch + OI_SORT_HEAD + 256ull * b->tiles};
     HIPCHK(c, oi_launch_mesh_box(s, b->d_bdesc, b->w_box, nullptr, b->n_box, b->partials, b->sort_scratch,
-                                 1032u + 256u * b->tiles));
-    HIPCHK(c, oi_launch_morton(s, b->d_bdesc, b->w_morton, nullptr, b->n_morton, b->partials, b->keys, ghist,
-                               b->fmesh));
-    // four Morton passes, then the mesh-id passes; lookback buffers alternate
-    const uint32_t passes = 4 + b->mesh_passes;
+                                 OI_SORT_HEAD + 256u * b->tiles));
+    HIPCHK(c, oi_launch_morton(s, b->d_bdesc, b->w_morton, nullptr, b->n_morton, b->partials, b->keys,
+                               m63 ? b->keys_hi : nullptr, ghist, b->fmesh));
+    // Morton passes (four over the low word with the keys moving along; for
+    // 63-bit keys four more over the high word, looked up by face id), then
+    // the mesh-id passes; lookback buffers alternate
+    const uint32_t mpasses = m63 ? 8 : 4;
+    const uint32_t passes = mpasses + b->mesh_passes;
     uint32_t* kin = b->keys;
     uint32_t* kout = b->keys_alt;
     const uint32_t* vin = nullptr;
     uint32_t* vout = b->vals_tmp;
     for (uint32_t p = 0; p < passes; p++) {
-        const bool morton = p < 4;
+        const bool lo = p < 4;
+        const bool morton = p < mpasses;
         const bool last = (p + 1 == passes);
-        const uint32_t* hist = morton ? ghist + 256 * p : b->mesh_hist + 256 * (p - 4);
-        const uint32_t shift = morton ? 8 * p : 8 * (p - 4);
-        uint32_t* ko = (morton && p < 3) ? kout : nullptr;
-        HIPCHK(c, oi_launch_sort_pass(s, morton ? kin : nullptr, vin, ko, vout, F, shift, hist, lb[p & 1],
-                                      last ? nullptr : lb[(p + 1) & 1], tctr + p, morton ? nullptr : b->fmesh));
+        const uint32_t* hist = morton ? ghist + 256 * p : b->mesh_hist + 256 * (p - mpasses);
+        const uint32_t shift = morton ? 8 * (p & 3) : 8 * (p - mpasses);
+        uint32_t* ko = (lo && p < 3) ? kout : nullptr;
+        const uint32_t* kov = lo ? nullptr : (morton ? b->keys_hi : b->fmesh);
+        HIPCHK(c, oi_launch_sort_pass(s, lo ? kin : nullptr, vin, ko, vout, F, shift, hist, lb[p & 1],
+                                      last ? nullptr : lb[(p + 1) & 1], tctr + p, kov));
         std::swap(kin, kout);
         vin = vout;
         vout = (vout == b->vals_tmp) ? b->vals : b->vals_tmp;
@@ -508,6 +518,15 @@ extern "C" oibvh_status oibvh_batch_build(oibvh_batch* b)
     HIPCHK(c, oi_launch_reduce(s, b->reduce_M, b->d_bdesc, b->w_reduce, nullptr, b->n_reduce, vin, 1));
     if (b->n_apex) HIPCHK(c, oi_launch_reduce_apex(s, b->reduce_M, b->d_bdesc, b->w_apex, nullptr, b->n_apex));
     b->built = true;
+    return OIBVH_OK;
+}
+
+// key width of later builds: 30 (oibvh.cu:6-22) or 63 (21 bits per axis)
+extern "C" oibvh_status oibvh_batch_set_morton_bits(oibvh_batch* b, uint32_t bits)
+{
+    if (!b) return OIBVH_ERR_INVALID;
+    if (bits != 30 && bits != 63) return oi_set_err(b->ctx, OIBVH_ERR_INVALID, "morton bits must be 30 or 63");
+    b->morton_bits = bits;
     return OIBVH_OK;
 }
 

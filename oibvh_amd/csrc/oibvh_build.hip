@@ -242,19 +242,43 @@ __device__ __forceinline__ uint32_t quantise10(float p)
 // mesh m. Keys go to keys[fbase + face]; the four 8-bit digit histograms of
 // all keys of the launch accumulate in ghist; fmesh (if not null) receives
 // the mesh id of each face (fmesh[fbase + face] = m).
-__global__ void __launch_bounds__(OI_BLOCK)
-k_morton(BuildTab T, const float* __restrict__ partials, uint32_t* __restrict__ keys, uint32_t* __restrict__ ghist,
-         uint32_t* __restrict__ fmesh)
+// 63-bit form (no reference counterpart): 21 bits per axis, quantised as
+// min(max(p * 2^21, 0), 2^21 - 1), NaN -> 0
+__device__ __forceinline__ uint64_t spread21(uint64_t x)
 {
-    __shared__ uint32_t s_h[4][256];
+    x &= 0x1fffffull;
+    x = (x | (x << 32)) & 0x1f00000000ffffull;
+    x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+__device__ __forceinline__ uint32_t quantise21(float p)
+{
+    float q = p * 2097152.0f;
+    q = (q < 0.0f) ? 0.0f : q;
+    q = (2097151.0f < q) ? 2097151.0f : q;
+    return (q != q) ? 0u : (uint32_t)q;
+}
+
+// keys_hi == null: 30-bit keys and 4 digit histograms; else 63-bit keys split
+// into keys (low word) and keys_hi (high word) and 8 digit histograms
+__global__ void __launch_bounds__(OI_BLOCK)
+k_morton(BuildTab T, const float* __restrict__ partials, uint32_t* __restrict__ keys, uint32_t* __restrict__ keys_hi,
+         uint32_t* __restrict__ ghist, uint32_t* __restrict__ fmesh)
+{
+    __shared__ uint32_t s_h[8][256];
     __shared__ float s_v[6][OI_BLOCK];
     const uint32_t tid = threadIdx.x;
     const uint2 it = T.item();
     const oi_build_desc D = T.mesh(it.x);
     Xf X;
     X.load(D.xf);
+    const int nd = keys_hi ? 8 : 4;
 #pragma unroll
-    for (int d = 0; d < 4; d++) s_h[d][tid] = 0u;
+    for (int d = 0; d < 8; d++) s_h[d][tid] = 0u;
     float bmn[3], bmx[3];
     fold_partials(partials + 6u * D.pbase, D.nparts, s_v, bmn, bmx);
     if (it.y == 0 && tid < 3) {
@@ -270,7 +294,7 @@ k_morton(BuildTab T, const float* __restrict__ partials, uint32_t* __restrict__ 
         X.vertex(D.pos, fv[0], a);
         X.vertex(D.pos, fv[1], b);
         X.vertex(D.pos, fv[2], c);
-        uint32_t q[3];
+        float u[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             const float lo = gmin(gmin(a[k], b[k]), c[k]);
@@ -278,17 +302,29 @@ k_morton(BuildTab T, const float* __restrict__ partials, uint32_t* __restrict__ 
             const float cen = (lo + hi) * 0.5f;
             const float off = cen - bmn[k];
             const float len = bmx[k] - bmn[k];
-            q[k] = quantise10(off / len);
+            u[k] = off / len;
         }
-        const uint32_t key = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+        uint32_t key, khi = 0;
+        if (keys_hi) {
+            const uint64_t k64 = (spread21(quantise21(u[0])) << 2) | (spread21(quantise21(u[1])) << 1) |
+                                 spread21(quantise21(u[2]));
+            key = (uint32_t)k64;
+            khi = (uint32_t)(k64 >> 32);
+            oi_gw(keys_hi)[D.fbase + i] = khi;
+        } else {
+            key = (spread10(quantise10(u[0])) << 2) | (spread10(quantise10(u[1])) << 1) | spread10(quantise10(u[2]));
+        }
         oi_gw(keys)[D.fbase + i] = key;
         if (fmesh) oi_gw(fmesh)[D.fbase + i] = it.x;
 #pragma unroll
         for (int d = 0; d < 4; d++) atomicAdd(&s_h[d][(key >> (8 * d)) & 0xffu], 1u);
+        if (keys_hi) {
+#pragma unroll
+            for (int d = 0; d < 4; d++) atomicAdd(&s_h[4 + d][(khi >> (8 * d)) & 0xffu], 1u);
+        }
     }
     __syncthreads();
-#pragma unroll
-    for (int d = 0; d < 4; d++) {
+    for (int d = 0; d < nd; d++) {
         uint32_t c = s_h[d][tid];
         if (c) atomicAdd(&ghist[256 * d + tid], c);
     }
@@ -752,11 +788,13 @@ hipError_t oi_launch_set_xf(hipStream_t st, float* dst, const float* mat4)
     return hipGetLastError();
 }
 
+// keys_hi: null for 30-bit keys, else the high words of 63-bit keys
 hipError_t oi_launch_morton(hipStream_t st, const oi_build_desc* d, const uint2* work, const oi_build_desc* one,
-                            uint32_t nblocks, const float* partials, uint32_t* keys, uint32_t* ghist, uint32_t* fmesh)
+                            uint32_t nblocks, const float* partials, uint32_t* keys, uint32_t* keys_hi,
+                            uint32_t* ghist, uint32_t* fmesh)
 {
     hipLaunchKernelGGL(k_morton, dim3(nblocks), dim3(OI_BLOCK), 0, st, make_btab(d, work, one), partials, keys,
-                       ghist, fmesh);
+                       keys_hi, ghist, fmesh);
     return hipGetLastError();
 }
 

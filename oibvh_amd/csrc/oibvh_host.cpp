@@ -50,10 +50,12 @@ struct oibvh_tree {
     float* nodes = nullptr;
     // build scratch (allocated at the first build)
     uint32_t* keys = nullptr;
+    uint32_t* keys_hi = nullptr; // high words of 63-bit keys
     uint32_t* keys_alt = nullptr;
     uint32_t* vals_tmp = nullptr;
     uint32_t* sort_scratch = nullptr;
     uint32_t tiles = 0;
+    uint32_t morton_bits = 30;
     float* box = nullptr;        // 6 floats + per-block partial boxes
     float* xf = nullptr;         // model matrix (16 floats, device)
     bool has_xf = false;
@@ -172,6 +174,7 @@ static void tree_free(oibvh_tree* t)
     dfree(t->perm);
     dfree(t->nodes);
     dfree(t->keys);
+    dfree(t->keys_hi);
     dfree(t->keys_alt);
     dfree(t->vals_tmp);
     dfree(t->sort_scratch);
@@ -309,6 +312,7 @@ extern "C" oibvh_status oibvh_tree_create_like(oibvh_tree* other, const float* x
         return st;
     }
     t->built = other->built;
+    t->morton_bits = other->morton_bits;
     ctx->live++;
     *out = t;
     return OIBVH_OK;
@@ -356,34 +360,50 @@ extern "C" oibvh_status oibvh_tree_build(oibvh_tree* t)
     const uint32_t F = t->F;
     const uint32_t tile = oi_sort_tile();
     const uint32_t tiles = (F + tile - 1) / tile;
+    const bool m63 = t->morton_bits == 63;
     if (!t->keys) {
         HIPCHK(c, dalloc(&t->keys, F));
         HIPCHK(c, dalloc(&t->keys_alt, F));
         HIPCHK(c, dalloc(&t->vals_tmp, F));
         t->tiles = tiles;
-        // ghist[1024] | tile counters[8] | lookback0[tiles*256] | lookback1[tiles*256]
-        HIPCHK(c, dalloc(&t->sort_scratch, 1032ull + 512ull * tiles));
+        HIPCHK(c, dalloc(&t->sort_scratch, OI_SORT_HEAD + 512ull * tiles));
     }
+    if (m63 && !t->keys_hi) HIPCHK(c, dalloc(&t->keys_hi, F));
     uint32_t* ghist = t->sort_scratch;
-    uint32_t* tctr = t->sort_scratch + 1024;
-    uint32_t* lb0 = t->sort_scratch + 1032;
-    uint32_t* lb1 = lb0 + 256ull * tiles;
+    uint32_t* tctr = t->sort_scratch + OI_SORT_HIST;
+    uint32_t* lb[2] = {t->sort_scratch + OI_SORT_HEAD, t->sort_scratch + OI_SORT_HEAD + 256ull * tiles};
     hipStream_t s = c->stream;
     const oi_build_desc d = tree_desc(t);
-    HIPCHK(c, oi_launch_mesh_box(s, nullptr, nullptr, &d, d.nparts, t->box + 6, t->sort_scratch, 1032u + 256u * tiles));
+    HIPCHK(c, oi_launch_mesh_box(s, nullptr, nullptr, &d, d.nparts, t->box + 6, t->sort_scratch,
+                                 OI_SORT_HEAD + 256u * tiles));
     const uint32_t mblocks = (F + oi_morton_chunk() - 1) / oi_morton_chunk();
-    HIPCHK(c, oi_launch_morton(s, nullptr, nullptr, &d, mblocks, t->box + 6, t->keys, ghist, nullptr));
-    HIPCHK(c, oi_launch_sort_pass(s, t->keys, nullptr, t->keys_alt, t->vals_tmp, F, 0, ghist, lb0, lb1, tctr + 0,
-                                  nullptr));
-    HIPCHK(c, oi_launch_sort_pass(s, t->keys_alt, t->vals_tmp, t->keys, t->perm, F, 8, ghist + 256, lb1, lb0,
-                                  tctr + 1, nullptr));
-    HIPCHK(c, oi_launch_sort_pass(s, t->keys, t->perm, t->keys_alt, t->vals_tmp, F, 16, ghist + 512, lb0, lb1,
-                                  tctr + 2, nullptr));
-    HIPCHK(c, oi_launch_sort_pass(s, t->keys_alt, t->vals_tmp, nullptr, t->perm, F, 24, ghist + 768, lb1, nullptr,
-                                  tctr + 3, nullptr));
+    HIPCHK(c, oi_launch_morton(s, nullptr, nullptr, &d, mblocks, t->box + 6, t->keys, m63 ? t->keys_hi : nullptr,
+                               ghist, nullptr));
+    // LSD passes over the low word (keys move with the values), then, for
+    // 63-bit keys, over the high word looked up by face id; values end in perm
+    const int np = m63 ? 8 : 4;
+    for (int p = 0; p < np; p++) {
+        const bool lo = p < 4;
+        const uint32_t* kin = lo ? ((p & 1) ? t->keys_alt : t->keys) : nullptr;
+        uint32_t* kout = (lo && p < 3) ? ((p & 1) ? t->keys : t->keys_alt) : nullptr;
+        const uint32_t* vin = p == 0 ? nullptr : ((p & 1) ? t->vals_tmp : t->perm);
+        uint32_t* vout = (p & 1) ? t->perm : t->vals_tmp;
+        HIPCHK(c, oi_launch_sort_pass(s, kin, vin, kout, vout, F, 8u * (p & 3), ghist + 256 * p, lb[p & 1],
+                                      p + 1 < np ? lb[(p + 1) & 1] : nullptr, tctr + p, lo ? nullptr : t->keys_hi));
+    }
     oibvh_status st = launch_reduce(t, true);
     if (st != OIBVH_OK) return st;
     t->built = true;
+    return OIBVH_OK;
+}
+
+// key width of later builds: 30 (oibvh.cu:6-22) or 63 (21 bits per axis, no
+// reference counterpart)
+extern "C" oibvh_status oibvh_tree_set_morton_bits(oibvh_tree* t, uint32_t bits)
+{
+    if (!t) return OIBVH_ERR_INVALID;
+    if (bits != 30 && bits != 63) return oi_set_err(t->ctx, OIBVH_ERR_INVALID, "morton bits must be 30 or 63");
+    t->morton_bits = bits;
     return OIBVH_OK;
 }
 

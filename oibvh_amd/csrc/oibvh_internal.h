@@ -19,7 +19,13 @@ hipError_t oi_launch_mesh_box(hipStream_t, const oi_build_desc*, const uint2*, c
                               uint32_t*, uint32_t);
 uint32_t oi_morton_chunk();
 hipError_t oi_launch_morton(hipStream_t, const oi_build_desc*, const uint2*, const oi_build_desc*, uint32_t,
-                            const float*, uint32_t*, uint32_t*, uint32_t*);
+                            const float*, uint32_t*, uint32_t*, uint32_t*, uint32_t*);
+
+// sort scratch: digit histograms of up to 8 key bytes | per-pass tile
+// counters | two look-back buffers of 256 words per tile
+#define OI_SORT_HIST 2048u
+#define OI_SORT_CTRS 16u
+#define OI_SORT_HEAD (OI_SORT_HIST + OI_SORT_CTRS)
 uint32_t oi_sort_tile();
 hipError_t oi_launch_sort_pass(hipStream_t, const uint32_t*, const uint32_t*, uint32_t*, uint32_t*, uint32_t,
                                uint32_t, const uint32_t*, uint32_t*, uint32_t*, uint32_t*, const uint32_t*);

@@ -61,6 +61,10 @@ def lib():
             ("orc_xform", None, [f32p, f32p, u32, f32p]),
             ("orc_quantise10", u32, [C.c_float]),
             ("orc_morton30", u32, [C.c_float, C.c_float, C.c_float]),
+            ("orc_quantise21", u32, [C.c_float]),
+            ("orc_morton63", C.c_uint64, [C.c_float, C.c_float, C.c_float]),
+            ("orc_leaf_keys63", None, [u32p, f32p, u32, f32p, C.POINTER(C.c_uint64)]),
+            ("orc_stable_order64", None, [C.POINTER(C.c_uint64), u32, u32p]),
             ("orc_leaf_keys", None, [u32p, f32p, u32, f32p, u32p, f32p]),
             ("orc_stable_order", None, [u32p, u32, u32p]),
             ("orc_tree_build", vp, [u32p, u32, f32p, u32, u32p]),
@@ -132,6 +136,32 @@ def real_to_implicit(r: int, L: int, vl: int) -> int:
     return lib().orc_real_to_implicit(r, L, vl)
 
 
+def quantise21(p: float) -> int:
+    return lib().orc_quantise21(p)
+
+
+def morton63(x: float, y: float, z: float) -> int:
+    return lib().orc_morton63(x, y, z)
+
+
+def leaf_keys63(faces, xyz, box=None) -> np.ndarray:
+    """63-bit Morton keys of the leaf-box centroids (no reference counterpart)."""
+    f = _u32(faces).reshape(-1, 3)
+    x = _f32(xyz).reshape(-1, 3)
+    b = _f32(mesh_box(x) if box is None else box)
+    out = np.empty(f.shape[0], np.uint64)
+    lib().orc_leaf_keys63(_p(f, C.c_uint32), _p(x, C.c_float), f.shape[0], _p(b, C.c_float),
+                          _p(out, C.c_uint64))
+    return out
+
+
+def stable_order64(keys) -> np.ndarray:
+    k = np.ascontiguousarray(keys, np.uint64)
+    out = np.empty(k.size, np.uint32)
+    lib().orc_stable_order64(_p(k, C.c_uint64), k.size, _p(out, C.c_uint32))
+    return out
+
+
 def xform(mat4, xyz) -> np.ndarray:
     """Mesh::transform's vertex map: mat4 @ (x, y, z, 1) in glm's evaluation
     order, FP32 without contraction. mat4 is a 4x4 matrix (row, column)."""
@@ -197,6 +227,8 @@ class Tree:
         F = self.faces.shape[0]
         if isinstance(order, str) and order == "morton":
             order = stable_order(leaf_keys(self.faces, self.xyz, box))
+        elif isinstance(order, str) and order == "morton63":
+            order = stable_order64(leaf_keys63(self.faces, self.xyz, box))
         self.order = None if order is None else _u32(order)
         h = lib().orc_tree_build(_p(self.faces, C.c_uint32), F, _p(self.xyz, C.c_float),
                                  self.xyz.shape[0],

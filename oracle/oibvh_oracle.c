@@ -211,6 +211,71 @@ EXPORT void orc_leaf_keys(const uint32_t* faces, const float* xyz, uint32_t F, c
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* Morton63: no reference counterpart. The 64-bit form of morton3D with 21    */
+/* bits per axis, quantised like quantise10:                                  */
+/* min(max(p * 2^21, 0), 2^21 - 1) then (unsigned), NaN -> 0; code =         */
+/* x << 2 | y << 1 | z over the bit-spread axes. Changes the leaf order (and  */
+/* so the tree), never the pair set.                                          */
+/* ------------------------------------------------------------------------ */
+static inline uint64_t spread21(uint64_t x)
+{
+    x &= 0x1fffffull;
+    x = (x | (x << 32)) & 0x1f00000000ffffull;
+    x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+EXPORT uint32_t orc_quantise21(float p)
+{
+    float q = p * 2097152.0f;
+    q = (q < 0.0f) ? 0.0f : q;
+    q = (2097151.0f < q) ? 2097151.0f : q;
+    if (q != q) return 0u;
+    return (uint32_t)q;
+}
+
+EXPORT uint64_t orc_morton63(float x, float y, float z)
+{
+    return (spread21(orc_quantise21(x)) << 2) | (spread21(orc_quantise21(y)) << 1) | spread21(orc_quantise21(z));
+}
+
+/* as orc_leaf_keys, 63-bit keys */
+EXPORT void orc_leaf_keys63(const uint32_t* faces, const float* xyz, uint32_t F, const float* box, uint64_t* keys)
+{
+    for (uint32_t i = 0; i < F; i++) {
+        float b[6], p[3];
+        leaf_box(faces + 3u * i, xyz, b);
+        for (int k = 0; k < 3; k++) {
+            float c = (b[k] + b[3 + k]) * 0.5f;
+            float off = c - box[k];
+            float len = box[3 + k] - box[k];
+            p[k] = off / len;
+        }
+        keys[i] = orc_morton63(p[0], p[1], p[2]);
+    }
+}
+
+/* stable order of 64-bit keys (the same comparison sort over (key, index)) */
+static const uint64_t* g_sort_keys64;
+static int cmp_key64_idx(const void* a, const void* b)
+{
+    uint32_t ia = *(const uint32_t*)a, ib = *(const uint32_t*)b;
+    uint64_t ka = g_sort_keys64[ia], kb = g_sort_keys64[ib];
+    if (ka != kb) return ka < kb ? -1 : 1;
+    return ia < ib ? -1 : (ia > ib);
+}
+EXPORT void orc_stable_order64(const uint64_t* keys, uint32_t F, uint32_t* perm)
+{
+    for (uint32_t i = 0; i < F; i++) perm[i] = i;
+    g_sort_keys64 = keys;
+    qsort(perm, F, sizeof(uint32_t), cmp_key64_idx);
+    g_sort_keys64 = NULL;
+}
+
 /* stable sort of keys -> permutation (sorted position -> original face),
  * thrust::stable_sort_by_key semantics (oibvhTree.cu:287-297): a
  * comparison sort over (key, original index), which is the stable order. */

@@ -142,6 +142,32 @@ def test_batch_many_meshes_two_mesh_passes(api, ctx, orc):
     b.close()
 
 
+def test_batch_morton63(api, ctx, orc):
+    """63-bit keys in a batch: every tree equals the oracle's over the 64-bit
+    stable order; the pair sets equal the 30-bit batch's."""
+    rs = small_rigid(n_scenes=5, seed=0x6363)
+    mats = rs.transforms(4)
+    out = []
+    for bits_ in (63, 30):
+        say(f"batch {bits_}-bit Morton: {len(rs.instances)} meshes")
+        b = make_batch(api, ctx, rs)
+        b.set_morton_bits(bits_)
+        b.set_transforms(mats)
+        b.build()
+        order = "morton63" if bits_ == 63 else "morton"
+        for i, (s, lib) in enumerate(rs.instances):
+            xyz, f = rs.library[lib]
+            o = orc.Tree(orc.xform(mats[i], xyz), f, order=order)
+            assert np.array_equal(b.perm(i), o.order), f"mesh {i}: leaf permutation differs"
+            assert np.array_equal(bits(b.nodes(i)), bits(o.nodes())), f"mesh {i}: nodes differ"
+        b.detect(1, 3)
+        out.append(b.pairs())
+        b.close()
+    assert len(out[0]) > 0
+    assert np.array_equal(out[0], out[1])
+    assert np.array_equal(out[0], all_pairs(orc, rs, mats))
+
+
 def test_batch_self_collision(api, ctx, orc):
     say("batch: cloth self collision")
     cl = [M.cloth(24 + 6 * i, 0.2, 0.25, seed=40 + i) for i in range(3)]

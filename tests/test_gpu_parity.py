@@ -103,6 +103,69 @@ def test_build_is_deterministic(api, ctx):
         assert np.array_equal(bits(t.nodes()), bits(a)) and np.array_equal(t.perm(), p)
 
 
+def gpu_tree63(api, ctx, xyz, f):
+    t = api.OibvhTree(ctx, xyz, f)
+    t.set_morton_bits(63)
+    t.build()
+    return t
+
+
+def cell_cluster(F, seed):
+    """Two unreferenced corner vertices fix the Morton box at [-1, 1]^3; every
+    centroid lies in [0.5, 0.5 + 2^-12]^3, one 30-bit cell (all 30-bit keys
+    equal) but about 2^8 cells per axis at 21 bits."""
+    r = M.Rng(seed)
+    V = F + 2
+    xyz = (0.5 + r.uniform(V * 3, 0.0, 1.0 / 4096)).reshape(V, 3).astype(np.float32)
+    xyz[0] = -1.0
+    xyz[1] = 1.0
+    f = r.integers(F * 3, 2, V - 1).reshape(F, 3).astype(np.uint32)
+    return xyz, f
+
+
+@pytest.mark.parametrize("F", [1, 2, 5, 257, 4097, 65537, 300001])
+def test_build_morton63_soup_bit_exact(api, ctx, orc, F):
+    say(f"build 63-bit soup F={F}")
+    xyz, f = soup(F, 2000 + F)
+    check_tree(orc, gpu_tree63(api, ctx, xyz, f), xyz, f, order="morton63")
+
+
+def test_build_morton63_splits_a_30bit_cell(api, ctx, orc):
+    say("build 63-bit cell cluster")
+    xyz, f = cell_cluster(20000, 5)
+    t30 = gpu_tree(api, ctx, xyz, f)
+    assert np.array_equal(t30.perm(), np.arange(len(f), dtype=np.uint32))  # one key: stable = face order
+    t63 = gpu_tree63(api, ctx, xyz, f)
+    check_tree(orc, t63, xyz, f, order="morton63")
+    assert not np.array_equal(t63.perm(), t30.perm())
+    # back to 30 bits on the same tree
+    t63.set_morton_bits(30)
+    t63.build()
+    check_tree(orc, t63, xyz, f)
+
+
+def test_morton63_flat_and_detect(api, ctx, orc):
+    """63-bit trees: NaN policy on a flat axis, refit, and a detect whose pair
+    set equals the 30-bit one."""
+    say("63-bit plane + icospheres detect")
+    xyz, f = M.plane(40)
+    check_tree(orc, gpu_tree63(api, ctx, xyz, f), xyz, f, order="morton63")
+    meshes = M.interpenetrating_icospheres(subdiv=5)
+    trees = [gpu_tree63(api, ctx, x, fa) for x, fa in meshes]
+    for t, (x, fa) in zip(trees, meshes):
+        check_tree(orc, t, x, fa, order="morton63")
+    sc = api.Scene(ctx)
+    for t in trees:
+        sc.addOibvhTree(t)
+    sc.detectCollision(2, 3)
+    want = orc.detect([orc.Tree(x, fa) for x, fa in meshes])
+    assert len(want) > 0
+    assert np.array_equal(sc.getIntTriPairs(), want)
+    x1 = (meshes[1][0] * np.float32(1.01)).astype(np.float32)
+    trees[1].refit(x1)
+    assert np.array_equal(bits(trees[1].nodes()), bits(orc.Tree(x1, meshes[1][1], order=trees[1].perm()).nodes()))
+
+
 # --------------------------------------------------------------------- refit
 def test_refit_after_deformation(api, ctx, orc):
     say("refit deforming uv sphere")

@@ -100,6 +100,48 @@ def test_morton_bit_interleave(orc):
         assert orc.morton30(*f) == want
 
 
+def soup_small(F, seed):
+    r = M.Rng(seed)
+    V = F + 2
+    xyz = r.uniform(V * 3, -1.0, 1.0).reshape(V, 3).astype(np.float32)
+    # repeated vertices make equal keys (ties)
+    xyz[V // 2:] = xyz[: V - V // 2]
+    f = r.integers(F * 3, 0, V - 1).reshape(F, 3).astype(np.uint32)
+    return xyz, f
+
+
+def _interleave(qx, qy, qz, bits):
+    k = 0
+    for b in range(bits):
+        k |= ((qx >> b) & 1) << (3 * b + 2) | ((qy >> b) & 1) << (3 * b + 1) | ((qz >> b) & 1) << (3 * b)
+    return k
+
+
+def test_morton63_bit_interleave(orc):
+    """63-bit codes against a bit-by-bit interleave of the 21-bit
+    quantisations, and the quantisation's closed form."""
+    assert orc.quantise21(0.0) == 0 and orc.quantise21(-1.0) == 0
+    assert orc.quantise21(1.0) == (1 << 21) - 1 and orc.quantise21(float("inf")) == (1 << 21) - 1
+    assert orc.quantise21(float("nan")) == 0
+    assert orc.quantise21(0.5) == 1 << 20
+    r = M.Rng(0x63)
+    for x, y, z in r.uniform(3 * 300, -0.1, 1.1).reshape(300, 3):
+        qx, qy, qz = (orc.quantise21(float(np.float32(v))) for v in (x, y, z))
+        assert orc.morton63(float(np.float32(x)), float(np.float32(y)), float(np.float32(z))) == \
+            _interleave(qx, qy, qz, 21)
+    assert orc.morton63(1.0, 1.0, 1.0) == (1 << 63) - 1
+
+
+def test_morton63_stable_order(orc):
+    """The 64-bit order is sorted by key and keeps equal keys in face order."""
+    xyz, f = soup_small(500, 17)
+    k63 = orc.leaf_keys63(f, xyz)
+    o = orc.stable_order64(k63)
+    assert np.all(np.diff(k63[o].astype(np.float64)) >= 0)
+    ties = np.nonzero(k63[o][1:] == k63[o][:-1])[0]
+    assert np.all(o[ties] < o[ties + 1])  # stable
+
+
 def test_leaf_keys_and_stable_order(orc):
     xyz, f = M.icosphere(3, 1.0, 0.02, 5)
     box = orc.mesh_box(xyz)
