@@ -178,6 +178,17 @@ OIBVH_API oibvh_status oibvh_scene_get_pair_count(oibvh_scene* scene, uint32_t* 
  * *n receives the pair count (OIBVH_ERR_CAPACITY if it exceeds cap). */
 OIBVH_API oibvh_status oibvh_scene_get_pairs(oibvh_scene* scene, uint32_t* out, uint64_t cap, uint32_t* n);
 
+/* Scene::convertToVertexArray (scene.cu:69-93; Scene::draw uploads the
+ * result to a GL buffer, scene.cu:55-66): the triangles of every colliding
+ * pair as 18 floats, the three xyz vertices of triangle A then the three of
+ * triangle B, in oibvh_scene_get_pairs order. Vertices are the trees' current
+ * positions through their model matrices (as a detect sees them), gathered on
+ * the device. out: host memory (ptr_kind OIBVH_PTR_HOST) or device memory
+ * (OIBVH_PTR_DEVICE, e.g. a mapped vertex buffer); cap in floats; *n receives
+ * the pair count (OIBVH_ERR_CAPACITY if 18 * n exceeds cap). Synchronises. */
+OIBVH_API oibvh_status oibvh_scene_get_pair_vertices(oibvh_scene* scene, float* out, uint64_t cap,
+                                                     uint32_t ptr_kind, uint32_t* n);
+
 /* Device pointer of the unsorted hit list (uint4 per pair, as above) and its
  * length, valid until the next detect. Synchronises. */
 OIBVH_API oibvh_status oibvh_scene_hits_device(oibvh_scene* scene, void** dptr, uint32_t* n);
@@ -294,12 +305,30 @@ OIBVH_API oibvh_status oibvh_batch_get_pair_count(oibvh_batch* batch, uint32_t* 
 /* Sorted (meshA, meshB, origTriA, origTriB) with batch mesh indices,
  * meshA <= meshB, 4 words each; cap in pairs. */
 OIBVH_API oibvh_status oibvh_batch_get_pairs(oibvh_batch* batch, uint32_t* out, uint64_t cap, uint32_t* n);
+/* as oibvh_scene_get_pair_vertices, over the batch's meshes */
+OIBVH_API oibvh_status oibvh_batch_get_pair_vertices(oibvh_batch* batch, float* out, uint64_t cap,
+                                                     uint32_t ptr_kind, uint32_t* n);
 OIBVH_API oibvh_status oibvh_batch_get_stats(oibvh_batch* batch, oibvh_detect_stats* stats);
 OIBVH_API oibvh_status oibvh_batch_get_level_counts(oibvh_batch* batch, uint32_t* out, uint64_t cap, uint32_t* n);
 /* One mesh's tree nodes (as oibvh_tree_get_nodes) and leaf permutation. */
 OIBVH_API oibvh_status oibvh_batch_get_nodes(oibvh_batch* batch, uint32_t mesh, float* out, uint64_t cap,
                                              uint32_t* n);
 OIBVH_API oibvh_status oibvh_batch_get_perm(oibvh_batch* batch, uint32_t mesh, uint32_t* out, uint64_t cap);
+
+/* ---------------------------------------------------------------- mesh files */
+
+/* Wavefront OBJ positions and triangles, the mesh ingest the reference gets
+ * from assimp (Model::loadModel, model.cpp:40-186, with aiProcess_Triangulate).
+ * 'v' lines give positions (file order), 'f' lines polygons (corners i, i/t,
+ * i//n, i/t/n; 1-based or negative indices), fan-triangulated as (c1, cj,
+ * cj+1); every other statement is skipped and all groups form one mesh.
+ * Host only, no device work. xyz: 3 floats per position (xyz_cap floats),
+ * idx: 3 indices per triangle (idx_cap words). *V and *F always receive the
+ * counts; with null or short buffers the call returns OIBVH_ERR_CAPACITY
+ * (call once to size, once to read). ctx may be NULL; with a context, parse
+ * errors (OIBVH_ERR_INVALID) leave their file:line text in oibvh_last_error. */
+OIBVH_API oibvh_status oibvh_obj_read(oibvh_ctx* ctx, const char* path, float* xyz, uint64_t xyz_cap,
+                                      uint32_t* idx, uint64_t idx_cap, uint32_t* V, uint32_t* F);
 
 #ifdef __cplusplus
 }

@@ -42,6 +42,22 @@ struct Mesh {
     uint32_t faceCount() const { return (uint32_t)(indices.size() / 3); }
 };
 
+// Model(path) (model.cpp:40-186) for the collision path: positions and
+// triangles of a Wavefront OBJ file (oibvh_obj_read)
+inline std::shared_ptr<Mesh> loadObj(const std::string& path)
+{
+    uint32_t V = 0, F = 0;
+    oibvh_status s = oibvh_obj_read(nullptr, path.c_str(), nullptr, 0, nullptr, 0, &V, &F);
+    if (s != OIBVH_OK && s != OIBVH_ERR_CAPACITY) throw Error(s, "loadObj: cannot read " + path);
+    auto m = std::make_shared<Mesh>();
+    m->positions.resize(3ull * V);
+    m->indices.resize(3ull * F);
+    s = oibvh_obj_read(nullptr, path.c_str(), m->positions.data(), m->positions.size(), m->indices.data(),
+                       m->indices.size(), &V, &F);
+    if (s != OIBVH_OK) throw Error(s, "loadObj: cannot read " + path);
+    return m;
+}
+
 // int_tri_pair_node_t (utils.h:57-61) with original face ids
 struct IntTriPair {
     uint32_t bvhIndex[2];
@@ -218,6 +234,20 @@ public:
             ctx_->check(oibvh_scene_get_pairs(h_, reinterpret_cast<uint32_t*>(out.data()), n, &got),
                         "getIntTriPairs");
         out.resize(got);
+        return out;
+    }
+
+    // Scene::convertToVertexArray (scene.cu:69-93): 6 vertices (xyz) per
+    // pair, triangle A's then triangle B's, in getIntTriPairs order
+    std::vector<float> convertToVertexArray() const
+    {
+        const uint32_t n = getIntTriPairCount();
+        std::vector<float> out(18ull * n);
+        uint32_t got = 0;
+        if (n)
+            ctx_->check(oibvh_scene_get_pair_vertices(h_, out.data(), out.size(), OIBVH_PTR_HOST, &got),
+                        "convertToVertexArray");
+        out.resize(18ull * got);
         return out;
     }
 

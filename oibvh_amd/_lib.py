@@ -14,6 +14,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboibvh_hip.so")
 
 OK = 0
+ERR_INVALID = 1
+ERR_CAPACITY = 4
 STATUS = {0: "OK", 1: "INVALID", 2: "NOT_BUILT", 3: "HIP", 4: "CAPACITY", 5: "OVERFLOW", 6: "TIMEOUT",
           7: "NO_DEVICE"}
 
@@ -74,6 +76,7 @@ API = [
     ("oibvh_scene_get_pair_count", _i, [_vp, _u32p]),
     ("oibvh_scene_get_pairs", _i, [_vp, _u32p, _u64, _u32p]),
     ("oibvh_scene_hits_device", _i, [_vp, C.POINTER(_vp), _u32p]),
+    ("oibvh_scene_get_pair_vertices", _i, [_vp, _vp, _u64, _u32, _u32p]),
     ("oibvh_scene_get_stats", _i, [_vp, C.POINTER(DetectStats)]),
     ("oibvh_scene_get_level_counts", _i, [_vp, _u32p, _u64, _u32p]),
     ("oibvh_scene_refit", _i, [_vp]),
@@ -92,6 +95,8 @@ API = [
     ("oibvh_batch_frame", _i, [_vp, _u32, _u32, _u32]),
     ("oibvh_batch_get_pair_count", _i, [_vp, _u32p]),
     ("oibvh_batch_get_pairs", _i, [_vp, _u32p, _u64, _u32p]),
+    ("oibvh_batch_get_pair_vertices", _i, [_vp, _vp, _u64, _u32, _u32p]),
+    ("oibvh_obj_read", _i, [_vp, C.c_char_p, _f32p, _u64, _u32p, _u64, _u32p, _u32p]),
     ("oibvh_batch_get_stats", _i, [_vp, C.POINTER(DetectStats)]),
     ("oibvh_batch_get_level_counts", _i, [_vp, _u32p, _u64, _u32p]),
     ("oibvh_batch_get_nodes", _i, [_vp, _u32, _f32p, _u64, _u32p]),

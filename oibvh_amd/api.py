@@ -7,6 +7,7 @@ Mesh data are numpy arrays: xyz float32 (V, 3), faces uint32 (F, 3).
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -89,6 +90,13 @@ class DeviceBuffer:
 
     def at(self, offset: int) -> int:
         return self.ptr.value + offset
+
+    def download(self, dtype, count: int, offset: int = 0) -> np.ndarray:
+        out = np.empty(count, dtype)
+        if count:
+            L.hip_check(L.hip().hipMemcpy(_ptr(out), C.c_void_p(self.ptr.value + offset), out.nbytes, 2),
+                        "hipMemcpy D2H")
+        return out
 
     def free(self) -> None:
         if getattr(self, "ptr", None) and self.ptr.value:
@@ -220,6 +228,31 @@ class OibvhTree:
         return p.value
 
 
+
+
+def load_obj(path: str, ctx: "Context | None" = None):
+    """Model(path) (model.cpp:40-186) reduced to the collision path: (xyz
+    (V, 3) float32, faces (F, 3) uint32) of a Wavefront OBJ file, read by the
+    library's native reader (oibvh_obj_read). Host only."""
+    lib = L.load()
+    h = ctx.h if ctx is not None else None
+    V, F = C.c_uint32(), C.c_uint32()
+    bpath = os.fsencode(path)
+    st = lib.oibvh_obj_read(h, bpath, None, 0, None, 0, C.byref(V), C.byref(F))
+    if st not in (L.OK, L.ERR_CAPACITY):
+        _check(h, st, f"load_obj({path})")
+    xyz = np.empty((V.value, 3), np.float32)
+    faces = np.empty((F.value, 3), np.uint32)
+    _check(h, lib.oibvh_obj_read(h, bpath, xyz.ctypes.data_as(L._f32p), xyz.size, faces.ctypes.data_as(L._u32p),
+                                 faces.size, C.byref(V), C.byref(F)), f"load_obj({path})")
+    return xyz, faces
+
+def _pair_vertices(obj, fn, n: int) -> np.ndarray:
+    out = np.empty((max(n, 1), 6, 3), np.float32)
+    got = C.c_uint32()
+    _check(obj.ctx.h, fn(obj.h, out.ctypes.data, out.size, L.PTR_HOST, C.byref(got)), "pair_vertices")
+    return out[: got.value]
+
 class Scene:
     """Scene (scene.cuh:26-182)."""
 
@@ -266,6 +299,18 @@ class Scene:
         _check(self.ctx.h, self._lib.oibvh_scene_get_pairs(self.h, out.ctypes.data_as(L._u32p), max(n, 1),
                                                            C.byref(got)), "get_pairs")
         return out[: got.value]
+
+    def pair_vertices(self) -> np.ndarray:
+        """Scene::convertToVertexArray: (n, 6, 3) float32, triangle A's three
+        vertices then triangle B's, in getIntTriPairs order."""
+        return _pair_vertices(self, self._lib.oibvh_scene_get_pair_vertices, self.getIntTriPairCount())
+
+    def pair_vertices_device(self, dptr: int, cap_floats: int) -> int:
+        """The same into device memory (18 floats per pair); returns n."""
+        got = C.c_uint32()
+        _check(self.ctx.h, self._lib.oibvh_scene_get_pair_vertices(self.h, dptr, cap_floats, L.PTR_DEVICE,
+                                                                   C.byref(got)), "pair_vertices")
+        return got.value
 
     def level_counts(self) -> np.ndarray:
         """BVTT front length per level of the last detect."""
@@ -389,6 +434,10 @@ class Batch:
         _check(self.ctx.h, self._lib.oibvh_batch_get_pairs(self.h, out.ctypes.data_as(L._u32p), max(n, 1),
                                                            C.byref(got)), "batch pairs")
         return out[: got.value]
+
+    def pair_vertices(self) -> np.ndarray:
+        """(n, 6, 3) float32 triangle vertices of every pair, pairs() order."""
+        return _pair_vertices(self, self._lib.oibvh_batch_get_pair_vertices, self.pair_count())
 
     def stats(self) -> dict:
         s = L.DetectStats()

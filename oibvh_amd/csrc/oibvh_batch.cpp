@@ -684,6 +684,15 @@ extern "C" oibvh_status oibvh_batch_get_pairs(oibvh_batch* b, uint32_t* out, uin
     return oi_det_get_pairs(&b->det, out, cap, n);
 }
 
+// Scene::convertToVertexArray (scene.cu:69-93) over the batch's meshes
+extern "C" oibvh_status oibvh_batch_get_pair_vertices(oibvh_batch* b, float* out, uint64_t cap, uint32_t ptr_kind,
+                                                      uint32_t* n)
+{
+    if (!b) return OIBVH_ERR_INVALID;
+    HIPCHK(b->ctx, hipSetDevice(b->ctx->device));
+    return oi_det_pair_vertices(&b->det, b->d_bdesc, out, cap, ptr_kind, n);
+}
+
 extern "C" oibvh_status oibvh_batch_get_stats(oibvh_batch* b, oibvh_detect_stats* st)
 {
     if (!b || !st) return OIBVH_ERR_INVALID;

@@ -903,6 +903,36 @@ hipError_t oi_launch_bvtt_step(hipStream_t st, uint32_t grid, const oi_tab_args*
                        expand, implicit_end, fused);
 }
 
+// Scene::convertToVertexArray (scene.cu:69-93): six vertices per hit, the
+// three of triangle A then the three of triangle B, each through its mesh's
+// model matrix as the detect saw it. One thread per output vertex.
+__global__ void __launch_bounds__(256)
+k_pair_vertices(const oi_build_desc* __restrict__ g, const uint4* __restrict__ pairs, uint32_t n,
+                float* __restrict__ out)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (i >= 6ull * n) return;
+    const uint32_t p = (uint32_t)(i / 6u), r = (uint32_t)(i % 6u);
+    const uint4 h = pairs[p];
+    const bool b = r >= 3;
+    const oi_build_desc& d = g[b ? h.y : h.x];
+    const uint32_t tri = b ? h.w : h.z;
+    const uint32_t v = oi_g(d.faces)[3ull * tri + (r % 3u)];
+    const v3f q = vertex(d.pos, d.xf, v);
+    OI_GLOBAL float* o = oi_gw(out) + 3ull * i;
+    o[0] = q.x;
+    o[1] = q.y;
+    o[2] = q.z;
+}
+
+hipError_t oi_launch_pair_vertices(hipStream_t st, const oi_build_desc* g, const uint4* pairs, uint32_t n, float* out)
+{
+    if (n == 0) return hipSuccess;
+    const uint64_t blocks = (6ull * n + 255u) / 256u;
+    hipLaunchKernelGGL(k_pair_vertices, dim3((uint32_t)blocks), dim3(256), 0, st, g, pairs, n, out);
+    return hipGetLastError();
+}
+
 hipError_t oi_launch_narrow(hipStream_t st, uint32_t grid, const oi_tab_args* t, const uint2* cands,
                             const uint32_t* cand_count, uint32_t cand_cap, uint4* hits, uint32_t* n_hits,
                             uint32_t hits_cap, oi_detect_ctl* ctl, int fp64)

@@ -39,6 +39,8 @@ void oi_det_free(oi_detector* d)
     dfree(d->counts);
     dfree(d->cands);
     dfree(d->hits);
+    dfree(d->d_sorted);
+    dfree(d->d_stage);
     dfree(d->ctl);
     if (d->h_ctl) (void)hipHostFree(d->h_ctl);
     d->h_ctl = nullptr;
@@ -104,6 +106,7 @@ oibvh_status oi_det_enqueue(oi_detector* d, bool reset_sticky)
                                    &d->ctl->n_hits, d->hits_cap, d->ctl, 1));
     d->unchecked = true;
     d->have_result = true;
+    d->sorted_ok = false;
     return OIBVH_OK;
 }
 
@@ -182,6 +185,27 @@ oibvh_status oi_det_check(oi_detector* d)
     }
 }
 
+// the checked hits, sorted (fetched once per query)
+static oibvh_status sorted_hits(oi_detector* d)
+{
+    oibvh_ctx* c = d->ctx;
+    if (d->sorted_ok) return OIBVH_OK;
+    std::vector<uint4>& h = d->h_sorted;
+    h.resize(d->n_pairs);
+    if (d->n_pairs) {
+        HIPCHK(c, hipMemcpyAsync(h.data(), d->hits, d->n_pairs * sizeof(uint4), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    std::sort(h.begin(), h.end(), [](const uint4& a, const uint4& b) {
+        if (a.x != b.x) return a.x < b.x;
+        if (a.y != b.y) return a.y < b.y;
+        if (a.z != b.z) return a.z < b.z;
+        return a.w < b.w;
+    });
+    d->sorted_ok = true;
+    return OIBVH_OK;
+}
+
 oibvh_status oi_det_get_pairs(oi_detector* d, uint32_t* out, uint64_t cap, uint32_t* n)
 {
     oibvh_ctx* c = d->ctx;
@@ -190,21 +214,51 @@ oibvh_status oi_det_get_pairs(oi_detector* d, uint32_t* out, uint64_t cap, uint3
     if (n) *n = d->n_pairs;
     if (d->n_pairs == 0) return OIBVH_OK;
     if (!out || cap < d->n_pairs) return oi_set_err(c, OIBVH_ERR_CAPACITY, "need %u pairs", d->n_pairs);
-    std::vector<uint4> h(d->n_pairs);
-    HIPCHK(c, hipMemcpyAsync(h.data(), d->hits, d->n_pairs * sizeof(uint4), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    std::sort(h.begin(), h.end(), [](const uint4& a, const uint4& b) {
-        if (a.x != b.x) return a.x < b.x;
-        if (a.y != b.y) return a.y < b.y;
-        if (a.z != b.z) return a.z < b.z;
-        return a.w < b.w;
-    });
+    if ((r = sorted_hits(d)) != OIBVH_OK) return r;
+    const std::vector<uint4>& h = d->h_sorted;
     for (uint32_t i = 0; i < d->n_pairs; i++) {
         out[4ull * i] = h[i].x;
         out[4ull * i + 1] = h[i].y;
         out[4ull * i + 2] = h[i].z;
         out[4ull * i + 3] = h[i].w;
     }
+    return OIBVH_OK;
+}
+
+oibvh_status oi_det_pair_vertices(oi_detector* d, const oi_build_desc* geom, float* out, uint64_t cap,
+                                  uint32_t ptr_kind, uint32_t* n)
+{
+    oibvh_ctx* c = d->ctx;
+    if (ptr_kind != OIBVH_PTR_HOST && ptr_kind != OIBVH_PTR_DEVICE)
+        return oi_set_err(c, OIBVH_ERR_INVALID, "pair vertices: ptr_kind must be HOST or DEVICE");
+    oibvh_status r = oi_det_check(d);
+    if (r != OIBVH_OK) return r;
+    if (n) *n = d->n_pairs;
+    if (d->n_pairs == 0) return OIBVH_OK;
+    const uint64_t need = 18ull * d->n_pairs;
+    if (!out || cap < need) return oi_set_err(c, OIBVH_ERR_CAPACITY, "need %llu floats", (unsigned long long)need);
+    if ((r = sorted_hits(d)) != OIBVH_OK) return r;
+    uint64_t gen = 0;
+    if ((r = grow(c, d->d_sorted, d->d_sorted_cap, d->n_pairs, gen)) != OIBVH_OK) return r;
+    // the host sort is complete; the copy only has to reach the device before the kernel
+    HIPCHK(c, hipMemcpyAsync(d->d_sorted, d->h_sorted.data(), d->n_pairs * sizeof(uint4), hipMemcpyHostToDevice,
+                             c->stream));
+    float* dst = out;
+    if (ptr_kind == OIBVH_PTR_HOST) {
+        if (d->stage_cap < need) {
+            dfree(d->d_stage);
+            d->stage_cap = 0;
+            HIPCHK(c, dalloc(&d->d_stage, need));
+            d->stage_cap = need;
+        }
+        dst = d->d_stage;
+    }
+    HIPCHK(c, oi_launch_pair_vertices(c->stream, geom, d->d_sorted, d->n_pairs, dst));
+    if (ptr_kind == OIBVH_PTR_HOST) {
+        HIPCHK(c, hipMemcpyAsync(out, d->d_stage, need * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    }
+    // the host copy of the hits must outlive the upload; host output must land
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return OIBVH_OK;
 }
 

@@ -68,6 +68,8 @@ struct oibvh_scene {
     oi_mesh_desc* d_desc = nullptr;
     uint32_t desc_cap = 0;
     std::vector<oi_mesh_desc> h_desc;  // host copies stay alive until the next detect
+    oi_build_desc* d_geom = nullptr;   // per-tree geometry for the vertex export
+    uint32_t geom_cap = 0;
     std::vector<uint2> h_front;
     uint32_t front_key[3] = {~0u, ~0u, ~0u}; // entry, flags&SELF, #trees
     oi_detector det;
@@ -552,6 +554,7 @@ extern "C" void oibvh_scene_destroy(oibvh_scene* s)
     (void)hipSetDevice(s->ctx->device);
     (void)hipStreamSynchronize(s->ctx->stream);
     dfree(s->d_desc);
+    dfree(s->d_geom);
     oi_det_free(&s->det);
     oibvh_ctx* c = s->ctx;
     delete s;
@@ -681,6 +684,7 @@ extern "C" oibvh_status oibvh_scene_detect(oibvh_scene* s, uint32_t entry, uint3
     if (s->trees.empty() || (s->trees.size() == 1 && !(flags & OIBVH_DETECT_SELF))) {
         d.have_result = true;
         d.unchecked = false;
+        d.sorted_ok = false;
         d.n_levels = 0;
         return OIBVH_OK;
     }
@@ -761,6 +765,30 @@ extern "C" oibvh_status oibvh_scene_get_pairs(oibvh_scene* s, uint32_t* out, uin
 {
     if (!s) return OIBVH_ERR_INVALID;
     return oi_det_get_pairs(&s->det, out, cap, n);
+}
+
+// Scene::convertToVertexArray (scene.cu:69-93), gathered on the device
+extern "C" oibvh_status oibvh_scene_get_pair_vertices(oibvh_scene* s, float* out, uint64_t cap, uint32_t ptr_kind,
+                                                      uint32_t* n)
+{
+    if (!s) return OIBVH_ERR_INVALID;
+    oibvh_ctx* c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    const uint32_t nt = (uint32_t)s->trees.size();
+    if (s->det.have_result && s->det.n_pairs && nt) {
+        std::vector<oi_build_desc> g(nt);
+        for (uint32_t m = 0; m < nt; m++) g[m] = tree_desc(s->trees[m]);
+        if (s->geom_cap < nt) {
+            dfree(s->d_geom);
+            s->geom_cap = 0;
+            HIPCHK(c, dalloc(&s->d_geom, nt));
+            s->geom_cap = nt;
+        }
+        // synchronous: the table is read by the export launched below
+        HIPCHK(c, hipMemcpyAsync(s->d_geom, g.data(), nt * sizeof(oi_build_desc), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return oi_det_pair_vertices(&s->det, s->d_geom, out, cap, ptr_kind, n);
 }
 
 extern "C" oibvh_status oibvh_scene_get_level_counts(oibvh_scene* s, uint32_t* out, uint64_t cap, uint32_t* n)

@@ -46,6 +46,7 @@ hipError_t oi_launch_bvtt_step(hipStream_t, uint32_t, const oi_tab_args*, int, c
                                uint32_t, oi_detect_ctl*, uint32_t, uint32_t, int);
 hipError_t oi_launch_narrow(hipStream_t, uint32_t, const oi_tab_args*, const uint2*, const uint32_t*, uint32_t,
                             uint4*, uint32_t*, uint32_t, oi_detect_ctl*, int);
+hipError_t oi_launch_pair_vertices(hipStream_t, const oi_build_desc*, const uint4*, uint32_t, float*);
 }
 
 // ---------------------------------------------------------------------------
@@ -137,6 +138,12 @@ struct oi_detector {
     bool unchecked = false;
     bool have_result = false;
     uint32_t n_pairs = 0;
+    std::vector<uint4> h_sorted;  // the hits in get_pairs order, once fetched
+    bool sorted_ok = false;
+    uint4* d_sorted = nullptr;    // the same on the device (vertex export)
+    uint32_t d_sorted_cap = 0;
+    float* d_stage = nullptr;     // vertex export staging for host output
+    uint64_t stage_cap = 0;
     oibvh_detect_stats stats{};
     uint64_t generation = 0;  // bumped whenever a device buffer is replaced
 };
@@ -155,3 +162,7 @@ oibvh_status oi_det_check(oi_detector* d);
 // sorted hits (4 words each)
 oibvh_status oi_det_get_pairs(oi_detector* d, uint32_t* out, uint64_t cap, uint32_t* n);
 oibvh_status oi_det_level_counts(oi_detector* d, uint32_t* out, uint64_t cap, uint32_t* n);
+// six transformed vertices (18 floats) per hit in get_pairs order; geom[m] =
+// positions, faces and model matrix of mesh m (device table)
+oibvh_status oi_det_pair_vertices(oi_detector* d, const oi_build_desc* geom, float* out, uint64_t cap,
+                                  uint32_t ptr_kind, uint32_t* n);

@@ -1,12 +1,16 @@
 // facade_test.cpp -- drives include/oibvh.hpp the way the reference demo drives
 // its classes (main.cpp:127-151, 284): build tree 1, copy-construct tree 2
 // over a second mesh with the same topology, refit it, add both to a Scene,
-// detectCollision(GPU0, 4, 3), read the pairs.
+// detectCollision(GPU0, 4, 3), read the pairs and their triangles
+// (convertToVertexArray, scene.cu:69-93); optionally load an OBJ file the way
+// the demo loads its model (main.cpp:127).
 //
-// usage: facade_test <in.bin> <out.bin>
+// usage: facade_test <in.bin> <out.bin> [model.obj]
 //   in:  uint32 V, uint32 F, float xyz1[3V], float xyz2[3V], uint32 faces[3F]
 //   out: uint32 N, float nodes1[6N], float nodes2[6N], uint32 perm[F],
-//        uint32 H, uint32 pairs[4H], uint32 depth, uint32 prim_count
+//        uint32 H, uint32 pairs[4H], uint32 depth, uint32 prim_count,
+//        float vertices[18H], and with model.obj: uint32 V', uint32 F',
+//        float xyz[3V'], uint32 faces[3F']
 #include <cstdio>
 #include <cstdlib>
 #include <memory>
@@ -27,8 +31,8 @@ static void wr(FILE* f, const T* p, size_t n)
 
 int main(int argc, char** argv)
 {
-    if (argc != 3) {
-        fprintf(stderr, "usage: %s in.bin out.bin\n", argv[0]);
+    if (argc != 3 && argc != 4) {
+        fprintf(stderr, "usage: %s in.bin out.bin [model.obj]\n", argv[0]);
         return 2;
     }
     FILE* in = fopen(argv[1], "rb");
@@ -69,6 +73,16 @@ int main(int argc, char** argv)
         uint32_t depth = t1->getDepth(), prims = t1->getPrimCount();
         wr(out, &depth, 1);
         wr(out, &prims, 1);
+        std::vector<float> verts = scene.convertToVertexArray();
+        wr(out, verts.data(), verts.size());
+        if (argc == 4) {
+            std::shared_ptr<oibvh::Mesh> model = oibvh::loadObj(argv[3]);
+            uint32_t mv = model->vertexCount(), mf = model->faceCount();
+            wr(out, &mv, 1);
+            wr(out, &mf, 1);
+            wr(out, model->positions.data(), model->positions.size());
+            wr(out, model->indices.data(), model->indices.size());
+        }
         fclose(out);
         printf("facade ok: %u nodes, %u pairs\n", N, H);
     } catch (const oibvh::Error& e) {

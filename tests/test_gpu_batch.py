@@ -92,6 +92,15 @@ def test_batch_trees_and_pairs(api, ctx, orc):
         got = b.pairs()
         assert len(want) > 0
         assert np.array_equal(got, want), f"entry {entry} expand {expand}"
+    pv = b.pair_vertices()
+    pairs = b.pairs()
+    assert pv.shape == (len(pairs), 6, 3)
+    for i, (ma, mb, ta, tb) in enumerate(pairs[:: max(1, len(pairs) // 200)]):
+        k = i * max(1, len(pairs) // 200)
+        xa, fa = rs.library[rs.instances[ma][1]]
+        xb, fb = rs.library[rs.instances[mb][1]]
+        ya, yb = orc.xform(mats[ma], xa), orc.xform(mats[mb], xb)
+        assert np.array_equal(bits(pv[k, :3]), bits(ya[fa[ta]])) and np.array_equal(bits(pv[k, 3:]), bits(yb[fb[tb]]))
     # refit after motion
     for frame in (3, 9):
         mats = rs.transforms(frame)

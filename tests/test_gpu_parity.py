@@ -166,6 +166,54 @@ def test_morton63_flat_and_detect(api, ctx, orc):
     assert np.array_equal(bits(trees[1].nodes()), bits(orc.Tree(x1, meshes[1][1], order=trees[1].perm()).nodes()))
 
 
+def expected_pair_vertices(pairs, meshes):
+    """Scene::convertToVertexArray (scene.cu:69-93) from the oracle side:
+    meshes[m] = (transformed xyz, faces)."""
+    out = np.empty((len(pairs), 6, 3), np.float32)
+    for i, (a, b, ta, tb) in enumerate(pairs):
+        xa, fa = meshes[a]
+        xb, fb = meshes[b]
+        out[i, :3] = xa[fa[ta]]
+        out[i, 3:] = xb[fb[tb]]
+    return out
+
+
+def test_pair_vertices_export(api, ctx, orc):
+    """The vertex export gathers each pair's triangles through the model
+    matrices, to host or device memory, bit for bit."""
+    say("pair vertex export")
+    (x0, f0), (x1, f1) = M.interpenetrating_icospheres(subdiv=4, offset=0.0)
+    t0, t1 = api.OibvhTree(ctx, x0, f0), api.OibvhTree(ctx, x1, f1)
+    A0, A1 = _motion(5, 0), _motion(6, 0)
+    t0.set_transform(A0)
+    t1.set_transform(A1)
+    t0.build()
+    t1.build()
+    sc = api.Scene(ctx)
+    sc.addOibvhTree(t0)
+    sc.addOibvhTree(t1)
+    sc.detectCollision(1, 2)
+    pairs = sc.getIntTriPairs()
+    assert len(pairs) > 0
+    want = expected_pair_vertices(pairs, [(orc.xform(A0, x0), f0), (orc.xform(A1, x1), f1)])
+    got = sc.pair_vertices()
+    assert got.shape == want.shape and np.array_equal(bits(got), bits(want))
+    buf = api.DeviceBuffer(want.nbytes)
+    assert sc.pair_vertices_device(buf.at(0), want.size) == len(pairs)
+    assert np.array_equal(bits(buf.download(np.float32, want.size)), bits(want.reshape(-1)))
+    with pytest.raises(api.L.OibvhError):
+        sc.pair_vertices_device(buf.at(0), want.size - 1)
+    buf.free()
+    # identity transform, self mode: the export reads the current positions
+    t0.set_transform(None)
+    t0.refit()
+    sc2 = api.Scene(ctx)
+    sc2.addOibvhTree(t0)
+    sc2.detectCollision(0, 3, self_collide=True)
+    p2 = sc2.getIntTriPairs()
+    assert np.array_equal(bits(sc2.pair_vertices()), bits(expected_pair_vertices(p2, [(x0, f0)])))
+
+
 # --------------------------------------------------------------------- refit
 def test_refit_after_deformation(api, ctx, orc):
     say("refit deforming uv sphere")
@@ -462,7 +510,11 @@ def test_cpp_facade_sequence(ctx, orc, tmp_path):
         xyz.tofile(fh)
         x2.tofile(fh)
         f.tofile(fh)
-    r = subprocess.run([exe, str(inp), str(out)], capture_output=True, text=True, timeout=120)
+    mx, mf = M.icosphere(2, 1.0, 0.05, 9)
+    obj = tmp_path / "model.obj"
+    obj.write_text("".join(f"v {a:.9g} {b:.9g} {c:.9g}\n" for a, b, c in mx.astype(np.float64))
+                   + "".join(f"f {a + 1}/1/1 {b + 1}/1/1 {c + 1}/1/1\n" for a, b, c in mf))
+    r = subprocess.run([exe, str(inp), str(out), str(obj)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     raw = np.fromfile(out, np.uint32)
     N = int(raw[0])
@@ -479,12 +531,22 @@ def test_cpp_facade_sequence(ctx, orc, tmp_path):
     pairs = raw[o:o + 4 * H].reshape(H, 4)
     o += 4 * H
     depth, prims = int(raw[o]), int(raw[o + 1])
+    o += 2
+    verts = raw[o:o + 18 * H].view(np.float32).reshape(H, 6, 3)
+    o += 18 * H
+    mv, mfc = int(raw[o]), int(raw[o + 1])
+    o += 2
+    assert (mv, mfc) == (mx.shape[0], mf.shape[0])
+    assert np.array_equal(raw[o:o + 3 * mv], mx.view(np.uint32).reshape(-1))
+    o += 3 * mv
+    assert np.array_equal(raw[o:o + 3 * mfc], mf.reshape(-1))
     t1 = orc.Tree(xyz, f, order="morton")
     t2 = orc.Tree(x2, f, order=t1.order)
     assert np.array_equal(perm, t1.order)
     assert np.array_equal(bits(n1), bits(t1.nodes())) and np.array_equal(bits(n2), bits(t2.nodes()))
     assert np.array_equal(pairs, orc.detect([t1, t2]))
     assert prims == F and depth == int(np.log2(N))
+    assert np.array_equal(bits(verts), bits(expected_pair_vertices(pairs, [(xyz, f), (x2, f)])))
 
 
 def test_smoke_entry():
