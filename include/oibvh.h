@@ -213,13 +213,24 @@ OIBVH_API oibvh_status oibvh_scene_get_stats(oibvh_scene* scene, oibvh_detect_st
  * select the defaults. */
 typedef struct {
     uint32_t reduce_leaves_per_thread; /* 1, 2, 4 or 8: the build/refit reduce gives each workgroup a
-                                          subtree of 256 x this many leaves (0 = by mesh size) */
+                                          subtree of 256 x this many leaves; 16: one subtree of 256
+                                          leaves per wave (4 per lane, interleaved), four per
+                                          workgroup (0 = by mesh size) */
     uint32_t traverse_blocks;          /* workgroups per BVTT-level and narrow-phase launch (0 = 4 per CU) */
     uint32_t traverse_expand;          /* levels each side descends per BVTT step, 1..3 (0 = the
                                           detect call's expandLevels, clamped to 1..3) */
     uint32_t front_capacity;           /* initial BVTT front entries per buffer, a power of two (0 = 2^22) */
     uint32_t hits_capacity;            /* initial hit-list entries (0 = 2^20) */
     uint32_t candidate_capacity;       /* initial candidate leaf-pair entries (0 = 2^20) */
+    uint32_t traverse_schedule;        /* levels descended by the first BVTT steps, 2 bits per step
+                                          (step k: bits 2k+1..2k, values 1..3); the first zero field
+                                          ends the list and later steps use the uniform setting
+                                          (0 = uniform) */
+    uint32_t traverse_local;           /* 0: BVTT steps (one launch per step, each spread over the whole
+                                          GPU); n >= 1: walk the BVTT inside workgroups of n initial
+                                          pairs each, depth first with the pairs on an LDS stack (one
+                                          launch), falling back to steps for the next 64 detects when a
+                                          workgroup's stack outgrows LDS */
 } oibvh_tuning;
 
 /* The detect buffers grow on overflow (the detect is re-run). */

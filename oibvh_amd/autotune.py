@@ -30,18 +30,30 @@ def _time(ctx: api.Context, scene: api.Scene, frame, entry: int, expand: int, re
     return ms
 
 
+def _time_refit(ctx: api.Context, frame, reps: int) -> float:
+    timer = api.Timer(ctx)
+    frame(0)
+    ctx.synchronize()
+    timer.start()
+    for i in range(reps):
+        frame(i + 1)
+    timer.stop()
+    return timer.ms() / reps
+
+
 def tune(ctx: api.Context, scene: api.Scene, frame, *, entries=(4, 6, 8), expands=(1, 2, 3),
-         blocks=(0, 2048), leaves=(0, 1, 2, 4), reps: int = 5) -> dict:
+         blocks=(0, 2048), leaves=(16, 2, 4, 8), reps: int = 5) -> dict:
     """Return {'entry_level', 'expand_levels', 'traverse_blocks',
     'reduce_leaves_per_thread', 'ms'} of the fastest setting and apply the
     tuning knobs to ctx."""
     best = None
-    # reduce subtree size first (refit only depends on it), then traversal shape
+    # reduce variant first, timed on the refit alone (the detect does not
+    # depend on it), then the traversal shape
     base = ctx.get_tuning()
     lpt_best, lpt_ms = 0, None
     for lpt in leaves:
         ctx.set_tuning(reduce_leaves_per_thread=lpt)
-        ms = _time(ctx, scene, frame, entries[0], expands[0], reps)
+        ms = min(_time_refit(ctx, frame, 4 * reps) for _ in range(2))
         if lpt_ms is None or ms < lpt_ms:
             lpt_best, lpt_ms = lpt, ms
     ctx.set_tuning(reduce_leaves_per_thread=lpt_best)

@@ -185,6 +185,8 @@ extern "C" oibvh_status oibvh_batch_mesh_count(const oibvh_batch* b, uint32_t* n
 static int batch_lpt(const oibvh_batch* b)
 {
     if (b->ctx->leaves_per_thread) return (int)b->ctx->leaves_per_thread;
+    // enough faces for about one wave-subtree workgroup per CU: k_reduce_w
+    if (b->Ft >= (1ull << 18)) return 16;
     // aim for >= ~2 workgroups per CU in the reduce
     const uint64_t target = 2ull * (uint64_t)(b->ctx->cus > 0 ? b->ctx->cus : 256) * 256ull;
     uint32_t m = 1;
@@ -596,6 +598,7 @@ static oibvh_status prepare_detect(oibvh_batch* b, uint32_t entry, uint32_t expa
     d.tab = oi_tab_args{b->d_mdesc, b->h_mdesc.data(), (uint32_t)b->meshes.size(), b->chunk_mesh, OI_CHUNK_SHIFT};
     d.depth = depth;
     d.expand = c->trav_expand ? c->trav_expand : std::min(3u, std::max(1u, expand));
+    d.schedule = c->trav_schedule;
     d.flags = flags;
     d.implicit_end = (uint32_t)b->It;
     return oi_det_prepare(&d);
@@ -636,7 +639,9 @@ extern "C" oibvh_status oibvh_batch_frame(oibvh_batch* b, uint32_t entry, uint32
     d.n_pairs = 0;
     d.stats = oibvh_detect_stats{};
     if ((r = prepare_detect(b, entry, expand, flags & ~OIBVH_DETECT_ASYNC)) != OIBVH_OK) return r;
-    const uint64_t key[6] = {entry, d.expand, flags & ~OIBVH_DETECT_ASYNC, (uint64_t)b->reduce_M,
+    const uint64_t key[6] = {entry | ((uint64_t)d.local_chunk << 32), d.expand | ((uint64_t)d.schedule << 32),
+                             flags & ~OIBVH_DETECT_ASYNC,
+                             (uint64_t)b->reduce_M,
                              d.generation, (uint64_t)b->desc_gen ^ ((uint64_t)oi_grid_for(c) << 32)};
     if (!b->gexec || std::memcmp(key, b->graph_key, sizeof key) != 0) {
         if (b->gexec) {
@@ -665,6 +670,7 @@ extern "C" oibvh_status oibvh_batch_frame(oibvh_batch* b, uint32_t entry, uint32
     HIPCHK(c, hipGraphLaunch(b->gexec, c->stream));
     d.unchecked = true;
     d.have_result = true;
+    d.sorted_ok = false;
     if (flags & OIBVH_DETECT_ASYNC) return OIBVH_OK;
     return oi_det_check(&d);
 }

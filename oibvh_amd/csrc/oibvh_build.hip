@@ -679,6 +679,239 @@ __global__ void __launch_bounds__(OI_BLOCK) k_reduce(ReduceArgs a)
     reduce_subtree<M>(ll, D.nodes, L, vl, L, h, it.y, stage);
 }
 
+// ---------------------------------------------------------------------------
+// k_reduce_w: stage 1 with one subtree per wave (the alternative to
+// k_reduce's one subtree per workgroup). Wave ws owns the 2^hw leaves
+// [ws 2^hw, (ws + 1) 2^hw) of the leaf level, hw = min(L, 8); lane l holds
+// leaves 64k + l (k < 4), so every face, vertex and node access of one
+// instruction covers consecutive leaves. Each level is staged in the wave's
+// own LDS and written out as a contiguous run; the wave climbs hw levels
+// with no workgroup barrier. The workgroup's waves then meet once, and
+// thread 0 merges their roots hb = min(L - hw, 2) levels further, so stage 1
+// ends at level L - hw - hb as k_reduce with M = 4 does (apex lgm = 2).
+// ---------------------------------------------------------------------------
+#define RW_LG 8
+#define RW_W (1u << RW_LG)
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Write `count` staged nodes (at most CMAX) to real index `first` onwards as
+// one contiguous run of float2 per lane; all LDS reads are issued before the
+// first store.
+template <uint32_t CMAX>
+__device__ __forceinline__ void wave_copy_out(float* nodes, uint32_t first, uint32_t count, const float* stage,
+                                              uint32_t lane)
+{
+    constexpr uint32_t IT = (3u * CMAX + 63u) / 64u;
+    OI_GLOBAL float* dst = oi_gw(nodes) + 6ull * first;
+    const float2* src = reinterpret_cast<const float2*>(stage);
+    const uint32_t n = 3u * count;
+    float2 v[IT];
+#pragma unroll
+    for (uint32_t it = 0; it < IT; it++) {
+        const uint32_t i = lane + 64u * it;
+        if (i < n) v[it] = src[i];
+    }
+#pragma unroll
+    for (uint32_t it = 0; it < IT; it++) {
+        const uint32_t i = lane + 64u * it;
+        if (i < n) {
+            dst[2u * i] = v[it].x;
+            dst[2u * i + 1u] = v[it].y;
+        }
+    }
+}
+
+// One level of a wave climb in place: slots 2t, 2t + 1 of stage (the child
+// level, first node 2f, cc real) -> slot t (first node f). Every read of the
+// level precedes every write.
+template <uint32_t WMAX>
+__device__ __forceinline__ void wave_level(float* stage, uint32_t width, uint32_t f, uint32_t cc, uint32_t lane)
+{
+    constexpr uint32_t IT = (WMAX + 63u) / 64u;
+    Box6 x[IT], y[IT];
+    bool has[IT], two[IT];
+#pragma unroll
+    for (uint32_t it = 0; it < IT; it++) {
+        const uint32_t t = lane + 64u * it;
+        const uint32_t q = f + t;
+        has[it] = t < width && 2u * q < cc;
+        two[it] = has[it] && 2u * q + 1u < cc;
+        if (has[it]) x[it] = get_box(stage, 2u * t);
+        if (two[it]) y[it] = get_box(stage, 2u * t + 1u);
+    }
+    wave_sync();
+#pragma unroll
+    for (uint32_t it = 0; it < IT; it++) {
+        if (two[it]) box_merge_into(x[it], y[it]);
+        if (has[it]) put_box(stage, lane + 64u * it, x[it]);
+    }
+    wave_sync();
+}
+
+// The leaf level of one wave subtree into stage (node layout): every load is
+// issued before any box is formed. BUILD reads the sorted face ids and writes
+// sfaces and perm; XF applies the model matrix. Both are compile-time, so the
+// loads form one straight run.
+template <bool BUILD, bool XF>
+__device__ __forceinline__ void wave_leaves(const ReduceArgs& A, const oi_build_desc& D, uint32_t f0, uint32_t hw,
+                                            uint32_t nleaf, uint32_t lane, float* stage)
+{
+    constexpr int K = RW_W / 64;
+    uint32_t f[K][3], p[K], fid[K];
+    bool ok[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const uint32_t off = 64u * k + lane;
+        p[k] = f0 + off;
+        ok[k] = off < (1u << hw) && p[k] < nleaf;
+        const uint32_t pc = ok[k] ? p[k] : f0;  // in range; only ok[k] results are kept
+        if constexpr (BUILD) {
+            fid[k] = oi_g(A.vals)[D.fbase + pc] - D.fbase;
+            oi_ld3(D.faces, fid[k], f[k]);
+        } else {
+            oi_ld3(D.sfaces, pc, f[k]);
+        }
+    }
+    float m[16];
+    if constexpr (XF) {
+#pragma unroll
+        for (int k = 0; k < 16; k++) m[k] = oi_g(D.xf)[k];
+    }
+    float v[K][3][3];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            if constexpr (XF) {
+                float q[3];
+                oi_ld3(D.pos, f[k][j], q);
+                oi_xform(m, q[0], q[1], q[2], v[k][j]);
+            } else {
+                oi_ld3(D.pos, f[k][j], v[k][j]);
+            }
+        }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        Box6 r;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            r.v[c] = gmin(gmin(v[k][0][c], v[k][1][c]), v[k][2][c]);
+            r.v[3 + c] = gmax(gmax(v[k][0][c], v[k][1][c]), v[k][2][c]);
+        }
+        if (ok[k]) put_box(stage, 64u * k + lane, r);
+    }
+    if constexpr (BUILD) {
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (ok[k]) {
+                OI_GLOBAL uint32_t* q = oi_gw(D.sfaces) + 3ull * p[k];
+                q[0] = f[k][0];
+                q[1] = f[k][1];
+                q[2] = f[k][2];
+                if (A.write_perm) oi_gw(D.perm)[p[k]] = fid[k];
+            }
+    }
+}
+
+// climb levels S..8 of a full-height (hw = 8) wave subtree; level s has
+// 256 >> s nodes
+template <uint32_t S>
+__device__ __forceinline__ void wave_climb8(float* stage, float* nodes, uint32_t L, uint32_t vl, uint32_t f0,
+                                            uint32_t cc, uint32_t lane)
+{
+    if constexpr (S <= RW_LG) {
+        constexpr uint32_t WIDTH = RW_W >> S;
+        const uint32_t l = L - S;
+        const uint32_t ncnt = oi_lvl_cnt(l, L, vl);
+        const uint32_t f = f0 >> S;
+        wave_level<WIDTH>(stage, WIDTH, f, cc, lane);
+        wave_copy_out<WIDTH>(nodes, oi_lvl_start(l, L, vl) + f, min(WIDTH, ncnt - f), stage, lane);
+        wave_climb8<S + 1>(stage, nodes, L, vl, f0, ncnt, lane);
+    }
+}
+
+__global__ void __launch_bounds__(OI_BLOCK) k_reduce_w(ReduceArgs A)
+{
+    __shared__ float S[OI_BLOCK / 64][6 * RW_W];
+    const uint2 it = A.T.item();
+    const oi_build_desc D = A.T.mesh(it.x);
+    const uint32_t L = D.L, vl = D.vl;
+    const uint32_t hw = (L < RW_LG) ? L : RW_LG;
+    const uint32_t hb = (L - hw < 2u) ? L - hw : 2u;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t ws = (it.y << hb) + wave;
+    const uint32_t f0 = ws << hw;
+    const uint32_t nleaf = oi_lvl_cnt(L, L, vl);
+    float* W = S[wave];
+    if (wave < (1u << hb) && f0 < nleaf) {
+        if (A.vals) {
+            if (D.xf)
+                wave_leaves<true, true>(A, D, f0, hw, nleaf, lane, W);
+            else
+                wave_leaves<true, false>(A, D, f0, hw, nleaf, lane, W);
+        } else {
+            if (D.xf)
+                wave_leaves<false, true>(A, D, f0, hw, nleaf, lane, W);
+            else
+                wave_leaves<false, false>(A, D, f0, hw, nleaf, lane, W);
+        }
+        wave_sync();
+        wave_copy_out<RW_W>(D.nodes, oi_lvl_start(L, L, vl) + f0, min(1u << hw, nleaf - f0), W, lane);
+        if (hw == RW_LG) {
+            wave_climb8<1>(W, D.nodes, L, vl, f0, nleaf, lane);
+        } else {
+            // short trees: the same climb with run-time widths
+            uint32_t ncnt = nleaf;
+            for (uint32_t s = 1; s <= hw; s++) {
+                const uint32_t l = L - s;
+                const uint32_t cc = ncnt;
+                ncnt = oi_lvl_cnt(l, L, vl);
+                const uint32_t width = 1u << (hw - s);
+                const uint32_t f = f0 >> s;
+                wave_level<RW_W / 2>(W, width, f, cc, lane);
+                wave_copy_out<RW_W / 2>(D.nodes, oi_lvl_start(l, L, vl) + f, min(width, ncnt - f), W, lane);
+            }
+        }
+    }
+    if (hb == 0) return;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // every wave root sits in slot 0 of its wave's stage
+        const uint32_t lw = L - hw;
+        uint32_t cc = oi_lvl_cnt(lw, L, vl);
+        Box6 m[2];
+        const uint32_t n1 = 1u << (hb - 1u);  // nodes of the first merged level
+        for (uint32_t t = 0; t < n1; t++) {
+            const uint32_t q = (it.y << (hb - 1u)) + t;
+            if (2u * q < cc) {
+                Box6 x = get_box(S[2u * t], 0);
+                if (2u * q + 1u < cc) box_merge_into(x, get_box(S[2u * t + 1u], 0));
+                m[t] = x;
+                OI_GLOBAL float* o = oi_gw(D.nodes) + 6ull * (oi_lvl_start(lw - 1u, L, vl) + q);
+#pragma unroll
+                for (int c = 0; c < 6; c++) o[c] = x.v[c];
+            }
+        }
+        if (hb == 2) {
+            cc = oi_lvl_cnt(lw - 1u, L, vl);
+            const uint32_t q = it.y;
+            if (2u * q < cc) {
+                Box6 x = m[0];
+                if (2u * q + 1u < cc) box_merge_into(x, m[1]);
+                OI_GLOBAL float* o = oi_gw(D.nodes) + 6ull * (oi_lvl_start(lw - 2u, L, vl) + q);
+#pragma unroll
+                for (int c = 0; c < 6; c++) o[c] = x.v[c];
+            }
+        }
+    }
+}
+
 // Stage 2, one workgroup per mesh: the levels above the stage-1 subtree roots
 // (level lc = L - h), in chunks of up to 2^11 nodes per climb, with as many
 // nodes per thread as keep all 256 threads busy. A separate launch, so the
@@ -818,9 +1051,12 @@ hipError_t oi_launch_sort_pass(hipStream_t st, const uint32_t* keys_in, const ui
 }
 
 // subtree height of stage 1 for M leaves per thread
+// M = 16 selects k_reduce_w, whose stage 1 climbs as high as k_reduce<4>
+static uint32_t lgm_of(int M) { return (M == 1) ? 0 : (M == 2) ? 1 : (M == 4 || M == 16) ? 2 : 3; }
+
 uint32_t oi_reduce_height(int M, uint32_t L)
 {
-    const uint32_t lgm = (M == 1) ? 0 : (M == 2) ? 1 : (M == 4) ? 2 : 3;
+    const uint32_t lgm = lgm_of(M);
     return (L < 8 + lgm) ? L : 8 + lgm;
 }
 
@@ -833,6 +1069,7 @@ hipError_t oi_launch_reduce(hipStream_t st, int M, const oi_build_desc* d, const
     case 1: hipLaunchKernelGGL(k_reduce<1>, dim3(nblocks), dim3(OI_BLOCK), 0, st, a); break;
     case 2: hipLaunchKernelGGL(k_reduce<2>, dim3(nblocks), dim3(OI_BLOCK), 0, st, a); break;
     case 4: hipLaunchKernelGGL(k_reduce<4>, dim3(nblocks), dim3(OI_BLOCK), 0, st, a); break;
+    case 16: hipLaunchKernelGGL(k_reduce_w, dim3(nblocks), dim3(OI_BLOCK), 0, st, a); break;
     default: hipLaunchKernelGGL(k_reduce<8>, dim3(nblocks), dim3(OI_BLOCK), 0, st, a); break;
     }
     return hipGetLastError();
@@ -852,6 +1089,7 @@ hipError_t oi_launch_reduce_inline(hipStream_t st, int M, const oi_build_desc* m
     case 1: hipLaunchKernelGGL(k_reduce<1>, g, b, 0, st, a); break;
     case 2: hipLaunchKernelGGL(k_reduce<2>, g, b, 0, st, a); break;
     case 4: hipLaunchKernelGGL(k_reduce<4>, g, b, 0, st, a); break;
+    case 16: hipLaunchKernelGGL(k_reduce_w, g, b, 0, st, a); break;
     default: hipLaunchKernelGGL(k_reduce<8>, g, b, 0, st, a); break;
     }
     return hipGetLastError();
@@ -864,7 +1102,7 @@ hipError_t oi_launch_reduce_apex_inline(hipStream_t st, int M, const oi_build_de
     uint32_t boff[OI_INLINE_MESHES + 1];
     for (uint32_t m = 0; m <= n; m++) boff[m] = m;
     ReduceArgs a{make_btab_inline(meshes, n, boff), nullptr, 0};
-    const uint32_t lgm = (M == 1) ? 0 : (M == 2) ? 1 : (M == 4) ? 2 : 3;
+    const uint32_t lgm = lgm_of(M);
     hipLaunchKernelGGL(k_reduce_apex, dim3(n), dim3(OI_BLOCK), 0, st, a, lgm);
     return hipGetLastError();
 }
@@ -874,7 +1112,7 @@ hipError_t oi_launch_reduce_apex(hipStream_t st, int M, const oi_build_desc* d, 
                                  const oi_build_desc* one, uint32_t nblocks)
 {
     ReduceArgs a{make_btab(d, work, one), nullptr, 0};
-    const uint32_t lgm = (M == 1) ? 0 : (M == 2) ? 1 : (M == 4) ? 2 : 3;
+    const uint32_t lgm = lgm_of(M);
     hipLaunchKernelGGL(k_reduce_apex, dim3(nblocks), dim3(OI_BLOCK), 0, st, a, lgm);
     return hipGetLastError();
 }

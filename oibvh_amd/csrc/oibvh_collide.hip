@@ -25,6 +25,8 @@
 // implicit space (mesh m spans [ibase_m, ibase_m + 2^(L_m+1) - 1)).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "oibvh_common.h"
 
 #define TB 256
@@ -446,22 +448,24 @@ struct StepArgs {
     uint32_t implicit_end;
 };
 
+// A run of n >= 1 real nodes from real index r0. Slots k >= n repeat the
+// last node (their results are never used), so every load is unconditional
+// and the run's loads are issued back to back.
 template <int E>
 __device__ __forceinline__ void load_run(const float* nodes, uint32_t r0, uint32_t n, float (&b)[1 << E][6])
 {
 #pragma unroll
-    for (int k = 0; k < (1 << E); k++)
-        if ((uint32_t)k < n) load_box(nodes, r0 + k, b[k]);
+    for (int k = 0; k < (1 << E); k++) load_box(nodes, r0 + min((uint32_t)k, n - 1u), b[k]);
 }
 
-// faces of leaf run [start, start + n) (implicit indices at level L)
+// faces of leaf run [leaf0, leaf0 + n), n >= 1 (leaf positions); slots
+// k >= n repeat the last face as load_run does
 template <int E>
 __device__ __forceinline__ void load_faces(const uint32_t* sfaces, uint32_t leaf0, uint32_t n,
                                            uint32_t (&f)[1 << E][3])
 {
 #pragma unroll
-    for (int k = 0; k < (1 << E); k++)
-        if ((uint32_t)k < n) oi_ld3(sfaces, leaf0 + k, f[k]);
+    for (int k = 0; k < (1 << E); k++) oi_ld3(sfaces, leaf0 + min((uint32_t)k, n - 1u), f[k]);
 }
 
 __device__ __forceinline__ bool share_vertex(const uint32_t* a, const uint32_t* b)
@@ -494,10 +498,140 @@ __device__ __forceinline__ uint32_t select_bit(u64 m, uint32_t k)
     return pos;
 }
 
+// One BVTT entry (a pair of implicit indices): with FIRST the pair itself is
+// tested first (an initial-front entry); then both sides descend up to E
+// levels and every pair of the two descendant runs is tested. Bit x * K + y
+// of `mask` keeps pair (sa.start + x, sb.start + y); `leafout` says the kept
+// pairs are leaf pairs. In self mode leaf pairs of one mesh that share a
+// vertex are dropped here.
+template <int E, bool FIRST, class Tab>
+__device__ __forceinline__ void entry_expand(const Tab& T, const oi_mesh_desc* s_desc, const uint32_t* s_ibase,
+                                             uint2 e, u64& mask, bool& leafout, Side& sa, Side& sb,
+                                             uint32_t& baseA, uint32_t& baseB, uint32_t& n_tests, uint32_t& n_adj)
+{
+    constexpr int K = 1 << E;
+    uint32_t ma, mb;
+    const MeshRef ra = tab_ref(T, s_desc, s_ibase, e.x, ma);
+    const MeshRef rb = tab_ref(T, s_desc, s_ibase, e.y, mb);
+    baseA = ra.ibase;
+    baseB = rb.ibase;
+    const uint32_t ia = e.x - ra.ibase, ib = e.y - rb.ibase;
+    const uint32_t la = oi_ilog2(ia + 1u), lb = oi_ilog2(ib + 1u);
+    const bool selfnode = (e.x == e.y);
+    const bool leafleaf = (la == ra.L && lb == rb.L);
+    bool go = !(selfnode && leafleaf);  // a triangle against itself
+    if (FIRST && go && !selfnode && !leafleaf) {
+        float ba[6], bb[6];
+        load_box(ra.nodes, ia - oi_virt_above(la, ra.L, ra.vl), ba);
+        load_box(rb.nodes, ib - oi_virt_above(lb, rb.L, rb.vl), bb);
+        go = overlap6(ba, bb);
+        n_tests++;
+    }
+    if (!go) return;
+    sa = expand_side(ia, la, ra.L, ra.vl, E);
+    sb = selfnode ? sa : expand_side(ib, lb, rb.L, rb.vl, E);
+    const uint32_t ca = oi_ilog2(sa.start + 1u), cb = oi_ilog2(sb.start + 1u);
+    leafout = (ca == ra.L && cb == rb.L);
+    // both runs unconditionally (a node paired with itself reads its run twice)
+    float bxa[K][6], bxb[K][6];
+    load_run<E>(ra.nodes, sa.start - oi_virt_above(ca, ra.L, ra.vl), sa.count, bxa);
+    load_run<E>(rb.nodes, sb.start - oi_virt_above(cb, rb.L, rb.vl), sb.count, bxb);
+#pragma unroll
+    for (int x = 0; x < K; x++) {
+#pragma unroll
+        for (int y = 0; y < K; y++) {
+            if ((uint32_t)x >= sa.count || (uint32_t)y >= sb.count) continue;
+            bool keep;
+            if (selfnode) {
+                // unordered child pairs; a child with itself needs no test and
+                // is no leaf pair
+                if (y < x) continue;
+                if (y == x) {
+                    keep = !leafout;
+                } else {
+                    keep = overlap6(bxa[x], bxb[y]);
+                    n_tests++;
+                }
+            } else {
+                keep = overlap6(bxa[x], bxb[y]);
+                n_tests++;
+            }
+            if (keep) mask |= 1ull << (x * K + y);
+        }
+    }
+    // inside one mesh (self mode) leaf pairs sharing a vertex are not
+    // collisions: drop them before they become candidates
+    if (leafout && ma == mb && mask) {
+        uint32_t fa[K][3], fb[K][3];
+        const uint32_t leaf0 = (1u << ra.L) - 1u;
+        load_faces<E>(ra.sfaces, sa.start - leaf0, sa.count, fa);
+        load_faces<E>(ra.sfaces, sb.start - leaf0, sb.count, fb);
+#pragma unroll
+        for (int x = 0; x < K; x++)
+#pragma unroll
+            for (int y = 0; y < K; y++) {
+                const u64 bit = 1ull << (x * K + y);
+                if ((mask & bit) && share_vertex(fa[x], fb[y])) {
+                    mask &= ~bit;
+                    n_adj++;
+                }
+            }
+    }
+}
+
+// LDS the workgroup needs to test its leaf pairs cooperatively
+struct FusedLds {
+    u64 cmask[TB];
+    uint32_t coff[TB], ga[TB], gb[TB];
+    uint32_t hbase;
+};
+
+// The workgroup's C leaf pairs (thread t owns the set bits of cmask[t],
+// numbered from coff[t]) through the FP32 SAT: pair c is found by a binary
+// search over the offsets and a select in the owner's mask; hits are
+// appended with one fetch_add per round.
+template <int E, class Tab>
+__device__ __forceinline__ void fused_narrow(const Tab& T, const uint32_t* s_ibase, const oi_mesh_desc* s_desc,
+                                             FusedLds& F, uint32_t C, uint32_t* s_wsum, uint4* hits,
+                                             uint32_t* n_hits, uint32_t hits_cap, uint32_t* error,
+                                             uint32_t& n_fp64, uint32_t& n_adj)
+{
+    constexpr int K = 1 << E;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t c0 = 0; c0 < C; c0 += TB) {
+        const uint32_t c = c0 + tid;
+        bool hit = false;
+        uint4 h = make_uint4(0, 0, 0, 0);
+        if (c < C) {
+            uint32_t lo = 0, hi = TB - 1;  // last thread whose offset is <= c
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1u) >> 1;
+                if (F.coff[mid] <= c)
+                    lo = mid;
+                else
+                    hi = mid - 1u;
+            }
+            const uint32_t bit = select_bit(F.cmask[lo], c - F.coff[lo]);
+            const uint2 pr = make_uint2(F.ga[lo] + (bit >> E), F.gb[lo] + (bit & (K - 1)));
+            narrow_one<false, Tab>(T, s_ibase, s_desc, pr, hit, h, n_fp64, n_adj);
+        }
+        uint32_t H;
+        const uint32_t hoff = block_scan(hit ? 1u : 0u, s_wsum, H);
+        if (tid == 0) F.hbase = H ? add_u32(n_hits, H) : 0u;
+        __syncthreads();
+        if (hit) {
+            const uint32_t o = F.hbase + hoff;
+            if (o < hits_cap)
+                hits[o] = h;
+            else
+                or_u32(error, OI_ERR_HITS_OVERFLOW);
+        }
+    }
+}
+
 // FUSED: overlapping leaf pairs are tested with the FP32 SAT by the
-// workgroup that found them (cooperatively: candidate c of the workgroup is
-// found by a binary search over the threads' candidate offsets and a select
-// in the owner's mask) instead of going to the candidate list.
+// workgroup that found them (fused_narrow) instead of going to the candidate
+// list.
 template <int E, bool FIRST, bool FUSED, class Tab>
 __global__ void __launch_bounds__(TB) k_bvtt_step(StepArgs<Tab> A)
 {
@@ -505,9 +639,8 @@ __global__ void __launch_bounds__(TB) k_bvtt_step(StepArgs<Tab> A)
     __shared__ oi_mesh_desc s_desc[MESH_LDS];
     __shared__ uint32_t s_ibase[MESH_LDS];
     __shared__ uint32_t s_wsum[4];
-    __shared__ uint32_t s_obase, s_cbase, s_hbase;
-    __shared__ u64 s_cmask[FUSED ? TB : 1];
-    __shared__ uint32_t s_coff[FUSED ? TB : 1], s_ga[FUSED ? TB : 1], s_gb[FUSED ? TB : 1];
+    __shared__ uint32_t s_obase, s_cbase;
+    __shared__ std::conditional_t<FUSED, FusedLds, char> s_f[1];
     // after an overflow the count keeps growing past the buffer (writes were
     // dropped and the detect will be re-run): read only what exists
     uint32_t n_in = A.count_in ? *A.count_in : A.n_in;
@@ -528,88 +661,23 @@ __global__ void __launch_bounds__(TB) k_bvtt_step(StepArgs<Tab> A)
         bool leafout = false;  // kept pairs are leaf pairs (candidates)
         Side sa = {0, 0}, sb = {0, 0};
         uint32_t baseA = 0, baseB = 0;
-        if (have) {
-            uint32_t ma, mb;
-            const MeshRef ra = tab_ref(A.tab, s_desc, s_ibase, e.x, ma);
-            const MeshRef rb = tab_ref(A.tab, s_desc, s_ibase, e.y, mb);
-            baseA = ra.ibase;
-            baseB = rb.ibase;
-            const uint32_t ia = e.x - ra.ibase, ib = e.y - rb.ibase;
-            const uint32_t la = oi_ilog2(ia + 1u), lb = oi_ilog2(ib + 1u);
-            const bool selfnode = (e.x == e.y);
-            const bool leafleaf = (la == ra.L && lb == rb.L);
-            bool go = !(selfnode && leafleaf);  // a triangle against itself
-            if (FIRST && go && !selfnode && !leafleaf) {
-                float ba[6], bb[6];
-                load_box(ra.nodes, ia - oi_virt_above(la, ra.L, ra.vl), ba);
-                load_box(rb.nodes, ib - oi_virt_above(lb, rb.L, rb.vl), bb);
-                go = overlap6(ba, bb);
-                n_tests++;
-            }
-            if (go) {
-                sa = expand_side(ia, la, ra.L, ra.vl, E);
-                sb = selfnode ? sa : expand_side(ib, lb, rb.L, rb.vl, E);
-                const uint32_t ca = oi_ilog2(sa.start + 1u), cb = oi_ilog2(sb.start + 1u);
-                leafout = (ca == ra.L && cb == rb.L);
-                float bxa[K][6], bxb[K][6];
-                load_run<E>(ra.nodes, sa.start - oi_virt_above(ca, ra.L, ra.vl), sa.count, bxa);
-                if (!selfnode) load_run<E>(rb.nodes, sb.start - oi_virt_above(cb, rb.L, rb.vl), sb.count, bxb);
-#pragma unroll
-                for (int x = 0; x < K; x++) {
-#pragma unroll
-                    for (int y = 0; y < K; y++) {
-                        if ((uint32_t)x >= sa.count || (uint32_t)y >= sb.count) continue;
-                        bool keep;
-                        if (selfnode) {
-                            // unordered child pairs; a child with itself needs
-                            // no test and is no leaf pair
-                            if (y < x) continue;
-                            if (y == x) {
-                                keep = !leafout;
-                            } else {
-                                keep = overlap6(bxa[x], bxa[y]);
-                                n_tests++;
-                            }
-                        } else {
-                            keep = overlap6(bxa[x], bxb[y]);
-                            n_tests++;
-                        }
-                        if (keep) mask |= 1ull << (x * K + y);
-                    }
-                }
-                // inside one mesh (self mode) leaf pairs sharing a vertex are
-                // not collisions: drop them before they become candidates
-                if (leafout && ma == mb && mask) {
-                    uint32_t fa[K][3], fb[K][3];
-                    const uint32_t leaf0 = (1u << ra.L) - 1u;
-                    load_faces<E>(ra.sfaces, sa.start - leaf0, sa.count, fa);
-                    load_faces<E>(ra.sfaces, sb.start - leaf0, sb.count, fb);
-#pragma unroll
-                    for (int x = 0; x < K; x++)
-#pragma unroll
-                        for (int y = 0; y < K; y++) {
-                            const u64 bit = 1ull << (x * K + y);
-                            if ((mask & bit) && share_vertex(fa[x], fb[y])) {
-                                mask &= ~bit;
-                                n_adj++;
-                            }
-                        }
-                }
-            }
-        }
+        if (have) entry_expand<E, FIRST, Tab>(A.tab, s_desc, s_ibase, e, mask, leafout, sa, sb, baseA, baseB,
+                                              n_tests, n_adj);
         const uint32_t nout = (uint32_t)__popcll(mask);
-        uint32_t T, C;
-        const uint32_t off = block_scan(leafout ? 0u : nout, s_wsum, T);
-        const uint32_t coff = block_scan(leafout ? nout : 0u, s_wsum, C);
+        // one scan for both counts: front pairs in the high half, leaf pairs
+        // in the low half (at most 256 x 64 each, so no carry crosses)
+        uint32_t TC;
+        const uint32_t oc = block_scan(leafout ? nout : (nout << 16), s_wsum, TC);
+        const uint32_t T = TC >> 16, C = TC & 0xffffu, off = oc >> 16, coff = oc & 0xffffu;
         if (tid == 0) {
             s_obase = T ? add_u32(A.count_out, T) : 0u;
             if (!FUSED) s_cbase = C ? add_u32(A.cand_count, C) : 0u;
         }
-        if (FUSED) {
-            s_cmask[tid] = leafout ? mask : 0ull;
-            s_coff[tid] = coff;
-            s_ga[tid] = baseA + sa.start;
-            s_gb[tid] = baseB + sb.start;
+        if constexpr (FUSED) {
+            s_f[0].cmask[tid] = leafout ? mask : 0ull;
+            s_f[0].coff[tid] = coff;
+            s_f[0].ga[tid] = baseA + sa.start;
+            s_f[0].gb[tid] = baseB + sb.start;
         }
         __syncthreads();
         if (nout && !(FUSED && leafout)) {
@@ -625,40 +693,155 @@ __global__ void __launch_bounds__(TB) k_bvtt_step(StepArgs<Tab> A)
                 o++;
             }
         }
-        if (FUSED && C) {
-            if (tid == 0) n_cands += C;
-            for (uint32_t c0 = 0; c0 < C; c0 += TB) {
-                const uint32_t c = c0 + tid;
-                bool hit = false;
-                uint4 h = make_uint4(0, 0, 0, 0);
-                if (c < C) {
-                    uint32_t lo = 0, hi = TB - 1;  // last thread whose offset is <= c
-                    while (lo < hi) {
-                        const uint32_t mid = (lo + hi + 1u) >> 1;
-                        if (s_coff[mid] <= c)
-                            lo = mid;
-                        else
-                            hi = mid - 1u;
-                    }
-                    const uint32_t bit = select_bit(s_cmask[lo], c - s_coff[lo]);
-                    const uint2 pr = make_uint2(s_ga[lo] + (bit >> E), s_gb[lo] + (bit & (K - 1)));
-                    narrow_one<false, Tab>(A.tab, s_ibase, s_desc, pr, hit, h, n_fp64, n_adj);
-                }
-                uint32_t H;
-                const uint32_t hoff = block_scan(hit ? 1u : 0u, s_wsum, H);
-                if (tid == 0) s_hbase = H ? add_u32(A.n_hits, H) : 0u;
-                __syncthreads();
-                if (hit) {
-                    const uint32_t o = s_hbase + hoff;
-                    if (o < A.hits_cap)
-                        A.hits[o] = h;
-                    else
-                        or_u32(A.error, OI_ERR_HITS_OVERFLOW);
-                }
+        if constexpr (FUSED) {
+            if (C) {
+                if (tid == 0) n_cands += C;
+                fused_narrow<E, Tab>(A.tab, s_ibase, s_desc, s_f[0], C, s_wsum, A.hits, A.n_hits, A.hits_cap,
+                                     A.error, n_fp64, n_adj);
             }
         }
         __syncthreads();
     }
+    uint32_t nt, na;
+    block_scan(n_tests, s_wsum, nt);
+    block_scan(n_adj, s_wsum, na);
+    if (tid == 0 && nt) __hip_atomic_fetch_add(A.tests, (u64)nt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0 && na) __hip_atomic_fetch_add(A.adj, (u64)na, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (FUSED && tid == 0 && n_cands) add_u32(A.cand_count, n_cands);
+}
+
+// ---------------------------------------------------------------------------
+// k_bvtt_local: the whole walk below a slice of the initial front inside one
+// workgroup. Workgroup b takes initial entries [b * chunk, (b + 1) * chunk),
+// tests and expands them (as the first step does) and keeps the surviving
+// pairs on a stack in LDS; every round pops up to 256 pairs from the top,
+// expands them E levels and pushes the kept non-leaf pairs back, so the
+// deepest pairs go first and the stack stays short (depth-first order bounds
+// it by the walk's depth rather than by the width of a level). Leaf pairs go
+// through the fused SAT (or to the candidate list). One launch replaces the
+// chain of steps; a workgroup whose stack outgrows LDS sets
+// OI_ERR_LOCAL_OVERFLOW and the detect is run again with steps.
+// ---------------------------------------------------------------------------
+#define LQ 4096
+
+template <class Tab>
+struct LocalArgs {
+    Tab tab;
+    const uint2* front_in;
+    uint32_t n_in;
+    uint32_t chunk;
+    uint2* cands;
+    uint32_t* cand_count;
+    uint32_t cand_cap;
+    uint32_t* error;
+    unsigned long long* tests;
+    unsigned long long* adj;
+    uint4* hits;
+    uint32_t* n_hits;
+    uint32_t hits_cap;
+    uint32_t implicit_end;
+};
+
+template <int E, bool FUSED, class Tab>
+__global__ void __launch_bounds__(TB) k_bvtt_local(LocalArgs<Tab> A)
+{
+    constexpr int K = 1 << E;
+    __shared__ oi_mesh_desc s_desc[MESH_LDS];
+    __shared__ uint32_t s_ibase[MESH_LDS];
+    __shared__ uint32_t s_wsum[4];
+    __shared__ uint32_t s_top, s_obase, s_cbase;
+    __shared__ uint2 s_q[LQ];
+    __shared__ std::conditional_t<FUSED, FusedLds, char> s_f[1];
+    const uint32_t c0 = blockIdx.x * A.chunk;
+    if (c0 >= A.n_in) return;
+    tab_init(A.tab, s_desc, s_ibase);
+    const uint32_t tid = threadIdx.x;
+    uint32_t n_tests = 0, n_adj = 0, n_cands = 0, n_fp64 = 0;
+    const uint32_t n_first = min(A.chunk, A.n_in - c0);
+    if (tid == 0) s_top = 0;
+    __syncthreads();
+    bool overflow = false;
+    uint32_t done_first = 0;  // initial entries consumed
+    uint32_t top = 0;         // stack height (uniform)
+    while (done_first < n_first || top > 0) {
+        // a round: the next initial entries while any are left, else the top of the stack
+        const bool first = done_first < n_first;
+        const uint32_t n = first ? min((uint32_t)TB, n_first - done_first) : min((uint32_t)TB, top);
+        const uint32_t start = first ? top : top - n;  // where this round's pushes begin
+        bool have = tid < n;
+        uint2 e = make_uint2(0, 0);
+        if (have) e = first ? A.front_in[c0 + done_first + tid] : s_q[start + tid];
+        if (have && (e.x >= A.implicit_end || e.y >= A.implicit_end)) {
+            or_u32(A.error, OI_ERR_CORRUPT);
+            have = false;
+        }
+        u64 mask = 0;
+        bool leafout = false;
+        Side sa = {0, 0}, sb = {0, 0};
+        uint32_t baseA = 0, baseB = 0;
+        if (have) {
+            if (first)
+                entry_expand<E, true, Tab>(A.tab, s_desc, s_ibase, e, mask, leafout, sa, sb, baseA, baseB, n_tests,
+                                           n_adj);
+            else
+                entry_expand<E, false, Tab>(A.tab, s_desc, s_ibase, e, mask, leafout, sa, sb, baseA, baseB,
+                                            n_tests, n_adj);
+        }
+        const uint32_t nout = (uint32_t)__popcll(mask);
+        // one scan for both counts (front pairs high, leaf pairs low); its
+        // barriers also order every read of the popped slots before the
+        // pushes that overwrite them
+        uint32_t TC;
+        const uint32_t oc = block_scan(leafout ? nout : (nout << 16), s_wsum, TC);
+        const uint32_t T = TC >> 16, C = TC & 0xffffu, off = oc >> 16, coff = oc & 0xffffu;
+        if (tid == 0) {
+            s_obase = start;
+            s_top = (start + T < LQ) ? start + T : LQ;
+            if (!FUSED) s_cbase = C ? add_u32(A.cand_count, C) : 0u;
+        }
+        if (start + T > LQ) overflow = true;
+        if constexpr (FUSED) {
+            s_f[0].cmask[tid] = leafout ? mask : 0ull;
+            s_f[0].coff[tid] = coff;
+            s_f[0].ga[tid] = baseA + sa.start;
+            s_f[0].gb[tid] = baseB + sb.start;
+        }
+        __syncthreads();
+        if (nout && !leafout) {
+            uint32_t o = s_obase + off;
+            u64 m = mask;
+            while (m) {
+                const uint32_t k = (uint32_t)__ffsll((long long)m) - 1u;
+                m &= m - 1ull;
+                if (o < LQ) s_q[o] = make_uint2(baseA + sa.start + (k >> E), baseB + sb.start + (k & (K - 1)));
+                o++;
+            }
+        }
+        if (!FUSED && nout && leafout) {
+            uint32_t o = s_cbase + coff;
+            if (o + nout > A.cand_cap) or_u32(A.error, OI_ERR_CAND_OVERFLOW);
+            u64 m = mask;
+            while (m) {
+                const uint32_t k = (uint32_t)__ffsll((long long)m) - 1u;
+                m &= m - 1ull;
+                if (o < A.cand_cap)
+                    A.cands[o] = make_uint2(baseA + sa.start + (k >> E), baseB + sb.start + (k & (K - 1)));
+                o++;
+            }
+        }
+        if constexpr (FUSED) {
+            if (C) {
+                if (tid == 0) n_cands += C;
+                fused_narrow<E, Tab>(A.tab, s_ibase, s_desc, s_f[0], C, s_wsum, A.hits, A.n_hits, A.hits_cap,
+                                     A.error, n_fp64, n_adj);
+            }
+        }
+        __syncthreads();
+        top = s_top;
+        if (first) done_first += n;
+        __syncthreads();
+    }
+    if (tid == 0 && overflow) or_u32(A.error, OI_ERR_LOCAL_OVERFLOW);
     uint32_t nt, na;
     block_scan(n_tests, s_wsum, nt);
     block_scan(n_adj, s_wsum, na);
@@ -853,6 +1036,44 @@ static hipError_t launch_step(hipStream_t st, uint32_t grid, const Tab& tab, int
 }
 
 template <class Tab>
+static hipError_t launch_local(hipStream_t st, const Tab& tab, const uint2* front_in, uint32_t n_in, uint32_t chunk,
+                               uint2* cands, uint32_t* cand_count, uint32_t cand_cap, uint4* hits,
+                               uint32_t hits_cap, oi_detect_ctl* ctl, uint32_t expand, uint32_t implicit_end,
+                               int fused)
+{
+    if (n_in == 0) return hipSuccess;
+    LocalArgs<Tab> a{};
+    a.tab = tab;
+    a.front_in = front_in;
+    a.n_in = n_in;
+    a.chunk = chunk;
+    a.cands = cands;
+    a.cand_count = cand_count;
+    a.cand_cap = cand_cap;
+    a.error = &ctl->error;
+    a.tests = &ctl->tests;
+    a.adj = &ctl->adj;
+    a.hits = hits;
+    a.n_hits = &ctl->n_hits;
+    a.hits_cap = hits_cap;
+    a.implicit_end = implicit_end;
+    const dim3 g((n_in + chunk - 1) / chunk), b(TB);
+#define OI_LOCAL(EE)                                                           \
+    if (fused)                                                                 \
+        hipLaunchKernelGGL((k_bvtt_local<EE, true, Tab>), g, b, 0, st, a);     \
+    else                                                                       \
+        hipLaunchKernelGGL((k_bvtt_local<EE, false, Tab>), g, b, 0, st, a);
+    switch (expand) {
+    case 1: OI_LOCAL(1) break;
+    case 2: OI_LOCAL(2) break;
+    case 3: OI_LOCAL(3) break;
+    default: return hipErrorInvalidValue;
+    }
+#undef OI_LOCAL
+    return hipGetLastError();
+}
+
+template <class Tab>
 static hipError_t launch_narrow(hipStream_t st, uint32_t grid, const Tab& tab, const uint2* cands,
                                 const uint32_t* cand_count, uint32_t cand_cap, uint4* hits, uint32_t* n_hits,
                                 uint32_t hits_cap, oi_detect_ctl* ctl, int fp64)
@@ -901,6 +1122,20 @@ hipError_t oi_launch_bvtt_step(hipStream_t st, uint32_t grid, const oi_tab_args*
     return launch_step(st, grid, make_tab(t->d_meshes, t->h_meshes, t->n_meshes), first, front_in, count_in, n_in,
                        cap_in, front_out, count_out, cap_out, cands, cand_count, cand_cap, hits, hits_cap, ctl,
                        expand, implicit_end, fused);
+}
+
+// the whole walk in one launch, one workgroup per `chunk` initial entries
+hipError_t oi_launch_bvtt_local(hipStream_t st, const oi_tab_args* t, const uint2* front_in, uint32_t n_in,
+                                uint32_t chunk, uint2* cands, uint32_t* cand_count, uint32_t cand_cap, uint4* hits,
+                                uint32_t hits_cap, oi_detect_ctl* ctl, uint32_t expand, uint32_t implicit_end,
+                                int fused)
+{
+    if (chunk == 0) return hipErrorInvalidValue;
+    if (t->chunk_mesh)
+        return launch_local(st, ChunkTab{t->d_meshes, t->chunk_mesh, t->shift}, front_in, n_in, chunk, cands,
+                            cand_count, cand_cap, hits, hits_cap, ctl, expand, implicit_end, fused);
+    return launch_local(st, make_tab(t->d_meshes, t->h_meshes, t->n_meshes), front_in, n_in, chunk, cands,
+                        cand_count, cand_cap, hits, hits_cap, ctl, expand, implicit_end, fused);
 }
 
 // Scene::convertToVertexArray (scene.cu:69-93): six vertices per hit, the

@@ -183,6 +183,7 @@ struct oi_detect_ctl {
 #define OI_ERR_CORRUPT 4u
 #define OI_ERR_HITS_OVERFLOW 8u
 #define OI_ERR_CAND_OVERFLOW 16u
+#define OI_ERR_LOCAL_OVERFLOW 32u  // a workgroup's front outgrew its LDS queue (local walk)
 
 // a scene's concatenated implicit index space must stay below this bound
 #define OI_IMPLICIT_LIMIT 0xfffffffeu

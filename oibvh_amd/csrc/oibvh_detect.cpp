@@ -64,8 +64,28 @@ oibvh_status oi_det_prepare(oi_detector* d)
     // every BVTT step takes each non-leaf side up to `expand` levels down and
     // tests the pairs it reaches: the walk ends at the leaves after
     // ceil(depth / expand) steps (one step tests entry-level leaf pairs)
-    d->n_levels = std::max(1u, (d->depth + d->expand - 1) / d->expand);
-    if (d->n_levels + 1 > 256) return oi_set_err(c, OIBVH_ERR_INVALID, "detect: %u BVTT steps", d->n_levels);
+    // the schedule's leading steps, then `expand` per step, until the deepest
+    // tree's leaves are reached (at least one step)
+    uint32_t k = 0, done = 0, sch = d->schedule;
+    bool lead = true;
+    while (k == 0 || done < d->depth) {
+        uint32_t e = lead ? (sch & 3u) : 0u;
+        if (e == 0) {
+            lead = false;
+            e = d->expand;
+        }
+        sch >>= 2;
+        if (k + 1 >= 256) return oi_set_err(c, OIBVH_ERR_INVALID, "detect: more than 255 BVTT steps");
+        d->step_e[k++] = (uint8_t)e;
+        done += e;
+    }
+    d->n_levels = k;
+    // the local walk (one launch) when asked for, unless it overflowed recently
+    const uint32_t mode = c->trav_local;
+    d->local_chunk = 0;
+    if (mode && d->local_off == 0 && d->n_front) d->local_chunk = mode;
+    if (d->local_off) d->local_off--;
+    if (d->local_chunk) d->n_levels = 1;
     const uint32_t fcap = std::max(c->queue_cap, oi_np2(std::max(1u, d->n_front)));
     if (d->front_cap < fcap) {
         dfree(d->front[0]);
@@ -93,13 +113,18 @@ oibvh_status oi_det_enqueue(oi_detector* d, bool reset_sticky)
     // its own narrow pass (its registers would slow every step)
     const bool fp64 = (d->flags & OIBVH_DETECT_FP64_FALLBACK) != 0;
     d->fused = !fp64;
-    for (uint32_t k = 0; k < d->n_levels; k++) {
+    if (d->local_chunk) {
+        HIPCHK(c, oi_launch_bvtt_local(st, &d->tab, d->d_init, d->n_front, d->local_chunk, d->cands, &d->ctl->cands,
+                                       d->cand_cap, d->hits, d->hits_cap, d->ctl, d->expand, d->implicit_end,
+                                       d->fused ? 1 : 0));
+    }
+    for (uint32_t k = 0; k < (d->local_chunk ? 0u : d->n_levels); k++) {
         // step 0 reads the initial front in place; then the fronts alternate
         const uint2* in = k ? d->front[(k - 1) & 1] : d->d_init;
         HIPCHK(c, oi_launch_bvtt_step(st, grid, &d->tab, k == 0, in, k ? d->counts + k : nullptr, d->n_front,
                                       k ? d->front_cap : d->n_front, d->front[k & 1], d->counts + k + 1,
                                       d->front_cap, d->cands, &d->ctl->cands, d->cand_cap, d->hits, d->hits_cap,
-                                      d->ctl, d->expand, d->implicit_end, d->fused ? 1 : 0));
+                                      d->ctl, d->step_e[k], d->implicit_end, d->fused ? 1 : 0));
     }
     if (!d->fused)
         HIPCHK(c, oi_launch_narrow(st, grid, &d->tab, d->cands, &d->ctl->cands, d->cand_cap, d->hits,
@@ -137,6 +162,11 @@ oibvh_status oi_det_check(oi_detector* d)
         }
         bool rerun = false;
         oibvh_status r = OIBVH_OK;
+        if (k.error & OI_ERR_LOCAL_OVERFLOW) {
+            // a workgroup's front outgrew LDS: steps for the next queries
+            d->local_off = 64;
+            rerun = true;
+        }
         uint32_t maxfront = 0;
         for (uint32_t v : d->h_counts) maxfront = std::max(maxfront, v);
         if (k.error & OI_ERR_QUEUE_OVERFLOW) {
@@ -176,6 +206,7 @@ oibvh_status oi_det_check(oi_detector* d)
             d->stats.retries = retries;
             d->stats.async_errors = earlier;
             d->unchecked = false;
+            d->sorted_ok = false;  // a new result: the sorted copy is refetched on demand
             return OIBVH_OK;
         }
         retries++;
@@ -189,7 +220,7 @@ oibvh_status oi_det_check(oi_detector* d)
 static oibvh_status sorted_hits(oi_detector* d)
 {
     oibvh_ctx* c = d->ctx;
-    if (d->sorted_ok) return OIBVH_OK;
+    if (d->sorted_ok && d->h_sorted.size() == d->n_pairs) return OIBVH_OK;
     std::vector<uint4>& h = d->h_sorted;
     h.resize(d->n_pairs);
     if (d->n_pairs) {

@@ -141,8 +141,8 @@ extern "C" oibvh_status oibvh_ctx_set_tuning(oibvh_ctx* ctx, const oibvh_tuning*
 {
     if (!ctx || !t) return OIBVH_ERR_INVALID;
     const uint32_t lpt = t->reduce_leaves_per_thread;
-    if (lpt != 0 && lpt != 1 && lpt != 2 && lpt != 4 && lpt != 8)
-        return oi_set_err(ctx, OIBVH_ERR_INVALID, "reduce_leaves_per_thread must be 0, 1, 2, 4 or 8");
+    if (lpt != 0 && lpt != 1 && lpt != 2 && lpt != 4 && lpt != 8 && lpt != 16)
+        return oi_set_err(ctx, OIBVH_ERR_INVALID, "reduce_leaves_per_thread must be 0, 1, 2, 4, 8 or 16");
     if (t->front_capacity && (t->front_capacity & (t->front_capacity - 1)))
         return oi_set_err(ctx, OIBVH_ERR_INVALID, "front_capacity must be a power of two");
     if (t->traverse_expand > 3) return oi_set_err(ctx, OIBVH_ERR_INVALID, "traverse_expand must be 0..3");
@@ -152,6 +152,8 @@ extern "C" oibvh_status oibvh_ctx_set_tuning(oibvh_ctx* ctx, const oibvh_tuning*
     ctx->queue_cap = t->front_capacity ? t->front_capacity : (1u << 22);
     ctx->cand_cap = t->candidate_capacity ? t->candidate_capacity : (1u << 20);
     ctx->hits_cap = t->hits_capacity ? t->hits_capacity : (1u << 20);
+    ctx->trav_schedule = t->traverse_schedule;
+    ctx->trav_local = t->traverse_local;
     return OIBVH_OK;
 }
 
@@ -163,6 +165,8 @@ extern "C" oibvh_status oibvh_ctx_get_tuning(const oibvh_ctx* ctx, oibvh_tuning*
     t->traverse_expand = ctx->trav_expand;
     t->front_capacity = ctx->queue_cap;
     t->candidate_capacity = ctx->cand_cap;
+    t->traverse_schedule = ctx->trav_schedule;
+    t->traverse_local = ctx->trav_local;
     t->hits_capacity = ctx->hits_cap;
     return OIBVH_OK;
 }
@@ -331,15 +335,23 @@ extern "C" void oibvh_tree_destroy(oibvh_tree* t)
     oi_ctx_release(c);
 }
 
+// reduce variant for F faces per launch: the wave-subtree kernel (16) once
+// there are about as many of its 1024-leaf workgroups as CUs, else M leaves
+// per thread with >= ~2 workgroups per CU
+static int oi_default_lpt(uint64_t F, int cus)
+{
+    if (F >= (1ull << 18)) return 16;
+    const uint64_t target = 2ull * (uint64_t)(cus > 0 ? cus : 256) * 256ull;
+    int m = 1;
+    while (m < 8 && F / (2ull * m) >= target) m *= 2;
+    return m;
+}
+
 static int pick_lpt(const oibvh_tree* t)
 {
     uint32_t lpt = t->ctx->leaves_per_thread;
     if (lpt) return (int)lpt;
-    // aim for >= ~2 workgroups per CU in the reduce
-    const uint64_t target = 2ull * (uint64_t)(t->ctx->cus > 0 ? t->ctx->cus : 256) * 256ull;
-    uint32_t m = 1;
-    while (m < 8 && (uint64_t)t->F / (2ull * m) >= target) m *= 2;
-    return (int)m;
+    return oi_default_lpt(t->F, t->ctx->cus);
 }
 
 static oibvh_status launch_reduce(oibvh_tree* t, bool build)
@@ -656,6 +668,7 @@ static oibvh_status enqueue_detect(oibvh_scene* s, bool reset_sticky)
     d.tab = oi_tab_args{n > 4 ? s->d_desc : nullptr, desc.data(), n, nullptr, 0};
     d.depth = depth;
     d.expand = c->trav_expand ? c->trav_expand : std::min(3u, std::max(1u, s->last_expand));
+    d.schedule = c->trav_schedule;
     d.flags = s->last_flags;
     d.implicit_end = (uint32_t)acc;
     oibvh_status r = oi_det_prepare(&d);
@@ -712,11 +725,7 @@ extern "C" oibvh_status oibvh_scene_refit(oibvh_scene* s)
         uint64_t F = 0;
         for (uint32_t k = 0; k < n; k++) F += s->trees[i0 + k]->F;
         int M = (int)c->leaves_per_thread;
-        if (!M) {
-            const uint64_t target = 2ull * (uint64_t)(c->cus > 0 ? c->cus : 256) * 256ull;
-            M = 1;
-            while (M < 8 && F / (2ull * M) >= target) M *= 2;
-        }
+        if (!M) M = oi_default_lpt(F, c->cus);
         oi_build_desc d[8], apex[8];
         uint32_t boff[9], na = 0;
         boff[0] = 0;

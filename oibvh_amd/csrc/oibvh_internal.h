@@ -46,6 +46,8 @@ hipError_t oi_launch_bvtt_step(hipStream_t, uint32_t, const oi_tab_args*, int, c
                                uint32_t, oi_detect_ctl*, uint32_t, uint32_t, int);
 hipError_t oi_launch_narrow(hipStream_t, uint32_t, const oi_tab_args*, const uint2*, const uint32_t*, uint32_t,
                             uint4*, uint32_t*, uint32_t, oi_detect_ctl*, int);
+hipError_t oi_launch_bvtt_local(hipStream_t, const oi_tab_args*, const uint2*, uint32_t, uint32_t, uint2*, uint32_t*,
+                                uint32_t, uint4*, uint32_t, oi_detect_ctl*, uint32_t, uint32_t, int);
 hipError_t oi_launch_pair_vertices(hipStream_t, const oi_build_desc*, const uint4*, uint32_t, float*);
 }
 
@@ -60,6 +62,8 @@ struct oibvh_ctx {
     uint32_t leaves_per_thread = 0;
     uint32_t trav_blocks = 0;
     uint32_t trav_expand = 0;
+    uint32_t trav_schedule = 0;
+    uint32_t trav_local = 0;  // 0: BVTT steps; n: the local walk with n initial pairs per workgroup
     uint32_t queue_cap = 1u << 22;
     uint32_t hits_cap = 1u << 20;
     uint32_t cand_cap = 1u << 20;
@@ -134,6 +138,10 @@ struct oi_detector {
     // the query (kept for re-runs)
     oi_tab_args tab{};
     uint32_t depth = 0, expand = 1, flags = 0, implicit_end = 0;
+    uint32_t schedule = 0;      // levels per step of the first steps (2 bits each), then `expand`
+    uint8_t step_e[256] = {};   // levels descended by each step of the current query
+    uint32_t local_chunk = 0;   // this query: initial pairs per workgroup of the local walk (0: steps)
+    uint32_t local_off = 0;     // queries left that use steps after a local walk overflowed
     bool fused = false;  // narrow phase inside the BVTT steps (FP32 queries)
     bool unchecked = false;
     bool have_result = false;

@@ -177,6 +177,31 @@ def test_batch_morton63(api, ctx, orc):
     assert np.array_equal(out[0], all_pairs(orc, rs, mats))
 
 
+@pytest.mark.parametrize("fp64", [False, True])
+def test_batch_local_walk(api, ctx, orc, fp64):
+    """The workgroup-local walk (traverse_local = n) over a batch front, FP32
+    fused and FP64-fallback: the same pairs as the oracle, eager and from a
+    captured frame."""
+    rs = small_rigid(n_scenes=6, seed=0x10CA)
+    mats = rs.transforms(2)
+    want = all_pairs(orc, rs, mats, fp64=fp64)
+    assert len(want) > 0
+    b = make_batch(api, ctx, rs)
+    b.set_transforms(mats)
+    b.build()
+    ctx.set_tuning(traverse_local=32)
+    try:
+        say(f"batch local walk fp64={fp64}")
+        b.detect(1, 2, fp64_fallback=fp64)
+        assert np.array_equal(b.pairs(), want)
+        b.frame(1, 2, fp64_fallback=fp64)
+        assert np.array_equal(b.pairs(), want)
+        assert b.stats()["retries"] == 0
+    finally:
+        ctx.set_tuning(traverse_local=0)
+    b.close()
+
+
 def test_batch_self_collision(api, ctx, orc):
     say("batch: cloth self collision")
     cl = [M.cloth(24 + 6 * i, 0.2, 0.25, seed=40 + i) for i in range(3)]

@@ -67,13 +67,42 @@ def test_build_soup_bit_exact(api, ctx, orc, F):
     check_tree(orc, gpu_tree(api, ctx, xyz, f), xyz, f)
 
 
-@pytest.mark.parametrize("lpt", [1, 2, 4, 8])
+@pytest.mark.parametrize("lpt", [1, 2, 4, 8, 16])
 def test_build_subtree_sizes(api, ctx, orc, lpt):
     say(f"build leaves_per_thread={lpt}")
     ctx.set_tuning(reduce_leaves_per_thread=lpt)
     try:
         for xyz, f in [M.icosphere(5, 1.0, 0.02, 3), M.uv_sphere(300, 151), soup(12345, 7)]:
             check_tree(orc, gpu_tree(api, ctx, xyz, f), xyz, f)
+    finally:
+        ctx.set_tuning(reduce_leaves_per_thread=0)
+
+
+def test_wave_reduce_all_heights(api, ctx, orc):
+    """The wave-subtree reduce (tuning 16) across tree heights 0..13: wave
+    subtrees shorter than 8 levels, one or two merged levels, partial waves,
+    build, refit and a scene refit of mixed sizes."""
+    ctx.set_tuning(reduce_leaves_per_thread=16)
+    try:
+        trees = []
+        for F in [1, 2, 3, 5, 100, 255, 256, 257, 511, 513, 1023, 1024, 1025, 2049, 4097, 8193]:
+            say(f"wave reduce F={F}")
+            xyz, f = soup(F, 3000 + F)
+            t = gpu_tree(api, ctx, xyz, f)
+            check_tree(orc, t, xyz, f)
+            x2 = (xyz * np.float32(0.75) + np.float32(0.125)).astype(np.float32)
+            t.refit(x2)
+            assert np.array_equal(bits(t.nodes()), bits(orc.Tree(x2, f, order=t.perm()).nodes()))
+            trees.append((t, x2, f))
+        sc = api.Scene(ctx)
+        for t, _, _ in trees[-4:]:
+            sc.addOibvhTree(t)
+        for t, x2, f in trees[-4:]:
+            t.set_positions(x2 * np.float32(1.5))
+        sc.refit()
+        for t, x2, f in trees[-4:]:
+            x3 = (x2 * np.float32(1.5)).astype(np.float32)
+            assert np.array_equal(bits(t.nodes()), bits(orc.Tree(x3, f, order=t.perm()).nodes()))
     finally:
         ctx.set_tuning(reduce_leaves_per_thread=0)
 
@@ -401,12 +430,46 @@ def test_detect_cfg1_full_size(api, ctx, orc):
 
 @pytest.mark.parametrize("entry", [0, 1, 3, 6, 30])
 @pytest.mark.parametrize("expand", [1, 2, 3, 5])
-def test_detect_entry_expand_invariance(api, ctx, orc, entry, expand):
-    say(f"detect entry={entry} expand={expand}")
+@pytest.mark.parametrize("mode", [0, 64, 7])
+def test_detect_entry_expand_invariance(api, ctx, orc, entry, expand, mode):
+    """mode: traverse_local (0 steps, n the local walk with n initial pairs
+    per workgroup)."""
+    say(f"detect entry={entry} expand={expand} traverse_local={mode}")
     meshes = M.interpenetrating_icospheres(subdiv=4, seed=21, offset=1.3)
     want = orc.detect([orc.Tree(x, f) for x, f in meshes])
-    p, _, _ = gpu_pairs(api, ctx, meshes, entry, expand)
+    ctx.set_tuning(traverse_local=mode)
+    try:
+        p, st, _ = gpu_pairs(api, ctx, meshes, entry, expand)
+    finally:
+        ctx.set_tuning(traverse_local=0)
     assert np.array_equal(p, want)
+    assert st["retries"] == 0
+
+
+def test_detect_local_walk_overflow_falls_back(api, ctx, orc):
+    """Two identical spheres walked from the roots (one workgroup) three
+    levels at a time: every node overlaps its twin and neighbours, so the
+    workgroup's LDS stack overflows; the detect is re-run with steps (same
+    pairs) and the next detects use steps."""
+    say("local walk overflow")
+    a = M.icosphere(5, 1.0, 0.0, 0)
+    want = orc.detect([orc.Tree(*a), orc.Tree(*a)])
+    ctx2 = api.Context(0)
+    ctx2.set_tuning(traverse_local=64)
+    try:
+        trees = [gpu_tree(api, ctx2, *a), gpu_tree(api, ctx2, *a)]
+        sc = api.Scene(ctx2)
+        for t in trees:
+            sc.addOibvhTree(t)
+        sc.detectCollision(0, 3)
+        assert np.array_equal(sc.getIntTriPairs(), want)
+        assert sc.stats()["retries"] >= 1
+        sc.detectCollision(0, 3)
+        assert np.array_equal(sc.getIntTriPairs(), want)
+        assert sc.stats()["retries"] == 0 and len(sc.level_counts()) > 1  # steps now
+        sc.close()
+    finally:
+        ctx2.close()
 
 
 def test_detect_many_meshes(api, ctx, orc):
