@@ -51,6 +51,14 @@ TRAFFIC_REFIT = 133.0e6
 METRIC = "frames/s & Mtri/s (build+refit+broad+narrow), batched vs plain, 0 pair diffs"
 
 
+def reduce_kernel(lpt: int, faces: int) -> str:
+    """Name of the stage-1 reduce kernel the library runs for a launch over
+    `faces` faces with tuning reduce_leaves_per_thread = lpt (0 = by size)."""
+    if lpt == 16 or (lpt == 0 and faces >= (1 << 18)):
+        return "k_reduce_w"
+    return f"k_reduce<{lpt}>" if lpt else "k_reduce<M>"
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -252,20 +260,23 @@ def run_plain(args, ctx):
     refit_bytes = sum(12 * V + 12 * F + 24 * n for F, n in zip(Fm, N))
     detect_bytes = sum(24 * n + 12 * V + 12 * F for F, n in zip(Fm, N)) + 16 * H_pairs
     share = {"refit": refit_ms * ev_refit.i, "detect": detect_ms * ev_detect.i, "build": build_ms * ev_build.i}
+    rk = reduce_kernel(ctx.get_tuning()["reduce_leaves_per_thread"], sum(Fm))
     kernels = {
-        "refit": {"kernel": "k_reduce + k_reduce_apex over both trees (oibvh_scene_refit)", "avg_ms": refit_ms, "launches": ev_refit.i,
+        "refit": {"kernel": f"{rk} + k_reduce_apex over both trees (oibvh_scene_refit)", "avg_ms": refit_ms,
+                  "launches": ev_refit.i,
                   "alg_bytes": refit_bytes, "gbps": refit_bytes / (refit_ms * 1e-3) / 1e9 if refit_ms else None},
-        "detect": {"kernel": "k_seed, k_bvtt_level x levels, k_narrow", "avg_ms": detect_ms, "launches": ev_detect.i,
+        "detect": {"kernel": "k_seed, k_bvtt_step x steps (leaf pairs tested inside the last step)",
+                   "avg_ms": detect_ms, "launches": ev_detect.i,
                    "alg_bytes": detect_bytes,
                    "gbps": detect_bytes / (detect_ms * 1e-3) / 1e9 if detect_ms else None},
         "build": {"kernels": "k_mesh_box, k_morton, 4 x k_sort_pass, k_reduce", "avg_ms": build_ms,
                   "launches": ev_build.i},
     }
-    # the roofline kernel is the refit (k_reduce, 2 launches per frame), the
-    # single kernel with the largest share of the frame; the events bracket
-    # one tree's refit launch pair (k_reduce + k_reduce_apex)
+    # the roofline kernel is the refit's stage-1 reduce, the single kernel
+    # with the largest share of the frame; the events bracket the scene
+    # refit's launch pair (stage 1 + k_reduce_apex, both trees)
     dk = kernels["refit"]
-    roofline = {"bound": "hbm", "kernel": "k_reduce (scene refit of both trees; events bracket k_reduce + k_reduce_apex)",
+    roofline = {"bound": "hbm", "kernel": f"{rk} (scene refit of both trees; events bracket {rk} + k_reduce_apex)",
                 "achieved": dk["gbps"], "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": (dk["gbps"] * 1e9 / HBM_PEAK) if dk["gbps"] else None, "traffic": TRAFFIC_REFIT,
                 "alg_bytes_per_launch": dk["alg_bytes"], "avg_launch_ms": dk["avg_ms"],

@@ -226,11 +226,12 @@ typedef struct {
                                           (step k: bits 2k+1..2k, values 1..3); the first zero field
                                           ends the list and later steps use the uniform setting
                                           (0 = uniform) */
-    uint32_t traverse_local;           /* 0: BVTT steps (one launch per step, each spread over the whole
-                                          GPU); n >= 1: walk the BVTT inside workgroups of n initial
-                                          pairs each, depth first with the pairs on an LDS stack (one
-                                          launch), falling back to steps for the next 64 detects when a
-                                          workgroup's stack outgrows LDS */
+    uint32_t traverse_local;           /* 0: BVTT steps only (one launch per step, each spread over the
+                                          whole GPU); n >= 1: the first steps as one launch in which
+                                          workgroups of n initial pairs walk breadth first with their
+                                          fronts in LDS, falling back to steps only for the next 64
+                                          detects when a workgroup's front outgrows LDS */
+    uint32_t traverse_local_levels;    /* steps the local walk replaces (0 = all of them) */
 } oibvh_tuning;
 
 /* The detect buffers grow on overflow (the detect is re-run). */

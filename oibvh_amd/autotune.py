@@ -42,10 +42,12 @@ def _time_refit(ctx: api.Context, frame, reps: int) -> float:
 
 
 def tune(ctx: api.Context, scene: api.Scene, frame, *, entries=(4, 6, 8), expands=(1, 2, 3),
-         blocks=(0, 2048), leaves=(16, 2, 4, 8), reps: int = 5) -> dict:
+         blocks=(0, 2048), leaves=(16, 2, 4, 8), local=((64, 2), (64, 3), (256, 3)), reps: int = 5) -> dict:
     """Return {'entry_level', 'expand_levels', 'traverse_blocks',
-    'reduce_leaves_per_thread', 'ms'} of the fastest setting and apply the
-    tuning knobs to ctx."""
+    'reduce_leaves_per_thread', 'traverse_local', 'traverse_local_levels',
+    'ms'} of the fastest setting and apply the tuning knobs to ctx. `local`
+    lists the (initial pairs per workgroup, steps) settings of the local walk
+    tried at the best step shape."""
     best = None
     # reduce variant first, timed on the refit alone (the detect does not
     # depend on it), then the traversal shape
@@ -58,19 +60,29 @@ def tune(ctx: api.Context, scene: api.Scene, frame, *, entries=(4, 6, 8), expand
             lpt_best, lpt_ms = lpt, ms
     ctx.set_tuning(reduce_leaves_per_thread=lpt_best)
     for entry, expand, nb in itertools.product(entries, expands, blocks):
-        ctx.set_tuning(traverse_blocks=nb, traverse_expand=expand)
+        ctx.set_tuning(traverse_blocks=nb, traverse_expand=expand, traverse_local=0, traverse_local_levels=0)
         ms = _time(ctx, scene, frame, entry, expand, reps)
         if best is None or ms < best["ms"]:
             best = {"entry_level": entry, "expand_levels": expand, "traverse_blocks": nb,
-                    "reduce_leaves_per_thread": lpt_best, "ms": ms}
+                    "reduce_leaves_per_thread": lpt_best, "traverse_local": 0, "traverse_local_levels": 0,
+                    "ms": ms}
+    # then the first steps as one workgroup-local launch, at the best shape
+    ctx.set_tuning(traverse_blocks=best["traverse_blocks"], traverse_expand=best["expand_levels"])
+    for chunk, levels in local:
+        ctx.set_tuning(traverse_local=chunk, traverse_local_levels=levels)
+        ms = _time(ctx, scene, frame, best["entry_level"], best["expand_levels"], reps)
+        if ms < best["ms"]:
+            best = {**best, "traverse_local": chunk, "traverse_local_levels": levels, "ms": ms}
     ctx.set_tuning(**{**{k: base[k] for k in base},
                       "reduce_leaves_per_thread": best["reduce_leaves_per_thread"],
-                      "traverse_blocks": best["traverse_blocks"], "traverse_expand": best["expand_levels"]})
+                      "traverse_blocks": best["traverse_blocks"], "traverse_expand": best["expand_levels"],
+                      "traverse_local": best["traverse_local"],
+                      "traverse_local_levels": best["traverse_local_levels"]})
     return best
 
 
 def tune_batch(ctx: api.Context, batch: api.Batch, before_frame, *, entries=(0, 2, 4), expands=(1, 2, 3),
-               blocks=(0, 4096), leaves=(0, 4, 8), flags=None, reps: int = 3) -> dict:
+               blocks=(0, 4096), leaves=(0, 4, 8), local=((64, 2), (256, 3)), flags=None, reps: int = 3) -> dict:
     """Pick the reduce subtree size (leaves per thread), the BVTT entry level,
     the levels per step and the workgroups per launch for a batch's captured
     frame (refit + detect). before_frame(i) stages frame i's inputs (model
@@ -101,11 +113,20 @@ def tune_batch(ctx: api.Context, batch: api.Batch, before_frame, *, entries=(0, 
     ctx.set_tuning(reduce_leaves_per_thread=lpt_best)
     best = None
     for entry, expand, nb in itertools.product(entries, expands, blocks):
-        ctx.set_tuning(traverse_blocks=nb, traverse_expand=expand)
+        ctx.set_tuning(traverse_blocks=nb, traverse_expand=expand, traverse_local=0, traverse_local_levels=0)
         ms = run(entry, expand)
         if best is None or ms < best["ms"]:
             best = {"entry_level": entry, "expand_levels": expand, "traverse_blocks": nb,
-                    "reduce_leaves_per_thread": lpt_best, "ms": ms}
+                    "reduce_leaves_per_thread": lpt_best, "traverse_local": 0, "traverse_local_levels": 0,
+                    "ms": ms}
+    ctx.set_tuning(traverse_blocks=best["traverse_blocks"], traverse_expand=best["expand_levels"])
+    for chunk, levels in local:
+        ctx.set_tuning(traverse_local=chunk, traverse_local_levels=levels)
+        ms = run(best["entry_level"], best["expand_levels"])
+        if ms < best["ms"]:
+            best = {**best, "traverse_local": chunk, "traverse_local_levels": levels, "ms": ms}
     ctx.set_tuning(**{**base, "reduce_leaves_per_thread": best["reduce_leaves_per_thread"],
-                      "traverse_blocks": best["traverse_blocks"], "traverse_expand": best["expand_levels"]})
+                      "traverse_blocks": best["traverse_blocks"], "traverse_expand": best["expand_levels"],
+                      "traverse_local": best["traverse_local"],
+                      "traverse_local_levels": best["traverse_local_levels"]})
     return best

@@ -639,7 +639,8 @@ extern "C" oibvh_status oibvh_batch_frame(oibvh_batch* b, uint32_t entry, uint32
     d.n_pairs = 0;
     d.stats = oibvh_detect_stats{};
     if ((r = prepare_detect(b, entry, expand, flags & ~OIBVH_DETECT_ASYNC)) != OIBVH_OK) return r;
-    const uint64_t key[6] = {entry | ((uint64_t)d.local_chunk << 32), d.expand | ((uint64_t)d.schedule << 32),
+    const uint64_t key[6] = {entry | ((uint64_t)d.local_chunk << 32) | ((uint64_t)d.local_levels << 48),
+                             d.expand | ((uint64_t)d.schedule << 32),
                              flags & ~OIBVH_DETECT_ASYNC,
                              (uint64_t)b->reduce_M,
                              d.generation, (uint64_t)b->desc_gen ^ ((uint64_t)oi_grid_for(c) << 32)};

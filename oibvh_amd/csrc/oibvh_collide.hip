@@ -711,18 +711,16 @@ __global__ void __launch_bounds__(TB) k_bvtt_step(StepArgs<Tab> A)
 }
 
 // ---------------------------------------------------------------------------
-// k_bvtt_local: the whole walk below a slice of the initial front inside one
-// workgroup. Workgroup b takes initial entries [b * chunk, (b + 1) * chunk),
-// tests and expands them (as the first step does) and keeps the surviving
-// pairs on a stack in LDS; every round pops up to 256 pairs from the top,
-// expands them E levels and pushes the kept non-leaf pairs back, so the
-// deepest pairs go first and the stack stays short (depth-first order bounds
-// it by the walk's depth rather than by the width of a level). Leaf pairs go
-// through the fused SAT (or to the candidate list). One launch replaces the
-// chain of steps; a workgroup whose stack outgrows LDS sets
-// OI_ERR_LOCAL_OVERFLOW and the detect is run again with steps.
+// k_bvtt_local: the first J steps of the walk inside workgroups. Workgroup b
+// takes initial entries [b * chunk, (b + 1) * chunk) and walks them J levels
+// of E (J x E tree levels) breadth first with its front in LDS, testing leaf
+// pairs as it meets them (fused SAT, or the candidate list); the pairs kept
+// by the J-th level go to the global front, where the steps take over. The
+// top of the walk, whose fronts are small, so costs one launch instead of J.
+// A workgroup whose front outgrows its LDS queue before level J sets
+// OI_ERR_LOCAL_OVERFLOW and the detect is run again with steps only.
 // ---------------------------------------------------------------------------
-#define LQ 4096
+#define LQ 2048
 
 template <class Tab>
 struct LocalArgs {
@@ -730,6 +728,10 @@ struct LocalArgs {
     const uint2* front_in;
     uint32_t n_in;
     uint32_t chunk;
+    uint32_t levels;     // J >= 1
+    uint2* front_out;    // pairs kept by level J
+    uint32_t* count_out;
+    uint32_t cap_out;
     uint2* cands;
     uint32_t* cand_count;
     uint32_t cand_cap;
@@ -749,96 +751,106 @@ __global__ void __launch_bounds__(TB) k_bvtt_local(LocalArgs<Tab> A)
     __shared__ oi_mesh_desc s_desc[MESH_LDS];
     __shared__ uint32_t s_ibase[MESH_LDS];
     __shared__ uint32_t s_wsum[4];
-    __shared__ uint32_t s_top, s_obase, s_cbase;
-    __shared__ uint2 s_q[LQ];
+    __shared__ uint32_t s_qn, s_obase, s_cbase;
+    __shared__ uint2 s_q[2][LQ];
     __shared__ std::conditional_t<FUSED, FusedLds, char> s_f[1];
     const uint32_t c0 = blockIdx.x * A.chunk;
     if (c0 >= A.n_in) return;
     tab_init(A.tab, s_desc, s_ibase);
     const uint32_t tid = threadIdx.x;
     uint32_t n_tests = 0, n_adj = 0, n_cands = 0, n_fp64 = 0;
-    const uint32_t n_first = min(A.chunk, A.n_in - c0);
-    if (tid == 0) s_top = 0;
-    __syncthreads();
+    uint32_t n_cur = min(A.chunk, A.n_in - c0);
+    uint32_t cur = 0;
     bool overflow = false;
-    uint32_t done_first = 0;  // initial entries consumed
-    uint32_t top = 0;         // stack height (uniform)
-    while (done_first < n_first || top > 0) {
-        // a round: the next initial entries while any are left, else the top of the stack
-        const bool first = done_first < n_first;
-        const uint32_t n = first ? min((uint32_t)TB, n_first - done_first) : min((uint32_t)TB, top);
-        const uint32_t start = first ? top : top - n;  // where this round's pushes begin
-        bool have = tid < n;
-        uint2 e = make_uint2(0, 0);
-        if (have) e = first ? A.front_in[c0 + done_first + tid] : s_q[start + tid];
-        if (have && (e.x >= A.implicit_end || e.y >= A.implicit_end)) {
-            or_u32(A.error, OI_ERR_CORRUPT);
-            have = false;
-        }
-        u64 mask = 0;
-        bool leafout = false;
-        Side sa = {0, 0}, sb = {0, 0};
-        uint32_t baseA = 0, baseB = 0;
-        if (have) {
-            if (first)
-                entry_expand<E, true, Tab>(A.tab, s_desc, s_ibase, e, mask, leafout, sa, sb, baseA, baseB, n_tests,
-                                           n_adj);
-            else
-                entry_expand<E, false, Tab>(A.tab, s_desc, s_ibase, e, mask, leafout, sa, sb, baseA, baseB,
-                                            n_tests, n_adj);
-        }
-        const uint32_t nout = (uint32_t)__popcll(mask);
-        // one scan for both counts (front pairs high, leaf pairs low); its
-        // barriers also order every read of the popped slots before the
-        // pushes that overwrite them
-        uint32_t TC;
-        const uint32_t oc = block_scan(leafout ? nout : (nout << 16), s_wsum, TC);
-        const uint32_t T = TC >> 16, C = TC & 0xffffu, off = oc >> 16, coff = oc & 0xffffu;
-        if (tid == 0) {
-            s_obase = start;
-            s_top = (start + T < LQ) ? start + T : LQ;
-            if (!FUSED) s_cbase = C ? add_u32(A.cand_count, C) : 0u;
-        }
-        if (start + T > LQ) overflow = true;
-        if constexpr (FUSED) {
-            s_f[0].cmask[tid] = leafout ? mask : 0ull;
-            s_f[0].coff[tid] = coff;
-            s_f[0].ga[tid] = baseA + sa.start;
-            s_f[0].gb[tid] = baseB + sb.start;
-        }
+    for (uint32_t lvl = 0; n_cur; lvl++) {
+        const bool first = lvl == 0;
+        const bool last = lvl + 1 >= A.levels;  // kept pairs go to the global front
+        if (tid == 0) s_qn = 0;
         __syncthreads();
-        if (nout && !leafout) {
-            uint32_t o = s_obase + off;
-            u64 m = mask;
-            while (m) {
-                const uint32_t k = (uint32_t)__ffsll((long long)m) - 1u;
-                m &= m - 1ull;
-                if (o < LQ) s_q[o] = make_uint2(baseA + sa.start + (k >> E), baseB + sb.start + (k & (K - 1)));
-                o++;
+        for (uint32_t base = 0; base < n_cur; base += TB) {
+            const uint32_t i = base + tid;
+            bool have = i < n_cur;
+            uint2 e = make_uint2(0, 0);
+            if (have) e = first ? A.front_in[c0 + i] : s_q[cur][i];
+            if (have && (e.x >= A.implicit_end || e.y >= A.implicit_end)) {
+                or_u32(A.error, OI_ERR_CORRUPT);
+                have = false;
             }
-        }
-        if (!FUSED && nout && leafout) {
-            uint32_t o = s_cbase + coff;
-            if (o + nout > A.cand_cap) or_u32(A.error, OI_ERR_CAND_OVERFLOW);
-            u64 m = mask;
-            while (m) {
-                const uint32_t k = (uint32_t)__ffsll((long long)m) - 1u;
-                m &= m - 1ull;
-                if (o < A.cand_cap)
-                    A.cands[o] = make_uint2(baseA + sa.start + (k >> E), baseB + sb.start + (k & (K - 1)));
-                o++;
+            u64 mask = 0;
+            bool leafout = false;
+            Side sa = {0, 0}, sb = {0, 0};
+            uint32_t baseA = 0, baseB = 0;
+            if (have) {
+                if (first)
+                    entry_expand<E, true, Tab>(A.tab, s_desc, s_ibase, e, mask, leafout, sa, sb, baseA, baseB,
+                                               n_tests, n_adj);
+                else
+                    entry_expand<E, false, Tab>(A.tab, s_desc, s_ibase, e, mask, leafout, sa, sb, baseA, baseB,
+                                                n_tests, n_adj);
             }
-        }
-        if constexpr (FUSED) {
-            if (C) {
-                if (tid == 0) n_cands += C;
-                fused_narrow<E, Tab>(A.tab, s_ibase, s_desc, s_f[0], C, s_wsum, A.hits, A.n_hits, A.hits_cap,
-                                     A.error, n_fp64, n_adj);
+            const uint32_t nout = (uint32_t)__popcll(mask);
+            // one scan for both counts (front pairs high, leaf pairs low)
+            uint32_t TC;
+            const uint32_t oc = block_scan(leafout ? nout : (nout << 16), s_wsum, TC);
+            const uint32_t T = TC >> 16, C = TC & 0xffffu, off = oc >> 16, coff = oc & 0xffffu;
+            if (tid == 0) {
+                if (last) {
+                    s_obase = T ? add_u32(A.count_out, T) : 0u;
+                } else {
+                    s_obase = s_qn;
+                    s_qn += T;
+                }
+                if (!FUSED) s_cbase = C ? add_u32(A.cand_count, C) : 0u;
             }
+            if constexpr (FUSED) {
+                s_f[0].cmask[tid] = leafout ? mask : 0ull;
+                s_f[0].coff[tid] = coff;
+                s_f[0].ga[tid] = baseA + sa.start;
+                s_f[0].gb[tid] = baseB + sb.start;
+            }
+            __syncthreads();
+            if (nout && !leafout) {
+                uint32_t o = s_obase + off;
+                if (last && o + nout > A.cap_out) or_u32(A.error, OI_ERR_QUEUE_OVERFLOW);
+                u64 m = mask;
+                while (m) {
+                    const uint32_t k = (uint32_t)__ffsll((long long)m) - 1u;
+                    m &= m - 1ull;
+                    const uint2 pr = make_uint2(baseA + sa.start + (k >> E), baseB + sb.start + (k & (K - 1)));
+                    if (last) {
+                        if (o < A.cap_out) A.front_out[o] = pr;
+                    } else if (o < LQ) {
+                        s_q[cur ^ 1u][o] = pr;
+                    }
+                    o++;
+                }
+            }
+            if (!FUSED && nout && leafout) {
+                uint32_t o = s_cbase + coff;
+                if (o + nout > A.cand_cap) or_u32(A.error, OI_ERR_CAND_OVERFLOW);
+                u64 m = mask;
+                while (m) {
+                    const uint32_t k = (uint32_t)__ffsll((long long)m) - 1u;
+                    m &= m - 1ull;
+                    if (o < A.cand_cap)
+                        A.cands[o] = make_uint2(baseA + sa.start + (k >> E), baseB + sb.start + (k & (K - 1)));
+                    o++;
+                }
+            }
+            if constexpr (FUSED) {
+                if (C) {
+                    if (tid == 0) n_cands += C;
+                    fused_narrow<E, Tab>(A.tab, s_ibase, s_desc, s_f[0], C, s_wsum, A.hits, A.n_hits, A.hits_cap,
+                                         A.error, n_fp64, n_adj);
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        top = s_top;
-        if (first) done_first += n;
+        if (last) break;
+        const uint32_t q = s_qn;
+        if (q > LQ) overflow = true;
+        n_cur = q < LQ ? q : LQ;
+        cur ^= 1u;
         __syncthreads();
     }
     if (tid == 0 && overflow) or_u32(A.error, OI_ERR_LOCAL_OVERFLOW);
@@ -1037,6 +1049,7 @@ static hipError_t launch_step(hipStream_t st, uint32_t grid, const Tab& tab, int
 
 template <class Tab>
 static hipError_t launch_local(hipStream_t st, const Tab& tab, const uint2* front_in, uint32_t n_in, uint32_t chunk,
+                               uint32_t levels, uint2* front_out, uint32_t* count_out, uint32_t cap_out,
                                uint2* cands, uint32_t* cand_count, uint32_t cand_cap, uint4* hits,
                                uint32_t hits_cap, oi_detect_ctl* ctl, uint32_t expand, uint32_t implicit_end,
                                int fused)
@@ -1047,6 +1060,10 @@ static hipError_t launch_local(hipStream_t st, const Tab& tab, const uint2* fron
     a.front_in = front_in;
     a.n_in = n_in;
     a.chunk = chunk;
+    a.levels = levels;
+    a.front_out = front_out;
+    a.count_out = count_out;
+    a.cap_out = cap_out;
     a.cands = cands;
     a.cand_count = cand_count;
     a.cand_cap = cand_cap;
@@ -1124,18 +1141,22 @@ hipError_t oi_launch_bvtt_step(hipStream_t st, uint32_t grid, const oi_tab_args*
                        expand, implicit_end, fused);
 }
 
-// the whole walk in one launch, one workgroup per `chunk` initial entries
+// the first `levels` steps in one launch, one workgroup per `chunk` initial
+// entries; the pairs kept by the last of them go to front_out / count_out
 hipError_t oi_launch_bvtt_local(hipStream_t st, const oi_tab_args* t, const uint2* front_in, uint32_t n_in,
-                                uint32_t chunk, uint2* cands, uint32_t* cand_count, uint32_t cand_cap, uint4* hits,
-                                uint32_t hits_cap, oi_detect_ctl* ctl, uint32_t expand, uint32_t implicit_end,
-                                int fused)
+                                uint32_t chunk, uint32_t levels, uint2* front_out, uint32_t* count_out,
+                                uint32_t cap_out, uint2* cands, uint32_t* cand_count, uint32_t cand_cap,
+                                uint4* hits, uint32_t hits_cap, oi_detect_ctl* ctl, uint32_t expand,
+                                uint32_t implicit_end, int fused)
 {
-    if (chunk == 0) return hipErrorInvalidValue;
+    if (chunk == 0 || levels == 0) return hipErrorInvalidValue;
     if (t->chunk_mesh)
-        return launch_local(st, ChunkTab{t->d_meshes, t->chunk_mesh, t->shift}, front_in, n_in, chunk, cands,
-                            cand_count, cand_cap, hits, hits_cap, ctl, expand, implicit_end, fused);
-    return launch_local(st, make_tab(t->d_meshes, t->h_meshes, t->n_meshes), front_in, n_in, chunk, cands,
-                        cand_count, cand_cap, hits, hits_cap, ctl, expand, implicit_end, fused);
+        return launch_local(st, ChunkTab{t->d_meshes, t->chunk_mesh, t->shift}, front_in, n_in, chunk, levels,
+                            front_out, count_out, cap_out, cands, cand_count, cand_cap, hits, hits_cap, ctl, expand,
+                            implicit_end, fused);
+    return launch_local(st, make_tab(t->d_meshes, t->h_meshes, t->n_meshes), front_in, n_in, chunk, levels,
+                        front_out, count_out, cap_out, cands, cand_count, cand_cap, hits, hits_cap, ctl, expand,
+                        implicit_end, fused);
 }
 
 // Scene::convertToVertexArray (scene.cu:69-93): six vertices per hit, the

@@ -80,12 +80,23 @@ oibvh_status oi_det_prepare(oi_detector* d)
         done += e;
     }
     d->n_levels = k;
-    // the local walk (one launch) when asked for, unless it overflowed recently
+    // the first steps as one local walk when asked for, unless it overflowed
+    // recently; its steps all descend `expand` levels
     const uint32_t mode = c->trav_local;
     d->local_chunk = 0;
-    if (mode && d->local_off == 0 && d->n_front) d->local_chunk = mode;
+    d->local_levels = 0;
+    if (mode && d->local_off == 0 && d->n_front) {
+        d->local_chunk = mode;
+        k = 0;
+        done = 0;
+        while (k == 0 || done < d->depth) {
+            d->step_e[k++] = (uint8_t)d->expand;
+            done += d->expand;
+        }
+        d->n_levels = k;
+        d->local_levels = c->trav_local_levels ? std::min(c->trav_local_levels, k) : k;
+    }
     if (d->local_off) d->local_off--;
-    if (d->local_chunk) d->n_levels = 1;
     const uint32_t fcap = std::max(c->queue_cap, oi_np2(std::max(1u, d->n_front)));
     if (d->front_cap < fcap) {
         dfree(d->front[0]);
@@ -113,12 +124,14 @@ oibvh_status oi_det_enqueue(oi_detector* d, bool reset_sticky)
     // its own narrow pass (its registers would slow every step)
     const bool fp64 = (d->flags & OIBVH_DETECT_FP64_FALLBACK) != 0;
     d->fused = !fp64;
-    if (d->local_chunk) {
-        HIPCHK(c, oi_launch_bvtt_local(st, &d->tab, d->d_init, d->n_front, d->local_chunk, d->cands, &d->ctl->cands,
-                                       d->cand_cap, d->hits, d->hits_cap, d->ctl, d->expand, d->implicit_end,
-                                       d->fused ? 1 : 0));
+    const uint32_t J = d->local_chunk ? d->local_levels : 0u;
+    if (J) {
+        // steps 0 .. J-1 in one launch; its front goes where step J - 1's would
+        HIPCHK(c, oi_launch_bvtt_local(st, &d->tab, d->d_init, d->n_front, d->local_chunk, J, d->front[(J - 1) & 1],
+                                       d->counts + J, d->front_cap, d->cands, &d->ctl->cands, d->cand_cap, d->hits,
+                                       d->hits_cap, d->ctl, d->expand, d->implicit_end, d->fused ? 1 : 0));
     }
-    for (uint32_t k = 0; k < (d->local_chunk ? 0u : d->n_levels); k++) {
+    for (uint32_t k = J; k < d->n_levels; k++) {
         // step 0 reads the initial front in place; then the fronts alternate
         const uint2* in = k ? d->front[(k - 1) & 1] : d->d_init;
         HIPCHK(c, oi_launch_bvtt_step(st, grid, &d->tab, k == 0, in, k ? d->counts + k : nullptr, d->n_front,

@@ -154,6 +154,7 @@ extern "C" oibvh_status oibvh_ctx_set_tuning(oibvh_ctx* ctx, const oibvh_tuning*
     ctx->hits_cap = t->hits_capacity ? t->hits_capacity : (1u << 20);
     ctx->trav_schedule = t->traverse_schedule;
     ctx->trav_local = t->traverse_local;
+    ctx->trav_local_levels = t->traverse_local_levels;
     return OIBVH_OK;
 }
 
@@ -167,6 +168,7 @@ extern "C" oibvh_status oibvh_ctx_get_tuning(const oibvh_ctx* ctx, oibvh_tuning*
     t->candidate_capacity = ctx->cand_cap;
     t->traverse_schedule = ctx->trav_schedule;
     t->traverse_local = ctx->trav_local;
+    t->traverse_local_levels = ctx->trav_local_levels;
     t->hits_capacity = ctx->hits_cap;
     return OIBVH_OK;
 }

@@ -46,8 +46,9 @@ hipError_t oi_launch_bvtt_step(hipStream_t, uint32_t, const oi_tab_args*, int, c
                                uint32_t, oi_detect_ctl*, uint32_t, uint32_t, int);
 hipError_t oi_launch_narrow(hipStream_t, uint32_t, const oi_tab_args*, const uint2*, const uint32_t*, uint32_t,
                             uint4*, uint32_t*, uint32_t, oi_detect_ctl*, int);
-hipError_t oi_launch_bvtt_local(hipStream_t, const oi_tab_args*, const uint2*, uint32_t, uint32_t, uint2*, uint32_t*,
-                                uint32_t, uint4*, uint32_t, oi_detect_ctl*, uint32_t, uint32_t, int);
+hipError_t oi_launch_bvtt_local(hipStream_t, const oi_tab_args*, const uint2*, uint32_t, uint32_t, uint32_t, uint2*,
+                                uint32_t*, uint32_t, uint2*, uint32_t*, uint32_t, uint4*, uint32_t, oi_detect_ctl*,
+                                uint32_t, uint32_t, int);
 hipError_t oi_launch_pair_vertices(hipStream_t, const oi_build_desc*, const uint4*, uint32_t, float*);
 }
 
@@ -63,7 +64,8 @@ struct oibvh_ctx {
     uint32_t trav_blocks = 0;
     uint32_t trav_expand = 0;
     uint32_t trav_schedule = 0;
-    uint32_t trav_local = 0;  // 0: BVTT steps; n: the local walk with n initial pairs per workgroup
+    uint32_t trav_local = 0;         // 0: BVTT steps only; n: the first steps as a local walk, n pairs per workgroup
+    uint32_t trav_local_levels = 0;  // steps the local walk replaces (0: all)
     uint32_t queue_cap = 1u << 22;
     uint32_t hits_cap = 1u << 20;
     uint32_t cand_cap = 1u << 20;
@@ -141,6 +143,7 @@ struct oi_detector {
     uint32_t schedule = 0;      // levels per step of the first steps (2 bits each), then `expand`
     uint8_t step_e[256] = {};   // levels descended by each step of the current query
     uint32_t local_chunk = 0;   // this query: initial pairs per workgroup of the local walk (0: steps)
+    uint32_t local_levels = 0;  // this query: steps done by the local walk
     uint32_t local_off = 0;     // queries left that use steps after a local walk overflowed
     bool fused = false;  // narrow phase inside the BVTT steps (FP32 queries)
     bool unchecked = false;

@@ -177,8 +177,8 @@ def test_batch_morton63(api, ctx, orc):
     assert np.array_equal(out[0], all_pairs(orc, rs, mats))
 
 
-@pytest.mark.parametrize("fp64", [False, True])
-def test_batch_local_walk(api, ctx, orc, fp64):
+@pytest.mark.parametrize("fp64,levels", [(False, 0), (True, 0), (False, 2)])
+def test_batch_local_walk(api, ctx, orc, fp64, levels):
     """The workgroup-local walk (traverse_local = n) over a batch front, FP32
     fused and FP64-fallback: the same pairs as the oracle, eager and from a
     captured frame."""
@@ -189,16 +189,16 @@ def test_batch_local_walk(api, ctx, orc, fp64):
     b = make_batch(api, ctx, rs)
     b.set_transforms(mats)
     b.build()
-    ctx.set_tuning(traverse_local=32)
+    ctx.set_tuning(traverse_local=32, traverse_local_levels=levels)
     try:
-        say(f"batch local walk fp64={fp64}")
+        say(f"batch local walk fp64={fp64} levels={levels}")
         b.detect(1, 2, fp64_fallback=fp64)
         assert np.array_equal(b.pairs(), want)
         b.frame(1, 2, fp64_fallback=fp64)
         assert np.array_equal(b.pairs(), want)
         assert b.stats()["retries"] == 0
     finally:
-        ctx.set_tuning(traverse_local=0)
+        ctx.set_tuning(traverse_local=0, traverse_local_levels=0)
     b.close()
 
 

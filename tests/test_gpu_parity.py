@@ -430,18 +430,19 @@ def test_detect_cfg1_full_size(api, ctx, orc):
 
 @pytest.mark.parametrize("entry", [0, 1, 3, 6, 30])
 @pytest.mark.parametrize("expand", [1, 2, 3, 5])
-@pytest.mark.parametrize("mode", [0, 64, 7])
+@pytest.mark.parametrize("mode", [(0, 0), (64, 0), (7, 0), (64, 2), (16, 1)])
 def test_detect_entry_expand_invariance(api, ctx, orc, entry, expand, mode):
-    """mode: traverse_local (0 steps, n the local walk with n initial pairs
-    per workgroup)."""
+    """mode: (traverse_local, traverse_local_levels): steps only, or the
+    first steps (all of them, 2 or 1) as a local walk of n initial pairs per
+    workgroup."""
     say(f"detect entry={entry} expand={expand} traverse_local={mode}")
     meshes = M.interpenetrating_icospheres(subdiv=4, seed=21, offset=1.3)
     want = orc.detect([orc.Tree(x, f) for x, f in meshes])
-    ctx.set_tuning(traverse_local=mode)
+    ctx.set_tuning(traverse_local=mode[0], traverse_local_levels=mode[1])
     try:
         p, st, _ = gpu_pairs(api, ctx, meshes, entry, expand)
     finally:
-        ctx.set_tuning(traverse_local=0)
+        ctx.set_tuning(traverse_local=0, traverse_local_levels=0)
     assert np.array_equal(p, want)
     assert st["retries"] == 0
 
