@@ -69,6 +69,10 @@ def parse():
                    help="BVTT entry level; default: picked by the autotuner during warm-up")
     p.add_argument("--expand-levels", type=int, default=None,
                    help="levels descended per BVTT step; default: picked by the autotuner")
+    p.add_argument("--traverse-local", type=int, default=None,
+                   help="with --entry-level/--expand-levels: initial pairs per workgroup of the local walk (0: steps)")
+    p.add_argument("--traverse-local-levels", type=int, default=0,
+                   help="steps the local walk replaces (0: all)")
     p.add_argument("--rebuild-every", type=int, default=50)
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -140,6 +144,8 @@ def run_plain(args, ctx):
     from oibvh_amd import meshgen as M
 
     K, W = args.steps, args.warmup
+    if args.traverse_local is not None:
+        ctx.set_tuning(traverse_local=args.traverse_local, traverse_local_levels=args.traverse_local_levels)
     if args.workload == "deforming-uv-pair":
         meshes, waves = M.deforming_uv_pair()
         n_frames = 200

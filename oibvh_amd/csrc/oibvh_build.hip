@@ -887,9 +887,10 @@ __global__ void __launch_bounds__(OI_BLOCK) k_reduce_w(ReduceArgs A)
         uint32_t cc = oi_lvl_cnt(lw, L, vl);
         Box6 m[2];
         const uint32_t n1 = 1u << (hb - 1u);  // nodes of the first merged level
-        for (uint32_t t = 0; t < n1; t++) {
+#pragma unroll
+        for (uint32_t t = 0; t < 2u; t++) {
             const uint32_t q = (it.y << (hb - 1u)) + t;
-            if (2u * q < cc) {
+            if (t < n1 && 2u * q < cc) {
                 Box6 x = get_box(S[2u * t], 0);
                 if (2u * q + 1u < cc) box_merge_into(x, get_box(S[2u * t + 1u], 0));
                 m[t] = x;
