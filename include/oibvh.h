@@ -244,6 +244,31 @@ OIBVH_API oibvh_status oibvh_ctx_get_tuning(const oibvh_ctx* ctx, oibvh_tuning* 
  * positions and transform. */
 OIBVH_API oibvh_status oibvh_scene_refit(oibvh_scene* scene);
 
+/* One frame of the reference's loop (main.cpp:284: every tree refit, then
+ * Scene::detectCollision) in one call: tree i first borrows device vertex
+ * memory positions[i] (as oibvh_tree_set_positions with
+ * OIBVH_PTR_DEVICE_BORROW; a NULL entry, or positions == NULL, keeps the
+ * current positions), then oibvh_scene_refit, then oibvh_scene_detect with
+ * entry_level, expand_levels and flags. */
+OIBVH_API oibvh_status oibvh_scene_frame(oibvh_scene* scene, const float* const* positions, uint32_t entry_level,
+                                         uint32_t expand_levels, uint32_t flags);
+
+/* Timing of a scene's work with HIP events on the context's stream (no
+ * reference counterpart; the reference times with cudaEvents around its own
+ * kernels, oibvhTree.cu:300-355, scene.cu:262-433). While enabled, every
+ * oibvh_scene_refit records an event pair around its stage-1 reduce launch
+ * (k_reduce_w or k_reduce) and every detect one around its launches.
+ * oibvh_scene_get_timing waits for the recorded events, sums their durations
+ * and starts the counts afresh. */
+typedef struct {
+    double refit_ms;   /* summed stage-1 reduce durations */
+    double detect_ms;  /* summed detect durations (first launch to last) */
+    uint32_t refits;
+    uint32_t detects;
+} oibvh_scene_timing;
+OIBVH_API oibvh_status oibvh_scene_set_timing(oibvh_scene* scene, uint32_t enable);
+OIBVH_API oibvh_status oibvh_scene_get_timing(oibvh_scene* scene, oibvh_scene_timing* out);
+
 /* BVTT front length of each level of the last detect (level 0 = the initial
  * front); *n receives the number of levels. Diagnostics; the reference prints
  * these (scene.cu:274-276). */

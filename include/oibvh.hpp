@@ -217,6 +217,15 @@ public:
         ctx_->check(oibvh_scene_detect(h_, entryLevel, expandLevels, flags), "Scene::detectCollision");
     }
 
+    // one frame of main.cpp's loop: tree i borrows device vertices
+    // positions[i] (nullptr keeps its positions), every tree is refit, then
+    // detectCollision
+    void frame(const float* const* positions, const unsigned int entryLevel = 0, const unsigned int expandLevels = 1,
+               uint32_t flags = 0)
+    {
+        ctx_->check(oibvh_scene_frame(h_, positions, entryLevel, expandLevels, flags), "Scene::frame");
+    }
+
     // getIntTriPairCount (scene.cuh:61)
     unsigned int getIntTriPairCount() const
     {

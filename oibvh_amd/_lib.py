@@ -39,6 +39,11 @@ class Tuning(C.Structure):
                 ("traverse_local", C.c_uint32), ("traverse_local_levels", C.c_uint32)]
 
 
+class SceneTiming(C.Structure):
+    _fields_ = [("refit_ms", C.c_double), ("detect_ms", C.c_double), ("refits", C.c_uint32),
+                ("detects", C.c_uint32)]
+
+
 class DetectStats(C.Structure):
     _fields_ = [("bvtt_tests", C.c_uint64), ("candidates", C.c_uint64), ("fp64_rechecks", C.c_uint64),
                 ("adjacent", C.c_uint64), ("pairs", C.c_uint32), ("retries", C.c_uint32),
@@ -76,6 +81,9 @@ API = [
     ("oibvh_scene_detect", _i, [_vp, _u32, _u32, _u32]),
     ("oibvh_scene_get_pair_count", _i, [_vp, _u32p]),
     ("oibvh_scene_get_pairs", _i, [_vp, _u32p, _u64, _u32p]),
+    ("oibvh_scene_frame", _i, [_vp, C.POINTER(_vp), _u32, _u32, _u32]),
+    ("oibvh_scene_set_timing", _i, [_vp, _u32]),
+    ("oibvh_scene_get_timing", _i, [_vp, C.c_void_p]),
     ("oibvh_scene_hits_device", _i, [_vp, C.POINTER(_vp), _u32p]),
     ("oibvh_scene_get_pair_vertices", _i, [_vp, _vp, _u64, _u32, _u32p]),
     ("oibvh_scene_get_stats", _i, [_vp, C.POINTER(DetectStats)]),

@@ -300,6 +300,27 @@ class Scene:
                                                            C.byref(got)), "get_pairs")
         return out[: got.value]
 
+    def frame(self, positions=None, entry_level: int = 0, expand_levels: int = 1, *, self_collide: bool = False,
+              fp64_fallback: bool = False, asynchronous: bool = False) -> None:
+        """One frame: tree i borrows device vertex memory positions[i] (an int
+        device address, or None to keep its positions), every tree is refit,
+        then detectCollision (oibvh_scene_frame)."""
+        arr = None
+        if positions is not None:
+            arr = (C.c_void_p * len(positions))(*[C.c_void_p(p) if p else None for p in positions])
+        _check(self.ctx.h, self._lib.oibvh_scene_frame(self.h, arr, entry_level, expand_levels,
+                                                       _flags(self_collide, fp64_fallback, asynchronous)), "frame")
+
+    def set_timing(self, enable: bool = True) -> None:
+        _check(self.ctx.h, self._lib.oibvh_scene_set_timing(self.h, 1 if enable else 0), "set_timing")
+
+    def timing(self) -> dict:
+        """Summed event durations since the last call: refit_ms (stage-1
+        reduce launches), detect_ms, and their counts."""
+        t = L.SceneTiming()
+        _check(self.ctx.h, self._lib.oibvh_scene_get_timing(self.h, C.byref(t)), "get_timing")
+        return {f: getattr(t, f) for f, _ in L.SceneTiming._fields_}
+
     def pair_vertices(self) -> np.ndarray:
         """Scene::convertToVertexArray: (n, 6, 3) float32, triangle A's three
         vertices then triangle B's, in getIntTriPairs order."""

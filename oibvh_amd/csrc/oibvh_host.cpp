@@ -70,6 +70,11 @@ struct oibvh_scene {
     std::vector<oi_mesh_desc> h_desc;  // host copies stay alive until the next detect
     oi_build_desc* d_geom = nullptr;   // per-tree geometry for the vertex export
     uint32_t geom_cap = 0;
+    // optional HIP-event timing of the refit's stage-1 launches and of the
+    // detects (pairs of events, read and reset by oibvh_scene_get_timing)
+    bool timing = false;
+    std::vector<hipEvent_t> ev_refit, ev_detect;
+    uint32_t n_refit = 0, n_detect = 0;
     std::vector<uint2> h_front;
     uint32_t front_key[3] = {~0u, ~0u, ~0u}; // entry, flags&SELF, #trees
     oi_detector det;
@@ -569,6 +574,8 @@ extern "C" void oibvh_scene_destroy(oibvh_scene* s)
     (void)hipStreamSynchronize(s->ctx->stream);
     dfree(s->d_desc);
     dfree(s->d_geom);
+    for (hipEvent_t e : s->ev_refit) (void)hipEventDestroy(e);
+    for (hipEvent_t e : s->ev_detect) (void)hipEventDestroy(e);
     oi_det_free(&s->det);
     oibvh_ctx* c = s->ctx;
     delete s;
@@ -678,6 +685,22 @@ static oibvh_status enqueue_detect(oibvh_scene* s, bool reset_sticky)
     return oi_det_enqueue(&d, reset_sticky);
 }
 
+// the next event pair of a timing list (created on first use)
+static oibvh_status timing_pair(oibvh_ctx* c, std::vector<hipEvent_t>& ev, uint32_t& n, hipEvent_t* a, hipEvent_t* b)
+{
+    if (2ull * n + 2 > ev.size()) {
+        for (int k = 0; k < 2; k++) {
+            hipEvent_t e = nullptr;
+            HIPCHK(c, hipEventCreate(&e));
+            ev.push_back(e);
+        }
+    }
+    *a = ev[2 * n];
+    *b = ev[2 * n + 1];
+    n++;
+    return OIBVH_OK;
+}
+
 extern "C" oibvh_status oibvh_scene_detect(oibvh_scene* s, uint32_t entry, uint32_t expand, uint32_t flags)
 {
     if (!s) return OIBVH_ERR_INVALID;
@@ -705,8 +728,15 @@ extern "C" oibvh_status oibvh_scene_detect(oibvh_scene* s, uint32_t entry, uint3
     }
     // an asynchronous detect keeps the error history of unchecked earlier
     // ones (reported as async_errors); a synchronous one starts afresh
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (s->timing) {
+        oibvh_status r = timing_pair(c, s->ev_detect, s->n_detect, &e0, &e1);
+        if (r != OIBVH_OK) return r;
+        HIPCHK(c, hipEventRecord(e0, c->stream));
+    }
     oibvh_status r = enqueue_detect(s, !async);
     if (r != OIBVH_OK) return r;
+    if (e1) HIPCHK(c, hipEventRecord(e1, c->stream));
     if (async) return OIBVH_OK;
     return oi_det_check(&d);
 }
@@ -738,9 +768,66 @@ extern "C" oibvh_status oibvh_scene_refit(oibvh_scene* s)
             boff[k + 1] = boff[k] + oi_lvl_cnt(t->L - h, t->L, t->vl);
             if (t->L > h) apex[na++] = d[k];
         }
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (s->timing) {
+            oibvh_status r = timing_pair(c, s->ev_refit, s->n_refit, &e0, &e1);
+            if (r != OIBVH_OK) return r;
+            HIPCHK(c, hipEventRecord(e0, c->stream));
+        }
         HIPCHK(c, oi_launch_reduce_inline(c->stream, M, d, n, boff, nullptr, 0));
+        if (e1) HIPCHK(c, hipEventRecord(e1, c->stream));
         if (na) HIPCHK(c, oi_launch_reduce_apex_inline(c->stream, M, apex, na));
     }
+    return OIBVH_OK;
+}
+
+// The reference's per-frame sequence (main.cpp:284 after the trees' refit):
+// point trees at new vertex memory, refit every tree, detect. One call, so a
+// frame costs the host one entry into the library.
+extern "C" oibvh_status oibvh_scene_frame(oibvh_scene* s, const float* const* positions, uint32_t entry,
+                                          uint32_t expand, uint32_t flags)
+{
+    if (!s) return OIBVH_ERR_INVALID;
+    if (positions)
+        for (size_t i = 0; i < s->trees.size(); i++)
+            if (positions[i]) s->trees[i]->pos = positions[i];
+    oibvh_status r = oibvh_scene_refit(s);
+    if (r != OIBVH_OK) return r;
+    return oibvh_scene_detect(s, entry, expand, flags);
+}
+
+extern "C" oibvh_status oibvh_scene_set_timing(oibvh_scene* s, uint32_t enable)
+{
+    if (!s) return OIBVH_ERR_INVALID;
+    s->timing = enable != 0;
+    s->n_refit = s->n_detect = 0;
+    return OIBVH_OK;
+}
+
+static oibvh_status sum_pairs(oibvh_ctx* c, const std::vector<hipEvent_t>& ev, uint32_t n, double* ms)
+{
+    *ms = 0.0;
+    for (uint32_t k = 0; k < n; k++) {
+        HIPCHK(c, hipEventSynchronize(ev[2 * k + 1]));
+        float v = 0.0f;
+        HIPCHK(c, hipEventElapsedTime(&v, ev[2 * k], ev[2 * k + 1]));
+        *ms += v;
+    }
+    return OIBVH_OK;
+}
+
+extern "C" oibvh_status oibvh_scene_get_timing(oibvh_scene* s, oibvh_scene_timing* out)
+{
+    if (!s || !out) return OIBVH_ERR_INVALID;
+    oibvh_ctx* c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    std::memset(out, 0, sizeof *out);
+    oibvh_status r = sum_pairs(c, s->ev_refit, s->n_refit, &out->refit_ms);
+    if (r == OIBVH_OK) r = sum_pairs(c, s->ev_detect, s->n_detect, &out->detect_ms);
+    if (r != OIBVH_OK) return r;
+    out->refits = s->n_refit;
+    out->detects = s->n_detect;
+    s->n_refit = s->n_detect = 0;
     return OIBVH_OK;
 }
 

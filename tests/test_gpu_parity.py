@@ -350,6 +350,39 @@ def test_scene_refit_equals_tree_refits(api, ctx, orc, n):
         assert np.array_equal(bits(trees[k].nodes()), bits(o.nodes())), f"tree {k}"
 
 
+def test_scene_frame_and_timing(api, ctx, orc):
+    """oibvh_scene_frame (borrow, refit, detect in one call) equals the
+    separate calls, and the scene's event timing counts every refit and
+    detect it bracketed."""
+    say("scene frame + timing")
+    meshes, waves = M.deforming_uv_pair(160, 81)
+    trees = [gpu_tree(api, ctx, x, f) for x, f in meshes]
+    sc = api.Scene(ctx)
+    for t in trees:
+        sc.addOibvhTree(t)
+    frames = [[waves[m].frame(k) for m in range(2)] for k in (3, 8)]
+    bufs = [[api.DeviceBuffer.from_array(x) for x in fr] for fr in frames]
+    sc.set_timing(True)
+    for k, fr in enumerate(frames):
+        sc.frame([b.at(0) for b in bufs[k]], 2, 2)
+        want = orc.detect([orc.Tree(fr[m], meshes[m][1], order=trees[m].perm()) for m in range(2)])
+        assert np.array_equal(sc.getIntTriPairs(), want), k
+        for m in range(2):
+            o = orc.Tree(fr[m], meshes[m][1], order=trees[m].perm())
+            assert np.array_equal(bits(trees[m].nodes()), bits(o.nodes()))
+    sc.frame(None, 2, 2)  # keeps the last positions
+    tm = sc.timing()
+    assert tm["refits"] == 3 and tm["detects"] == 3 and tm["refit_ms"] > 0 and tm["detect_ms"] > 0
+    assert sc.timing()["refits"] == 0
+    sc.set_timing(False)
+    for t, (x, f) in zip(trees, meshes):
+        t.refit(x)  # owned memory again before the buffers go away
+    ctx.synchronize()
+    for bb in bufs:
+        for b in bb:
+            b.free()
+
+
 def test_create_like_then_refit(api, ctx, orc):
     """main.cpp:127-151: tree 2 copy-constructed from tree 1 over a
     translated mesh with the same faces, then refit."""
