@@ -255,9 +255,11 @@ OIBVH_API oibvh_status oibvh_scene_frame(oibvh_scene* scene, const float* const*
 
 /* Timing of a scene's work with HIP events on the context's stream (no
  * reference counterpart; the reference times with cudaEvents around its own
- * kernels, oibvhTree.cu:300-355, scene.cu:262-433). While enabled, every
- * oibvh_scene_refit records an event pair around its stage-1 reduce launch
- * (k_reduce_w or k_reduce) and every detect one around its launches.
+ * kernels, oibvhTree.cu:300-355, scene.cu:262-433). With every = n > 0, every
+ * n-th oibvh_scene_refit records an event pair around its (first) stage-1
+ * reduce launch (k_reduce_w or k_reduce), and every n-th detect one around its
+ * launches; every = 0 stops. An event pair costs the GPU a few microseconds
+ * of pipeline drain, so a sample keeps the timed work close to untimed work.
  * oibvh_scene_get_timing waits for the recorded events, sums their durations
  * and starts the counts afresh. */
 typedef struct {
@@ -266,7 +268,7 @@ typedef struct {
     uint32_t refits;
     uint32_t detects;
 } oibvh_scene_timing;
-OIBVH_API oibvh_status oibvh_scene_set_timing(oibvh_scene* scene, uint32_t enable);
+OIBVH_API oibvh_status oibvh_scene_set_timing(oibvh_scene* scene, uint32_t every);
 OIBVH_API oibvh_status oibvh_scene_get_timing(oibvh_scene* scene, oibvh_scene_timing* out);
 
 /* BVTT front length of each level of the last detect (level 0 = the initial

@@ -311,8 +311,9 @@ class Scene:
         _check(self.ctx.h, self._lib.oibvh_scene_frame(self.h, arr, entry_level, expand_levels,
                                                        _flags(self_collide, fp64_fallback, asynchronous)), "frame")
 
-    def set_timing(self, enable: bool = True) -> None:
-        _check(self.ctx.h, self._lib.oibvh_scene_set_timing(self.h, 1 if enable else 0), "set_timing")
+    def set_timing(self, every: int = 1) -> None:
+        """Time every `every`-th refit and detect with HIP events (0 stops)."""
+        _check(self.ctx.h, self._lib.oibvh_scene_set_timing(self.h, int(every)), "set_timing")
 
     def timing(self) -> dict:
         """Summed event durations since the last call: refit_ms (stage-1

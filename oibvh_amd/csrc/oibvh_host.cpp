@@ -72,7 +72,8 @@ struct oibvh_scene {
     uint32_t geom_cap = 0;
     // optional HIP-event timing of the refit's stage-1 launches and of the
     // detects (pairs of events, read and reset by oibvh_scene_get_timing)
-    bool timing = false;
+    uint32_t timing = 0;              // 0: off; n: every n-th refit and detect
+    uint64_t seq_refit = 0, seq_detect = 0;
     std::vector<hipEvent_t> ev_refit, ev_detect;
     uint32_t n_refit = 0, n_detect = 0;
     std::vector<uint2> h_front;
@@ -729,7 +730,7 @@ extern "C" oibvh_status oibvh_scene_detect(oibvh_scene* s, uint32_t entry, uint3
     // an asynchronous detect keeps the error history of unchecked earlier
     // ones (reported as async_errors); a synchronous one starts afresh
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (s->timing) {
+    if (s->timing && (s->seq_detect++ % s->timing) == 0) {
         oibvh_status r = timing_pair(c, s->ev_detect, s->n_detect, &e0, &e1);
         if (r != OIBVH_OK) return r;
         HIPCHK(c, hipEventRecord(e0, c->stream));
@@ -769,7 +770,7 @@ extern "C" oibvh_status oibvh_scene_refit(oibvh_scene* s)
             if (t->L > h) apex[na++] = d[k];
         }
         hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (s->timing) {
+        if (s->timing && i0 == 0 && (s->seq_refit++ % s->timing) == 0) {
             oibvh_status r = timing_pair(c, s->ev_refit, s->n_refit, &e0, &e1);
             if (r != OIBVH_OK) return r;
             HIPCHK(c, hipEventRecord(e0, c->stream));
@@ -796,10 +797,11 @@ extern "C" oibvh_status oibvh_scene_frame(oibvh_scene* s, const float* const* po
     return oibvh_scene_detect(s, entry, expand, flags);
 }
 
-extern "C" oibvh_status oibvh_scene_set_timing(oibvh_scene* s, uint32_t enable)
+extern "C" oibvh_status oibvh_scene_set_timing(oibvh_scene* s, uint32_t every)
 {
     if (!s) return OIBVH_ERR_INVALID;
-    s->timing = enable != 0;
+    s->timing = every;
+    s->seq_refit = s->seq_detect = 0;
     s->n_refit = s->n_detect = 0;
     return OIBVH_OK;
 }

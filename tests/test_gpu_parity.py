@@ -362,7 +362,7 @@ def test_scene_frame_and_timing(api, ctx, orc):
         sc.addOibvhTree(t)
     frames = [[waves[m].frame(k) for m in range(2)] for k in (3, 8)]
     bufs = [[api.DeviceBuffer.from_array(x) for x in fr] for fr in frames]
-    sc.set_timing(True)
+    sc.set_timing(1)
     for k, fr in enumerate(frames):
         sc.frame([b.at(0) for b in bufs[k]], 2, 2)
         want = orc.detect([orc.Tree(fr[m], meshes[m][1], order=trees[m].perm()) for m in range(2)])
@@ -374,7 +374,12 @@ def test_scene_frame_and_timing(api, ctx, orc):
     tm = sc.timing()
     assert tm["refits"] == 3 and tm["detects"] == 3 and tm["refit_ms"] > 0 and tm["detect_ms"] > 0
     assert sc.timing()["refits"] == 0
-    sc.set_timing(False)
+    sc.set_timing(2)  # every second one
+    for _ in range(5):
+        sc.frame(None, 2, 2)
+    tm = sc.timing()
+    assert tm["refits"] == 3 and tm["detects"] == 3
+    sc.set_timing(0)
     for t, (x, f) in zip(trees, meshes):
         t.refit(x)  # owned memory again before the buffers go away
     ctx.synchronize()
