@@ -232,6 +232,9 @@ typedef struct {
                                           fronts in LDS, falling back to steps only for the next 64
                                           detects when a workgroup's front outgrows LDS */
     uint32_t traverse_local_levels;    /* steps the local walk replaces (0 = all of them) */
+    uint32_t narrow_separate;          /* FP32 queries: 0 tests leaf pairs inside the BVTT launches that
+                                          find them; 1 collects them and tests them in one k_narrow
+                                          launch over the whole GPU (the FP64 fallback always does) */
 } oibvh_tuning;
 
 /* The detect buffers grow on overflow (the detect is re-run). */

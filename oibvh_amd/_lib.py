@@ -36,7 +36,8 @@ class Tuning(C.Structure):
     _fields_ = [("reduce_leaves_per_thread", C.c_uint32), ("traverse_blocks", C.c_uint32),
                 ("traverse_expand", C.c_uint32), ("front_capacity", C.c_uint32), ("hits_capacity", C.c_uint32),
                 ("candidate_capacity", C.c_uint32), ("traverse_schedule", C.c_uint32),
-                ("traverse_local", C.c_uint32), ("traverse_local_levels", C.c_uint32)]
+                ("traverse_local", C.c_uint32), ("traverse_local_levels", C.c_uint32),
+                ("narrow_separate", C.c_uint32)]
 
 
 class SceneTiming(C.Structure):

@@ -42,7 +42,7 @@ def _time_refit(ctx: api.Context, frame, reps: int) -> float:
 
 
 def tune(ctx: api.Context, scene: api.Scene, frame, *, entries=(4, 6, 8), expands=(1, 2, 3),
-         blocks=(0, 2048), leaves=(16, 2, 4, 8), local=((64, 2), (64, 3), (256, 3)), reps: int = 5) -> dict:
+         blocks=(0, 2048), leaves=(16, 2, 4, 8), local=((64, 2), (128, 3), (256, 3)), reps: int = 5) -> dict:
     """Return {'entry_level', 'expand_levels', 'traverse_blocks',
     'reduce_leaves_per_thread', 'traverse_local', 'traverse_local_levels',
     'ms'} of the fastest setting and apply the tuning knobs to ctx. `local`

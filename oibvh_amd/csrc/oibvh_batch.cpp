@@ -456,12 +456,18 @@ extern "C" oibvh_status oibvh_batch_set_positions(oibvh_batch* b, uint32_t mesh,
     return OIBVH_OK;
 }
 
-static oibvh_status enqueue_refit(oibvh_batch* b)
+// refit launches; with seed, the apex launch (if any) also resets the
+// counters of the detect that follows and *seeded says so
+static oibvh_status enqueue_refit(oibvh_batch* b, const oi_seed_args* seed = nullptr, bool* seeded = nullptr)
 {
     oibvh_ctx* c = b->ctx;
     const int M = b->reduce_M;
+    if (seeded) *seeded = false;
     HIPCHK(c, oi_launch_reduce(c->stream, M, b->d_bdesc, b->w_reduce, nullptr, b->n_reduce, nullptr, 0));
-    if (b->n_apex) HIPCHK(c, oi_launch_reduce_apex(c->stream, M, b->d_bdesc, b->w_apex, nullptr, b->n_apex));
+    if (b->n_apex) {
+        HIPCHK(c, oi_launch_reduce_apex(c->stream, M, b->d_bdesc, b->w_apex, nullptr, b->n_apex, seed));
+        if (seed && seeded) *seeded = true;
+    }
     return OIBVH_OK;
 }
 
@@ -518,7 +524,8 @@ extern "C" oibvh_status oibvh_batch_build(oibvh_batch* b)
         vout = (vout == b->vals_tmp) ? b->vals : b->vals_tmp;
     }
     HIPCHK(c, oi_launch_reduce(s, b->reduce_M, b->d_bdesc, b->w_reduce, nullptr, b->n_reduce, vin, 1));
-    if (b->n_apex) HIPCHK(c, oi_launch_reduce_apex(s, b->reduce_M, b->d_bdesc, b->w_apex, nullptr, b->n_apex));
+    if (b->n_apex)
+        HIPCHK(c, oi_launch_reduce_apex(s, b->reduce_M, b->d_bdesc, b->w_apex, nullptr, b->n_apex, nullptr));
     b->built = true;
     return OIBVH_OK;
 }
@@ -641,7 +648,7 @@ extern "C" oibvh_status oibvh_batch_frame(oibvh_batch* b, uint32_t entry, uint32
     if ((r = prepare_detect(b, entry, expand, flags & ~OIBVH_DETECT_ASYNC)) != OIBVH_OK) return r;
     const uint64_t key[6] = {entry | ((uint64_t)d.local_chunk << 32) | ((uint64_t)d.local_levels << 48),
                              d.expand | ((uint64_t)d.schedule << 32),
-                             flags & ~OIBVH_DETECT_ASYNC,
+                             (flags & ~OIBVH_DETECT_ASYNC) | ((uint64_t)c->narrow_separate << 32),
                              (uint64_t)b->reduce_M,
                              d.generation, (uint64_t)b->desc_gen ^ ((uint64_t)oi_grid_for(c) << 32)};
     if (!b->gexec || std::memcmp(key, b->graph_key, sizeof key) != 0) {
@@ -652,8 +659,11 @@ extern "C" oibvh_status oibvh_batch_frame(oibvh_batch* b, uint32_t entry, uint32
         }
         hipGraph_t g = nullptr;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        oibvh_status rr = enqueue_refit(b);
-        if (rr == OIBVH_OK) rr = oi_det_enqueue(&d, false);
+        // the detect's counter reset rides on the refit's apex launch
+        const oi_seed_args seed = oi_det_seed_args(&d, false);
+        bool seeded = false;
+        oibvh_status rr = enqueue_refit(b, &seed, &seeded);
+        if (rr == OIBVH_OK) rr = oi_det_enqueue(&d, false, seeded);
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (rr != OIBVH_OK) {
             if (g) (void)hipGraphDestroy(g);

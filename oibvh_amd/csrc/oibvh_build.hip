@@ -926,9 +926,32 @@ __device__ __forceinline__ void apex_chunks(const oi_build_desc& D, uint32_t lc,
     for (uint32_t c = 0; c < nchunk; c++) reduce_subtree<M>(nl, D.nodes, D.L, D.vl, lc, h2, c, stage);
 }
 
-__global__ void __launch_bounds__(OI_BLOCK) k_reduce_apex(ReduceArgs a, uint32_t lgm)
+// zeroed level counts and result counters for the detect that follows
+// (the work of the collide side's k_seed, one workgroup)
+__device__ __forceinline__ void seed_counters(const oi_seed_args& s)
+{
+    const uint32_t i = threadIdx.x;
+    if (i < s.n_counts) s.counts[i] = (i == 0) ? s.n0 : 0u;
+    if (i == 0) {
+        s.ctl->sticky = s.reset_sticky ? 0u : (s.ctl->sticky | s.ctl->error);
+        s.ctl->error = 0;
+        s.ctl->n_hits = 0;
+        s.ctl->cands = 0;
+        s.ctl->fp64 = 0;
+        s.ctl->adj = 0;
+        s.ctl->tests = 0;
+    }
+}
+
+// with seed.counts set, the launch has one more workgroup, which resets the
+// counters of the detect that follows instead of climbing a mesh
+__global__ void __launch_bounds__(OI_BLOCK) k_reduce_apex(ReduceArgs a, uint32_t lgm, oi_seed_args seed)
 {
     __shared__ float stage[StageSize<8>::floats];
+    if (seed.counts && blockIdx.x == gridDim.x - 1) {
+        seed_counters(seed);
+        return;
+    }
     const uint2 it = a.T.item();
     const oi_build_desc D = a.T.mesh(it.x);
     const uint32_t L = D.L;
@@ -1096,25 +1119,31 @@ hipError_t oi_launch_reduce_inline(hipStream_t st, int M, const oi_build_desc* m
     return hipGetLastError();
 }
 
-// stage 2 for n inline meshes, one workgroup each
-hipError_t oi_launch_reduce_apex_inline(hipStream_t st, int M, const oi_build_desc* meshes, uint32_t n)
+// stage 2 for n inline meshes, one workgroup each; with seed (may be null)
+// one more workgroup resets the following detect's counters
+hipError_t oi_launch_reduce_apex_inline(hipStream_t st, int M, const oi_build_desc* meshes, uint32_t n,
+                                        const oi_seed_args* seed)
 {
     if (n == 0 || n > OI_INLINE_MESHES) return hipErrorInvalidValue;
+    if (seed && seed->n_counts > OI_BLOCK) return hipErrorInvalidValue;
     uint32_t boff[OI_INLINE_MESHES + 1];
     for (uint32_t m = 0; m <= n; m++) boff[m] = m;
     ReduceArgs a{make_btab_inline(meshes, n, boff), nullptr, 0};
     const uint32_t lgm = lgm_of(M);
-    hipLaunchKernelGGL(k_reduce_apex, dim3(n), dim3(OI_BLOCK), 0, st, a, lgm);
+    const oi_seed_args sd = seed ? *seed : oi_seed_args{};
+    hipLaunchKernelGGL(k_reduce_apex, dim3(n + (seed ? 1u : 0u)), dim3(OI_BLOCK), 0, st, a, lgm, sd);
     return hipGetLastError();
 }
 
 // stage 2, one workgroup per work item (mesh, 0) whose L exceeds the stage-1 height
 hipError_t oi_launch_reduce_apex(hipStream_t st, int M, const oi_build_desc* d, const uint2* work,
-                                 const oi_build_desc* one, uint32_t nblocks)
+                                 const oi_build_desc* one, uint32_t nblocks, const oi_seed_args* seed)
 {
+    if (seed && seed->n_counts > OI_BLOCK) return hipErrorInvalidValue;
     ReduceArgs a{make_btab(d, work, one), nullptr, 0};
     const uint32_t lgm = lgm_of(M);
-    hipLaunchKernelGGL(k_reduce_apex, dim3(nblocks), dim3(OI_BLOCK), 0, st, a, lgm);
+    const oi_seed_args sd = seed ? *seed : oi_seed_args{};
+    hipLaunchKernelGGL(k_reduce_apex, dim3(nblocks + (seed ? 1u : 0u)), dim3(OI_BLOCK), 0, st, a, lgm, sd);
     return hipGetLastError();
 }
 

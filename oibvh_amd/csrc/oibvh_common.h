@@ -178,6 +178,15 @@ struct oi_detect_ctl {
     unsigned long long tests;  // BVTT box pair tests
 };
 
+// a detect's counter reset (zeroed level counts and result counters), done
+// by the last workgroup of a refit's apex launch when a detect follows it in
+// the same frame call, instead of a launch of its own
+struct oi_seed_args {
+    uint32_t* counts;  // null: no reset in this launch
+    oi_detect_ctl* ctl;
+    uint32_t n0, n_counts, reset_sticky, pad;
+};
+
 #define OI_ERR_QUEUE_OVERFLOW 1u  // a BVTT front exceeded its buffer
 #define OI_ERR_TIMEOUT 2u
 #define OI_ERR_CORRUPT 4u

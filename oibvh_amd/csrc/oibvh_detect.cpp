@@ -114,16 +114,28 @@ oibvh_status oi_det_prepare(oi_detector* d)
     return OIBVH_OK;
 }
 
-oibvh_status oi_det_enqueue(oi_detector* d, bool reset_sticky)
+oi_seed_args oi_det_seed_args(const oi_detector* d, bool reset_sticky)
+{
+    oi_seed_args s{};
+    s.counts = d->counts;
+    s.ctl = d->ctl;
+    s.n0 = d->n_front;
+    s.n_counts = d->n_levels + 1;
+    s.reset_sticky = reset_sticky ? 1u : 0u;
+    return s;
+}
+
+oibvh_status oi_det_enqueue(oi_detector* d, bool reset_sticky, bool seeded)
 {
     oibvh_ctx* c = d->ctx;
     hipStream_t st = c->stream;
-    HIPCHK(c, oi_launch_seed(st, d->n_front, d->counts, d->n_levels + 1, d->ctl, reset_sticky ? 1u : 0u));
+    if (!seeded)
+        HIPCHK(c, oi_launch_seed(st, d->n_front, d->counts, d->n_levels + 1, d->ctl, reset_sticky ? 1u : 0u));
     const uint32_t grid = oi_grid_for(c);
     // FP32 queries test leaf pairs inside the steps; the FP64 fallback keeps
     // its own narrow pass (its registers would slow every step)
     const bool fp64 = (d->flags & OIBVH_DETECT_FP64_FALLBACK) != 0;
-    d->fused = !fp64;
+    d->fused = !fp64 && !c->narrow_separate;
     const uint32_t J = d->local_chunk ? d->local_levels : 0u;
     if (J) {
         // steps 0 .. J-1 in one launch; its front goes where step J - 1's would
@@ -141,7 +153,7 @@ oibvh_status oi_det_enqueue(oi_detector* d, bool reset_sticky)
     }
     if (!d->fused)
         HIPCHK(c, oi_launch_narrow(st, grid, &d->tab, d->cands, &d->ctl->cands, d->cand_cap, d->hits,
-                                   &d->ctl->n_hits, d->hits_cap, d->ctl, 1));
+                                   &d->ctl->n_hits, d->hits_cap, d->ctl, fp64 ? 1 : 0));
     d->unchecked = true;
     d->have_result = true;
     d->sorted_ok = false;

@@ -161,6 +161,7 @@ extern "C" oibvh_status oibvh_ctx_set_tuning(oibvh_ctx* ctx, const oibvh_tuning*
     ctx->trav_schedule = t->traverse_schedule;
     ctx->trav_local = t->traverse_local;
     ctx->trav_local_levels = t->traverse_local_levels;
+    ctx->narrow_separate = t->narrow_separate;
     return OIBVH_OK;
 }
 
@@ -175,6 +176,7 @@ extern "C" oibvh_status oibvh_ctx_get_tuning(const oibvh_ctx* ctx, oibvh_tuning*
     t->traverse_schedule = ctx->trav_schedule;
     t->traverse_local = ctx->trav_local;
     t->traverse_local_levels = ctx->trav_local_levels;
+    t->narrow_separate = ctx->narrow_separate;
     t->hits_capacity = ctx->hits_cap;
     return OIBVH_OK;
 }
@@ -370,7 +372,7 @@ static oibvh_status launch_reduce(oibvh_tree* t, bool build)
     const oi_build_desc d = tree_desc(t);
     HIPCHK(c, oi_launch_reduce(c->stream, M, nullptr, nullptr, &d, oi_lvl_cnt(t->L - h, t->L, t->vl),
                                build ? t->perm : nullptr, 0));
-    if (t->L > h) HIPCHK(c, oi_launch_reduce_apex(c->stream, M, nullptr, nullptr, &d, 1));
+    if (t->L > h) HIPCHK(c, oi_launch_reduce_apex(c->stream, M, nullptr, nullptr, &d, 1, nullptr));
     return OIBVH_OK;
 }
 
@@ -627,7 +629,8 @@ static void make_front(oibvh_scene* s, uint32_t entry, bool self)
     }
 }
 
-static oibvh_status enqueue_detect(oibvh_scene* s, bool reset_sticky)
+// size the detect buffers and set up the query of s->last_* (allocates)
+static oibvh_status prepare_scene_detect(oibvh_scene* s)
 {
     oibvh_ctx* c = s->ctx;
     oi_detector& d = s->det;
@@ -681,9 +684,14 @@ static oibvh_status enqueue_detect(oibvh_scene* s, bool reset_sticky)
     d.schedule = c->trav_schedule;
     d.flags = s->last_flags;
     d.implicit_end = (uint32_t)acc;
-    oibvh_status r = oi_det_prepare(&d);
+    return oi_det_prepare(&d);
+}
+
+static oibvh_status enqueue_detect(oibvh_scene* s, bool reset_sticky)
+{
+    oibvh_status r = prepare_scene_detect(s);
     if (r != OIBVH_OK) return r;
-    return oi_det_enqueue(&d, reset_sticky);
+    return oi_det_enqueue(&s->det, reset_sticky);
 }
 
 // the next event pair of a timing list (created on first use)
@@ -745,10 +753,12 @@ extern "C" oibvh_status oibvh_scene_detect(oibvh_scene* s, uint32_t entry, uint3
 // Every tree of the scene refit at once: up to four trees per launch pair
 // (their descriptors travel in the kernel arguments), so the reduce fills the
 // GPU once instead of once per tree. Same results as oibvh_tree_refit on each.
-extern "C" oibvh_status oibvh_scene_refit(oibvh_scene* s)
+// the scene refit; with seed, the last apex launch (if there is one) also
+// resets the counters of the detect that follows, and *seeded says so
+static oibvh_status scene_refit(oibvh_scene* s, const oi_seed_args* seed, bool* seeded)
 {
-    if (!s) return OIBVH_ERR_INVALID;
     oibvh_ctx* c = s->ctx;
+    if (seeded) *seeded = false;
     HIPCHK(c, hipSetDevice(c->device));
     for (oibvh_tree* t : s->trees)
         if (!t->built) return oi_set_err(c, OIBVH_ERR_NOT_BUILT, "scene refit: tree not built");
@@ -777,9 +787,18 @@ extern "C" oibvh_status oibvh_scene_refit(oibvh_scene* s)
         }
         HIPCHK(c, oi_launch_reduce_inline(c->stream, M, d, n, boff, nullptr, 0));
         if (e1) HIPCHK(c, hipEventRecord(e1, c->stream));
-        if (na) HIPCHK(c, oi_launch_reduce_apex_inline(c->stream, M, apex, na));
+        const bool last = i0 + per >= s->trees.size();
+        const oi_seed_args* sd = (last && seed && na) ? seed : nullptr;
+        if (na) HIPCHK(c, oi_launch_reduce_apex_inline(c->stream, M, apex, na, sd));
+        if (sd && seeded) *seeded = true;
     }
     return OIBVH_OK;
+}
+
+extern "C" oibvh_status oibvh_scene_refit(oibvh_scene* s)
+{
+    if (!s) return OIBVH_ERR_INVALID;
+    return scene_refit(s, nullptr, nullptr);
 }
 
 // The reference's per-frame sequence (main.cpp:284 after the trees' refit):
@@ -789,12 +808,47 @@ extern "C" oibvh_status oibvh_scene_frame(oibvh_scene* s, const float* const* po
                                           uint32_t expand, uint32_t flags)
 {
     if (!s) return OIBVH_ERR_INVALID;
+    oibvh_ctx* c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
     if (positions)
         for (size_t i = 0; i < s->trees.size(); i++)
             if (positions[i]) s->trees[i]->pos = positions[i];
-    oibvh_status r = oibvh_scene_refit(s);
+    const bool async = (flags & OIBVH_DETECT_ASYNC) != 0;
+    oi_detector& d = s->det;
+    const bool trivial = s->trees.empty() || (s->trees.size() == 1 && !(flags & OIBVH_DETECT_SELF));
+    for (oibvh_tree* t : s->trees)
+        if (!t->built) return oi_set_err(c, OIBVH_ERR_NOT_BUILT, "frame: tree not built");
+    if (trivial) {
+        oibvh_status r = scene_refit(s, nullptr, nullptr);
+        if (r != OIBVH_OK) return r;
+        return oibvh_scene_detect(s, entry, expand, flags);
+    }
+    // as oibvh_scene_detect, with the detect's counter reset folded into the
+    // refit's apex launch: check an earlier synchronous result first, size
+    // the query, refit, then the detect's own launches
+    if (d.unchecked && !async) {
+        oibvh_status r = oi_det_check(&d);
+        if (r != OIBVH_OK) return r;
+    }
+    s->last_entry = entry;
+    s->last_expand = expand;
+    s->last_flags = flags;
+    d.n_pairs = 0;
+    d.stats = oibvh_detect_stats{};
+    oibvh_status r = prepare_scene_detect(s);
     if (r != OIBVH_OK) return r;
-    return oibvh_scene_detect(s, entry, expand, flags);
+    const oi_seed_args seed = oi_det_seed_args(&d, !async);
+    bool seeded = false;
+    if ((r = scene_refit(s, &seed, &seeded)) != OIBVH_OK) return r;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (s->timing && (s->seq_detect++ % s->timing) == 0) {
+        if ((r = timing_pair(c, s->ev_detect, s->n_detect, &e0, &e1)) != OIBVH_OK) return r;
+        HIPCHK(c, hipEventRecord(e0, c->stream));
+    }
+    if ((r = oi_det_enqueue(&d, !async, seeded)) != OIBVH_OK) return r;
+    if (e1) HIPCHK(c, hipEventRecord(e1, c->stream));
+    if (async) return OIBVH_OK;
+    return oi_det_check(&d);
 }
 
 extern "C" oibvh_status oibvh_scene_set_timing(oibvh_scene* s, uint32_t every)

@@ -33,12 +33,12 @@ uint32_t oi_reduce_height(int, uint32_t);
 hipError_t oi_launch_reduce(hipStream_t, int, const oi_build_desc*, const uint2*, const oi_build_desc*, uint32_t,
                             const uint32_t*, uint32_t);
 hipError_t oi_launch_reduce_apex(hipStream_t, int, const oi_build_desc*, const uint2*, const oi_build_desc*,
-                                 uint32_t);
+                                 uint32_t, const oi_seed_args*);
 hipError_t oi_launch_set_xf(hipStream_t, float*, const float*);
 uint32_t oi_inline_meshes();
 hipError_t oi_launch_reduce_inline(hipStream_t, int, const oi_build_desc*, uint32_t, const uint32_t*, const uint32_t*,
                                    uint32_t);
-hipError_t oi_launch_reduce_apex_inline(hipStream_t, int, const oi_build_desc*, uint32_t);
+hipError_t oi_launch_reduce_apex_inline(hipStream_t, int, const oi_build_desc*, uint32_t, const oi_seed_args*);
 hipError_t oi_launch_set_mesh_pos(hipStream_t, oi_build_desc*, oi_mesh_desc*, uint32_t, const float*);
 hipError_t oi_launch_seed(hipStream_t, uint32_t, uint32_t*, uint32_t, oi_detect_ctl*, uint32_t);
 hipError_t oi_launch_bvtt_step(hipStream_t, uint32_t, const oi_tab_args*, int, const uint2*, const uint32_t*,
@@ -66,6 +66,7 @@ struct oibvh_ctx {
     uint32_t trav_schedule = 0;
     uint32_t trav_local = 0;         // 0: BVTT steps only; n: the first steps as a local walk, n pairs per workgroup
     uint32_t trav_local_levels = 0;  // steps the local walk replaces (0: all)
+    uint32_t narrow_separate = 0;    // FP32 queries test leaf pairs in one k_narrow launch
     uint32_t queue_cap = 1u << 22;
     uint32_t hits_cap = 1u << 20;
     uint32_t cand_cap = 1u << 20;
@@ -166,8 +167,12 @@ void oi_det_free(oi_detector* d);
 oibvh_status oi_det_set_front(oi_detector* d, const std::vector<uint2>& front);
 // size every buffer for the current query (allocates; not capturable)
 oibvh_status oi_det_prepare(oi_detector* d);
-// launch seed, steps and narrow phase (no allocation, no synchronisation)
-oibvh_status oi_det_enqueue(oi_detector* d, bool reset_sticky);
+// launch seed, steps and narrow phase (no allocation, no synchronisation);
+// seeded: the counters were reset by a launch already queued (see
+// oi_det_seed_args), so no seed launch
+oibvh_status oi_det_enqueue(oi_detector* d, bool reset_sticky, bool seeded = false);
+// the counter reset of the prepared query, for a launch queued before it
+oi_seed_args oi_det_seed_args(const oi_detector* d, bool reset_sticky);
 // wait; on overflow grow the buffers and run the query again
 oibvh_status oi_det_check(oi_detector* d);
 // sorted hits (4 words each)

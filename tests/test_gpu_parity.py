@@ -485,6 +485,27 @@ def test_detect_entry_expand_invariance(api, ctx, orc, entry, expand, mode):
     assert st["retries"] == 0
 
 
+@pytest.mark.parametrize("local", [0, 64])
+def test_detect_narrow_separate(api, ctx, orc, local):
+    """FP32 leaf pairs collected and tested by one k_narrow launch
+    (narrow_separate = 1): the same pairs, pair sets and self mode."""
+    say(f"detect narrow_separate local={local}")
+    meshes = M.interpenetrating_icospheres(subdiv=4, seed=23, offset=1.2)
+    want = orc.detect([orc.Tree(x, f) for x, f in meshes])
+    x, f = M.cloth(64, amplitude=0.2, wavelength=0.2, seed=5)
+    want_self = orc.detect([orc.Tree(x, f)], self_collide=True)
+    ctx.set_tuning(narrow_separate=1, traverse_local=local, traverse_local_levels=2)
+    try:
+        for entry, expand in [(0, 1), (3, 2), (2, 3)]:
+            p, st, _ = gpu_pairs(api, ctx, meshes, entry, expand)
+            assert np.array_equal(p, want), (entry, expand)
+            assert st["candidates"] >= len(want)
+        p, _, _ = gpu_pairs(api, ctx, [(x, f)], 2, 2, self_collide=True)
+        assert np.array_equal(p, want_self)
+    finally:
+        ctx.set_tuning(narrow_separate=0, traverse_local=0, traverse_local_levels=0)
+
+
 def test_detect_local_walk_overflow_falls_back(api, ctx, orc):
     """Two identical spheres walked from the roots (one workgroup) three
     levels at a time: every node overlaps its twin and neighbours, so the
