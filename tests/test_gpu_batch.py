@@ -202,6 +202,56 @@ def test_batch_local_walk(api, ctx, orc, fp64, levels):
     b.close()
 
 
+def test_batch_tiny_and_mixed_meshes(api, ctx, orc):
+    """Meshes of 1, 2, 3 and 5 triangles next to a few thousand, scenes of one
+    mesh and an empty scene index in between: every tree equals the oracle's
+    and the pairs equal the oracle's detect, scene by scene (self mode too)."""
+    r = M.Rng(0x7171)
+
+    def blob(F, c, s):
+        V = F + 2
+        x = (c + s * r.uniform(V * 3, -1.0, 1.0).reshape(V, 3)).astype(np.float32)
+        f = r.integers(F * 3, 0, V - 1).reshape(F, 3).astype(np.uint32)
+        return x, f
+
+    scenes = [
+        [blob(1, 0.0, 0.5), blob(2, 0.1, 0.5), blob(3000, 0.0, 0.6)],
+        [blob(5, 0.0, 0.4)],                     # one mesh: no pairs unless self mode
+        [],                                      # scene 2 holds no mesh
+        [blob(3, 0.2, 0.3), blob(1, 0.2, 0.3), blob(7, 0.25, 0.3), blob(2, 0.9, 0.1)],
+    ]
+    b = api.Batch(ctx)
+    ids = []
+    for si, ms in enumerate(scenes):
+        for x, f in ms:
+            ids.append((si, b.add_mesh(si, x, f), x, f))
+    b.build()
+    for si, idx, x, f in ids:
+        o = orc.Tree(x, f, order="morton")
+        assert np.array_equal(b.perm(idx), o.order), idx
+        assert np.array_equal(bits(b.nodes(idx)), bits(o.nodes())), idx
+    for self_c in (False, True):
+        say(f"batch tiny meshes self={self_c}")
+        want = []
+        for si, ms in enumerate(scenes):
+            sel = [(idx, x, f) for s_, idx, x, f in ids if s_ == si]
+            if not sel:
+                continue
+            p = orc.detect([orc.Tree(x, f) for _, x, f in sel], self_collide=self_c)
+            if len(p):
+                p = p.copy()
+                gid = np.asarray([idx for idx, _, _ in sel], np.uint32)
+                p[:, 0] = gid[p[:, 0]]
+                p[:, 1] = gid[p[:, 1]]
+                want.append(p)
+        want = np.concatenate(want) if want else np.zeros((0, 4), np.uint32)
+        want = want[np.lexsort((want[:, 3], want[:, 2], want[:, 1], want[:, 0]))]
+        for entry, expand in [(0, 1), (2, 3)]:
+            b.detect(entry, expand, self_collide=self_c)
+            assert np.array_equal(b.pairs(), want), (self_c, entry, expand)
+    b.close()
+
+
 def test_batch_self_collision(api, ctx, orc):
     say("batch: cloth self collision")
     cl = [M.cloth(24 + 6 * i, 0.2, 0.25, seed=40 + i) for i in range(3)]

@@ -388,6 +388,25 @@ def test_scene_frame_and_timing(api, ctx, orc):
             b.free()
 
 
+def test_scene_edge_cases(api, ctx, orc):
+    """An empty scene, a one-tree scene without and with self mode, and the
+    frame call on them: no pairs where the reference has none."""
+    say("scene edge cases")
+    sc = api.Scene(ctx)
+    sc.detectCollision(0, 1)
+    assert sc.getIntTriPairCount() == 0 and sc.getIntTriPairs().shape == (0, 4)
+    sc.frame(None, 0, 1)
+    assert sc.getIntTriPairCount() == 0
+    x, f = M.cloth(24, amplitude=0.2, wavelength=0.2, seed=9)
+    t = gpu_tree(api, ctx, x, f)
+    sc.addOibvhTree(t)
+    sc.frame(None, 1, 2)
+    assert sc.getIntTriPairCount() == 0
+    sc.frame(None, 1, 2, self_collide=True)
+    assert np.array_equal(sc.getIntTriPairs(), orc.detect([orc.Tree(x, f)], self_collide=True))
+    sc.close()
+
+
 def test_create_like_then_refit(api, ctx, orc):
     """main.cpp:127-151: tree 2 copy-constructed from tree 1 over a
     translated mesh with the same faces, then refit."""
