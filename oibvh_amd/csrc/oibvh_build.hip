@@ -944,11 +944,14 @@ __device__ __forceinline__ void seed_counters(const oi_seed_args& s)
 }
 
 // with seed.counts set, the launch has one more workgroup, which resets the
-// counters of the detect that follows instead of climbing a mesh
+// counters of the detect that follows instead of climbing a mesh. Work items
+// are numbered through the XCD remap (BuildTab::item), so the extra workgroup
+// is the one whose remapped index is the last: the others then cover exactly
+// the items 0 .. n - 1 of the work list.
 __global__ void __launch_bounds__(OI_BLOCK) k_reduce_apex(ReduceArgs a, uint32_t lgm, oi_seed_args seed)
 {
     __shared__ float stage[StageSize<8>::floats];
-    if (seed.counts && blockIdx.x == gridDim.x - 1) {
+    if (seed.counts && oi_xcd_remap(blockIdx.x, gridDim.x) == gridDim.x - 1) {
         seed_counters(seed);
         return;
     }

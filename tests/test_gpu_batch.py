@@ -252,6 +252,34 @@ def test_batch_tiny_and_mixed_meshes(api, ctx, orc):
     b.close()
 
 
+def test_batch_frame_many_apex_meshes(api, ctx, orc):
+    """Regression: a captured frame whose refit has more than eight apex work
+    items (meshes taller than the stage-1 subtrees) carries the detect's
+    counter reset in an extra apex workgroup; every mesh must still be climbed
+    and no work item read past the list. FP32 and FP64-fallback frames equal
+    the oracle."""
+    rs = M.RigidScenes(n_scenes=6, seed=0xA9E7, meshes=(2, 4), faces=(2000, 6000), radius=0.3)
+    b = make_batch(api, ctx, rs)
+    mats = rs.transforms(1)
+    b.set_transforms(mats)
+    b.build()
+    tall = sum(1 for _, lib in rs.instances if rs.library[lib][1].shape[0] > 1024)
+    assert tall >= 9, tall
+    for fp64 in (False, True):
+        say(f"batch frame, {tall} apex meshes, fp64={fp64}")
+        for frame in (2, 3):
+            mats = rs.transforms(frame)
+            b.set_transforms(mats)
+            b.frame(1, 2, fp64_fallback=fp64)
+            perms = [b.perm(i) for i in range(b.mesh_count)]
+            assert np.array_equal(b.pairs(), all_pairs(orc, rs, mats, perms=perms, fp64=fp64)), (fp64, frame)
+            for i in range(b.mesh_count):
+                xyz, f = rs.library[rs.instances[i][1]]
+                o = orc.Tree(orc.xform(mats[i], xyz), f, order=perms[i])
+                assert np.array_equal(bits(b.nodes(i)), bits(o.nodes())), (fp64, frame, i)
+    b.close()
+
+
 def test_batch_self_collision(api, ctx, orc):
     say("batch: cloth self collision")
     cl = [M.cloth(24 + 6 * i, 0.2, 0.25, seed=40 + i) for i in range(3)]
