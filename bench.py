@@ -688,7 +688,7 @@ def run_self(args, ctx):
     progress("self: built; autotuning")
     ctx.set_tuning(front_capacity=1 << 29)
     tuned = autotune.tune_batch(ctx, b, stage, entries=(4, 6), expands=(2, 3), blocks=(0,), leaves=(0,),
-                                flags={"self_collide": True}, reps=2)
+                                local=((256, 3), (64, 2)), flags={"self_collide": True}, reps=4)
     progress(f"self: tuned {tuned}")
     entry, expand = tuned["entry_level"], tuned["expand_levels"]
     stage(0)
