@@ -705,6 +705,29 @@ def run_self(args, ctx):
     st = b.stats()
     pairs = b.pairs()
     tri = int(sum(f.shape[0] for _, f in base))
+    # full-size cross-check of the last frame (the oracle is too slow at this
+    # size): every sheet as its own plain tree and scene, self mode, eagerly,
+    # against the batched frame's pairs (same positions: frame k - 1)
+    plain_pairs = None
+    if not args.no_verify:
+        last = bufs[(k - 1) % n_distinct]
+        rows = []
+        for i, (x, f) in enumerate(base):
+            t = api.OibvhTree(ctx, x, f)
+            t.set_positions_device(last.at(i * V * 12))
+            t.build()
+            sc = api.Scene(ctx)
+            sc.addOibvhTree(t)
+            sc.detectCollision(entry, expand, self_collide=True)
+            p = sc.getIntTriPairs().copy()
+            p[:, 0] = i
+            p[:, 1] = i
+            rows.append(p)
+            sc.close()
+            t.close()
+        plain_pairs = np.concatenate(rows)
+        plain_pairs = plain_pairs[np.lexsort((plain_pairs[:, 3], plain_pairs[:, 2], plain_pairs[:, 1],
+                                              plain_pairs[:, 0]))]
     res = {"workload": (f"{n_inst} folded cloth sheets of {nv} x {nv} vertices ({base[0][1].shape[0]} triangles "
                         f"each, {tri} in all), amplitude 0.2, wavelength 0.05, fold 1.5; the height field's phase "
                         f"advances {dphase} rad per frame ({n_distinct} distinct frames cycled, resident in HBM); "
@@ -713,6 +736,8 @@ def run_self(args, ctx):
            "mtri_per_s": tri / (ms * 1e-3) / 1e6, "pairs_last_frame": int(len(pairs)),
            "adjacent_skipped": st["adjacent"], "candidates": st["candidates"], "bvtt_tests": st["bvtt_tests"],
            "bvtt_levels_last_frame": b.level_counts().tolist(), "autotune": tuned, "host_gen_s": t_gen}
+    if plain_pairs is not None:
+        res["pair_diffs_batched_vs_plain_full_size"] = _pair_diff(pairs, plain_pairs)
     if not args.no_verify:
         import oracle
 
