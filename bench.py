@@ -84,6 +84,10 @@ def parse():
                    help="with --entry-level/--expand-levels: initial pairs per workgroup of the local walk (0: steps)")
     p.add_argument("--traverse-local-levels", type=int, default=0,
                    help="steps the local walk replaces (0: all)")
+    p.add_argument("--traverse-tail", type=int, default=None,
+                   help="with --entry-level/--expand-levels: front pairs per workgroup pass of the tail walk (0: none)")
+    p.add_argument("--traverse-tail-steps", type=int, default=0,
+                   help="steps the tail walk replaces, counted from the last (0: all after the head walk)")
     p.add_argument("--rebuild-every", type=int, default=50)
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -157,6 +161,8 @@ def run_plain(args, ctx):
     K, W = args.steps, args.warmup
     if args.traverse_local is not None:
         ctx.set_tuning(traverse_local=args.traverse_local, traverse_local_levels=args.traverse_local_levels)
+    if args.traverse_tail is not None:
+        ctx.set_tuning(traverse_tail=args.traverse_tail, traverse_tail_steps=args.traverse_tail_steps)
     if args.workload == "deforming-uv-pair":
         meshes, waves = M.deforming_uv_pair()
         n_frames = 200
@@ -688,7 +694,7 @@ def run_self(args, ctx):
     progress("self: built; autotuning")
     ctx.set_tuning(front_capacity=1 << 29)
     tuned = autotune.tune_batch(ctx, b, stage, entries=(4, 6), expands=(2, 3), blocks=(0,), leaves=(0,),
-                                local=((256, 3), (64, 2)), flags={"self_collide": True}, reps=4)
+                                local=((256, 3), (64, 2)), tail=((256, 2),), flags={"self_collide": True}, reps=4)
     progress(f"self: tuned {tuned}")
     entry, expand = tuned["entry_level"], tuned["expand_levels"]
     stage(0)

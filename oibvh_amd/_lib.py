@@ -37,7 +37,8 @@ class Tuning(C.Structure):
                 ("traverse_expand", C.c_uint32), ("front_capacity", C.c_uint32), ("hits_capacity", C.c_uint32),
                 ("candidate_capacity", C.c_uint32), ("traverse_schedule", C.c_uint32),
                 ("traverse_local", C.c_uint32), ("traverse_local_levels", C.c_uint32),
-                ("narrow_separate", C.c_uint32)]
+                ("narrow_separate", C.c_uint32), ("traverse_tail", C.c_uint32),
+                ("traverse_tail_steps", C.c_uint32)]
 
 
 class SceneTiming(C.Structure):

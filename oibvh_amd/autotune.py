@@ -42,12 +42,15 @@ def _time_refit(ctx: api.Context, frame, reps: int) -> float:
 
 
 def tune(ctx: api.Context, scene: api.Scene, frame, *, entries=(4, 6, 8), expands=(1, 2, 3),
-         blocks=(0, 2048), leaves=(16, 2, 4, 8), local=((64, 2), (128, 3), (256, 3)), reps: int = 5) -> dict:
+         blocks=(0, 2048), leaves=(16, 2, 4, 8), local=((64, 2), (128, 3), (256, 3)),
+         tail=((32, 0), (64, 0), (128, 0), (256, 2)), reps: int = 5) -> dict:
     """Return {'entry_level', 'expand_levels', 'traverse_blocks',
     'reduce_leaves_per_thread', 'traverse_local', 'traverse_local_levels',
-    'ms'} of the fastest setting and apply the tuning knobs to ctx. `local`
-    lists the (initial pairs per workgroup, steps) settings of the local walk
-    tried at the best step shape."""
+    'traverse_tail', 'traverse_tail_steps', 'ms'} of the fastest setting and
+    apply the tuning knobs to ctx. `local` lists the (initial pairs per
+    workgroup, steps) settings of the head local walk tried at the best step
+    shape, `tail` the (front pairs per workgroup pass, steps) settings of the
+    tail walk tried after the best head setting."""
     best = None
     # reduce variant first, timed on the refit alone (the detect does not
     # depend on it), then the traversal shape
@@ -60,12 +63,13 @@ def tune(ctx: api.Context, scene: api.Scene, frame, *, entries=(4, 6, 8), expand
             lpt_best, lpt_ms = lpt, ms
     ctx.set_tuning(reduce_leaves_per_thread=lpt_best)
     for entry, expand, nb in itertools.product(entries, expands, blocks):
-        ctx.set_tuning(traverse_blocks=nb, traverse_expand=expand, traverse_local=0, traverse_local_levels=0)
+        ctx.set_tuning(traverse_blocks=nb, traverse_expand=expand, traverse_local=0, traverse_local_levels=0,
+                       traverse_tail=0, traverse_tail_steps=0)
         ms = _time(ctx, scene, frame, entry, expand, reps)
         if best is None or ms < best["ms"]:
             best = {"entry_level": entry, "expand_levels": expand, "traverse_blocks": nb,
                     "reduce_leaves_per_thread": lpt_best, "traverse_local": 0, "traverse_local_levels": 0,
-                    "ms": ms}
+                    "traverse_tail": 0, "traverse_tail_steps": 0, "ms": ms}
     # then the first steps as one workgroup-local launch, at the best shape
     ctx.set_tuning(traverse_blocks=best["traverse_blocks"], traverse_expand=best["expand_levels"])
     for chunk, levels in local:
@@ -73,16 +77,25 @@ def tune(ctx: api.Context, scene: api.Scene, frame, *, entries=(4, 6, 8), expand
         ms = _time(ctx, scene, frame, best["entry_level"], best["expand_levels"], reps)
         if ms < best["ms"]:
             best = {**best, "traverse_local": chunk, "traverse_local_levels": levels, "ms": ms}
+    # then the last steps as a tail walk, after the best head setting
+    ctx.set_tuning(traverse_local=best["traverse_local"], traverse_local_levels=best["traverse_local_levels"])
+    for chunk, steps in tail:
+        ctx.set_tuning(traverse_tail=chunk, traverse_tail_steps=steps)
+        ms = _time(ctx, scene, frame, best["entry_level"], best["expand_levels"], reps)
+        if ms < best["ms"]:
+            best = {**best, "traverse_tail": chunk, "traverse_tail_steps": steps, "ms": ms}
     ctx.set_tuning(**{**{k: base[k] for k in base},
                       "reduce_leaves_per_thread": best["reduce_leaves_per_thread"],
                       "traverse_blocks": best["traverse_blocks"], "traverse_expand": best["expand_levels"],
                       "traverse_local": best["traverse_local"],
-                      "traverse_local_levels": best["traverse_local_levels"]})
+                      "traverse_local_levels": best["traverse_local_levels"],
+                      "traverse_tail": best["traverse_tail"], "traverse_tail_steps": best["traverse_tail_steps"]})
     return best
 
 
 def tune_batch(ctx: api.Context, batch: api.Batch, before_frame, *, entries=(0, 2, 4), expands=(1, 2, 3),
-               blocks=(0, 4096), leaves=(0, 4, 8), local=((64, 2), (256, 3)), flags=None, reps: int = 3) -> dict:
+               blocks=(0, 4096), leaves=(0, 4, 8), local=((64, 2), (256, 3)), tail=((64, 0), (128, 2)),
+               flags=None, reps: int = 3) -> dict:
     """Pick the reduce subtree size (leaves per thread), the BVTT entry level,
     the levels per step and the workgroups per launch for a batch's captured
     frame (refit + detect). before_frame(i) stages frame i's inputs (model
@@ -113,20 +126,28 @@ def tune_batch(ctx: api.Context, batch: api.Batch, before_frame, *, entries=(0, 
     ctx.set_tuning(reduce_leaves_per_thread=lpt_best)
     best = None
     for entry, expand, nb in itertools.product(entries, expands, blocks):
-        ctx.set_tuning(traverse_blocks=nb, traverse_expand=expand, traverse_local=0, traverse_local_levels=0)
+        ctx.set_tuning(traverse_blocks=nb, traverse_expand=expand, traverse_local=0, traverse_local_levels=0,
+                       traverse_tail=0, traverse_tail_steps=0)
         ms = run(entry, expand)
         if best is None or ms < best["ms"]:
             best = {"entry_level": entry, "expand_levels": expand, "traverse_blocks": nb,
                     "reduce_leaves_per_thread": lpt_best, "traverse_local": 0, "traverse_local_levels": 0,
-                    "ms": ms}
+                    "traverse_tail": 0, "traverse_tail_steps": 0, "ms": ms}
     ctx.set_tuning(traverse_blocks=best["traverse_blocks"], traverse_expand=best["expand_levels"])
     for chunk, levels in local:
         ctx.set_tuning(traverse_local=chunk, traverse_local_levels=levels)
         ms = run(best["entry_level"], best["expand_levels"])
         if ms < best["ms"]:
             best = {**best, "traverse_local": chunk, "traverse_local_levels": levels, "ms": ms}
+    ctx.set_tuning(traverse_local=best["traverse_local"], traverse_local_levels=best["traverse_local_levels"])
+    for chunk, steps in tail:
+        ctx.set_tuning(traverse_tail=chunk, traverse_tail_steps=steps)
+        ms = run(best["entry_level"], best["expand_levels"])
+        if ms < best["ms"]:
+            best = {**best, "traverse_tail": chunk, "traverse_tail_steps": steps, "ms": ms}
     ctx.set_tuning(**{**base, "reduce_leaves_per_thread": best["reduce_leaves_per_thread"],
                       "traverse_blocks": best["traverse_blocks"], "traverse_expand": best["expand_levels"],
                       "traverse_local": best["traverse_local"],
-                      "traverse_local_levels": best["traverse_local_levels"]})
+                      "traverse_local_levels": best["traverse_local_levels"],
+                      "traverse_tail": best["traverse_tail"], "traverse_tail_steps": best["traverse_tail_steps"]})
     return best

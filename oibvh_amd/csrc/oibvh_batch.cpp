@@ -649,7 +649,7 @@ extern "C" oibvh_status oibvh_batch_frame(oibvh_batch* b, uint32_t entry, uint32
     const uint64_t key[6] = {entry | ((uint64_t)d.local_chunk << 32) | ((uint64_t)d.local_levels << 48),
                              d.expand | ((uint64_t)d.schedule << 32),
                              (flags & ~OIBVH_DETECT_ASYNC) | ((uint64_t)c->narrow_separate << 32),
-                             (uint64_t)b->reduce_M,
+                             (uint64_t)b->reduce_M | ((uint64_t)d.tail_chunk << 16) | ((uint64_t)d.tail_from << 40),
                              d.generation, (uint64_t)b->desc_gen ^ ((uint64_t)oi_grid_for(c) << 32)};
     if (!b->gexec || std::memcmp(key, b->graph_key, sizeof key) != 0) {
         if (b->gexec) {

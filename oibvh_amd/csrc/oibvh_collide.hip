@@ -711,14 +711,21 @@ __global__ void __launch_bounds__(TB) k_bvtt_step(StepArgs<Tab> A)
 }
 
 // ---------------------------------------------------------------------------
-// k_bvtt_local: the first J steps of the walk inside workgroups. Workgroup b
-// takes initial entries [b * chunk, (b + 1) * chunk) and walks them J levels
-// of E (J x E tree levels) breadth first with its front in LDS, testing leaf
-// pairs as it meets them (fused SAT, or the candidate list); the pairs kept
-// by the J-th level go to the global front, where the steps take over. The
-// top of the walk, whose fronts are small, so costs one launch instead of J.
-// A workgroup whose front outgrows its LDS queue before level J sets
-// OI_ERR_LOCAL_OVERFLOW and the detect is run again with steps only.
+// k_bvtt_local: J steps of the walk inside workgroups. Workgroup b takes
+// entries [c0, c0 + chunk) for c0 = b * chunk, (b + grid) * chunk, ... of
+// its input front and walks them J levels of E (J x E tree levels) breadth
+// first with its front in LDS, testing leaf pairs as it meets them (fused
+// SAT, or the candidate list); the pairs kept by the J-th level go to the
+// global front, where the steps take over. Two uses:
+//   head: the first J steps over the initial front (test_entries: the entry
+//         pairs are tested first, as the first step does), whose fronts are
+//         small, in one launch instead of J;
+//   tail: the last steps over a front the steps wrote (its length read from
+//         device memory), down to the leaves: nothing reaches the global
+//         front, and the front is read and written once instead of once per
+//         step.
+// A workgroup whose front outgrows its LDS queue sets `overflow_bit` and the
+// detect is run again without that walk.
 // ---------------------------------------------------------------------------
 #define LQ 2048
 
@@ -726,9 +733,13 @@ template <class Tab>
 struct LocalArgs {
     Tab tab;
     const uint2* front_in;
+    const uint32_t* count_in;  // length of front_in; null: n_in
     uint32_t n_in;
+    uint32_t cap_in;           // entries front_in can hold
     uint32_t chunk;
-    uint32_t levels;     // J >= 1
+    uint32_t levels;           // J >= 1
+    uint32_t test_entries;     // test the input pairs themselves (initial front)
+    uint32_t overflow_bit;     // OI_ERR_* bit set when a workgroup's LDS front overflows
     uint2* front_out;    // pairs kept by level J
     uint32_t* count_out;
     uint32_t cap_out;
@@ -754,106 +765,112 @@ __global__ void __launch_bounds__(TB) k_bvtt_local(LocalArgs<Tab> A)
     __shared__ uint32_t s_qn, s_obase, s_cbase;
     __shared__ uint2 s_q[2][LQ];
     __shared__ std::conditional_t<FUSED, FusedLds, char> s_f[1];
-    const uint32_t c0 = blockIdx.x * A.chunk;
-    if (c0 >= A.n_in) return;
+    // after an overflow the count keeps growing past the buffer (writes were
+    // dropped and the detect will be re-run): read only what exists
+    uint32_t n_in = A.count_in ? *A.count_in : A.n_in;
+    if (n_in > A.cap_in) n_in = A.cap_in;
+    if (blockIdx.x * A.chunk >= n_in) return;
     tab_init(A.tab, s_desc, s_ibase);
     const uint32_t tid = threadIdx.x;
     uint32_t n_tests = 0, n_adj = 0, n_cands = 0, n_fp64 = 0;
-    uint32_t n_cur = min(A.chunk, A.n_in - c0);
-    uint32_t cur = 0;
     bool overflow = false;
-    for (uint32_t lvl = 0; n_cur; lvl++) {
-        const bool first = lvl == 0;
-        const bool last = lvl + 1 >= A.levels;  // kept pairs go to the global front
-        if (tid == 0) s_qn = 0;
-        __syncthreads();
-        for (uint32_t base = 0; base < n_cur; base += TB) {
-            const uint32_t i = base + tid;
-            bool have = i < n_cur;
-            uint2 e = make_uint2(0, 0);
-            if (have) e = first ? A.front_in[c0 + i] : s_q[cur][i];
-            if (have && (e.x >= A.implicit_end || e.y >= A.implicit_end)) {
-                or_u32(A.error, OI_ERR_CORRUPT);
-                have = false;
-            }
-            u64 mask = 0;
-            bool leafout = false;
-            Side sa = {0, 0}, sb = {0, 0};
-            uint32_t baseA = 0, baseB = 0;
-            if (have) {
-                if (first)
-                    entry_expand<E, true, Tab>(A.tab, s_desc, s_ibase, e, mask, leafout, sa, sb, baseA, baseB,
-                                               n_tests, n_adj);
-                else
-                    entry_expand<E, false, Tab>(A.tab, s_desc, s_ibase, e, mask, leafout, sa, sb, baseA, baseB,
-                                                n_tests, n_adj);
-            }
-            const uint32_t nout = (uint32_t)__popcll(mask);
-            // one scan for both counts (front pairs high, leaf pairs low)
-            uint32_t TC;
-            const uint32_t oc = block_scan(leafout ? nout : (nout << 16), s_wsum, TC);
-            const uint32_t T = TC >> 16, C = TC & 0xffffu, off = oc >> 16, coff = oc & 0xffffu;
-            if (tid == 0) {
-                if (last) {
-                    s_obase = T ? add_u32(A.count_out, T) : 0u;
-                } else {
-                    s_obase = s_qn;
-                    s_qn += T;
-                }
-                if (!FUSED) s_cbase = C ? add_u32(A.cand_count, C) : 0u;
-            }
-            if constexpr (FUSED) {
-                s_f[0].cmask[tid] = leafout ? mask : 0ull;
-                s_f[0].coff[tid] = coff;
-                s_f[0].ga[tid] = baseA + sa.start;
-                s_f[0].gb[tid] = baseB + sb.start;
-            }
+    for (uint32_t c0 = blockIdx.x * A.chunk; c0 < n_in; c0 += gridDim.x * A.chunk) {
+        uint32_t n_cur = min(A.chunk, n_in - c0);
+        uint32_t cur = 0;
+        for (uint32_t lvl = 0; n_cur; lvl++) {
+            const bool first = lvl == 0;
+            const bool test = first && A.test_entries;
+            const bool last = lvl + 1 >= A.levels;  // kept pairs go to the global front
+            if (tid == 0) s_qn = 0;
             __syncthreads();
-            if (nout && !leafout) {
-                uint32_t o = s_obase + off;
-                if (last && o + nout > A.cap_out) or_u32(A.error, OI_ERR_QUEUE_OVERFLOW);
-                u64 m = mask;
-                while (m) {
-                    const uint32_t k = (uint32_t)__ffsll((long long)m) - 1u;
-                    m &= m - 1ull;
-                    const uint2 pr = make_uint2(baseA + sa.start + (k >> E), baseB + sb.start + (k & (K - 1)));
+            for (uint32_t base = 0; base < n_cur; base += TB) {
+                const uint32_t i = base + tid;
+                bool have = i < n_cur;
+                uint2 e = make_uint2(0, 0);
+                if (have) e = first ? A.front_in[c0 + i] : s_q[cur][i];
+                if (have && (e.x >= A.implicit_end || e.y >= A.implicit_end)) {
+                    or_u32(A.error, OI_ERR_CORRUPT);
+                    have = false;
+                }
+                u64 mask = 0;
+                bool leafout = false;
+                Side sa = {0, 0}, sb = {0, 0};
+                uint32_t baseA = 0, baseB = 0;
+                if (have) {
+                    if (test)
+                        entry_expand<E, true, Tab>(A.tab, s_desc, s_ibase, e, mask, leafout, sa, sb, baseA, baseB,
+                                                   n_tests, n_adj);
+                    else
+                        entry_expand<E, false, Tab>(A.tab, s_desc, s_ibase, e, mask, leafout, sa, sb, baseA, baseB,
+                                                    n_tests, n_adj);
+                }
+                const uint32_t nout = (uint32_t)__popcll(mask);
+                // one scan for both counts (front pairs high, leaf pairs low)
+                uint32_t TC;
+                const uint32_t oc = block_scan(leafout ? nout : (nout << 16), s_wsum, TC);
+                const uint32_t T = TC >> 16, C = TC & 0xffffu, off = oc >> 16, coff = oc & 0xffffu;
+                if (tid == 0) {
                     if (last) {
-                        if (o < A.cap_out) A.front_out[o] = pr;
-                    } else if (o < LQ) {
-                        s_q[cur ^ 1u][o] = pr;
+                        s_obase = T ? add_u32(A.count_out, T) : 0u;
+                    } else {
+                        s_obase = s_qn;
+                        s_qn += T;
                     }
-                    o++;
+                    if (!FUSED) s_cbase = C ? add_u32(A.cand_count, C) : 0u;
                 }
-            }
-            if (!FUSED && nout && leafout) {
-                uint32_t o = s_cbase + coff;
-                if (o + nout > A.cand_cap) or_u32(A.error, OI_ERR_CAND_OVERFLOW);
-                u64 m = mask;
-                while (m) {
-                    const uint32_t k = (uint32_t)__ffsll((long long)m) - 1u;
-                    m &= m - 1ull;
-                    if (o < A.cand_cap)
-                        A.cands[o] = make_uint2(baseA + sa.start + (k >> E), baseB + sb.start + (k & (K - 1)));
-                    o++;
+                if constexpr (FUSED) {
+                    s_f[0].cmask[tid] = leafout ? mask : 0ull;
+                    s_f[0].coff[tid] = coff;
+                    s_f[0].ga[tid] = baseA + sa.start;
+                    s_f[0].gb[tid] = baseB + sb.start;
                 }
-            }
-            if constexpr (FUSED) {
-                if (C) {
-                    if (tid == 0) n_cands += C;
-                    fused_narrow<E, Tab>(A.tab, s_ibase, s_desc, s_f[0], C, s_wsum, A.hits, A.n_hits, A.hits_cap,
-                                         A.error, n_fp64, n_adj);
+                __syncthreads();
+                if (nout && !leafout) {
+                    uint32_t o = s_obase + off;
+                    if (last && o + nout > A.cap_out) or_u32(A.error, OI_ERR_QUEUE_OVERFLOW);
+                    u64 m = mask;
+                    while (m) {
+                        const uint32_t k = (uint32_t)__ffsll((long long)m) - 1u;
+                        m &= m - 1ull;
+                        const uint2 pr = make_uint2(baseA + sa.start + (k >> E), baseB + sb.start + (k & (K - 1)));
+                        if (last) {
+                            if (o < A.cap_out) A.front_out[o] = pr;
+                        } else if (o < LQ) {
+                            s_q[cur ^ 1u][o] = pr;
+                        }
+                        o++;
+                    }
                 }
+                if (!FUSED && nout && leafout) {
+                    uint32_t o = s_cbase + coff;
+                    if (o + nout > A.cand_cap) or_u32(A.error, OI_ERR_CAND_OVERFLOW);
+                    u64 m = mask;
+                    while (m) {
+                        const uint32_t k = (uint32_t)__ffsll((long long)m) - 1u;
+                        m &= m - 1ull;
+                        if (o < A.cand_cap)
+                            A.cands[o] = make_uint2(baseA + sa.start + (k >> E), baseB + sb.start + (k & (K - 1)));
+                        o++;
+                    }
+                }
+                if constexpr (FUSED) {
+                    if (C) {
+                        if (tid == 0) n_cands += C;
+                        fused_narrow<E, Tab>(A.tab, s_ibase, s_desc, s_f[0], C, s_wsum, A.hits, A.n_hits, A.hits_cap,
+                                             A.error, n_fp64, n_adj);
+                    }
+                }
+                __syncthreads();
             }
+            if (last) break;
+            const uint32_t q = s_qn;
+            if (q > LQ) overflow = true;
+            n_cur = q < LQ ? q : LQ;
+            cur ^= 1u;
             __syncthreads();
         }
-        if (last) break;
-        const uint32_t q = s_qn;
-        if (q > LQ) overflow = true;
-        n_cur = q < LQ ? q : LQ;
-        cur ^= 1u;
-        __syncthreads();
     }
-    if (tid == 0 && overflow) or_u32(A.error, OI_ERR_LOCAL_OVERFLOW);
+    if (tid == 0 && overflow) or_u32(A.error, A.overflow_bit);
     uint32_t nt, na;
     block_scan(n_tests, s_wsum, nt);
     block_scan(n_adj, s_wsum, na);
@@ -1048,22 +1065,29 @@ static hipError_t launch_step(hipStream_t st, uint32_t grid, const Tab& tab, int
 }
 
 template <class Tab>
-static hipError_t launch_local(hipStream_t st, const Tab& tab, const uint2* front_in, uint32_t n_in, uint32_t chunk,
-                               uint32_t levels, uint2* front_out, uint32_t* count_out, uint32_t cap_out,
-                               uint2* cands, uint32_t* cand_count, uint32_t cand_cap, uint4* hits,
-                               uint32_t hits_cap, oi_detect_ctl* ctl, uint32_t expand, uint32_t implicit_end,
-                               int fused)
+static hipError_t launch_local(hipStream_t st, const Tab& tab, const oi_local_args& w, uint2* cands,
+                               uint32_t* cand_count, uint32_t cand_cap, uint4* hits, uint32_t hits_cap,
+                               oi_detect_ctl* ctl, uint32_t expand, uint32_t implicit_end, int fused)
 {
-    if (n_in == 0) return hipSuccess;
+    // without a device count the grid is one workgroup per chunk
+    const uint64_t chunks = w.count_in ? ((uint64_t)w.cap_in + w.chunk - 1) / w.chunk
+                                       : ((uint64_t)w.n_in + w.chunk - 1) / w.chunk;
+    const uint64_t grid = (w.count_in && w.grid && w.grid < chunks) ? w.grid : chunks;
+    if (grid == 0) return hipSuccess;
+    if (grid > 0x7fffffffull) return hipErrorInvalidValue;
     LocalArgs<Tab> a{};
     a.tab = tab;
-    a.front_in = front_in;
-    a.n_in = n_in;
-    a.chunk = chunk;
-    a.levels = levels;
-    a.front_out = front_out;
-    a.count_out = count_out;
-    a.cap_out = cap_out;
+    a.front_in = w.front_in;
+    a.count_in = w.count_in;
+    a.n_in = w.n_in;
+    a.cap_in = w.count_in ? w.cap_in : w.n_in;
+    a.chunk = w.chunk;
+    a.levels = w.levels;
+    a.test_entries = w.test_entries;
+    a.overflow_bit = w.overflow_bit;
+    a.front_out = w.front_out;
+    a.count_out = w.count_out;
+    a.cap_out = w.cap_out;
     a.cands = cands;
     a.cand_count = cand_count;
     a.cand_cap = cand_cap;
@@ -1074,7 +1098,7 @@ static hipError_t launch_local(hipStream_t st, const Tab& tab, const uint2* fron
     a.n_hits = &ctl->n_hits;
     a.hits_cap = hits_cap;
     a.implicit_end = implicit_end;
-    const dim3 g((n_in + chunk - 1) / chunk), b(TB);
+    const dim3 g((uint32_t)grid), b(TB);
 #define OI_LOCAL(EE)                                                           \
     if (fused)                                                                 \
         hipLaunchKernelGGL((k_bvtt_local<EE, true, Tab>), g, b, 0, st, a);     \
@@ -1143,20 +1167,16 @@ hipError_t oi_launch_bvtt_step(hipStream_t st, uint32_t grid, const oi_tab_args*
 
 // the first `levels` steps in one launch, one workgroup per `chunk` initial
 // entries; the pairs kept by the last of them go to front_out / count_out
-hipError_t oi_launch_bvtt_local(hipStream_t st, const oi_tab_args* t, const uint2* front_in, uint32_t n_in,
-                                uint32_t chunk, uint32_t levels, uint2* front_out, uint32_t* count_out,
-                                uint32_t cap_out, uint2* cands, uint32_t* cand_count, uint32_t cand_cap,
-                                uint4* hits, uint32_t hits_cap, oi_detect_ctl* ctl, uint32_t expand,
-                                uint32_t implicit_end, int fused)
+hipError_t oi_launch_bvtt_local(hipStream_t st, const oi_tab_args* t, const oi_local_args* w, uint2* cands,
+                                uint32_t* cand_count, uint32_t cand_cap, uint4* hits, uint32_t hits_cap,
+                                oi_detect_ctl* ctl, uint32_t expand, uint32_t implicit_end, int fused)
 {
-    if (chunk == 0 || levels == 0) return hipErrorInvalidValue;
+    if (w->chunk == 0 || w->levels == 0 || w->chunk > LQ) return hipErrorInvalidValue;
     if (t->chunk_mesh)
-        return launch_local(st, ChunkTab{t->d_meshes, t->chunk_mesh, t->shift}, front_in, n_in, chunk, levels,
-                            front_out, count_out, cap_out, cands, cand_count, cand_cap, hits, hits_cap, ctl, expand,
-                            implicit_end, fused);
-    return launch_local(st, make_tab(t->d_meshes, t->h_meshes, t->n_meshes), front_in, n_in, chunk, levels,
-                        front_out, count_out, cap_out, cands, cand_count, cand_cap, hits, hits_cap, ctl, expand,
-                        implicit_end, fused);
+        return launch_local(st, ChunkTab{t->d_meshes, t->chunk_mesh, t->shift}, *w, cands, cand_count, cand_cap,
+                            hits, hits_cap, ctl, expand, implicit_end, fused);
+    return launch_local(st, make_tab(t->d_meshes, t->h_meshes, t->n_meshes), *w, cands, cand_count, cand_cap, hits,
+                        hits_cap, ctl, expand, implicit_end, fused);
 }
 
 // Scene::convertToVertexArray (scene.cu:69-93): six vertices per hit, the

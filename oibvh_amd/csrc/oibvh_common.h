@@ -192,7 +192,24 @@ struct oi_seed_args {
 #define OI_ERR_CORRUPT 4u
 #define OI_ERR_HITS_OVERFLOW 8u
 #define OI_ERR_CAND_OVERFLOW 16u
-#define OI_ERR_LOCAL_OVERFLOW 32u  // a workgroup's front outgrew its LDS queue (local walk)
+#define OI_ERR_LOCAL_OVERFLOW 32u  // a workgroup's front outgrew its LDS queue (head local walk)
+#define OI_ERR_TAIL_OVERFLOW 64u   // the same in the tail local walk
+
+// one launch of the workgroup-local walk (k_bvtt_local)
+struct oi_local_args {
+    const uint2* front_in;
+    const uint32_t* count_in;  // device length of front_in (null: n_in, one workgroup per chunk)
+    uint32_t n_in;
+    uint32_t cap_in;           // entries front_in holds (with count_in)
+    uint32_t chunk;            // entries per workgroup and pass, at most 2048
+    uint32_t levels;           // steps walked; the last one's non-leaf pairs go to front_out
+    uint32_t grid;             // with count_in: at most this many workgroups (0: one per chunk of cap_in)
+    uint32_t test_entries;     // test the input pairs themselves (the initial front)
+    uint32_t overflow_bit;     // error bit set on an LDS front overflow
+    uint2* front_out;
+    uint32_t* count_out;
+    uint32_t cap_out;
+};
 
 // a scene's concatenated implicit index space must stay below this bound
 #define OI_IMPLICIT_LIMIT 0xfffffffeu

@@ -97,6 +97,23 @@ oibvh_status oi_det_prepare(oi_detector* d)
         d->local_levels = c->trav_local_levels ? std::min(c->trav_local_levels, k) : k;
     }
     if (d->local_off) d->local_off--;
+    // the last steps as a tail walk when asked for (unless it overflowed
+    // recently): it starts after the head walk and after every step whose
+    // depth the schedule set, since it descends `expand` levels per step
+    d->tail_chunk = 0;
+    d->tail_from = 0;
+    if (c->trav_tail && d->tail_off == 0 && d->n_front) {
+        const uint32_t S = c->trav_tail_steps;
+        uint32_t T = (S && S < d->n_levels) ? d->n_levels - S : 0u;
+        T = std::max(T, d->local_chunk ? d->local_levels : 0u);
+        for (uint32_t j = T; j < d->n_levels; j++)
+            if (d->step_e[j] != d->expand) T = j + 1;
+        if (T < d->n_levels) {
+            d->tail_chunk = c->trav_tail;
+            d->tail_from = T;
+        }
+    }
+    if (d->tail_off) d->tail_off--;
     const uint32_t fcap = std::max(c->queue_cap, oi_np2(std::max(1u, d->n_front)));
     if (d->front_cap < fcap) {
         dfree(d->front[0]);
@@ -139,17 +156,48 @@ oibvh_status oi_det_enqueue(oi_detector* d, bool reset_sticky, bool seeded)
     const uint32_t J = d->local_chunk ? d->local_levels : 0u;
     if (J) {
         // steps 0 .. J-1 in one launch; its front goes where step J - 1's would
-        HIPCHK(c, oi_launch_bvtt_local(st, &d->tab, d->d_init, d->n_front, d->local_chunk, J, d->front[(J - 1) & 1],
-                                       d->counts + J, d->front_cap, d->cands, &d->ctl->cands, d->cand_cap, d->hits,
-                                       d->hits_cap, d->ctl, d->expand, d->implicit_end, d->fused ? 1 : 0));
+        oi_local_args w{};
+        w.front_in = d->d_init;
+        w.n_in = d->n_front;
+        w.chunk = d->local_chunk;
+        w.levels = J;
+        w.test_entries = 1;
+        w.overflow_bit = OI_ERR_LOCAL_OVERFLOW;
+        w.front_out = d->front[(J - 1) & 1];
+        w.count_out = d->counts + J;
+        w.cap_out = d->front_cap;
+        HIPCHK(c, oi_launch_bvtt_local(st, &d->tab, &w, d->cands, &d->ctl->cands, d->cand_cap, d->hits, d->hits_cap,
+                                       d->ctl, d->expand, d->implicit_end, d->fused ? 1 : 0));
     }
-    for (uint32_t k = J; k < d->n_levels; k++) {
+    const uint32_t T = d->tail_chunk ? d->tail_from : d->n_levels;
+    for (uint32_t k = J; k < T; k++) {
         // step 0 reads the initial front in place; then the fronts alternate
         const uint2* in = k ? d->front[(k - 1) & 1] : d->d_init;
         HIPCHK(c, oi_launch_bvtt_step(st, grid, &d->tab, k == 0, in, k ? d->counts + k : nullptr, d->n_front,
                                       k ? d->front_cap : d->n_front, d->front[k & 1], d->counts + k + 1,
                                       d->front_cap, d->cands, &d->ctl->cands, d->cand_cap, d->hits, d->hits_cap,
                                       d->ctl, d->step_e[k], d->implicit_end, d->fused ? 1 : 0));
+    }
+    if (T < d->n_levels) {
+        // steps T .. n_levels - 1 in one launch over the front step T - 1
+        // left: every pair ends at the leaves, so the last level writes no
+        // front (its count, checked below, stays 0); front_out is the buffer
+        // not being read
+        oi_local_args w{};
+        w.front_in = T ? d->front[(T - 1) & 1] : d->d_init;
+        w.count_in = T ? d->counts + T : nullptr;
+        w.n_in = d->n_front;
+        w.cap_in = T ? d->front_cap : d->n_front;
+        w.chunk = d->tail_chunk;
+        w.levels = d->n_levels - T;
+        w.grid = 4u * (uint32_t)std::max(1, c->cus);
+        w.test_entries = T == 0 ? 1u : 0u;
+        w.overflow_bit = OI_ERR_TAIL_OVERFLOW;
+        w.front_out = d->front[T & 1];
+        w.count_out = d->counts + d->n_levels;
+        w.cap_out = d->front_cap;
+        HIPCHK(c, oi_launch_bvtt_local(st, &d->tab, &w, d->cands, &d->ctl->cands, d->cand_cap, d->hits, d->hits_cap,
+                                       d->ctl, d->expand, d->implicit_end, d->fused ? 1 : 0));
     }
     if (!d->fused)
         HIPCHK(c, oi_launch_narrow(st, grid, &d->tab, d->cands, &d->ctl->cands, d->cand_cap, d->hits,
@@ -190,6 +238,10 @@ oibvh_status oi_det_check(oi_detector* d)
         if (k.error & OI_ERR_LOCAL_OVERFLOW) {
             // a workgroup's front outgrew LDS: steps for the next queries
             d->local_off = 64;
+            rerun = true;
+        }
+        if (k.error & OI_ERR_TAIL_OVERFLOW) {
+            d->tail_off = 64;
             rerun = true;
         }
         uint32_t maxfront = 0;

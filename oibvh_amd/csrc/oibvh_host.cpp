@@ -152,6 +152,8 @@ extern "C" oibvh_status oibvh_ctx_set_tuning(oibvh_ctx* ctx, const oibvh_tuning*
     if (t->front_capacity && (t->front_capacity & (t->front_capacity - 1)))
         return oi_set_err(ctx, OIBVH_ERR_INVALID, "front_capacity must be a power of two");
     if (t->traverse_expand > 3) return oi_set_err(ctx, OIBVH_ERR_INVALID, "traverse_expand must be 0..3");
+    if (t->traverse_local > 2048 || t->traverse_tail > 2048)
+        return oi_set_err(ctx, OIBVH_ERR_INVALID, "traverse_local and traverse_tail must be 0..2048");
     ctx->leaves_per_thread = lpt;
     ctx->trav_blocks = t->traverse_blocks;
     ctx->trav_expand = t->traverse_expand;
@@ -161,6 +163,8 @@ extern "C" oibvh_status oibvh_ctx_set_tuning(oibvh_ctx* ctx, const oibvh_tuning*
     ctx->trav_schedule = t->traverse_schedule;
     ctx->trav_local = t->traverse_local;
     ctx->trav_local_levels = t->traverse_local_levels;
+    ctx->trav_tail = t->traverse_tail;
+    ctx->trav_tail_steps = t->traverse_tail_steps;
     ctx->narrow_separate = t->narrow_separate;
     return OIBVH_OK;
 }
@@ -176,6 +180,8 @@ extern "C" oibvh_status oibvh_ctx_get_tuning(const oibvh_ctx* ctx, oibvh_tuning*
     t->traverse_schedule = ctx->trav_schedule;
     t->traverse_local = ctx->trav_local;
     t->traverse_local_levels = ctx->trav_local_levels;
+    t->traverse_tail = ctx->trav_tail;
+    t->traverse_tail_steps = ctx->trav_tail_steps;
     t->narrow_separate = ctx->narrow_separate;
     t->hits_capacity = ctx->hits_cap;
     return OIBVH_OK;

@@ -46,9 +46,8 @@ hipError_t oi_launch_bvtt_step(hipStream_t, uint32_t, const oi_tab_args*, int, c
                                uint32_t, oi_detect_ctl*, uint32_t, uint32_t, int);
 hipError_t oi_launch_narrow(hipStream_t, uint32_t, const oi_tab_args*, const uint2*, const uint32_t*, uint32_t,
                             uint4*, uint32_t*, uint32_t, oi_detect_ctl*, int);
-hipError_t oi_launch_bvtt_local(hipStream_t, const oi_tab_args*, const uint2*, uint32_t, uint32_t, uint32_t, uint2*,
-                                uint32_t*, uint32_t, uint2*, uint32_t*, uint32_t, uint4*, uint32_t, oi_detect_ctl*,
-                                uint32_t, uint32_t, int);
+hipError_t oi_launch_bvtt_local(hipStream_t, const oi_tab_args*, const oi_local_args*, uint2*, uint32_t*, uint32_t,
+                                uint4*, uint32_t, oi_detect_ctl*, uint32_t, uint32_t, int);
 hipError_t oi_launch_pair_vertices(hipStream_t, const oi_build_desc*, const uint4*, uint32_t, float*);
 }
 
@@ -66,6 +65,8 @@ struct oibvh_ctx {
     uint32_t trav_schedule = 0;
     uint32_t trav_local = 0;         // 0: BVTT steps only; n: the first steps as a local walk, n pairs per workgroup
     uint32_t trav_local_levels = 0;  // steps the local walk replaces (0: all)
+    uint32_t trav_tail = 0;          // 0: no tail walk; n: the last steps as a local walk, n pairs per workgroup
+    uint32_t trav_tail_steps = 0;    // steps the tail walk replaces (0: all after the head walk)
     uint32_t narrow_separate = 0;    // FP32 queries test leaf pairs in one k_narrow launch
     uint32_t queue_cap = 1u << 22;
     uint32_t hits_cap = 1u << 20;
@@ -146,6 +147,9 @@ struct oi_detector {
     uint32_t local_chunk = 0;   // this query: initial pairs per workgroup of the local walk (0: steps)
     uint32_t local_levels = 0;  // this query: steps done by the local walk
     uint32_t local_off = 0;     // queries left that use steps after a local walk overflowed
+    uint32_t tail_chunk = 0;    // this query: front pairs per workgroup of the tail walk (0: none)
+    uint32_t tail_from = 0;     // this query: first step done by the tail walk
+    uint32_t tail_off = 0;      // queries left without a tail walk after one overflowed
     bool fused = false;  // narrow phase inside the BVTT steps (FP32 queries)
     bool unchecked = false;
     bool have_result = false;

@@ -72,3 +72,29 @@ def test_facade_header_compiles():
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), "-x", "c++", "-"],
                        input=src, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def header_struct_fields(name):
+    """Field names of `typedef struct { ... } name;` in include/oibvh.h, in
+    order (comments dropped)."""
+    txt = re.sub(r"/\*.*?\*/", "", open(HDR).read(), flags=re.S)
+    body = re.search(r"typedef struct \{([^}]*)\}\s*" + name + r"\s*;", txt).group(1)
+    return re.findall(r"\b(\w+)\s*;", body)
+
+
+@pytest.mark.parametrize("cname,pyname", [("oibvh_tuning", "Tuning"), ("oibvh_detect_stats", "DetectStats"),
+                                          ("oibvh_scene_timing", "SceneTiming")])
+def test_ctypes_structs_match_header(cname, pyname):
+    """The ctypes mirrors of the header's structs have the same fields in the
+    same order (the size follows: every field is a fixed-width type)."""
+    from oibvh_amd import _lib
+
+    py = getattr(_lib, pyname)
+    assert [f for f, _ in py._fields_] == header_struct_fields(cname)
+    src = '#include "oibvh.h"\n#include <stdio.h>\nint main(){ printf("%zu", sizeof(' + cname + ')); return 0; }\n'
+    exe = os.path.join(ROOT, "_build", "abi_sizeof_" + cname)
+    os.makedirs(os.path.dirname(exe), exist_ok=True)
+    r = subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-x", "c", "-", "-o", exe], input=src,
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert int(subprocess.run([exe], capture_output=True, text=True).stdout) == C.sizeof(py)

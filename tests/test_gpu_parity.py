@@ -487,33 +487,39 @@ def test_detect_cfg1_full_size(api, ctx, orc):
 
 @pytest.mark.parametrize("entry", [0, 1, 3, 6, 30])
 @pytest.mark.parametrize("expand", [1, 2, 3, 5])
-@pytest.mark.parametrize("mode", [(0, 0), (64, 0), (7, 0), (64, 2), (16, 1)])
+@pytest.mark.parametrize("mode", [(0, 0), (64, 0), (7, 0), (64, 2), (16, 1),
+                                  (0, 0, 32, 0), (64, 1, 16, 0), (0, 0, 64, 2), (7, 2, 5, 1)])
 def test_detect_entry_expand_invariance(api, ctx, orc, entry, expand, mode):
-    """mode: (traverse_local, traverse_local_levels): steps only, or the
-    first steps (all of them, 2 or 1) as a local walk of n initial pairs per
-    workgroup."""
-    say(f"detect entry={entry} expand={expand} traverse_local={mode}")
+    """mode: (traverse_local, traverse_local_levels[, traverse_tail,
+    traverse_tail_steps]): steps only; the first steps (all of them, 2 or 1)
+    as a head local walk of n initial pairs per workgroup; the last steps
+    (all, or the last 2 or 1) as a tail walk of n front pairs per workgroup
+    pass, alone or after a head walk."""
+    say(f"detect entry={entry} expand={expand} local/tail={mode}")
     meshes = M.interpenetrating_icospheres(subdiv=4, seed=21, offset=1.3)
     want = orc.detect([orc.Tree(x, f) for x, f in meshes])
-    ctx.set_tuning(traverse_local=mode[0], traverse_local_levels=mode[1])
+    tail = mode[2:] or (0, 0)
+    ctx.set_tuning(traverse_local=mode[0], traverse_local_levels=mode[1], traverse_tail=tail[0],
+                   traverse_tail_steps=tail[1])
     try:
         p, st, _ = gpu_pairs(api, ctx, meshes, entry, expand)
     finally:
-        ctx.set_tuning(traverse_local=0, traverse_local_levels=0)
+        ctx.set_tuning(traverse_local=0, traverse_local_levels=0, traverse_tail=0, traverse_tail_steps=0)
     assert np.array_equal(p, want)
     assert st["retries"] == 0
 
 
-@pytest.mark.parametrize("local", [0, 64])
-def test_detect_narrow_separate(api, ctx, orc, local):
+@pytest.mark.parametrize("local,tail", [(0, 0), (64, 0), (64, 32)])
+def test_detect_narrow_separate(api, ctx, orc, local, tail):
     """FP32 leaf pairs collected and tested by one k_narrow launch
-    (narrow_separate = 1): the same pairs, pair sets and self mode."""
-    say(f"detect narrow_separate local={local}")
+    (narrow_separate = 1): the same pairs, pair sets and self mode, with
+    steps, a head walk, or a head and a tail walk."""
+    say(f"detect narrow_separate local={local} tail={tail}")
     meshes = M.interpenetrating_icospheres(subdiv=4, seed=23, offset=1.2)
     want = orc.detect([orc.Tree(x, f) for x, f in meshes])
     x, f = M.cloth(64, amplitude=0.2, wavelength=0.2, seed=5)
     want_self = orc.detect([orc.Tree(x, f)], self_collide=True)
-    ctx.set_tuning(narrow_separate=1, traverse_local=local, traverse_local_levels=2)
+    ctx.set_tuning(narrow_separate=1, traverse_local=local, traverse_local_levels=2, traverse_tail=tail)
     try:
         for entry, expand in [(0, 1), (3, 2), (2, 3)]:
             p, st, _ = gpu_pairs(api, ctx, meshes, entry, expand)
@@ -522,7 +528,7 @@ def test_detect_narrow_separate(api, ctx, orc, local):
         p, _, _ = gpu_pairs(api, ctx, [(x, f)], 2, 2, self_collide=True)
         assert np.array_equal(p, want_self)
     finally:
-        ctx.set_tuning(narrow_separate=0, traverse_local=0, traverse_local_levels=0)
+        ctx.set_tuning(narrow_separate=0, traverse_local=0, traverse_local_levels=0, traverse_tail=0)
 
 
 def test_detect_local_walk_overflow_falls_back(api, ctx, orc):
@@ -546,6 +552,34 @@ def test_detect_local_walk_overflow_falls_back(api, ctx, orc):
         sc.detectCollision(0, 3)
         assert np.array_equal(sc.getIntTriPairs(), want)
         assert sc.stats()["retries"] == 0 and len(sc.level_counts()) > 1  # steps now
+        sc.close()
+    finally:
+        ctx2.close()
+
+
+def test_detect_tail_walk_overflow_falls_back(api, ctx, orc):
+    """The same twin spheres walked from the roots by the tail walk alone
+    (64 pairs per workgroup, every step): its LDS front overflows, the
+    detect is re-run without it (same pairs) and the next detects use
+    steps."""
+    say("tail walk overflow")
+    a = M.icosphere(5, 1.0, 0.0, 0)
+    want = orc.detect([orc.Tree(*a), orc.Tree(*a)])
+    ctx2 = api.Context(0)
+    ctx2.set_tuning(traverse_tail=64)
+    try:
+        trees = [gpu_tree(api, ctx2, *a), gpu_tree(api, ctx2, *a)]
+        sc = api.Scene(ctx2)
+        for t in trees:
+            sc.addOibvhTree(t)
+        sc.detectCollision(0, 3)
+        assert np.array_equal(sc.getIntTriPairs(), want)
+        assert sc.stats()["retries"] >= 1
+        sc.detectCollision(0, 3)
+        assert np.array_equal(sc.getIntTriPairs(), want)
+        assert sc.stats()["retries"] == 0
+        lc = sc.level_counts()
+        assert len(lc) > 1 and all(v > 0 for v in lc)  # every step wrote its front
         sc.close()
     finally:
         ctx2.close()
@@ -587,8 +621,9 @@ def test_detect_self_collision(api, ctx, orc):
     p, st, _ = gpu_pairs(api, ctx, [(x, f)], 0, 1, self_collide=True)
     assert np.array_equal(p, want)
     assert st["adjacent"] > 0
-    p, _, _ = gpu_pairs(api, ctx, [(x, f)], 3, 3, self_collide=True)
-    assert np.array_equal(p, want)
+    for entry, expand in [(3, 3), (2, 2)]:
+        p, _, _ = gpu_pairs(api, ctx, [(x, f)], entry, expand, self_collide=True)
+        assert np.array_equal(p, want)
 
 
 def test_detect_overflow_retry(api, ctx, orc):
