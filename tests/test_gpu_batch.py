@@ -177,14 +177,13 @@ def test_batch_morton63(api, ctx, orc):
     assert np.array_equal(out[0], all_pairs(orc, rs, mats))
 
 
-@pytest.mark.parametrize("fp64,levels,tail,split", [(False, 0, 0, 0), (True, 0, 0, 0), (False, 2, 0, 0),
-                                                    (False, 1, 16, 0), (True, 1, 16, 0), (False, 1, 16, 1),
-                                                    (True, 2, 0, 1)])
-def test_batch_local_walk(api, ctx, orc, fp64, levels, tail, split):
+@pytest.mark.parametrize("fp64,levels,tail", [(False, 0, 0), (True, 0, 0), (False, 2, 0), (False, 1, 16),
+                                              (True, 1, 16)])
+def test_batch_local_walk(api, ctx, orc, fp64, levels, tail):
     """The workgroup-local walk (traverse_local = n) over a batch front, FP32
     fused and FP64-fallback, alone or followed by a tail walk (traverse_tail
-    = n) for the steps after it, one lane or 2^E lanes per entry: the same
-    pairs as the oracle, eager and from a captured frame."""
+    = n) for the steps after it: the same pairs as the oracle, eager and
+    from a captured frame."""
     rs = small_rigid(n_scenes=6, seed=0x10CA)
     mats = rs.transforms(2)
     want = all_pairs(orc, rs, mats, fp64=fp64)
@@ -192,16 +191,16 @@ def test_batch_local_walk(api, ctx, orc, fp64, levels, tail, split):
     b = make_batch(api, ctx, rs)
     b.set_transforms(mats)
     b.build()
-    ctx.set_tuning(traverse_local=32, traverse_local_levels=levels, traverse_tail=tail, traverse_split=split)
+    ctx.set_tuning(traverse_local=32, traverse_local_levels=levels, traverse_tail=tail)
     try:
-        say(f"batch local walk fp64={fp64} levels={levels} tail={tail} split={split}")
+        say(f"batch local walk fp64={fp64} levels={levels} tail={tail}")
         b.detect(1, 2, fp64_fallback=fp64)
         assert np.array_equal(b.pairs(), want)
         b.frame(1, 2, fp64_fallback=fp64)
         assert np.array_equal(b.pairs(), want)
         assert b.stats()["retries"] == 0
     finally:
-        ctx.set_tuning(traverse_local=0, traverse_local_levels=0, traverse_tail=0, traverse_split=0)
+        ctx.set_tuning(traverse_local=0, traverse_local_levels=0, traverse_tail=0)
     b.close()
 
 
