@@ -88,6 +88,8 @@ def parse():
                    help="with --entry-level/--expand-levels: front pairs per workgroup pass of the tail walk (0: none)")
     p.add_argument("--traverse-tail-steps", type=int, default=0,
                    help="steps the tail walk replaces, counted from the last (0: all after the head walk)")
+    p.add_argument("--narrow-separate", type=int, default=None,
+                   help="with --entry-level/--expand-levels: 1 tests leaf pairs in one k_narrow launch")
     p.add_argument("--rebuild-every", type=int, default=50)
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -163,6 +165,8 @@ def run_plain(args, ctx):
         ctx.set_tuning(traverse_local=args.traverse_local, traverse_local_levels=args.traverse_local_levels)
     if args.traverse_tail is not None:
         ctx.set_tuning(traverse_tail=args.traverse_tail, traverse_tail_steps=args.traverse_tail_steps)
+    if args.narrow_separate is not None:
+        ctx.set_tuning(narrow_separate=args.narrow_separate)
     if args.workload == "deforming-uv-pair":
         meshes, waves = M.deforming_uv_pair()
         n_frames = 200

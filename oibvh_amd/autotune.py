@@ -42,15 +42,16 @@ def _time_refit(ctx: api.Context, frame, reps: int) -> float:
 
 
 def tune(ctx: api.Context, scene: api.Scene, frame, *, entries=(4, 6, 8), expands=(1, 2, 3),
-         blocks=(0, 2048), leaves=(16, 2, 4, 8), local=((64, 2), (128, 3), (256, 3)),
-         tail=((32, 0), (64, 0), (128, 0), (256, 2)), reps: int = 5) -> dict:
+         blocks=(0, 2048), leaves=(16, 2, 4, 8), local=((64, 2), (128, 3), (256, 2), (256, 3)),
+         tail=((32, 0), (64, 0), (64, 2), (128, 2)), narrow=(0, 1), reps: int = 5) -> dict:
     """Return {'entry_level', 'expand_levels', 'traverse_blocks',
     'reduce_leaves_per_thread', 'traverse_local', 'traverse_local_levels',
-    'traverse_tail', 'traverse_tail_steps', 'ms'} of the fastest setting and
-    apply the tuning knobs to ctx. `local` lists the (initial pairs per
-    workgroup, steps) settings of the head local walk tried at the best step
-    shape, `tail` the (front pairs per workgroup pass, steps) settings of the
-    tail walk tried after the best head setting."""
+    'traverse_tail', 'traverse_tail_steps', 'narrow_separate', 'ms'} of the
+    fastest setting and apply the tuning knobs to ctx. `local` lists the
+    (initial pairs per workgroup, steps) settings of the head local walk
+    tried at the best step shape, `tail` the (front pairs per workgroup pass,
+    steps) settings of the tail walk tried after the best head setting, with
+    leaf pairs tested inside the walk or in one k_narrow launch (`narrow`)."""
     best = None
     # reduce variant first, timed on the refit alone (the detect does not
     # depend on it), then the traversal shape
@@ -77,19 +78,24 @@ def tune(ctx: api.Context, scene: api.Scene, frame, *, entries=(4, 6, 8), expand
         ms = _time(ctx, scene, frame, best["entry_level"], best["expand_levels"], reps)
         if ms < best["ms"]:
             best = {**best, "traverse_local": chunk, "traverse_local_levels": levels, "ms": ms}
-    # then the last steps as a tail walk, after the best head setting
+    # then the last steps as a tail walk, after the best head setting, and
+    # where the leaf pairs are tested
     ctx.set_tuning(traverse_local=best["traverse_local"], traverse_local_levels=best["traverse_local_levels"])
-    for chunk, steps in tail:
-        ctx.set_tuning(traverse_tail=chunk, traverse_tail_steps=steps)
+    best["narrow_separate"] = 0
+    for ns, (chunk, steps) in itertools.product(narrow, ((0, 0),) + tuple(tail)):
+        if ns == 0 and chunk == 0:
+            continue  # timed above
+        ctx.set_tuning(narrow_separate=ns, traverse_tail=chunk, traverse_tail_steps=steps)
         ms = _time(ctx, scene, frame, best["entry_level"], best["expand_levels"], reps)
         if ms < best["ms"]:
-            best = {**best, "traverse_tail": chunk, "traverse_tail_steps": steps, "ms": ms}
+            best = {**best, "narrow_separate": ns, "traverse_tail": chunk, "traverse_tail_steps": steps, "ms": ms}
     ctx.set_tuning(**{**{k: base[k] for k in base},
                       "reduce_leaves_per_thread": best["reduce_leaves_per_thread"],
                       "traverse_blocks": best["traverse_blocks"], "traverse_expand": best["expand_levels"],
                       "traverse_local": best["traverse_local"],
                       "traverse_local_levels": best["traverse_local_levels"],
-                      "traverse_tail": best["traverse_tail"], "traverse_tail_steps": best["traverse_tail_steps"]})
+                      "traverse_tail": best["traverse_tail"], "traverse_tail_steps": best["traverse_tail_steps"],
+                      "narrow_separate": best["narrow_separate"]})
     return best
 
 
