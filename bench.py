@@ -91,6 +91,9 @@ def parse():
     p.add_argument("--narrow-separate", type=int, default=None,
                    help="with --entry-level/--expand-levels: 1 tests leaf pairs in one k_narrow launch")
     p.add_argument("--rebuild-every", type=int, default=50)
+    p.add_argument("--no-overlap", action="store_true",
+                   help="plain form: frames one after another (default: a frame's refit overlaps the previous "
+                        "frame's detect, OIBVH_FRAME_OVERLAP)")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-verify", action="store_true")
@@ -224,8 +227,10 @@ def run_plain(args, ctx):
             scene.detectCollision(shape[0], shape[1], asynchronous=True)
         else:
             # the frame in one call: borrow frame f's vertices, refit both
-            # trees (one launch pair), detect (oibvh_scene_frame)
-            scene.frame(pos[f], shape[0], shape[1], asynchronous=True)
+            # trees (one launch pair), detect (oibvh_scene_frame); the
+            # positions are known ahead, so the refit may overlap the
+            # previous frame's detect
+            scene.frame(pos[f], shape[0], shape[1], asynchronous=True, overlap=not args.no_overlap)
 
     shape = [args.entry_level if args.entry_level is not None else 4,
              args.expand_levels if args.expand_levels is not None else 1]
@@ -372,7 +377,7 @@ def run_plain(args, ctx):
         "data": "synthetic",
         "config": {"workload": workload, "form": "plain", "triangles_per_frame": tri_per_frame,
                    "entry_level": args.entry_level, "expand_levels": args.expand_levels,
-                   "tuning": ctx.get_tuning()},
+                   "frame_overlap": not args.no_overlap, "tuning": ctx.get_tuning()},
         "autotune": tuned,
         "bvtt_levels_last_frame": scene.level_counts().tolist(),
         "mtri_per_s": tri_per_frame * fps / 1e6,

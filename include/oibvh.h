@@ -52,6 +52,8 @@ typedef struct oibvh_batch oibvh_batch;
 #define OIBVH_DETECT_ASYNC 4u         /* enqueue only: no host wait. Results are checked at the next
                                          get/stats call or synchronous detect; a further ASYNC detect
                                          replaces an unchecked one, whose errors go to async_errors */
+#define OIBVH_FRAME_OVERLAP 8u        /* oibvh_scene_frame only: this frame's refit may run while the
+                                         previous frame's detect still runs (see oibvh_scene_frame) */
 
 /* ---------------------------------------------------------------- context */
 
@@ -259,7 +261,18 @@ OIBVH_API oibvh_status oibvh_scene_refit(oibvh_scene* scene);
  * memory positions[i] (as oibvh_tree_set_positions with
  * OIBVH_PTR_DEVICE_BORROW; a NULL entry, or positions == NULL, keeps the
  * current positions), then oibvh_scene_refit, then oibvh_scene_detect with
- * entry_level, expand_levels and flags. */
+ * entry_level, expand_levels and flags.
+ *
+ * With OIBVH_FRAME_OVERLAP (scenes of up to four trees; others ignore it),
+ * frames whose positions are known ahead (a prescribed animation) overlap:
+ * each tree keeps a second node buffer, the refit writes the one the
+ * previous frame's detect is not reading, and the detect runs on a second
+ * stream of the context, so frame f + 1's refit runs while frame f's detect
+ * does. Results and errors are those of the same frames without the flag.
+ * The caller keeps frame f's position memory (and model matrices) unchanged
+ * until frame f's results are read or a later call that waits for it; every
+ * other entry point first waits for the overlapped detects queued before it
+ * (on the GPU, not the host). */
 OIBVH_API oibvh_status oibvh_scene_frame(oibvh_scene* scene, const float* const* positions, uint32_t entry_level,
                                          uint32_t expand_levels, uint32_t flags);
 

@@ -26,6 +26,7 @@ PTR_DEVICE_BORROW = 2
 DETECT_SELF = 1
 DETECT_FP64_FALLBACK = 2
 DETECT_ASYNC = 4
+FRAME_OVERLAP = 8
 
 # every entry point declared in include/oibvh.h: (name, restype, argtypes)
 _vp, _u32, _u64, _i = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int

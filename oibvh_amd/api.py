@@ -301,15 +301,18 @@ class Scene:
         return out[: got.value]
 
     def frame(self, positions=None, entry_level: int = 0, expand_levels: int = 1, *, self_collide: bool = False,
-              fp64_fallback: bool = False, asynchronous: bool = False) -> None:
+              fp64_fallback: bool = False, asynchronous: bool = False, overlap: bool = False) -> None:
         """One frame: tree i borrows device vertex memory positions[i] (an int
         device address, or None to keep its positions), every tree is refit,
-        then detectCollision (oibvh_scene_frame)."""
+        then detectCollision (oibvh_scene_frame). overlap: this frame's refit
+        may run while the previous frame's detect does (OIBVH_FRAME_OVERLAP;
+        keep each frame's position memory unchanged until its results are
+        read)."""
         arr = None
         if positions is not None:
             arr = (C.c_void_p * len(positions))(*[C.c_void_p(p) if p else None for p in positions])
-        _check(self.ctx.h, self._lib.oibvh_scene_frame(self.h, arr, entry_level, expand_levels,
-                                                       _flags(self_collide, fp64_fallback, asynchronous)), "frame")
+        flags = _flags(self_collide, fp64_fallback, asynchronous) | (L.FRAME_OVERLAP if overlap else 0)
+        _check(self.ctx.h, self._lib.oibvh_scene_frame(self.h, arr, entry_level, expand_levels, flags), "frame")
 
     def set_timing(self, every: int = 1) -> None:
         """Time every `every`-th refit and detect with HIP events (0 stops)."""

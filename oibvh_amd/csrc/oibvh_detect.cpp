@@ -4,11 +4,21 @@
 // front size to the host after every level and uses fixed 10M-entry buffers).
 #include "oibvh_internal.h"
 
+// before a buffer an overlapped frame's detect may still use is freed or
+// rewritten
+static oibvh_status idle(oibvh_ctx* c)
+{
+    if (c->side) HIPCHK(c, oi_ctx_quiesce(c));
+    return OIBVH_OK;
+}
+
 template <class T>
 static oibvh_status grow(oibvh_ctx* c, T*& p, uint32_t& cap, uint64_t need, uint64_t& gen)
 {
     if (cap >= need) return OIBVH_OK;
     if (need > 0x7fffffffull) return oi_set_err(c, OIBVH_ERR_OVERFLOW, "detect buffer would exceed 2^31 entries");
+    oibvh_status r = idle(c);
+    if (r != OIBVH_OK) return r;
     dfree(p);
     cap = 0;
     gen++;
@@ -49,8 +59,10 @@ void oi_det_free(oi_detector* d)
 oibvh_status oi_det_set_front(oi_detector* d, const std::vector<uint2>& front)
 {
     oibvh_ctx* c = d->ctx;
+    oibvh_status r = idle(c);
+    if (r != OIBVH_OK) return r;
     d->n_front = (uint32_t)front.size();
-    oibvh_status r = grow(c, d->d_init, d->init_cap, std::max(1u, d->n_front), d->generation);
+    r = grow(c, d->d_init, d->init_cap, std::max(1u, d->n_front), d->generation);
     if (r != OIBVH_OK) return r;
     if (d->n_front)
         HIPCHK(c, hipMemcpyAsync(d->d_init, front.data(), d->n_front * sizeof(uint2), hipMemcpyHostToDevice,
@@ -116,6 +128,8 @@ oibvh_status oi_det_prepare(oi_detector* d)
     if (d->tail_off) d->tail_off--;
     const uint32_t fcap = std::max(c->queue_cap, oi_np2(std::max(1u, d->n_front)));
     if (d->front_cap < fcap) {
+        oibvh_status ri = idle(c);
+        if (ri != OIBVH_OK) return ri;
         dfree(d->front[0]);
         dfree(d->front[1]);
         d->front_cap = 0;
@@ -145,7 +159,7 @@ oi_seed_args oi_det_seed_args(const oi_detector* d, bool reset_sticky)
 oibvh_status oi_det_enqueue(oi_detector* d, bool reset_sticky, bool seeded)
 {
     oibvh_ctx* c = d->ctx;
-    hipStream_t st = c->stream;
+    hipStream_t st = d->stream ? d->stream : c->stream;
     if (!seeded)
         HIPCHK(c, oi_launch_seed(st, d->n_front, d->counts, d->n_levels + 1, d->ctl, reset_sticky ? 1u : 0u));
     const uint32_t grid = oi_grid_for(c);
@@ -212,6 +226,7 @@ oibvh_status oi_det_check(oi_detector* d)
 {
     oibvh_ctx* c = d->ctx;
     if (!d->unchecked) return OIBVH_OK;
+    HIPCHK(c, oi_ctx_join(c));  // an overlapped frame's detect ran on the side stream
     uint32_t retries = 0, earlier = 0;
     for (;;) {
         d->h_counts.resize(d->n_levels + 1);

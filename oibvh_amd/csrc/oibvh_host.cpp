@@ -11,11 +11,52 @@
 static void ctx_free(oibvh_ctx* ctx)
 {
     (void)hipSetDevice(ctx->device);
-    if (ctx->own_stream) {
-        (void)hipStreamSynchronize(ctx->stream);
-        (void)hipStreamDestroy(ctx->stream);
+    (void)oi_ctx_quiesce(ctx);
+    if (ctx->side) {
+        (void)hipEventDestroy(ctx->ev_main);
+        for (hipEvent_t e : ctx->ev_ring) (void)hipEventDestroy(e);
+        (void)hipStreamDestroy(ctx->side);
     }
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+}
+
+hipError_t oi_ctx_join(oibvh_ctx* ctx)
+{
+    if (!ctx->side_pending) return hipSuccess;
+    ctx->side_pending = false;
+    return hipStreamWaitEvent(ctx->stream, ctx->ev_ring[(ctx->side_seq - 1) % OI_SIDE_RING], 0);
+}
+
+hipError_t oi_ctx_quiesce(oibvh_ctx* ctx)
+{
+    hipError_t e = oi_ctx_join(ctx);
+    if (e == hipSuccess && ctx->side) e = hipStreamSynchronize(ctx->side);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e;
+}
+
+// the side stream of overlapped frames and its events (first use)
+static oibvh_status side_init(oibvh_ctx* c)
+{
+    if (c->side) return OIBVH_OK;
+    hipStream_t s = nullptr;
+    HIPCHK(c, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipError_t e = hipEventCreateWithFlags(&c->ev_main, hipEventDisableTiming);
+    for (uint32_t k = 0; k < OI_SIDE_RING && e == hipSuccess; k++)
+        e = hipEventCreateWithFlags(&c->ev_ring[k], hipEventDisableTiming);
+    if (e != hipSuccess) {
+        if (c->ev_main) (void)hipEventDestroy(c->ev_main);
+        c->ev_main = nullptr;
+        for (hipEvent_t& x : c->ev_ring) {
+            if (x) (void)hipEventDestroy(x);
+            x = nullptr;
+        }
+        (void)hipStreamDestroy(s);
+        return oi_set_err(c, OIBVH_ERR_HIP, "side stream: %s", hipGetErrorString(e));
+    }
+    c->side = s;
+    return OIBVH_OK;
 }
 
 // a context outlives its trees, scenes and batches: destroying it while they
@@ -47,7 +88,14 @@ struct oibvh_tree {
     uint32_t* faces = nullptr;  // original order
     uint32_t* sfaces = nullptr; // leaf order
     uint32_t* perm = nullptr;   // leaf -> original
-    float* nodes = nullptr;
+    float* nodes = nullptr;     // nbuf[cur]
+    // node buffers: overlapped frames refit into the one the previous frame's
+    // detect is not reading (nbuf[1] is allocated at the first such frame);
+    // read_seq[k] is 1 + the side-stream detect that last read nbuf[k]
+    // (0: none)
+    float* nbuf[2] = {nullptr, nullptr};
+    uint32_t cur = 0;
+    uint64_t read_seq[2] = {0, 0};
     // build scratch (allocated at the first build)
     uint32_t* keys = nullptr;
     uint32_t* keys_hi = nullptr; // high words of 63-bit keys
@@ -137,7 +185,7 @@ extern "C" const char* oibvh_last_error(const oibvh_ctx* ctx) { return ctx ? ctx
 extern "C" oibvh_status oibvh_ctx_synchronize(oibvh_ctx* ctx)
 {
     if (!ctx) return OIBVH_ERR_INVALID;
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, oi_ctx_quiesce(ctx));
     return OIBVH_OK;
 }
 
@@ -194,7 +242,9 @@ static void tree_free(oibvh_tree* t)
     dfree(t->faces);
     dfree(t->sfaces);
     dfree(t->perm);
-    dfree(t->nodes);
+    dfree(t->nbuf[0]);
+    dfree(t->nbuf[1]);
+    t->nodes = nullptr;
     dfree(t->keys);
     dfree(t->keys_hi);
     dfree(t->keys_alt);
@@ -214,7 +264,8 @@ static oibvh_status tree_alloc_common(oibvh_tree* t)
     HIPCHK(c, dalloc(&t->faces, 3ull * t->F));
     HIPCHK(c, dalloc(&t->sfaces, 3ull * t->F));
     HIPCHK(c, dalloc(&t->perm, t->F));
-    HIPCHK(c, dalloc(&t->nodes, 6ull * t->N));
+    HIPCHK(c, dalloc(&t->nbuf[0], 6ull * t->N));
+    t->nodes = t->nbuf[0];
     HIPCHK(c, dalloc(&t->box, 6ull + 6ull * 256));
     HIPCHK(c, dalloc(&t->xf, 16));
     t->pos = t->pos_own;
@@ -345,7 +396,7 @@ extern "C" void oibvh_tree_destroy(oibvh_tree* t)
     if (!t) return;
     oibvh_ctx* c = t->ctx;
     (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
+    (void)oi_ctx_quiesce(c);
     tree_free(t);
     delete t;
     oi_ctx_release(c);
@@ -387,6 +438,7 @@ extern "C" oibvh_status oibvh_tree_build(oibvh_tree* t)
     if (!t) return OIBVH_ERR_INVALID;
     oibvh_ctx* c = t->ctx;
     HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, oi_ctx_join(c));
     const uint32_t F = t->F;
     const uint32_t tile = oi_sort_tile();
     const uint32_t tiles = (F + tile - 1) / tile;
@@ -444,6 +496,7 @@ extern "C" oibvh_status oibvh_tree_set_transform(oibvh_tree* t, const float* mat
     if (!t) return OIBVH_ERR_INVALID;
     oibvh_ctx* c = t->ctx;
     HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, oi_ctx_join(c));
     if (!mat4) {
         t->has_xf = false;
         return OIBVH_OK;
@@ -460,6 +513,7 @@ extern "C" oibvh_status oibvh_tree_refit(oibvh_tree* t, const float* xyz, uint32
     if (ptr_kind > OIBVH_PTR_DEVICE_BORROW) return oi_set_err(c, OIBVH_ERR_INVALID, "bad ptr_kind");
     if (!t->built) return oi_set_err(c, OIBVH_ERR_NOT_BUILT, "refit before build");
     HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, oi_ctx_join(c));
     if (xyz) {
         oibvh_status st = set_positions(t, xyz, ptr_kind);
         if (st != OIBVH_OK) return st;
@@ -475,6 +529,7 @@ extern "C" oibvh_status oibvh_tree_set_positions(oibvh_tree* t, const float* xyz
     oibvh_ctx* c = t->ctx;
     if (ptr_kind > OIBVH_PTR_DEVICE_BORROW) return oi_set_err(c, OIBVH_ERR_INVALID, "bad ptr_kind");
     HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, oi_ctx_join(c));
     oibvh_status st = set_positions(t, xyz, ptr_kind);
     if (st == OIBVH_OK && ptr_kind == OIBVH_PTR_HOST) HIPCHK(c, hipStreamSynchronize(c->stream));
     return st;
@@ -512,6 +567,7 @@ static oibvh_status readback(oibvh_tree* t, void* dst, const void* src, size_t b
 {
     oibvh_ctx* c = t->ctx;
     HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, oi_ctx_join(c));
     HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return OIBVH_OK;
@@ -580,7 +636,7 @@ extern "C" void oibvh_scene_destroy(oibvh_scene* s)
 {
     if (!s) return;
     (void)hipSetDevice(s->ctx->device);
-    (void)hipStreamSynchronize(s->ctx->stream);
+    (void)oi_ctx_quiesce(s->ctx);
     dfree(s->d_desc);
     dfree(s->d_geom);
     for (hipEvent_t e : s->ev_refit) (void)hipEventDestroy(e);
@@ -721,6 +777,7 @@ extern "C" oibvh_status oibvh_scene_detect(oibvh_scene* s, uint32_t entry, uint3
     if (!s) return OIBVH_ERR_INVALID;
     oibvh_ctx* c = s->ctx;
     HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, oi_ctx_join(c));
     for (oibvh_tree* t : s->trees)
         if (!t->built) return oi_set_err(c, OIBVH_ERR_NOT_BUILT, "detect: tree not built");
     const bool async = (flags & OIBVH_DETECT_ASYNC) != 0;
@@ -804,7 +861,67 @@ static oibvh_status scene_refit(oibvh_scene* s, const oi_seed_args* seed, bool* 
 extern "C" oibvh_status oibvh_scene_refit(oibvh_scene* s)
 {
     if (!s) return OIBVH_ERR_INVALID;
+    oibvh_ctx* c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, oi_ctx_join(c));
     return scene_refit(s, nullptr, nullptr);
+}
+
+// An overlapped frame (OIBVH_FRAME_OVERLAP): every tree is refit on the
+// context's stream into its other node buffer, once the last detect that read
+// that buffer has finished, and the detect runs on the side stream after the
+// refit. The refit of the next frame so runs while this frame's detect does;
+// the detects stay in order on the side stream, and any other call first
+// orders the context's stream after them (oi_ctx_join).
+static oibvh_status overlapped_frame(oibvh_scene* s, uint32_t entry, uint32_t expand, uint32_t flags)
+{
+    oibvh_ctx* c = s->ctx;
+    oi_detector& d = s->det;
+    const bool async = (flags & OIBVH_DETECT_ASYNC) != 0;
+    oibvh_status r = side_init(c);
+    if (r != OIBVH_OK) return r;
+    if (d.unchecked && !async && (r = oi_det_check(&d)) != OIBVH_OK) return r;
+    // wait for the latest side detect that read a buffer about to be
+    // rewritten (the side stream runs its detects in order, so that covers
+    // the earlier ones); once OI_SIDE_RING detects later its ring event has
+    // been recorded again, and the wait is for that later point
+    uint64_t need = 0;
+    for (oibvh_tree* t : s->trees) {
+        const uint32_t k = t->cur ^ 1u;
+        if (!t->nbuf[k]) {
+            HIPCHK(c, oi_ctx_quiesce(c));  // the allocation is the first of this tree's second buffer
+            HIPCHK(c, dalloc(&t->nbuf[k], 6ull * t->N));
+        }
+        need = std::max(need, t->read_seq[k]);
+        t->cur = k;
+        t->nodes = t->nbuf[k];
+    }
+    if (need) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_ring[(need - 1) % OI_SIDE_RING], 0));
+    s->last_entry = entry;
+    s->last_expand = expand;
+    s->last_flags = flags;
+    d.n_pairs = 0;
+    d.stats = oibvh_detect_stats{};
+    if ((r = prepare_scene_detect(s)) != OIBVH_OK) return r;
+    if ((r = scene_refit(s, nullptr, nullptr)) != OIBVH_OK) return r;
+    HIPCHK(c, hipEventRecord(c->ev_main, c->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_main, 0));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (s->timing && (s->seq_detect++ % s->timing) == 0) {
+        if ((r = timing_pair(c, s->ev_detect, s->n_detect, &e0, &e1)) != OIBVH_OK) return r;
+        HIPCHK(c, hipEventRecord(e0, c->side));
+    }
+    d.stream = c->side;
+    r = oi_det_enqueue(&d, !async, false);
+    d.stream = nullptr;
+    if (r != OIBVH_OK) return r;
+    if (e1) HIPCHK(c, hipEventRecord(e1, c->side));
+    const uint64_t q = c->side_seq++;
+    HIPCHK(c, hipEventRecord(c->ev_ring[q % OI_SIDE_RING], c->side));
+    for (oibvh_tree* t : s->trees) t->read_seq[t->cur] = q + 1;
+    c->side_pending = true;
+    if (async) return OIBVH_OK;
+    return oi_det_check(&d);
 }
 
 // The reference's per-frame sequence (main.cpp:284 after the trees' refit):
@@ -824,6 +941,9 @@ extern "C" oibvh_status oibvh_scene_frame(oibvh_scene* s, const float* const* po
     const bool trivial = s->trees.empty() || (s->trees.size() == 1 && !(flags & OIBVH_DETECT_SELF));
     for (oibvh_tree* t : s->trees)
         if (!t->built) return oi_set_err(c, OIBVH_ERR_NOT_BUILT, "frame: tree not built");
+    if ((flags & OIBVH_FRAME_OVERLAP) && !trivial && s->trees.size() <= oi_inline_meshes())
+        return overlapped_frame(s, entry, expand, flags);
+    HIPCHK(c, oi_ctx_join(c));
     if (trivial) {
         oibvh_status r = scene_refit(s, nullptr, nullptr);
         if (r != OIBVH_OK) return r;
@@ -883,6 +1003,7 @@ extern "C" oibvh_status oibvh_scene_get_timing(oibvh_scene* s, oibvh_scene_timin
     if (!s || !out) return OIBVH_ERR_INVALID;
     oibvh_ctx* c = s->ctx;
     HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, oi_ctx_join(c));
     std::memset(out, 0, sizeof *out);
     oibvh_status r = sum_pairs(c, s->ev_refit, s->n_refit, &out->refit_ms);
     if (r == OIBVH_OK) r = sum_pairs(c, s->ev_detect, s->n_detect, &out->detect_ms);
@@ -934,6 +1055,7 @@ extern "C" oibvh_status oibvh_scene_get_pair_vertices(oibvh_scene* s, float* out
     if (!s) return OIBVH_ERR_INVALID;
     oibvh_ctx* c = s->ctx;
     HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, oi_ctx_join(c));
     const uint32_t nt = (uint32_t)s->trees.size();
     if (s->det.have_result && s->det.n_pairs && nt) {
         std::vector<oi_build_desc> g(nt);

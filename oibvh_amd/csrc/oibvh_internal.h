@@ -71,13 +71,31 @@ struct oibvh_ctx {
     uint32_t queue_cap = 1u << 22;
     uint32_t hits_cap = 1u << 20;
     uint32_t cand_cap = 1u << 20;
+    // overlapped frames (OIBVH_FRAME_OVERLAP) run their detects on `side`,
+    // created at default priority on first use
+    hipStream_t side = nullptr;
+    hipEvent_t ev_main = nullptr;  // hand-over from the context's stream (a frame's refit done)
+    // the side stream after its last OI_SIDE_RING detects: detect q ends
+    // with ev_ring[q % OI_SIDE_RING]; side_seq counts them
+    hipEvent_t ev_ring[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint64_t side_seq = 0;
+    bool side_pending = false;     // side work the context's stream has not waited for
     int live = 0;                 // trees, scenes and batches of this context not yet destroyed
     bool destroy_pending = false; // oibvh_ctx_destroy called while objects were live
 };
 
+#define OI_SIDE_RING 4u
+
 // drop one reference held by a tree, scene or batch (frees a context whose
 // destroy was deferred)
 void oi_ctx_release(oibvh_ctx* ctx);
+// order the context's stream after everything queued on its side stream
+// (every entry point but an overlapped frame calls it before queueing work
+// or reading results)
+hipError_t oi_ctx_join(oibvh_ctx* ctx);
+// wait until both streams are idle (before memory they may use is freed or
+// rewritten from the host)
+hipError_t oi_ctx_quiesce(oibvh_ctx* ctx);
 
 oibvh_status oi_set_err(oibvh_ctx* ctx, oibvh_status st, const char* fmt, ...);
 
@@ -151,6 +169,7 @@ struct oi_detector {
     uint32_t tail_from = 0;     // this query: first step done by the tail walk
     uint32_t tail_off = 0;      // queries left without a tail walk after one overflowed
     bool fused = false;  // narrow phase inside the BVTT steps (FP32 queries)
+    hipStream_t stream = nullptr;  // stream of the next enqueue (null: the context's)
     bool unchecked = false;
     bool have_result = false;
     uint32_t n_pairs = 0;
