@@ -30,15 +30,24 @@ static oibvh_status grow(oibvh_ctx* c, T*& p, uint32_t& cap, uint64_t need, uint
 oibvh_status oi_det_init(oi_detector* d, oibvh_ctx* c)
 {
     d->ctx = c;
-    hipError_t e = dalloc(&d->ctl, 1);
-    if (e == hipSuccess) e = hipMemsetAsync(d->ctl, 0, sizeof(oi_detect_ctl), c->stream);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&d->h_ctl, sizeof(oi_detect_ctl), hipHostMallocDefault);
+    hipError_t e = dalloc(&d->ctl_base, 2);
+    if (e == hipSuccess) e = hipMemsetAsync(d->ctl_base, 0, 2 * sizeof(oi_detect_ctl), c->stream);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&d->h_ctl, 2 * sizeof(oi_detect_ctl), hipHostMallocDefault);
     if (e != hipSuccess) {
         oi_det_free(d);
         return oi_set_err(c, OIBVH_ERR_HIP, "detector alloc: %s", hipGetErrorString(e));
     }
-    std::memset(d->h_ctl, 0, sizeof(oi_detect_ctl));
+    std::memset(d->h_ctl, 0, 2 * sizeof(oi_detect_ctl));
+    d->ctl = d->ctl_base;
     return OIBVH_OK;
+}
+
+void oi_det_flip_set(oi_detector* d)
+{
+    d->prev_unchecked = d->unchecked;
+    d->set ^= 1u;
+    d->ctl = d->ctl_base + d->set;
+    d->counts = d->counts_base ? d->counts_base + 256u * d->set : nullptr;
 }
 
 void oi_det_free(oi_detector* d)
@@ -46,12 +55,14 @@ void oi_det_free(oi_detector* d)
     dfree(d->d_init);
     dfree(d->front[0]);
     dfree(d->front[1]);
-    dfree(d->counts);
+    dfree(d->counts_base);
+    d->counts = nullptr;
     dfree(d->cands);
     dfree(d->hits);
     dfree(d->d_sorted);
     dfree(d->d_stage);
-    dfree(d->ctl);
+    dfree(d->ctl_base);
+    d->ctl = nullptr;
     if (d->h_ctl) (void)hipHostFree(d->h_ctl);
     d->h_ctl = nullptr;
 }
@@ -139,7 +150,8 @@ oibvh_status oi_det_prepare(oi_detector* d)
         d->front_cap = fcap;
     }
     oibvh_status r;
-    if ((r = grow(c, d->counts, d->counts_cap, 256, d->generation)) != OIBVH_OK) return r;
+    if ((r = grow(c, d->counts_base, d->counts_cap, 512, d->generation)) != OIBVH_OK) return r;
+    d->counts = d->counts_base + 256u * d->set;
     if ((r = grow(c, d->cands, d->cand_cap, c->cand_cap, d->generation)) != OIBVH_OK) return r;
     if ((r = grow(c, d->hits, d->hits_cap, c->hits_cap, d->generation)) != OIBVH_OK) return r;
     return OIBVH_OK;
@@ -231,11 +243,19 @@ oibvh_status oi_det_check(oi_detector* d)
     for (;;) {
         d->h_counts.resize(d->n_levels + 1);
         HIPCHK(c, hipMemcpyAsync(d->h_ctl, d->ctl, sizeof(oi_detect_ctl), hipMemcpyDeviceToHost, c->stream));
+        if (d->prev_unchecked)
+            HIPCHK(c, hipMemcpyAsync(d->h_ctl + 1, d->ctl_base + (d->set ^ 1u), sizeof(oi_detect_ctl),
+                                     hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipMemcpyAsync(d->h_counts.data(), d->counts, (d->n_levels + 1) * sizeof(uint32_t),
                                  hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        const oi_detect_ctl& k = *d->h_ctl;
+        const oi_detect_ctl& k = d->h_ctl[0];
         earlier |= k.sticky;  // errors of unchecked asynchronous detects before this one
+        if (d->prev_unchecked) {
+            // the previous overlapped frame's detect and the unchecked ones before it
+            earlier |= d->h_ctl[1].sticky | d->h_ctl[1].error;
+            d->prev_unchecked = false;
+        }
         if (d->h_counts[d->n_levels] != 0) {
             // the last step reaches the leaves on every side: it writes no front
             d->unchecked = false;

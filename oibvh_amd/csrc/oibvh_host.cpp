@@ -882,10 +882,12 @@ static oibvh_status overlapped_frame(oibvh_scene* s, uint32_t entry, uint32_t ex
     if (r != OIBVH_OK) return r;
     if (d.unchecked && !async && (r = oi_det_check(&d)) != OIBVH_OK) return r;
     // wait for the latest side detect that read a buffer about to be
-    // rewritten (the side stream runs its detects in order, so that covers
-    // the earlier ones); once OI_SIDE_RING detects later its ring event has
-    // been recorded again, and the wait is for that later point
-    uint64_t need = 0;
+    // rewritten or used the counter set about to be reset (the side stream
+    // runs its detects in order, so that covers the earlier ones); once
+    // OI_SIDE_RING detects later its ring event has been recorded again, and
+    // the wait is for that later point
+    oi_det_flip_set(&d);
+    uint64_t need = d.set_seq[d.set];
     for (oibvh_tree* t : s->trees) {
         const uint32_t k = t->cur ^ 1u;
         if (!t->nbuf[k]) {
@@ -903,7 +905,10 @@ static oibvh_status overlapped_frame(oibvh_scene* s, uint32_t entry, uint32_t ex
     d.n_pairs = 0;
     d.stats = oibvh_detect_stats{};
     if ((r = prepare_scene_detect(s)) != OIBVH_OK) return r;
-    if ((r = scene_refit(s, nullptr, nullptr)) != OIBVH_OK) return r;
+    // the refit's apex launch also resets this frame's counter set
+    const oi_seed_args seed = oi_det_seed_args(&d, !async);
+    bool seeded = false;
+    if ((r = scene_refit(s, &seed, &seeded)) != OIBVH_OK) return r;
     HIPCHK(c, hipEventRecord(c->ev_main, c->stream));
     HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_main, 0));
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -912,13 +917,14 @@ static oibvh_status overlapped_frame(oibvh_scene* s, uint32_t entry, uint32_t ex
         HIPCHK(c, hipEventRecord(e0, c->side));
     }
     d.stream = c->side;
-    r = oi_det_enqueue(&d, !async, false);
+    r = oi_det_enqueue(&d, !async, seeded);
     d.stream = nullptr;
     if (r != OIBVH_OK) return r;
     if (e1) HIPCHK(c, hipEventRecord(e1, c->side));
     const uint64_t q = c->side_seq++;
     HIPCHK(c, hipEventRecord(c->ev_ring[q % OI_SIDE_RING], c->side));
     for (oibvh_tree* t : s->trees) t->read_seq[t->cur] = q + 1;
+    d.set_seq[d.set] = q + 1;
     c->side_pending = true;
     if (async) return OIBVH_OK;
     return oi_det_check(&d);

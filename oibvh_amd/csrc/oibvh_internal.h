@@ -147,15 +147,24 @@ struct oi_detector {
     uint32_t n_front = 0;
     uint2* front[2] = {nullptr, nullptr};  // ping-pong BVTT fronts
     uint32_t front_cap = 0;
-    uint32_t* counts = nullptr;  // front length per step (+1)
+    // two sets of level counts and counters: overlapped frames alternate
+    // between them, so a frame's refit can reset one while the previous
+    // frame's detect still uses the other (set k: counts_base + 256 k,
+    // ctl_base + k)
+    uint32_t* counts_base = nullptr;
     uint32_t counts_cap = 0;
+    oi_detect_ctl* ctl_base = nullptr;
+    uint32_t set = 0;
+    uint64_t set_seq[2] = {0, 0};  // 1 + the side-stream detect that last used set k (0: none)
+    bool prev_unchecked = false;  // the other set holds an unchecked detect's result
+    uint32_t* counts = nullptr;  // front length per step (+1), of the current set
     uint32_t n_levels = 0;
     uint2* cands = nullptr;
     uint32_t cand_cap = 0;
     uint4* hits = nullptr;
     uint32_t hits_cap = 0;
-    oi_detect_ctl* ctl = nullptr;
-    oi_detect_ctl* h_ctl = nullptr;  // pinned
+    oi_detect_ctl* ctl = nullptr;    // counters of the current set
+    oi_detect_ctl* h_ctl = nullptr;  // pinned, two (the current set's, the other's)
     std::vector<uint32_t> h_counts;
     // the query (kept for re-runs)
     oi_tab_args tab{};
@@ -196,6 +205,8 @@ oibvh_status oi_det_prepare(oi_detector* d);
 oibvh_status oi_det_enqueue(oi_detector* d, bool reset_sticky, bool seeded = false);
 // the counter reset of the prepared query, for a launch queued before it
 oi_seed_args oi_det_seed_args(const oi_detector* d, bool reset_sticky);
+// switch to the other counter set (an overlapped frame, before prepare)
+void oi_det_flip_set(oi_detector* d);
 // wait; on overflow grow the buffers and run the query again
 oibvh_status oi_det_check(oi_detector* d);
 // sorted hits (4 words each)

@@ -454,6 +454,37 @@ def test_scene_frame_overlap(api, ctx, orc, self_collide):
             b.free()
 
 
+@pytest.mark.parametrize("overlap", [False, True])
+def test_async_frames_report_earlier_errors(api, ctx, orc, overlap):
+    """Frames queued without a host wait whose hit lists overflow: reading
+    the last one re-runs it with a larger buffer (the oracle's pairs) and
+    reports the overflow of the unchecked frames before it in async_errors,
+    with overlapped frames (two counter sets) as without."""
+    say(f"async frame errors overlap={overlap}")
+    meshes = M.interpenetrating_icospheres(subdiv=4, seed=27, offset=1.2)
+    want = orc.detect([orc.Tree(x, f) for x, f in meshes])
+    assert len(want) > 16
+    ctx2 = api.Context(0)
+    ctx2.set_tuning(hits_capacity=16)
+    try:
+        trees = [gpu_tree(api, ctx2, x, f) for x, f in meshes]
+        sc = api.Scene(ctx2)
+        for t in trees:
+            sc.addOibvhTree(t)
+        for _ in range(3):
+            sc.frame(None, 2, 2, asynchronous=True, overlap=overlap)
+        assert np.array_equal(sc.getIntTriPairs(), want)
+        st = sc.stats()
+        assert st["retries"] >= 1
+        assert st["async_errors"] & 8  # hit-list overflow of the earlier frames
+        sc.frame(None, 2, 2, overlap=overlap)  # a checked frame starts afresh
+        assert np.array_equal(sc.getIntTriPairs(), want)
+        assert sc.stats()["async_errors"] == 0 and sc.stats()["retries"] == 0
+        sc.close()
+    finally:
+        ctx2.close()
+
+
 def test_scene_edge_cases(api, ctx, orc):
     """An empty scene, a one-tree scene without and with self mode, and the
     frame call on them: no pairs where the reference has none."""
