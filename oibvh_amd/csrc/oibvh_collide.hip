@@ -498,6 +498,22 @@ __device__ __forceinline__ uint32_t select_bit(u64 m, uint32_t k)
     return pos;
 }
 
+// bits x * K + y of the K x K child pairs (K = 2^E) with y > x, and y == x
+template <int E>
+struct PairMasks {
+    static constexpr int K = 1 << E;
+    static constexpr u64 make(bool diagonal)
+    {
+        u64 m = 0;
+        for (int x = 0; x < K; x++)
+            for (int y = 0; y < K; y++)
+                if (diagonal ? (y == x) : (y > x)) m |= 1ull << (x * K + y);
+        return m;
+    }
+    static constexpr u64 upper = make(false);
+    static constexpr u64 diag = make(true);
+};
+
 // One BVTT entry (a pair of implicit indices): with FIRST the pair itself is
 // tested first (an initial-front entry); then both sides descend up to E
 // levels and every pair of the two descendant runs is tested. Bit x * K + y
@@ -536,29 +552,25 @@ __device__ __forceinline__ void entry_expand(const Tab& T, const oi_mesh_desc* s
     float bxa[K][6], bxb[K][6];
     load_run<E>(ra.nodes, sa.start - oi_virt_above(ca, ra.L, ra.vl), sa.count, bxa);
     load_run<E>(rb.nodes, sb.start - oi_virt_above(cb, rb.L, rb.vl), sb.count, bxb);
+    // every slot pair is tested with no branch (slots past a run's end repeat
+    // its last node), then masked to the pairs that exist: x < sa.count,
+    // y < sb.count, and for a node paired with itself the unordered child
+    // pairs, y > x tested and y == x kept unless they are leaves
+    u64 hit = 0;
 #pragma unroll
-    for (int x = 0; x < K; x++) {
+    for (int x = 0; x < K; x++)
 #pragma unroll
-        for (int y = 0; y < K; y++) {
-            if ((uint32_t)x >= sa.count || (uint32_t)y >= sb.count) continue;
-            bool keep;
-            if (selfnode) {
-                // unordered child pairs; a child with itself needs no test and
-                // is no leaf pair
-                if (y < x) continue;
-                if (y == x) {
-                    keep = !leafout;
-                } else {
-                    keep = overlap6(bxa[x], bxb[y]);
-                    n_tests++;
-                }
-            } else {
-                keep = overlap6(bxa[x], bxb[y]);
-                n_tests++;
-            }
-            if (keep) mask |= 1ull << (x * K + y);
-        }
-    }
+        for (int y = 0; y < K; y++)
+            if (overlap6(bxa[x], bxb[y])) hit |= 1ull << (x * K + y);
+    u64 rows = 0;
+#pragma unroll
+    for (int x = 0; x < K; x++)
+        if ((uint32_t)x < sa.count) rows |= 1ull << (x * K);
+    const u64 exist = rows * ((1ull << sb.count) - 1ull);  // sb.count <= K: no carry between rows
+    const u64 tested = selfnode ? (exist & PairMasks<E>::upper) : exist;
+    n_tests += (uint32_t)__popcll(tested);
+    mask = hit & tested;
+    if (selfnode && !leafout) mask |= exist & PairMasks<E>::diag;
     // inside one mesh (self mode) leaf pairs sharing a vertex are not
     // collisions: drop them before they become candidates
     if (leafout && ma == mb && mask) {
