@@ -100,12 +100,14 @@ def tune(ctx: api.Context, scene: api.Scene, frame, *, entries=(4, 6, 8), expand
 
 
 def tune_batch(ctx: api.Context, batch: api.Batch, before_frame, *, entries=(0, 2, 4), expands=(1, 2, 3),
-               blocks=(0, 4096), leaves=(0, 4, 8), local=((64, 2), (256, 3)), tail=((64, 0), (128, 2)),
-               flags=None, reps: int = 3) -> dict:
+               blocks=(0, 4096), leaves=(0, 4, 8), local=((64, 2), (256, 3)), tail=(), flags=None,
+               reps: int = 3) -> dict:
     """Pick the reduce subtree size (leaves per thread), the BVTT entry level,
     the levels per step and the workgroups per launch for a batch's captured
     frame (refit + detect). before_frame(i) stages frame i's inputs (model
-    matrices or vertex pointers) in stream order."""
+    matrices or vertex pointers) in stream order. `tail` (none by default)
+    lists tail-walk settings to try: a batch's late fronts are large, and a
+    tail walk that outgrows LDS on a later frame costs that frame a re-run."""
     flags = flags or {}
     base = ctx.get_tuning()
     timer = api.Timer(ctx)
