@@ -383,7 +383,7 @@ def run_plain(args, ctx):
         "mtri_per_s": tri_per_frame * fps / 1e6,
         "pair_diffs": diffs,
         "pairs_last_frame": H_pairs,
-        "async_errors": st["async_errors"],
+        "async_errors": st["async_errors"], "retries_last_frame": st["retries"],
         "bvtt_tests_last_frame": st["bvtt_tests"],
         "candidates_last_frame": st["candidates"],
         "kernels": kernels,
@@ -588,6 +588,9 @@ def run_batched(args, ctx):
         "pair_diffs_vs_fp64_oracle_sampled": diffs64,
         "oracle_sample_scenes": sample,
         "fp64_rechecks_last_frame": st_b["fp64_rechecks"],
+        # an overflow in a timed frame (which runs without a host check)
+        # would have left that frame incomplete: both must be 0
+        "async_errors": st_b["async_errors"], "retries_last_frame": st_b["retries"],
         "candidates_last_frame": st_b["candidates"],
         "bvtt_tests_last_frame": st_b["bvtt_tests"],
         "bvtt_levels_last_frame": lc,
@@ -641,7 +644,8 @@ def run_precision(args, ctx):
            "triangles": tri, "frames": k, "fps": 1000.0 / ms, "ms_per_frame": ms,
            "pairs_fp64_fallback": int(len(p64)), "pairs_fp32": int(len(p32)),
            "candidates": st["candidates"], "fp64_axis_rechecks": st["fp64_rechecks"],
-           "fp64_recheck_rate_per_axis_test": st["fp64_rechecks"] / max(1, 17 * st["candidates"])}
+           "fp64_recheck_rate_per_axis_test": st["fp64_rechecks"] / max(1, 17 * st["candidates"]),
+           "async_errors": st["async_errors"], "retries_last_frame": st["retries"]}
     progress(f"precision: {1000.0 / ms:.1f} frames/s; oracle check")
     if not args.no_verify:
         import oracle
@@ -702,8 +706,10 @@ def run_self(args, ctx):
 
     progress("self: built; autotuning")
     ctx.set_tuning(front_capacity=1 << 29)
+    # steps only: a sheet's fronts outgrow any workgroup's LDS, and a local
+    # walk that overflows on some frames costs them a re-run
     tuned = autotune.tune_batch(ctx, b, stage, entries=(4, 6), expands=(2, 3), blocks=(0,), leaves=(0,),
-                                local=((256, 3), (64, 2)), tail=((256, 2),), flags={"self_collide": True}, reps=4)
+                                local=(), tail=(), flags={"self_collide": True}, reps=4)
     progress(f"self: tuned {tuned}")
     entry, expand = tuned["entry_level"], tuned["expand_levels"]
     stage(0)
@@ -750,6 +756,7 @@ def run_self(args, ctx):
            "triangles": tri, "frames": k, "fps": 1000.0 / ms, "ms_per_frame": ms,
            "mtri_per_s": tri / (ms * 1e-3) / 1e6, "pairs_last_frame": int(len(pairs)),
            "adjacent_skipped": st["adjacent"], "candidates": st["candidates"], "bvtt_tests": st["bvtt_tests"],
+           "async_errors": st["async_errors"], "retries_last_frame": st["retries"],
            "bvtt_levels_last_frame": b.level_counts().tolist(), "autotune": tuned, "host_gen_s": t_gen}
     if plain_pairs is not None:
         res["pair_diffs_batched_vs_plain_full_size"] = _pair_diff(pairs, plain_pairs)

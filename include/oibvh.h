@@ -231,17 +231,18 @@ typedef struct {
     uint32_t traverse_local;           /* 0: BVTT steps only (one launch per step, each spread over the
                                           whole GPU); n >= 1: the first steps as one launch in which
                                           workgroups of n initial pairs walk breadth first with their
-                                          fronts in LDS, falling back to steps only for the next 64
-                                          detects when a workgroup's front outgrows LDS (1..2048) */
+                                          fronts in LDS (1..2048). When a workgroup's front outgrows
+                                          LDS the detect is re-run with steps and the walk stays off
+                                          until the tuning is set again; an OIBVH_DETECT_ASYNC detect
+                                          whose walk overflowed is incomplete (async_errors) */
     uint32_t traverse_local_levels;    /* steps the local walk replaces (0 = all of them) */
     uint32_t narrow_separate;          /* FP32 queries: 0 tests leaf pairs inside the BVTT launches that
                                           find them; 1 collects them and tests them in one k_narrow
                                           launch over the whole GPU (the FP64 fallback always does) */
     uint32_t traverse_tail;            /* 0: none; n >= 1: the last steps as one launch in which
                                           workgroups take n pairs at a time of the front the steps
-                                          left and walk them to the leaves with their fronts in LDS,
-                                          off for the next 64 detects when a workgroup's front outgrows
-                                          LDS (1..2048) */
+                                          left and walk them to the leaves with their fronts in LDS
+                                          (1..2048); an overflow is handled as for traverse_local */
     uint32_t traverse_tail_steps;      /* steps the tail walk replaces, counted from the last (0 = all
                                           after the head local walk, or all) */
 } oibvh_tuning;

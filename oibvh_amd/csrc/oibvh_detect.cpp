@@ -108,7 +108,10 @@ oibvh_status oi_det_prepare(oi_detector* d)
     const uint32_t mode = c->trav_local;
     d->local_chunk = 0;
     d->local_levels = 0;
-    if (mode && d->local_off == 0 && d->n_front) {
+    // a walk that overflowed stays off until the tuning is set again: an
+    // asynchronous detect cannot re-run, so one that overflowed again
+    // would leave its frame incomplete
+    if (mode && d->local_off != c->tuning_gen + 1 && d->n_front) {
         d->local_chunk = mode;
         k = 0;
         done = 0;
@@ -119,13 +122,12 @@ oibvh_status oi_det_prepare(oi_detector* d)
         d->n_levels = k;
         d->local_levels = c->trav_local_levels ? std::min(c->trav_local_levels, k) : k;
     }
-    if (d->local_off) d->local_off--;
     // the last steps as a tail walk when asked for (unless it overflowed
     // recently): it starts after the head walk and after every step whose
     // depth the schedule set, since it descends `expand` levels per step
     d->tail_chunk = 0;
     d->tail_from = 0;
-    if (c->trav_tail && d->tail_off == 0 && d->n_front) {
+    if (c->trav_tail && d->tail_off != c->tuning_gen + 1 && d->n_front) {
         const uint32_t S = c->trav_tail_steps;
         uint32_t T = (S && S < d->n_levels) ? d->n_levels - S : 0u;
         T = std::max(T, d->local_chunk ? d->local_levels : 0u);
@@ -136,7 +138,6 @@ oibvh_status oi_det_prepare(oi_detector* d)
             d->tail_from = T;
         }
     }
-    if (d->tail_off) d->tail_off--;
     const uint32_t fcap = std::max(c->queue_cap, oi_np2(std::max(1u, d->n_front)));
     if (d->front_cap < fcap) {
         oibvh_status ri = idle(c);
@@ -272,11 +273,11 @@ oibvh_status oi_det_check(oi_detector* d)
         oibvh_status r = OIBVH_OK;
         if (k.error & OI_ERR_LOCAL_OVERFLOW) {
             // a workgroup's front outgrew LDS: steps for the next queries
-            d->local_off = 64;
+            d->local_off = c->tuning_gen + 1;
             rerun = true;
         }
         if (k.error & OI_ERR_TAIL_OVERFLOW) {
-            d->tail_off = 64;
+            d->tail_off = c->tuning_gen + 1;
             rerun = true;
         }
         uint32_t maxfront = 0;

@@ -214,6 +214,7 @@ extern "C" oibvh_status oibvh_ctx_set_tuning(oibvh_ctx* ctx, const oibvh_tuning*
     ctx->trav_tail = t->traverse_tail;
     ctx->trav_tail_steps = t->traverse_tail_steps;
     ctx->narrow_separate = t->narrow_separate;
+    ctx->tuning_gen++;
     return OIBVH_OK;
 }
 

@@ -68,6 +68,7 @@ struct oibvh_ctx {
     uint32_t trav_tail = 0;          // 0: no tail walk; n: the last steps as a local walk, n pairs per workgroup
     uint32_t trav_tail_steps = 0;    // steps the tail walk replaces (0: all after the head walk)
     uint32_t narrow_separate = 0;    // FP32 queries test leaf pairs in one k_narrow launch
+    uint64_t tuning_gen = 0;         // counts oibvh_ctx_set_tuning calls
     uint32_t queue_cap = 1u << 22;
     uint32_t hits_cap = 1u << 20;
     uint32_t cand_cap = 1u << 20;
@@ -173,10 +174,10 @@ struct oi_detector {
     uint8_t step_e[256] = {};   // levels descended by each step of the current query
     uint32_t local_chunk = 0;   // this query: initial pairs per workgroup of the local walk (0: steps)
     uint32_t local_levels = 0;  // this query: steps done by the local walk
-    uint32_t local_off = 0;     // queries left that use steps after a local walk overflowed
+    uint64_t local_off = 0;     // 1 + the tuning generation in which the head walk overflowed (0: none)
     uint32_t tail_chunk = 0;    // this query: front pairs per workgroup of the tail walk (0: none)
     uint32_t tail_from = 0;     // this query: first step done by the tail walk
-    uint32_t tail_off = 0;      // queries left without a tail walk after one overflowed
+    uint64_t tail_off = 0;      // the same for the tail walk
     bool fused = false;  // narrow phase inside the BVTT steps (FP32 queries)
     hipStream_t stream = nullptr;  // stream of the next enqueue (null: the context's)
     bool unchecked = false;
