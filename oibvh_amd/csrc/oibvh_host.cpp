@@ -887,6 +887,7 @@ static oibvh_status overlapped_frame(oibvh_scene* s, uint32_t entry, uint32_t ex
     // runs its detects in order, so that covers the earlier ones); once
     // OI_SIDE_RING detects later its ring event has been recorded again, and
     // the wait is for that later point
+    const bool prev_unchecked = d.prev_unchecked;
     oi_det_flip_set(&d);
     uint64_t need = d.set_seq[d.set];
     for (oibvh_tree* t : s->trees) {
@@ -905,7 +906,18 @@ static oibvh_status overlapped_frame(oibvh_scene* s, uint32_t entry, uint32_t ex
     s->last_flags = flags;
     d.n_pairs = 0;
     d.stats = oibvh_detect_stats{};
-    if ((r = prepare_scene_detect(s)) != OIBVH_OK) return r;
+    if ((r = prepare_scene_detect(s)) != OIBVH_OK) {
+        // nothing was queued: the trees keep the buffers they were refit into
+        for (oibvh_tree* t : s->trees) {
+            t->cur ^= 1u;
+            t->nodes = t->nbuf[t->cur];
+        }
+        d.set ^= 1u;
+        d.ctl = d.ctl_base + d.set;
+        d.counts = d.counts_base ? d.counts_base + 256u * d.set : nullptr;
+        d.prev_unchecked = prev_unchecked;
+        return r;
+    }
     // the refit's apex launch also resets this frame's counter set
     const oi_seed_args seed = oi_det_seed_args(&d, !async);
     bool seeded = false;
