@@ -388,15 +388,15 @@ def test_scene_frame_and_timing(api, ctx, orc):
             b.free()
 
 
-@pytest.mark.parametrize("self_collide", [False, True])
-def test_scene_frame_overlap(api, ctx, orc, self_collide):
+@pytest.mark.parametrize("self_collide,fp64", [(False, False), (True, False), (False, True)])
+def test_scene_frame_overlap(api, ctx, orc, self_collide, fp64):
     """Overlapped frames (OIBVH_FRAME_OVERLAP): frames queued back to back,
     each refit into the node buffer the previous frame's detect is not
     reading. The last queued frame's pairs and every tree's nodes equal the
     oracle's; so do the pairs of frames read one by one, a frame without the
     flag after them, the event timing, and calls that write the trees
     (refit from host memory, rebuild) between overlapped frames."""
-    say(f"scene frame overlap self={self_collide}")
+    say(f"scene frame overlap self={self_collide} fp64={fp64}")
     meshes, waves = M.deforming_uv_pair(160, 81)
     trees = [gpu_tree(api, ctx, x, f) for x, f in meshes]
     sc = api.Scene(ctx)
@@ -409,7 +409,7 @@ def test_scene_frame_overlap(api, ctx, orc, self_collide):
     def want(k):
         fr = frames[k]
         return orc.detect([orc.Tree(fr[m], meshes[m][1], order=trees[m].perm()) for m in range(2)],
-                          self_collide=self_collide)
+                          self_collide=self_collide, fp64=fp64)
 
     def check_nodes(k):
         for m in range(2):
@@ -418,32 +418,37 @@ def test_scene_frame_overlap(api, ctx, orc, self_collide):
 
     sc.set_timing(1)
     for k in range(len(ks)):  # all queued, then the last one read
-        sc.frame([b.at(0) for b in bufs[k]], 3, 2, self_collide=self_collide, asynchronous=True, overlap=True)
+        sc.frame([b.at(0) for b in bufs[k]], 3, 2, self_collide=self_collide, fp64_fallback=fp64, asynchronous=True,
+                 overlap=True)
     assert np.array_equal(sc.getIntTriPairs(), want(len(ks) - 1))
     check_nodes(len(ks) - 1)
     tm = sc.timing()
     assert tm["refits"] == len(ks) and tm["detects"] == len(ks) and tm["detect_ms"] > 0
     sc.set_timing(0)
     for k in (2, 0, 5):  # read one by one (synchronous frames)
-        sc.frame([b.at(0) for b in bufs[k]], 3, 2, self_collide=self_collide, overlap=True)
+        sc.frame([b.at(0) for b in bufs[k]], 3, 2, self_collide=self_collide, fp64_fallback=fp64, overlap=True)
         assert np.array_equal(sc.getIntTriPairs(), want(k)), k
         check_nodes(k)
     for k in (1, 4):  # queued, then a frame without the flag
-        sc.frame([b.at(0) for b in bufs[k]], 3, 2, self_collide=self_collide, asynchronous=True, overlap=True)
-    sc.frame([b.at(0) for b in bufs[6]], 3, 2, self_collide=self_collide)
+        sc.frame([b.at(0) for b in bufs[k]], 3, 2, self_collide=self_collide, fp64_fallback=fp64, asynchronous=True,
+                 overlap=True)
+    sc.frame([b.at(0) for b in bufs[6]], 3, 2, self_collide=self_collide, fp64_fallback=fp64)
     assert np.array_equal(sc.getIntTriPairs(), want(6))
     check_nodes(6)
     # a host refit and a rebuild between overlapped frames
-    sc.frame([b.at(0) for b in bufs[3]], 3, 2, self_collide=self_collide, asynchronous=True, overlap=True)
+    sc.frame([b.at(0) for b in bufs[3]], 3, 2, self_collide=self_collide, fp64_fallback=fp64, asynchronous=True,
+                 overlap=True)
     for t, x in zip(trees, frames[2]):
         t.refit(x)
     check_nodes(2)
-    sc.frame([b.at(0) for b in bufs[5]], 3, 2, self_collide=self_collide, asynchronous=True, overlap=True)
+    sc.frame([b.at(0) for b in bufs[5]], 3, 2, self_collide=self_collide, fp64_fallback=fp64, asynchronous=True,
+                 overlap=True)
     for t in trees:
         t.build()
     check_nodes(5)
     for k in (4, 1):
-        sc.frame([b.at(0) for b in bufs[k]], 3, 2, self_collide=self_collide, asynchronous=True, overlap=True)
+        sc.frame([b.at(0) for b in bufs[k]], 3, 2, self_collide=self_collide, fp64_fallback=fp64, asynchronous=True,
+                 overlap=True)
     assert np.array_equal(sc.getIntTriPairs(), want(1))
     check_nodes(1)
     for t, (x, f) in zip(trees, meshes):
@@ -695,8 +700,18 @@ def test_detect_many_meshes(api, ctx, orc):
             meshes.append(((x + c.astype(np.float32)).astype(np.float32), f))
     want = orc.detect([orc.Tree(x, f) for x, f in meshes])
     assert len(want) > 0
-    p, _, _ = gpu_pairs(api, ctx, meshes, 1, 2)
+    p, _, trees = gpu_pairs(api, ctx, meshes, 1, 2)
     assert np.array_equal(p, want)
+    # frames with OIBVH_FRAME_OVERLAP on a scene of more than four trees run
+    # one after another, with the same pairs
+    sc = api.Scene(ctx)
+    for t in trees:
+        sc.addOibvhTree(t)
+    for _ in range(3):
+        sc.frame(None, 1, 2, asynchronous=True, overlap=True)
+    assert np.array_equal(sc.getIntTriPairs(), want)
+    sc.frame(None, 1, 2, fp64_fallback=True, overlap=True)
+    assert np.array_equal(sc.getIntTriPairs(), orc.detect([orc.Tree(x, f) for x, f in meshes], fp64=True))
 
 
 def test_detect_separated_and_identical(api, ctx, orc):
