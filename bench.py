@@ -784,7 +784,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this pipeline runs on a single "
-              "MI355X; multi-GPU sharding is described in DESIGN.md and not built)", file=sys.stderr)
+              "MI355X; sharding scenes or trees over several GPUs is not built)", file=sys.stderr)
         sys.exit(2)
     from oibvh_amd import api
 
