@@ -918,9 +918,12 @@ __device__ __forceinline__ void narrow_one(const Tab& T, const uint32_t* s_ibase
     const MeshRef rb = tab_ref(T, s_desc, s_ibase, c.y, mb);
     const uint32_t sa = c.x - ra.ibase - ((1u << ra.L) - 1u);
     const uint32_t sb = c.y - rb.ibase - ((1u << rb.L) - 1u);
+    // the original ids are read with the faces, and the vertices before the
+    // ids arrive: two dependent rounds of loads per candidate
     uint32_t fa[3], fb[3];
     oi_ld3(ra.sfaces, sa, fa);
     oi_ld3(rb.sfaces, sb, fb);
+    uint32_t oa = oi_g(ra.perm)[sa], ob = oi_g(rb.perm)[sb];
     if (ma == mb) {
         bool adj = false;
 #pragma unroll
@@ -932,26 +935,25 @@ __device__ __forceinline__ void narrow_one(const Tab& T, const uint32_t* s_ibase
             return;
         }
     }
-    uint32_t oa = oi_g(ra.perm)[sa], ob = oi_g(rb.perm)[sb];
-    const float* pa = ra.pos;
-    const float* pb = rb.pos;
+    v3f P[3], Q[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        P[k] = vertex(ra.pos, ra.xf, fa[k]);
+        Q[k] = vertex(rb.pos, rb.xf, fb[k]);
+    }
     if (ma == mb && oa > ob) {
-        // inside one mesh P is the face with the smaller original id
-        uint32_t t = oa;
+        // inside one mesh P is the face with the smaller original id (both
+        // faces share the mesh's positions and model matrix, so the loaded
+        // vertices are swapped)
+        const uint32_t t = oa;
         oa = ob;
         ob = t;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            uint32_t u = fa[k];
-            fa[k] = fb[k];
-            fb[k] = u;
+            const v3f u = P[k];
+            P[k] = Q[k];
+            Q[k] = u;
         }
-    }
-    v3f P[3], Q[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        P[k] = vertex(pa, ra.xf, fa[k]);
-        Q[k] = vertex(pb, rb.xf, fb[k]);
     }
     const bool h = FP64 ? sat_filtered(P, Q, n_fp64) : sat_f32(P, Q);
     if (h) {
