@@ -51,12 +51,7 @@ __device__ __forceinline__ void st_relaxed(uint32_t* p, uint32_t v)
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wsum, uint32_t& total)
 {
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (uint32_t o = 1; o < 64; o <<= 1) {
-        uint32_t t = __shfl_up(x, o, 64);
-        if (lane >= o) x += t;
-    }
+    const uint32_t x = oi_wave_scan_incl(v);
     if (lane == 63) s_wsum[wave] = x;
     __syncthreads();
     uint32_t w0 = s_wsum[0], w1 = s_wsum[1], w2 = s_wsum[2], w3 = s_wsum[3];

@@ -35,26 +35,10 @@
 
 typedef unsigned long long u64;
 
-// inclusive prefix sum over the 64 lanes of a wave (every lane active) with
-// DPP moves instead of LDS permutes: a Kogge-Stone scan inside each row of
-// 16 lanes (row_shr 1, 2, 4, 8; a lane shifted in from outside its row
-// reads 0), then row 1 and row 3 add the last lane of the row before them
-// (row_bcast:15), then rows 2 and 3 add lane 31 (row_bcast:31)
-__device__ __forceinline__ uint32_t wave_scan_incl(uint32_t x)
-{
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, true);
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true);
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, true);
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, true);
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);
-    return x;
-}
-
 __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t* s_wsum, uint32_t& total)
 {
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t x = wave_scan_incl(v);
+    const uint32_t x = oi_wave_scan_incl(v);
     if (lane == 63) s_wsum[wave] = x;
     __syncthreads();
     uint32_t w0 = s_wsum[0], w1 = s_wsum[1], w2 = s_wsum[2], w3 = s_wsum[3];

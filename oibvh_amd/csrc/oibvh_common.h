@@ -107,6 +107,21 @@ __device__ __forceinline__ void oi_ld3(const uint32_t* base, uint32_t i, uint32_
     o[1] = q[1];
     o[2] = q[2];
 }
+// inclusive prefix sum over the 64 lanes of a wave (every lane active) with
+// DPP moves instead of LDS permutes: a Kogge-Stone scan inside each row of
+// 16 lanes (row_shr 1, 2, 4, 8; a lane shifted in from outside its row
+// reads 0), then rows 1 and 3 add the last lane of the row before them
+// (row_bcast:15), then rows 2 and 3 add lane 31 (row_bcast:31)
+__device__ __forceinline__ uint32_t oi_wave_scan_incl(uint32_t x)
+{
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, true);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, true);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, true);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);
+    return x;
+}
 #endif
 
 // ---------------------------------------------------------------------------
