@@ -308,9 +308,9 @@ class Scene:
         may run while the previous frame's detect does (OIBVH_FRAME_OVERLAP;
         keep each frame's position memory unchanged until its results are
         read)."""
-        arr = None
-        if positions is not None:
-            arr = (C.c_void_p * len(positions))(*[C.c_void_p(p) if p else None for p in positions])
+        arr = positions if isinstance(positions, C.Array) else None
+        if positions is not None and arr is None:
+            arr = position_array(positions)
         flags = _flags(self_collide, fp64_fallback, asynchronous) | (L.FRAME_OVERLAP if overlap else 0)
         _check(self.ctx.h, self._lib.oibvh_scene_frame(self.h, arr, entry_level, expand_levels, flags), "frame")
 
@@ -349,6 +349,13 @@ class Scene:
         s = L.DetectStats()
         _check(self.ctx.h, self._lib.oibvh_scene_get_stats(self.h, C.byref(s)), "get_stats")
         return {f: getattr(s, f) for f, _ in L.DetectStats._fields_}
+
+
+def position_array(positions):
+    """The ctypes array of device addresses Scene.frame takes (int or None
+    per tree); building it once per frame buffer saves the conversion on
+    every call."""
+    return (C.c_void_p * len(positions))(*[C.c_void_p(p) if p else None for p in positions])
 
 
 def _flags(self_collide: bool, fp64_fallback: bool, asynchronous: bool) -> int:
