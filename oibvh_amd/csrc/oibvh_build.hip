@@ -26,6 +26,7 @@
 //
 // Compiled with -ffp-contract=off. min/max are compare-selects with glm's tie
 // rules: min(a,b) = b < a ? b : a, max(a,b) = a < b ? b : a.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "oibvh_common.h"
@@ -1120,7 +1121,7 @@ hipError_t oi_launch_reduce_inline(hipStream_t st, int M, const oi_build_desc* m
 // stage 2 for n inline meshes, one workgroup each; with seed (may be null)
 // one more workgroup resets the following detect's counters
 hipError_t oi_launch_reduce_apex_inline(hipStream_t st, int M, const oi_build_desc* meshes, uint32_t n,
-                                        const oi_seed_args* seed)
+                                        const oi_seed_args* seed, hipEvent_t done)
 {
     if (n == 0 || n > OI_INLINE_MESHES) return hipErrorInvalidValue;
     if (seed && seed->n_counts > OI_BLOCK) return hipErrorInvalidValue;
@@ -1129,7 +1130,12 @@ hipError_t oi_launch_reduce_apex_inline(hipStream_t st, int M, const oi_build_de
     ReduceArgs a{make_btab_inline(meshes, n, boff), nullptr, 0};
     const uint32_t lgm = lgm_of(M);
     const oi_seed_args sd = seed ? *seed : oi_seed_args{};
-    hipLaunchKernelGGL(k_reduce_apex, dim3(n + (seed ? 1u : 0u)), dim3(OI_BLOCK), 0, st, a, lgm, sd);
+    // with done the launch records it on completion (no separate event call)
+    if (done)
+        hipExtLaunchKernelGGL(k_reduce_apex, dim3(n + (seed ? 1u : 0u)), dim3(OI_BLOCK), 0, st, nullptr, done, 0, a,
+                              lgm, sd);
+    else
+        hipLaunchKernelGGL(k_reduce_apex, dim3(n + (seed ? 1u : 0u)), dim3(OI_BLOCK), 0, st, a, lgm, sd);
     return hipGetLastError();
 }
 

@@ -23,6 +23,7 @@
 //
 // BVTT entries are two 32-bit implicit indices in the scene's concatenated
 // implicit space (mesh m spans [ibase_m, ibase_m + 2^(L_m+1) - 1)).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -1018,6 +1019,17 @@ __global__ void k_seed(uint32_t n0, uint32_t* counts, uint32_t n_counts, oi_dete
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+// a launch whose completion also records `done` (when not null) as part of
+// the dispatch, which saves a separate hipEventRecord call on the host
+template <class K, class... Args>
+static void launch_done(K kernel, dim3 g, dim3 b, hipStream_t st, hipEvent_t done, Args... args)
+{
+    if (done)
+        hipExtLaunchKernelGGL(kernel, g, b, 0, st, nullptr, done, 0, args...);
+    else
+        hipLaunchKernelGGL(kernel, g, b, 0, st, args...);
+}
+
 static MeshTab make_tab(const oi_mesh_desc* d_meshes, const oi_mesh_desc* h_meshes, uint32_t n)
 {
     MeshTab t{};
@@ -1032,7 +1044,7 @@ static hipError_t launch_step(hipStream_t st, uint32_t grid, const Tab& tab, int
                               const uint32_t* count_in, uint32_t n_in, uint32_t cap_in, uint2* front_out,
                               uint32_t* count_out, uint32_t cap_out, uint2* cands, uint32_t* cand_count,
                               uint32_t cand_cap, uint4* hits, uint32_t hits_cap, oi_detect_ctl* ctl, uint32_t expand,
-                              uint32_t implicit_end, int fused)
+                              uint32_t implicit_end, int fused, hipEvent_t done)
 {
     StepArgs<Tab> a{};
     a.tab = tab;
@@ -1056,13 +1068,13 @@ static hipError_t launch_step(hipStream_t st, uint32_t grid, const Tab& tab, int
     const dim3 g(grid), b(TB);
 #define OI_STEP(EE)                                                                    \
     if (first && fused)                                                                \
-        hipLaunchKernelGGL((k_bvtt_step<EE, true, true, Tab>), g, b, 0, st, a);        \
+        launch_done((k_bvtt_step<EE, true, true, Tab>), g, b, st, done, a);            \
     else if (first)                                                                    \
-        hipLaunchKernelGGL((k_bvtt_step<EE, true, false, Tab>), g, b, 0, st, a);       \
+        launch_done((k_bvtt_step<EE, true, false, Tab>), g, b, st, done, a);           \
     else if (fused)                                                                    \
-        hipLaunchKernelGGL((k_bvtt_step<EE, false, true, Tab>), g, b, 0, st, a);       \
+        launch_done((k_bvtt_step<EE, false, true, Tab>), g, b, st, done, a);           \
     else                                                                               \
-        hipLaunchKernelGGL((k_bvtt_step<EE, false, false, Tab>), g, b, 0, st, a);
+        launch_done((k_bvtt_step<EE, false, false, Tab>), g, b, st, done, a);
     switch (expand) {
     case 1: OI_STEP(1) break;
     case 2: OI_STEP(2) break;
@@ -1082,7 +1094,7 @@ static hipError_t launch_local(hipStream_t st, const Tab& tab, const oi_local_ar
     const uint64_t chunks = w.count_in ? ((uint64_t)w.cap_in + w.chunk - 1) / w.chunk
                                        : ((uint64_t)w.n_in + w.chunk - 1) / w.chunk;
     const uint64_t grid = (w.count_in && w.grid && w.grid < chunks) ? w.grid : chunks;
-    if (grid == 0) return hipSuccess;
+    if (grid == 0) return w.done ? hipEventRecord(w.done, st) : hipSuccess;
     if (grid > 0x7fffffffull) return hipErrorInvalidValue;
     LocalArgs<Tab> a{};
     a.tab = tab;
@@ -1110,9 +1122,9 @@ static hipError_t launch_local(hipStream_t st, const Tab& tab, const oi_local_ar
     const dim3 g((uint32_t)grid), b(TB);
 #define OI_LOCAL(EE)                                                           \
     if (fused)                                                                 \
-        hipLaunchKernelGGL((k_bvtt_local<EE, true, Tab>), g, b, 0, st, a);     \
+        launch_done((k_bvtt_local<EE, true, Tab>), g, b, st, w.done, a);       \
     else                                                                       \
-        hipLaunchKernelGGL((k_bvtt_local<EE, false, Tab>), g, b, 0, st, a);
+        launch_done((k_bvtt_local<EE, false, Tab>), g, b, st, w.done, a);
     switch (expand) {
     case 1: OI_LOCAL(1) break;
     case 2: OI_LOCAL(2) break;
@@ -1126,7 +1138,7 @@ static hipError_t launch_local(hipStream_t st, const Tab& tab, const oi_local_ar
 template <class Tab>
 static hipError_t launch_narrow(hipStream_t st, uint32_t grid, const Tab& tab, const uint2* cands,
                                 const uint32_t* cand_count, uint32_t cand_cap, uint4* hits, uint32_t* n_hits,
-                                uint32_t hits_cap, oi_detect_ctl* ctl, int fp64)
+                                uint32_t hits_cap, oi_detect_ctl* ctl, int fp64, hipEvent_t done)
 {
     NarrowArgs<Tab> a{};
     a.tab = tab;
@@ -1140,9 +1152,9 @@ static hipError_t launch_narrow(hipStream_t st, uint32_t grid, const Tab& tab, c
     a.fp64 = &ctl->fp64;
     a.adj = &ctl->adj;
     if (fp64)
-        hipLaunchKernelGGL((k_narrow<true, Tab>), dim3(grid), dim3(TB), 0, st, a);
+        launch_done((k_narrow<true, Tab>), dim3(grid), dim3(TB), st, done, a);
     else
-        hipLaunchKernelGGL((k_narrow<false, Tab>), dim3(grid), dim3(TB), 0, st, a);
+        launch_done((k_narrow<false, Tab>), dim3(grid), dim3(TB), st, done, a);
     return hipGetLastError();
 }
 
@@ -1163,15 +1175,15 @@ hipError_t oi_launch_bvtt_step(hipStream_t st, uint32_t grid, const oi_tab_args*
                                const uint2* front_in, const uint32_t* count_in, uint32_t n_in, uint32_t cap_in,
                                uint2* front_out, uint32_t* count_out, uint32_t cap_out, uint2* cands,
                                uint32_t* cand_count, uint32_t cand_cap, uint4* hits, uint32_t hits_cap,
-                               oi_detect_ctl* ctl, uint32_t expand, uint32_t implicit_end, int fused)
+                               oi_detect_ctl* ctl, uint32_t expand, uint32_t implicit_end, int fused, hipEvent_t done)
 {
     if (t->chunk_mesh)
         return launch_step(st, grid, ChunkTab{t->d_meshes, t->chunk_mesh, t->shift}, first, front_in, count_in, n_in,
                            cap_in, front_out, count_out, cap_out, cands, cand_count, cand_cap, hits, hits_cap, ctl,
-                           expand, implicit_end, fused);
+                           expand, implicit_end, fused, done);
     return launch_step(st, grid, make_tab(t->d_meshes, t->h_meshes, t->n_meshes), first, front_in, count_in, n_in,
                        cap_in, front_out, count_out, cap_out, cands, cand_count, cand_cap, hits, hits_cap, ctl,
-                       expand, implicit_end, fused);
+                       expand, implicit_end, fused, done);
 }
 
 // the first `levels` steps in one launch, one workgroup per `chunk` initial
@@ -1220,13 +1232,13 @@ hipError_t oi_launch_pair_vertices(hipStream_t st, const oi_build_desc* g, const
 
 hipError_t oi_launch_narrow(hipStream_t st, uint32_t grid, const oi_tab_args* t, const uint2* cands,
                             const uint32_t* cand_count, uint32_t cand_cap, uint4* hits, uint32_t* n_hits,
-                            uint32_t hits_cap, oi_detect_ctl* ctl, int fp64)
+                            uint32_t hits_cap, oi_detect_ctl* ctl, int fp64, hipEvent_t done)
 {
     if (t->chunk_mesh)
         return launch_narrow(st, grid, ChunkTab{t->d_meshes, t->chunk_mesh, t->shift}, cands, cand_count, cand_cap,
-                             hits, n_hits, hits_cap, ctl, fp64);
+                             hits, n_hits, hits_cap, ctl, fp64, done);
     return launch_narrow(st, grid, make_tab(t->d_meshes, t->h_meshes, t->n_meshes), cands, cand_count, cand_cap,
-                         hits, n_hits, hits_cap, ctl, fp64);
+                         hits, n_hits, hits_cap, ctl, fp64, done);
 }
 
 }  // extern "C"

@@ -224,6 +224,7 @@ struct oi_local_args {
     uint2* front_out;
     uint32_t* count_out;
     uint32_t cap_out;
+    hipEvent_t done;           // recorded when the launch completes (null: none)
 };
 
 // a scene's concatenated implicit index space must stay below this bound

@@ -169,7 +169,7 @@ oi_seed_args oi_det_seed_args(const oi_detector* d, bool reset_sticky)
     return s;
 }
 
-oibvh_status oi_det_enqueue(oi_detector* d, bool reset_sticky, bool seeded)
+oibvh_status oi_det_enqueue(oi_detector* d, bool reset_sticky, bool seeded, hipEvent_t done)
 {
     oibvh_ctx* c = d->ctx;
     hipStream_t st = d->stream ? d->stream : c->stream;
@@ -181,6 +181,12 @@ oibvh_status oi_det_enqueue(oi_detector* d, bool reset_sticky, bool seeded)
     const bool fp64 = (d->flags & OIBVH_DETECT_FP64_FALLBACK) != 0;
     d->fused = !fp64 && !c->narrow_separate;
     const uint32_t J = d->local_chunk ? d->local_levels : 0u;
+    const uint32_t T = d->tail_chunk ? d->tail_from : d->n_levels;
+    // `done` rides on the last launch: k_narrow, else the tail walk, else
+    // the last step, else the head walk
+    const bool narrow = !d->fused, tail = T < d->n_levels, steps = T > J;
+    hipEvent_t done_head = (!narrow && !tail && !steps) ? done : nullptr;
+    hipEvent_t done_tail = (!narrow && tail) ? done : nullptr;
     if (J) {
         // steps 0 .. J-1 in one launch; its front goes where step J - 1's would
         oi_local_args w{};
@@ -193,19 +199,21 @@ oibvh_status oi_det_enqueue(oi_detector* d, bool reset_sticky, bool seeded)
         w.front_out = d->front[(J - 1) & 1];
         w.count_out = d->counts + J;
         w.cap_out = d->front_cap;
+        w.done = done_head;
         HIPCHK(c, oi_launch_bvtt_local(st, &d->tab, &w, d->cands, &d->ctl->cands, d->cand_cap, d->hits, d->hits_cap,
                                        d->ctl, d->expand, d->implicit_end, d->fused ? 1 : 0));
+        done_head = nullptr;
     }
-    const uint32_t T = d->tail_chunk ? d->tail_from : d->n_levels;
     for (uint32_t k = J; k < T; k++) {
         // step 0 reads the initial front in place; then the fronts alternate
         const uint2* in = k ? d->front[(k - 1) & 1] : d->d_init;
         HIPCHK(c, oi_launch_bvtt_step(st, grid, &d->tab, k == 0, in, k ? d->counts + k : nullptr, d->n_front,
                                       k ? d->front_cap : d->n_front, d->front[k & 1], d->counts + k + 1,
                                       d->front_cap, d->cands, &d->ctl->cands, d->cand_cap, d->hits, d->hits_cap,
-                                      d->ctl, d->step_e[k], d->implicit_end, d->fused ? 1 : 0));
+                                      d->ctl, d->step_e[k], d->implicit_end, d->fused ? 1 : 0,
+                                      (!narrow && !tail && k + 1 == T) ? done : nullptr));
     }
-    if (T < d->n_levels) {
+    if (tail) {
         // steps T .. n_levels - 1 in one launch over the front step T - 1
         // left: every pair ends at the leaves, so the last level writes no
         // front (its count, checked below, stays 0); front_out is the buffer
@@ -223,12 +231,15 @@ oibvh_status oi_det_enqueue(oi_detector* d, bool reset_sticky, bool seeded)
         w.front_out = d->front[T & 1];
         w.count_out = d->counts + d->n_levels;
         w.cap_out = d->front_cap;
+        w.done = done_tail;
         HIPCHK(c, oi_launch_bvtt_local(st, &d->tab, &w, d->cands, &d->ctl->cands, d->cand_cap, d->hits, d->hits_cap,
                                        d->ctl, d->expand, d->implicit_end, d->fused ? 1 : 0));
     }
-    if (!d->fused)
+    if (narrow)
         HIPCHK(c, oi_launch_narrow(st, grid, &d->tab, d->cands, &d->ctl->cands, d->cand_cap, d->hits,
-                                   &d->ctl->n_hits, d->hits_cap, d->ctl, fp64 ? 1 : 0));
+                                   &d->ctl->n_hits, d->hits_cap, d->ctl, fp64 ? 1 : 0, done));
+    else if (done_head)  // a query with no BVTT launch
+        HIPCHK(c, hipEventRecord(done, st));
     d->unchecked = true;
     d->have_result = true;
     d->sorted_ok = false;

@@ -42,9 +42,10 @@ static oibvh_status side_init(oibvh_ctx* c)
     if (c->side) return OIBVH_OK;
     hipStream_t s = nullptr;
     HIPCHK(c, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    hipError_t e = hipEventCreateWithFlags(&c->ev_main, hipEventDisableTiming);
-    for (uint32_t k = 0; k < OI_SIDE_RING && e == hipSuccess; k++)
-        e = hipEventCreateWithFlags(&c->ev_ring[k], hipEventDisableTiming);
+    // recorded by kernel launches (hipExtLaunchKernel's stop event), so
+    // created as ordinary timing events
+    hipError_t e = hipEventCreate(&c->ev_main);
+    for (uint32_t k = 0; k < OI_SIDE_RING && e == hipSuccess; k++) e = hipEventCreate(&c->ev_ring[k]);
     if (e != hipSuccess) {
         if (c->ev_main) (void)hipEventDestroy(c->ev_main);
         c->ev_main = nullptr;
@@ -819,7 +820,7 @@ extern "C" oibvh_status oibvh_scene_detect(oibvh_scene* s, uint32_t entry, uint3
 // GPU once instead of once per tree. Same results as oibvh_tree_refit on each.
 // the scene refit; with seed, the last apex launch (if there is one) also
 // resets the counters of the detect that follows, and *seeded says so
-static oibvh_status scene_refit(oibvh_scene* s, const oi_seed_args* seed, bool* seeded)
+static oibvh_status scene_refit(oibvh_scene* s, const oi_seed_args* seed, bool* seeded, hipEvent_t done = nullptr)
 {
     oibvh_ctx* c = s->ctx;
     if (seeded) *seeded = false;
@@ -853,7 +854,9 @@ static oibvh_status scene_refit(oibvh_scene* s, const oi_seed_args* seed, bool* 
         if (e1) HIPCHK(c, hipEventRecord(e1, c->stream));
         const bool last = i0 + per >= s->trees.size();
         const oi_seed_args* sd = (last && seed && na) ? seed : nullptr;
-        if (na) HIPCHK(c, oi_launch_reduce_apex_inline(c->stream, M, apex, na, sd));
+        // `done` (may be null) is recorded as the refit's last launch completes
+        if (na) HIPCHK(c, oi_launch_reduce_apex_inline(c->stream, M, apex, na, sd, last ? done : nullptr));
+        if (last && done && !na) HIPCHK(c, hipEventRecord(done, c->stream));
         if (sd && seeded) *seeded = true;
     }
     return OIBVH_OK;
@@ -921,21 +924,21 @@ static oibvh_status overlapped_frame(oibvh_scene* s, uint32_t entry, uint32_t ex
     // the refit's apex launch also resets this frame's counter set
     const oi_seed_args seed = oi_det_seed_args(&d, !async);
     bool seeded = false;
-    if ((r = scene_refit(s, &seed, &seeded)) != OIBVH_OK) return r;
-    HIPCHK(c, hipEventRecord(c->ev_main, c->stream));
+    if ((r = scene_refit(s, &seed, &seeded, c->ev_main)) != OIBVH_OK) return r;
     HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_main, 0));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (s->timing && (s->seq_detect++ % s->timing) == 0) {
         if ((r = timing_pair(c, s->ev_detect, s->n_detect, &e0, &e1)) != OIBVH_OK) return r;
         HIPCHK(c, hipEventRecord(e0, c->side));
     }
+    // the detect's last launch records this detect's ring event
+    const uint64_t q = c->side_seq;
     d.stream = c->side;
-    r = oi_det_enqueue(&d, !async, seeded);
+    r = oi_det_enqueue(&d, !async, seeded, c->ev_ring[q % OI_SIDE_RING]);
     d.stream = nullptr;
     if (r != OIBVH_OK) return r;
+    c->side_seq++;
     if (e1) HIPCHK(c, hipEventRecord(e1, c->side));
-    const uint64_t q = c->side_seq++;
-    HIPCHK(c, hipEventRecord(c->ev_ring[q % OI_SIDE_RING], c->side));
     for (oibvh_tree* t : s->trees) t->read_seq[t->cur] = q + 1;
     d.set_seq[d.set] = q + 1;
     c->side_pending = true;

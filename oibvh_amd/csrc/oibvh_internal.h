@@ -38,14 +38,15 @@ hipError_t oi_launch_set_xf(hipStream_t, float*, const float*);
 uint32_t oi_inline_meshes();
 hipError_t oi_launch_reduce_inline(hipStream_t, int, const oi_build_desc*, uint32_t, const uint32_t*, const uint32_t*,
                                    uint32_t);
-hipError_t oi_launch_reduce_apex_inline(hipStream_t, int, const oi_build_desc*, uint32_t, const oi_seed_args*);
+hipError_t oi_launch_reduce_apex_inline(hipStream_t, int, const oi_build_desc*, uint32_t, const oi_seed_args*,
+                                        hipEvent_t);
 hipError_t oi_launch_set_mesh_pos(hipStream_t, oi_build_desc*, oi_mesh_desc*, uint32_t, const float*);
 hipError_t oi_launch_seed(hipStream_t, uint32_t, uint32_t*, uint32_t, oi_detect_ctl*, uint32_t);
 hipError_t oi_launch_bvtt_step(hipStream_t, uint32_t, const oi_tab_args*, int, const uint2*, const uint32_t*,
                                uint32_t, uint32_t, uint2*, uint32_t*, uint32_t, uint2*, uint32_t*, uint32_t, uint4*,
-                               uint32_t, oi_detect_ctl*, uint32_t, uint32_t, int);
+                               uint32_t, oi_detect_ctl*, uint32_t, uint32_t, int, hipEvent_t);
 hipError_t oi_launch_narrow(hipStream_t, uint32_t, const oi_tab_args*, const uint2*, const uint32_t*, uint32_t,
-                            uint4*, uint32_t*, uint32_t, oi_detect_ctl*, int);
+                            uint4*, uint32_t*, uint32_t, oi_detect_ctl*, int, hipEvent_t);
 hipError_t oi_launch_bvtt_local(hipStream_t, const oi_tab_args*, const oi_local_args*, uint2*, uint32_t*, uint32_t,
                                 uint4*, uint32_t, oi_detect_ctl*, uint32_t, uint32_t, int);
 hipError_t oi_launch_pair_vertices(hipStream_t, const oi_build_desc*, const uint4*, uint32_t, float*);
@@ -203,7 +204,8 @@ oibvh_status oi_det_prepare(oi_detector* d);
 // launch seed, steps and narrow phase (no allocation, no synchronisation);
 // seeded: the counters were reset by a launch already queued (see
 // oi_det_seed_args), so no seed launch
-oibvh_status oi_det_enqueue(oi_detector* d, bool reset_sticky, bool seeded = false);
+// done (may be null) is recorded as the query's last launch completes
+oibvh_status oi_det_enqueue(oi_detector* d, bool reset_sticky, bool seeded = false, hipEvent_t done = nullptr);
 // the counter reset of the prepared query, for a launch queued before it
 oi_seed_args oi_det_seed_args(const oi_detector* d, bool reset_sticky);
 // switch to the other counter set (an overlapped frame, before prepare)
