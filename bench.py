@@ -263,10 +263,11 @@ def run_plain(args, ctx):
         scene.getIntTriPairCount()
     timer = api.Timer(ctx)
     ctx.synchronize()
-    # the library's own HIP events bracket the stage-1 launch of every 8th
-    # refit and every 8th detect inside the timed region (an event pair
-    # drains the GPU pipeline for a few microseconds: a sample keeps the
-    # timed frames close to untimed ones)
+    # the library's own HIP events time the stage-1 launch of every 8th
+    # refit (recorded by the launch itself as it starts and completes) and
+    # bracket every 8th detect inside the timed region (an event pair around
+    # a detect drains the GPU pipeline for a few microseconds: a sample keeps
+    # the timed frames close to untimed ones)
     scene.timing()
     scene.set_timing(8)
     timer.start()

@@ -280,10 +280,11 @@ OIBVH_API oibvh_status oibvh_scene_frame(oibvh_scene* scene, const float* const*
 /* Timing of a scene's work with HIP events on the context's stream (no
  * reference counterpart; the reference times with cudaEvents around its own
  * kernels, oibvhTree.cu:300-355, scene.cu:262-433). With every = n > 0, every
- * n-th oibvh_scene_refit records an event pair around its (first) stage-1
- * reduce launch (k_reduce_w or k_reduce), and every n-th detect one around its
- * launches; every = 0 stops. An event pair costs the GPU a few microseconds
- * of pipeline drain, so a sample keeps the timed work close to untimed work.
+ * n-th oibvh_scene_refit has its (first) stage-1 reduce launch (k_reduce_w or
+ * k_reduce) record an event pair as it starts and completes (the kernel's own
+ * duration), and every n-th detect records one around its launches; every = 0
+ * stops. A detect's event pair costs the GPU a few microseconds of pipeline
+ * drain, so a sample keeps the timed work close to untimed work.
  * oibvh_scene_get_timing waits for the recorded events, sums their durations
  * and starts the counts afresh. */
 typedef struct {

@@ -1021,6 +1021,16 @@ static BuildTab make_btab_inline(const oi_build_desc* meshes, uint32_t n, const 
     return t;
 }
 
+// a launch that records t0 as it starts and t1 as it completes (when set)
+template <class K, class... Args>
+static void launch_timed(K kernel, dim3 g, dim3 b, hipStream_t st, hipEvent_t t0, hipEvent_t t1, Args... args)
+{
+    if (t1)
+        hipExtLaunchKernelGGL(kernel, g, b, 0, st, t0, t1, 0, args...);
+    else
+        hipLaunchKernelGGL(kernel, g, b, 0, st, args...);
+}
+
 extern "C" {
 
 // Every launcher takes the mesh table as (d, work, one): a device descriptor
@@ -1102,18 +1112,22 @@ uint32_t oi_inline_meshes() { return OI_INLINE_MESHES; }
 
 // stage 1 over up to OI_INLINE_MESHES meshes passed by value; boff[n + 1]
 // gives each mesh's first workgroup
+// with t0 and t1 (both or neither) the launch itself records them as it
+// starts and completes: their elapsed time is the kernel's own duration
 hipError_t oi_launch_reduce_inline(hipStream_t st, int M, const oi_build_desc* meshes, uint32_t n,
-                                   const uint32_t* boff, const uint32_t* vals, uint32_t write_perm)
+                                   const uint32_t* boff, const uint32_t* vals, uint32_t write_perm, hipEvent_t t0,
+                                   hipEvent_t t1)
 {
     if (n == 0 || n > OI_INLINE_MESHES) return hipErrorInvalidValue;
+    if (!t0 != !t1) return hipErrorInvalidValue;
     ReduceArgs a{make_btab_inline(meshes, n, boff), vals, write_perm};
     const dim3 g(boff[n]), b(OI_BLOCK);
     switch (M) {
-    case 1: hipLaunchKernelGGL(k_reduce<1>, g, b, 0, st, a); break;
-    case 2: hipLaunchKernelGGL(k_reduce<2>, g, b, 0, st, a); break;
-    case 4: hipLaunchKernelGGL(k_reduce<4>, g, b, 0, st, a); break;
-    case 16: hipLaunchKernelGGL(k_reduce_w, g, b, 0, st, a); break;
-    default: hipLaunchKernelGGL(k_reduce<8>, g, b, 0, st, a); break;
+    case 1: launch_timed(k_reduce<1>, g, b, st, t0, t1, a); break;
+    case 2: launch_timed(k_reduce<2>, g, b, st, t0, t1, a); break;
+    case 4: launch_timed(k_reduce<4>, g, b, st, t0, t1, a); break;
+    case 16: launch_timed(k_reduce_w, g, b, st, t0, t1, a); break;
+    default: launch_timed(k_reduce<8>, g, b, st, t0, t1, a); break;
     }
     return hipGetLastError();
 }

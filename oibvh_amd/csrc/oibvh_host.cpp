@@ -848,10 +848,9 @@ static oibvh_status scene_refit(oibvh_scene* s, const oi_seed_args* seed, bool* 
         if (s->timing && i0 == 0 && (s->seq_refit++ % s->timing) == 0) {
             oibvh_status r = timing_pair(c, s->ev_refit, s->n_refit, &e0, &e1);
             if (r != OIBVH_OK) return r;
-            HIPCHK(c, hipEventRecord(e0, c->stream));
         }
-        HIPCHK(c, oi_launch_reduce_inline(c->stream, M, d, n, boff, nullptr, 0));
-        if (e1) HIPCHK(c, hipEventRecord(e1, c->stream));
+        // the timing pair is recorded by the stage-1 launch itself
+        HIPCHK(c, oi_launch_reduce_inline(c->stream, M, d, n, boff, nullptr, 0, e0, e1));
         const bool last = i0 + per >= s->trees.size();
         const oi_seed_args* sd = (last && seed && na) ? seed : nullptr;
         // `done` (may be null) is recorded as the refit's last launch completes

@@ -37,7 +37,7 @@ hipError_t oi_launch_reduce_apex(hipStream_t, int, const oi_build_desc*, const u
 hipError_t oi_launch_set_xf(hipStream_t, float*, const float*);
 uint32_t oi_inline_meshes();
 hipError_t oi_launch_reduce_inline(hipStream_t, int, const oi_build_desc*, uint32_t, const uint32_t*, const uint32_t*,
-                                   uint32_t);
+                                   uint32_t, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 hipError_t oi_launch_reduce_apex_inline(hipStream_t, int, const oi_build_desc*, uint32_t, const oi_seed_args*,
                                         hipEvent_t);
 hipError_t oi_launch_set_mesh_pos(hipStream_t, oi_build_desc*, oi_mesh_desc*, uint32_t, const float*);
