@@ -62,11 +62,13 @@ def apex_nodes(F: int, kernel: str) -> int:
     return sum((1 << l) - (vl >> (L - l)) for l in range(0, L - h))
 
 
-def reduce_kernel(lpt: int, faces: int) -> str:
+def reduce_kernel(lpt: int, faces: int, blocks: int = 0, items: int = 0) -> str:
     """Name of the stage-1 reduce kernel the library runs for a launch over
-    `faces` faces with tuning reduce_leaves_per_thread = lpt (0 = by size)."""
+    `faces` faces with tuning reduce_leaves_per_thread = lpt (0 = by size)
+    and refit_blocks = blocks, the launch holding `items` subtrees of 1024
+    leaves."""
     if lpt == 16 or (lpt == 0 and faces >= (1 << 18)):
-        return "k_reduce_w"
+        return "k_reduce_wl" if blocks and ((blocks + 7) & ~7) < items else "k_reduce_w"
     return f"k_reduce<{lpt}>" if lpt else "k_reduce<M>"
 
 
@@ -90,6 +92,11 @@ def parse():
                    help="steps the tail walk replaces, counted from the last (0: all after the head walk)")
     p.add_argument("--narrow-separate", type=int, default=None,
                    help="with --entry-level/--expand-levels: 1 tests leaf pairs in one k_narrow launch")
+    p.add_argument("--reduce-leaves", type=int, default=None,
+                   help="with --entry-level/--expand-levels: leaves per thread of the build/refit reduce "
+                        "(16: k_reduce_w, 1/2/4/8: k_reduce<M>, 0: by size)")
+    p.add_argument("--refit-blocks", type=int, default=None,
+                   help="at most this many workgroups in the scene refit's k_reduce_w (0: one per 1024 leaves)")
     p.add_argument("--rebuild-every", type=int, default=50)
     p.add_argument("--no-overlap", action="store_true",
                    help="plain form: frames one after another (default: a frame's refit overlaps the previous "
@@ -170,6 +177,10 @@ def run_plain(args, ctx):
         ctx.set_tuning(traverse_tail=args.traverse_tail, traverse_tail_steps=args.traverse_tail_steps)
     if args.narrow_separate is not None:
         ctx.set_tuning(narrow_separate=args.narrow_separate)
+    if args.reduce_leaves is not None:
+        ctx.set_tuning(reduce_leaves_per_thread=args.reduce_leaves)
+    if args.refit_blocks is not None:
+        ctx.set_tuning(refit_blocks=args.refit_blocks)
     if args.workload == "deforming-uv-pair":
         meshes, waves = M.deforming_uv_pair()
         n_frames = 200
@@ -261,6 +272,16 @@ def run_plain(args, ctx):
         shape[:] = [args.entry_level, args.expand_levels]
         scene.detectCollision(args.entry_level, args.expand_levels)
         scene.getIntTriPairCount()
+    tuned_rb = None
+    if not args.no_overlap and args.refit_blocks is None:
+        from oibvh_amd import autotune
+
+        def overlapped(i0, n):
+            for i in range(n):
+                f = (W + i0 + i) % len(frames[0])
+                scene.frame(pos_arr[f], shape[0], shape[1], asynchronous=True, overlap=True)
+
+        tuned_rb = autotune.tune_refit_blocks(ctx, scene, overlapped)
     timer = api.Timer(ctx)
     ctx.synchronize()
     # the library's own HIP events time the stage-1 launch of every 8th
@@ -273,8 +294,11 @@ def run_plain(args, ctx):
     timer.start()
     for fi in range(K):
         step(fi, True)
-    timer.stop()
+    # the last detects may still run on the library's side stream: wait for
+    # them before the stop event (the wait's host round trip, a few
+    # microseconds, is counted)
     ctx.synchronize()
+    timer.stop()
     total_ms = timer.ms()
     st = scene.stats()
     pairs_last = scene.getIntTriPairs()
@@ -288,7 +312,8 @@ def run_plain(args, ctx):
     detect_ms = tm["detect_ms"] / max(1, tm["detects"])
     N = [t.node_count for t in trees]
     H_pairs = len(pairs_last)
-    rk = reduce_kernel(ctx.get_tuning()["reduce_leaves_per_thread"], sum(Fm))
+    tn = ctx.get_tuning()
+    rk = reduce_kernel(tn["reduce_leaves_per_thread"], sum(Fm), tn["refit_blocks"], sum(-(-F // 1024) for F in Fm))
     # algorithmic bytes per launch: the refit's stage 1 reads the positions
     # and sorted faces once and writes every node below the apex (12 V + 12 F
     # + 24 N' per tree, N' = N minus the nodes k_reduce_apex writes), both
@@ -381,6 +406,7 @@ def run_plain(args, ctx):
                    "entry_level": args.entry_level, "expand_levels": args.expand_levels,
                    "frame_overlap": not args.no_overlap, "tuning": ctx.get_tuning()},
         "autotune": tuned,
+        "autotune_refit_blocks": tuned_rb,
         "bvtt_levels_last_frame": scene.level_counts().tolist(),
         "mtri_per_s": tri_per_frame * fps / 1e6,
         "pair_diffs": diffs,

@@ -39,7 +39,7 @@ class Tuning(C.Structure):
                 ("candidate_capacity", C.c_uint32), ("traverse_schedule", C.c_uint32),
                 ("traverse_local", C.c_uint32), ("traverse_local_levels", C.c_uint32),
                 ("narrow_separate", C.c_uint32), ("traverse_tail", C.c_uint32),
-                ("traverse_tail_steps", C.c_uint32)]
+                ("traverse_tail_steps", C.c_uint32), ("refit_blocks", C.c_uint32)]
 
 
 class SceneTiming(C.Structure):

@@ -159,3 +159,34 @@ def tune_batch(ctx: api.Context, batch: api.Batch, before_frame, *, entries=(0, 
                       "traverse_local_levels": best["traverse_local_levels"],
                       "traverse_tail": best["traverse_tail"], "traverse_tail_steps": best["traverse_tail_steps"]})
     return best
+
+
+def tune_refit_blocks(ctx: api.Context, scene: api.Scene, run, *, blocks=(0, 384, 448, 512, 640),
+                      frames: int = 40, reps: int = 2) -> dict:
+    """Pick refit_blocks (the cap on the scene refit's k_reduce_w workgroups)
+    by the rate of overlapped frames: run(i0, n) queues n overlapped frames
+    starting at frame index i0. A capped refit leaves CU slots to the detect
+    running beside it; the candidates are 1.5, 1.75, 2 and 2.5 workgroups per
+    CU of an MI355X (256 CUs) and no cap. Sets the best on ctx and returns
+    {'refit_blocks', 'ms', 'tried'}."""
+    timer = api.Timer(ctx)
+    tried = {}
+    for nb in blocks:
+        ctx.set_tuning(refit_blocks=nb)
+        run(0, 4)
+        scene.getIntTriPairCount()
+        best = None
+        for r in range(reps):
+            ctx.synchronize()
+            timer.start()
+            run(4 + r * frames, frames)
+            ctx.synchronize()
+            timer.stop()
+            ms = timer.ms() / frames
+            best = ms if best is None else min(best, ms)
+        scene.getIntTriPairCount()
+        tried[nb] = best
+    nb = min(tried, key=tried.get)
+    ctx.set_tuning(refit_blocks=nb)
+    return {"refit_blocks": nb, "ms": tried[nb], "tried": tried}
+

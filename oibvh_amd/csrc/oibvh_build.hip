@@ -75,9 +75,13 @@ struct BuildTab {
     uint32_t n_inl;
     uint32_t boff[OI_INLINE_MESHES + 1];
     oi_build_desc inl[OI_INLINE_MESHES];
-    __device__ __forceinline__ uint2 item() const
+    __device__ __forceinline__ uint2 item() const { return item_at(blockIdx.x, gridDim.x); }
+    // work item of virtual workgroup v of n (a launch of fewer workgroups
+    // than items loops v over b, b + gridDim.x, ...: with gridDim.x a
+    // multiple of 8, v stays on workgroup b's XCD)
+    __device__ __forceinline__ uint2 item_at(uint32_t v, uint32_t n) const
     {
-        const uint32_t w = oi_xcd_remap(blockIdx.x, gridDim.x);
+        const uint32_t w = oi_xcd_remap(v, n);
         if (work) return work[w];
         uint32_t m = 0;
 #pragma unroll
@@ -500,6 +504,7 @@ struct ReduceArgs {
     BuildTab T;
     const uint32_t* vals;  // build: Morton-sorted face ids (fbase + face); null: refit over sfaces
     uint32_t write_perm;   // build: also write perm (faces local to the mesh)
+    uint32_t nwork;        // k_reduce_w: work items (0: one per workgroup)
 };
 
 struct LeafLoad {
@@ -832,10 +837,10 @@ __device__ __forceinline__ void wave_climb8(float* stage, float* nodes, uint32_t
     }
 }
 
-__global__ void __launch_bounds__(OI_BLOCK) k_reduce_w(ReduceArgs A)
+// one work item (1024 leaves: a subtree per wave, then up to two levels
+// above them); S is the workgroup's staging, free again on return
+__device__ __forceinline__ void reduce_w_item(const ReduceArgs& A, uint2 it, float (*S)[6 * RW_W])
 {
-    __shared__ float S[OI_BLOCK / 64][6 * RW_W];
-    const uint2 it = A.T.item();
     const oi_build_desc D = A.T.mesh(it.x);
     const uint32_t L = D.L, vl = D.vl;
     const uint32_t hw = (L < RW_LG) ? L : RW_LG;
@@ -906,6 +911,25 @@ __global__ void __launch_bounds__(OI_BLOCK) k_reduce_w(ReduceArgs A)
                 for (int c = 0; c < 6; c++) o[c] = x.v[c];
             }
         }
+    }
+}
+
+__global__ void __launch_bounds__(OI_BLOCK) k_reduce_w(ReduceArgs A)
+{
+    __shared__ float S[OI_BLOCK / 64][6 * RW_W];
+    reduce_w_item(A, A.T.item(), S);
+}
+
+// k_reduce_w with fewer workgroups than work items (A.nwork): each takes
+// items b, b + gridDim.x, ...; the barrier keeps a workgroup's next item off
+// S until thread 0 has read the wave roots of this one (hb is uniform over
+// the workgroup). A kernel of its own: the loop costs registers
+__global__ void __launch_bounds__(OI_BLOCK) k_reduce_wl(ReduceArgs A)
+{
+    __shared__ float S[OI_BLOCK / 64][6 * RW_W];
+    for (uint32_t v = blockIdx.x; v < A.nwork; v += gridDim.x) {
+        reduce_w_item(A, A.T.item_at(v, A.nwork), S);
+        if (v + gridDim.x < A.nwork) __syncthreads();
     }
 }
 
@@ -1113,20 +1137,36 @@ uint32_t oi_inline_meshes() { return OI_INLINE_MESHES; }
 // stage 1 over up to OI_INLINE_MESHES meshes passed by value; boff[n + 1]
 // gives each mesh's first workgroup
 // with t0 and t1 (both or neither) the launch itself records them as it
-// starts and completes: their elapsed time is the kernel's own duration
+// starts and completes: their elapsed time is the kernel's own duration.
+// max_blocks > 0 (k_reduce_w only): at most that many workgroups (rounded up
+// to a multiple of 8), each looping over work items, which leaves the rest
+// of the GPU to work running beside the launch
 hipError_t oi_launch_reduce_inline(hipStream_t st, int M, const oi_build_desc* meshes, uint32_t n,
                                    const uint32_t* boff, const uint32_t* vals, uint32_t write_perm, hipEvent_t t0,
-                                   hipEvent_t t1)
+                                   hipEvent_t t1, uint32_t max_blocks)
 {
     if (n == 0 || n > OI_INLINE_MESHES) return hipErrorInvalidValue;
     if (!t0 != !t1) return hipErrorInvalidValue;
     ReduceArgs a{make_btab_inline(meshes, n, boff), vals, write_perm};
-    const dim3 g(boff[n]), b(OI_BLOCK);
+    dim3 g(boff[n]);
+    const dim3 b(OI_BLOCK);
+    if (M == 16 && max_blocks && max_blocks < boff[n]) {
+        const uint32_t cap = (max_blocks + 7u) & ~7u;
+        if (cap < boff[n]) {
+            a.nwork = boff[n];
+            g = dim3(cap);
+        }
+    }
     switch (M) {
     case 1: launch_timed(k_reduce<1>, g, b, st, t0, t1, a); break;
     case 2: launch_timed(k_reduce<2>, g, b, st, t0, t1, a); break;
     case 4: launch_timed(k_reduce<4>, g, b, st, t0, t1, a); break;
-    case 16: launch_timed(k_reduce_w, g, b, st, t0, t1, a); break;
+    case 16:
+        if (a.nwork)
+            launch_timed(k_reduce_wl, g, b, st, t0, t1, a);
+        else
+            launch_timed(k_reduce_w, g, b, st, t0, t1, a);
+        break;
     default: launch_timed(k_reduce<8>, g, b, st, t0, t1, a); break;
     }
     return hipGetLastError();

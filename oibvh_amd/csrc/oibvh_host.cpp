@@ -203,6 +203,7 @@ extern "C" oibvh_status oibvh_ctx_set_tuning(oibvh_ctx* ctx, const oibvh_tuning*
     if (t->traverse_expand > 3) return oi_set_err(ctx, OIBVH_ERR_INVALID, "traverse_expand must be 0..3");
     if (t->traverse_local > 2048 || t->traverse_tail > 2048)
         return oi_set_err(ctx, OIBVH_ERR_INVALID, "traverse_local and traverse_tail must be 0..2048");
+    if (t->refit_blocks > (1u << 20)) return oi_set_err(ctx, OIBVH_ERR_INVALID, "refit_blocks must be 0..2^20");
     ctx->leaves_per_thread = lpt;
     ctx->trav_blocks = t->traverse_blocks;
     ctx->trav_expand = t->traverse_expand;
@@ -215,6 +216,7 @@ extern "C" oibvh_status oibvh_ctx_set_tuning(oibvh_ctx* ctx, const oibvh_tuning*
     ctx->trav_tail = t->traverse_tail;
     ctx->trav_tail_steps = t->traverse_tail_steps;
     ctx->narrow_separate = t->narrow_separate;
+    ctx->refit_blocks = t->refit_blocks;
     ctx->tuning_gen++;
     return OIBVH_OK;
 }
@@ -234,6 +236,7 @@ extern "C" oibvh_status oibvh_ctx_get_tuning(const oibvh_ctx* ctx, oibvh_tuning*
     t->traverse_tail_steps = ctx->trav_tail_steps;
     t->narrow_separate = ctx->narrow_separate;
     t->hits_capacity = ctx->hits_cap;
+    t->refit_blocks = ctx->refit_blocks;
     return OIBVH_OK;
 }
 
@@ -850,7 +853,7 @@ static oibvh_status scene_refit(oibvh_scene* s, const oi_seed_args* seed, bool* 
             if (r != OIBVH_OK) return r;
         }
         // the timing pair is recorded by the stage-1 launch itself
-        HIPCHK(c, oi_launch_reduce_inline(c->stream, M, d, n, boff, nullptr, 0, e0, e1));
+        HIPCHK(c, oi_launch_reduce_inline(c->stream, M, d, n, boff, nullptr, 0, e0, e1, c->refit_blocks));
         const bool last = i0 + per >= s->trees.size();
         const oi_seed_args* sd = (last && seed && na) ? seed : nullptr;
         // `done` (may be null) is recorded as the refit's last launch completes

@@ -37,7 +37,8 @@ hipError_t oi_launch_reduce_apex(hipStream_t, int, const oi_build_desc*, const u
 hipError_t oi_launch_set_xf(hipStream_t, float*, const float*);
 uint32_t oi_inline_meshes();
 hipError_t oi_launch_reduce_inline(hipStream_t, int, const oi_build_desc*, uint32_t, const uint32_t*, const uint32_t*,
-                                   uint32_t, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
+                                   uint32_t, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr,
+                                   uint32_t max_blocks = 0);
 hipError_t oi_launch_reduce_apex_inline(hipStream_t, int, const oi_build_desc*, uint32_t, const oi_seed_args*,
                                         hipEvent_t);
 hipError_t oi_launch_set_mesh_pos(hipStream_t, oi_build_desc*, oi_mesh_desc*, uint32_t, const float*);
@@ -68,6 +69,7 @@ struct oibvh_ctx {
     uint32_t trav_local_levels = 0;  // steps the local walk replaces (0: all)
     uint32_t trav_tail = 0;          // 0: no tail walk; n: the last steps as a local walk, n pairs per workgroup
     uint32_t trav_tail_steps = 0;    // steps the tail walk replaces (0: all after the head walk)
+    uint32_t refit_blocks = 0;       // at most this many workgroups in a scene refit's k_reduce_w (0: one per item)
     uint32_t narrow_separate = 0;    // FP32 queries test leaf pairs in one k_narrow launch
     uint64_t tuning_gen = 0;         // counts oibvh_ctx_set_tuning calls
     uint32_t queue_cap = 1u << 22;

@@ -350,6 +350,40 @@ def test_scene_refit_equals_tree_refits(api, ctx, orc, n):
         assert np.array_equal(bits(trees[k].nodes()), bits(o.nodes())), f"tree {k}"
 
 
+@pytest.mark.parametrize("blocks", [8, 16, 40, 1 << 20])
+def test_scene_refit_capped_workgroups(api, orc, blocks):
+    """refit_blocks caps the scene refit's k_reduce_w launch; each workgroup
+    then loops over subtrees of 1024 leaves (k_reduce_wl). Nodes stay those
+    of the oracle's refit: two trees sharing a launch, sizes with a partial
+    last subtree, one with a model matrix, in overlapped frames too."""
+    say(f"scene refit, at most {blocks} workgroups")
+    c = api.Context(0)
+    c.set_tuning(reduce_leaves_per_thread=16, refit_blocks=blocks)
+    shapes = [M.uv_sphere(200, 101), M.icosphere(5, 1.0, 0.02, 5), soup(3001, 13)]
+    sc = api.Scene(c)
+    trees = []
+    for x, f in shapes:
+        t = api.OibvhTree(c, x, f)
+        t.build()
+        sc.addOibvhTree(t)
+        trees.append(t)
+    trees[2].set_transform(_motion(3, 2))
+    for rep in range(3):
+        moved = [(x * np.float32(1.0 + 0.01 * rep) + np.float32(0.002 * k)).astype(np.float32)
+                 for k, (x, f) in enumerate(shapes)]
+        for t, y in zip(trees, moved):
+            t.set_positions(y)
+        if rep < 2:
+            sc.refit()
+        else:
+            sc.frame(None, 3, 1, overlap=True)
+            sc.getIntTriPairCount()
+        for k, (x, f) in enumerate(shapes):
+            y = orc.xform(_motion(3, 2), moved[k]) if k == 2 else moved[k]
+            o = orc.Tree(y, f, order=trees[k].perm())
+            assert np.array_equal(bits(trees[k].nodes()), bits(o.nodes())), f"rep {rep} tree {k}"
+
+
 def test_scene_frame_and_timing(api, ctx, orc):
     """oibvh_scene_frame (borrow, refit, detect in one call) equals the
     separate calls, and the scene's event timing counts every refit and
