@@ -2,6 +2,8 @@
 // management, the launch sequence of one query and the overflow re-run loop
 // (Scene::detectCollisionOnGPU, scene.cu:225-446, which instead copies the
 // front size to the host after every level and uses fixed 10M-entry buffers).
+#include <cstddef>
+
 #include "oibvh_internal.h"
 
 // before a buffer an overlapped frame's detect may still use is freed or
@@ -30,22 +32,27 @@ static oibvh_status grow(oibvh_ctx* c, T*& p, uint32_t& cap, uint64_t need, uint
 oibvh_status oi_det_init(oi_detector* d, oibvh_ctx* c)
 {
     d->ctx = c;
-    hipError_t e = dalloc(&d->ctl_base, 2);
-    if (e == hipSuccess) e = hipMemsetAsync(d->ctl_base, 0, 2 * sizeof(oi_detect_ctl), c->stream);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&d->h_ctl, 2 * sizeof(oi_detect_ctl), hipHostMallocDefault);
+    hipError_t e = dalloc(&d->ctl_base, OI_FRAME_DEPTH);
+    if (e == hipSuccess) e = hipMemsetAsync(d->ctl_base, 0, OI_FRAME_DEPTH * sizeof(oi_detect_ctl), c->stream);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void**)&d->h_ctl, OI_FRAME_DEPTH * sizeof(oi_detect_ctl), hipHostMallocDefault);
     if (e != hipSuccess) {
         oi_det_free(d);
         return oi_set_err(c, OIBVH_ERR_HIP, "detector alloc: %s", hipGetErrorString(e));
     }
-    std::memset(d->h_ctl, 0, 2 * sizeof(oi_detect_ctl));
+    std::memset(d->h_ctl, 0, OI_FRAME_DEPTH * sizeof(oi_detect_ctl));
     d->ctl = d->ctl_base;
     return OIBVH_OK;
 }
 
 void oi_det_flip_set(oi_detector* d)
 {
-    d->prev_unchecked = d->unchecked;
-    d->set ^= 1u;
+    // the set left behind keeps an unchecked result until a check reads it;
+    // the set taken holds none any more: an asynchronous seed folds its
+    // errors into its sticky bits, and a synchronous frame checks first
+    if (d->unchecked) d->set_unchecked[d->set] = true;
+    d->set = (d->set + 1u) % OI_FRAME_DEPTH;
+    d->set_unchecked[d->set] = false;
     d->ctl = d->ctl_base + d->set;
     d->counts = d->counts_base ? d->counts_base + 256u * d->set : nullptr;
 }
@@ -151,7 +158,7 @@ oibvh_status oi_det_prepare(oi_detector* d)
         d->front_cap = fcap;
     }
     oibvh_status r;
-    if ((r = grow(c, d->counts_base, d->counts_cap, 512, d->generation)) != OIBVH_OK) return r;
+    if ((r = grow(c, d->counts_base, d->counts_cap, 256u * OI_FRAME_DEPTH, d->generation)) != OIBVH_OK) return r;
     d->counts = d->counts_base + 256u * d->set;
     if ((r = grow(c, d->cands, d->cand_cap, c->cand_cap, d->generation)) != OIBVH_OK) return r;
     if ((r = grow(c, d->hits, d->hits_cap, c->hits_cap, d->generation)) != OIBVH_OK) return r;
@@ -255,18 +262,29 @@ oibvh_status oi_det_check(oi_detector* d)
     for (;;) {
         d->h_counts.resize(d->n_levels + 1);
         HIPCHK(c, hipMemcpyAsync(d->h_ctl, d->ctl, sizeof(oi_detect_ctl), hipMemcpyDeviceToHost, c->stream));
-        if (d->prev_unchecked)
-            HIPCHK(c, hipMemcpyAsync(d->h_ctl + 1, d->ctl_base + (d->set ^ 1u), sizeof(oi_detect_ctl),
+        // the other sets still holding unchecked overlapped frames' results
+        uint32_t others[OI_FRAME_DEPTH], n_others = 0;
+        for (uint32_t s = 0; s < OI_FRAME_DEPTH; s++)
+            if (s != d->set && d->set_unchecked[s]) others[n_others++] = s;
+        for (uint32_t j = 0; j < n_others; j++)
+            HIPCHK(c, hipMemcpyAsync(d->h_ctl + 1 + j, d->ctl_base + others[j], sizeof(oi_detect_ctl),
                                      hipMemcpyDeviceToHost, c->stream));
+        // what is read here is reported once: clear the error and sticky
+        // bits of every set read, so later asynchronous seeds do not fold
+        // them in again (error and sticky are adjacent words)
+        static_assert(offsetof(oi_detect_ctl, sticky) == offsetof(oi_detect_ctl, error) + 4, "ctl layout");
+        HIPCHK(c, hipMemsetAsync(&d->ctl->error, 0, 8, c->stream));
+        for (uint32_t j = 0; j < n_others; j++)
+            HIPCHK(c, hipMemsetAsync(&d->ctl_base[others[j]].error, 0, 8, c->stream));
         HIPCHK(c, hipMemcpyAsync(d->h_counts.data(), d->counts, (d->n_levels + 1) * sizeof(uint32_t),
                                  hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         const oi_detect_ctl& k = d->h_ctl[0];
         earlier |= k.sticky;  // errors of unchecked asynchronous detects before this one
-        if (d->prev_unchecked) {
-            // the previous overlapped frame's detect and the unchecked ones before it
-            earlier |= d->h_ctl[1].sticky | d->h_ctl[1].error;
-            d->prev_unchecked = false;
+        for (uint32_t j = 0; j < n_others; j++) {
+            // earlier overlapped frames' detects and the unchecked ones before them
+            earlier |= d->h_ctl[1 + j].sticky | d->h_ctl[1 + j].error;
+            d->set_unchecked[others[j]] = false;
         }
         if (d->h_counts[d->n_levels] != 0) {
             // the last step reaches the leaves on every side: it writes no front

@@ -90,13 +90,13 @@ struct oibvh_tree {
     uint32_t* sfaces = nullptr; // leaf order
     uint32_t* perm = nullptr;   // leaf -> original
     float* nodes = nullptr;     // nbuf[cur]
-    // node buffers: overlapped frames refit into the one the previous frame's
-    // detect is not reading (nbuf[1] is allocated at the first such frame);
-    // read_seq[k] is 1 + the side-stream detect that last read nbuf[k]
-    // (0: none)
-    float* nbuf[2] = {nullptr, nullptr};
+    // node buffers: overlapped frames refit into them in turn, each into
+    // the one the detects of the frames before it are not reading (the
+    // others are allocated at the first such frames); read_seq[k] is 1 + the
+    // side-stream detect that last read nbuf[k] (0: none)
+    float* nbuf[OI_FRAME_DEPTH] = {};
     uint32_t cur = 0;
-    uint64_t read_seq[2] = {0, 0};
+    uint64_t read_seq[OI_FRAME_DEPTH] = {};
     // build scratch (allocated at the first build)
     uint32_t* keys = nullptr;
     uint32_t* keys_hi = nullptr; // high words of 63-bit keys
@@ -247,8 +247,7 @@ static void tree_free(oibvh_tree* t)
     dfree(t->faces);
     dfree(t->sfaces);
     dfree(t->perm);
-    dfree(t->nbuf[0]);
-    dfree(t->nbuf[1]);
+    for (float*& b : t->nbuf) dfree(b);
     t->nodes = nullptr;
     dfree(t->keys);
     dfree(t->keys_hi);
@@ -892,11 +891,12 @@ static oibvh_status overlapped_frame(oibvh_scene* s, uint32_t entry, uint32_t ex
     // runs its detects in order, so that covers the earlier ones); once
     // OI_SIDE_RING detects later its ring event has been recorded again, and
     // the wait is for that later point
-    const bool prev_unchecked = d.prev_unchecked;
+    bool set_unchecked[OI_FRAME_DEPTH];
+    std::copy(d.set_unchecked, d.set_unchecked + OI_FRAME_DEPTH, set_unchecked);
     oi_det_flip_set(&d);
     uint64_t need = d.set_seq[d.set];
     for (oibvh_tree* t : s->trees) {
-        const uint32_t k = t->cur ^ 1u;
+        const uint32_t k = (t->cur + 1u) % OI_FRAME_DEPTH;
         if (!t->nbuf[k]) {
             HIPCHK(c, oi_ctx_quiesce(c));  // the allocation is the first of this tree's second buffer
             HIPCHK(c, dalloc(&t->nbuf[k], 6ull * t->N));
@@ -914,13 +914,13 @@ static oibvh_status overlapped_frame(oibvh_scene* s, uint32_t entry, uint32_t ex
     if ((r = prepare_scene_detect(s)) != OIBVH_OK) {
         // nothing was queued: the trees keep the buffers they were refit into
         for (oibvh_tree* t : s->trees) {
-            t->cur ^= 1u;
+            t->cur = (t->cur + OI_FRAME_DEPTH - 1u) % OI_FRAME_DEPTH;
             t->nodes = t->nbuf[t->cur];
         }
-        d.set ^= 1u;
+        d.set = (d.set + OI_FRAME_DEPTH - 1u) % OI_FRAME_DEPTH;
         d.ctl = d.ctl_base + d.set;
         d.counts = d.counts_base ? d.counts_base + 256u * d.set : nullptr;
-        d.prev_unchecked = prev_unchecked;
+        std::copy(set_unchecked, set_unchecked + OI_FRAME_DEPTH, d.set_unchecked);
         return r;
     }
     // the refit's apex launch also resets this frame's counter set

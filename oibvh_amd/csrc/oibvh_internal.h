@@ -89,6 +89,14 @@ struct oibvh_ctx {
 };
 
 #define OI_SIDE_RING 4u
+// node buffers per tree and counter sets per detector of overlapped frames:
+// frame f's refit rewrites the buffer and resets the set that frame
+// f - OI_FRAME_DEPTH's detect used, so it waits for that detect only and may
+// run beside the detects of the frames in between. Measured on the deforming
+// UV pair (2 x 1M triangles): 3 gave 75.9-80.5 us per frame against 73.9 us
+// with 2 (more node buffers in flight, and a refit beside more detect work)
+#define OI_FRAME_DEPTH 2u
+static_assert(OI_SIDE_RING >= OI_FRAME_DEPTH, "the ring must still hold the event a refit waits for");
 
 // drop one reference held by a tree, scene or batch (frees a context whose
 // destroy was deferred)
@@ -151,16 +159,16 @@ struct oi_detector {
     uint32_t n_front = 0;
     uint2* front[2] = {nullptr, nullptr};  // ping-pong BVTT fronts
     uint32_t front_cap = 0;
-    // two sets of level counts and counters: overlapped frames alternate
-    // between them, so a frame's refit can reset one while the previous
-    // frame's detect still uses the other (set k: counts_base + 256 k,
-    // ctl_base + k)
+    // OI_FRAME_DEPTH sets of level counts and counters: overlapped frames
+    // take them in turn, so a frame's refit can reset one while the detects
+    // of the frames before it still use the others (set k: counts_base +
+    // 256 k, ctl_base + k)
     uint32_t* counts_base = nullptr;
     uint32_t counts_cap = 0;
     oi_detect_ctl* ctl_base = nullptr;
     uint32_t set = 0;
-    uint64_t set_seq[2] = {0, 0};  // 1 + the side-stream detect that last used set k (0: none)
-    bool prev_unchecked = false;  // the other set holds an unchecked detect's result
+    uint64_t set_seq[OI_FRAME_DEPTH] = {};       // 1 + the side-stream detect that last used set k (0: none)
+    bool set_unchecked[OI_FRAME_DEPTH] = {};     // set k (not the current one) holds an unchecked result
     uint32_t* counts = nullptr;  // front length per step (+1), of the current set
     uint32_t n_levels = 0;
     uint2* cands = nullptr;
@@ -168,7 +176,7 @@ struct oi_detector {
     uint4* hits = nullptr;
     uint32_t hits_cap = 0;
     oi_detect_ctl* ctl = nullptr;    // counters of the current set
-    oi_detect_ctl* h_ctl = nullptr;  // pinned, two (the current set's, the other's)
+    oi_detect_ctl* h_ctl = nullptr;  // pinned, OI_FRAME_DEPTH (the current set's, then the others read)
     std::vector<uint32_t> h_counts;
     // the query (kept for re-runs)
     oi_tab_args tab{};

@@ -498,7 +498,9 @@ def test_async_frames_report_earlier_errors(api, ctx, orc, overlap):
     """Frames queued without a host wait whose hit lists overflow: reading
     the last one re-runs it with a larger buffer (the oracle's pairs) and
     reports the overflow of the unchecked frames before it in async_errors,
-    with overlapped frames (two counter sets) as without."""
+    with overlapped frames (a counter set per frame in flight) as without.
+    Errors are reported once: clean asynchronous frames after that check
+    report none."""
     say(f"async frame errors overlap={overlap}")
     meshes = M.interpenetrating_icospheres(subdiv=4, seed=27, offset=1.2)
     want = orc.detect([orc.Tree(x, f) for x, f in meshes])
@@ -516,6 +518,12 @@ def test_async_frames_report_earlier_errors(api, ctx, orc, overlap):
         st = sc.stats()
         assert st["retries"] >= 1
         assert st["async_errors"] & 8  # hit-list overflow of the earlier frames
+        for _ in range(5):  # the grown buffer holds these frames' hits
+            sc.frame(None, 2, 2, asynchronous=True, overlap=overlap)
+        assert np.array_equal(sc.getIntTriPairs(), want)
+        assert sc.stats()["async_errors"] == 0 and sc.stats()["retries"] == 0
+        for _ in range(3):
+            sc.frame(None, 2, 2, asynchronous=True, overlap=overlap)
         sc.frame(None, 2, 2, overlap=overlap)  # a checked frame starts afresh
         assert np.array_equal(sc.getIntTriPairs(), want)
         assert sc.stats()["async_errors"] == 0 and sc.stats()["retries"] == 0
