@@ -755,30 +755,26 @@ __device__ __forceinline__ void wave_level(float* stage, uint32_t width, uint32_
     wave_sync();
 }
 
-// The leaf level of one wave subtree into stage (node layout): every load is
-// issued before any box is formed. BUILD reads the sorted face ids and writes
-// sfaces and perm; XF applies the model matrix. Both are compile-time, so the
-// loads form one straight run.
-template <bool BUILD, bool XF>
-__device__ __forceinline__ void wave_leaves(const ReduceArgs& A, const oi_build_desc& D, uint32_t f0, uint32_t hw,
-                                            uint32_t nleaf, uint32_t lane, float* stage)
+// the refit's sorted faces of the leaves a wave owns
+__device__ __forceinline__ void wave_sfaces(const oi_build_desc& D, uint32_t f0, uint32_t hw, uint32_t nleaf,
+                                            uint32_t lane, uint32_t (&f)[RW_W / 64][3])
+{
+#pragma unroll
+    for (int k = 0; k < RW_W / 64; k++) {
+        const uint32_t off = 64u * k + lane;
+        const uint32_t p = f0 + off;
+        const bool ok = off < (1u << hw) && p < nleaf;
+        oi_ld3(D.sfaces, ok ? p : f0, f[k]);  // in range; only ok results are kept
+    }
+}
+
+// the leaf boxes of a wave's faces f (vertex gathers, model matrix if XF),
+// staged at their leaf slots
+template <bool XF>
+__device__ __forceinline__ void wave_boxes(const oi_build_desc& D, uint32_t f0, uint32_t hw, uint32_t nleaf,
+                                           uint32_t lane, float* stage, const uint32_t (&f)[RW_W / 64][3])
 {
     constexpr int K = RW_W / 64;
-    uint32_t f[K][3], p[K], fid[K];
-    bool ok[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        const uint32_t off = 64u * k + lane;
-        p[k] = f0 + off;
-        ok[k] = off < (1u << hw) && p[k] < nleaf;
-        const uint32_t pc = ok[k] ? p[k] : f0;  // in range; only ok[k] results are kept
-        if constexpr (BUILD) {
-            fid[k] = oi_g(A.vals)[D.fbase + pc] - D.fbase;
-            oi_ld3(D.faces, fid[k], f[k]);
-        } else {
-            oi_ld3(D.sfaces, pc, f[k]);
-        }
-    }
     float m[16];
     if constexpr (XF) {
 #pragma unroll
@@ -805,18 +801,47 @@ __device__ __forceinline__ void wave_leaves(const ReduceArgs& A, const oi_build_
             r.v[c] = gmin(gmin(v[k][0][c], v[k][1][c]), v[k][2][c]);
             r.v[3 + c] = gmax(gmax(v[k][0][c], v[k][1][c]), v[k][2][c]);
         }
-        if (ok[k]) put_box(stage, 64u * k + lane, r);
+        const uint32_t off = 64u * k + lane;
+        if (off < (1u << hw) && f0 + off < nleaf) put_box(stage, off, r);
     }
+}
+
+// The leaf level of one wave subtree into stage (node layout): every load is
+// issued before any box is formed. BUILD reads the sorted face ids and writes
+// sfaces and perm; XF applies the model matrix. Both are compile-time, so the
+// loads form one straight run.
+template <bool BUILD, bool XF>
+__device__ __forceinline__ void wave_leaves(const ReduceArgs& A, const oi_build_desc& D, uint32_t f0, uint32_t hw,
+                                            uint32_t nleaf, uint32_t lane, float* stage)
+{
+    constexpr int K = RW_W / 64;
+    uint32_t f[K][3], fid[K];
     if constexpr (BUILD) {
 #pragma unroll
-        for (int k = 0; k < K; k++)
-            if (ok[k]) {
-                OI_GLOBAL uint32_t* q = oi_gw(D.sfaces) + 3ull * p[k];
+        for (int k = 0; k < K; k++) {
+            const uint32_t off = 64u * k + lane;
+            const uint32_t p = f0 + off;
+            const uint32_t pc = (off < (1u << hw) && p < nleaf) ? p : f0;  // only in-range results are kept
+            fid[k] = oi_g(A.vals)[D.fbase + pc] - D.fbase;
+            oi_ld3(D.faces, fid[k], f[k]);
+        }
+    } else {
+        wave_sfaces(D, f0, hw, nleaf, lane, f);
+    }
+    wave_boxes<XF>(D, f0, hw, nleaf, lane, stage, f);
+    if constexpr (BUILD) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const uint32_t off = 64u * k + lane;
+            const uint32_t p = f0 + off;
+            if (off < (1u << hw) && p < nleaf) {
+                OI_GLOBAL uint32_t* q = oi_gw(D.sfaces) + 3ull * p;
                 q[0] = f[k][0];
                 q[1] = f[k][1];
                 q[2] = f[k][2];
-                if (A.write_perm) oi_gw(D.perm)[p[k]] = fid[k];
+                if (A.write_perm) oi_gw(D.perm)[p] = fid[k];
             }
+        }
     }
 }
 
@@ -837,81 +862,111 @@ __device__ __forceinline__ void wave_climb8(float* stage, float* nodes, uint32_t
     }
 }
 
-// one work item (1024 leaves: a subtree per wave, then up to two levels
-// above them); S is the workgroup's staging, free again on return
+// a wave's part of one work item once its leaf boxes are staged: write the
+// leaves, climb hw levels (wave 0 .. 2^hb - 1 of the workgroup)
+__device__ __forceinline__ void wave_climb_all(const oi_build_desc& D, uint32_t f0, uint32_t hw, uint32_t nleaf,
+                                               uint32_t lane, float* W)
+{
+    const uint32_t L = D.L, vl = D.vl;
+    wave_sync();
+    wave_copy_out<RW_W>(D.nodes, oi_lvl_start(L, L, vl) + f0, min(1u << hw, nleaf - f0), W, lane);
+    if (hw == RW_LG) {
+        wave_climb8<1>(W, D.nodes, L, vl, f0, nleaf, lane);
+    } else {
+        // short trees: the same climb with run-time widths
+        uint32_t ncnt = nleaf;
+        for (uint32_t s = 1; s <= hw; s++) {
+            const uint32_t l = L - s;
+            const uint32_t cc = ncnt;
+            ncnt = oi_lvl_cnt(l, L, vl);
+            const uint32_t width = 1u << (hw - s);
+            const uint32_t f = f0 >> s;
+            wave_level<RW_W / 2>(W, width, f, cc, lane);
+            wave_copy_out<RW_W / 2>(D.nodes, oi_lvl_start(l, L, vl) + f, min(width, ncnt - f), W, lane);
+        }
+    }
+}
+
+// thread 0 merges the wave roots of work item `it` (slot 0 of each wave's
+// stage) hb levels further; called by every thread after a barrier
+__device__ __forceinline__ void item_top(const oi_build_desc& D, uint2 it, uint32_t hw, uint32_t hb,
+                                         float (*S)[6 * RW_W])
+{
+    if (threadIdx.x != 0) return;
+    const uint32_t L = D.L, vl = D.vl;
+    const uint32_t lw = L - hw;
+    uint32_t cc = oi_lvl_cnt(lw, L, vl);
+    Box6 m[2];
+    const uint32_t n1 = 1u << (hb - 1u);  // nodes of the first merged level
+#pragma unroll
+    for (uint32_t t = 0; t < 2u; t++) {
+        const uint32_t q = (it.y << (hb - 1u)) + t;
+        if (t < n1 && 2u * q < cc) {
+            Box6 x = get_box(S[2u * t], 0);
+            if (2u * q + 1u < cc) box_merge_into(x, get_box(S[2u * t + 1u], 0));
+            m[t] = x;
+            OI_GLOBAL float* o = oi_gw(D.nodes) + 6ull * (oi_lvl_start(lw - 1u, L, vl) + q);
+#pragma unroll
+            for (int c = 0; c < 6; c++) o[c] = x.v[c];
+        }
+    }
+    if (hb == 2) {
+        cc = oi_lvl_cnt(lw - 1u, L, vl);
+        const uint32_t q = it.y;
+        if (2u * q < cc) {
+            Box6 x = m[0];
+            if (2u * q + 1u < cc) box_merge_into(x, m[1]);
+            OI_GLOBAL float* o = oi_gw(D.nodes) + 6ull * (oi_lvl_start(lw - 2u, L, vl) + q);
+#pragma unroll
+            for (int c = 0; c < 6; c++) o[c] = x.v[c];
+        }
+    }
+}
+
+// what one wave of a workgroup does for work item `it` (1024 leaves: a
+// subtree per wave, then up to two levels above them)
+struct WaveItem {
+    oi_build_desc D;
+    uint32_t hw, hb, f0, nleaf;
+    bool on;  // this wave owns leaves of the item
+};
+__device__ __forceinline__ WaveItem wave_item(const ReduceArgs& A, uint2 it, uint32_t wave)
+{
+    WaveItem w;
+    w.D = A.T.mesh(it.x);
+    const uint32_t L = w.D.L;
+    w.hw = (L < RW_LG) ? L : RW_LG;
+    w.hb = (L - w.hw < 2u) ? L - w.hw : 2u;
+    w.f0 = ((it.y << w.hb) + wave) << w.hw;
+    w.nleaf = oi_lvl_cnt(L, L, w.D.vl);
+    w.on = wave < (1u << w.hb) && w.f0 < w.nleaf;
+    return w;
+}
+
+// one work item; S is the workgroup's staging, free again on return
 __device__ __forceinline__ void reduce_w_item(const ReduceArgs& A, uint2 it, float (*S)[6 * RW_W])
 {
-    const oi_build_desc D = A.T.mesh(it.x);
-    const uint32_t L = D.L, vl = D.vl;
-    const uint32_t hw = (L < RW_LG) ? L : RW_LG;
-    const uint32_t hb = (L - hw < 2u) ? L - hw : 2u;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t ws = (it.y << hb) + wave;
-    const uint32_t f0 = ws << hw;
-    const uint32_t nleaf = oi_lvl_cnt(L, L, vl);
+    const WaveItem w = wave_item(A, it, wave);
+    const oi_build_desc& D = w.D;
     float* W = S[wave];
-    if (wave < (1u << hb) && f0 < nleaf) {
+    if (w.on) {
         if (A.vals) {
             if (D.xf)
-                wave_leaves<true, true>(A, D, f0, hw, nleaf, lane, W);
+                wave_leaves<true, true>(A, D, w.f0, w.hw, w.nleaf, lane, W);
             else
-                wave_leaves<true, false>(A, D, f0, hw, nleaf, lane, W);
+                wave_leaves<true, false>(A, D, w.f0, w.hw, w.nleaf, lane, W);
         } else {
             if (D.xf)
-                wave_leaves<false, true>(A, D, f0, hw, nleaf, lane, W);
+                wave_leaves<false, true>(A, D, w.f0, w.hw, w.nleaf, lane, W);
             else
-                wave_leaves<false, false>(A, D, f0, hw, nleaf, lane, W);
+                wave_leaves<false, false>(A, D, w.f0, w.hw, w.nleaf, lane, W);
         }
-        wave_sync();
-        wave_copy_out<RW_W>(D.nodes, oi_lvl_start(L, L, vl) + f0, min(1u << hw, nleaf - f0), W, lane);
-        if (hw == RW_LG) {
-            wave_climb8<1>(W, D.nodes, L, vl, f0, nleaf, lane);
-        } else {
-            // short trees: the same climb with run-time widths
-            uint32_t ncnt = nleaf;
-            for (uint32_t s = 1; s <= hw; s++) {
-                const uint32_t l = L - s;
-                const uint32_t cc = ncnt;
-                ncnt = oi_lvl_cnt(l, L, vl);
-                const uint32_t width = 1u << (hw - s);
-                const uint32_t f = f0 >> s;
-                wave_level<RW_W / 2>(W, width, f, cc, lane);
-                wave_copy_out<RW_W / 2>(D.nodes, oi_lvl_start(l, L, vl) + f, min(width, ncnt - f), W, lane);
-            }
-        }
+        wave_climb_all(D, w.f0, w.hw, w.nleaf, lane, W);
     }
-    if (hb == 0) return;
+    if (w.hb == 0) return;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        // every wave root sits in slot 0 of its wave's stage
-        const uint32_t lw = L - hw;
-        uint32_t cc = oi_lvl_cnt(lw, L, vl);
-        Box6 m[2];
-        const uint32_t n1 = 1u << (hb - 1u);  // nodes of the first merged level
-#pragma unroll
-        for (uint32_t t = 0; t < 2u; t++) {
-            const uint32_t q = (it.y << (hb - 1u)) + t;
-            if (t < n1 && 2u * q < cc) {
-                Box6 x = get_box(S[2u * t], 0);
-                if (2u * q + 1u < cc) box_merge_into(x, get_box(S[2u * t + 1u], 0));
-                m[t] = x;
-                OI_GLOBAL float* o = oi_gw(D.nodes) + 6ull * (oi_lvl_start(lw - 1u, L, vl) + q);
-#pragma unroll
-                for (int c = 0; c < 6; c++) o[c] = x.v[c];
-            }
-        }
-        if (hb == 2) {
-            cc = oi_lvl_cnt(lw - 1u, L, vl);
-            const uint32_t q = it.y;
-            if (2u * q < cc) {
-                Box6 x = m[0];
-                if (2u * q + 1u < cc) box_merge_into(x, m[1]);
-                OI_GLOBAL float* o = oi_gw(D.nodes) + 6ull * (oi_lvl_start(lw - 2u, L, vl) + q);
-#pragma unroll
-                for (int c = 0; c < 6; c++) o[c] = x.v[c];
-            }
-        }
-    }
+    item_top(D, it, w.hw, w.hb, S);
 }
 
 __global__ void __launch_bounds__(OI_BLOCK) k_reduce_w(ReduceArgs A)
@@ -920,10 +975,13 @@ __global__ void __launch_bounds__(OI_BLOCK) k_reduce_w(ReduceArgs A)
     reduce_w_item(A, A.T.item(), S);
 }
 
-// k_reduce_w with fewer workgroups than work items (A.nwork): each takes
-// items b, b + gridDim.x, ...; the barrier keeps a workgroup's next item off
-// S until thread 0 has read the wave roots of this one (hb is uniform over
-// the workgroup). A kernel of its own: the loop costs registers
+// k_reduce_w for a refit with fewer workgroups than work items (A.nwork):
+// each takes items b, b + gridDim.x, ...; the barrier keeps a workgroup's
+// next item off S until thread 0 has read the wave roots of this one (hb is
+// uniform over the workgroup). A kernel of its own: the loop costs
+// registers. (Loading the next item's faces before the current item's
+// vertices was measured: the refit was no faster beside a detect, and the
+// detect slower.)
 __global__ void __launch_bounds__(OI_BLOCK) k_reduce_wl(ReduceArgs A)
 {
     __shared__ float S[OI_BLOCK / 64][6 * RW_W];
@@ -1138,7 +1196,7 @@ uint32_t oi_inline_meshes() { return OI_INLINE_MESHES; }
 // gives each mesh's first workgroup
 // with t0 and t1 (both or neither) the launch itself records them as it
 // starts and completes: their elapsed time is the kernel's own duration.
-// max_blocks > 0 (k_reduce_w only): at most that many workgroups (rounded up
+// max_blocks > 0 (refits with k_reduce_w only): at most that many workgroups (rounded up
 // to a multiple of 8), each looping over work items, which leaves the rest
 // of the GPU to work running beside the launch
 hipError_t oi_launch_reduce_inline(hipStream_t st, int M, const oi_build_desc* meshes, uint32_t n,
@@ -1150,7 +1208,7 @@ hipError_t oi_launch_reduce_inline(hipStream_t st, int M, const oi_build_desc* m
     ReduceArgs a{make_btab_inline(meshes, n, boff), vals, write_perm};
     dim3 g(boff[n]);
     const dim3 b(OI_BLOCK);
-    if (M == 16 && max_blocks && max_blocks < boff[n]) {
+    if (M == 16 && !vals && max_blocks && max_blocks < boff[n]) {
         const uint32_t cap = (max_blocks + 7u) & ~7u;
         if (cap < boff[n]) {
             a.nwork = boff[n];
