@@ -245,12 +245,13 @@ typedef struct {
                                           (1..2048); an overflow is handled as for traverse_local */
     uint32_t traverse_tail_steps;      /* steps the tail walk replaces, counted from the last (0 = all
                                           after the head local walk, or all) */
-    uint32_t refit_blocks;             /* scene refits with reduce_leaves_per_thread 16: at most this
-                                          many workgroups in the stage-1 launch (rounded up to a
-                                          multiple of 8, up to 2^20), each looping over subtrees of
-                                          1024 leaves, so that an overlapped frame's refit leaves
-                                          room on the GPU for the detect beside it (0 = one
-                                          workgroup per 1024 leaves) */
+    uint32_t refit_blocks;             /* scene refits whose stage 1 is the wave kernel
+                                          (reduce_leaves_per_thread 16, or 0 on launches of 2^18
+                                          faces or more): at most this many workgroups in that
+                                          launch (rounded up to a multiple of 8, up to 2^20), each
+                                          looping over subtrees of 1024 leaves, so that an
+                                          overlapped frame's refit leaves room on the GPU for the
+                                          detect beside it (0 = one workgroup per 1024 leaves) */
 } oibvh_tuning;
 
 /* The detect buffers grow on overflow (the detect is re-run). */
