@@ -811,8 +811,8 @@ def run_self(args, ctx):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this pipeline runs on a single "
-              "MI355X; sharding scenes or trees over several GPUs is not built)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the library runs each scene "
+              "or batch on one MI355X; sharding scenes or trees over several GPUs is not built)", file=sys.stderr)
         sys.exit(2)
     from oibvh_amd import api
 
