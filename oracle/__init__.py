@@ -6,7 +6,8 @@ this module; the product path (oibvh_amd) never does.
 
 ``load_ref()`` returns oracle/_ref/libref.so -- the reference's own
 src/utils/utils.cpp compiled from the reference tree -- or None where that
-tree is not mounted (it never is on the GPU box).
+build is absent (a machine with the repository but not the reference tree
+runs the checks without it).
 """
 from __future__ import annotations
 
