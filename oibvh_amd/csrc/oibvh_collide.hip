@@ -170,7 +170,11 @@ __device__ bool sat_f32(const v3f* P, const v3f* Q)
 // exact FP32 inputs (p2 = P2 - P1 and so on) are the leaves: their only error
 // is their own rounding, so S = |fl(difference)| at depth 1; above them S is
 // built by the same expression over magnitudes (sums of magnitudes for +/-,
-// products for *). The deepest SAT projection (a g/h axis) has c = 13. A
+// products for *). With c counted as roundings along the chain (a product
+// adds its operands' depths plus one, a sum takes the larger plus one): the
+// differences p = P - P1 have c = 1, edges 2, n1 = e x e 6, g = e x n1 10,
+// the products of a g/h projection 12 and its two additions 14, the deepest
+// SAT quantity. A
 // comparison mn1 > mx2 is decided in FP32 only when |mn1 - mx2| exceeds
 // T = 32 * 2^-24 * (S_P + S_Q) + 2^-100, which covers the FP32 error of both
 // sides, the FP64 error of the oracle's own evaluation, the rounding of the
