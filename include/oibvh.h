@@ -386,6 +386,32 @@ OIBVH_API oibvh_status oibvh_batch_get_nodes(oibvh_batch* batch, uint32_t mesh, 
                                              uint32_t* n);
 OIBVH_API oibvh_status oibvh_batch_get_perm(oibvh_batch* batch, uint32_t mesh, uint32_t* out, uint64_t cap);
 
+/* ---------------------------------------------------------------- no allocation */
+
+/* Largest F that oibvh_build_noalloc accepts. */
+#define OIBVH_NOALLOC_MAX_FACES (1u << 20)
+
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the build of oibvh_tree_build (OibvhTree::build,
+ * oibvhTree.cu:237-388; 30-bit keys, no model matrix) of the mesh xyz (V*3
+ * floats) / idx (F*3 vertex ids, each < V), all caller-owned device memory,
+ * writing the tree's nodes (N*6 floats, N = 2F + popcount(next_pow2(F) - F) - 1,
+ * real order as oibvh_tree_get_nodes), the leaf -> original face permutation
+ * (F words, as oibvh_tree_get_perm) and the faces in leaf order (F*3 words, as
+ * oibvh_tree_get_sorted_faces). Results equal those of oibvh_tree_create with
+ * the same data plus oibvh_tree_build, bit for bit.
+ *
+ * Allocates nothing: the build's scratch (sort keys and values, digit
+ * histograms, look-back tiles, mesh-box partials; about 12.6 MB) is a fixed
+ * device array the library holds on every GPU it runs on until the process
+ * ends. F above OIBVH_NOALLOC_MAX_FACES is refused (OIBVH_ERR_CAPACITY).
+ * Enqueued on the context's stream; nothing waits. All calls on one GPU share
+ * that GPU's array: calls that overlap on one GPU (from different streams or
+ * contexts of the device) race; calls on one stream are ordered. */
+OIBVH_API oibvh_status oibvh_build_noalloc(oibvh_ctx* ctx, const float* xyz, uint32_t V, const uint32_t* idx,
+                                           uint32_t F, float* nodes, uint32_t* perm, uint32_t* sorted_faces);
+
 /* ---------------------------------------------------------------- mesh files */
 
 /* Wavefront OBJ positions and triangles, the mesh ingest the reference gets

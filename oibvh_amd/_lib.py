@@ -113,7 +113,10 @@ API = [
     ("oibvh_batch_get_level_counts", _i, [_vp, _u32p, _u64, _u32p]),
     ("oibvh_batch_get_nodes", _i, [_vp, _u32, _f32p, _u64, _u32p]),
     ("oibvh_batch_get_perm", _i, [_vp, _u32, _u32p, _u64]),
+    ("oibvh_build_noalloc", _i, [_vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp]),
 ]
+
+NOALLOC_MAX_FACES = 1 << 20
 
 _lib = None
 _hip = None

@@ -25,7 +25,7 @@ COMMON = ["-std=c++17", "-fPIC", "-fvisibility=hidden", f"-I{INC}", f"-I{CSRC}"]
 DEVICE = ["-O3", f"--offload-arch={ARCH}", "-ffp-contract=off",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 SOURCES = ["oibvh_build.hip", "oibvh_collide.hip", "oibvh_host.cpp", "oibvh_detect.cpp", "oibvh_batch.cpp",
-           "oibvh_obj.cpp"]
+           "oibvh_obj.cpp", "oibvh_noalloc.hip"]
 
 
 def _compile(src: str) -> str:

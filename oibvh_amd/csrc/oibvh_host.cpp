@@ -155,6 +155,9 @@ extern "C" oibvh_status oibvh_ctx_create(int device, void* stream, oibvh_ctx** o
         return OIBVH_ERR_NO_DEVICE;
     }
     c->cus = prop.multiProcessorCount;
+    // the no-allocation build's scratch address on this device (a failure
+    // is reported by that entry point, which tries again)
+    (void)oi_noalloc_attach(device);
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
         c->clock_khz = (uint64_t)khz;
@@ -276,7 +279,7 @@ static oibvh_status tree_alloc_common(oibvh_tree* t)
     return OIBVH_OK;
 }
 
-static uint32_t box_blocks(const oibvh_tree* t) { return std::max(1u, std::min(256u, (t->V + 4095u) / 4096u)); }
+uint32_t oi_box_parts(uint32_t V) { return std::max(1u, std::min(256u, (V + 4095u) / 4096u)); }
 
 // the tree as a one-mesh build table
 static oi_build_desc tree_desc(const oibvh_tree* t)
@@ -295,7 +298,7 @@ static oi_build_desc tree_desc(const oibvh_tree* t)
     d.vl = t->vl;
     d.fbase = 0;
     d.pbase = 0;
-    d.nparts = box_blocks(t);
+    d.nparts = oi_box_parts(t->V);
     d.vchunk = (t->V + d.nparts - 1) / d.nparts;
     return d;
 }
@@ -409,7 +412,7 @@ extern "C" void oibvh_tree_destroy(oibvh_tree* t)
 // reduce variant for F faces per launch: the wave-subtree kernel (16) once
 // there are about as many of its 1024-leaf workgroups as CUs, else M leaves
 // per thread with >= ~2 workgroups per CU
-static int oi_default_lpt(uint64_t F, int cus)
+int oi_default_lpt(uint64_t F, int cus)
 {
     if (F >= (1ull << 18)) return 16;
     const uint64_t target = 2ull * (uint64_t)(cus > 0 ? cus : 256) * 256ull;
@@ -437,6 +440,48 @@ static oibvh_status launch_reduce(oibvh_tree* t, bool build)
     return OIBVH_OK;
 }
 
+// The launches of one build (OibvhTree::build, oibvhTree.cu:237-388) of the
+// mesh in d (d.box: the Morton box followed by the mesh-box partials) with
+// scratch s: mesh box, Morton keys, stable LSD radix sort (values end in
+// d.perm), then the reduce in build mode (writes d.sfaces and d.nodes). No
+// allocation and no synchronisation, so oibvh_build_noalloc can share it.
+oibvh_status oi_enqueue_build(oibvh_ctx* c, const oi_build_desc& d, const oi_build_scratch& s, int M)
+{
+    const uint32_t F = d.F;
+    const uint32_t tiles = (F + oi_sort_tile() - 1) / oi_sort_tile();
+    const bool m63 = s.keys_hi != nullptr;
+    uint32_t* ghist = s.sort;
+    uint32_t* tctr = s.sort + OI_SORT_HIST;
+    uint32_t* lb[2] = {s.sort + OI_SORT_HEAD, s.sort + OI_SORT_HEAD + 256ull * tiles};
+    hipStream_t st = c->stream;
+    // the mesh-box launch also clears the histograms, tile counters and the
+    // first look-back buffer (a kernel, not a memset: capturable)
+    HIPCHK(c, oi_launch_mesh_box(st, nullptr, nullptr, &d, d.nparts, d.box + 6, s.sort, OI_SORT_HEAD + 256u * tiles));
+    const uint32_t mblocks = (F + oi_morton_chunk() - 1) / oi_morton_chunk();
+    HIPCHK(c, oi_launch_morton(st, nullptr, nullptr, &d, mblocks, d.box + 6, s.keys, s.keys_hi, ghist, nullptr));
+    // LSD passes over the low word (keys move with the values), then, for
+    // 63-bit keys, over the high word looked up by face id; values end in perm
+    const int np = m63 ? 8 : 4;
+    for (int p = 0; p < np; p++) {
+        const bool lo = p < 4;
+        const uint32_t* kin = lo ? ((p & 1) ? s.keys_alt : s.keys) : nullptr;
+        uint32_t* kout = (lo && p < 3) ? ((p & 1) ? s.keys : s.keys_alt) : nullptr;
+        const uint32_t* vin = p == 0 ? nullptr : ((p & 1) ? s.vals_tmp : d.perm);
+        uint32_t* vout = (p & 1) ? d.perm : s.vals_tmp;
+        HIPCHK(c, oi_launch_sort_pass(st, kin, vin, kout, vout, F, 8u * (p & 3), ghist + 256 * p, lb[p & 1],
+                                      p + 1 < np ? lb[(p + 1) & 1] : nullptr, tctr + p, lo ? nullptr : s.keys_hi));
+    }
+    const uint32_t h = oi_reduce_height(M, d.L);
+    HIPCHK(c, oi_launch_reduce(st, M, nullptr, nullptr, &d, oi_lvl_cnt(d.L - h, d.L, d.vl), d.perm, 0));
+    if (d.L > h) HIPCHK(c, oi_launch_reduce_apex(st, M, nullptr, nullptr, &d, 1, nullptr));
+    return OIBVH_OK;
+}
+
+int oi_build_lpt(const oibvh_ctx* c, uint32_t F)
+{
+    return c->leaves_per_thread ? (int)c->leaves_per_thread : oi_default_lpt(F, c->cus);
+}
+
 extern "C" oibvh_status oibvh_tree_build(oibvh_tree* t)
 {
     if (!t) return OIBVH_ERR_INVALID;
@@ -444,8 +489,7 @@ extern "C" oibvh_status oibvh_tree_build(oibvh_tree* t)
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, oi_ctx_join(c));
     const uint32_t F = t->F;
-    const uint32_t tile = oi_sort_tile();
-    const uint32_t tiles = (F + tile - 1) / tile;
+    const uint32_t tiles = (F + oi_sort_tile() - 1) / oi_sort_tile();
     const bool m63 = t->morton_bits == 63;
     if (!t->keys) {
         HIPCHK(c, dalloc(&t->keys, F));
@@ -455,29 +499,8 @@ extern "C" oibvh_status oibvh_tree_build(oibvh_tree* t)
         HIPCHK(c, dalloc(&t->sort_scratch, OI_SORT_HEAD + 512ull * tiles));
     }
     if (m63 && !t->keys_hi) HIPCHK(c, dalloc(&t->keys_hi, F));
-    uint32_t* ghist = t->sort_scratch;
-    uint32_t* tctr = t->sort_scratch + OI_SORT_HIST;
-    uint32_t* lb[2] = {t->sort_scratch + OI_SORT_HEAD, t->sort_scratch + OI_SORT_HEAD + 256ull * tiles};
-    hipStream_t s = c->stream;
-    const oi_build_desc d = tree_desc(t);
-    HIPCHK(c, oi_launch_mesh_box(s, nullptr, nullptr, &d, d.nparts, t->box + 6, t->sort_scratch,
-                                 OI_SORT_HEAD + 256u * tiles));
-    const uint32_t mblocks = (F + oi_morton_chunk() - 1) / oi_morton_chunk();
-    HIPCHK(c, oi_launch_morton(s, nullptr, nullptr, &d, mblocks, t->box + 6, t->keys, m63 ? t->keys_hi : nullptr,
-                               ghist, nullptr));
-    // LSD passes over the low word (keys move with the values), then, for
-    // 63-bit keys, over the high word looked up by face id; values end in perm
-    const int np = m63 ? 8 : 4;
-    for (int p = 0; p < np; p++) {
-        const bool lo = p < 4;
-        const uint32_t* kin = lo ? ((p & 1) ? t->keys_alt : t->keys) : nullptr;
-        uint32_t* kout = (lo && p < 3) ? ((p & 1) ? t->keys : t->keys_alt) : nullptr;
-        const uint32_t* vin = p == 0 ? nullptr : ((p & 1) ? t->vals_tmp : t->perm);
-        uint32_t* vout = (p & 1) ? t->perm : t->vals_tmp;
-        HIPCHK(c, oi_launch_sort_pass(s, kin, vin, kout, vout, F, 8u * (p & 3), ghist + 256 * p, lb[p & 1],
-                                      p + 1 < np ? lb[(p + 1) & 1] : nullptr, tctr + p, lo ? nullptr : t->keys_hi));
-    }
-    oibvh_status st = launch_reduce(t, true);
+    const oi_build_scratch s{t->keys, t->keys_alt, m63 ? t->keys_hi : nullptr, t->vals_tmp, t->sort_scratch};
+    oibvh_status st = oi_enqueue_build(c, tree_desc(t), s, pick_lpt(t));
     if (st != OIBVH_OK) return st;
     t->built = true;
     return OIBVH_OK;

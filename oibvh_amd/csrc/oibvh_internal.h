@@ -98,6 +98,10 @@ struct oibvh_ctx {
 #define OI_FRAME_DEPTH 2u
 static_assert(OI_SIDE_RING >= OI_FRAME_DEPTH, "the ring must still hold the event a refit waits for");
 
+// device address of oibvh_build_noalloc's scratch array on `device`, taken
+// once per device with that device current (oibvh_noalloc.hip)
+hipError_t oi_noalloc_attach(int device);
+
 // drop one reference held by a tree, scene or batch (frees a context whose
 // destroy was deferred)
 void oi_ctx_release(oibvh_ctx* ctx);
@@ -110,6 +114,26 @@ hipError_t oi_ctx_join(oibvh_ctx* ctx);
 hipError_t oi_ctx_quiesce(oibvh_ctx* ctx);
 
 oibvh_status oi_set_err(oibvh_ctx* ctx, oibvh_status st, const char* fmt, ...);
+
+// device scratch of one build: sort keys (two buffers, plus the high words
+// of 63-bit keys or null), the values' ping-pong buffer and the sort's
+// histograms, tile counters and look-back tiles (OI_SORT_HEAD + 512 words
+// per sort tile)
+struct oi_build_scratch {
+    uint32_t* keys;
+    uint32_t* keys_alt;
+    uint32_t* keys_hi;
+    uint32_t* vals_tmp;
+    uint32_t* sort;
+};
+// the build's launches over mesh d (d.box holds 6 + 6 * oi_box_parts(V)
+// floats) with M leaves per thread in the reduce; no allocation
+oibvh_status oi_enqueue_build(oibvh_ctx* c, const oi_build_desc& d, const oi_build_scratch& s, int M);
+// mesh-box partials (workgroups of k_mesh_box) for V vertices, 1..256
+uint32_t oi_box_parts(uint32_t V);
+// reduce variant by launch size, and the one a build of F faces uses
+int oi_default_lpt(uint64_t F, int cus);
+int oi_build_lpt(const oibvh_ctx* c, uint32_t F);
 
 #define HIPCHK(ctx, expr)                                                                           \
     do {                                                                                            \
