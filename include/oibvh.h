@@ -52,8 +52,8 @@ typedef struct oibvh_batch oibvh_batch;
 #define OIBVH_DETECT_ASYNC 4u         /* enqueue only: no host wait. Results are checked at the next
                                          get/stats call or synchronous detect; a further ASYNC detect
                                          replaces an unchecked one, whose errors go to async_errors */
-#define OIBVH_FRAME_OVERLAP 8u        /* oibvh_scene_frame only: this frame's refit may run while the
-                                         previous frame's detect still runs (see oibvh_scene_frame) */
+#define OIBVH_FRAME_OVERLAP 8u        /* oibvh_scene_frame and oibvh_batch_frame: this frame's refit may
+                                         run while the previous frame's detect still runs (see there) */
 
 /* ---------------------------------------------------------------- context */
 
@@ -258,6 +258,17 @@ typedef struct {
 OIBVH_API oibvh_status oibvh_ctx_set_tuning(oibvh_ctx* ctx, const oibvh_tuning* tuning);
 OIBVH_API oibvh_status oibvh_ctx_get_tuning(const oibvh_ctx* ctx, oibvh_tuning* tuning);
 
+/* Mesh::transform (mesh.cpp:189-214) for every tree of the scene at once:
+ * mats holds 16 floats per tree (column-major like glm::mat4) in
+ * oibvh_scene_add_tree order, and each tree then behaves as after
+ * oibvh_tree_set_transform with its matrix. OIBVH_PTR_HOST: staged in stream
+ * order before return (one small launch per tree); OIBVH_PTR_DEVICE: one
+ * copy, in stream order, into a table the scene owns; OIBVH_PTR_DEVICE_BORROW:
+ * read in place by every later build, refit and detect of these trees (no
+ * device work; the caller keeps the memory unchanged until that work is
+ * done). A tree in several scenes keeps the matrix of the last call. */
+OIBVH_API oibvh_status oibvh_scene_set_transforms(oibvh_scene* scene, const float* mats, uint32_t ptr_kind);
+
 /* Refit every tree of the scene (OibvhTree::refit on each, oibvhTree.cu:193-235)
  * with one launch per stage for up to four trees instead of one per tree.
  * Results are those of oibvh_tree_refit on every tree with its current
@@ -337,7 +348,8 @@ OIBVH_API oibvh_status oibvh_batch_mesh_count(const oibvh_batch* batch, uint32_t
 /* Model matrices of all meshes: mesh_count x 16 floats, column-major like
  * glm::mat4, applied on the fly as oibvh_tree_set_transform does. Host memory
  * is read before return; device memory (OIBVH_PTR_DEVICE) is copied in stream
- * order, so a caller can keep every frame's matrices in HBM. */
+ * order (OIBVH_PTR_DEVICE_BORROW too), so a caller can keep every frame's
+ * matrices in HBM. */
 OIBVH_API oibvh_status oibvh_batch_set_transforms(oibvh_batch* batch, const float* mats, uint32_t ptr_kind);
 
 /* Morton key width (30 or 63) of the next build of every mesh, as
@@ -368,9 +380,28 @@ OIBVH_API oibvh_status oibvh_batch_detect(oibvh_batch* batch, uint32_t entry_lev
                                           uint32_t flags);
 /* One frame, refit + detect, replayed from a hipGraph captured at the first
  * call and again whenever the parameters, the tuning or a device buffer
- * change. With OIBVH_DETECT_ASYNC it returns after the launch. */
+ * change. With OIBVH_DETECT_ASYNC it returns after the launch.
+ *
+ * With OIBVH_FRAME_OVERLAP, frames whose inputs are known ahead (prescribed
+ * motion) overlap: the batch keeps a second node buffer, matrix table and
+ * descriptor tables; frame f refits into one (a graph on the context's
+ * stream) while frame f - 1's detect reads the other (a graph on the side
+ * stream). After an overlapped frame, oibvh_batch_set_transforms and
+ * oibvh_batch_set_positions with OIBVH_PTR_DEVICE_BORROW stage the next
+ * overlapped frame's inputs without waiting for this frame's detect;
+ * set_positions with a copy, and every other call, first wait for the queued
+ * detects. Results and errors equal those of the same frames without the
+ * flag; the refit of such a frame is capped at tuning.refit_blocks
+ * workgroups (0: uncapped). */
 OIBVH_API oibvh_status oibvh_batch_frame(oibvh_batch* batch, uint32_t entry_level, uint32_t expand_levels,
                                          uint32_t flags);
+
+/* HIP-event timing as oibvh_scene_set_timing / oibvh_scene_get_timing: every
+ * n-th refit (oibvh_batch_refit or a frame's refit, which then runs without
+ * its graph) has its stage-1 launch record an event pair as it starts and
+ * completes; every n-th detect records a pair around its launches. */
+OIBVH_API oibvh_status oibvh_batch_set_timing(oibvh_batch* batch, uint32_t every);
+OIBVH_API oibvh_status oibvh_batch_get_timing(oibvh_batch* batch, oibvh_scene_timing* out);
 
 OIBVH_API oibvh_status oibvh_batch_get_pair_count(oibvh_batch* batch, uint32_t* count);
 /* Sorted (meshA, meshB, origTriA, origTriB) with batch mesh indices,

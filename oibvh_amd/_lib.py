@@ -92,6 +92,7 @@ API = [
     ("oibvh_scene_get_stats", _i, [_vp, C.POINTER(DetectStats)]),
     ("oibvh_scene_get_level_counts", _i, [_vp, _u32p, _u64, _u32p]),
     ("oibvh_scene_refit", _i, [_vp]),
+    ("oibvh_scene_set_transforms", _i, [_vp, _vp, _u32]),
     ("oibvh_batch_create", _i, [_vp, C.POINTER(_vp)]),
     ("oibvh_batch_destroy", None, [_vp]),
     ("oibvh_batch_add_mesh", _i, [_vp, _u32, _vp, _u32, _vp, _u32, _u32p]),
@@ -113,6 +114,8 @@ API = [
     ("oibvh_batch_get_level_counts", _i, [_vp, _u32p, _u64, _u32p]),
     ("oibvh_batch_get_nodes", _i, [_vp, _u32, _f32p, _u64, _u32p]),
     ("oibvh_batch_get_perm", _i, [_vp, _u32, _u32p, _u64]),
+    ("oibvh_batch_set_timing", _i, [_vp, _u32]),
+    ("oibvh_batch_get_timing", _i, [_vp, C.c_void_p]),
     ("oibvh_build_noalloc", _i, [_vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp]),
 ]
 

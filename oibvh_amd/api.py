@@ -286,6 +286,19 @@ class Scene:
         """Refit every tree of the scene in one launch per stage."""
         _check(self.ctx.h, self._lib.oibvh_scene_refit(self.h), "scene refit")
 
+    def set_transforms(self, mats=None, *, device: int | None = None, borrow: bool = False) -> None:
+        """Model matrices of every tree (addOibvhTree order): `mats` is an
+        (n, 4, 4) array in the usual row/column sense (host memory), or
+        `device` the address of n column-major matrices in device memory,
+        copied (borrow=False) or read in place (borrow=True)."""
+        if device is not None:
+            kind = L.PTR_DEVICE_BORROW if borrow else L.PTR_DEVICE
+            _check(self.ctx.h, self._lib.oibvh_scene_set_transforms(self.h, C.c_void_p(device), kind),
+                   "set_transforms")
+            return
+        m = np.ascontiguousarray(np.swapaxes(np.asarray(mats, np.float32).reshape(-1, 4, 4), -1, -2))
+        _check(self.ctx.h, self._lib.oibvh_scene_set_transforms(self.h, _ptr(m), L.PTR_HOST), "set_transforms")
+
     def getIntTriPairCount(self) -> int:
         v = C.c_uint32()
         _check(self.ctx.h, self._lib.oibvh_scene_get_pair_count(self.h, C.byref(v)), "getIntTriPairCount")
@@ -446,11 +459,22 @@ class Batch:
                "batch detect")
 
     def frame(self, entry_level: int = 0, expand_levels: int = 1, *, self_collide: bool = False,
-              fp64_fallback: bool = False, asynchronous: bool = False) -> None:
-        """refit + detect replayed from a captured hipGraph."""
-        _check(self.ctx.h, self._lib.oibvh_batch_frame(self.h, entry_level, expand_levels,
-                                                       _flags(self_collide, fp64_fallback, asynchronous)),
-               "batch frame")
+              fp64_fallback: bool = False, asynchronous: bool = False, overlap: bool = False) -> None:
+        """refit + detect replayed from a captured hipGraph. overlap: this
+        frame's refit may run while the previous frame's detect does
+        (OIBVH_FRAME_OVERLAP)."""
+        flags = _flags(self_collide, fp64_fallback, asynchronous) | (L.FRAME_OVERLAP if overlap else 0)
+        _check(self.ctx.h, self._lib.oibvh_batch_frame(self.h, entry_level, expand_levels, flags), "batch frame")
+
+    def set_timing(self, every: int = 1) -> None:
+        """Time every `every`-th refit (its stage-1 launch) and detect with
+        HIP events (0 stops)."""
+        _check(self.ctx.h, self._lib.oibvh_batch_set_timing(self.h, int(every)), "batch set_timing")
+
+    def timing(self) -> dict:
+        t = L.SceneTiming()
+        _check(self.ctx.h, self._lib.oibvh_batch_get_timing(self.h, C.byref(t)), "batch get_timing")
+        return {f: getattr(t, f) for f, _ in L.SceneTiming._fields_}
 
     def pair_count(self) -> int:
         v = C.c_uint32()

@@ -100,14 +100,16 @@ def tune(ctx: api.Context, scene: api.Scene, frame, *, entries=(4, 6, 8), expand
 
 
 def tune_batch(ctx: api.Context, batch: api.Batch, before_frame, *, entries=(0, 2, 4), expands=(1, 2, 3),
-               blocks=(0, 4096), leaves=(0, 4, 8), local=((64, 2), (256, 3)), tail=(), flags=None,
-               reps: int = 3) -> dict:
+               blocks=(0, 4096), leaves=(0, 4, 8), local=((64, 2), (256, 3)), tail=(), refit_blocks=(0,),
+               flags=None, reps: int = 3) -> dict:
     """Pick the reduce subtree size (leaves per thread), the BVTT entry level,
     the levels per step and the workgroups per launch for a batch's captured
     frame (refit + detect). before_frame(i) stages frame i's inputs (model
     matrices or vertex pointers) in stream order. `tail` (none by default)
     lists tail-walk settings to try: a batch's late fronts are large, and a
-    tail walk that outgrows LDS on a later frame costs that frame a re-run."""
+    tail walk that outgrows LDS on a later frame costs that frame a re-run.
+    `refit_blocks` lists caps on an overlapped frame's refit workgroups (with
+    flags {"overlap": True}; none changes results)."""
     flags = flags or {}
     base = ctx.get_tuning()
     timer = api.Timer(ctx)
@@ -153,11 +155,19 @@ def tune_batch(ctx: api.Context, batch: api.Batch, before_frame, *, entries=(0, 
         ms = run(best["entry_level"], best["expand_levels"])
         if ms < best["ms"]:
             best = {**best, "traverse_tail": chunk, "traverse_tail_steps": steps, "ms": ms}
+    ctx.set_tuning(traverse_tail=best["traverse_tail"], traverse_tail_steps=best["traverse_tail_steps"])
+    best["refit_blocks"] = base.get("refit_blocks", 0)
+    for rb in refit_blocks:
+        ctx.set_tuning(refit_blocks=rb)
+        ms = run(best["entry_level"], best["expand_levels"])
+        if ms < best["ms"]:
+            best = {**best, "refit_blocks": rb, "ms": ms}
     ctx.set_tuning(**{**base, "reduce_leaves_per_thread": best["reduce_leaves_per_thread"],
                       "traverse_blocks": best["traverse_blocks"], "traverse_expand": best["expand_levels"],
                       "traverse_local": best["traverse_local"],
                       "traverse_local_levels": best["traverse_local_levels"],
-                      "traverse_tail": best["traverse_tail"], "traverse_tail_steps": best["traverse_tail_steps"]})
+                      "traverse_tail": best["traverse_tail"], "traverse_tail_steps": best["traverse_tail_steps"],
+                      "refit_blocks": best["refit_blocks"]})
     return best
 
 

@@ -12,6 +12,12 @@
 // itself); meshes of different scenes never meet: the initial front only
 // pairs meshes of one scene. A frame (refit + detect) can be replayed from a
 // captured hipGraph.
+//
+// Overlapped frames (OIBVH_FRAME_OVERLAP) keep two slots of per-frame state:
+// node buffer, model-matrix table and descriptor tables. Frame f refits into
+// one slot on the context's stream while frame f - 1's detect reads the other
+// on the side stream; staging calls after an overlapped frame write the slot
+// the next one will refit, once the detect that last read it has finished.
 #include <new>
 
 #include "oibvh_internal.h"
@@ -39,11 +45,25 @@ struct oibvh_batch {
     uint32_t* faces = nullptr;
     uint32_t* sfaces = nullptr;
     uint32_t* perm = nullptr;
-    float* nodes = nullptr;
     float* boxes = nullptr;
     float* partials = nullptr;
-    float* xf = nullptr;
     bool has_xf = false;
+    // per-frame state: slot[cur] is the one the last build or refit wrote
+    // (and the one every non-overlapped call uses); the other exists once
+    // an overlapped frame has run
+    struct Slot {
+        float* nodes = nullptr;
+        float* xf = nullptr;                  // model matrices (16 floats per mesh), if has_xf
+        oi_build_desc* d_bdesc = nullptr;     // build/refit table
+        oi_mesh_desc* d_mdesc = nullptr;      // traversal table
+        std::vector<oi_build_desc> h_bdesc;   // host copies (positions track k_set_mesh_pos updates)
+        std::vector<oi_mesh_desc> h_mdesc;
+        uint64_t read_seq = 0;                // 1 + the side-stream detect that last read this slot (0: none)
+    };
+    Slot slot[2];
+    uint32_t cur = 0;
+    bool ov_active = false;     // the last frame was overlapped: staging goes to the other slot
+    bool staged_other = false;  // the other slot has been opened for the next overlapped frame
     bool borrowed = false;  // some mesh reads caller-owned vertex memory
     uint32_t *keys = nullptr, *keys_alt = nullptr, *vals = nullptr, *vals_tmp = nullptr, *fmesh = nullptr;
     uint32_t* keys_hi = nullptr;       // high words of 63-bit keys
@@ -51,8 +71,6 @@ struct oibvh_batch {
     uint32_t* sort_scratch = nullptr;  // histograms | tile counters | lookback0 | lookback1
     uint32_t* mesh_hist = nullptr;     // digit histograms of the mesh ids, 256 per pass
     uint32_t tiles = 0, mesh_passes = 0;
-    oi_build_desc* d_bdesc = nullptr;
-    std::vector<oi_build_desc> h_bdesc;
     uint2* w_box = nullptr;
     uint32_t n_box = 0;
     uint2* w_morton = nullptr;
@@ -63,17 +81,26 @@ struct oibvh_batch {
     uint32_t n_apex = 0;
     int reduce_M = 0;  // M the reduce tables were made for
     // traversal
-    oi_mesh_desc* d_mdesc = nullptr;
-    std::vector<oi_mesh_desc> h_mdesc;
     uint32_t* chunk_mesh = nullptr;
     std::vector<uint2> h_front;
     uint32_t front_key[2] = {~0u, ~0u};  // entry, self
     oi_detector det;
-    // captured frame
-    hipGraphExec_t gexec = nullptr;
-    uint64_t graph_key[6] = {0, 0, 0, 0, 0, 0};
+    // captured graphs: the whole frame, and an overlapped frame's refit and
+    // detect halves, per slot
+    struct Graph {
+        hipGraphExec_t exec = nullptr;
+        uint64_t key[8] = {};
+    };
+    Graph g_frame[2], g_refit[2], g_detect[2];
     uint32_t desc_gen = 0;
+    // HIP-event timing of the refit's stage-1 launch and of detects
+    uint32_t timing = 0;
+    uint64_t seq_refit = 0, seq_detect = 0;
+    std::vector<hipEvent_t> ev_refit, ev_detect;
+    uint32_t n_refit = 0, n_detect = 0;
 };
+
+static inline oibvh_batch::Slot& cur_slot(oibvh_batch* b) { return b->slot[b->cur]; }
 
 static oibvh_status batch_err(oibvh_batch* b, oibvh_status st, const char* what)
 {
@@ -105,10 +132,14 @@ static void batch_free_device(oibvh_batch* b)
     dfree(b->faces);
     dfree(b->sfaces);
     dfree(b->perm);
-    dfree(b->nodes);
+    for (oibvh_batch::Slot& sl : b->slot) {
+        dfree(sl.nodes);
+        dfree(sl.xf);
+        dfree(sl.d_bdesc);
+        dfree(sl.d_mdesc);
+    }
     dfree(b->boxes);
     dfree(b->partials);
-    dfree(b->xf);
     dfree(b->keys);
     dfree(b->keys_alt);
     dfree(b->keys_hi);
@@ -117,12 +148,10 @@ static void batch_free_device(oibvh_batch* b)
     dfree(b->fmesh);
     dfree(b->sort_scratch);
     dfree(b->mesh_hist);
-    dfree(b->d_bdesc);
     dfree(b->w_box);
     dfree(b->w_morton);
     dfree(b->w_reduce);
     dfree(b->w_apex);
-    dfree(b->d_mdesc);
     dfree(b->chunk_mesh);
 }
 
@@ -131,8 +160,12 @@ extern "C" void oibvh_batch_destroy(oibvh_batch* b)
     if (!b) return;
     oibvh_ctx* c = b->ctx;
     (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
-    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    (void)oi_ctx_quiesce(c);
+    for (oibvh_batch::Graph* gs : {b->g_frame, b->g_refit, b->g_detect})
+        for (int k = 0; k < 2; k++)
+            if (gs[k].exec) (void)hipGraphExecDestroy(gs[k].exec);
+    for (hipEvent_t e : b->ev_refit) (void)hipEventDestroy(e);
+    for (hipEvent_t e : b->ev_detect) (void)hipEventDestroy(e);
     batch_free_device(b);
     oi_det_free(&b->det);
     delete b;
@@ -202,21 +235,23 @@ static oibvh_status upload(oibvh_ctx* c, T*& dst, const std::vector<T>& src)
     return OIBVH_OK;
 }
 
-static void fill_descs(oibvh_batch* b)
+// the descriptor tables of slot k (positions: the batch's own buffers)
+static void fill_descs(oibvh_batch* b, uint32_t k)
 {
+    oibvh_batch::Slot& sl = b->slot[k];
     const uint32_t n = (uint32_t)b->meshes.size();
-    b->h_bdesc.assign(n, oi_build_desc{});
-    b->h_mdesc.assign(n, oi_mesh_desc{});
+    sl.h_bdesc.assign(n, oi_build_desc{});
+    sl.h_mdesc.assign(n, oi_mesh_desc{});
     for (uint32_t i = 0; i < n; i++) {
         const oibvh_batch::Mesh& m = b->meshes[i];
         const oibvh_batch::Mesh& s = b->meshes[m.src];
-        oi_build_desc& d = b->h_bdesc[i];
+        oi_build_desc& d = sl.h_bdesc[i];
         d.pos = b->pos + 3ull * s.vbase;
-        d.xf = b->has_xf ? b->xf + 16ull * i : nullptr;
+        d.xf = b->has_xf ? sl.xf + 16ull * i : nullptr;
         d.faces = b->faces + 3ull * s.fbase;
         d.sfaces = b->sfaces + 3ull * m.fbase;
         d.perm = b->perm + m.fbase;
-        d.nodes = b->nodes + 6ull * m.nbase;
+        d.nodes = sl.nodes + 6ull * m.nbase;
         d.box = b->boxes + 6ull * i;
         d.F = m.F;
         d.V = m.V;
@@ -226,7 +261,7 @@ static void fill_descs(oibvh_batch* b)
         d.pbase = m.pbase;
         d.nparts = m.nparts;
         d.vchunk = m.vchunk;
-        oi_mesh_desc& t = b->h_mdesc[i];
+        oi_mesh_desc& t = sl.h_mdesc[i];
         t.nodes = d.nodes;
         t.sfaces = d.sfaces;
         t.pos = d.pos;
@@ -240,22 +275,26 @@ static void fill_descs(oibvh_batch* b)
     }
 }
 
+// refill and upload the tables of every allocated slot, keeping each slot's
+// vertex pointers (borrowed ones included)
 static oibvh_status upload_descs(oibvh_batch* b)
 {
     oibvh_ctx* c = b->ctx;
-    // keep borrowed vertex pointers across a refill
-    std::vector<const float*> keep;
-    for (const oi_build_desc& d : b->h_bdesc) keep.push_back(d.pos);
-    fill_descs(b);
-    if (b->borrowed)
+    for (uint32_t k = 0; k < 2; k++) {
+        oibvh_batch::Slot& sl = b->slot[k];
+        if (!sl.d_bdesc) continue;
+        std::vector<const float*> keep;
+        for (const oi_build_desc& d : sl.h_bdesc) keep.push_back(d.pos);
+        fill_descs(b, k);
         for (size_t i = 0; i < keep.size(); i++) {
-            b->h_bdesc[i].pos = keep[i];
-            b->h_mdesc[i].pos = keep[i];
+            sl.h_bdesc[i].pos = keep[i];
+            sl.h_mdesc[i].pos = keep[i];
         }
-    HIPCHK(c, hipMemcpy(b->d_bdesc, b->h_bdesc.data(), b->h_bdesc.size() * sizeof(oi_build_desc),
-                        hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(b->d_mdesc, b->h_mdesc.data(), b->h_mdesc.size() * sizeof(oi_mesh_desc),
-                        hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(sl.d_bdesc, sl.h_bdesc.data(), sl.h_bdesc.size() * sizeof(oi_build_desc),
+                            hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(sl.d_mdesc, sl.h_mdesc.data(), sl.h_mdesc.size() * sizeof(oi_mesh_desc),
+                            hipMemcpyHostToDevice));
+    }
     b->desc_gen++;
     return OIBVH_OK;
 }
@@ -340,7 +379,7 @@ static oibvh_status commit(oibvh_batch* b)
     }
     HIPCHK(c, dalloc(&b->sfaces, 3 * F));
     HIPCHK(c, dalloc(&b->perm, F));
-    HIPCHK(c, dalloc(&b->nodes, 6 * N));
+    HIPCHK(c, dalloc(&b->slot[0].nodes, 6 * N));
     HIPCHK(c, dalloc(&b->boxes, 6ull * n));
     HIPCHK(c, dalloc(&b->partials, 6 * P));
     // sort
@@ -383,10 +422,109 @@ static oibvh_status commit(oibvh_batch* b)
         }
         if ((r = upload(c, b->chunk_mesh, cm)) != OIBVH_OK) return r;
     }
-    HIPCHK(c, dalloc(&b->d_bdesc, n));
-    HIPCHK(c, dalloc(&b->d_mdesc, n));
+    HIPCHK(c, dalloc(&b->slot[0].d_bdesc, n));
+    HIPCHK(c, dalloc(&b->slot[0].d_mdesc, n));
+    fill_descs(b, 0);
     if ((r = upload_descs(b)) != OIBVH_OK) return r;
     b->committed = true;
+    return OIBVH_OK;
+}
+
+// ---------------------------------------------------------------- slots
+// Make slot k's buffers and tables exist (the second slot is allocated by
+// the first overlapped frame).
+static oibvh_status ensure_slot(oibvh_batch* b, uint32_t k)
+{
+    oibvh_ctx* c = b->ctx;
+    oibvh_batch::Slot& sl = b->slot[k];
+    const uint32_t n = (uint32_t)b->meshes.size();
+    if (sl.d_bdesc && (!b->has_xf || sl.xf)) return OIBVH_OK;
+    // the tables are uploaded again below with synchronous copies, which are
+    // not ordered with the library's streams: let queued work finish first
+    HIPCHK(c, oi_ctx_quiesce(c));
+    if (!sl.nodes) HIPCHK(c, dalloc(&sl.nodes, 6 * b->Nt));
+    if (b->has_xf && !sl.xf) HIPCHK(c, dalloc(&sl.xf, 16ull * n));
+    if (!sl.d_bdesc) {
+        HIPCHK(c, dalloc(&sl.d_bdesc, n));
+        HIPCHK(c, dalloc(&sl.d_mdesc, n));
+        // a new slot starts with the current slot's vertex pointers
+        fill_descs(b, k);
+        const oibvh_batch::Slot& cs = b->slot[b->cur];
+        for (uint32_t i = 0; i < n && i < cs.h_bdesc.size(); i++) {
+            sl.h_bdesc[i].pos = cs.h_bdesc[i].pos;
+            sl.h_mdesc[i].pos = cs.h_mdesc[i].pos;
+        }
+    }
+    // upload_descs refills every allocated slot (xf pointers may have appeared)
+    return upload_descs(b);
+}
+
+// main stream waits for the side-stream detect that last read slot k
+static oibvh_status wait_slot_readers(oibvh_batch* b, uint32_t k)
+{
+    oibvh_ctx* c = b->ctx;
+    const uint64_t q = b->slot[k].read_seq;
+    // the ring event of detect q - 1 is recorded again OI_SIDE_RING detects
+    // later; a wait then covers a later point, which is still safe
+    if (q && c->side) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_ring[(q - 1) % OI_SIDE_RING], 0));
+    return OIBVH_OK;
+}
+
+// copy slot `from`'s per-frame inputs (matrices, vertex pointers) to slot
+// `to`, in stream order
+static oibvh_status copy_slot_inputs(oibvh_batch* b, uint32_t from, uint32_t to)
+{
+    oibvh_ctx* c = b->ctx;
+    oibvh_batch::Slot& f = b->slot[from];
+    oibvh_batch::Slot& t = b->slot[to];
+    if (b->has_xf)
+        HIPCHK(c, hipMemcpyAsync(t.xf, f.xf, 64ull * b->meshes.size(), hipMemcpyDeviceToDevice, c->stream));
+    for (uint32_t i = 0; i < b->meshes.size(); i++)
+        if (t.h_bdesc[i].pos != f.h_bdesc[i].pos) {
+            HIPCHK(c, oi_launch_set_mesh_pos(c->stream, t.d_bdesc, t.d_mdesc, i, f.h_bdesc[i].pos));
+            t.h_bdesc[i].pos = f.h_bdesc[i].pos;
+            t.h_mdesc[i].pos = f.h_bdesc[i].pos;
+        }
+    return OIBVH_OK;
+}
+
+// The slot staging calls write. After an overlapped frame it is the other
+// slot, opened once per frame: wait for its last reader, then carry the
+// current slot's inputs over, so that meshes not staged again keep theirs.
+static oibvh_status stage_slot(oibvh_batch* b, uint32_t* k)
+{
+    if (!b->ov_active) {
+        *k = b->cur;
+        return OIBVH_OK;
+    }
+    const uint32_t o = b->cur ^ 1u;
+    if (!b->staged_other) {
+        oibvh_status r = ensure_slot(b, o);
+        if (r == OIBVH_OK) r = wait_slot_readers(b, o);
+        if (r == OIBVH_OK) r = copy_slot_inputs(b, b->cur, o);
+        if (r != OIBVH_OK) return r;
+        b->staged_other = true;
+    }
+    *k = o;
+    return OIBVH_OK;
+}
+
+// Before a call that is not an overlapped frame: order the context's stream
+// after the queued side-stream detects and leave overlapped mode. A call
+// that computes (build, refit, detect, frame) first takes over inputs staged
+// for the next overlapped frame into the current slot.
+static oibvh_status settle(oibvh_batch* b, bool compute)
+{
+    oibvh_ctx* c = b->ctx;
+    if (b->ov_active) {
+        HIPCHK(c, oi_ctx_join(c));
+        b->ov_active = false;
+    }
+    if (compute && b->staged_other) {
+        oibvh_status r = copy_slot_inputs(b, b->cur ^ 1u, b->cur);
+        if (r != OIBVH_OK) return r;
+        b->staged_other = false;
+    }
     return OIBVH_OK;
 }
 
@@ -398,18 +536,27 @@ extern "C" oibvh_status oibvh_batch_set_transforms(oibvh_batch* b, const float* 
     oibvh_status r = commit(b);
     if (r != OIBVH_OK) return r;
     const size_t bytes = 64ull * b->meshes.size();
-    if (!b->xf) {
-        HIPCHK(c, dalloc(&b->xf, 16ull * b->meshes.size()));
+    if (!b->has_xf) {
+        // the first matrices: every allocated slot gets a table
+        HIPCHK(c, oi_ctx_quiesce(c));
         b->has_xf = true;
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (uint32_t k = 0; k < 2; k++) {
+            oibvh_batch::Slot& sl = b->slot[k];
+            if (!sl.d_bdesc) continue;
+            HIPCHK(c, dalloc(&sl.xf, 16ull * b->meshes.size()));
+        }
         if ((r = upload_descs(b)) != OIBVH_OK) return r;
     }
+    uint32_t k = 0;
+    if ((r = stage_slot(b, &k)) != OIBVH_OK) return r;
+    float* dst = b->slot[k].xf;
     if (ptr_kind == OIBVH_PTR_HOST) {
-        // host memory: copied before return
+        // host memory: copied before return, after the work queued before
+        // (the stream has waited for the slot's last reader)
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        HIPCHK(c, hipMemcpy(b->xf, mats, bytes, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(dst, mats, bytes, hipMemcpyHostToDevice));
     } else {
-        HIPCHK(c, hipMemcpyAsync(b->xf, mats, bytes, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(dst, mats, bytes, hipMemcpyDeviceToDevice, c->stream));
     }
     return OIBVH_OK;
 }
@@ -433,19 +580,26 @@ extern "C" oibvh_status oibvh_batch_set_positions(oibvh_batch* b, uint32_t mesh,
     if (r != OIBVH_OK) return r;
     const oibvh_batch::Mesh& m = b->meshes[mesh];
     if (ptr_kind == OIBVH_PTR_DEVICE_BORROW) {
-        // the kernels read the descriptor tables at run time: update them in
-        // stream order (the host copies follow for later uploads)
-        HIPCHK(c, oi_launch_set_mesh_pos(c->stream, b->d_bdesc, b->d_mdesc, mesh, xyz));
-        b->h_bdesc[mesh].pos = xyz;
-        b->h_mdesc[mesh].pos = xyz;
+        // the kernels read the descriptor tables at run time: update the
+        // staged slot's in stream order (the host copies follow)
+        uint32_t k = 0;
+        if ((r = stage_slot(b, &k)) != OIBVH_OK) return r;
+        oibvh_batch::Slot& sl = b->slot[k];
+        HIPCHK(c, oi_launch_set_mesh_pos(c->stream, sl.d_bdesc, sl.d_mdesc, mesh, xyz));
+        sl.h_bdesc[mesh].pos = xyz;
+        sl.h_mdesc[mesh].pos = xyz;
         b->borrowed = true;
         return OIBVH_OK;
     }
+    // a copy into the batch's own vertex buffer, which both slots read: the
+    // queued overlapped detects finish first
+    if ((r = settle(b, true)) != OIBVH_OK) return r;
+    oibvh_batch::Slot& sl = cur_slot(b);
     float* own = b->pos + 3ull * b->meshes[m.src].vbase;
-    if (b->h_bdesc[mesh].pos != own) {
-        HIPCHK(c, oi_launch_set_mesh_pos(c->stream, b->d_bdesc, b->d_mdesc, mesh, own));
-        b->h_bdesc[mesh].pos = own;
-        b->h_mdesc[mesh].pos = own;
+    if (sl.h_bdesc[mesh].pos != own) {
+        HIPCHK(c, oi_launch_set_mesh_pos(c->stream, sl.d_bdesc, sl.d_mdesc, mesh, own));
+        sl.h_bdesc[mesh].pos = own;
+        sl.h_mdesc[mesh].pos = own;
     }
     if (ptr_kind == OIBVH_PTR_HOST) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -456,18 +610,55 @@ extern "C" oibvh_status oibvh_batch_set_positions(oibvh_batch* b, uint32_t mesh,
     return OIBVH_OK;
 }
 
-// refit launches; with seed, the apex launch (if any) also resets the
-// counters of the detect that follows and *seeded says so
-static oibvh_status enqueue_refit(oibvh_batch* b, const oi_seed_args* seed = nullptr, bool* seeded = nullptr)
+// the next event pair of a timing list (created on first use)
+static oibvh_status timing_pair(oibvh_ctx* c, std::vector<hipEvent_t>& ev, uint32_t& n, hipEvent_t* e0, hipEvent_t* e1)
+{
+    if (2ull * n + 2 > ev.size()) {
+        for (int k = 0; k < 2; k++) {
+            hipEvent_t e = nullptr;
+            HIPCHK(c, hipEventCreate(&e));
+            ev.push_back(e);
+        }
+    }
+    *e0 = ev[2 * n];
+    *e1 = ev[2 * n + 1];
+    n++;
+    return OIBVH_OK;
+}
+
+// refit launches over the current slot; with seed, the apex launch (if any)
+// also resets the counters of the detect that follows and *seeded says so;
+// t0/t1 (both or neither) are recorded by the stage-1 launch itself;
+// max_blocks caps the stage-1 launch (k_reduce_wl)
+static oibvh_status enqueue_refit(oibvh_batch* b, const oi_seed_args* seed = nullptr, bool* seeded = nullptr,
+                                  hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr, uint32_t max_blocks = 0)
 {
     oibvh_ctx* c = b->ctx;
     const int M = b->reduce_M;
+    const oibvh_batch::Slot& sl = cur_slot(b);
     if (seeded) *seeded = false;
-    HIPCHK(c, oi_launch_reduce(c->stream, M, b->d_bdesc, b->w_reduce, nullptr, b->n_reduce, nullptr, 0));
+    HIPCHK(c, oi_launch_reduce(c->stream, M, sl.d_bdesc, b->w_reduce, nullptr, b->n_reduce, nullptr, 0, t0, t1,
+                               max_blocks));
     if (b->n_apex) {
-        HIPCHK(c, oi_launch_reduce_apex(c->stream, M, b->d_bdesc, b->w_apex, nullptr, b->n_apex, seed));
+        HIPCHK(c, oi_launch_reduce_apex(c->stream, M, sl.d_bdesc, b->w_apex, nullptr, b->n_apex, seed));
         if (seed && seeded) *seeded = true;
     }
+    return OIBVH_OK;
+}
+
+// event pair for this refit when timing samples it
+static oibvh_status refit_timing(oibvh_batch* b, hipEvent_t* t0, hipEvent_t* t1)
+{
+    *t0 = *t1 = nullptr;
+    if (b->timing && (b->seq_refit++ % b->timing) == 0) return timing_pair(b->ctx, b->ev_refit, b->n_refit, t0, t1);
+    return OIBVH_OK;
+}
+
+static oibvh_status detect_timing(oibvh_batch* b, hipEvent_t* t0, hipEvent_t* t1)
+{
+    *t0 = *t1 = nullptr;
+    if (b->timing && (b->seq_detect++ % b->timing) == 0)
+        return timing_pair(b->ctx, b->ev_detect, b->n_detect, t0, t1);
     return OIBVH_OK;
 }
 
@@ -475,7 +666,7 @@ static oibvh_status ensure_reduce_tables(oibvh_batch* b)
 {
     const int M = batch_lpt(b);
     if (M != b->reduce_M) {
-        HIPCHK(b->ctx, hipStreamSynchronize(b->ctx->stream));
+        HIPCHK(b->ctx, oi_ctx_quiesce(b->ctx));
         return make_reduce_tables(b, M);
     }
     return OIBVH_OK;
@@ -488,17 +679,19 @@ extern "C" oibvh_status oibvh_batch_build(oibvh_batch* b)
     HIPCHK(c, hipSetDevice(c->device));
     oibvh_status r = commit(b);
     if (r != OIBVH_OK) return r;
+    if ((r = settle(b, true)) != OIBVH_OK) return r;
     if ((r = ensure_reduce_tables(b)) != OIBVH_OK) return r;
     hipStream_t s = c->stream;
+    const oibvh_batch::Slot& sl = cur_slot(b);
     const uint32_t F = (uint32_t)b->Ft;
     const bool m63 = b->morton_bits == 63;
     if (m63 && !b->keys_hi) HIPCHK(c, dalloc(&b->keys_hi, F));
     uint32_t* ghist = b->sort_scratch;
     uint32_t* tctr = b->sort_scratch + OI_SORT_HIST;
     uint32_t* lb[2] = {b->sort_scratch + OI_SORT_HEAD, b->sort_scratch + OI_SORT_HEAD + 256ull * b->tiles};
-    HIPCHK(c, oi_launch_mesh_box(s, b->d_bdesc, b->w_box, nullptr, b->n_box, b->partials, b->sort_scratch,
+    HIPCHK(c, oi_launch_mesh_box(s, sl.d_bdesc, b->w_box, nullptr, b->n_box, b->partials, b->sort_scratch,
                                  OI_SORT_HEAD + 256u * b->tiles));
-    HIPCHK(c, oi_launch_morton(s, b->d_bdesc, b->w_morton, nullptr, b->n_morton, b->partials, b->keys,
+    HIPCHK(c, oi_launch_morton(s, sl.d_bdesc, b->w_morton, nullptr, b->n_morton, b->partials, b->keys,
                                m63 ? b->keys_hi : nullptr, ghist, b->fmesh));
     // Morton passes (four over the low word with the keys moving along; for
     // 63-bit keys four more over the high word, looked up by face id), then
@@ -523,9 +716,9 @@ extern "C" oibvh_status oibvh_batch_build(oibvh_batch* b)
         vin = vout;
         vout = (vout == b->vals_tmp) ? b->vals : b->vals_tmp;
     }
-    HIPCHK(c, oi_launch_reduce(s, b->reduce_M, b->d_bdesc, b->w_reduce, nullptr, b->n_reduce, vin, 1));
+    HIPCHK(c, oi_launch_reduce(s, b->reduce_M, sl.d_bdesc, b->w_reduce, nullptr, b->n_reduce, vin, 1));
     if (b->n_apex)
-        HIPCHK(c, oi_launch_reduce_apex(s, b->reduce_M, b->d_bdesc, b->w_apex, nullptr, b->n_apex, nullptr));
+        HIPCHK(c, oi_launch_reduce_apex(s, b->reduce_M, sl.d_bdesc, b->w_apex, nullptr, b->n_apex, nullptr));
     b->built = true;
     return OIBVH_OK;
 }
@@ -545,9 +738,12 @@ extern "C" oibvh_status oibvh_batch_refit(oibvh_batch* b)
     oibvh_ctx* c = b->ctx;
     if (!b->built) return oi_set_err(c, OIBVH_ERR_NOT_BUILT, "batch refit before build");
     HIPCHK(c, hipSetDevice(c->device));
-    oibvh_status r = ensure_reduce_tables(b);
+    oibvh_status r = settle(b, true);
+    if (r == OIBVH_OK) r = ensure_reduce_tables(b);
     if (r != OIBVH_OK) return r;
-    return enqueue_refit(b);
+    hipEvent_t t0, t1;
+    if ((r = refit_timing(b, &t0, &t1)) != OIBVH_OK) return r;
+    return enqueue_refit(b, nullptr, nullptr, t0, t1);
 }
 
 // initial front: for every scene, the entry-level node pairs of every mesh
@@ -586,7 +782,8 @@ static void make_front(oibvh_batch* b, uint32_t entry, bool self)
     }
 }
 
-// set the detector up for (entry, expand, flags): front, table, depth, buffers
+// set the detector up for (entry, expand, flags) over the current slot:
+// front, table, depth, buffers
 static oibvh_status prepare_detect(oibvh_batch* b, uint32_t entry, uint32_t expand, uint32_t flags)
 {
     oibvh_ctx* c = b->ctx;
@@ -594,7 +791,7 @@ static oibvh_status prepare_detect(oibvh_batch* b, uint32_t entry, uint32_t expa
     const bool self = (flags & OIBVH_DETECT_SELF) != 0;
     const uint32_t key[2] = {entry, self ? 1u : 0u};
     if (std::memcmp(key, b->front_key, sizeof key) != 0) {
-        HIPCHK(c, hipStreamSynchronize(c->stream));  // the previous front may still be in flight
+        HIPCHK(c, oi_ctx_quiesce(c));  // the previous front may still be in flight
         make_front(b, entry, self);
         oibvh_status r = oi_det_set_front(&d, b->h_front);
         if (r != OIBVH_OK) return r;
@@ -602,7 +799,8 @@ static oibvh_status prepare_detect(oibvh_batch* b, uint32_t entry, uint32_t expa
     }
     uint32_t depth = 0;
     for (const oibvh_batch::Mesh& m : b->meshes) depth = std::max(depth, m.L - std::min(m.L, entry));
-    d.tab = oi_tab_args{b->d_mdesc, b->h_mdesc.data(), (uint32_t)b->meshes.size(), b->chunk_mesh, OI_CHUNK_SHIFT};
+    const oibvh_batch::Slot& sl = cur_slot(b);
+    d.tab = oi_tab_args{sl.d_mdesc, sl.h_mdesc.data(), (uint32_t)b->meshes.size(), b->chunk_mesh, OI_CHUNK_SHIFT};
     d.depth = depth;
     d.expand = c->trav_expand ? c->trav_expand : std::min(3u, std::max(1u, expand));
     d.schedule = c->trav_schedule;
@@ -617,17 +815,137 @@ extern "C" oibvh_status oibvh_batch_detect(oibvh_batch* b, uint32_t entry, uint3
     oibvh_ctx* c = b->ctx;
     if (!b->built) return oi_set_err(c, OIBVH_ERR_NOT_BUILT, "batch detect before build");
     HIPCHK(c, hipSetDevice(c->device));
+    oibvh_status r = settle(b, true);
+    if (r != OIBVH_OK) return r;
     oi_detector& d = b->det;
     const bool async = (flags & OIBVH_DETECT_ASYNC) != 0;
     if (d.unchecked && !async) {
-        oibvh_status r = oi_det_check(&d);
+        r = oi_det_check(&d);
         if (r != OIBVH_OK) return r;
     }
+    oi_det_use_set(&d, b->cur);
     d.n_pairs = 0;
     d.stats = oibvh_detect_stats{};
-    oibvh_status r = prepare_detect(b, entry, expand, flags);
-    if (r != OIBVH_OK) return r;
+    if ((r = prepare_detect(b, entry, expand, flags)) != OIBVH_OK) return r;
+    hipEvent_t e0, e1;
+    if ((r = detect_timing(b, &e0, &e1)) != OIBVH_OK) return r;
+    if (e0) HIPCHK(c, hipEventRecord(e0, c->stream));
     if ((r = oi_det_enqueue(&d, !async)) != OIBVH_OK) return r;
+    if (e1) HIPCHK(c, hipEventRecord(e1, c->stream));
+    if (async) return OIBVH_OK;
+    return oi_det_check(&d);
+}
+
+// capture the work `enqueue` queues on the context's stream into g, unless
+// g already holds a graph made under the same key
+template <class F>
+static oibvh_status captured(oibvh_batch* b, oibvh_batch::Graph& g, const uint64_t (&key)[8], F enqueue)
+{
+    oibvh_ctx* c = b->ctx;
+    if (g.exec && std::memcmp(key, g.key, sizeof key) == 0) return OIBVH_OK;
+    if (g.exec) {
+        HIPCHK(c, oi_ctx_quiesce(c));
+        (void)hipGraphExecDestroy(g.exec);
+        g.exec = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    oibvh_status rr = enqueue();
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (rr != OIBVH_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rr;
+    }
+    if (e != hipSuccess) return oi_set_err(c, OIBVH_ERR_HIP, "frame capture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        g.exec = nullptr;
+        return oi_set_err(c, OIBVH_ERR_HIP, "frame instantiate: %s", hipGetErrorString(e));
+    }
+    std::memcpy(g.key, key, sizeof key);
+    return OIBVH_OK;
+}
+
+// what a captured detect depends on beyond the buffers (slot, query, tuning)
+static void detect_key(const oibvh_batch* b, uint32_t flags, uint64_t* k)
+{
+    const oibvh_ctx* c = b->ctx;
+    const oi_detector& d = b->det;
+    k[0] = b->front_key[0] | ((uint64_t)d.local_chunk << 32) | ((uint64_t)d.local_levels << 48);
+    k[1] = d.expand | ((uint64_t)d.schedule << 32);
+    k[2] = (flags & ~(OIBVH_DETECT_ASYNC | OIBVH_FRAME_OVERLAP)) | ((uint64_t)c->narrow_separate << 32);
+    k[3] = (uint64_t)b->reduce_M | ((uint64_t)d.tail_chunk << 16) | ((uint64_t)d.tail_from << 40);
+    k[4] = d.generation;
+    k[5] = (uint64_t)b->desc_gen ^ ((uint64_t)oi_grid_for(c) << 32);
+    k[6] = (uint64_t)b->cur | ((uint64_t)d.n_levels << 8) | ((uint64_t)d.n_front << 32);
+    k[7] = (uint64_t)c->refit_blocks;
+}
+
+// An overlapped frame: refit into the slot the staging calls since the last
+// frame wrote, on the context's stream, then this frame's detect on the side
+// stream, while the previous frame's detect (reading the other slot) may
+// still run there. Each half is a captured graph per slot.
+static oibvh_status overlapped_frame(oibvh_batch* b, uint32_t entry, uint32_t expand, uint32_t flags)
+{
+    oibvh_ctx* c = b->ctx;
+    oi_detector& d = b->det;
+    const bool async = (flags & OIBVH_DETECT_ASYNC) != 0;
+    oibvh_status r = oi_ctx_side_init(c);
+    if (r != OIBVH_OK) return r;
+    if (d.unchecked && !async && (r = oi_det_check(&d)) != OIBVH_OK) return r;
+    if (b->ov_active) {
+        // the staged slot becomes current (opened now if nothing was staged)
+        uint32_t k = 0;
+        if ((r = stage_slot(b, &k)) != OIBVH_OK) return r;
+        b->cur = k;
+        b->staged_other = false;
+    } else {
+        // first overlapped frame: staging went to the current slot, whose
+        // readers the last settle already waited for
+        if ((r = wait_slot_readers(b, b->cur)) != OIBVH_OK) return r;
+    }
+    // the detector's counter set follows the slot, so the refit's apex
+    // launch resets the set the previous frame's detect is not using
+    oi_det_use_set(&d, b->cur);
+    d.n_pairs = 0;
+    d.stats = oibvh_detect_stats{};
+    if ((r = prepare_detect(b, entry, expand, flags & ~(OIBVH_DETECT_ASYNC | OIBVH_FRAME_OVERLAP))) != OIBVH_OK)
+        return r;
+    const oi_seed_args seed = oi_det_seed_args(&d, false);
+    bool seeded = b->n_apex != 0;
+    uint64_t key[8];
+    detect_key(b, flags, key);
+    // refit half (eager when timing samples it)
+    hipEvent_t t0, t1;
+    if ((r = refit_timing(b, &t0, &t1)) != OIBVH_OK) return r;
+    if (t0) {
+        r = enqueue_refit(b, &seed, &seeded, t0, t1, c->refit_blocks);
+    } else {
+        oibvh_batch::Graph& gr = b->g_refit[b->cur];
+        r = captured(b, gr, key, [&]() { return enqueue_refit(b, &seed, &seeded, nullptr, nullptr, c->refit_blocks); });
+        if (r == OIBVH_OK) HIPCHK(c, hipGraphLaunch(gr.exec, c->stream));
+    }
+    if (r != OIBVH_OK) return r;
+    HIPCHK(c, hipEventRecord(c->ev_main, c->stream));
+    // detect half, captured on the context's stream, run on the side stream
+    oibvh_batch::Graph& gd = b->g_detect[b->cur];
+    key[7] |= seeded ? (1ull << 40) : 0ull;
+    if ((r = captured(b, gd, key, [&]() { return oi_det_enqueue(&d, false, seeded); })) != OIBVH_OK) return r;
+    HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_main, 0));
+    hipEvent_t e0, e1;
+    if ((r = detect_timing(b, &e0, &e1)) != OIBVH_OK) return r;
+    if (e0) HIPCHK(c, hipEventRecord(e0, c->side));
+    HIPCHK(c, hipGraphLaunch(gd.exec, c->side));
+    if (e1) HIPCHK(c, hipEventRecord(e1, c->side));
+    const uint64_t q = c->side_seq++;
+    HIPCHK(c, hipEventRecord(c->ev_ring[q % OI_SIDE_RING], c->side));
+    b->slot[b->cur].read_seq = q + 1;
+    c->side_pending = true;
+    b->ov_active = true;
+    d.unchecked = true;
+    d.have_result = true;
+    d.sorted_ok = false;
     if (async) return OIBVH_OK;
     return oi_det_check(&d);
 }
@@ -642,48 +960,85 @@ extern "C" oibvh_status oibvh_batch_frame(oibvh_batch* b, uint32_t entry, uint32
     HIPCHK(c, hipSetDevice(c->device));
     oibvh_status r = ensure_reduce_tables(b);
     if (r != OIBVH_OK) return r;
+    if (flags & OIBVH_FRAME_OVERLAP) {
+        if ((r = ensure_slot(b, b->cur ^ 1u)) != OIBVH_OK) return r;
+        return overlapped_frame(b, entry, expand, flags);
+    }
+    if ((r = settle(b, true)) != OIBVH_OK) return r;
     oi_detector& d = b->det;
+    if (d.unchecked && !(flags & OIBVH_DETECT_ASYNC) && (r = oi_det_check(&d)) != OIBVH_OK) return r;
+    oi_det_use_set(&d, b->cur);
     d.n_pairs = 0;
     d.stats = oibvh_detect_stats{};
     if ((r = prepare_detect(b, entry, expand, flags & ~OIBVH_DETECT_ASYNC)) != OIBVH_OK) return r;
-    const uint64_t key[6] = {entry | ((uint64_t)d.local_chunk << 32) | ((uint64_t)d.local_levels << 48),
-                             d.expand | ((uint64_t)d.schedule << 32),
-                             (flags & ~OIBVH_DETECT_ASYNC) | ((uint64_t)c->narrow_separate << 32),
-                             (uint64_t)b->reduce_M | ((uint64_t)d.tail_chunk << 16) | ((uint64_t)d.tail_from << 40),
-                             d.generation, (uint64_t)b->desc_gen ^ ((uint64_t)oi_grid_for(c) << 32)};
-    if (!b->gexec || std::memcmp(key, b->graph_key, sizeof key) != 0) {
-        if (b->gexec) {
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-            (void)hipGraphExecDestroy(b->gexec);
-            b->gexec = nullptr;
-        }
-        hipGraph_t g = nullptr;
-        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        // the detect's counter reset rides on the refit's apex launch
-        const oi_seed_args seed = oi_det_seed_args(&d, false);
-        bool seeded = false;
-        oibvh_status rr = enqueue_refit(b, &seed, &seeded);
-        if (rr == OIBVH_OK) rr = oi_det_enqueue(&d, false, seeded);
-        hipError_t e = hipStreamEndCapture(c->stream, &g);
-        if (rr != OIBVH_OK) {
-            if (g) (void)hipGraphDestroy(g);
+    // the detect's counter reset rides on the refit's apex launch
+    const oi_seed_args seed = oi_det_seed_args(&d, false);
+    bool seeded = false;
+    hipEvent_t t0, t1, e0, e1;
+    if ((r = refit_timing(b, &t0, &t1)) != OIBVH_OK) return r;
+    if ((r = detect_timing(b, &e0, &e1)) != OIBVH_OK) return r;
+    if (t0 || e0) {
+        // a sampled frame runs eagerly, with its event pairs
+        if ((r = enqueue_refit(b, &seed, &seeded, t0, t1)) != OIBVH_OK) return r;
+        if (e0) HIPCHK(c, hipEventRecord(e0, c->stream));
+        if ((r = oi_det_enqueue(&d, false, seeded)) != OIBVH_OK) return r;
+        if (e1) HIPCHK(c, hipEventRecord(e1, c->stream));
+    } else {
+        uint64_t key[8];
+        detect_key(b, flags, key);
+        key[7] = 0;
+        oibvh_batch::Graph& g = b->g_frame[b->cur];
+        r = captured(b, g, key, [&]() {
+            oibvh_status rr = enqueue_refit(b, &seed, &seeded);
+            if (rr == OIBVH_OK) rr = oi_det_enqueue(&d, false, seeded);
             return rr;
-        }
-        if (e != hipSuccess) return oi_set_err(c, OIBVH_ERR_HIP, "frame capture: %s", hipGetErrorString(e));
-        e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) {
-            b->gexec = nullptr;
-            return oi_set_err(c, OIBVH_ERR_HIP, "frame instantiate: %s", hipGetErrorString(e));
-        }
-        std::memcpy(b->graph_key, key, sizeof key);
+        });
+        if (r != OIBVH_OK) return r;
+        HIPCHK(c, hipGraphLaunch(g.exec, c->stream));
     }
-    HIPCHK(c, hipGraphLaunch(b->gexec, c->stream));
     d.unchecked = true;
     d.have_result = true;
     d.sorted_ok = false;
     if (flags & OIBVH_DETECT_ASYNC) return OIBVH_OK;
     return oi_det_check(&d);
+}
+
+// HIP-event timing of the refit's stage-1 launch and of detects (every n-th
+// of each; 0 stops), as oibvh_scene_set_timing
+extern "C" oibvh_status oibvh_batch_set_timing(oibvh_batch* b, uint32_t every)
+{
+    if (!b) return OIBVH_ERR_INVALID;
+    b->timing = every;
+    b->seq_refit = b->seq_detect = 0;
+    b->n_refit = b->n_detect = 0;
+    return OIBVH_OK;
+}
+
+static oibvh_status sum_pairs(oibvh_ctx* c, const std::vector<hipEvent_t>& ev, uint32_t n, double* ms)
+{
+    *ms = 0.0;
+    for (uint32_t k = 0; k < n; k++) {
+        HIPCHK(c, hipEventSynchronize(ev[2 * k + 1]));
+        float v = 0.0f;
+        HIPCHK(c, hipEventElapsedTime(&v, ev[2 * k], ev[2 * k + 1]));
+        *ms += v;
+    }
+    return OIBVH_OK;
+}
+
+extern "C" oibvh_status oibvh_batch_get_timing(oibvh_batch* b, oibvh_scene_timing* out)
+{
+    if (!b || !out) return OIBVH_ERR_INVALID;
+    oibvh_ctx* c = b->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    std::memset(out, 0, sizeof *out);
+    oibvh_status r = sum_pairs(c, b->ev_refit, b->n_refit, &out->refit_ms);
+    if (r == OIBVH_OK) r = sum_pairs(c, b->ev_detect, b->n_detect, &out->detect_ms);
+    if (r != OIBVH_OK) return r;
+    out->refits = b->n_refit;
+    out->detects = b->n_detect;
+    b->n_refit = b->n_detect = 0;
+    return OIBVH_OK;
 }
 
 extern "C" oibvh_status oibvh_batch_get_pair_count(oibvh_batch* b, uint32_t* count)
@@ -707,7 +1062,7 @@ extern "C" oibvh_status oibvh_batch_get_pair_vertices(oibvh_batch* b, float* out
 {
     if (!b) return OIBVH_ERR_INVALID;
     HIPCHK(b->ctx, hipSetDevice(b->ctx->device));
-    return oi_det_pair_vertices(&b->det, b->d_bdesc, out, cap, ptr_kind, n);
+    return oi_det_pair_vertices(&b->det, cur_slot(b).d_bdesc, out, cap, ptr_kind, n);
 }
 
 extern "C" oibvh_status oibvh_batch_get_stats(oibvh_batch* b, oibvh_detect_stats* st)
@@ -742,7 +1097,7 @@ extern "C" oibvh_status oibvh_batch_get_nodes(oibvh_batch* b, uint32_t mesh, flo
     const oibvh_batch::Mesh& m = b->meshes[mesh];
     if (n) *n = oi_tree_size(m.F);
     if (!out || cap < oi_tree_size(m.F)) return oi_set_err(b->ctx, OIBVH_ERR_CAPACITY, "need %u nodes", oi_tree_size(m.F));
-    return mesh_readback(b, mesh, out, b->nodes + 6ull * m.nbase, 24ull * m.N);
+    return mesh_readback(b, mesh, out, cur_slot(b).nodes + 6ull * m.nbase, 24ull * m.N);
 }
 
 extern "C" oibvh_status oibvh_batch_get_perm(oibvh_batch* b, uint32_t mesh, uint32_t* out, uint64_t cap)

@@ -1175,17 +1175,37 @@ uint32_t oi_reduce_height(int M, uint32_t L)
     return (L < 8 + lgm) ? L : 8 + lgm;
 }
 
-// stage 1 (M = leaves per thread: 1, 2, 4 or 8); vals: build, null: refit
+// stage 1 (M = leaves per thread: 1, 2, 4 or 8; 16: wave subtrees); vals:
+// build, null: refit. With t0 and t1 (both or neither) the launch records
+// them as it starts and completes. max_blocks > 0 (refits with M = 16 only):
+// at most that many workgroups (rounded up to a multiple of 8), each looping
+// over work items, as in oi_launch_reduce_inline
 hipError_t oi_launch_reduce(hipStream_t st, int M, const oi_build_desc* d, const uint2* work,
-                            const oi_build_desc* one, uint32_t nblocks, const uint32_t* vals, uint32_t write_perm)
+                            const oi_build_desc* one, uint32_t nblocks, const uint32_t* vals, uint32_t write_perm,
+                            hipEvent_t t0, hipEvent_t t1, uint32_t max_blocks)
 {
+    if (!t0 != !t1) return hipErrorInvalidValue;
     ReduceArgs a{make_btab(d, work, one), vals, write_perm};
+    dim3 g(nblocks);
+    const dim3 b(OI_BLOCK);
+    if (M == 16 && !vals && max_blocks && max_blocks < nblocks) {
+        const uint32_t cap = (max_blocks + 7u) & ~7u;
+        if (cap < nblocks) {
+            a.nwork = nblocks;
+            g = dim3(cap);
+        }
+    }
     switch (M) {
-    case 1: hipLaunchKernelGGL(k_reduce<1>, dim3(nblocks), dim3(OI_BLOCK), 0, st, a); break;
-    case 2: hipLaunchKernelGGL(k_reduce<2>, dim3(nblocks), dim3(OI_BLOCK), 0, st, a); break;
-    case 4: hipLaunchKernelGGL(k_reduce<4>, dim3(nblocks), dim3(OI_BLOCK), 0, st, a); break;
-    case 16: hipLaunchKernelGGL(k_reduce_w, dim3(nblocks), dim3(OI_BLOCK), 0, st, a); break;
-    default: hipLaunchKernelGGL(k_reduce<8>, dim3(nblocks), dim3(OI_BLOCK), 0, st, a); break;
+    case 1: launch_timed(k_reduce<1>, g, b, st, t0, t1, a); break;
+    case 2: launch_timed(k_reduce<2>, g, b, st, t0, t1, a); break;
+    case 4: launch_timed(k_reduce<4>, g, b, st, t0, t1, a); break;
+    case 16:
+        if (a.nwork)
+            launch_timed(k_reduce_wl, g, b, st, t0, t1, a);
+        else
+            launch_timed(k_reduce_w, g, b, st, t0, t1, a);
+        break;
+    default: launch_timed(k_reduce<8>, g, b, st, t0, t1, a); break;
     }
     return hipGetLastError();
 }

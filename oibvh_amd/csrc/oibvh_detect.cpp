@@ -45,17 +45,20 @@ oibvh_status oi_det_init(oi_detector* d, oibvh_ctx* c)
     return OIBVH_OK;
 }
 
-void oi_det_flip_set(oi_detector* d)
+void oi_det_use_set(oi_detector* d, uint32_t k)
 {
+    if (k == d->set) return;
     // the set left behind keeps an unchecked result until a check reads it;
     // the set taken holds none any more: an asynchronous seed folds its
     // errors into its sticky bits, and a synchronous frame checks first
     if (d->unchecked) d->set_unchecked[d->set] = true;
-    d->set = (d->set + 1u) % OI_FRAME_DEPTH;
+    d->set = k % OI_FRAME_DEPTH;
     d->set_unchecked[d->set] = false;
     d->ctl = d->ctl_base + d->set;
     d->counts = d->counts_base ? d->counts_base + 256u * d->set : nullptr;
 }
+
+void oi_det_flip_set(oi_detector* d) { oi_det_use_set(d, (d->set + 1u) % OI_FRAME_DEPTH); }
 
 void oi_det_free(oi_detector* d)
 {

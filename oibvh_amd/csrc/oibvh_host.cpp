@@ -37,7 +37,7 @@ hipError_t oi_ctx_quiesce(oibvh_ctx* ctx)
 }
 
 // the side stream of overlapped frames and its events (first use)
-static oibvh_status side_init(oibvh_ctx* c)
+oibvh_status oi_ctx_side_init(oibvh_ctx* c)
 {
     if (c->side) return OIBVH_OK;
     hipStream_t s = nullptr;
@@ -107,6 +107,7 @@ struct oibvh_tree {
     uint32_t morton_bits = 30;
     float* box = nullptr;        // 6 floats + per-block partial boxes
     float* xf = nullptr;         // model matrix (16 floats, device)
+    const float* xf_cur = nullptr; // the matrix later launches read: xf, or a scene's table / caller memory
     bool has_xf = false;
     bool built = false;
 };
@@ -119,6 +120,8 @@ struct oibvh_scene {
     std::vector<oi_mesh_desc> h_desc;  // host copies stay alive until the next detect
     oi_build_desc* d_geom = nullptr;   // per-tree geometry for the vertex export
     uint32_t geom_cap = 0;
+    float* d_xf = nullptr;             // model matrices copied by oibvh_scene_set_transforms (16 per tree)
+    uint32_t xf_cap = 0;
     // optional HIP-event timing of the refit's stage-1 launches and of the
     // detects (pairs of events, read and reset by oibvh_scene_get_timing)
     uint32_t timing = 0;              // 0: off; n: every n-th refit and detect
@@ -286,7 +289,7 @@ static oi_build_desc tree_desc(const oibvh_tree* t)
 {
     oi_build_desc d{};
     d.pos = t->pos;
-    d.xf = t->has_xf ? t->xf : nullptr;
+    d.xf = t->has_xf ? t->xf_cur : nullptr;
     d.faces = t->faces;
     d.sfaces = t->sfaces;
     d.perm = t->perm;
@@ -529,6 +532,7 @@ extern "C" oibvh_status oibvh_tree_set_transform(oibvh_tree* t, const float* mat
         return OIBVH_OK;
     }
     HIPCHK(c, oi_launch_set_xf(c->stream, t->xf, mat4));
+    t->xf_cur = t->xf;
     t->has_xf = true;
     return OIBVH_OK;
 }
@@ -666,6 +670,7 @@ extern "C" void oibvh_scene_destroy(oibvh_scene* s)
     (void)oi_ctx_quiesce(s->ctx);
     dfree(s->d_desc);
     dfree(s->d_geom);
+    dfree(s->d_xf);
     for (hipEvent_t e : s->ev_refit) (void)hipEventDestroy(e);
     for (hipEvent_t e : s->ev_detect) (void)hipEventDestroy(e);
     oi_det_free(&s->det);
@@ -737,7 +742,7 @@ static oibvh_status prepare_scene_detect(oibvh_scene* s)
         md.sfaces = t->sfaces;
         md.pos = t->pos;
         md.perm = t->perm;
-        md.xf = t->has_xf ? t->xf : nullptr;
+        md.xf = t->has_xf ? t->xf_cur : nullptr;
         md.F = t->F;
         md.L = t->L;
         md.vl = t->vl;
@@ -840,6 +845,52 @@ extern "C" oibvh_status oibvh_scene_detect(oibvh_scene* s, uint32_t entry, uint3
     return oi_det_check(&d);
 }
 
+// Mesh::transform (mesh.cpp:189-214) for every tree of the scene in one call:
+// host matrices are staged per tree in stream order, device ones copied into
+// the scene's table with one copy, borrowed ones read in place; each tree then
+// reads its matrix from there
+extern "C" oibvh_status oibvh_scene_set_transforms(oibvh_scene* s, const float* mats, uint32_t ptr_kind)
+{
+    if (!s || !mats || ptr_kind > OIBVH_PTR_DEVICE_BORROW) return OIBVH_ERR_INVALID;
+    oibvh_ctx* c = s->ctx;
+    const uint32_t n = (uint32_t)s->trees.size();
+    if (n == 0) return OIBVH_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (ptr_kind == OIBVH_PTR_DEVICE_BORROW) {
+        // no device work: launches queued later read the new pointers, earlier
+        // ones keep the ones they were given
+        for (uint32_t i = 0; i < n; i++) {
+            s->trees[i]->xf_cur = mats + 16ull * i;
+            s->trees[i]->has_xf = true;
+        }
+        return OIBVH_OK;
+    }
+    HIPCHK(c, oi_ctx_join(c));
+    if (ptr_kind == OIBVH_PTR_HOST) {
+        for (uint32_t i = 0; i < n; i++) {
+            oibvh_tree* t = s->trees[i];
+            HIPCHK(c, oi_launch_set_xf(c->stream, t->xf, mats + 16ull * i));
+            t->xf_cur = t->xf;
+            t->has_xf = true;
+        }
+        return OIBVH_OK;
+    }
+    if (s->xf_cap < n) {
+        // the table may still be read by queued work
+        HIPCHK(c, oi_ctx_quiesce(c));
+        dfree(s->d_xf);
+        s->xf_cap = 0;
+        HIPCHK(c, dalloc(&s->d_xf, 16ull * n));
+        s->xf_cap = n;
+    }
+    HIPCHK(c, hipMemcpyAsync(s->d_xf, mats, 64ull * n, hipMemcpyDeviceToDevice, c->stream));
+    for (uint32_t i = 0; i < n; i++) {
+        s->trees[i]->xf_cur = s->d_xf + 16ull * i;
+        s->trees[i]->has_xf = true;
+    }
+    return OIBVH_OK;
+}
+
 // Every tree of the scene refit at once: up to four trees per launch pair
 // (their descriptors travel in the kernel arguments), so the reduce fills the
 // GPU once instead of once per tree. Same results as oibvh_tree_refit on each.
@@ -906,7 +957,7 @@ static oibvh_status overlapped_frame(oibvh_scene* s, uint32_t entry, uint32_t ex
     oibvh_ctx* c = s->ctx;
     oi_detector& d = s->det;
     const bool async = (flags & OIBVH_DETECT_ASYNC) != 0;
-    oibvh_status r = side_init(c);
+    oibvh_status r = oi_ctx_side_init(c);
     if (r != OIBVH_OK) return r;
     if (d.unchecked && !async && (r = oi_det_check(&d)) != OIBVH_OK) return r;
     // wait for the latest side detect that read a buffer about to be

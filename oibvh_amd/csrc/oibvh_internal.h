@@ -31,7 +31,8 @@ hipError_t oi_launch_sort_pass(hipStream_t, const uint32_t*, const uint32_t*, ui
                                uint32_t, const uint32_t*, uint32_t*, uint32_t*, uint32_t*, const uint32_t*);
 uint32_t oi_reduce_height(int, uint32_t);
 hipError_t oi_launch_reduce(hipStream_t, int, const oi_build_desc*, const uint2*, const oi_build_desc*, uint32_t,
-                            const uint32_t*, uint32_t);
+                            const uint32_t*, uint32_t, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr,
+                            uint32_t max_blocks = 0);
 hipError_t oi_launch_reduce_apex(hipStream_t, int, const oi_build_desc*, const uint2*, const oi_build_desc*,
                                  uint32_t, const oi_seed_args*);
 hipError_t oi_launch_set_xf(hipStream_t, float*, const float*);
@@ -112,6 +113,8 @@ hipError_t oi_ctx_join(oibvh_ctx* ctx);
 // wait until both streams are idle (before memory they may use is freed or
 // rewritten from the host)
 hipError_t oi_ctx_quiesce(oibvh_ctx* ctx);
+// create the side stream and its events of overlapped frames (first use)
+oibvh_status oi_ctx_side_init(oibvh_ctx* c);
 
 oibvh_status oi_set_err(oibvh_ctx* ctx, oibvh_status st, const char* fmt, ...);
 
@@ -244,6 +247,8 @@ oibvh_status oi_det_enqueue(oi_detector* d, bool reset_sticky, bool seeded = fal
 oi_seed_args oi_det_seed_args(const oi_detector* d, bool reset_sticky);
 // switch to the other counter set (an overlapped frame, before prepare)
 void oi_det_flip_set(oi_detector* d);
+// use counter set k (a batch's overlapped frames tie it to their slot)
+void oi_det_use_set(oi_detector* d, uint32_t k);
 // wait; on overflow grow the buffers and run the query again
 oibvh_status oi_det_check(oi_detector* d);
 // sorted hits (4 words each)

@@ -396,3 +396,193 @@ def test_batch_overflow_retry(api, orc):
     assert np.array_equal(b.pairs(), got)
     b.close()
     ctx2.close()
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_batch_overlapped_frames(api, ctx, orc, fp64):
+    """OIBVH_FRAME_OVERLAP: frame f's refit runs beside frame f - 1's detect,
+    each half a captured graph per slot; matrices staged from HBM after each
+    frame. Every checked frame equals an eager refit + detect of the same
+    matrices on a second batch: synchronous overlapped frames, a run of
+    asynchronous ones read at the end, non-overlapped calls in between (with
+    inputs staged for an overlapped frame that never ran), and overlapped
+    frames again."""
+    say(f"batch: overlapped frames (fp64={fp64})")
+    rs = small_rigid(n_scenes=7, seed=0xA7)
+    b = make_batch(api, ctx, rs)
+    ref = make_batch(api, ctx, rs)
+    nf = 12
+    mats = np.stack([rs.transforms(f) for f in range(nf)])
+    buf = api.DeviceBuffer(mats.nbytes)
+    buf.upload(np.ascontiguousarray(np.swapaxes(mats, -1, -2)))
+    stride = mats.shape[1] * 64
+    b.set_transforms(mats[0])
+    ref.set_transforms(mats[0])
+    b.build()
+    ref.build()
+
+    def want(f):
+        ref.set_transforms(mats[f])
+        ref.refit()
+        ref.detect(2, 2, fp64_fallback=fp64)
+        return ref.pairs()
+
+    def check(f, msg):
+        got = b.pairs()
+        assert np.array_equal(got, want(f)), msg
+        for i in (0, b.mesh_count - 1):
+            assert np.array_equal(bits(b.nodes(i)), bits(ref.nodes(i))), msg
+
+    # synchronous overlapped frames
+    for f in range(0, 4):
+        b.set_transforms_device(buf.at(f * stride))
+        b.frame(2, 2, fp64_fallback=fp64, overlap=True)
+        check(f, f"sync overlapped frame {f}")
+    # asynchronous run, read at the end
+    for f in range(4, 9):
+        b.set_transforms_device(buf.at(f * stride))
+        b.frame(2, 2, fp64_fallback=fp64, overlap=True, asynchronous=True)
+    st = b.stats()
+    assert st["async_errors"] == 0 and st["retries"] == 0
+    check(8, "async overlapped run")
+    # inputs staged for an overlapped frame, then a plain refit + detect
+    b.set_transforms_device(buf.at(9 * stride))
+    b.refit()
+    b.detect(2, 2, fp64_fallback=fp64)
+    check(9, "refit + detect after staging")
+    b.frame(2, 2, fp64_fallback=fp64)  # non-overlapped graph frame, same matrices
+    check(9, "plain frame")
+    for f in (10, 11):
+        b.set_transforms_device(buf.at(f * stride))
+        b.frame(2, 2, fp64_fallback=fp64, overlap=True, asynchronous=f == 10)
+    check(11, "overlapped again")
+    perms = [b.perm(i) for i in range(b.mesh_count)]
+    assert np.array_equal(b.pairs(), all_pairs(orc, rs, mats[11], perms=perms, fp64=fp64))
+    b.close()
+    ref.close()
+    buf.free()
+
+
+def test_batch_overlapped_positions(api, ctx, orc):
+    """Overlapped frames of deforming meshes: each frame points every mesh
+    at its own vertex memory (set_positions_device, borrowed), self mode;
+    frames that stage only some meshes keep the others' last positions. Each
+    frame equals the oracle."""
+    say("batch: overlapped frames, borrowed positions")
+    nv = 40
+    sheets = [M.cloth(nv, 0.2, 0.25, 0x77 + i) for i in range(3)]
+    frames = 5
+    pos = [[M.cloth_positions(nv, 0.2, 0.25, 0x77 + i, phase=0.2 * f) for f in range(frames)] for i in range(3)]
+    bufs = [[api.DeviceBuffer.from_array(p) for p in row] for row in pos]
+    b = api.Batch(ctx)
+    for i, (x, f) in enumerate(sheets):
+        b.add_mesh(i, x, f)
+    b.build()
+    cur = [0, 0, 0]
+    for f in range(frames):
+        movers = range(3) if f % 2 == 0 else (1,)  # odd frames move sheet 1 only
+        for i in movers:
+            b.set_positions_device(i, bufs[i][f].at(0))
+            cur[i] = f
+        b.frame(2, 2, self_collide=True, overlap=True)
+        got = b.pairs()
+        rows = []
+        for i, (x, fc) in enumerate(sheets):
+            p = orc.detect([orc.Tree(pos[i][cur[i]], fc, order=b.perm(i))], self_collide=True)
+            if len(p):
+                p = p.copy()
+                p[:, :2] = i
+                rows.append(p)
+        want = np.concatenate(rows)
+        want = want[np.lexsort((want[:, 3], want[:, 2], want[:, 1], want[:, 0]))]
+        assert len(want) > 0
+        assert np.array_equal(got, want), f"frame {f}"
+    b.close()
+    for row in bufs:
+        for x in row:
+            x.free()
+
+
+def test_batch_overlapped_overflow(api, orc):
+    """Overlapped frames whose detect buffers overflow: a synchronous frame
+    re-runs its detect over the grown buffers; an asynchronous run reports
+    the overflow of frames it could not re-run as async_errors, and later
+    frames are complete again."""
+    say("batch: overlapped frames, tiny detect buffers")
+    rs = small_rigid(n_scenes=4, seed=0x9A)
+    ctx2 = api.Context(0)
+    ctx2.set_tuning(front_capacity=256, hits_capacity=16, candidate_capacity=64)
+    b = make_batch(api, ctx2, rs)
+    b.set_transforms(rs.transforms(0))
+    b.build()
+    b.set_transforms(rs.transforms(1))
+    b.frame(1, 1, overlap=True)
+    assert b.stats()["retries"] > 0
+    perms = [b.perm(i) for i in range(b.mesh_count)]
+    assert np.array_equal(b.pairs(), all_pairs(orc, rs, rs.transforms(1), perms=perms))
+    for f in (2, 3, 4):
+        b.set_transforms(rs.transforms(f))
+        b.frame(1, 1, overlap=True)
+        assert np.array_equal(b.pairs(), all_pairs(orc, rs, rs.transforms(f), perms=perms)), f"frame {f}"
+    b.close()
+    ctx2.close()
+
+
+def test_batch_timing(api, ctx):
+    """oibvh_batch_set_timing: sampled refits run eagerly with an event pair
+    on their stage-1 launch, sampled detects with a pair around them; both in
+    overlapped and plain frames."""
+    rs = small_rigid(n_scenes=5, seed=0xB1)
+    b = make_batch(api, ctx, rs)
+    b.set_transforms(rs.transforms(0))
+    b.build()
+    b.set_timing(2)
+    for f in range(6):
+        b.set_transforms(rs.transforms(f))
+        b.frame(1, 2, overlap=f >= 3, asynchronous=True)
+    b.refit()
+    t = b.timing()
+    assert t["refits"] == 4 and t["detects"] == 3
+    assert t["refit_ms"] > 0 and t["detect_ms"] > 0
+    b.set_timing(0)
+    b.close()
+
+
+def _scene_rows(p, lo, hi):
+    if not len(p):
+        return p
+    a, b = np.searchsorted(p[:, 0], lo), np.searchsorted(p[:, 0], hi)
+    return p[a:b]
+
+
+def test_precision_full_size_vs_oracles(api, ctx, orc):
+    """The precision workload at full size: 64 scenes of two ~250k-triangle
+    planes or boxes, coplanar or edge-touching. On every scene where the FP32
+    and the FP32 + FP64-fallback pair sets differ, and on one scene of each
+    kind, the fallback's pairs equal the FP64 oracle's and the FP32 pairs the
+    FP32 oracle's (the reference CPU path), at full size."""
+    say("precision scenes at full size")
+    scenes = M.precision_scenes()
+    b = api.Batch(ctx)
+    for si, ms in enumerate(scenes):
+        for x, f in ms:
+            b.add_mesh(si, x, f)
+    b.build()
+    b.detect(4, 2, fp64_fallback=True)
+    p64 = b.pairs()
+    assert b.stats()["fp64_rechecks"] > 0
+    b.detect(4, 2)
+    p32 = b.pairs()
+    changed = [si for si in range(len(scenes))
+               if not np.array_equal(_scene_rows(p32, 2 * si, 2 * si + 2), _scene_rows(p64, 2 * si, 2 * si + 2))]
+    assert changed, "the workload should hold scenes where FP32 and FP64 disagree"
+    for si in sorted(set(changed) | {0, 1, 2, 3}):
+        say(f"  scene {si} ({'FP32 != FP64' if si in changed else 'kind ' + str(si % 4)})")
+        ot = [orc.Tree(x, f, order=b.perm(2 * si + j)) for j, (x, f) in enumerate(scenes[si])]
+        for fp64, got in ((True, p64), (False, p32)):
+            want = orc.detect(ot, fp64=fp64)
+            if len(want):
+                want = want.copy()
+                want[:, :2] += 2 * si
+            assert np.array_equal(_scene_rows(got, 2 * si, 2 * si + 2), want), (si, fp64)
+    b.close()

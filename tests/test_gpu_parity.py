@@ -319,6 +319,49 @@ def test_transform_build_refit_detect(api, ctx, orc):
     assert np.array_equal(bits(t0.nodes()), bits(orc.Tree(x0, f0, order=t0.perm()).nodes()))
 
 
+@pytest.mark.parametrize("kind", ["host", "device", "borrow"])
+@pytest.mark.parametrize("n", [2, 6])
+def test_scene_set_transforms(api, ctx, orc, kind, n):
+    """oibvh_scene_set_transforms gives every tree its matrix in one call
+    (host staging, one device copy, or matrices read in place from HBM):
+    refit nodes and pairs equal the oracle's over the transformed positions,
+    frame after frame, with scene frames (scenes above four trees take the
+    device mesh table) and a plain refit + detect."""
+    say(f"scene set_transforms ({kind}, {n} trees)")
+    shapes = [M.icosphere(3, 1.0, 0.02, 20 + i) for i in range(n)]
+    trees = [api.OibvhTree(ctx, x, f) for x, f in shapes]
+    for t in trees:
+        t.build()
+    sc = api.Scene(ctx)
+    for t in trees:
+        sc.addOibvhTree(t)
+    frames = 4
+    mats = np.stack([np.stack([_motion(30 + i, k) for i in range(n)]) for k in range(frames)])
+    buf = api.DeviceBuffer(mats.nbytes)
+    buf.upload(np.ascontiguousarray(np.swapaxes(mats, -1, -2)))  # column-major
+    for k in range(frames):
+        if kind == "host":
+            sc.set_transforms(mats[k])
+        else:
+            sc.set_transforms(device=buf.at(k * n * 64), borrow=kind == "borrow")
+        if k % 2:
+            sc.frame(None, 1, 2)
+        else:
+            sc.refit()
+            sc.detectCollision(1, 2)
+        ot = [orc.Tree(orc.xform(mats[k][i], shapes[i][0]), shapes[i][1], order=trees[i].perm()) for i in range(n)]
+        for t, o in zip(trees, ot):
+            assert np.array_equal(bits(t.nodes()), bits(o.nodes()))
+        want = orc.detect(ot)
+        assert np.array_equal(sc.getIntTriPairs(), want)
+    # a per-tree matrix afterwards replaces the scene's
+    trees[0].set_transform(None)
+    trees[0].refit()
+    assert np.array_equal(bits(trees[0].nodes()), bits(orc.Tree(*shapes[0], order=trees[0].perm()).nodes()))
+    sc.close()
+    buf.free()
+
+
 @pytest.mark.parametrize("n", [1, 3, 6])
 def test_scene_refit_equals_tree_refits(api, ctx, orc, n):
     """oibvh_scene_refit (up to four trees per launch) gives every tree the
