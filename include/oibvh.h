@@ -48,7 +48,8 @@ typedef struct oibvh_batch oibvh_batch;
 
 /* detect flags */
 #define OIBVH_DETECT_SELF 1u          /* also collide every mesh with itself (pairs sharing a vertex skipped) */
-#define OIBVH_DETECT_FP64_FALLBACK 2u /* FP32 SAT with forward error filter, FP64 re-check of uncertain axes */
+#define OIBVH_DETECT_FP64_FALLBACK 2u /* FP32 SAT with a forward error filter; pairs it leaves undecided are
+                                         tested again by the FP64 SAT */
 #define OIBVH_DETECT_ASYNC 4u         /* enqueue only: no host wait. Results are checked at the next
                                          get/stats call or synchronous detect; a further ASYNC detect
                                          replaces an unchecked one, whose errors go to async_errors */
@@ -195,16 +196,23 @@ OIBVH_API oibvh_status oibvh_scene_get_pair_vertices(oibvh_scene* scene, float* 
  * length, valid until the next detect. Synchronises. */
 OIBVH_API oibvh_status oibvh_scene_hits_device(oibvh_scene* scene, void** dptr, uint32_t* n);
 
+/* The colliding pairs in oibvh_scene_get_pairs order, sorted on the device:
+ * device pointer (4 uint32 per pair) and count, valid until the next detect.
+ * The reference downloads its pair list (scene.cu:414-416); here it can stay
+ * in HBM for a device-side consumer. Synchronises (the sort is complete on
+ * return). */
+OIBVH_API oibvh_status oibvh_scene_pairs_device(oibvh_scene* scene, const uint32_t** dptr, uint32_t* n);
+
 typedef struct {
     uint64_t bvtt_tests;    /* BVTT node pairs tested */
     uint64_t candidates;    /* leaf pairs sent to the narrow phase */
-    uint64_t fp64_rechecks; /* SAT axis comparisons re-evaluated in FP64 */
+    uint64_t fp64_rechecks; /* SAT axis comparisons evaluated in FP64 (OIBVH_DETECT_FP64_FALLBACK) */
     uint64_t adjacent;      /* self-mode leaf pairs skipped for sharing a vertex */
     uint32_t pairs;         /* colliding pairs */
     uint32_t retries;       /* detect re-runs after buffer growth */
     uint32_t async_errors;  /* error bits of earlier OIBVH_DETECT_ASYNC detects whose results were
                                never checked (their pair lists were incomplete); 0 when all were clean */
-    uint32_t reserved;
+    uint32_t fp64_candidates; /* candidate pairs the FP32 filter left to the FP64 SAT */
 } oibvh_detect_stats;
 
 /* counters of the last detect (no reference counterpart; the reference prints
@@ -407,6 +415,8 @@ OIBVH_API oibvh_status oibvh_batch_get_pair_count(oibvh_batch* batch, uint32_t* 
 /* Sorted (meshA, meshB, origTriA, origTriB) with batch mesh indices,
  * meshA <= meshB, 4 words each; cap in pairs. */
 OIBVH_API oibvh_status oibvh_batch_get_pairs(oibvh_batch* batch, uint32_t* out, uint64_t cap, uint32_t* n);
+/* as oibvh_scene_pairs_device, with batch mesh indices */
+OIBVH_API oibvh_status oibvh_batch_pairs_device(oibvh_batch* batch, const uint32_t** dptr, uint32_t* n);
 /* as oibvh_scene_get_pair_vertices, over the batch's meshes */
 OIBVH_API oibvh_status oibvh_batch_get_pair_vertices(oibvh_batch* batch, float* out, uint64_t cap,
                                                      uint32_t ptr_kind, uint32_t* n);

@@ -50,7 +50,7 @@ class SceneTiming(C.Structure):
 class DetectStats(C.Structure):
     _fields_ = [("bvtt_tests", C.c_uint64), ("candidates", C.c_uint64), ("fp64_rechecks", C.c_uint64),
                 ("adjacent", C.c_uint64), ("pairs", C.c_uint32), ("retries", C.c_uint32),
-                ("async_errors", C.c_uint32), ("reserved", C.c_uint32)]
+                ("async_errors", C.c_uint32), ("fp64_candidates", C.c_uint32)]
 
 
 API = [
@@ -88,6 +88,7 @@ API = [
     ("oibvh_scene_set_timing", _i, [_vp, _u32]),
     ("oibvh_scene_get_timing", _i, [_vp, C.c_void_p]),
     ("oibvh_scene_hits_device", _i, [_vp, C.POINTER(_vp), _u32p]),
+    ("oibvh_scene_pairs_device", _i, [_vp, C.POINTER(_vp), _u32p]),
     ("oibvh_scene_get_pair_vertices", _i, [_vp, _vp, _u64, _u32, _u32p]),
     ("oibvh_scene_get_stats", _i, [_vp, C.POINTER(DetectStats)]),
     ("oibvh_scene_get_level_counts", _i, [_vp, _u32p, _u64, _u32p]),
@@ -109,6 +110,7 @@ API = [
     ("oibvh_batch_get_pair_count", _i, [_vp, _u32p]),
     ("oibvh_batch_get_pairs", _i, [_vp, _u32p, _u64, _u32p]),
     ("oibvh_batch_get_pair_vertices", _i, [_vp, _vp, _u64, _u32, _u32p]),
+    ("oibvh_batch_pairs_device", _i, [_vp, C.POINTER(_vp), _u32p]),
     ("oibvh_obj_read", _i, [_vp, C.c_char_p, _f32p, _u64, _u32p, _u64, _u32p, _u32p]),
     ("oibvh_batch_get_stats", _i, [_vp, C.POINTER(DetectStats)]),
     ("oibvh_batch_get_level_counts", _i, [_vp, _u32p, _u64, _u32p]),

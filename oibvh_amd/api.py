@@ -253,6 +253,15 @@ def _pair_vertices(obj, fn, n: int) -> np.ndarray:
     _check(obj.ctx.h, fn(obj.h, out.ctypes.data, out.size, L.PTR_HOST, C.byref(got)), "pair_vertices")
     return out[: got.value]
 
+
+def download_u32(dptr: int, count: int) -> np.ndarray:
+    """count uint32 words from device memory (harness plumbing)."""
+    out = np.empty(count, np.uint32)
+    if count:
+        L.hip_check(L.hip().hipMemcpy(_ptr(out), C.c_void_p(dptr), out.nbytes, 2), "hipMemcpy D2H")
+    return out
+
+
 class Scene:
     """Scene (scene.cuh:26-182)."""
 
@@ -342,6 +351,13 @@ class Scene:
         """Scene::convertToVertexArray: (n, 6, 3) float32, triangle A's three
         vertices then triangle B's, in getIntTriPairs order."""
         return _pair_vertices(self, self._lib.oibvh_scene_get_pair_vertices, self.getIntTriPairCount())
+
+    def pairs_device(self) -> tuple[int, int]:
+        """(device address, n) of the pairs in getIntTriPairs order, sorted on
+        the device (4 uint32 each; valid until the next detect)."""
+        p, n = C.c_void_p(), C.c_uint32()
+        _check(self.ctx.h, self._lib.oibvh_scene_pairs_device(self.h, C.byref(p), C.byref(n)), "pairs_device")
+        return p.value or 0, n.value
 
     def pair_vertices_device(self, dptr: int, cap_floats: int) -> int:
         """The same into device memory (18 floats per pair); returns n."""
@@ -490,6 +506,12 @@ class Batch:
         _check(self.ctx.h, self._lib.oibvh_batch_get_pairs(self.h, out.ctypes.data_as(L._u32p), max(n, 1),
                                                            C.byref(got)), "batch pairs")
         return out[: got.value]
+
+    def pairs_device(self) -> tuple[int, int]:
+        """(device address, n) of pairs() sorted on the device."""
+        p, n = C.c_void_p(), C.c_uint32()
+        _check(self.ctx.h, self._lib.oibvh_batch_pairs_device(self.h, C.byref(p), C.byref(n)), "pairs_device")
+        return p.value or 0, n.value
 
     def pair_vertices(self) -> np.ndarray:
         """(n, 6, 3) float32 triangle vertices of every pair, pairs() order."""

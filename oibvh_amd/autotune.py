@@ -122,6 +122,9 @@ def tune_batch(ctx: api.Context, batch: api.Batch, before_frame, *, entries=(0, 
         for i in range(reps):
             before_frame(i + 1)
             batch.frame(entry, expand, asynchronous=True, **flags)
+        # an overlapped frame's detect runs on the library's side stream:
+        # wait for it before the stop event
+        ctx.synchronize()
         timer.stop()
         ms = timer.ms() / reps
         batch.pair_count()
@@ -157,11 +160,15 @@ def tune_batch(ctx: api.Context, batch: api.Batch, before_frame, *, entries=(0, 
             best = {**best, "traverse_tail": chunk, "traverse_tail_steps": steps, "ms": ms}
     ctx.set_tuning(traverse_tail=best["traverse_tail"], traverse_tail_steps=best["traverse_tail_steps"])
     best["refit_blocks"] = base.get("refit_blocks", 0)
+    # a cap applies to the wave-subtree refit (16 leaves per thread) only:
+    # each capped candidate runs with it
     for rb in refit_blocks:
-        ctx.set_tuning(refit_blocks=rb)
+        lpt = 16 if rb else best["reduce_leaves_per_thread"]
+        ctx.set_tuning(refit_blocks=rb, reduce_leaves_per_thread=lpt)
         ms = run(best["entry_level"], best["expand_levels"])
         if ms < best["ms"]:
-            best = {**best, "refit_blocks": rb, "ms": ms}
+            best = {**best, "refit_blocks": rb, "reduce_leaves_per_thread": lpt, "ms": ms}
+    ctx.set_tuning(refit_blocks=best["refit_blocks"], reduce_leaves_per_thread=best["reduce_leaves_per_thread"])
     ctx.set_tuning(**{**base, "reduce_leaves_per_thread": best["reduce_leaves_per_thread"],
                       "traverse_blocks": best["traverse_blocks"], "traverse_expand": best["expand_levels"],
                       "traverse_local": best["traverse_local"],

@@ -1056,6 +1056,16 @@ extern "C" oibvh_status oibvh_batch_get_pairs(oibvh_batch* b, uint32_t* out, uin
     return oi_det_get_pairs(&b->det, out, cap, n);
 }
 
+extern "C" oibvh_status oibvh_batch_pairs_device(oibvh_batch* b, const uint32_t** dptr, uint32_t* n)
+{
+    if (!b || !dptr || !n) return OIBVH_ERR_INVALID;
+    HIPCHK(b->ctx, hipSetDevice(b->ctx->device));
+    const uint4* p = nullptr;
+    oibvh_status r = oi_det_sorted_device(&b->det, &p, n);
+    *dptr = reinterpret_cast<const uint32_t*>(p);
+    return r;
+}
+
 // Scene::convertToVertexArray (scene.cu:69-93) over the batch's meshes
 extern "C" oibvh_status oibvh_batch_get_pair_vertices(oibvh_batch* b, float* out, uint64_t cap, uint32_t ptr_kind,
                                                       uint32_t* n)

@@ -1018,6 +1018,7 @@ __device__ __forceinline__ void seed_counters(const oi_seed_args& s)
         s.ctl->fp64 = 0;
         s.ctl->adj = 0;
         s.ctl->tests = 0;
+        s.ctl->deferred = 0;
     }
 }
 

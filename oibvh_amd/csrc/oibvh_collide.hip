@@ -165,114 +165,78 @@ __device__ bool sat_f32(const v3f* P, const v3f* Q)
 
 // ---------------------------------------------------------------------------
 // FP32 filter with FP64 re-check (precision form).
-// Every FP32 quantity z carries a magnitude S(z) with |z - exact z| <=
-// c * 2^-24 * S(z) (+ O(2^-48)) at rounding depth c. The differences of the
-// exact FP32 inputs (p2 = P2 - P1 and so on) are the leaves: their only error
-// is their own rounding, so S = |fl(difference)| at depth 1; above them S is
-// built by the same expression over magnitudes (sums of magnitudes for +/-,
-// products for *). With c counted as roundings along the chain (a product
-// adds its operands' depths plus one, a sum takes the larger plus one): the
-// differences p = P - P1 have c = 1, edges 2, n1 = e x e 6, g = e x n1 10,
-// the products of a g/h projection 12 and its two additions 14, the deepest
-// SAT quantity. A
-// comparison mn1 > mx2 is decided in FP32 only when |mn1 - mx2| exceeds
-// T = 32 * 2^-24 * (S_P + S_Q) + 2^-100, which covers the FP32 error of both
-// sides, the FP64 error of the oracle's own evaluation, the rounding of the
-// magnitudes themselves and of T; then the FP32 decision equals the exact one
-// and the FP64 one. Otherwise the axis is evaluated again in double exactly
-// as the FP64 oracle evaluates it. Non-finite magnitudes never pass the test,
-// so they also go to FP64.
+// Every FP32 SAT quantity z has a magnitude S(z) >= |z| built by the same
+// expression over |operands| (sums of magnitudes for +/-, products for *),
+// and |fl(z) - z| <= c * 2^-24 * S(z) (+ O(2^-48)) at rounding depth c, c
+// counted as roundings along the chain (a product adds its operands' depths
+// plus one, a sum takes the larger plus one): the differences p = P - P1
+// have c = 1, edges 2, n1 = e x e 6, g = e x n1 10, the products of a g/h
+// projection 12 and its two additions 14, the deepest SAT quantity.
+// Here the magnitudes are bounded per component through three scalars per
+// triangle pair: M = (Mx, My, Mz), the largest |x|, |y| and |z| components of
+// the five differences p2, p3, q1, q2, q3. Every magnitude is a sum of
+// products of leaf magnitudes with non-negative coefficients, so it grows
+// with them: edges have magnitudes <= 2M (componentwise), n1 <= 4C, m1 and
+// e x f <= 8C with C = (My Mz, Mz Mx, Mx My), g <= 8D and h <= 16D with
+// D = M x C (magnitude cross product) = (Mx (My^2 + Mz^2), My (Mz^2 + Mx^2),
+// Mz (Mx^2 + My^2)); a projection pair S_P + S_Q <= 2 (axis bound . M):
+// 24 Mx My Mz (n1), 48 Mx My Mz (m1, e x f), 32 W (g), 64 W (h), with
+// W = Mx^2 My^2 + My^2 Mz^2 + Mz^2 Mx^2. A flat axis (a plane perpendicular
+// to it) keeps the n1 bound near zero. A comparison mn1 > mx2 is decided in
+// FP32 only when |mn1 - mx2| exceeds T = 32 * 2^-24 * (that bound) + 2^-100,
+// which covers the FP32 error of both sides (c <= 14), the FP64 oracle's own
+// error, the rounding of the subtraction, of the bound and of T; the FP32
+// decision then equals the exact one and the FP64 one. A pair with some
+// undecided axis and no decided separating axis is evaluated again, every
+// axis in double exactly as the FP64 oracle evaluates it (k_narrow's second
+// pass). Non-finite or NaN values never decide.
 // ---------------------------------------------------------------------------
+enum { SAT_SEPARATED = 0, SAT_HIT = 1, SAT_UNDECIDED = 2 };
+
 __device__ __forceinline__ v3f vabs(v3f a) { return v3f{fabsf(a.x), fabsf(a.y), fabsf(a.z)}; }
-__device__ __forceinline__ v3f vaddm(v3f a, v3f b) { return v3f{a.x + b.x, a.y + b.y, a.z + b.z}; }
-__device__ __forceinline__ v3f vcrossm(v3f a, v3f b)
-{
-    return v3f{a.y * b.z + b.y * a.z, a.z * b.x + b.z * a.x, a.x * b.y + b.x * a.y};
-}
-__device__ __forceinline__ float vdotm(v3f a, v3f b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ v3f vmaxc(v3f a, v3f b) { return v3f{fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)}; }
 
-struct SatMag {
-    v3f p2, p3, q1, q2, q3, e1, e2, e3, f1, f2, f3, n1, m1;
-    // s: the FP32 frame of the same pair (its differences are the leaves)
-    __device__ __forceinline__ void init(const SatFrame<float, v3f>& s)
-    {
-        p2 = vabs(s.p2);
-        p3 = vabs(s.p3);
-        q1 = vabs(s.q1);
-        q2 = vabs(s.q2);
-        q3 = vabs(s.q3);
-        e1 = p2;
-        e2 = vaddm(p3, p2);
-        e3 = p3;
-        f1 = vaddm(q2, q1);
-        f2 = vaddm(q3, q2);
-        f3 = vaddm(q1, q3);
-        n1 = vcrossm(e1, e2);
-        m1 = vcrossm(f1, f2);
-    }
-    __device__ __forceinline__ v3f axis(int k) const
-    {
-        switch (k) {
-        case AX_N1: return n1;
-        case AX_M1: return m1;
-        case AX_E11: return vcrossm(e1, f1);
-        case AX_E12: return vcrossm(e1, f2);
-        case AX_E13: return vcrossm(e1, f3);
-        case AX_E21: return vcrossm(e2, f1);
-        case AX_E22: return vcrossm(e2, f2);
-        case AX_E23: return vcrossm(e2, f3);
-        case AX_E31: return vcrossm(e3, f1);
-        case AX_E32: return vcrossm(e3, f2);
-        case AX_E33: return vcrossm(e3, f3);
-        case AX_G1: return vcrossm(e1, n1);
-        case AX_G2: return vcrossm(e2, n1);
-        case AX_G3: return vcrossm(e3, n1);
-        case AX_H1: return vcrossm(f1, m1);
-        case AX_H2: return vcrossm(f2, m1);
-        default: return vcrossm(f3, m1);
-        }
-    }
-};
-
-// returns 1 = not separated, 0 = separated; counts FP64 re-evaluations
-__device__ bool sat_filtered(const v3f* P, const v3f* Q, uint32_t& n_fp64)
+__device__ int sat_filter(const v3f* P, const v3f* Q)
 {
     SatFrame<float, v3f> s;
     s.init(P, Q);
-    SatMag m;
-    m.init(s);
-    bool have_d = false;
-    SatFrame<double, v3d> sd;
+    const v3f M = vmaxc(vmaxc(vmaxc(vabs(s.p2), vabs(s.p3)), vmaxc(vabs(s.q1), vabs(s.q2))), vabs(s.q3));
+    const float xyz = M.x * M.y * M.z;
+    const float xx = M.x * M.x, yy = M.y * M.y, zz = M.z * M.z;
+    const float W = (xx * yy + yy * zz) + zz * xx;
+    constexpr float U = 32.0f * 5.9604645e-8f, TINY = 7.888609e-31f;
+    const float t_n1 = U * 24.0f * xyz + TINY, t_ef = U * 48.0f * xyz + TINY;
+    const float t_g = U * 32.0f * W + TINY, t_h = U * 64.0f * W + TINY;
+    bool undecided = false;
     for (int k = 0; k < 17; k++) {
         const v3f ax = s.axis(k);
-        const v3f am = m.axis(k);
         const v3f p1 = v3f{0, 0, 0};
         const float P1 = vdot<float>(ax, p1), P2 = vdot<float>(ax, s.p2), P3 = vdot<float>(ax, s.p3);
         const float Q1 = vdot<float>(ax, s.q1), Q2 = vdot<float>(ax, s.q2), Q3 = vdot<float>(ax, s.q3);
         const float mx1 = fmaxf(fmaxf(P1, P2), P3), mn1 = fminf(fminf(P1, P2), P3);
         const float mx2 = fmaxf(fmaxf(Q1, Q2), Q3), mn2 = fminf(fminf(Q1, Q2), Q3);
-        const float SP = fmaxf(vdotm(am, m.p2), vdotm(am, m.p3));
-        const float SQ = fmaxf(fmaxf(vdotm(am, m.q1), vdotm(am, m.q2)), vdotm(am, m.q3));
-        const float T = 32.0f * 5.9604645e-8f * (SP + SQ) + 7.888609e-31f;
+        const float T = k == AX_N1 ? t_n1 : k <= AX_E33 ? t_ef : k <= AX_G3 ? t_g : t_h;
         const bool finite = (T < __builtin_inff());
         // separation on this axis: mn1 > mx2 or mn2 > mx1
-        const bool sep1_sure = finite && (mn1 - mx2 > T);
-        const bool nsep1_sure = finite && (mx2 - mn1 > T);
-        const bool sep2_sure = finite && (mn2 - mx1 > T);
-        const bool nsep2_sure = finite && (mx1 - mn2 > T);
-        if (sep1_sure || sep2_sure) return false;
-        if (nsep1_sure && nsep2_sure) continue;
-        // uncertain: evaluate this axis exactly like the FP64 oracle
-        if (!have_d) {
-            v3d Pd[3], Qd[3];
+        if (finite && (mn1 - mx2 > T || mn2 - mx1 > T)) return SAT_SEPARATED;
+        if (!(finite && mx2 - mn1 > T && mx1 - mn2 > T)) undecided = true;
+    }
+    return undecided ? SAT_UNDECIDED : SAT_HIT;
+}
+
+// the FP64 oracle's SAT over the widened FP32 inputs (every axis until one
+// separates); counts the axes evaluated
+__device__ bool sat_f64_counted(const v3f* P, const v3f* Q, uint32_t& n_fp64)
+{
+    v3d Pd[3], Qd[3];
 #pragma unroll
-            for (int i = 0; i < 3; i++) {
-                Pd[i] = v3d{(double)P[i].x, (double)P[i].y, (double)P[i].z};
-                Qd[i] = v3d{(double)Q[i].x, (double)Q[i].y, (double)Q[i].z};
-            }
-            sd.init(Pd, Qd);
-            have_d = true;
-        }
+    for (int i = 0; i < 3; i++) {
+        Pd[i] = v3d{(double)P[i].x, (double)P[i].y, (double)P[i].z};
+        Qd[i] = v3d{(double)Q[i].x, (double)Q[i].y, (double)Q[i].z};
+    }
+    SatFrame<double, v3d> sd;
+    sd.init(Pd, Qd);
+    for (int k = 0; k < 17; k++) {
         n_fp64++;
         if (!project6<double, v3d>(sd, sd.axis(k))) return false;
     }
@@ -904,15 +868,27 @@ struct NarrowArgs {
     uint32_t* n_hits;
     uint32_t hits_cap;
     uint32_t* error;
-    unsigned long long* fp64;   // SAT axis comparisons re-evaluated in double
+    unsigned long long* fp64;   // SAT axis comparisons evaluated in double
     unsigned long long* adj;    // self-mode leaf pairs dropped as adjacent
+    uint2* defer;               // NARROW_FILTER: undecided candidates for NARROW_FP64
+    uint32_t* defer_count;
+    uint32_t defer_cap;
 };
 
-template <bool FP64, class Tab>
-__device__ __forceinline__ void narrow_one(const Tab& T, const uint32_t* s_ibase, const oi_mesh_desc* s_desc,
-                                           uint2 c, bool& hit, uint4& out, uint32_t& n_fp64, uint32_t& n_adj)
+// NARROW_FP32: the reference's FP32 SAT. NARROW_FILTER: the FP32 filter;
+// undecided candidates go to the defer list. NARROW_FP64: the FP64 oracle's
+// SAT over the defer list (the precision form's second pass).
+enum { NARROW_FP32 = 0, NARROW_FILTER = 1, NARROW_FP64 = 2 };
+
+// The two triangles of candidate c as the SAT takes them: sorted faces and
+// original ids, then the vertices through the model matrices; inside one
+// mesh P is the face with the smaller original id, across meshes the face of
+// the lower mesh. False (nothing to test) for self-mode pairs that share a
+// vertex, counted in n_adj when count_adj.
+template <class Tab>
+__device__ __forceinline__ bool narrow_gather(const Tab& T, const uint32_t* s_ibase, const oi_mesh_desc* s_desc,
+                                              uint2 c, bool count_adj, uint32_t& n_adj, v3f* P, v3f* Q, uint4& id)
 {
-    hit = false;
     uint32_t ma, mb;
     const MeshRef ra = tab_ref(T, s_desc, s_ibase, c.x, ma);
     const MeshRef rb = tab_ref(T, s_desc, s_ibase, c.y, mb);
@@ -931,20 +907,18 @@ __device__ __forceinline__ void narrow_one(const Tab& T, const uint32_t* s_ibase
 #pragma unroll
             for (int j = 0; j < 3; j++) adj |= (fa[i] == fb[j]);
         if (adj) {
-            n_adj++;
-            return;
+            if (count_adj) n_adj++;
+            return false;
         }
     }
-    v3f P[3], Q[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         P[k] = vertex(ra.pos, ra.xf, fa[k]);
         Q[k] = vertex(rb.pos, rb.xf, fb[k]);
     }
     if (ma == mb && oa > ob) {
-        // inside one mesh P is the face with the smaller original id (both
-        // faces share the mesh's positions and model matrix, so the loaded
-        // vertices are swapped)
+        // both faces share the mesh's positions and model matrix, so the
+        // loaded vertices are swapped
         const uint32_t t = oa;
         oa = ob;
         ob = t;
@@ -955,20 +929,33 @@ __device__ __forceinline__ void narrow_one(const Tab& T, const uint32_t* s_ibase
             Q[k] = u;
         }
     }
-    const bool h = FP64 ? sat_filtered(P, Q, n_fp64) : sat_f32(P, Q);
-    if (h) {
+    id = make_uint4(ma, mb, oa, ob);
+    return true;
+}
+
+// FP32 narrow phase of one candidate (the fused BVTT steps use it)
+template <bool FP64, class Tab>
+__device__ __forceinline__ void narrow_one(const Tab& T, const uint32_t* s_ibase, const oi_mesh_desc* s_desc,
+                                           uint2 c, bool& hit, uint4& out, uint32_t& n_fp64, uint32_t& n_adj)
+{
+    static_assert(!FP64, "the FP64 fallback runs in k_narrow's two passes");
+    hit = false;
+    v3f P[3], Q[3];
+    uint4 id;
+    if (!narrow_gather(T, s_ibase, s_desc, c, true, n_adj, P, Q, id)) return;
+    if (sat_f32(P, Q)) {
         hit = true;
-        out = make_uint4(ma, mb, oa, ob);
+        out = id;
     }
 }
 
-template <bool FP64, class Tab>
+template <int MODE, class Tab>
 __global__ void __launch_bounds__(TB) k_narrow(NarrowArgs<Tab> A)
 {
     __shared__ oi_mesh_desc s_desc[MESH_LDS];
     __shared__ uint32_t s_ibase[MESH_LDS];
     __shared__ uint32_t s_wsum[4];
-    __shared__ uint32_t s_hbase;
+    __shared__ uint32_t s_hbase, s_dbase;
     uint32_t n = *A.cand_count;
     if (n > A.cand_cap) n = A.cand_cap;
     if (blockIdx.x * TB >= n) return;
@@ -977,19 +964,48 @@ __global__ void __launch_bounds__(TB) k_narrow(NarrowArgs<Tab> A)
     uint32_t n_fp64 = 0, n_adj = 0;
     for (uint32_t base = blockIdx.x * TB; base < n; base += gridDim.x * TB) {
         const uint32_t i = base + tid;
-        bool hit = false;
-        uint4 out;
-        if (i < n) narrow_one<FP64, Tab>(A.tab, s_ibase, s_desc, A.cands[i], hit, out, n_fp64, n_adj);
-        uint32_t H;
-        const uint32_t hoff = block_scan(hit ? 1u : 0u, s_wsum, H);
-        if (tid == 0) s_hbase = H ? add_u32(A.n_hits, H) : 0u;
+        bool hit = false, defer = false;
+        uint4 out = make_uint4(0, 0, 0, 0);
+        if (i < n) {
+            const uint2 c = A.cands[i];
+            v3f P[3], Q[3];
+            // deferred candidates were checked for adjacency by the first pass
+            if (narrow_gather(A.tab, s_ibase, s_desc, c, MODE != NARROW_FP64, n_adj, P, Q, out)) {
+                if (MODE == NARROW_FP32) {
+                    hit = sat_f32(P, Q);
+                } else if (MODE == NARROW_FILTER) {
+                    const int r = sat_filter(P, Q);
+                    hit = r == SAT_HIT;
+                    defer = r == SAT_UNDECIDED;
+                } else {
+                    hit = sat_f64_counted(P, Q, n_fp64);
+                }
+            }
+            if (defer) out = make_uint4(c.x, c.y, 0, 0);
+        }
+        // one scan for both counts: hits in the high half, deferred pairs in
+        // the low half (at most 256 each, so no carry crosses)
+        uint32_t HD;
+        const uint32_t o = block_scan((hit ? 65536u : 0u) + (defer ? 1u : 0u), s_wsum, HD);
+        const uint32_t H = HD >> 16, D = HD & 0xffffu;
+        if (tid == 0) {
+            s_hbase = H ? add_u32(A.n_hits, H) : 0u;
+            if (MODE == NARROW_FILTER) s_dbase = D ? add_u32(A.defer_count, D) : 0u;
+        }
         __syncthreads();
         if (hit) {
-            const uint32_t pos = s_hbase + hoff;
+            const uint32_t pos = s_hbase + (o >> 16);
             if (pos < A.hits_cap)
                 A.hits[pos] = out;
             else
                 or_u32(A.error, OI_ERR_HITS_OVERFLOW);
+        }
+        if (MODE == NARROW_FILTER && defer) {
+            const uint32_t pos = s_dbase + (o & 0xffffu);
+            if (pos < A.defer_cap)
+                A.defer[pos] = make_uint2(out.x, out.y);
+            else
+                or_u32(A.error, OI_ERR_QUEUE_OVERFLOW);
         }
         __syncthreads();
     }
@@ -1017,6 +1033,7 @@ __global__ void k_seed(uint32_t n0, uint32_t* counts, uint32_t n_counts, oi_dete
         ctl->fp64 = 0;
         ctl->adj = 0;
         ctl->tests = 0;
+        ctl->deferred = 0;
     }
 }
 
@@ -1142,7 +1159,8 @@ static hipError_t launch_local(hipStream_t st, const Tab& tab, const oi_local_ar
 template <class Tab>
 static hipError_t launch_narrow(hipStream_t st, uint32_t grid, const Tab& tab, const uint2* cands,
                                 const uint32_t* cand_count, uint32_t cand_cap, uint4* hits, uint32_t* n_hits,
-                                uint32_t hits_cap, oi_detect_ctl* ctl, int fp64, hipEvent_t done)
+                                uint32_t hits_cap, oi_detect_ctl* ctl, int mode, uint2* defer, uint32_t defer_cap,
+                                hipEvent_t done)
 {
     NarrowArgs<Tab> a{};
     a.tab = tab;
@@ -1155,10 +1173,15 @@ static hipError_t launch_narrow(hipStream_t st, uint32_t grid, const Tab& tab, c
     a.error = &ctl->error;
     a.fp64 = &ctl->fp64;
     a.adj = &ctl->adj;
-    if (fp64)
-        launch_done((k_narrow<true, Tab>), dim3(grid), dim3(TB), st, done, a);
-    else
-        launch_done((k_narrow<false, Tab>), dim3(grid), dim3(TB), st, done, a);
+    a.defer = defer;
+    a.defer_count = &ctl->deferred;
+    a.defer_cap = defer_cap;
+    switch (mode) {
+    case NARROW_FP32: launch_done((k_narrow<NARROW_FP32, Tab>), dim3(grid), dim3(TB), st, done, a); break;
+    case NARROW_FILTER: launch_done((k_narrow<NARROW_FILTER, Tab>), dim3(grid), dim3(TB), st, done, a); break;
+    case NARROW_FP64: launch_done((k_narrow<NARROW_FP64, Tab>), dim3(grid), dim3(TB), st, done, a); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -1226,6 +1249,79 @@ k_pair_vertices(const oi_build_desc* __restrict__ g, const uint4* __restrict__ p
     o[2] = q.z;
 }
 
+// ---------------------------------------------------------------------------
+// The hit list in get_pairs order (meshA, meshB, triA, triB, lexicographic),
+// sorted on the device: k_hits_digits splits the hits into four word arrays
+// and builds the 256-bin histogram of every byte the sort visits (the bytes
+// that can be non-zero, from the mesh count and the largest face count);
+// LSD radix passes (k_sort_pass, keys looked up by hit index) then order the
+// hit indices by each byte in turn, least significant first, and
+// k_hits_gather writes the hits in that order.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_zero_u32(uint32_t* __restrict__ p, uint32_t n)
+{
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) p[i] = 0u;
+}
+
+__global__ void __launch_bounds__(256)
+k_hits_digits(const uint4* __restrict__ hits, uint32_t n, uint32_t* __restrict__ words, uint32_t* __restrict__ hist,
+              oi_hit_digits dg)
+{
+    __shared__ uint32_t s_h[OI_HIT_DIGITS][256];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t p = 0; p < dg.nd; p++) s_h[p][tid] = 0u;
+    __syncthreads();
+    OI_GLOBAL uint32_t* w = oi_gw(words);
+    for (uint32_t i = blockIdx.x * 256u + tid; i < n; i += gridDim.x * 256u) {
+        const uint4 h = hits[i];
+        w[i] = h.x;
+        w[(uint64_t)n + i] = h.y;
+        w[2ull * n + i] = h.z;
+        w[3ull * n + i] = h.w;
+        for (uint32_t p = 0; p < dg.nd; p++) {
+            const uint32_t k = dg.word[p] == 0 ? h.x : dg.word[p] == 1 ? h.y : dg.word[p] == 2 ? h.z : h.w;
+            atomicAdd(&s_h[p][(k >> dg.shift[p]) & 0xffu], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t p = 0; p < dg.nd; p++) {
+        const uint32_t c = s_h[p][tid];
+        if (c) atomicAdd(&hist[256u * p + tid], c);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_hits_gather(const uint4* __restrict__ hits, const uint32_t* __restrict__ order, uint32_t n, uint4* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) out[i] = hits[order[i]];
+}
+
+hipError_t oi_launch_zero_u32(hipStream_t st, uint32_t* p, uint32_t n)
+{
+    if (n == 0) return hipSuccess;
+    const uint32_t blocks = std::min<uint32_t>((n + 255u) / 256u, 1024u);
+    hipLaunchKernelGGL(k_zero_u32, dim3(blocks), dim3(256), 0, st, p, n);
+    return hipGetLastError();
+}
+
+hipError_t oi_launch_hits_digits(hipStream_t st, const uint4* hits, uint32_t n, uint32_t* words, uint32_t* hist,
+                                 const oi_hit_digits* dg)
+{
+    if (n == 0) return hipSuccess;
+    if (dg->nd > OI_HIT_DIGITS) return hipErrorInvalidValue;
+    const uint32_t blocks = std::min<uint32_t>((n + 255u) / 256u, 2048u);
+    hipLaunchKernelGGL(k_hits_digits, dim3(blocks), dim3(256), 0, st, hits, n, words, hist, *dg);
+    return hipGetLastError();
+}
+
+hipError_t oi_launch_hits_gather(hipStream_t st, const uint4* hits, const uint32_t* order, uint32_t n, uint4* out)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_hits_gather, dim3((n + 255u) / 256u), dim3(256), 0, st, hits, order, n, out);
+    return hipGetLastError();
+}
+
 hipError_t oi_launch_pair_vertices(hipStream_t st, const oi_build_desc* g, const uint4* pairs, uint32_t n, float* out)
 {
     if (n == 0) return hipSuccess;
@@ -1234,15 +1330,19 @@ hipError_t oi_launch_pair_vertices(hipStream_t st, const oi_build_desc* g, const
     return hipGetLastError();
 }
 
+// mode: 0 the FP32 SAT over the candidates; 1 the FP32 filter, undecided
+// candidates appended to `defer` (capacity defer_cap, count ctl->deferred);
+// 2 the FP64 SAT over those (cands = defer, cand_count = &ctl->deferred)
 hipError_t oi_launch_narrow(hipStream_t st, uint32_t grid, const oi_tab_args* t, const uint2* cands,
                             const uint32_t* cand_count, uint32_t cand_cap, uint4* hits, uint32_t* n_hits,
-                            uint32_t hits_cap, oi_detect_ctl* ctl, int fp64, hipEvent_t done)
+                            uint32_t hits_cap, oi_detect_ctl* ctl, int mode, uint2* defer, uint32_t defer_cap,
+                            hipEvent_t done)
 {
     if (t->chunk_mesh)
         return launch_narrow(st, grid, ChunkTab{t->d_meshes, t->chunk_mesh, t->shift}, cands, cand_count, cand_cap,
-                             hits, n_hits, hits_cap, ctl, fp64, done);
+                             hits, n_hits, hits_cap, ctl, mode, defer, defer_cap, done);
     return launch_narrow(st, grid, make_tab(t->d_meshes, t->h_meshes, t->n_meshes), cands, cand_count, cand_cap,
-                         hits, n_hits, hits_cap, ctl, fp64, done);
+                         hits, n_hits, hits_cap, ctl, mode, defer, defer_cap, done);
 }
 
 }  // extern "C"

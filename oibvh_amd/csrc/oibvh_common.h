@@ -118,6 +118,8 @@ struct oi_detect_ctl {
     unsigned long long fp64;   // SAT axis comparisons re-evaluated in double
     unsigned long long adj;    // leaf pairs dropped as adjacent (self mode)
     unsigned long long tests;  // BVTT box pair tests
+    uint32_t deferred;         // FP64 fallback: candidates the FP32 filter left undecided
+    uint32_t pad;
 };
 
 // a detect's counter reset (zeroed level counts and result counters), done
@@ -152,6 +154,16 @@ struct oi_local_args {
     uint32_t* count_out;
     uint32_t cap_out;
     hipEvent_t done;           // recorded when the launch completes (null: none)
+};
+
+// the bytes a device sort of the hit list visits, least significant first:
+// digit p is byte shift[p] / 8 of word word[p] (0 meshA, 1 meshB, 2 triA,
+// 3 triB)
+#define OI_HIT_DIGITS 16
+struct oi_hit_digits {
+    uint32_t nd;
+    uint8_t word[OI_HIT_DIGITS];
+    uint8_t shift[OI_HIT_DIGITS];
 };
 
 // a scene's concatenated implicit index space must stay below this bound

@@ -70,6 +70,9 @@ void oi_det_free(oi_detector* d)
     dfree(d->cands);
     dfree(d->hits);
     dfree(d->d_sorted);
+    dfree(d->sort_words);
+    dfree(d->sort_vals);
+    dfree(d->sort_scratch);
     dfree(d->d_stage);
     dfree(d->ctl_base);
     d->ctl = nullptr;
@@ -245,9 +248,18 @@ oibvh_status oi_det_enqueue(oi_detector* d, bool reset_sticky, bool seeded, hipE
         HIPCHK(c, oi_launch_bvtt_local(st, &d->tab, &w, d->cands, &d->ctl->cands, d->cand_cap, d->hits, d->hits_cap,
                                        d->ctl, d->expand, d->implicit_end, d->fused ? 1 : 0));
     }
-    if (narrow)
+    if (narrow && !fp64)
         HIPCHK(c, oi_launch_narrow(st, grid, &d->tab, d->cands, &d->ctl->cands, d->cand_cap, d->hits,
-                                   &d->ctl->n_hits, d->hits_cap, d->ctl, fp64 ? 1 : 0, done));
+                                   &d->ctl->n_hits, d->hits_cap, d->ctl, 0, nullptr, 0, done));
+    if (narrow && fp64) {
+        // the FP32 filter over the candidates, then the FP64 SAT over the
+        // pairs it left undecided; the BVTT fronts are idle by now, and the
+        // first holds that list (an overflow grows them like a front's)
+        HIPCHK(c, oi_launch_narrow(st, grid, &d->tab, d->cands, &d->ctl->cands, d->cand_cap, d->hits,
+                                   &d->ctl->n_hits, d->hits_cap, d->ctl, 1, d->front[0], d->front_cap, nullptr));
+        HIPCHK(c, oi_launch_narrow(st, grid, &d->tab, d->front[0], &d->ctl->deferred, d->front_cap, d->hits,
+                                   &d->ctl->n_hits, d->hits_cap, d->ctl, 2, nullptr, 0, done));
+    }
     else if (done_head)  // a query with no BVTT launch
         HIPCHK(c, hipEventRecord(done, st));
     d->unchecked = true;
@@ -346,12 +358,13 @@ oibvh_status oi_det_check(oi_detector* d)
             d->stats.bvtt_tests = k.tests;
             d->stats.candidates = k.cands;
             d->stats.fp64_rechecks = k.fp64;
+            d->stats.fp64_candidates = k.deferred;
             d->stats.adjacent = k.adj;
             d->stats.pairs = k.n_hits;
             d->stats.retries = retries;
             d->stats.async_errors = earlier;
             d->unchecked = false;
-            d->sorted_ok = false;  // a new result: the sorted copy is refetched on demand
+            d->sorted_ok = false;  // a new result: sorted again on demand
             return OIBVH_OK;
         }
         retries++;
@@ -361,23 +374,72 @@ oibvh_status oi_det_check(oi_detector* d)
     }
 }
 
-// the checked hits, sorted (fetched once per query)
-static oibvh_status sorted_hits(oi_detector* d)
+static uint32_t bits_for(uint32_t max_value)
+{
+    return max_value ? 32u - (uint32_t)__builtin_clz(max_value) : 0u;
+}
+
+// The checked hits sorted on the device into d_sorted, in get_pairs order
+// (meshA, meshB, triA, triB): LSD radix passes over the bytes that can be
+// non-zero (mesh ids below the table's mesh count, face ids below its
+// largest face count), sorting hit indices with keys looked up by index,
+// then one gather. Once per query.
+static oibvh_status sort_hits(oi_detector* d)
 {
     oibvh_ctx* c = d->ctx;
-    if (d->sorted_ok && d->h_sorted.size() == d->n_pairs) return OIBVH_OK;
-    std::vector<uint4>& h = d->h_sorted;
-    h.resize(d->n_pairs);
-    if (d->n_pairs) {
-        HIPCHK(c, hipMemcpyAsync(h.data(), d->hits, d->n_pairs * sizeof(uint4), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+    const uint32_t n = d->n_pairs;
+    if (d->sorted_ok || n == 0) return OIBVH_OK;
+    if (n >= (1u << 30)) return oi_set_err(c, OIBVH_ERR_OVERFLOW, "pair sort: more than 2^30 pairs");
+    uint32_t max_f = 1;
+    for (uint32_t m = 0; m < d->tab.n_meshes; m++) max_f = std::max(max_f, d->tab.h_meshes[m].F);
+    const uint32_t mbits = bits_for(d->tab.n_meshes ? d->tab.n_meshes - 1 : 0), tbits = bits_for(max_f - 1);
+    oi_hit_digits dg{};
+    for (uint32_t w : {3u, 2u, 1u, 0u}) {
+        const uint32_t nb = ((w >= 2 ? tbits : mbits) + 7u) / 8u;
+        for (uint32_t k = 0; k < nb; k++) {
+            dg.word[dg.nd] = (uint8_t)w;
+            dg.shift[dg.nd] = (uint8_t)(8u * k);
+            dg.nd++;
+        }
     }
-    std::sort(h.begin(), h.end(), [](const uint4& a, const uint4& b) {
-        if (a.x != b.x) return a.x < b.x;
-        if (a.y != b.y) return a.y < b.y;
-        if (a.z != b.z) return a.z < b.z;
-        return a.w < b.w;
-    });
+    const uint32_t tiles = (n + oi_sort_tile() - 1) / oi_sort_tile();
+    uint64_t gen = 0;
+    oibvh_status r;
+    if (d->sort_cap < n) {
+        // sized with headroom, so that the next queries rarely grow them
+        const uint64_t cap = std::min<uint64_t>((1ull << 30) - 1, (uint64_t)n + (n >> 2));
+        const uint64_t ctiles = (cap + oi_sort_tile() - 1) / oi_sort_tile();
+        dfree(d->sort_words);
+        dfree(d->sort_vals);
+        dfree(d->sort_scratch);
+        d->sort_cap = 0;
+        HIPCHK(c, dalloc(&d->sort_words, 4 * cap));
+        HIPCHK(c, dalloc(&d->sort_vals, 2 * cap));
+        HIPCHK(c, dalloc(&d->sort_scratch, 256ull * OI_HIT_DIGITS + OI_HIT_DIGITS + 512ull * ctiles));
+        d->sort_cap = (uint32_t)cap;
+    }
+    if ((r = grow(c, d->d_sorted, d->d_sorted_cap, n, gen)) != OIBVH_OK) return r;
+    uint32_t* hist = d->sort_scratch;
+    uint32_t* tctr = hist + 256u * OI_HIT_DIGITS;
+    uint32_t* lb[2] = {tctr + OI_HIT_DIGITS, tctr + OI_HIT_DIGITS + 256ull * tiles};
+    uint32_t* vals[2] = {d->sort_vals, d->sort_vals + d->sort_cap};
+    hipStream_t st = c->stream;
+    // histograms, tile counters and the first look-back buffer start at zero
+    HIPCHK(c, oi_launch_zero_u32(st, hist, 256u * OI_HIT_DIGITS + OI_HIT_DIGITS + 256u * tiles));
+    HIPCHK(c, oi_launch_hits_digits(st, d->hits, n, d->sort_words, hist, &dg));
+    const uint32_t* order = nullptr;  // identity
+    for (uint32_t p = 0; p < dg.nd; p++) {
+        HIPCHK(c, oi_launch_sort_pass(st, nullptr, order, nullptr, vals[p & 1], n, dg.shift[p], hist + 256u * p,
+                                      lb[p & 1], p + 1 < dg.nd ? lb[(p + 1) & 1] : nullptr, tctr + p,
+                                      d->sort_words + (uint64_t)n * dg.word[p]));
+        order = vals[p & 1];
+    }
+    if (order) {
+        HIPCHK(c, oi_launch_hits_gather(st, d->hits, order, n, d->d_sorted));
+    } else {
+        // one mesh of one face: at most one distinct pair
+        HIPCHK(c, hipMemcpyAsync(d->d_sorted, d->hits, n * sizeof(uint4), hipMemcpyDeviceToDevice, st));
+    }
     d->sorted_ok = true;
     return OIBVH_OK;
 }
@@ -390,14 +452,22 @@ oibvh_status oi_det_get_pairs(oi_detector* d, uint32_t* out, uint64_t cap, uint3
     if (n) *n = d->n_pairs;
     if (d->n_pairs == 0) return OIBVH_OK;
     if (!out || cap < d->n_pairs) return oi_set_err(c, OIBVH_ERR_CAPACITY, "need %u pairs", d->n_pairs);
-    if ((r = sorted_hits(d)) != OIBVH_OK) return r;
-    const std::vector<uint4>& h = d->h_sorted;
-    for (uint32_t i = 0; i < d->n_pairs; i++) {
-        out[4ull * i] = h[i].x;
-        out[4ull * i + 1] = h[i].y;
-        out[4ull * i + 2] = h[i].z;
-        out[4ull * i + 3] = h[i].w;
-    }
+    if ((r = sort_hits(d)) != OIBVH_OK) return r;
+    HIPCHK(c, hipMemcpyAsync(out, d->d_sorted, d->n_pairs * sizeof(uint4), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return OIBVH_OK;
+}
+
+oibvh_status oi_det_sorted_device(oi_detector* d, const uint4** dptr, uint32_t* n)
+{
+    oibvh_status r = oi_det_check(d);
+    if (r != OIBVH_OK) return r;
+    if ((r = sort_hits(d)) != OIBVH_OK) return r;
+    // the sort runs on the context's stream: complete it before the pointer
+    // is handed to a consumer on any stream or the host
+    HIPCHK(d->ctx, hipStreamSynchronize(d->ctx->stream));
+    *dptr = d->d_sorted;
+    *n = d->n_pairs;
     return OIBVH_OK;
 }
 
@@ -413,12 +483,7 @@ oibvh_status oi_det_pair_vertices(oi_detector* d, const oi_build_desc* geom, flo
     if (d->n_pairs == 0) return OIBVH_OK;
     const uint64_t need = 18ull * d->n_pairs;
     if (!out || cap < need) return oi_set_err(c, OIBVH_ERR_CAPACITY, "need %llu floats", (unsigned long long)need);
-    if ((r = sorted_hits(d)) != OIBVH_OK) return r;
-    uint64_t gen = 0;
-    if ((r = grow(c, d->d_sorted, d->d_sorted_cap, d->n_pairs, gen)) != OIBVH_OK) return r;
-    // the host sort is complete; the copy only has to reach the device before the kernel
-    HIPCHK(c, hipMemcpyAsync(d->d_sorted, d->h_sorted.data(), d->n_pairs * sizeof(uint4), hipMemcpyHostToDevice,
-                             c->stream));
+    if ((r = sort_hits(d)) != OIBVH_OK) return r;
     float* dst = out;
     if (ptr_kind == OIBVH_PTR_HOST) {
         if (d->stage_cap < need) {
@@ -433,7 +498,7 @@ oibvh_status oi_det_pair_vertices(oi_detector* d, const oi_build_desc* geom, flo
     if (ptr_kind == OIBVH_PTR_HOST) {
         HIPCHK(c, hipMemcpyAsync(out, d->d_stage, need * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     }
-    // the host copy of the hits must outlive the upload; host output must land
+    // host output must land; device output is complete when this returns
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return OIBVH_OK;
 }

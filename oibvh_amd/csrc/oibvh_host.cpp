@@ -1140,6 +1140,16 @@ extern "C" oibvh_status oibvh_scene_hits_device(oibvh_scene* s, void** dptr, uin
     return OIBVH_OK;
 }
 
+extern "C" oibvh_status oibvh_scene_pairs_device(oibvh_scene* s, const uint32_t** dptr, uint32_t* n)
+{
+    if (!s || !dptr || !n) return OIBVH_ERR_INVALID;
+    HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    const uint4* p = nullptr;
+    oibvh_status r = oi_det_sorted_device(&s->det, &p, n);
+    *dptr = reinterpret_cast<const uint32_t*>(p);
+    return r;
+}
+
 extern "C" oibvh_status oibvh_scene_get_pairs(oibvh_scene* s, uint32_t* out, uint64_t cap, uint32_t* n)
 {
     if (!s) return OIBVH_ERR_INVALID;

@@ -48,10 +48,13 @@ hipError_t oi_launch_bvtt_step(hipStream_t, uint32_t, const oi_tab_args*, int, c
                                uint32_t, uint32_t, uint2*, uint32_t*, uint32_t, uint2*, uint32_t*, uint32_t, uint4*,
                                uint32_t, oi_detect_ctl*, uint32_t, uint32_t, int, hipEvent_t);
 hipError_t oi_launch_narrow(hipStream_t, uint32_t, const oi_tab_args*, const uint2*, const uint32_t*, uint32_t,
-                            uint4*, uint32_t*, uint32_t, oi_detect_ctl*, int, hipEvent_t);
+                            uint4*, uint32_t*, uint32_t, oi_detect_ctl*, int, uint2*, uint32_t, hipEvent_t);
 hipError_t oi_launch_bvtt_local(hipStream_t, const oi_tab_args*, const oi_local_args*, uint2*, uint32_t*, uint32_t,
                                 uint4*, uint32_t, oi_detect_ctl*, uint32_t, uint32_t, int);
 hipError_t oi_launch_pair_vertices(hipStream_t, const oi_build_desc*, const uint4*, uint32_t, float*);
+hipError_t oi_launch_zero_u32(hipStream_t, uint32_t*, uint32_t);
+hipError_t oi_launch_hits_digits(hipStream_t, const uint4*, uint32_t, uint32_t*, uint32_t*, const oi_hit_digits*);
+hipError_t oi_launch_hits_gather(hipStream_t, const uint4*, const uint32_t*, uint32_t, uint4*);
 }
 
 // ---------------------------------------------------------------------------
@@ -221,10 +224,13 @@ struct oi_detector {
     bool unchecked = false;
     bool have_result = false;
     uint32_t n_pairs = 0;
-    std::vector<uint4> h_sorted;  // the hits in get_pairs order, once fetched
-    bool sorted_ok = false;
-    uint4* d_sorted = nullptr;    // the same on the device (vertex export)
+    bool sorted_ok = false;       // d_sorted holds the checked hits in get_pairs order
+    uint4* d_sorted = nullptr;
     uint32_t d_sorted_cap = 0;
+    uint32_t* sort_words = nullptr;  // device sort of the hits: the four words of every hit
+    uint32_t* sort_vals = nullptr;   // two buffers of hit indices
+    uint32_t* sort_scratch = nullptr;  // histograms | tile counters | two look-back buffers
+    uint32_t sort_cap = 0;           // hits the sort buffers hold
     float* d_stage = nullptr;     // vertex export staging for host output
     uint64_t stage_cap = 0;
     oibvh_detect_stats stats{};
@@ -253,6 +259,8 @@ void oi_det_use_set(oi_detector* d, uint32_t k);
 oibvh_status oi_det_check(oi_detector* d);
 // sorted hits (4 words each)
 oibvh_status oi_det_get_pairs(oi_detector* d, uint32_t* out, uint64_t cap, uint32_t* n);
+// the same on the device (valid until the next detect)
+oibvh_status oi_det_sorted_device(oi_detector* d, const uint4** dptr, uint32_t* n);
 oibvh_status oi_det_level_counts(oi_detector* d, uint32_t* out, uint64_t cap, uint32_t* n);
 // six transformed vertices (18 floats) per hit in get_pairs order; geom[m] =
 // positions, faces and model matrix of mesh m (device table)

@@ -13,6 +13,7 @@ Meshes are returned as ``(xyz, faces)``: ``xyz`` float32 of shape (V, 3),
 """
 from __future__ import annotations
 
+import functools
 import math
 
 import numpy as np
@@ -71,8 +72,10 @@ def _normalize(v: np.ndarray) -> np.ndarray:
     return v / np.sqrt((v * v).sum(axis=1))[:, None]
 
 
+@functools.lru_cache(maxsize=None)
 def icosphere_unit(subdiv: int) -> tuple[np.ndarray, np.ndarray]:
-    """Unit icosphere in float64: V = 10*4^s + 2, F = 20*4^s."""
+    """Unit icosphere in float64: V = 10*4^s + 2, F = 20*4^s (cached: treat
+    the arrays as read-only)."""
     t = (1.0 + math.sqrt(5.0)) / 2.0
     v = np.array(
         [[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0],
@@ -336,6 +339,8 @@ class RigidScenes:
                 self.instances.append((s, index[key]))
         n = len(self.instances)
         self.n_scenes = n_scenes
+        self.seed = seed
+        self.noise = noise
         self.radius = radius
         self.R0 = np.stack([random_rotation(r) for _ in range(n)])
         self.t0 = r.uniform(3 * n, 0.0, 1.0).reshape(n, 3)
@@ -346,6 +351,16 @@ class RigidScenes:
     @property
     def triangles(self) -> int:
         return int(sum(self.library[b][1].shape[0] for _, b in self.instances))
+
+    def unique_mesh(self, i: int) -> tuple[np.ndarray, np.ndarray]:
+        """Instance i with geometry of its own: the base shape of its library
+        entry (same kind and size) with a noise draw seeded by the instance
+        index, so that no two meshes share vertex data."""
+        key = self.kinds[self.instances[i][1]]
+        vs = self.seed * 131 + 0x9E37 * (i + 1)
+        if key[0] == "ico":
+            return icosphere(key[1], 1.0, self.noise, vs)
+        return box(key[1], 0.6, self.noise, vs)
 
     def transforms(self, frame: int) -> np.ndarray:
         n = len(self.instances)

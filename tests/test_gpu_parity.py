@@ -929,3 +929,47 @@ def test_smoke_entry():
     import __graft_entry__
 
     __graft_entry__.smoke()
+
+
+def test_pairs_sorted_on_device(api, ctx, orc):
+    """The pair list sorted on the device (oibvh_scene_pairs_device) equals
+    the oracle's sorted pairs; face ids above 2^16 (three key bytes) and a
+    scene of more than 256 trees (two mesh-id bytes, many trees with no
+    pair); empty results."""
+    say("pairs sorted on the device")
+    (x0, f0), (x1, f1) = M.interpenetrating_icospheres(subdiv=6, offset=1.0)  # 81,920 faces each
+    t0, t1 = api.OibvhTree(ctx, x0, f0), api.OibvhTree(ctx, x1, f1)
+    t0.build()
+    t1.build()
+    sc = api.Scene(ctx)
+    sc.addOibvhTree(t0)
+    sc.addOibvhTree(t1)
+    sc.detectCollision(4, 3)
+    want = orc.detect([orc.Tree(x0, f0, order=t0.perm()), orc.Tree(x1, f1, order=t1.perm())])
+    assert len(want) > 1000 and want[:, 2].max() > 65536
+    p, n = sc.pairs_device()
+    assert n == len(want)
+    assert np.array_equal(api.download_u32(p, 4 * n).reshape(-1, 4), want)
+    assert np.array_equal(sc.getIntTriPairs(), want)
+    # 300 small trees in a row, neighbours touching
+    trees, shapes = [], []
+    for i in range(300):
+        x, f = M.icosphere(1, 0.6, 0.02, 900 + i, center=(0.9 * i, 0.0, 0.0))
+        t = api.OibvhTree(ctx, x, f)
+        t.build()
+        trees.append(t)
+        shapes.append((x, f))
+    big = api.Scene(ctx)
+    for t in trees:
+        big.addOibvhTree(t)
+    big.detectCollision(0, 2)
+    want = orc.detect([orc.Tree(x, f, order=t.perm()) for (x, f), t in zip(shapes, trees)])
+    assert len(want) > 0 and want[:, 1].max() > 256
+    p, n = big.pairs_device()
+    assert np.array_equal(api.download_u32(p, 4 * n).reshape(-1, 4), want)
+    # nothing collides
+    far = api.Scene(ctx)
+    far.addOibvhTree(trees[0])
+    far.addOibvhTree(trees[10])
+    far.detectCollision(0, 2)
+    assert far.pairs_device()[1] == 0 and len(far.getIntTriPairs()) == 0
