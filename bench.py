@@ -820,11 +820,13 @@ def run_self(args, ctx):
     progress("self: built; autotuning")
     tb = ctx.get_tuning()
     ctx.set_tuning(front_capacity=1 << 29)
-    # steps only: a sheet's fronts outgrow any workgroup's LDS, and a local
-    # walk that overflows on some frames costs them a re-run
+    # no head walk: a sheet's early fronts outgrow any workgroup's LDS. Tail
+    # walks of at most 128 front pairs per workgroup over the last one or two
+    # steps cannot overflow (two levels of 4 x 4 children: 16 x 128 = 2048
+    # pairs, the LDS queue) and keep the largest fronts out of HBM
     tuned = autotune.tune_batch(ctx, b, stage, entries=(4, 6), expands=(2, 3), blocks=(0,), leaves=(0,),
-                                local=(), tail=(), flags={"self_collide": True, "overlap": True},
-                                refit_blocks=(0, 1024), reps=4)
+                                local=(), tail=((32, 1), (64, 2), (128, 2)),
+                                flags={"self_collide": True, "overlap": True}, refit_blocks=(0, 1024), reps=4)
     progress(f"self: tuned {tuned}")
     entry, expand = tuned["entry_level"], tuned["expand_levels"]
     stage(0)
