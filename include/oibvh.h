@@ -277,6 +277,13 @@ OIBVH_API oibvh_status oibvh_ctx_get_tuning(const oibvh_ctx* ctx, oibvh_tuning* 
  * done). A tree in several scenes keeps the matrix of the last call. */
 OIBVH_API oibvh_status oibvh_scene_set_transforms(oibvh_scene* scene, const float* mats, uint32_t ptr_kind);
 
+/* Build every tree of the scene (OibvhTree::build on each, oibvhTree.cu:237-388;
+ * no reference counterpart as one call): the trees' build chains run side by
+ * side on the context's two streams. Results are those of oibvh_tree_build on
+ * every tree. Trees of a scene must be built before oibvh_scene_add_tree, so
+ * this rebuilds (after new positions or matrices). */
+OIBVH_API oibvh_status oibvh_scene_build(oibvh_scene* scene);
+
 /* Refit every tree of the scene (OibvhTree::refit on each, oibvhTree.cu:193-235)
  * with one launch per stage for up to four trees instead of one per tree.
  * Results are those of oibvh_tree_refit on every tree with its current

@@ -93,6 +93,7 @@ API = [
     ("oibvh_scene_get_stats", _i, [_vp, C.POINTER(DetectStats)]),
     ("oibvh_scene_get_level_counts", _i, [_vp, _u32p, _u64, _u32p]),
     ("oibvh_scene_refit", _i, [_vp]),
+    ("oibvh_scene_build", _i, [_vp]),
     ("oibvh_scene_set_transforms", _i, [_vp, _vp, _u32]),
     ("oibvh_batch_create", _i, [_vp, C.POINTER(_vp)]),
     ("oibvh_batch_destroy", None, [_vp]),

@@ -295,6 +295,10 @@ class Scene:
         """Refit every tree of the scene in one launch per stage."""
         _check(self.ctx.h, self._lib.oibvh_scene_refit(self.h), "scene refit")
 
+    def build(self) -> None:
+        """Rebuild every tree of the scene, two build chains side by side."""
+        _check(self.ctx.h, self._lib.oibvh_scene_build(self.h), "scene build")
+
     def set_transforms(self, mats=None, *, device: int | None = None, borrow: bool = False) -> None:
         """Model matrices of every tree (addOibvhTree order): `mats` is an
         (n, 4, 4) array in the usual row/column sense (host memory), or

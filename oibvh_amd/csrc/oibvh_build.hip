@@ -112,10 +112,9 @@ struct Xf {
 #pragma unroll
         for (int k = 0; k < 16; k++) m[k] = on ? oi_g(xf)[k] : 0.0f;
     }
-    __device__ __forceinline__ void vertex(const float* pos, uint32_t v, float* o) const
+    // a loaded position through the matrix (or as is)
+    __device__ __forceinline__ void apply(const float* p, float* o) const
     {
-        float p[3];
-        oi_ld3(pos, v, p);
         if (on) {
             oi_xform(m, p[0], p[1], p[2], o);
         } else {
@@ -123,6 +122,12 @@ struct Xf {
             o[1] = p[1];
             o[2] = p[2];
         }
+    }
+    __device__ __forceinline__ void vertex(const float* pos, uint32_t v, float* o) const
+    {
+        float p[3];
+        oi_ld3(pos, v, p);
+        apply(p, o);
     }
 };
 
@@ -157,13 +162,20 @@ k_mesh_box(BuildTab T, float* __restrict__ partials, uint32_t* __restrict__ clea
     const uint32_t sub = (chunk + OI_BLOCK - 1) / OI_BLOCK;
     const uint32_t s0 = b0 + tid * sub;
     const uint32_t s1 = min(b1, s0 + sub);
-    for (uint32_t v = s0; v < s1; v++) {
-        float p[3];
-        X.vertex(D.pos, v, p);
+    // the thread's run in index order, eight vertices loaded per round trip
+    // (positions past the run load vertex s0 and are not folded)
+    for (uint32_t v0 = s0; v0 < s1; v0 += 8u) {
+        float p[8][3];
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            mn[k] = lo_c(mn[k], p[k]);
-            mx[k] = hi_c(mx[k], p[k]);
+        for (int j = 0; j < 8; j++) X.vertex(D.pos, v0 + j < s1 ? v0 + j : s0, p[j]);
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            if (v0 + j >= s1) continue;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                mn[k] = lo_c(mn[k], p[j][k]);
+                mx[k] = hi_c(mx[k], p[j][k]);
+            }
         }
     }
 #pragma unroll
@@ -286,14 +298,33 @@ k_morton(BuildTab T, const float* __restrict__ partials, uint32_t* __restrict__ 
         oi_gw(D.box)[3 + tid] = bmx[tid];
     }
     const uint32_t f0 = it.y * OI_MORTON_CHUNK;
-    const uint32_t f1 = min(D.F, f0 + OI_MORTON_CHUNK);
-    for (uint32_t i = f0 + tid; i < f1; i += OI_BLOCK) {
-        uint32_t fv[3];
-        oi_ld3(D.faces, i, fv);
+    const uint32_t f1 = min(D.F, f0 + OI_MORTON_CHUNK);  // f0 < f1: a work item holds a face
+    // a thread's faces i = f0 + tid + 256 j: every face load is issued, then
+    // every vertex load, then the keys are formed (two rounds of memory
+    // latency per thread instead of two per face); faces past f1 load face f0
+    // and are dropped
+    constexpr int NJ = OI_MORTON_CHUNK / OI_BLOCK;
+    uint32_t fv[NJ][3];
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+        const uint32_t i = f0 + tid + (uint32_t)j * OI_BLOCK;
+        oi_ld3(D.faces, i < f1 ? i : f0, fv[j]);
+    }
+    float p[NJ][9];
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+        oi_ld3(D.pos, fv[j][0], &p[j][0]);
+        oi_ld3(D.pos, fv[j][1], &p[j][3]);
+        oi_ld3(D.pos, fv[j][2], &p[j][6]);
+    }
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+        const uint32_t i = f0 + tid + (uint32_t)j * OI_BLOCK;
+        if (i >= f1) continue;
         float a[3], b[3], c[3];
-        X.vertex(D.pos, fv[0], a);
-        X.vertex(D.pos, fv[1], b);
-        X.vertex(D.pos, fv[2], c);
+        X.apply(&p[j][0], a);
+        X.apply(&p[j][3], b);
+        X.apply(&p[j][6], c);
         float u[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
@@ -348,6 +379,7 @@ k_morton(BuildTab T, const float* __restrict__ partials, uint32_t* __restrict__ 
 #define LB_AGG (1u << 30)
 #define LB_PRE (2u << 30)
 #define LB_CNT 0x3fffffffu
+#define LB_WIN 8  // look-back statuses loaded per round trip
 
 __global__ void __launch_bounds__(OI_BLOCK)
 k_sort_pass(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
@@ -422,20 +454,38 @@ k_sort_pass(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ v
     uint32_t ltot;
     const uint32_t lstart = block_excl_scan(tot, s_wsum, ltot);
     s_lstart[tid] = lstart;
-    // decoupled look-back over preceding tiles for digit tid
+    // decoupled look-back over preceding tiles for digit tid: the statuses of
+    // LB_WIN tiles i, i - 1, ... are loaded in one round trip (the loads cross
+    // XCDs) and consumed in order, summing aggregates until an inclusive
+    // prefix; a tile not yet published ends the window, which is loaded
+    // again from there. Tile 0 publishes its prefix at once, so every walk
+    // ends there at the latest.
     uint32_t excl = 0;
     if (tile > 0) {
         int32_t i = (int32_t)tile - 1;
-        while (i >= 0) {
-            const uint32_t s = ld_acq_relaxed(lookback + (uint32_t)i * 256u + tid);
-            const uint32_t fl = s & ~LB_CNT;
-            if (fl == 0u) {
-                __builtin_amdgcn_s_sleep(1);
-                continue;
+        bool done = false;
+        while (!done) {
+            uint32_t st[LB_WIN];
+#pragma unroll
+            for (int w = 0; w < LB_WIN; w++) {
+                const int32_t t = i - w;
+                st[w] = t >= 0 ? ld_acq_relaxed(lookback + (uint32_t)t * 256u + tid) : 0u;
             }
-            excl += s & LB_CNT;
-            if (fl == LB_PRE) break;
-            i--;
+            int32_t used = LB_WIN;
+#pragma unroll
+            for (int w = 0; w < LB_WIN; w++) {
+                if (used != LB_WIN || done) continue;
+                const uint32_t fl = st[w] & ~LB_CNT;
+                if (fl == 0u) {
+                    used = w;
+                } else {
+                    excl += st[w] & LB_CNT;
+                    if (fl == LB_PRE) done = true;
+                }
+            }
+            if (done) break;
+            i -= used;
+            if (used != LB_WIN) __builtin_amdgcn_s_sleep(1);
         }
         st_relaxed(my, LB_PRE | (excl + tot));
     }

@@ -448,7 +448,8 @@ static oibvh_status launch_reduce(oibvh_tree* t, bool build)
 // scratch s: mesh box, Morton keys, stable LSD radix sort (values end in
 // d.perm), then the reduce in build mode (writes d.sfaces and d.nodes). No
 // allocation and no synchronisation, so oibvh_build_noalloc can share it.
-oibvh_status oi_enqueue_build(oibvh_ctx* c, const oi_build_desc& d, const oi_build_scratch& s, int M)
+oibvh_status oi_enqueue_build(oibvh_ctx* c, const oi_build_desc& d, const oi_build_scratch& s, int M,
+                              hipStream_t stream)
 {
     const uint32_t F = d.F;
     const uint32_t tiles = (F + oi_sort_tile() - 1) / oi_sort_tile();
@@ -456,7 +457,7 @@ oibvh_status oi_enqueue_build(oibvh_ctx* c, const oi_build_desc& d, const oi_bui
     uint32_t* ghist = s.sort;
     uint32_t* tctr = s.sort + OI_SORT_HIST;
     uint32_t* lb[2] = {s.sort + OI_SORT_HEAD, s.sort + OI_SORT_HEAD + 256ull * tiles};
-    hipStream_t st = c->stream;
+    hipStream_t st = stream ? stream : c->stream;
     // the mesh-box launch also clears the histograms, tile counters and the
     // first look-back buffer (a kernel, not a memset: capturable)
     HIPCHK(c, oi_launch_mesh_box(st, nullptr, nullptr, &d, d.nparts, d.box + 6, s.sort, OI_SORT_HEAD + 256u * tiles));
@@ -485,12 +486,10 @@ int oi_build_lpt(const oibvh_ctx* c, uint32_t F)
     return c->leaves_per_thread ? (int)c->leaves_per_thread : oi_default_lpt(F, c->cus);
 }
 
-extern "C" oibvh_status oibvh_tree_build(oibvh_tree* t)
+// the tree's build scratch (allocated at its first build)
+static oibvh_status build_scratch(oibvh_tree* t, oi_build_scratch* out)
 {
-    if (!t) return OIBVH_ERR_INVALID;
     oibvh_ctx* c = t->ctx;
-    HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, oi_ctx_join(c));
     const uint32_t F = t->F;
     const uint32_t tiles = (F + oi_sort_tile() - 1) / oi_sort_tile();
     const bool m63 = t->morton_bits == 63;
@@ -502,8 +501,19 @@ extern "C" oibvh_status oibvh_tree_build(oibvh_tree* t)
         HIPCHK(c, dalloc(&t->sort_scratch, OI_SORT_HEAD + 512ull * tiles));
     }
     if (m63 && !t->keys_hi) HIPCHK(c, dalloc(&t->keys_hi, F));
-    const oi_build_scratch s{t->keys, t->keys_alt, m63 ? t->keys_hi : nullptr, t->vals_tmp, t->sort_scratch};
-    oibvh_status st = oi_enqueue_build(c, tree_desc(t), s, pick_lpt(t));
+    *out = oi_build_scratch{t->keys, t->keys_alt, m63 ? t->keys_hi : nullptr, t->vals_tmp, t->sort_scratch};
+    return OIBVH_OK;
+}
+
+extern "C" oibvh_status oibvh_tree_build(oibvh_tree* t)
+{
+    if (!t) return OIBVH_ERR_INVALID;
+    oibvh_ctx* c = t->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, oi_ctx_join(c));
+    oi_build_scratch s;
+    oibvh_status st = build_scratch(t, &s);
+    if (st == OIBVH_OK) st = oi_enqueue_build(c, tree_desc(t), s, pick_lpt(t));
     if (st != OIBVH_OK) return st;
     t->built = true;
     return OIBVH_OK;
@@ -933,6 +943,45 @@ static oibvh_status scene_refit(oibvh_scene* s, const oi_seed_args* seed, bool* 
         if (na) HIPCHK(c, oi_launch_reduce_apex_inline(c->stream, M, apex, na, sd, last ? done : nullptr));
         if (last && done && !na) HIPCHK(c, hipEventRecord(done, c->stream));
         if (sd && seeded) *seeded = true;
+    }
+    return OIBVH_OK;
+}
+
+// OibvhTree::build on every tree of the scene (oibvhTree.cu:237-388), the
+// trees dealt alternately to the context's stream and its side stream: one
+// tree's build is a chain of small launches that leave most of the GPU idle,
+// so two chains run side by side. Results equal oibvh_tree_build on each.
+extern "C" oibvh_status oibvh_scene_build(oibvh_scene* s)
+{
+    if (!s) return OIBVH_ERR_INVALID;
+    oibvh_ctx* c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, oi_ctx_join(c));
+    std::vector<oi_build_scratch> scr(s->trees.size());
+    for (size_t i = 0; i < s->trees.size(); i++) {
+        oibvh_status r = build_scratch(s->trees[i], &scr[i]);
+        if (r != OIBVH_OK) return r;
+    }
+    const bool two = s->trees.size() > 1;
+    if (two) {
+        oibvh_status r = oi_ctx_side_init(c);
+        if (r != OIBVH_OK) return r;
+        // the side stream starts after everything queued before this call
+        HIPCHK(c, hipEventRecord(c->ev_main, c->stream));
+        HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_main, 0));
+    }
+    for (size_t i = 0; i < s->trees.size(); i++) {
+        oibvh_tree* t = s->trees[i];
+        oibvh_status r = oi_enqueue_build(c, tree_desc(t), scr[i], pick_lpt(t), (i & 1) ? c->side : c->stream);
+        if (r != OIBVH_OK) return r;
+        t->built = true;
+    }
+    if (two) {
+        // later work on the context's stream waits for the side stream's
+        // builds (oi_ctx_join at the next entry point)
+        const uint64_t q = c->side_seq++;
+        HIPCHK(c, hipEventRecord(c->ev_ring[q % OI_SIDE_RING], c->side));
+        c->side_pending = true;
     }
     return OIBVH_OK;
 }

@@ -134,7 +134,8 @@ struct oi_build_scratch {
 };
 // the build's launches over mesh d (d.box holds 6 + 6 * oi_box_parts(V)
 // floats) with M leaves per thread in the reduce; no allocation
-oibvh_status oi_enqueue_build(oibvh_ctx* c, const oi_build_desc& d, const oi_build_scratch& s, int M);
+oibvh_status oi_enqueue_build(oibvh_ctx* c, const oi_build_desc& d, const oi_build_scratch& s, int M,
+                              hipStream_t stream = nullptr);
 // mesh-box partials (workgroups of k_mesh_box) for V vertices, 1..256
 uint32_t oi_box_parts(uint32_t V);
 // reduce variant by launch size, and the one a build of F faces uses

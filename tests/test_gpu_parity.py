@@ -973,3 +973,37 @@ def test_pairs_sorted_on_device(api, ctx, orc):
     far.addOibvhTree(trees[10])
     far.detectCollision(0, 2)
     assert far.pairs_device()[1] == 0 and len(far.getIntTriPairs()) == 0
+
+
+def test_scene_build_equals_tree_builds(api, ctx, orc):
+    """oibvh_scene_build (the trees' build chains on the context's two
+    streams) gives every tree what oibvh_tree_build gives it: leaf order,
+    sorted faces and node bits; also right after overlapped frames (the side
+    stream busy), followed by a detect and more frames."""
+    say("scene build on two streams")
+    meshes = [M.icosphere(5, 1.0, 0.02, 60 + i, center=(0.8 * i, 0.0, 0.0)) for i in range(3)]
+    trees = [api.OibvhTree(ctx, x, f) for x, f in meshes]
+    for t in trees:
+        t.build()
+    sc = api.Scene(ctx)
+    for t in trees:
+        sc.addOibvhTree(t)
+    for k in range(3):
+        sc.frame(None, 2, 2, overlap=True, asynchronous=True)
+    moved = [(x * np.float32(1.0 + 0.01 * i)).astype(np.float32) for i, (x, _) in enumerate(meshes)]
+    for t, x in zip(trees, moved):
+        t.set_positions(x)
+    sc.build()
+    for t, x, (_, f) in zip(trees, moved, meshes):
+        check_tree(orc, t, x, f)
+    sc.detectCollision(2, 2)
+    ot = [orc.Tree(x, f, order=t.perm()) for t, x, (_, f) in zip(trees, moved, meshes)]
+    want = orc.detect(ot)
+    assert len(want) > 0
+    assert np.array_equal(sc.getIntTriPairs(), want)
+    sc.frame(None, 2, 2, overlap=True)
+    assert np.array_equal(sc.getIntTriPairs(), want)
+    one = api.Scene(ctx)
+    one.addOibvhTree(trees[0])
+    one.build()  # one tree: no side stream
+    check_tree(orc, trees[0], moved[0], meshes[0][1])
