@@ -138,7 +138,8 @@ struct Xf {
 // index order; +-inf are the identities, the default box (+-FLT_MAX) is folded
 // in first, as aabb_box_t's initial value (utils.h:17-18).
 // Work item (m, c): vertices [c * vchunk, (c + 1) * vchunk) of mesh m, partial
-// box c of the mesh.
+// box c of the mesh (at partials + 6 (pbase + c), or with partials null in the
+// mesh's own box buffer at box + 6 + 6 c).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float lo_c(float l, float r) { return (l < r) ? l : r; }
 __device__ __forceinline__ float hi_c(float l, float r) { return (r < l) ? l : r; }
@@ -194,7 +195,8 @@ k_mesh_box(BuildTab T, float* __restrict__ partials, uint32_t* __restrict__ clea
         }
         __syncthreads();
     }
-    if (tid < 6) oi_gw(partials)[6u * (D.pbase + it.y) + tid] = s_v[tid][0];
+    float* P = partials ? partials + 6u * D.pbase : D.box + 6;
+    if (tid < 6) oi_gw(P)[6u * it.y + tid] = s_v[tid][0];
 }
 
 // Fold the per-block partial boxes (at most OI_BLOCK of them) in block order and
@@ -251,9 +253,11 @@ __device__ __forceinline__ uint32_t quantise10(float p)
 }
 
 // Work item (m, c): faces [c * OI_MORTON_CHUNK, (c + 1) * OI_MORTON_CHUNK) of
-// mesh m. Keys go to keys[fbase + face]; the four 8-bit digit histograms of
-// all keys of the launch accumulate in ghist; fmesh (if not null) receives
-// the mesh id of each face (fmesh[fbase + face] = m).
+// mesh m. Keys go to keys[fbase + face]; the 8-bit digit histograms of the
+// keys accumulate in ghist + m * hstride (hstride 0: one set for all keys of
+// the launch); fmesh (if not null) receives the mesh id of each face
+// (fmesh[fbase + face] = m). The mesh-box partials are read from partials +
+// 6 pbase, or with partials null from the mesh's own box buffer (box + 6).
 // 63-bit form (no reference counterpart): 21 bits per axis, quantised as
 // min(max(p * 2^21, 0), 2^21 - 1), NaN -> 0
 __device__ __forceinline__ uint64_t spread21(uint64_t x)
@@ -275,11 +279,33 @@ __device__ __forceinline__ uint32_t quantise21(float p)
     return (q != q) ? 0u : (uint32_t)q;
 }
 
+// h[bin] += 1 for every active lane, one LDS atomic per run of consecutive
+// active lanes with the same bin: consecutive faces lie close together, so
+// the high digits of neighbouring lanes mostly agree, and atomics of a wave
+// to one address would serialise.
+__device__ __forceinline__ void hist_add(uint32_t* h, uint32_t bin, bool active)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    // the bin of the lane below (lane 0 reads its own; it is always a head)
+    const uint32_t below = (uint32_t)__shfl_up((int)bin, 1u, 64);
+    const unsigned long long act = __ballot(active);
+    const bool below_active = lane > 0 && ((act >> (lane - 1u)) & 1ull);
+    const bool head = active && (!below_active || below != bin);
+    const unsigned long long heads = __ballot(head);
+    if (head) {
+        // the run ends at the next head or the next inactive lane
+        const unsigned long long above = (lane == 63u) ? 0ull : ~0ull << (lane + 1u);
+        const unsigned long long stop = (heads | ~act) & above;
+        const uint32_t end = stop ? (uint32_t)__builtin_ctzll(stop) : 64u;
+        atomicAdd(&h[bin], end - lane);
+    }
+}
+
 // keys_hi == null: 30-bit keys and 4 digit histograms; else 63-bit keys split
 // into keys (low word) and keys_hi (high word) and 8 digit histograms
 __global__ void __launch_bounds__(OI_BLOCK)
 k_morton(BuildTab T, const float* __restrict__ partials, uint32_t* __restrict__ keys, uint32_t* __restrict__ keys_hi,
-         uint32_t* __restrict__ ghist, uint32_t* __restrict__ fmesh)
+         uint32_t* __restrict__ ghist, uint32_t hstride, uint32_t* __restrict__ fmesh)
 {
     __shared__ uint32_t s_h[8][256];
     __shared__ float s_v[6][OI_BLOCK];
@@ -292,7 +318,8 @@ k_morton(BuildTab T, const float* __restrict__ partials, uint32_t* __restrict__ 
 #pragma unroll
     for (int d = 0; d < 8; d++) s_h[d][tid] = 0u;
     float bmn[3], bmx[3];
-    fold_partials(partials + 6u * D.pbase, D.nparts, s_v, bmn, bmx);
+    fold_partials(partials ? partials + 6u * D.pbase : D.box + 6, D.nparts, s_v, bmn, bmx);
+    ghist += it.x * hstride;
     if (it.y == 0 && tid < 3) {
         oi_gw(D.box)[tid] = bmn[tid];
         oi_gw(D.box)[3 + tid] = bmx[tid];
@@ -320,7 +347,7 @@ k_morton(BuildTab T, const float* __restrict__ partials, uint32_t* __restrict__ 
 #pragma unroll
     for (int j = 0; j < NJ; j++) {
         const uint32_t i = f0 + tid + (uint32_t)j * OI_BLOCK;
-        if (i >= f1) continue;
+        const bool live = i < f1;
         float a[3], b[3], c[3];
         X.apply(&p[j][0], a);
         X.apply(&p[j][3], b);
@@ -341,17 +368,20 @@ k_morton(BuildTab T, const float* __restrict__ partials, uint32_t* __restrict__ 
                                  spread21(quantise21(u[2]));
             key = (uint32_t)k64;
             khi = (uint32_t)(k64 >> 32);
-            oi_gw(keys_hi)[D.fbase + i] = khi;
+            if (live) oi_gw(keys_hi)[D.fbase + i] = khi;
         } else {
             key = (spread10(quantise10(u[0])) << 2) | (spread10(quantise10(u[1])) << 1) | spread10(quantise10(u[2]));
         }
-        oi_gw(keys)[D.fbase + i] = key;
-        if (fmesh) oi_gw(fmesh)[D.fbase + i] = it.x;
+        if (live) {
+            oi_gw(keys)[D.fbase + i] = key;
+            if (fmesh) oi_gw(fmesh)[D.fbase + i] = it.x;
+        }
+        // every lane of the wave takes part (ballots), dead lanes add nothing
 #pragma unroll
-        for (int d = 0; d < 4; d++) atomicAdd(&s_h[d][(key >> (8 * d)) & 0xffu], 1u);
+        for (int d = 0; d < 4; d++) hist_add(s_h[d], (key >> (8 * d)) & 0xffu, live);
         if (keys_hi) {
 #pragma unroll
-            for (int d = 0; d < 4; d++) atomicAdd(&s_h[4 + d][(khi >> (8 * d)) & 0xffu], 1u);
+            for (int d = 0; d < 4; d++) hist_add(s_h[4 + d], (khi >> (8 * d)) & 0xffu, live);
         }
     }
     __syncthreads();
@@ -362,51 +392,107 @@ k_morton(BuildTab T, const float* __restrict__ partials, uint32_t* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// k_sort_pass: one stable 8-bit LSD pass, onesweep style.
-// A tile is 4 waves x 16 rounds x 64 lanes = 4096 keys in input order (wave w
-// owns keys [1024w, 1024w + 1024) of the tile, round j lane l holds key
-// 1024w + 64j + l). Keys are ranked per wave with 8 ballots per round
-// (stable: earlier rounds and lower lanes come first), staged through LDS in
-// tile-local sorted order and written out in runs. Tiles take their id from an
-// atomic counter, so every tile a look-back waits on is already running.
+// k_sort_pass: one stable 8-bit LSD pass, onesweep style, over up to
+// OI_SORT_SEGS independent segments of the concatenated key arrays (one per
+// tree when several trees are built in one launch).
+// A tile is 16 waves x 8 rounds x 64 lanes = 8192 keys of one segment in
+// input order (wave w owns keys [512w, 512w + 512) of the tile, round j lane l
+// holds key 512w + 64j + l): fewer, larger tiles than one wave per SIMD would
+// give shorten the look-back chains. Keys are ranked per wave with 8 ballots
+// per round (stable: earlier rounds and lower lanes come first), staged
+// through LDS in tile-local sorted order and written out in runs. Segment s
+// owns the workgroups and tile ids [tile0[s], tile0[s + 1]); a workgroup takes
+// its tile from its segment's atomic counter (tile_counter[s]), so every tile
+// a look-back waits on is already running, and the look-back ends at the
+// segment's own first tile.
 // Look-back status word: bits 31:30 = 1 aggregate / 2 inclusive prefix,
-// bits 29:0 = count. With key_of_val the key of a value v is key_of_val[v]
-// (a batch's mesh-id passes: a stable partition of the Morton-sorted faces
-// of all meshes back into their meshes).
+// bits 29:0 = count. A digit that no key of the segment has (its histogram
+// entry is 0) is neither published nor looked back on. Values default to the
+// key's index in the concatenated arrays. With key_of_val the key of a value v
+// is key_of_val[v] (a batch's mesh-id passes: a stable partition of the
+// Morton-sorted faces of all meshes back into their meshes).
 // ---------------------------------------------------------------------------
-#define RS_ITEMS 16
-#define RS_TILE (OI_BLOCK * RS_ITEMS)
+#define RS_WAVES 16
+#define RS_THREADS (64 * RS_WAVES)
+#define RS_ITEMS 8
+#define RS_TILE (RS_THREADS * RS_ITEMS)
 #define LB_AGG (1u << 30)
 #define LB_PRE (2u << 30)
 #define LB_CNT 0x3fffffffu
 #define LB_WIN 8  // look-back statuses loaded per round trip
+#define OI_SORT_SEGS 4
 
-__global__ void __launch_bounds__(OI_BLOCK)
+struct SortSegs {
+    uint32_t ns;                        // segments, 1..OI_SORT_SEGS
+    uint32_t hstride;                   // words between the segments' digit histograms
+    uint32_t tile0[OI_SORT_SEGS + 1];   // segment s owns tiles [tile0[s], tile0[s + 1])
+    uint32_t base[OI_SORT_SEGS];        // its first key in the concatenated arrays
+    uint32_t n[OI_SORT_SEGS];           // its keys
+};
+
+// block-wide exclusive scan over NW waves (one uint32 per thread)
+template <int NW>
+__device__ __forceinline__ uint32_t block_excl_scan_w(uint32_t v, uint32_t* s_wsum, uint32_t& total)
+{
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t x = oi_wave_scan_incl(v);
+    if (lane == 63) s_wsum[wave] = x;
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+        const uint32_t t = s_wsum[w];
+        before += (w < (int)wave) ? t : 0u;
+        tot += t;
+    }
+    total = tot;
+    __syncthreads();
+    return before + x - v;
+}
+
+__global__ void __launch_bounds__(RS_THREADS)
 k_sort_pass(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
-            uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, uint32_t n, uint32_t shift,
+            uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, SortSegs S, uint32_t shift,
             const uint32_t* __restrict__ ghist, uint32_t* lookback, uint32_t* lookback_next,
             uint32_t* tile_counter, const uint32_t* __restrict__ key_of_val)
 {
     __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_whist[4][256];
+    __shared__ uint32_t s_whist[RS_WAVES][256];
     __shared__ uint32_t s_lstart[256];
     __shared__ uint32_t s_gbase[256];
-    __shared__ uint32_t s_wsum[4];
+    __shared__ uint32_t s_wsum[RS_WAVES];
     __shared__ uint32_t s_keys[RS_TILE];
     __shared__ uint32_t s_vals[RS_TILE];
 
+    // threads tid < 256 also own digit tid
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    if (tid == 0) s_tile = __hip_atomic_fetch_add(tile_counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const bool dig = tid < 256u;
+    // the workgroup's segment (constant indices keep the argument block out
+    // of scratch memory)
+    uint32_t seg = 0, seg0 = S.tile0[0], sbase = S.base[0], n = S.n[0];
 #pragma unroll
-    for (int w = 0; w < 4; w++) s_whist[w][tid] = 0u;
+    for (int k = 1; k < OI_SORT_SEGS; k++) {
+        if (k < (int)S.ns && blockIdx.x >= S.tile0[k]) {
+            seg = k;
+            seg0 = S.tile0[k];
+            sbase = S.base[k];
+            n = S.n[k];
+        }
+    }
+    if (tid == 0)
+        s_tile = __hip_atomic_fetch_add(tile_counter + seg, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // the histogram entry is loaded while the tile id is in flight
+    const uint32_t hcnt = dig ? ghist[seg * S.hstride + tid] : 0u;
+    for (uint32_t i = tid; i < RS_WAVES * 256u; i += RS_THREADS) (&s_whist[0][0])[i] = 0u;
     __syncthreads();
     const uint32_t tile = s_tile;
+    const uint32_t gt = seg0 + tile;  // tile id over all segments
     const uint32_t base = tile * RS_TILE;
-    if (lookback_next) lookback_next[tile * 256u + tid] = 0u;
+    if (lookback_next && dig) lookback_next[gt * 256u + tid] = 0u;
 
-    // exclusive prefix of the global digit histogram
+    // exclusive prefix of the segment's digit histogram
     uint32_t gtot;
-    const uint32_t gexcl = block_excl_scan(ghist[tid], s_wsum, gtot);
+    const uint32_t gexcl = block_excl_scan_w<RS_WAVES>(hcnt, s_wsum, gtot);
 
     uint32_t k[RS_ITEMS], v[RS_ITEMS], rk[RS_ITEMS];
     const uint32_t wbase = base + wave * (64u * RS_ITEMS);
@@ -414,8 +500,9 @@ k_sort_pass(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ v
     for (int j = 0; j < RS_ITEMS; j++) {
         const uint32_t idx = wbase + 64u * j + lane;
         const bool ok = idx < n;
-        v[j] = ok ? (vals_in ? vals_in[idx] : idx) : 0u;
-        k[j] = ok ? (key_of_val ? key_of_val[v[j]] : keys_in[idx]) : 0u;
+        const uint32_t gi = sbase + idx;
+        v[j] = ok ? (vals_in ? vals_in[gi] : gi) : 0u;
+        k[j] = ok ? (key_of_val ? key_of_val[v[j]] : keys_in[gi]) : 0u;
     }
     const unsigned long long lt = (1ull << lane) - 1ull;
 #pragma unroll
@@ -439,30 +526,32 @@ k_sort_pass(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ v
     __syncthreads();
     // per-wave exclusive offsets and tile totals (thread tid owns digit tid)
     uint32_t tot = 0;
+    if (dig) {
 #pragma unroll
-    for (int w = 0; w < 4; w++) {
-        const uint32_t c = s_whist[w][tid];
-        s_whist[w][tid] = tot;
-        tot += c;
+        for (int w = 0; w < RS_WAVES; w++) {
+            const uint32_t c = s_whist[w][tid];
+            s_whist[w][tid] = tot;
+            tot += c;
+        }
     }
-    // publish this tile's aggregate (or inclusive prefix for tile 0)
-    uint32_t* my = lookback + tile * 256u + tid;
-    if (tile == 0)
-        st_relaxed(my, LB_PRE | tot);
-    else
-        st_relaxed(my, LB_AGG | tot);
+    // publish this tile's aggregate (or inclusive prefix for the segment's
+    // first tile)
+    uint32_t* my = lookback + gt * 256u + tid;
+    const bool live = dig && hcnt != 0u;
+    if (live) st_relaxed(my, (tile == 0 ? LB_PRE : LB_AGG) | tot);
     uint32_t ltot;
-    const uint32_t lstart = block_excl_scan(tot, s_wsum, ltot);
-    s_lstart[tid] = lstart;
-    // decoupled look-back over preceding tiles for digit tid: the statuses of
-    // LB_WIN tiles i, i - 1, ... are loaded in one round trip (the loads cross
-    // XCDs) and consumed in order, summing aggregates until an inclusive
-    // prefix; a tile not yet published ends the window, which is loaded
-    // again from there. Tile 0 publishes its prefix at once, so every walk
-    // ends there at the latest.
+    const uint32_t lstart = block_excl_scan_w<RS_WAVES>(tot, s_wsum, ltot);
+    if (dig) s_lstart[tid] = lstart;
+    // decoupled look-back over the segment's preceding tiles for digit tid:
+    // the statuses of LB_WIN tiles i, i - 1, ... are loaded in one round trip
+    // (the loads cross XCDs) and consumed in order, summing aggregates until
+    // an inclusive prefix; a tile not yet published ends the window, which is
+    // loaded again from there. The segment's first tile publishes its prefix
+    // at once, so every walk ends there at the latest (statuses loaded beyond
+    // it are not consumed).
     uint32_t excl = 0;
-    if (tile > 0) {
-        int32_t i = (int32_t)tile - 1;
+    if (tile > 0 && live) {
+        int32_t i = (int32_t)gt - 1;
         bool done = false;
         while (!done) {
             uint32_t st[LB_WIN];
@@ -489,7 +578,7 @@ k_sort_pass(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ v
         }
         st_relaxed(my, LB_PRE | (excl + tot));
     }
-    s_gbase[tid] = gexcl + excl;
+    if (dig) s_gbase[tid] = sbase + gexcl + excl;
     __syncthreads();
     // stage in tile-local sorted order
 #pragma unroll
@@ -504,7 +593,7 @@ k_sort_pass(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ v
     }
     __syncthreads();
     const uint32_t cnt_tile = (n - base < RS_TILE) ? (n - base) : RS_TILE;
-    for (uint32_t i = tid; i < cnt_tile; i += OI_BLOCK) {
+    for (uint32_t i = tid; i < cnt_tile; i += RS_THREADS) {
         const uint32_t key = s_keys[i];
         const uint32_t d = (key >> shift) & 0xffu;
         const uint32_t g = s_gbase[d] + (i - s_lstart[d]);
@@ -1193,7 +1282,29 @@ hipError_t oi_launch_morton(hipStream_t st, const oi_build_desc* d, const uint2*
                             uint32_t* ghist, uint32_t* fmesh)
 {
     hipLaunchKernelGGL(k_morton, dim3(nblocks), dim3(OI_BLOCK), 0, st, make_btab(d, work, one), partials, keys,
-                       keys_hi, ghist, fmesh);
+                       keys_hi, ghist, 0u, fmesh);
+    return hipGetLastError();
+}
+
+// n (<= OI_INLINE_MESHES) meshes in the kernel arguments, mesh m owning the
+// workgroups [boff[m], boff[m + 1]): the mesh boxes (partials in each mesh's
+// own box buffer), and the Morton keys with mesh m's digit histograms at
+// ghist + m * hstride
+hipError_t oi_launch_mesh_box_inline(hipStream_t st, const oi_build_desc* meshes, uint32_t n, const uint32_t* boff,
+                                     uint32_t* clear_u32, uint32_t clear_n)
+{
+    if (n == 0 || n > OI_INLINE_MESHES) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_mesh_box, dim3(boff[n]), dim3(OI_BLOCK), 0, st, make_btab_inline(meshes, n, boff), nullptr,
+                       clear_u32, clear_n);
+    return hipGetLastError();
+}
+
+hipError_t oi_launch_morton_inline(hipStream_t st, const oi_build_desc* meshes, uint32_t n, const uint32_t* boff,
+                                   uint32_t* keys, uint32_t* keys_hi, uint32_t* ghist, uint32_t hstride)
+{
+    if (n == 0 || n > OI_INLINE_MESHES) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_morton, dim3(boff[n]), dim3(OI_BLOCK), 0, st, make_btab_inline(meshes, n, boff), nullptr,
+                       keys, keys_hi, ghist, hstride, nullptr);
     return hipGetLastError();
 }
 
@@ -1205,13 +1316,48 @@ hipError_t oi_launch_set_mesh_pos(hipStream_t st, oi_build_desc* bd, oi_mesh_des
 
 uint32_t oi_sort_tile() { return RS_TILE; }
 
+hipError_t oi_launch_sort_pass_segs(hipStream_t st, const uint32_t* keys_in, const uint32_t* vals_in,
+                                    uint32_t* keys_out, uint32_t* vals_out, uint32_t ns, const uint32_t* base,
+                                    const uint32_t* n, uint32_t shift, const uint32_t* ghist, uint32_t hstride,
+                                    uint32_t* lookback, uint32_t* lookback_next, uint32_t* tile_counter,
+                                    const uint32_t* key_of_val);
+
 hipError_t oi_launch_sort_pass(hipStream_t st, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
                                uint32_t* vals_out, uint32_t n, uint32_t shift, const uint32_t* ghist,
                                uint32_t* lookback, uint32_t* lookback_next, uint32_t* tile_counter,
                                const uint32_t* key_of_val)
 {
-    const uint32_t tiles = (n + RS_TILE - 1) / RS_TILE;
-    hipLaunchKernelGGL(k_sort_pass, dim3(tiles), dim3(OI_BLOCK), 0, st, keys_in, vals_in, keys_out, vals_out, n,
+    const uint32_t base = 0;
+    return oi_launch_sort_pass_segs(st, keys_in, vals_in, keys_out, vals_out, 1, &base, &n, shift, ghist, 0,
+                                    lookback, lookback_next, tile_counter, key_of_val);
+}
+
+uint32_t oi_sort_segs_max() { return OI_SORT_SEGS; }
+
+// ns segments: segment s is keys [base[s], base[s] + n[s]) of the arrays, its
+// digit histogram at ghist + s * hstride, its tile counter tile_counter[s];
+// look-back statuses are indexed by the tile id over all segments (sum of
+// ceil(n[s] / tile) tiles)
+hipError_t oi_launch_sort_pass_segs(hipStream_t st, const uint32_t* keys_in, const uint32_t* vals_in,
+                                    uint32_t* keys_out, uint32_t* vals_out, uint32_t ns, const uint32_t* base,
+                                    const uint32_t* n, uint32_t shift, const uint32_t* ghist, uint32_t hstride,
+                                    uint32_t* lookback, uint32_t* lookback_next, uint32_t* tile_counter,
+                                    const uint32_t* key_of_val)
+{
+    if (ns == 0 || ns > OI_SORT_SEGS) return hipErrorInvalidValue;
+    SortSegs S{};
+    S.ns = ns;
+    S.hstride = hstride;
+    uint32_t tiles = 0;
+    for (uint32_t k = 0; k < ns; k++) {
+        if (n[k] == 0) return hipErrorInvalidValue;
+        S.tile0[k] = tiles;
+        S.base[k] = base[k];
+        S.n[k] = n[k];
+        tiles += (n[k] + RS_TILE - 1) / RS_TILE;
+    }
+    S.tile0[ns] = tiles;
+    hipLaunchKernelGGL(k_sort_pass, dim3(tiles), dim3(RS_THREADS), 0, st, keys_in, vals_in, keys_out, vals_out, S,
                        shift, ghist, lookback, lookback_next, tile_counter, key_of_val);
     return hipGetLastError();
 }

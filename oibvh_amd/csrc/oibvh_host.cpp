@@ -132,6 +132,13 @@ struct oibvh_scene {
     uint32_t front_key[3] = {~0u, ~0u, ~0u}; // entry, flags&SELF, #trees
     oi_detector det;
     uint32_t last_entry = 0, last_expand = 1, last_flags = 0;
+    // scratch of oibvh_scene_build, grown on demand: per-face keys, alternate
+    // keys, values, sorted values and 63-bit high words over the concatenated
+    // faces of up to four trees, and the sort's digit histograms (one set per
+    // tree), pass counters and two look-back buffers
+    uint32_t *b_keys = nullptr, *b_keys_alt = nullptr, *b_vals_tmp = nullptr, *b_vals = nullptr;
+    uint32_t *b_keys_hi = nullptr, *b_sort = nullptr;
+    uint64_t b_cap = 0, b_hi_cap = 0, b_sort_cap = 0;
 };
 
 // ---------------------------------------------------------------- context
@@ -282,7 +289,7 @@ static oibvh_status tree_alloc_common(oibvh_tree* t)
     return OIBVH_OK;
 }
 
-uint32_t oi_box_parts(uint32_t V) { return std::max(1u, std::min(256u, (V + 4095u) / 4096u)); }
+uint32_t oi_box_parts(uint32_t V) { return std::max(1u, std::min(256u, (V + 2047u) / 2048u)); }
 
 // the tree as a one-mesh build table
 static oi_build_desc tree_desc(const oibvh_tree* t)
@@ -681,6 +688,12 @@ extern "C" void oibvh_scene_destroy(oibvh_scene* s)
     dfree(s->d_desc);
     dfree(s->d_geom);
     dfree(s->d_xf);
+    dfree(s->b_keys);
+    dfree(s->b_keys_alt);
+    dfree(s->b_vals_tmp);
+    dfree(s->b_vals);
+    dfree(s->b_keys_hi);
+    dfree(s->b_sort);
     for (hipEvent_t e : s->ev_refit) (void)hipEventDestroy(e);
     for (hipEvent_t e : s->ev_detect) (void)hipEventDestroy(e);
     oi_det_free(&s->det);
@@ -947,41 +960,133 @@ static oibvh_status scene_refit(oibvh_scene* s, const oi_seed_args* seed, bool* 
     return OIBVH_OK;
 }
 
-// OibvhTree::build on every tree of the scene (oibvhTree.cu:237-388), the
-// trees dealt alternately to the context's stream and its side stream: one
+// trees per launch of the scene build; tile counters of its sort (one per
+// pass and tree)
+#define OI_SCENE_BUILD_GROUP 4
+#define OI_SCENE_CTRS (8u * OI_SCENE_BUILD_GROUP)
+
+// The scene build's scratch for a group of n trees with F faces and `tiles`
+// sort tiles in all (grown on demand, after the context's streams drain).
+static oibvh_status scene_build_scratch(oibvh_scene* s, uint64_t F, uint32_t n, uint32_t tiles, bool m63)
+{
+    oibvh_ctx* c = s->ctx;
+    const uint64_t words = (uint64_t)n * OI_SORT_HIST + OI_SCENE_CTRS + 512ull * tiles;
+    const bool grow = F > s->b_cap || words > s->b_sort_cap || (m63 && F > s->b_hi_cap);
+    if (!grow) return OIBVH_OK;
+    HIPCHK(c, oi_ctx_quiesce(c));
+    if (F > s->b_cap) {
+        dfree(s->b_keys);
+        dfree(s->b_keys_alt);
+        dfree(s->b_vals_tmp);
+        dfree(s->b_vals);
+        s->b_cap = 0;
+        HIPCHK(c, dalloc(&s->b_keys, F));
+        HIPCHK(c, dalloc(&s->b_keys_alt, F));
+        HIPCHK(c, dalloc(&s->b_vals_tmp, F));
+        HIPCHK(c, dalloc(&s->b_vals, F));
+        s->b_cap = F;
+    }
+    if (m63 && F > s->b_hi_cap) {
+        dfree(s->b_keys_hi);
+        s->b_hi_cap = 0;
+        HIPCHK(c, dalloc(&s->b_keys_hi, F));
+        s->b_hi_cap = F;
+    }
+    if (words > s->b_sort_cap) {
+        dfree(s->b_sort);
+        s->b_sort_cap = 0;
+        HIPCHK(c, dalloc(&s->b_sort, words));
+        s->b_sort_cap = words;
+    }
+    return OIBVH_OK;
+}
+
+// OibvhTree::build (oibvhTree.cu:237-388) of n <= 4 trees with the same
+// Morton width, every stage one launch for all of them: mesh boxes, Morton
+// keys over the concatenated faces (each tree's own digit histograms), the
+// LSD passes sorting each tree's keys as a segment of their own, and the
+// reduce, which also writes each tree's permutation from the sorted values.
+static oibvh_status scene_build_group(oibvh_scene* s, oibvh_tree* const* trees, uint32_t n)
+{
+    oibvh_ctx* c = s->ctx;
+    const bool m63 = trees[0]->morton_bits == 63;
+    const uint32_t tile = oi_sort_tile(), chunk = oi_morton_chunk();
+    oi_build_desc d[OI_SCENE_BUILD_GROUP];
+    uint32_t box_off[OI_SCENE_BUILD_GROUP + 1] = {0}, mort_off[OI_SCENE_BUILD_GROUP + 1] = {0};
+    uint32_t base[OI_SCENE_BUILD_GROUP], nf[OI_SCENE_BUILD_GROUP];
+    uint64_t F = 0;
+    uint32_t tiles = 0;
+    for (uint32_t k = 0; k < n; k++) {
+        d[k] = tree_desc(trees[k]);
+        d[k].fbase = (uint32_t)F;
+        base[k] = (uint32_t)F;
+        nf[k] = d[k].F;
+        box_off[k + 1] = box_off[k] + d[k].nparts;
+        mort_off[k + 1] = mort_off[k] + (d[k].F + chunk - 1) / chunk;
+        tiles += (d[k].F + tile - 1) / tile;
+        F += d[k].F;
+    }
+    if (F > 0xffffffffull) return oi_set_err(c, OIBVH_ERR_CAPACITY, "scene build: too many faces in one group");
+    oibvh_status r = scene_build_scratch(s, F, n, tiles, m63);
+    if (r != OIBVH_OK) return r;
+    hipStream_t st = c->stream;
+    uint32_t* ghist = s->b_sort;
+    uint32_t* ctrs = ghist + (uint64_t)n * OI_SORT_HIST;
+    uint32_t* lb[2] = {ctrs + OI_SCENE_CTRS, ctrs + OI_SCENE_CTRS + 256ull * tiles};
+    // the mesh-box launch also clears the histograms, counters and the first
+    // look-back buffer (a kernel, not a memset: capturable)
+    HIPCHK(c, oi_launch_mesh_box_inline(st, d, n, box_off, ghist, n * OI_SORT_HIST + OI_SCENE_CTRS + 256u * tiles));
+    HIPCHK(c, oi_launch_morton_inline(st, d, n, mort_off, s->b_keys, m63 ? s->b_keys_hi : nullptr, ghist,
+                                      OI_SORT_HIST));
+    // LSD passes over the low words, then, for 63-bit keys, over the high
+    // words looked up by value; the sorted values (face ids over the
+    // concatenated faces) end in b_vals
+    const int np = m63 ? 8 : 4;
+    for (int p = 0; p < np; p++) {
+        const bool lo = p < 4;
+        const uint32_t* kin = lo ? ((p & 1) ? s->b_keys_alt : s->b_keys) : nullptr;
+        uint32_t* kout = (lo && p < 3) ? ((p & 1) ? s->b_keys : s->b_keys_alt) : nullptr;
+        const uint32_t* vin = p == 0 ? nullptr : ((p & 1) ? s->b_vals_tmp : s->b_vals);
+        uint32_t* vout = (p & 1) ? s->b_vals : s->b_vals_tmp;
+        HIPCHK(c, oi_launch_sort_pass_segs(st, kin, vin, kout, vout, n, base, nf, 8u * (p & 3), ghist + 256 * p,
+                                           OI_SORT_HIST, lb[p & 1], p + 1 < np ? lb[(p + 1) & 1] : nullptr,
+                                           ctrs + p * OI_SCENE_BUILD_GROUP, lo ? nullptr : s->b_keys_hi));
+    }
+    int M = (int)c->leaves_per_thread;
+    if (!M) M = oi_default_lpt(F, c->cus);
+    uint32_t red_off[OI_SCENE_BUILD_GROUP + 1] = {0}, na = 0;
+    oi_build_desc apex[OI_SCENE_BUILD_GROUP];
+    for (uint32_t k = 0; k < n; k++) {
+        const uint32_t h = oi_reduce_height(M, d[k].L);
+        red_off[k + 1] = red_off[k] + oi_lvl_cnt(d[k].L - h, d[k].L, d[k].vl);
+        if (d[k].L > h) apex[na++] = d[k];
+    }
+    HIPCHK(c, oi_launch_reduce_inline(st, M, d, n, red_off, s->b_vals, 1, nullptr, nullptr, 0));
+    if (na) HIPCHK(c, oi_launch_reduce_apex_inline(st, M, apex, na, nullptr, nullptr));
+    for (uint32_t k = 0; k < n; k++) trees[k]->built = true;
+    return OIBVH_OK;
+}
+
+// OibvhTree::build on every tree of the scene (oibvhTree.cu:237-388). A single
 // tree's build is a chain of small launches that leave most of the GPU idle,
-// so two chains run side by side. Results equal oibvh_tree_build on each.
+// so up to four consecutive trees of the same Morton width share each launch
+// of the chain. Results equal oibvh_tree_build on each tree.
 extern "C" oibvh_status oibvh_scene_build(oibvh_scene* s)
 {
     if (!s) return OIBVH_ERR_INVALID;
     oibvh_ctx* c = s->ctx;
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, oi_ctx_join(c));
-    std::vector<oi_build_scratch> scr(s->trees.size());
-    for (size_t i = 0; i < s->trees.size(); i++) {
-        oibvh_status r = build_scratch(s->trees[i], &scr[i]);
+    const uint32_t per = std::min<uint32_t>(OI_SCENE_BUILD_GROUP, std::min(oi_inline_meshes(), oi_sort_segs_max()));
+    size_t i = 0;
+    while (i < s->trees.size()) {
+        uint32_t n = 1;
+        while (n < per && i + n < s->trees.size() &&
+               s->trees[i + n]->morton_bits == s->trees[i]->morton_bits)
+            n++;
+        oibvh_status r = scene_build_group(s, &s->trees[i], n);
         if (r != OIBVH_OK) return r;
-    }
-    const bool two = s->trees.size() > 1;
-    if (two) {
-        oibvh_status r = oi_ctx_side_init(c);
-        if (r != OIBVH_OK) return r;
-        // the side stream starts after everything queued before this call
-        HIPCHK(c, hipEventRecord(c->ev_main, c->stream));
-        HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_main, 0));
-    }
-    for (size_t i = 0; i < s->trees.size(); i++) {
-        oibvh_tree* t = s->trees[i];
-        oibvh_status r = oi_enqueue_build(c, tree_desc(t), scr[i], pick_lpt(t), (i & 1) ? c->side : c->stream);
-        if (r != OIBVH_OK) return r;
-        t->built = true;
-    }
-    if (two) {
-        // later work on the context's stream waits for the side stream's
-        // builds (oi_ctx_join at the next entry point)
-        const uint64_t q = c->side_seq++;
-        HIPCHK(c, hipEventRecord(c->ev_ring[q % OI_SIDE_RING], c->side));
-        c->side_pending = true;
+        i += n;
     }
     return OIBVH_OK;
 }

@@ -18,6 +18,10 @@ extern "C" {
 hipError_t oi_launch_mesh_box(hipStream_t, const oi_build_desc*, const uint2*, const oi_build_desc*, uint32_t, float*,
                               uint32_t*, uint32_t);
 uint32_t oi_morton_chunk();
+hipError_t oi_launch_mesh_box_inline(hipStream_t, const oi_build_desc*, uint32_t, const uint32_t*, uint32_t*,
+                                     uint32_t);
+hipError_t oi_launch_morton_inline(hipStream_t, const oi_build_desc*, uint32_t, const uint32_t*, uint32_t*,
+                                   uint32_t*, uint32_t*, uint32_t);
 hipError_t oi_launch_morton(hipStream_t, const oi_build_desc*, const uint2*, const oi_build_desc*, uint32_t,
                             const float*, uint32_t*, uint32_t*, uint32_t*, uint32_t*);
 
@@ -29,6 +33,10 @@ hipError_t oi_launch_morton(hipStream_t, const oi_build_desc*, const uint2*, con
 uint32_t oi_sort_tile();
 hipError_t oi_launch_sort_pass(hipStream_t, const uint32_t*, const uint32_t*, uint32_t*, uint32_t*, uint32_t,
                                uint32_t, const uint32_t*, uint32_t*, uint32_t*, uint32_t*, const uint32_t*);
+uint32_t oi_sort_segs_max();
+hipError_t oi_launch_sort_pass_segs(hipStream_t, const uint32_t*, const uint32_t*, uint32_t*, uint32_t*, uint32_t,
+                                    const uint32_t*, const uint32_t*, uint32_t, const uint32_t*, uint32_t, uint32_t*,
+                                    uint32_t*, uint32_t*, const uint32_t*);
 uint32_t oi_reduce_height(int, uint32_t);
 hipError_t oi_launch_reduce(hipStream_t, int, const oi_build_desc*, const uint2*, const oi_build_desc*, uint32_t,
                             const uint32_t*, uint32_t, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr,

@@ -976,11 +976,11 @@ def test_pairs_sorted_on_device(api, ctx, orc):
 
 
 def test_scene_build_equals_tree_builds(api, ctx, orc):
-    """oibvh_scene_build (the trees' build chains on the context's two
-    streams) gives every tree what oibvh_tree_build gives it: leaf order,
-    sorted faces and node bits; also right after overlapped frames (the side
-    stream busy), followed by a detect and more frames."""
-    say("scene build on two streams")
+    """oibvh_scene_build (the trees sharing each launch of the build chain)
+    gives every tree what oibvh_tree_build gives it: leaf order, sorted faces
+    and node bits; also right after overlapped frames (the side stream busy),
+    followed by a detect and more frames."""
+    say("scene build, trees sharing the launches")
     meshes = [M.icosphere(5, 1.0, 0.02, 60 + i, center=(0.8 * i, 0.0, 0.0)) for i in range(3)]
     trees = [api.OibvhTree(ctx, x, f) for x, f in meshes]
     for t in trees:
@@ -1005,5 +1005,47 @@ def test_scene_build_equals_tree_builds(api, ctx, orc):
     assert np.array_equal(sc.getIntTriPairs(), want)
     one = api.Scene(ctx)
     one.addOibvhTree(trees[0])
-    one.build()  # one tree: no side stream
+    one.build()  # a group of one tree
     check_tree(orc, trees[0], moved[0], meshes[0][1])
+
+
+def test_scene_build_groups_mixed(api, ctx, orc):
+    """Seven trees of very different sizes (one face to 70,001 faces, one to
+    18 sort tiles), 30- and 63-bit Morton keys and model matrices on some:
+    the scene build groups consecutive trees of one Morton width, at most
+    four per launch, and every tree ends bit-identical to its own
+    oibvh_tree_build (itself pinned to the oracle by the build tests), twice
+    in a row; the 30-bit trees without a matrix also against the oracle."""
+    say("scene build, mixed groups")
+    sizes = [1, 3, 4097, 70001, 20000, 2, 300]
+    bits63 = [False, False, True, True, False, False, False]
+    xforms = [None, _motion(2, 0), None, _motion(3, 1), None, None, _motion(4, 2)]
+    meshes = [soup(F, 500 + i) for i, F in enumerate(sizes)]
+    trees = []
+    for (x, f), b63, A in zip(meshes, bits63, xforms):
+        t = api.OibvhTree(ctx, x, f)
+        if b63:
+            t.set_morton_bits(63)
+        if A is not None:
+            t.set_transform(A)
+        t.build()
+        trees.append(t)
+    want = [(t.perm().copy(), t.sorted_faces().copy(), bits(t.nodes()).copy()) for t in trees]
+    sc = api.Scene(ctx)
+    for t in trees:
+        sc.addOibvhTree(t)
+    for rep in range(2):
+        for t, (x, _) in zip(trees, meshes):
+            # positions mirrored through the origin: another leaf order and
+            # other boxes over the outputs of the first builds
+            t.set_positions(np.ascontiguousarray(-x[::-1]).astype(np.float32))
+            t.build()
+        for t, (x, _) in zip(trees, meshes):
+            t.set_positions(x)
+        sc.build()
+        for i, (t, (p, sf, nb)) in enumerate(zip(trees, want)):
+            assert np.array_equal(t.perm(), p), f"tree {i}: leaf permutation differs"
+            assert np.array_equal(t.sorted_faces(), sf), f"tree {i}: sorted faces differ"
+            assert np.array_equal(bits(t.nodes()), nb), f"tree {i}: node bits differ"
+    for i in (0, 4, 5):
+        check_tree(orc, trees[i], meshes[i][0], meshes[i][1])
