@@ -139,6 +139,9 @@ struct oibvh_scene {
     uint32_t *b_keys = nullptr, *b_keys_alt = nullptr, *b_vals_tmp = nullptr, *b_vals = nullptr;
     uint32_t *b_keys_hi = nullptr, *b_sort = nullptr;
     uint64_t b_cap = 0, b_hi_cap = 0, b_sort_cap = 0;
+    // the last build's launch key and, once a key repeats, its graph
+    std::vector<uint8_t> b_key;
+    hipGraphExec_t b_graph = nullptr;
 };
 
 // ---------------------------------------------------------------- context
@@ -694,6 +697,7 @@ extern "C" void oibvh_scene_destroy(oibvh_scene* s)
     dfree(s->b_vals);
     dfree(s->b_keys_hi);
     dfree(s->b_sort);
+    if (s->b_graph) (void)hipGraphExecDestroy(s->b_graph);
     for (hipEvent_t e : s->ev_refit) (void)hipEventDestroy(e);
     for (hipEvent_t e : s->ev_detect) (void)hipEventDestroy(e);
     oi_det_free(&s->det);
@@ -1001,76 +1005,91 @@ static oibvh_status scene_build_scratch(oibvh_scene* s, uint64_t F, uint32_t n, 
     return OIBVH_OK;
 }
 
-// OibvhTree::build (oibvhTree.cu:237-388) of n <= 4 trees with the same
-// Morton width, every stage one launch for all of them: mesh boxes, Morton
-// keys over the concatenated faces (each tree's own digit histograms), the
-// LSD passes sorting each tree's keys as a segment of their own, and the
-// reduce, which also writes each tree's permutation from the sorted values.
-static oibvh_status scene_build_group(oibvh_scene* s, oibvh_tree* const* trees, uint32_t n)
-{
-    oibvh_ctx* c = s->ctx;
-    const bool m63 = trees[0]->morton_bits == 63;
-    const uint32_t tile = oi_sort_tile(), chunk = oi_morton_chunk();
+// One launch group of the scene build: n <= 4 consecutive trees of one
+// Morton width, their build descriptors over the concatenated faces and the
+// launch geometry.
+struct SceneBuildGroup {
+    uint32_t n = 0, tiles = 0;
+    bool m63 = false;
+    int M = 0;
+    uint64_t F = 0;
     oi_build_desc d[OI_SCENE_BUILD_GROUP];
     uint32_t box_off[OI_SCENE_BUILD_GROUP + 1] = {0}, mort_off[OI_SCENE_BUILD_GROUP + 1] = {0};
-    uint32_t base[OI_SCENE_BUILD_GROUP], nf[OI_SCENE_BUILD_GROUP];
-    uint64_t F = 0;
-    uint32_t tiles = 0;
+    uint32_t red_off[OI_SCENE_BUILD_GROUP + 1] = {0};
+    uint32_t base[OI_SCENE_BUILD_GROUP] = {0}, nf[OI_SCENE_BUILD_GROUP] = {0};
+};
+
+static void plan_group(const oibvh_ctx* c, oibvh_tree* const* trees, uint32_t n, SceneBuildGroup& g)
+{
+    const uint32_t tile = oi_sort_tile(), chunk = oi_morton_chunk();
+    g.n = n;
+    g.m63 = trees[0]->morton_bits == 63;
     for (uint32_t k = 0; k < n; k++) {
-        d[k] = tree_desc(trees[k]);
-        d[k].fbase = (uint32_t)F;
-        base[k] = (uint32_t)F;
-        nf[k] = d[k].F;
-        box_off[k + 1] = box_off[k] + d[k].nparts;
-        mort_off[k + 1] = mort_off[k] + (d[k].F + chunk - 1) / chunk;
-        tiles += (d[k].F + tile - 1) / tile;
-        F += d[k].F;
+        oi_build_desc& d = g.d[k];
+        d = tree_desc(trees[k]);
+        d.fbase = (uint32_t)g.F;
+        g.base[k] = (uint32_t)g.F;
+        g.nf[k] = d.F;
+        g.box_off[k + 1] = g.box_off[k] + d.nparts;
+        g.mort_off[k + 1] = g.mort_off[k] + (d.F + chunk - 1) / chunk;
+        g.tiles += (d.F + tile - 1) / tile;
+        g.F += d.F;
     }
-    if (F > 0xffffffffull) return oi_set_err(c, OIBVH_ERR_CAPACITY, "scene build: too many faces in one group");
-    oibvh_status r = scene_build_scratch(s, F, n, tiles, m63);
-    if (r != OIBVH_OK) return r;
+    g.M = (int)c->leaves_per_thread;
+    if (!g.M) g.M = oi_default_lpt(g.F, c->cus);
+    for (uint32_t k = 0; k < n; k++) {
+        const uint32_t h = oi_reduce_height(g.M, g.d[k].L);
+        g.red_off[k + 1] = g.red_off[k] + oi_lvl_cnt(g.d[k].L - h, g.d[k].L, g.d[k].vl);
+    }
+}
+
+// OibvhTree::build (oibvhTree.cu:237-388) of one group, every stage one
+// launch for all its trees: mesh boxes, Morton keys over the concatenated
+// faces (each tree's own digit histograms), the LSD passes sorting each
+// tree's keys as a segment of their own, and the reduce, which also writes
+// each tree's permutation from the sorted values. Enqueues only (capturable).
+static oibvh_status enqueue_group(oibvh_scene* s, const SceneBuildGroup& g)
+{
+    oibvh_ctx* c = s->ctx;
+    const uint32_t n = g.n, tiles = g.tiles;
     hipStream_t st = c->stream;
     uint32_t* ghist = s->b_sort;
     uint32_t* ctrs = ghist + (uint64_t)n * OI_SORT_HIST;
     uint32_t* lb[2] = {ctrs + OI_SCENE_CTRS, ctrs + OI_SCENE_CTRS + 256ull * tiles};
     // the mesh-box launch also clears the histograms, counters and the first
     // look-back buffer (a kernel, not a memset: capturable)
-    HIPCHK(c, oi_launch_mesh_box_inline(st, d, n, box_off, ghist, n * OI_SORT_HIST + OI_SCENE_CTRS + 256u * tiles));
-    HIPCHK(c, oi_launch_morton_inline(st, d, n, mort_off, s->b_keys, m63 ? s->b_keys_hi : nullptr, ghist,
+    HIPCHK(c, oi_launch_mesh_box_inline(st, g.d, n, g.box_off, ghist, n * OI_SORT_HIST + OI_SCENE_CTRS + 256u * tiles));
+    HIPCHK(c, oi_launch_morton_inline(st, g.d, n, g.mort_off, s->b_keys, g.m63 ? s->b_keys_hi : nullptr, ghist,
                                       OI_SORT_HIST));
     // LSD passes over the low words, then, for 63-bit keys, over the high
     // words looked up by value; the sorted values (face ids over the
     // concatenated faces) end in b_vals
-    const int np = m63 ? 8 : 4;
+    const int np = g.m63 ? 8 : 4;
     for (int p = 0; p < np; p++) {
         const bool lo = p < 4;
         const uint32_t* kin = lo ? ((p & 1) ? s->b_keys_alt : s->b_keys) : nullptr;
         uint32_t* kout = (lo && p < 3) ? ((p & 1) ? s->b_keys : s->b_keys_alt) : nullptr;
         const uint32_t* vin = p == 0 ? nullptr : ((p & 1) ? s->b_vals_tmp : s->b_vals);
         uint32_t* vout = (p & 1) ? s->b_vals : s->b_vals_tmp;
-        HIPCHK(c, oi_launch_sort_pass_segs(st, kin, vin, kout, vout, n, base, nf, 8u * (p & 3), ghist + 256 * p,
+        HIPCHK(c, oi_launch_sort_pass_segs(st, kin, vin, kout, vout, n, g.base, g.nf, 8u * (p & 3), ghist + 256 * p,
                                            OI_SORT_HIST, lb[p & 1], p + 1 < np ? lb[(p + 1) & 1] : nullptr,
                                            ctrs + p * OI_SCENE_BUILD_GROUP, lo ? nullptr : s->b_keys_hi));
     }
-    int M = (int)c->leaves_per_thread;
-    if (!M) M = oi_default_lpt(F, c->cus);
-    uint32_t red_off[OI_SCENE_BUILD_GROUP + 1] = {0}, na = 0;
+    uint32_t na = 0;
     oi_build_desc apex[OI_SCENE_BUILD_GROUP];
-    for (uint32_t k = 0; k < n; k++) {
-        const uint32_t h = oi_reduce_height(M, d[k].L);
-        red_off[k + 1] = red_off[k] + oi_lvl_cnt(d[k].L - h, d[k].L, d[k].vl);
-        if (d[k].L > h) apex[na++] = d[k];
-    }
-    HIPCHK(c, oi_launch_reduce_inline(st, M, d, n, red_off, s->b_vals, 1, nullptr, nullptr, 0));
-    if (na) HIPCHK(c, oi_launch_reduce_apex_inline(st, M, apex, na, nullptr, nullptr));
-    for (uint32_t k = 0; k < n; k++) trees[k]->built = true;
+    for (uint32_t k = 0; k < n; k++)
+        if (g.d[k].L > oi_reduce_height(g.M, g.d[k].L)) apex[na++] = g.d[k];
+    HIPCHK(c, oi_launch_reduce_inline(st, g.M, g.d, n, g.red_off, s->b_vals, 1, nullptr, nullptr, 0));
+    if (na) HIPCHK(c, oi_launch_reduce_apex_inline(st, g.M, apex, na, nullptr, nullptr));
     return OIBVH_OK;
 }
 
 // OibvhTree::build on every tree of the scene (oibvhTree.cu:237-388). A single
 // tree's build is a chain of small launches that leave most of the GPU idle,
 // so up to four consecutive trees of the same Morton width share each launch
-// of the chain. Results equal oibvh_tree_build on each tree.
+// of the chain. A build whose launches equal the previous call's (same trees,
+// buffers and tuning) replays them from a hipGraph captured on its second
+// occurrence. Results equal oibvh_tree_build on each tree.
 extern "C" oibvh_status oibvh_scene_build(oibvh_scene* s)
 {
     if (!s) return OIBVH_ERR_INVALID;
@@ -1078,16 +1097,72 @@ extern "C" oibvh_status oibvh_scene_build(oibvh_scene* s)
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, oi_ctx_join(c));
     const uint32_t per = std::min<uint32_t>(OI_SCENE_BUILD_GROUP, std::min(oi_inline_meshes(), oi_sort_segs_max()));
-    size_t i = 0;
-    while (i < s->trees.size()) {
+    std::vector<SceneBuildGroup> groups;
+    uint64_t Fmax = 0;
+    uint32_t nmax = 0, tmax = 0;
+    bool m63 = false;
+    for (size_t i = 0; i < s->trees.size();) {
         uint32_t n = 1;
-        while (n < per && i + n < s->trees.size() &&
-               s->trees[i + n]->morton_bits == s->trees[i]->morton_bits)
-            n++;
-        oibvh_status r = scene_build_group(s, &s->trees[i], n);
-        if (r != OIBVH_OK) return r;
+        while (n < per && i + n < s->trees.size() && s->trees[i + n]->morton_bits == s->trees[i]->morton_bits) n++;
+        groups.emplace_back();
+        plan_group(c, &s->trees[i], n, groups.back());
+        const SceneBuildGroup& g = groups.back();
+        if (g.F > 0xffffffffull) return oi_set_err(c, OIBVH_ERR_CAPACITY, "scene build: too many faces in one group");
+        Fmax = std::max(Fmax, g.F);
+        nmax = std::max(nmax, g.n);
+        tmax = std::max(tmax, g.tiles);
+        m63 = m63 || g.m63;
         i += n;
     }
+    if (groups.empty()) return OIBVH_OK;
+    oibvh_status r = scene_build_scratch(s, Fmax, nmax, tmax, m63);
+    if (r != OIBVH_OK) return r;
+    // what the launches depend on: every group's geometry and descriptors,
+    // the scratch buffers and the stream
+    std::vector<uint8_t> key(groups.size() * sizeof(SceneBuildGroup) + 8 * sizeof(void*));
+    std::memset(key.data(), 0, key.size());
+    for (size_t k = 0; k < groups.size(); k++) std::memcpy(&key[k * sizeof(SceneBuildGroup)], &groups[k], sizeof(SceneBuildGroup));
+    const void* ptrs[8] = {s->b_keys, s->b_keys_alt, s->b_vals_tmp, s->b_vals, s->b_keys_hi, s->b_sort, c->stream, nullptr};
+    std::memcpy(&key[groups.size() * sizeof(SceneBuildGroup)], ptrs, sizeof ptrs);
+    const bool same = key == s->b_key;
+    if (!same) {
+        if (s->b_graph) {
+            HIPCHK(c, oi_ctx_quiesce(c));
+            (void)hipGraphExecDestroy(s->b_graph);
+            s->b_graph = nullptr;
+        }
+        s->b_key = key;
+        // a first occurrence runs as plain launches
+        for (const SceneBuildGroup& g : groups)
+            if ((r = enqueue_group(s, g)) != OIBVH_OK) return r;
+    } else {
+        if (!s->b_graph) {
+            hipGraph_t graph = nullptr;
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            oibvh_status rr = OIBVH_OK;
+            for (const SceneBuildGroup& g : groups)
+                if ((rr = enqueue_group(s, g)) != OIBVH_OK) break;
+            hipError_t e = hipStreamEndCapture(c->stream, &graph);
+            if (rr != OIBVH_OK) {
+                if (graph) (void)hipGraphDestroy(graph);
+                s->b_key.clear();
+                return rr;
+            }
+            if (e != hipSuccess) {
+                s->b_key.clear();
+                return oi_set_err(c, OIBVH_ERR_HIP, "scene build capture: %s", hipGetErrorString(e));
+            }
+            e = hipGraphInstantiate(&s->b_graph, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) {
+                s->b_graph = nullptr;
+                s->b_key.clear();
+                return oi_set_err(c, OIBVH_ERR_HIP, "scene build instantiate: %s", hipGetErrorString(e));
+            }
+        }
+        HIPCHK(c, hipGraphLaunch(s->b_graph, c->stream));
+    }
+    for (oibvh_tree* t : s->trees) t->built = true;
     return OIBVH_OK;
 }
 

@@ -1014,8 +1014,9 @@ def test_scene_build_groups_mixed(api, ctx, orc):
     18 sort tiles), 30- and 63-bit Morton keys and model matrices on some:
     the scene build groups consecutive trees of one Morton width, at most
     four per launch, and every tree ends bit-identical to its own
-    oibvh_tree_build (itself pinned to the oracle by the build tests), twice
-    in a row; the 30-bit trees without a matrix also against the oracle."""
+    oibvh_tree_build (itself pinned to the oracle by the build tests), three
+    times in a row (plain launches, then the captured graph, then its
+    replay); the 30-bit trees without a matrix also against the oracle."""
     say("scene build, mixed groups")
     sizes = [1, 3, 4097, 70001, 20000, 2, 300]
     bits63 = [False, False, True, True, False, False, False]
@@ -1034,7 +1035,7 @@ def test_scene_build_groups_mixed(api, ctx, orc):
     sc = api.Scene(ctx)
     for t in trees:
         sc.addOibvhTree(t)
-    for rep in range(2):
+    for rep in range(3):
         for t, (x, _) in zip(trees, meshes):
             # positions mirrored through the origin: another leaf order and
             # other boxes over the outputs of the first builds
