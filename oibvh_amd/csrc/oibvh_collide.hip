@@ -36,6 +36,15 @@
 
 typedef unsigned long long u64;
 
+// 0, computed from v by an instruction the compiler cannot see through: an
+// address offset by it is formed only once v is available
+__device__ __forceinline__ uint32_t opaque_zero(float v)
+{
+    uint32_t z;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(z) : "v"(v));
+    return z;
+}
+
 __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t* s_wsum, uint32_t& total)
 {
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -330,6 +339,9 @@ __device__ __forceinline__ MeshRef tab_ref(const ChunkTab& T, const oi_mesh_desc
     const oi_mesh_desc& d = T.meshes[m];
     return MeshRef{d.nodes, d.sfaces, d.pos, d.perm, d.xf, d.L, d.vl, d.ibase};
 }
+
+// 16 zeros, read in place of a model matrix by meshes that have none
+__device__ const float k_no_xf[16] = {};
 
 // a vertex through the mesh's model matrix, if any
 __device__ __forceinline__ v3f vertex(const float* pos, const float* xf, uint32_t v)
@@ -911,10 +923,38 @@ __device__ __forceinline__ bool narrow_gather(const Tab& T, const uint32_t* s_ib
             return false;
         }
     }
+    // all six vertices, then both model matrices, are loaded before any is
+    // used (one round of loads, not one per vertex)
+    float pa[3][3], pb[3][3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        P[k] = vertex(ra.pos, ra.xf, fa[k]);
-        Q[k] = vertex(rb.pos, rb.xf, fb[k]);
+        oi_ld3(ra.pos, fa[k], pa[k]);
+        oi_ld3(rb.pos, fb[k], pb[k]);
+    }
+    // (a mesh without a matrix reads a zero block in its place, unused)
+    const float* sxa = ra.xf ? ra.xf : k_no_xf;
+    const float* sxb = rb.xf ? rb.xf : k_no_xf;
+    float xa[16], xb[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        xa[k] = oi_g(sxa)[k];
+        xb[k] = oi_g(sxb)[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        float o[3];
+        if (ra.xf) {
+            oi_xform(xa, pa[k][0], pa[k][1], pa[k][2], o);
+            P[k] = v3f{o[0], o[1], o[2]};
+        } else {
+            P[k] = v3f{pa[k][0], pa[k][1], pa[k][2]};
+        }
+        if (rb.xf) {
+            oi_xform(xb, pb[k][0], pb[k][1], pb[k][2], o);
+            Q[k] = v3f{o[0], o[1], o[2]};
+        } else {
+            Q[k] = v3f{pb[k][0], pb[k][1], pb[k][2]};
+        }
     }
     if (ma == mb && oa > ob) {
         // both faces share the mesh's positions and model matrix, so the
@@ -962,15 +1002,25 @@ __global__ void __launch_bounds__(TB) k_narrow(NarrowArgs<Tab> A)
     tab_init(A.tab, s_desc, s_ibase);
     const uint32_t tid = threadIdx.x;
     uint32_t n_fp64 = 0, n_adj = 0;
-    for (uint32_t base = blockIdx.x * TB; base < n; base += gridDim.x * TB) {
+    const uint32_t stride = gridDim.x * TB;
+    uint2 c_next = (blockIdx.x * TB + tid < n) ? A.cands[blockIdx.x * TB + tid] : make_uint2(0, 0);
+    for (uint32_t base = blockIdx.x * TB; base < n; base += stride) {
         const uint32_t i = base + tid;
         bool hit = false, defer = false;
         uint4 out = make_uint4(0, 0, 0, 0);
+        const uint2 c = c_next;
         if (i < n) {
-            const uint2 c = A.cands[i];
             v3f P[3], Q[3];
             // deferred candidates were checked for adjacency by the first pass
-            if (narrow_gather(A.tab, s_ibase, s_desc, c, MODE != NARROW_FP64, n_adj, P, Q, out)) {
+            const bool live = narrow_gather(A.tab, s_ibase, s_desc, c, MODE != NARROW_FP64, n_adj, P, Q, out);
+            // the next candidate is loaded once this one's vertices have
+            // arrived: loads complete in issue order, so issued earlier it
+            // would hold up the wait for them
+            if (i + stride < n) {
+                const float dep = live ? P[2].z + Q[2].z : 0.0f;
+                c_next = A.cands[i + stride + opaque_zero(dep)];
+            }
+            if (live) {
                 if (MODE == NARROW_FP32) {
                     hit = sat_f32(P, Q);
                 } else if (MODE == NARROW_FILTER) {
