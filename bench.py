@@ -672,6 +672,7 @@ def run_deforming(args, ctx):
     N = [t.node_count for t in trees]
     rk = reduce_kernel(tn["reduce_leaves_per_thread"], sum(Fm), tn["refit_blocks"], sum(-(-F // 1024) for F in Fm))
     refit_bytes = sum(12 * V + 12 * F + 24 * (n - apex_nodes(F, rk)) for F, n in zip(Fm, N))
+    detect_bytes = sum(24 * n + 12 * V + 12 * F for F, n in zip(Fm, N)) + 16 * len(pairs_last)
     diffs = None
     if not args.no_verify:
         import oracle
@@ -709,7 +710,11 @@ def run_deforming(args, ctx):
            "async_errors": st["async_errors"], "retries_last_frame": st["retries"],
            "kernels": {"refit_stage1": {"kernel": rk, "avg_ms": refit_ms, "launches": tm["refits"],
                                         "alg_bytes": refit_bytes},
-                       "detect": {"avg_ms": detect_ms, "launches": tm["detects"]}},
+                       "detect": {"avg_ms": detect_ms, "launches": tm["detects"], "alg_bytes": detect_bytes,
+                                  "alg_bytes_unit": "24 N per tree (broad) + 12 V + 12 F per tree (narrow) + 16 B "
+                                                    "per hit",
+                                  "gbps": detect_bytes / (detect_ms * 1e-3) / 1e9 if detect_ms else None,
+                                  "frac": detect_bytes / (detect_ms * 1e-3) / HBM_PEAK if detect_ms else None}},
            "roofline": {"bound": "hbm", "kernel": f"{rk} (refit stage 1, both trees in one launch)",
                         "achieved": refit_bytes / (refit_ms * 1e-3) / 1e9 if refit_ms else None,
                         "peak": HBM_PEAK / 1e9, "unit": "GB/s",

@@ -100,7 +100,7 @@ def tune(ctx: api.Context, scene: api.Scene, frame, *, entries=(4, 6, 8), expand
 
 
 def tune_batch(ctx: api.Context, batch: api.Batch, before_frame, *, entries=(0, 2, 4), expands=(1, 2, 3),
-               blocks=(0, 4096), leaves=(0, 4, 8), local=((64, 2), (256, 3)), tail=(), refit_blocks=(0,),
+               blocks=(0, 4096, 8192), leaves=(0, 4, 8), local=((64, 2), (256, 3)), tail=(), refit_blocks=(0,),
                flags=None, reps: int = 3) -> dict:
     """Pick the reduce subtree size (leaves per thread), the BVTT entry level,
     the levels per step and the workgroups per launch for a batch's captured

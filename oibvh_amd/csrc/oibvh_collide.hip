@@ -421,6 +421,9 @@ struct StepArgs {
     uint4* hits;                // fused narrow phase: hits and their count
     uint32_t* n_hits;
     uint32_t hits_cap;
+    uint2* defer;               // fused FP32 filter: undecided leaf pairs for the FP64 pass
+    uint32_t* defer_count;
+    uint32_t defer_cap;
     uint32_t implicit_end;
 };
 
@@ -457,6 +460,9 @@ __device__ __forceinline__ bool share_vertex(const uint32_t* a, const uint32_t* 
 template <bool FP64, class Tab>
 __device__ __forceinline__ void narrow_one(const Tab& T, const uint32_t* s_ibase, const oi_mesh_desc* s_desc,
                                            uint2 c, bool& hit, uint4& out, uint32_t& n_fp64, uint32_t& n_adj);
+template <class Tab>
+__device__ __forceinline__ void narrow_one_filter(const Tab& T, const uint32_t* s_ibase, const oi_mesh_desc* s_desc,
+                                                  uint2 c, bool& hit, bool& defer, uint4& out, uint32_t& n_adj);
 
 // position of the k-th set bit of m (k < popcount(m))
 __device__ __forceinline__ uint32_t select_bit(u64 m, uint32_t k)
@@ -571,24 +577,27 @@ __device__ __forceinline__ void entry_expand(const Tab& T, const oi_mesh_desc* s
 struct FusedLds {
     u64 cmask[TB];
     uint32_t coff[TB], ga[TB], gb[TB];
-    uint32_t hbase;
+    uint32_t hbase, dbase;
 };
 
 // The workgroup's C leaf pairs (thread t owns the set bits of cmask[t],
-// numbered from coff[t]) through the FP32 SAT: pair c is found by a binary
-// search over the offsets and a select in the owner's mask; hits are
-// appended with one fetch_add per round.
-template <int E, class Tab>
+// numbered from coff[t]) through the FP32 SAT (FILTER: the FP32 filter,
+// undecided pairs to the defer list for the FP64 pass): pair c is found by a
+// binary search over the offsets and a select in the owner's mask; hits (and
+// deferred pairs) are appended with one fetch_add each per round.
+template <int E, bool FILTER, class Tab>
 __device__ __forceinline__ void fused_narrow(const Tab& T, const uint32_t* s_ibase, const oi_mesh_desc* s_desc,
                                              FusedLds& F, uint32_t C, uint32_t* s_wsum, uint4* hits,
-                                             uint32_t* n_hits, uint32_t hits_cap, uint32_t* error,
+                                             uint32_t* n_hits, uint32_t hits_cap, uint2* defer,
+                                             uint32_t* defer_count, uint32_t defer_cap, uint32_t* error,
                                              uint32_t& n_fp64, uint32_t& n_adj)
 {
     constexpr int K = 1 << E;
     const uint32_t tid = threadIdx.x;
     for (uint32_t c0 = 0; c0 < C; c0 += TB) {
         const uint32_t c = c0 + tid;
-        bool hit = false;
+        bool hit = false, dfr = false;
+        uint2 pr = make_uint2(0, 0);
         uint4 h = make_uint4(0, 0, 0, 0);
         if (c < C) {
             uint32_t lo = 0, hi = TB - 1;  // last thread whose offset is <= c
@@ -600,29 +609,47 @@ __device__ __forceinline__ void fused_narrow(const Tab& T, const uint32_t* s_iba
                     hi = mid - 1u;
             }
             const uint32_t bit = select_bit(F.cmask[lo], c - F.coff[lo]);
-            const uint2 pr = make_uint2(F.ga[lo] + (bit >> E), F.gb[lo] + (bit & (K - 1)));
-            narrow_one<false, Tab>(T, s_ibase, s_desc, pr, hit, h, n_fp64, n_adj);
+            pr = make_uint2(F.ga[lo] + (bit >> E), F.gb[lo] + (bit & (K - 1)));
+            if (FILTER)
+                narrow_one_filter<Tab>(T, s_ibase, s_desc, pr, hit, dfr, h, n_adj);
+            else
+                narrow_one<false, Tab>(T, s_ibase, s_desc, pr, hit, h, n_fp64, n_adj);
         }
-        uint32_t H;
-        const uint32_t hoff = block_scan(hit ? 1u : 0u, s_wsum, H);
-        if (tid == 0) F.hbase = H ? add_u32(n_hits, H) : 0u;
+        // one scan for both counts: hits in the high half, deferred pairs in
+        // the low half (at most 256 each)
+        uint32_t HD;
+        const uint32_t o = block_scan((hit ? 65536u : 0u) + (dfr ? 1u : 0u), s_wsum, HD);
+        const uint32_t H = HD >> 16, D = HD & 0xffffu;
+        if (tid == 0) {
+            F.hbase = H ? add_u32(n_hits, H) : 0u;
+            if (FILTER) F.dbase = D ? add_u32(defer_count, D) : 0u;
+        }
         __syncthreads();
         if (hit) {
-            const uint32_t o = F.hbase + hoff;
-            if (o < hits_cap)
-                hits[o] = h;
+            const uint32_t pos = F.hbase + (o >> 16);
+            if (pos < hits_cap)
+                hits[pos] = h;
             else
                 or_u32(error, OI_ERR_HITS_OVERFLOW);
+        }
+        if (FILTER && dfr) {
+            const uint32_t pos = F.dbase + (o & 0xffffu);
+            if (pos < defer_cap)
+                defer[pos] = pr;
+            else
+                or_u32(error, OI_ERR_CAND_OVERFLOW);
         }
     }
 }
 
-// FUSED: overlapping leaf pairs are tested with the FP32 SAT by the
+// FUSE 1: overlapping leaf pairs are tested with the FP32 SAT by the
 // workgroup that found them (fused_narrow) instead of going to the candidate
-// list.
-template <int E, bool FIRST, bool FUSED, class Tab>
+// list; FUSE 2: with the FP32 filter, the pairs it leaves undecided going to
+// A.defer for the FP64 pass.
+template <int E, bool FIRST, int FUSE, class Tab>
 __global__ void __launch_bounds__(TB) k_bvtt_step(StepArgs<Tab> A)
 {
+    constexpr bool FUSED = FUSE != 0;
     constexpr int K = 1 << E;
     __shared__ oi_mesh_desc s_desc[MESH_LDS];
     __shared__ uint32_t s_ibase[MESH_LDS];
@@ -684,8 +711,9 @@ __global__ void __launch_bounds__(TB) k_bvtt_step(StepArgs<Tab> A)
         if constexpr (FUSED) {
             if (C) {
                 if (tid == 0) n_cands += C;
-                fused_narrow<E, Tab>(A.tab, s_ibase, s_desc, s_f[0], C, s_wsum, A.hits, A.n_hits, A.hits_cap,
-                                     A.error, n_fp64, n_adj);
+                fused_narrow<E, FUSE == 2, Tab>(A.tab, s_ibase, s_desc, s_f[0], C, s_wsum, A.hits, A.n_hits,
+                                                A.hits_cap, A.defer, A.defer_count, A.defer_cap, A.error, n_fp64,
+                                                n_adj);
             }
         }
         __syncthreads();
@@ -844,8 +872,8 @@ __global__ void __launch_bounds__(TB) k_bvtt_local(LocalArgs<Tab> A)
                 if constexpr (FUSED) {
                     if (C) {
                         if (tid == 0) n_cands += C;
-                        fused_narrow<E, Tab>(A.tab, s_ibase, s_desc, s_f[0], C, s_wsum, A.hits, A.n_hits, A.hits_cap,
-                                             A.error, n_fp64, n_adj);
+                        fused_narrow<E, false, Tab>(A.tab, s_ibase, s_desc, s_f[0], C, s_wsum, A.hits, A.n_hits,
+                                                    A.hits_cap, nullptr, nullptr, 0, A.error, n_fp64, n_adj);
                     }
                 }
                 __syncthreads();
@@ -989,6 +1017,23 @@ __device__ __forceinline__ void narrow_one(const Tab& T, const uint32_t* s_ibase
     }
 }
 
+// the FP32 filter on one candidate (the fused steps of an FP64-fallback
+// query): a hit, a pair left to the FP64 pass (defer), or neither
+template <class Tab>
+__device__ __forceinline__ void narrow_one_filter(const Tab& T, const uint32_t* s_ibase, const oi_mesh_desc* s_desc,
+                                                  uint2 c, bool& hit, bool& defer, uint4& out, uint32_t& n_adj)
+{
+    hit = false;
+    defer = false;
+    v3f P[3], Q[3];
+    uint4 id;
+    if (!narrow_gather(T, s_ibase, s_desc, c, true, n_adj, P, Q, id)) return;
+    const int r = sat_filter(P, Q);
+    hit = r == SAT_HIT;
+    defer = r == SAT_UNDECIDED;
+    out = id;
+}
+
 template <int MODE, class Tab>
 __global__ void __launch_bounds__(TB) k_narrow(NarrowArgs<Tab> A)
 {
@@ -1117,6 +1162,7 @@ static hipError_t launch_step(hipStream_t st, uint32_t grid, const Tab& tab, int
                               uint32_t cand_cap, uint4* hits, uint32_t hits_cap, oi_detect_ctl* ctl, uint32_t expand,
                               uint32_t implicit_end, int fused, hipEvent_t done)
 {
+    if (fused < 0 || fused > 2) return hipErrorInvalidValue;
     StepArgs<Tab> a{};
     a.tab = tab;
     a.front_in = front_in;
@@ -1135,17 +1181,26 @@ static hipError_t launch_step(hipStream_t st, uint32_t grid, const Tab& tab, int
     a.hits = hits;
     a.n_hits = &ctl->n_hits;
     a.hits_cap = hits_cap;
+    // the filter's deferred pairs go to the candidate buffer, which fused
+    // steps leave unused
+    a.defer = cands;
+    a.defer_count = &ctl->deferred;
+    a.defer_cap = cand_cap;
     a.implicit_end = implicit_end;
     const dim3 g(grid), b(TB);
 #define OI_STEP(EE)                                                                    \
-    if (first && fused)                                                                \
-        launch_done((k_bvtt_step<EE, true, true, Tab>), g, b, st, done, a);            \
+    if (first && fused == 2)                                                           \
+        launch_done((k_bvtt_step<EE, true, 2, Tab>), g, b, st, done, a);               \
+    else if (first && fused)                                                           \
+        launch_done((k_bvtt_step<EE, true, 1, Tab>), g, b, st, done, a);               \
     else if (first)                                                                    \
-        launch_done((k_bvtt_step<EE, true, false, Tab>), g, b, st, done, a);           \
+        launch_done((k_bvtt_step<EE, true, 0, Tab>), g, b, st, done, a);               \
+    else if (fused == 2)                                                               \
+        launch_done((k_bvtt_step<EE, false, 2, Tab>), g, b, st, done, a);              \
     else if (fused)                                                                    \
-        launch_done((k_bvtt_step<EE, false, true, Tab>), g, b, st, done, a);           \
+        launch_done((k_bvtt_step<EE, false, 1, Tab>), g, b, st, done, a);              \
     else                                                                               \
-        launch_done((k_bvtt_step<EE, false, false, Tab>), g, b, st, done, a);
+        launch_done((k_bvtt_step<EE, false, 0, Tab>), g, b, st, done, a);
     switch (expand) {
     case 1: OI_STEP(1) break;
     case 2: OI_STEP(2) break;
@@ -1246,8 +1301,10 @@ hipError_t oi_launch_seed(hipStream_t st, uint32_t n0, uint32_t* counts, uint32_
 }
 
 // one BVTT step; front_in's length is *count_in, or n_in when count_in is null
-// fused: overlapping leaf pairs are tested with the FP32 SAT inside the step
-// (hits appended directly) instead of going to the candidate list
+// fused 1: overlapping leaf pairs are tested with the FP32 SAT inside the step
+// (hits appended directly) instead of going to the candidate list; fused 2:
+// with the FP32 filter, undecided pairs to the candidate buffer (count in
+// ctl->deferred) for the FP64 pass
 hipError_t oi_launch_bvtt_step(hipStream_t st, uint32_t grid, const oi_tab_args* t, int first,
                                const uint2* front_in, const uint32_t* count_in, uint32_t n_in, uint32_t cap_in,
                                uint2* front_out, uint32_t* count_out, uint32_t cap_out, uint2* cands,

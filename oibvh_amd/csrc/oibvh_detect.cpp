@@ -189,15 +189,18 @@ oibvh_status oi_det_enqueue(oi_detector* d, bool reset_sticky, bool seeded, hipE
     if (!seeded)
         HIPCHK(c, oi_launch_seed(st, d->n_front, d->counts, d->n_levels + 1, d->ctl, reset_sticky ? 1u : 0u));
     const uint32_t grid = oi_grid_for(c);
-    // FP32 queries test leaf pairs inside the steps; the FP64 fallback keeps
-    // its own narrow pass (its registers would slow every step)
     const bool fp64 = (d->flags & OIBVH_DETECT_FP64_FALLBACK) != 0;
-    d->fused = !fp64 && !c->narrow_separate;
     const uint32_t J = d->local_chunk ? d->local_levels : 0u;
     const uint32_t T = d->tail_chunk ? d->tail_from : d->n_levels;
+    // Leaf pairs are tested inside the BVTT launches that find them (unless
+    // narrow_separate): FP32 queries with the SAT; FP64-fallback queries with
+    // the FP32 filter in the steps (not in local walks), the pairs it leaves
+    // undecided collected in the candidate buffer for one FP64 pass
+    d->fused = !c->narrow_separate && (!fp64 || (J == 0 && T >= d->n_levels));
+    const int fuse = d->fused ? (fp64 ? 2 : 1) : 0;
     // `done` rides on the last launch: k_narrow, else the tail walk, else
     // the last step, else the head walk
-    const bool narrow = !d->fused, tail = T < d->n_levels, steps = T > J;
+    const bool narrow = !d->fused || fp64, tail = T < d->n_levels, steps = T > J;
     hipEvent_t done_head = (!narrow && !tail && !steps) ? done : nullptr;
     hipEvent_t done_tail = (!narrow && tail) ? done : nullptr;
     if (J) {
@@ -223,7 +226,7 @@ oibvh_status oi_det_enqueue(oi_detector* d, bool reset_sticky, bool seeded, hipE
         HIPCHK(c, oi_launch_bvtt_step(st, grid, &d->tab, k == 0, in, k ? d->counts + k : nullptr, d->n_front,
                                       k ? d->front_cap : d->n_front, d->front[k & 1], d->counts + k + 1,
                                       d->front_cap, d->cands, &d->ctl->cands, d->cand_cap, d->hits, d->hits_cap,
-                                      d->ctl, d->step_e[k], d->implicit_end, d->fused ? 1 : 0,
+                                      d->ctl, d->step_e[k], d->implicit_end, fuse,
                                       (!narrow && !tail && k + 1 == T) ? done : nullptr));
     }
     if (tail) {
@@ -251,7 +254,11 @@ oibvh_status oi_det_enqueue(oi_detector* d, bool reset_sticky, bool seeded, hipE
     if (narrow && !fp64)
         HIPCHK(c, oi_launch_narrow(st, grid, &d->tab, d->cands, &d->ctl->cands, d->cand_cap, d->hits,
                                    &d->ctl->n_hits, d->hits_cap, d->ctl, 0, nullptr, 0, done));
-    if (narrow && fp64) {
+    if (fp64 && d->fused) {
+        // the FP64 SAT over the pairs the steps' filter left undecided
+        HIPCHK(c, oi_launch_narrow(st, grid, &d->tab, d->cands, &d->ctl->deferred, d->cand_cap, d->hits,
+                                   &d->ctl->n_hits, d->hits_cap, d->ctl, 2, nullptr, 0, done));
+    } else if (narrow && fp64) {
         // the FP32 filter over the candidates, then the FP64 SAT over the
         // pairs it left undecided; the BVTT fronts are idle by now, and the
         // first holds that list (an overflow grows them like a front's)
