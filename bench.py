@@ -108,6 +108,9 @@ def parse():
     p.add_argument("--precision-frames", type=int, default=20)
     p.add_argument("--self-frames", type=int, default=100)
     p.add_argument("--cloth-nv", type=int, default=2048)
+    p.add_argument("--deforming-tuning", default="",
+                   help="JSON object: fixed entry_level, expand_levels and tuning knobs for the deforming section "
+                        "instead of the autotuner (counter runs)")
     p.add_argument("--copy-probe-mb", type=int, default=0,
                    help="first copy this many MiB device to device (a kernel of known traffic, for counter scaling)")
     return p.parse_args()
@@ -603,25 +606,33 @@ def run_deforming(args, ctx):
         scene.addOibvhTree(t)
     pos = [[bufs[m].at(f * V * 12) for m in range(len(meshes))] for f in range(n_frames)]
     pos_arr = [api.position_array(p) for p in pos]
-    progress("deforming pair: autotuning")
-
     def tune_frame(i):
         for m, t in enumerate(trees):
             t.set_positions_device(pos[(1 + i) % n_frames][m])
         scene.refit()
 
-    base = ctx.get_tuning()
-    tuned = autotune.tune(ctx, scene, tune_frame)
-    entry, expand = tuned["entry_level"], tuned["expand_levels"]
-    ctx.set_tuning(traverse_expand=expand)
-    scene.detectCollision(entry, expand)
-    scene.getIntTriPairCount()
-
     def overlapped(i0, n):
         for i in range(n):
             scene.frame(pos_arr[(1 + i0 + i) % n_frames], entry, expand, asynchronous=True, overlap=True)
 
-    tuned_rb = autotune.tune_refit_blocks(ctx, scene, overlapped)
+    base = ctx.get_tuning()
+    if args.deforming_tuning:
+        # fixed settings (counter runs, whose passes must see the same kernels)
+        tuned = json.loads(args.deforming_tuning)
+        entry, expand = tuned.pop("entry_level"), tuned.pop("expand_levels")
+        ctx.set_tuning(**{**tuned, "traverse_expand": expand})
+        tuned = {"fixed": True, "entry_level": entry, "expand_levels": expand, **tuned}
+        tuned_rb = None
+        scene.detectCollision(entry, expand)
+        scene.getIntTriPairCount()
+    else:
+        progress("deforming pair: autotuning")
+        tuned = autotune.tune(ctx, scene, tune_frame)
+        entry, expand = tuned["entry_level"], tuned["expand_levels"]
+        ctx.set_tuning(traverse_expand=expand)
+        scene.detectCollision(entry, expand)
+        scene.getIntTriPairCount()
+        tuned_rb = autotune.tune_refit_blocks(ctx, scene, overlapped)
 
     def step(f):
         if f % rebuild == 0:
