@@ -278,10 +278,12 @@ OIBVH_API oibvh_status oibvh_ctx_get_tuning(const oibvh_ctx* ctx, oibvh_tuning* 
 OIBVH_API oibvh_status oibvh_scene_set_transforms(oibvh_scene* scene, const float* mats, uint32_t ptr_kind);
 
 /* Build every tree of the scene (OibvhTree::build on each, oibvhTree.cu:237-388;
- * no reference counterpart as one call): the trees' build chains run side by
- * side on the context's two streams. Results are those of oibvh_tree_build on
- * every tree. Trees of a scene must be built before oibvh_scene_add_tree, so
- * this rebuilds (after new positions or matrices). */
+ * no reference counterpart as one call): up to four consecutive trees of one
+ * Morton width share every launch of the build chain, the sort treating each
+ * tree's keys as a segment of its own; a build whose launches repeat the
+ * previous call's is replayed from a captured hipGraph. Results are those of
+ * oibvh_tree_build on every tree. Trees of a scene must be built before
+ * oibvh_scene_add_tree, so this rebuilds (after new positions or matrices). */
 OIBVH_API oibvh_status oibvh_scene_build(oibvh_scene* scene);
 
 /* Refit every tree of the scene (OibvhTree::refit on each, oibvhTree.cu:193-235)
