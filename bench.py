@@ -638,7 +638,7 @@ def run_deforming(args, ctx):
         if f % rebuild == 0:
             for m, t in enumerate(trees):
                 t.set_positions_device(pos[f][m])
-            scene.build()  # both trees' build chains side by side
+            scene.build()  # both trees in every launch of one build chain
             scene.detectCollision(entry, expand, asynchronous=True)
         else:
             scene.frame(pos_arr[f], entry, expand, asynchronous=True, overlap=True)

@@ -296,7 +296,8 @@ class Scene:
         _check(self.ctx.h, self._lib.oibvh_scene_refit(self.h), "scene refit")
 
     def build(self) -> None:
-        """Rebuild every tree of the scene, two build chains side by side."""
+        """Rebuild every tree of the scene (oibvh_scene_build): up to four trees
+        share each launch of the build chain."""
         _check(self.ctx.h, self._lib.oibvh_scene_build(self.h), "scene build")
 
     def set_transforms(self, mats=None, *, device: int | None = None, borrow: bool = False) -> None:
