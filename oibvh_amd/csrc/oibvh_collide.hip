@@ -1404,6 +1404,106 @@ k_hits_gather(const uint4* __restrict__ hits, const uint32_t* __restrict__ order
     if (i < n) out[i] = hits[order[i]];
 }
 
+// A hit list of at most HS_MAX pairs sorted by one workgroup of HS_WAVES
+// waves: the hit indices stay in LDS and every byte the sort visits is one
+// pass inside the kernel (one launch instead of the zeroing, histogram, pass
+// and gather launches). Pass layout as k_sort_pass: wave w owns positions
+// [512 w, 512 w + 512), round j lane l holds position 512 w + 64 j + l; each
+// thread ranks its keys with 8 ballots per round (stable), the per-wave digit
+// counts give each key its slot, and the indices move to the other LDS
+// buffer. The key byte of hit v is read from hits[v]. The sorted hits are
+// then gathered to out.
+#define HS_WAVES 16
+#define HS_THREADS (64 * HS_WAVES)
+#define HS_ROUNDS 8
+#define HS_MAX (HS_THREADS * HS_ROUNDS)
+
+__global__ void __launch_bounds__(HS_THREADS)
+k_hits_sort_small(const uint4* __restrict__ hits, uint32_t n, oi_hit_digits dg, uint4* __restrict__ out)
+{
+    __shared__ uint32_t s_idx[2][HS_MAX];
+    __shared__ uint32_t s_whist[HS_WAVES][256];
+    __shared__ uint32_t s_start[256];
+    __shared__ uint32_t s_wsum[HS_WAVES];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    for (uint32_t i = tid; i < n; i += HS_THREADS) s_idx[0][i] = i;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    uint32_t cur = 0;
+    for (uint32_t p = 0; p < dg.nd; p++) {
+        const uint32_t word = dg.word[p], shift = dg.shift[p];
+        for (uint32_t i = tid; i < HS_WAVES * 256u; i += HS_THREADS) (&s_whist[0][0])[i] = 0u;
+        __syncthreads();
+        uint32_t v[HS_ROUNDS], d[HS_ROUNDS], rk[HS_ROUNDS];
+#pragma unroll
+        for (int j = 0; j < HS_ROUNDS; j++) {
+            const uint32_t pos = 512u * wave + 64u * j + lane;
+            v[j] = pos < n ? s_idx[cur][pos] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < HS_ROUNDS; j++) {
+            const uint32_t pos = 512u * wave + 64u * j + lane;
+            const OI_GLOBAL uint32_t* h = (const OI_GLOBAL uint32_t*)oi_g(hits) + 4ull * v[j] + word;
+            d[j] = pos < n ? (*h >> shift) & 0xffu : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < HS_ROUNDS; j++) {
+            const uint32_t pos = 512u * wave + 64u * j + lane;
+            const bool ok = pos < n;
+            unsigned long long peers = __ballot(ok);
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                const bool bit = (d[j] >> b) & 1u;
+                const unsigned long long m = __ballot(bit);
+                peers &= bit ? m : ~m;
+            }
+            const uint32_t prev = s_whist[wave][d[j]];
+            rk[j] = prev + (uint32_t)__popcll(peers & lt);
+            if (ok && (peers & lt) == 0ull) s_whist[wave][d[j]] = prev + (uint32_t)__popcll(peers);
+        }
+        __syncthreads();
+        // threads tid < 256 own digit tid: per-wave offsets, then the digit
+        // starts (a scan over the first four waves)
+        uint32_t tot = 0;
+        if (tid < 256u) {
+#pragma unroll
+            for (int w = 0; w < HS_WAVES; w++) {
+                const uint32_t c = s_whist[w][tid];
+                s_whist[w][tid] = tot;
+                tot += c;
+            }
+        }
+        const uint32_t x = oi_wave_scan_incl(tot);
+        if (lane == 63u) s_wsum[wave] = x;
+        __syncthreads();
+        if (tid < 256u) {
+            uint32_t before = 0;
+#pragma unroll
+            for (int w = 0; w < 4; w++) before += (w < (int)wave) ? s_wsum[w] : 0u;
+            s_start[tid] = before + x - tot;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < HS_ROUNDS; j++) {
+            const uint32_t pos = 512u * wave + 64u * j + lane;
+            if (pos < n) s_idx[cur ^ 1u][s_start[d[j]] + s_whist[wave][d[j]] + rk[j]] = v[j];
+        }
+        cur ^= 1u;
+        __syncthreads();
+    }
+    for (uint32_t i = tid; i < n; i += HS_THREADS) out[i] = hits[s_idx[cur][i]];
+}
+
+hipError_t oi_launch_hits_sort_small(hipStream_t st, const uint4* hits, uint32_t n, const oi_hit_digits* dg,
+                                     uint4* out)
+{
+    if (n == 0) return hipSuccess;
+    if (n > HS_MAX || dg->nd > OI_HIT_DIGITS) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_hits_sort_small, dim3(1), dim3(HS_THREADS), 0, st, hits, n, *dg, out);
+    return hipGetLastError();
+}
+
+uint32_t oi_hits_sort_small_max() { return HS_MAX; }
+
 hipError_t oi_launch_zero_u32(hipStream_t st, uint32_t* p, uint32_t n)
 {
     if (n == 0) return hipSuccess;

@@ -412,6 +412,13 @@ static oibvh_status sort_hits(oi_detector* d)
     const uint32_t tiles = (n + oi_sort_tile() - 1) / oi_sort_tile();
     uint64_t gen = 0;
     oibvh_status r;
+    if (n <= oi_hits_sort_small_max()) {
+        // one workgroup, every pass in LDS, one launch
+        if ((r = grow(c, d->d_sorted, d->d_sorted_cap, n, gen)) != OIBVH_OK) return r;
+        HIPCHK(c, oi_launch_hits_sort_small(c->stream, d->hits, n, &dg, d->d_sorted));
+        d->sorted_ok = true;
+        return OIBVH_OK;
+    }
     if (d->sort_cap < n) {
         // sized with headroom, so that the next queries rarely grow them
         const uint64_t cap = std::min<uint64_t>((1ull << 30) - 1, (uint64_t)n + (n >> 2));

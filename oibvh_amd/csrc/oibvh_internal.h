@@ -63,6 +63,8 @@ hipError_t oi_launch_pair_vertices(hipStream_t, const oi_build_desc*, const uint
 hipError_t oi_launch_zero_u32(hipStream_t, uint32_t*, uint32_t);
 hipError_t oi_launch_hits_digits(hipStream_t, const uint4*, uint32_t, uint32_t*, uint32_t*, const oi_hit_digits*);
 hipError_t oi_launch_hits_gather(hipStream_t, const uint4*, const uint32_t*, uint32_t, uint4*);
+hipError_t oi_launch_hits_sort_small(hipStream_t, const uint4*, uint32_t, const oi_hit_digits*, uint4*);
+uint32_t oi_hits_sort_small_max();
 }
 
 // ---------------------------------------------------------------------------

@@ -935,7 +935,9 @@ def test_pairs_sorted_on_device(api, ctx, orc):
     """The pair list sorted on the device (oibvh_scene_pairs_device) equals
     the oracle's sorted pairs; face ids above 2^16 (three key bytes) and a
     scene of more than 256 trees (two mesh-id bytes, many trees with no
-    pair); empty results."""
+    pair), both sorted by one workgroup (at most 8192 pairs); a cloth's
+    18,350 self-collision pairs (the multi-workgroup passes); empty
+    results."""
     say("pairs sorted on the device")
     (x0, f0), (x1, f1) = M.interpenetrating_icospheres(subdiv=6, offset=1.0)  # 81,920 faces each
     t0, t1 = api.OibvhTree(ctx, x0, f0), api.OibvhTree(ctx, x1, f1)
@@ -966,6 +968,17 @@ def test_pairs_sorted_on_device(api, ctx, orc):
     want = orc.detect([orc.Tree(x, f, order=t.perm()) for (x, f), t in zip(shapes, trees)])
     assert len(want) > 0 and want[:, 1].max() > 256
     p, n = big.pairs_device()
+    assert np.array_equal(api.download_u32(p, 4 * n).reshape(-1, 4), want)
+    # more pairs than one workgroup sorts
+    x, f = M.cloth(256, amplitude=0.2, wavelength=0.05, seed=3)
+    tc = api.OibvhTree(ctx, x, f)
+    tc.build()
+    cl = api.Scene(ctx)
+    cl.addOibvhTree(tc)
+    cl.detectCollision(2, 2, self_collide=True)
+    want = orc.detect([orc.Tree(x, f, order=tc.perm())], self_collide=True)
+    assert len(want) > 8192
+    p, n = cl.pairs_device()
     assert np.array_equal(api.download_u32(p, 4 * n).reshape(-1, 4), want)
     # nothing collides
     far = api.Scene(ctx)
