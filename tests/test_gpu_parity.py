@@ -257,6 +257,62 @@ def test_refit_after_deformation(api, ctx, orc):
         assert np.array_equal(bits(t.nodes()), bits(o.nodes()))
 
 
+def test_deforming_pair_full_size_vs_oracle(api, ctx, orc):
+    """The deforming workload at full size (two UV spheres of 1000 x 501,
+    1,000,000 triangles each): both trees built by one scene build, refit to
+    a later frame and rebuilt at a third, every tree's leaf order and node
+    bits and the pair set of each state against the oracle (the reference's
+    CPU path restated)."""
+    say("deforming pair, full size")
+    meshes, waves = M.deforming_uv_pair()
+    trees = [api.OibvhTree(ctx, waves[m].frame(0), meshes[m][1]) for m in range(2)]
+    for t in trees:
+        t.build()
+    sc = api.Scene(ctx)
+    for t in trees:
+        sc.addOibvhTree(t)
+    for step, frame in (("build", 3), ("refit", 37), ("rebuild", 50)):
+        xs = [waves[m].frame(frame) for m in range(2)]
+        for t, x in zip(trees, xs):
+            t.set_positions(x)
+        if step == "refit":
+            sc.refit()
+        else:
+            sc.build()
+        ot = []
+        for t, x, (_, f) in zip(trees, xs, meshes):
+            o = orc.Tree(x, f, order="morton") if step != "refit" else orc.Tree(x, f, order=t.perm())
+            assert np.array_equal(t.perm(), o.order), step
+            assert np.array_equal(bits(t.nodes()), bits(o.nodes())), step
+            ot.append(o)
+        sc.detectCollision(8, 2)
+        want = orc.detect(ot)
+        assert len(want) > 1000
+        assert np.array_equal(sc.getIntTriPairs(), want), step
+    sc.close()
+
+
+def test_cloth_self_collision_vs_oracle(api, ctx, orc):
+    """A folded cloth sheet of 512 x 512 vertices (522,242 triangles), the
+    self-collision workload's shape at a quarter of its side: the self-mode
+    pair set against the oracle, after the build and after a refit to the
+    next phase of the height field."""
+    say("cloth self-collision 512 x 512")
+    x, f = M.cloth(512, amplitude=0.2, wavelength=0.05, seed=11)
+    t = gpu_tree(api, ctx, x, f)
+    sc = api.Scene(ctx)
+    sc.addOibvhTree(t)
+    for phase in (0.0, 0.02):
+        xp = M.cloth_positions(512, amplitude=0.2, wavelength=0.05, seed=11, phase=phase)
+        if phase:
+            t.refit(xp)
+        sc.detectCollision(4, 2, self_collide=True)
+        want = orc.detect([orc.Tree(xp, f, order=t.perm())], self_collide=True)
+        assert len(want) > 1000
+        assert np.array_equal(sc.getIntTriPairs(), want), phase
+    sc.close()
+
+
 def test_refit_from_device_pointer(api, ctx, orc):
     say("refit from borrowed device positions")
     meshes, waves = M.deforming_uv_pair(120, 61)
