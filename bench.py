@@ -846,9 +846,11 @@ def run_self(args, ctx):
     # walks of at most 128 front pairs per workgroup over the last one or two
     # steps cannot overflow (two levels of 4 x 4 children: 16 x 128 = 2048
     # pairs, the LDS queue) and keep the largest fronts out of HBM
+    depth = max((int(f.shape[0]) - 1).bit_length() for _, f in base)
     tuned = autotune.tune_batch(ctx, b, stage, entries=(4, 6), expands=(2, 3), blocks=(0,), leaves=(0,),
                                 local=(), tail=((32, 1), (64, 2), (128, 2)),
-                                flags={"self_collide": True, "overlap": True}, refit_blocks=(0, 1024), reps=4)
+                                flags={"self_collide": True, "overlap": True}, refit_blocks=(0, 1024), reps=4,
+                                depth=depth)
     progress(f"self: tuned {tuned}")
     entry, expand = tuned["entry_level"], tuned["expand_levels"]
     stage(0)

@@ -99,9 +99,24 @@ def tune(ctx: api.Context, scene: api.Scene, frame, *, entries=(4, 6, 8), expand
     return best
 
 
+def deepen_last_step(depth: int, entry: int, expand: int) -> int:
+    """traverse_schedule word for steps of `expand` levels from `entry` with a
+    last step of expand + 1 that ends at `depth` (one front fewer than steps
+    of `expand` only), or 0 when no such split exists."""
+    if expand >= 3:
+        return 0
+    rest = depth - entry - (expand + 1)
+    if rest <= 0 or rest % expand or rest // expand > 16:
+        return 0
+    word = 0
+    for k in range(rest // expand):
+        word |= expand << (2 * k)
+    return word
+
+
 def tune_batch(ctx: api.Context, batch: api.Batch, before_frame, *, entries=(0, 2, 4), expands=(1, 2, 3),
                blocks=(0, 4096, 8192), leaves=(0, 4, 8), local=((64, 2), (256, 3)), tail=(), refit_blocks=(0,),
-               flags=None, reps: int = 3) -> dict:
+               flags=None, reps: int = 3, depth: int = 0) -> dict:
     """Pick the reduce subtree size (leaves per thread), the BVTT entry level,
     the levels per step and the workgroups per launch for a batch's captured
     frame (refit + detect). before_frame(i) stages frame i's inputs (model
@@ -109,7 +124,9 @@ def tune_batch(ctx: api.Context, batch: api.Batch, before_frame, *, entries=(0, 
     lists tail-walk settings to try: a batch's late fronts are large, and a
     tail walk that outgrows LDS on a later frame costs that frame a re-run.
     `refit_blocks` lists caps on an overlapped frame's refit workgroups (with
-    flags {"overlap": True}; none changes results)."""
+    flags {"overlap": True}; none changes results). With `depth` (the deepest
+    tree's depth) a last step one level deeper than the others is tried as
+    well (deepen_last_step)."""
     flags = flags or {}
     base = ctx.get_tuning()
     timer = api.Timer(ctx)
@@ -147,6 +164,15 @@ def tune_batch(ctx: api.Context, batch: api.Batch, before_frame, *, entries=(0, 
                     "reduce_leaves_per_thread": lpt_best, "traverse_local": 0, "traverse_local_levels": 0,
                     "traverse_tail": 0, "traverse_tail_steps": 0, "ms": ms}
     ctx.set_tuning(traverse_blocks=best["traverse_blocks"], traverse_expand=best["expand_levels"])
+    best["traverse_schedule"] = 0
+    word = deepen_last_step(depth, best["entry_level"], best["expand_levels"]) if depth else 0
+    if word:
+        e = best["expand_levels"] + 1
+        ctx.set_tuning(traverse_expand=e, traverse_schedule=word)
+        ms = run(best["entry_level"], e)
+        if ms < best["ms"]:
+            best = {**best, "expand_levels": e, "traverse_schedule": word, "ms": ms}
+        ctx.set_tuning(traverse_expand=best["expand_levels"], traverse_schedule=best["traverse_schedule"])
     for chunk, levels in local:
         ctx.set_tuning(traverse_local=chunk, traverse_local_levels=levels)
         ms = run(best["entry_level"], best["expand_levels"])
@@ -171,6 +197,7 @@ def tune_batch(ctx: api.Context, batch: api.Batch, before_frame, *, entries=(0, 
     ctx.set_tuning(refit_blocks=best["refit_blocks"], reduce_leaves_per_thread=best["reduce_leaves_per_thread"])
     ctx.set_tuning(**{**base, "reduce_leaves_per_thread": best["reduce_leaves_per_thread"],
                       "traverse_blocks": best["traverse_blocks"], "traverse_expand": best["expand_levels"],
+                      "traverse_schedule": best["traverse_schedule"],
                       "traverse_local": best["traverse_local"],
                       "traverse_local_levels": best["traverse_local_levels"],
                       "traverse_tail": best["traverse_tail"], "traverse_tail_steps": best["traverse_tail_steps"],
