@@ -282,15 +282,27 @@ def test_batch_frame_many_apex_meshes(api, ctx, orc):
     b.close()
 
 
-def test_batch_self_collision(api, ctx, orc):
-    say("batch: cloth self collision")
+@pytest.mark.parametrize("schedule", [0, 0b1010])
+def test_batch_self_collision(api, ctx, orc, schedule):
+    """Cloth sheets in self mode, as one batch, against the oracle; also with
+    steps of 2 levels and a deeper uniform 3 after them (traverse_schedule),
+    eager and from a captured overlapped frame."""
+    say(f"batch: cloth self collision, schedule {schedule:#x}")
     cl = [M.cloth(24 + 6 * i, 0.2, 0.25, seed=40 + i) for i in range(3)]
     b = api.Batch(ctx)
     for s, (x, f) in enumerate(cl):
         b.add_mesh(s, x, f)
     b.build()
-    b.detect(2, 2, self_collide=True)
-    got = b.pairs()
+    expand = 3 if schedule else 2
+    ctx.set_tuning(traverse_expand=expand if schedule else 0, traverse_schedule=schedule)
+    try:
+        b.detect(2, expand, self_collide=True)
+        got = b.pairs()
+        b.frame(2, expand, self_collide=True, overlap=True)
+        b.frame(2, expand, self_collide=True, overlap=True)
+        assert np.array_equal(b.pairs(), got)
+    finally:
+        ctx.set_tuning(traverse_expand=0, traverse_schedule=0)
     rows = []
     for s, (x, f) in enumerate(cl):
         p = orc.detect([orc.Tree(x, f, order="morton")], self_collide=True)

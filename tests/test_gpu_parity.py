@@ -828,6 +828,28 @@ def test_detect_tail_walk_overflow_falls_back(api, ctx, orc):
         ctx2.close()
 
 
+@pytest.mark.parametrize("expand,schedule", [(3, 0b1010), (3, 0b01), (2, 0b1111), (3, 0xAAAA), (1, 0b111011)])
+def test_detect_step_schedule(api, ctx, orc, expand, schedule):
+    """traverse_schedule: the first steps descend the listed levels (2 bits
+    per step), later steps the uniform setting; pairs as the oracle's, for
+    two meshes and for a cloth in self mode, including a schedule longer
+    than the walk."""
+    say(f"detect schedule {schedule:#x} then {expand}")
+    meshes = M.interpenetrating_icospheres(subdiv=4, seed=31, offset=1.2)
+    want = orc.detect([orc.Tree(x, f) for x, f in meshes])
+    x, f = M.cloth(64, amplitude=0.2, wavelength=0.2, seed=9)
+    want_self = orc.detect([orc.Tree(x, f)], self_collide=True)
+    ctx.set_tuning(traverse_expand=expand, traverse_schedule=schedule)
+    try:
+        for entry in (0, 3):
+            p, _, _ = gpu_pairs(api, ctx, meshes, entry, expand)
+            assert np.array_equal(p, want), entry
+        p, _, _ = gpu_pairs(api, ctx, [(x, f)], 2, expand, self_collide=True)
+        assert np.array_equal(p, want_self)
+    finally:
+        ctx.set_tuning(traverse_expand=0, traverse_schedule=0)
+
+
 def test_detect_many_meshes(api, ctx, orc):
     say("detect 6 meshes")
     r = M.Rng(99)
