@@ -259,7 +259,7 @@ def run_batched(args, ctx, out):
         b.set_transforms_device(W.xbuf.at((i % W.nf) * W.stride))
 
     progress(f"batched: built in {t_build:.2f} s; autotuning")
-    tuned = autotune.tune_batch(ctx, b, stage, flags={"fp64_fallback": True}, reps=args.tune_reps)
+    tuned = autotune.tune_batch(ctx, b, stage, flags={"fp64_fallback": True}, reps=args.tune_reps, narrow=(0, 1))
     progress(f"batched: tuned {tuned}")
     entry, expand = tuned["entry_level"], tuned["expand_levels"]
     # frames one after another, then overlapped (refit of frame f beside the
@@ -558,7 +558,7 @@ def run_instanced(args, ctx, W):
         b.set_transforms_device(W.xbuf.at((i % W.nf) * W.stride))
 
     base = ctx.get_tuning()
-    tuned = autotune.tune_batch(ctx, b, stage, flags={"fp64_fallback": True}, reps=args.tune_reps)
+    tuned = autotune.tune_batch(ctx, b, stage, flags={"fp64_fallback": True}, reps=args.tune_reps, narrow=(0, 1))
     ms, tm, st, _ = _time_batched(args, api, ctx, b, W, tuned["entry_level"], tuned["expand_levels"],
                                   max(1, args.steps), max(0, args.warmup), False)
     res = {"workload": f"as the headline, the {W.n} meshes rigid instances of {len(W.rs.library)} base shapes",
@@ -850,7 +850,7 @@ def run_self(args, ctx):
     tuned = autotune.tune_batch(ctx, b, stage, entries=(4, 6), expands=(2, 3), blocks=(0,), leaves=(0,),
                                 local=(), tail=((32, 1), (64, 2), (128, 2)),
                                 flags={"self_collide": True, "overlap": True}, refit_blocks=(0, 1024), reps=4,
-                                depth=depth)
+                                depth=depth, narrow=(0, 1))
     progress(f"self: tuned {tuned}")
     entry, expand = tuned["entry_level"], tuned["expand_levels"]
     stage(0)

@@ -116,7 +116,7 @@ def deepen_last_step(depth: int, entry: int, expand: int) -> int:
 
 def tune_batch(ctx: api.Context, batch: api.Batch, before_frame, *, entries=(0, 2, 4), expands=(1, 2, 3),
                blocks=(0, 4096, 8192), leaves=(0, 4, 8), local=((64, 2), (256, 3)), tail=(), refit_blocks=(0,),
-               flags=None, reps: int = 3, depth: int = 0) -> dict:
+               flags=None, reps: int = 3, depth: int = 0, narrow=(0,)) -> dict:
     """Pick the reduce subtree size (leaves per thread), the BVTT entry level,
     the levels per step and the workgroups per launch for a batch's captured
     frame (refit + detect). before_frame(i) stages frame i's inputs (model
@@ -126,7 +126,9 @@ def tune_batch(ctx: api.Context, batch: api.Batch, before_frame, *, entries=(0, 
     `refit_blocks` lists caps on an overlapped frame's refit workgroups (with
     flags {"overlap": True}; none changes results). With `depth` (the deepest
     tree's depth) a last step one level deeper than the others is tried as
-    well (deepen_last_step)."""
+    well (deepen_last_step). `narrow` lists narrow_separate settings to try
+    (FP32 queries: leaf pairs tested inside the steps, or in one k_narrow
+    launch)."""
     flags = flags or {}
     base = ctx.get_tuning()
     timer = api.Timer(ctx)
@@ -185,6 +187,15 @@ def tune_batch(ctx: api.Context, batch: api.Batch, before_frame, *, entries=(0, 
         if ms < best["ms"]:
             best = {**best, "traverse_tail": chunk, "traverse_tail_steps": steps, "ms": ms}
     ctx.set_tuning(traverse_tail=best["traverse_tail"], traverse_tail_steps=best["traverse_tail_steps"])
+    best["narrow_separate"] = base.get("narrow_separate", 0)
+    for ns in narrow:
+        if ns == best["narrow_separate"]:
+            continue
+        ctx.set_tuning(narrow_separate=ns)
+        ms = run(best["entry_level"], best["expand_levels"])
+        if ms < best["ms"]:
+            best = {**best, "narrow_separate": ns, "ms": ms}
+    ctx.set_tuning(narrow_separate=best["narrow_separate"])
     best["refit_blocks"] = base.get("refit_blocks", 0)
     # a cap applies to the wave-subtree refit (16 leaves per thread) only:
     # each capped candidate runs with it
@@ -201,7 +212,7 @@ def tune_batch(ctx: api.Context, batch: api.Batch, before_frame, *, entries=(0, 
                       "traverse_local": best["traverse_local"],
                       "traverse_local_levels": best["traverse_local_levels"],
                       "traverse_tail": best["traverse_tail"], "traverse_tail_steps": best["traverse_tail_steps"],
-                      "refit_blocks": best["refit_blocks"]})
+                      "narrow_separate": best["narrow_separate"], "refit_blocks": best["refit_blocks"]})
     return best
 
 
