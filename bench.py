@@ -111,6 +111,9 @@ def parse():
     p.add_argument("--deforming-tuning", default="",
                    help="JSON object: fixed entry_level, expand_levels and tuning knobs for the deforming section "
                         "instead of the autotuner (counter runs)")
+    p.add_argument("--self-tuning", default="",
+                   help="JSON object: fixed entry_level, expand_levels and tuning knobs for the self-collision "
+                        "section instead of the autotuner (counter runs)")
     p.add_argument("--copy-probe-mb", type=int, default=0,
                    help="first copy this many MiB device to device (a kernel of known traffic, for counter scaling)")
     return p.parse_args()
@@ -271,19 +274,23 @@ def run_batched(args, ctx, out):
                            "detect_ms": tm["detect_ms"] / max(1, tm["detects"]), "tuning": ctx.get_tuning()}
     progress(f"batched: {1000.0 / ms_b:.1f} frames/s one after another")
     chosen = ("sequential", ms_b, tm, st_b, f_last, ctx.get_tuning())
+    modes["sequential"]["autotune"] = tuned
     if overlap:
         seq_tuning = ctx.get_tuning()
-        ov = autotune.tune_batch(ctx, b, stage, entries=(entry,), expands=(expand,),
-                                 blocks=(tuned["traverse_blocks"],), leaves=(tuned["reduce_leaves_per_thread"],),
-                                 local=(), flags={"fp64_fallback": True, "overlap": True},
-                                 refit_blocks=(0, 512, 1024, 2048), reps=args.tune_reps)
-        ms_o, tm_o, st_o, fl_o = _time_batched(args, api, ctx, b, W, entry, expand, K, Wm, True)
+        # tuned again with overlapped frames: a detect that leaves room on the
+        # GPU for the refit beside it can win there while losing alone
+        ov = autotune.tune_batch(ctx, b, stage, flags={"fp64_fallback": True, "overlap": True},
+                                 refit_blocks=(0, 512, 1024, 2048), reps=args.tune_reps, narrow=(0, 1))
+        progress(f"batched: tuned for overlapped frames {ov}")
+        e_o, x_o = ov["entry_level"], ov["expand_levels"]
+        ms_o, tm_o, st_o, fl_o = _time_batched(args, api, ctx, b, W, e_o, x_o, K, Wm, True)
         modes["overlapped"] = {"fps": 1000.0 / ms_o, "refit_stage1_ms": tm_o["refit_ms"] / max(1, tm_o["refits"]),
                                "detect_ms": tm_o["detect_ms"] / max(1, tm_o["detects"]),
-                               "refit_blocks": ov["refit_blocks"], "tuning": ctx.get_tuning()}
+                               "refit_blocks": ov["refit_blocks"], "tuning": ctx.get_tuning(), "autotune": ov}
         progress(f"batched: {1000.0 / ms_o:.1f} frames/s overlapped")
         if ms_o < ms_b:
             chosen = ("overlapped", ms_o, tm_o, st_o, fl_o, ctx.get_tuning())
+            entry, expand, tuned = e_o, x_o, ov
         else:
             ctx.set_tuning(**seq_tuning)
             # the comparisons below read the last frame: run it again in the
@@ -402,6 +409,13 @@ def run_batched(args, ctx, out):
         "mesh_pairs_overlapping": hit_pairs, "mesh_pairs_total": int(sum(k * (k - 1) // 2 for k in per_scene)),
         "kernels": kernels,
         "roofline": roofline,
+        # the frame as a whole: refit and detect algorithmic bytes over the
+        # frame time (with overlapped frames both halves share HBM, so each
+        # kernel's own duration stretches while the frame shortens)
+        "frame_roofline": {"bound": "hbm", "alg_bytes_per_frame": refit_bytes + detect_bytes,
+                           "achieved": (refit_bytes + detect_bytes) / (ms_b * 1e-3) / 1e9,
+                           "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                           "frac": (refit_bytes + detect_bytes) / (ms_b * 1e-3) / HBM_PEAK},
         "cpu_baseline": cpu,
         "autotune": tuned,
         "build_s": t_build, "host_gen_s": W.t_gen + t_geo,
@@ -547,8 +561,8 @@ def cpu_baseline_batched(args, W, geoms, b):
 
 def run_instanced(args, ctx, W):
     """The same scenes with every mesh a rigid instance of one of the base
-    shapes (shared vertex and face buffers): batched form only, frames one
-    after another."""
+    shapes (shared vertex and face buffers): batched form only, overlapped
+    frames (unless --no-overlap), tuned for them."""
     from oibvh_amd import api, autotune
 
     progress("instanced: building")
@@ -558,10 +572,14 @@ def run_instanced(args, ctx, W):
         b.set_transforms_device(W.xbuf.at((i % W.nf) * W.stride))
 
     base = ctx.get_tuning()
-    tuned = autotune.tune_batch(ctx, b, stage, flags={"fp64_fallback": True}, reps=args.tune_reps, narrow=(0, 1))
+    overlap = not args.no_overlap
+    tuned = autotune.tune_batch(ctx, b, stage, flags={"fp64_fallback": True, "overlap": overlap},
+                                refit_blocks=(0, 512, 1024, 2048) if overlap else (0,), reps=args.tune_reps,
+                                narrow=(0, 1))
     ms, tm, st, _ = _time_batched(args, api, ctx, b, W, tuned["entry_level"], tuned["expand_levels"],
-                                  max(1, args.steps), max(0, args.warmup), False)
+                                  max(1, args.steps), max(0, args.warmup), overlap)
     res = {"workload": f"as the headline, the {W.n} meshes rigid instances of {len(W.rs.library)} base shapes",
+           "frames": "overlapped" if overlap else "one after another",
            "fps": 1000.0 / ms, "ms_per_frame": ms, "mtri_per_s": W.tri / (ms * 1e-3) / 1e6,
            "refit_stage1_ms": tm["refit_ms"] / max(1, tm["refits"]),
            "detect_ms": tm["detect_ms"] / max(1, tm["detects"]),
@@ -847,10 +865,17 @@ def run_self(args, ctx):
     # steps cannot overflow (two levels of 4 x 4 children: 16 x 128 = 2048
     # pairs, the LDS queue) and keep the largest fronts out of HBM
     depth = max((int(f.shape[0]) - 1).bit_length() for _, f in base)
-    tuned = autotune.tune_batch(ctx, b, stage, entries=(4, 6), expands=(2, 3), blocks=(0,), leaves=(0,),
-                                local=(), tail=((32, 1), (64, 2), (128, 2)),
-                                flags={"self_collide": True, "overlap": True}, refit_blocks=(0, 1024), reps=4,
-                                depth=depth, narrow=(0, 1))
+    if args.self_tuning:
+        # fixed settings (counter runs, whose passes must see the same kernels)
+        tuned = json.loads(args.self_tuning)
+        ctx.set_tuning(**{k: v for k, v in tuned.items() if k not in ("entry_level", "expand_levels")},
+                       traverse_expand=tuned["expand_levels"])
+        tuned = {"fixed": True, **tuned}
+    else:
+        tuned = autotune.tune_batch(ctx, b, stage, entries=(4, 6), expands=(2, 3), blocks=(0,), leaves=(0,),
+                                    local=(), tail=((32, 1), (64, 2), (128, 2)),
+                                    flags={"self_collide": True, "overlap": True}, refit_blocks=(0, 1024), reps=4,
+                                    depth=depth, narrow=(0, 1))
     progress(f"self: tuned {tuned}")
     entry, expand = tuned["entry_level"], tuned["expand_levels"]
     stage(0)

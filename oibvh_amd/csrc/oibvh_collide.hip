@@ -646,8 +646,14 @@ __device__ __forceinline__ void fused_narrow(const Tab& T, const uint32_t* s_iba
 // workgroup that found them (fused_narrow) instead of going to the candidate
 // list; FUSE 2: with the FP32 filter, the pairs it leaves undecided going to
 // A.defer for the FP64 pass.
+// Occupancy: the step waits on its child-run loads, so resident waves matter
+// more than loads in flight per wave. E = 3 holds two runs of 8 boxes and
+// would take 153-161 VGPRs (3 waves per SIMD); capped at 128 (4 waves) it
+// keeps the runs in registers (the fused forms spill 2-5 dwords). E = 2 fits
+// 80 VGPRs (6 waves) unspilled.
 template <int E, bool FIRST, int FUSE, class Tab>
-__global__ void __launch_bounds__(TB) k_bvtt_step(StepArgs<Tab> A)
+__global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(E == 3 ? 4 : E == 2 ? 6 : 1, 8)))
+k_bvtt_step(StepArgs<Tab> A)
 {
     constexpr bool FUSED = FUSE != 0;
     constexpr int K = 1 << E;
