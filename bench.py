@@ -105,6 +105,7 @@ def parse():
     p.add_argument("--no-plain", action="store_true")
     p.add_argument("--tune-reps", type=int, default=3)
     p.add_argument("--rebuild-every", type=int, default=50)
+    p.add_argument("--rebuild-reps", type=int, default=21, help="timed rebuilds after the sequence (median)")
     p.add_argument("--precision-frames", type=int, default=20)
     p.add_argument("--self-frames", type=int, default=100)
     p.add_argument("--cloth-nv", type=int, default=2048)
@@ -279,8 +280,10 @@ def run_batched(args, ctx, out):
         seq_tuning = ctx.get_tuning()
         # tuned again with overlapped frames: a detect that leaves room on the
         # GPU for the refit beside it can win there while losing alone
+        depth = max((int(F) - 1).bit_length() for F in W.faces)
         ov = autotune.tune_batch(ctx, b, stage, flags={"fp64_fallback": True, "overlap": True},
-                                 refit_blocks=(0, 512, 1024, 2048), reps=args.tune_reps, narrow=(0, 1))
+                                 blocks=(0, 512, 2048, 4096, 8192), refit_blocks=(0, 512, 1024, 2048),
+                                 reps=args.tune_reps, narrow=(0, 1), depth=depth)
         progress(f"batched: tuned for overlapped frames {ov}")
         e_o, x_o = ov["entry_level"], ov["expand_levels"]
         ms_o, tm_o, st_o, fl_o = _time_batched(args, api, ctx, b, W, e_o, x_o, K, Wm, True)
@@ -679,20 +682,26 @@ def run_deforming(args, ctx):
     tm = scene.timing()
     scene.set_timing(0)
     # the rebuild's own cost, event-timed apart (four per sequence): both
-    # trees (oibvh_scene_build), and one tree after the other
+    # trees (oibvh_scene_build), and one tree after the other; medians over
+    # --rebuild-reps calls, each waited for
     tb = api.Timer(ctx)
-    ctx.synchronize()
-    tb.start()
-    scene.build()
-    ctx.synchronize()
-    tb.stop()
-    build_ms = tb.ms()
-    ctx.synchronize()
-    tb.start()
-    for t in trees:
-        t.build()
-    tb.stop()
-    build_seq_ms = tb.ms()
+    build_v, build_seq_v = [], []
+    for _ in range(max(1, args.rebuild_reps)):
+        ctx.synchronize()
+        tb.start()
+        scene.build()
+        ctx.synchronize()
+        tb.stop()
+        build_v.append(tb.ms())
+    for _ in range(max(1, args.rebuild_reps)):
+        ctx.synchronize()
+        tb.start()
+        for t in trees:
+            t.build()
+        tb.stop()
+        build_seq_v.append(tb.ms())
+    build_ms = float(np.median(build_v))
+    build_seq_ms = float(np.median(build_seq_v))
     ms_step = total_ms / n_frames
     fps = 1000.0 / ms_step
     refit_ms = tm["refit_ms"] / max(1, tm["refits"])
@@ -733,7 +742,9 @@ def run_deforming(args, ctx):
                         "along the normal by 0.05 sin(wt + k.p); the 200-frame sequence timed whole: refit every "
                         f"frame, rebuild of both trees every {rebuild} frames, broad + narrow phase every frame"),
            "fps": fps, "ms_per_frame": ms_step, "mtri_per_s": sum(Fm) * fps / 1e6, "frames": n_frames,
-           "rebuild_ms_both_trees": build_ms, "rebuild_ms_one_after_another": build_seq_ms, "frame_overlap": True,
+           "rebuild_ms_both_trees": build_ms, "rebuild_ms_one_after_another": build_seq_ms,
+           "rebuild_reps": len(build_v), "rebuild_ms_range": [float(min(build_v)), float(max(build_v))],
+           "frame_overlap": True,
            "entry_level": entry, "expand_levels": expand, "autotune": tuned, "autotune_refit_blocks": tuned_rb,
            "pair_diffs_last_frame_vs_oracle": diffs, "pairs_last_frame": int(len(pairs_last)),
            "async_errors": st["async_errors"], "retries_last_frame": st["retries"],

@@ -453,7 +453,7 @@ OIBVH_API oibvh_status oibvh_batch_get_perm(oibvh_batch* batch, uint32_t mesh, u
  * the same data plus oibvh_tree_build, bit for bit.
  *
  * Allocates nothing: the build's scratch (sort keys and values, digit
- * histograms, look-back tiles, mesh-box partials; about 12.6 MB) is a fixed
+ * histograms, look-back tiles, mesh-box partials; about 13.7 MB) is a fixed
  * device array the library holds on every GPU it runs on until the process
  * ends. F above OIBVH_NOALLOC_MAX_FACES is refused (OIBVH_ERR_CAPACITY).
  * Enqueued on the context's stream; nothing waits. All calls on one GPU share

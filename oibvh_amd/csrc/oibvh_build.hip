@@ -7,8 +7,9 @@
 //   k_morton       folds the block boxes, then leaf AABB centroid -> 30-bit
 //                  Morton key per face
 //                  (calculate_aabb_and_morton_kernel, oibvh.cu:42-70) and the
-//                  four 8-bit digit histograms of the keys
-//   k_sort_pass x4 onesweep LSD radix pass with decoupled look-back; stable,
+//                  three 10-bit digit histograms of the keys (8-bit digits
+//                  for 63-bit keys and batches)
+//   k_sort_pass x3 onesweep LSD radix pass with decoupled look-back; stable,
 //                  so the permutation equals thrust::stable_sort_by_key's
 //                  (oibvhTree.cu:287-297)
 //   k_reduce       leaves in sorted order + bottom-up AABB reduce, one subtree
@@ -33,6 +34,7 @@
 
 #define OI_BLOCK 256
 #define OI_MORTON_CHUNK 2048
+#define OI_HIST_WORDS 3072  // k_morton's digit histograms: 8 x 256 or 3 x 1024 bins
 
 __device__ __forceinline__ float gmin(float a, float b) { return (b < a) ? b : a; }
 __device__ __forceinline__ float gmax(float a, float b) { return (a < b) ? b : a; }
@@ -301,22 +303,24 @@ __device__ __forceinline__ void hist_add(uint32_t* h, uint32_t bin, bool active)
     }
 }
 
-// keys_hi == null: 30-bit keys and 4 digit histograms; else 63-bit keys split
-// into keys (low word) and keys_hi (high word) and 8 digit histograms
+// keys_hi == null: 30-bit keys and the histograms of 4 8-bit digits (hbits 8)
+// or of 3 10-bit digits (hbits 10); else 63-bit keys split into keys (low
+// word) and keys_hi (high word) and 8 8-bit digit histograms. Digit d's
+// histogram is ghist[d << hbits .. (d + 1) << hbits).
 __global__ void __launch_bounds__(OI_BLOCK)
 k_morton(BuildTab T, const float* __restrict__ partials, uint32_t* __restrict__ keys, uint32_t* __restrict__ keys_hi,
-         uint32_t* __restrict__ ghist, uint32_t hstride, uint32_t* __restrict__ fmesh)
+         uint32_t* __restrict__ ghist, uint32_t hstride, uint32_t* __restrict__ fmesh, uint32_t hbits)
 {
-    __shared__ uint32_t s_h[8][256];
+    __shared__ uint32_t s_h[OI_HIST_WORDS];
     __shared__ float s_v[6][OI_BLOCK];
     const uint32_t tid = threadIdx.x;
     const uint2 it = T.item();
     const oi_build_desc D = T.mesh(it.x);
     Xf X;
     X.load(D.xf);
-    const int nd = keys_hi ? 8 : 4;
-#pragma unroll
-    for (int d = 0; d < 8; d++) s_h[d][tid] = 0u;
+    const bool h10 = hbits == 10u;
+    const uint32_t nd = keys_hi ? 8u : h10 ? 3u : 4u;
+    for (uint32_t i = tid; i < OI_HIST_WORDS; i += OI_BLOCK) s_h[i] = 0u;
     float bmn[3], bmx[3];
     fold_partials(partials ? partials + 6u * D.pbase : D.box + 6, D.nparts, s_v, bmn, bmx);
     ghist += it.x * hstride;
@@ -377,29 +381,36 @@ k_morton(BuildTab T, const float* __restrict__ partials, uint32_t* __restrict__ 
             if (fmesh) oi_gw(fmesh)[D.fbase + i] = it.x;
         }
         // every lane of the wave takes part (ballots), dead lanes add nothing
+        if (h10) {
 #pragma unroll
-        for (int d = 0; d < 4; d++) hist_add(s_h[d], (key >> (8 * d)) & 0xffu, live);
+            for (int d = 0; d < 3; d++) hist_add(s_h + 1024 * d, (key >> (10 * d)) & 0x3ffu, live);
+        } else {
+#pragma unroll
+            for (int d = 0; d < 4; d++) hist_add(s_h + 256 * d, (key >> (8 * d)) & 0xffu, live);
+        }
         if (keys_hi) {
 #pragma unroll
-            for (int d = 0; d < 4; d++) hist_add(s_h[4 + d], (khi >> (8 * d)) & 0xffu, live);
+            for (int d = 0; d < 4; d++) hist_add(s_h + 256 * (4 + d), (khi >> (8 * d)) & 0xffu, live);
         }
     }
     __syncthreads();
-    for (int d = 0; d < nd; d++) {
-        uint32_t c = s_h[d][tid];
-        if (c) atomicAdd(&ghist[256 * d + tid], c);
+    const uint32_t words = nd << hbits;
+    for (uint32_t i = tid; i < words; i += OI_BLOCK) {
+        const uint32_t c = s_h[i];
+        if (c) atomicAdd(&ghist[i], c);
     }
 }
 
 // ---------------------------------------------------------------------------
-// k_sort_pass: one stable 8-bit LSD pass, onesweep style, over up to
+// k_sort_pass<DB>: one stable LSD pass over a DB-bit digit (8 or 10),
+// onesweep style, over up to
 // OI_SORT_SEGS independent segments of the concatenated key arrays (one per
 // tree when several trees are built in one launch).
 // A tile is 16 waves x 8 rounds x 64 lanes = 8192 keys of one segment in
 // input order (wave w owns keys [512w, 512w + 512) of the tile, round j lane l
 // holds key 512w + 64j + l): fewer, larger tiles than one wave per SIMD would
 // give shorten the look-back chains. Keys are ranked per wave with 8 ballots
-// per round (stable: earlier rounds and lower lanes come first), staged
+// per round (DB ballots; stable: earlier rounds and lower lanes come first), staged
 // through LDS in tile-local sorted order and written out in runs. Segment s
 // owns the workgroups and tile ids [tile0[s], tile0[s + 1]); a workgroup takes
 // its tile from its segment's atomic counter (tile_counter[s]), so every tile
@@ -450,23 +461,26 @@ __device__ __forceinline__ uint32_t block_excl_scan_w(uint32_t v, uint32_t* s_ws
     return before + x - v;
 }
 
+template <int DB>
 __global__ void __launch_bounds__(RS_THREADS)
 k_sort_pass(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
             uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, SortSegs S, uint32_t shift,
             const uint32_t* __restrict__ ghist, uint32_t* lookback, uint32_t* lookback_next,
             uint32_t* tile_counter, const uint32_t* __restrict__ key_of_val)
 {
+    constexpr uint32_t NB = 1u << DB;  // bins; at most RS_THREADS
+    static_assert(NB <= RS_THREADS, "one digit per thread");
     __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_whist[RS_WAVES][256];
-    __shared__ uint32_t s_lstart[256];
-    __shared__ uint32_t s_gbase[256];
+    __shared__ uint32_t s_whist[RS_WAVES][NB];
+    __shared__ uint32_t s_lstart[NB];
+    __shared__ uint32_t s_gbase[NB];
     __shared__ uint32_t s_wsum[RS_WAVES];
     __shared__ uint32_t s_keys[RS_TILE];
     __shared__ uint32_t s_vals[RS_TILE];
 
-    // threads tid < 256 also own digit tid
+    // threads tid < NB also own digit tid
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    const bool dig = tid < 256u;
+    const bool dig = tid < NB;
     // the workgroup's segment (constant indices keep the argument block out
     // of scratch memory)
     uint32_t seg = 0, seg0 = S.tile0[0], sbase = S.base[0], n = S.n[0];
@@ -483,12 +497,12 @@ k_sort_pass(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ v
         s_tile = __hip_atomic_fetch_add(tile_counter + seg, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     // the histogram entry is loaded while the tile id is in flight
     const uint32_t hcnt = dig ? ghist[seg * S.hstride + tid] : 0u;
-    for (uint32_t i = tid; i < RS_WAVES * 256u; i += RS_THREADS) (&s_whist[0][0])[i] = 0u;
+    for (uint32_t i = tid; i < RS_WAVES * NB; i += RS_THREADS) (&s_whist[0][0])[i] = 0u;
     __syncthreads();
     const uint32_t tile = s_tile;
     const uint32_t gt = seg0 + tile;  // tile id over all segments
     const uint32_t base = tile * RS_TILE;
-    if (lookback_next && dig) lookback_next[gt * 256u + tid] = 0u;
+    if (lookback_next && dig) lookback_next[gt * NB + tid] = 0u;
 
     // exclusive prefix of the segment's digit histogram
     uint32_t gtot;
@@ -509,10 +523,10 @@ k_sort_pass(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ v
     for (int j = 0; j < RS_ITEMS; j++) {
         const uint32_t idx = wbase + 64u * j + lane;
         const bool ok = idx < n;
-        const uint32_t d = (k[j] >> shift) & 0xffu;
+        const uint32_t d = (k[j] >> shift) & (NB - 1u);
         unsigned long long peers = __ballot(ok);
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
+        for (int b = 0; b < DB; b++) {
             const bool bit = (d >> b) & 1u;
             const unsigned long long m = __ballot(bit);
             peers &= bit ? m : ~m;
@@ -536,7 +550,7 @@ k_sort_pass(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ v
     }
     // publish this tile's aggregate (or inclusive prefix for the segment's
     // first tile)
-    uint32_t* my = lookback + gt * 256u + tid;
+    uint32_t* my = lookback + gt * NB + tid;
     const bool live = dig && hcnt != 0u;
     if (live) st_relaxed(my, (tile == 0 ? LB_PRE : LB_AGG) | tot);
     uint32_t ltot;
@@ -558,7 +572,7 @@ k_sort_pass(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ v
 #pragma unroll
             for (int w = 0; w < LB_WIN; w++) {
                 const int32_t t = i - w;
-                st[w] = t >= 0 ? ld_acq_relaxed(lookback + (uint32_t)t * 256u + tid) : 0u;
+                st[w] = t >= 0 ? ld_acq_relaxed(lookback + (uint32_t)t * NB + tid) : 0u;
             }
             int32_t used = LB_WIN;
 #pragma unroll
@@ -585,7 +599,7 @@ k_sort_pass(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ v
     for (int j = 0; j < RS_ITEMS; j++) {
         const uint32_t idx = wbase + 64u * j + lane;
         if (idx < n) {
-            const uint32_t d = (k[j] >> shift) & 0xffu;
+            const uint32_t d = (k[j] >> shift) & (NB - 1u);
             const uint32_t lp = s_lstart[d] + s_whist[wave][d] + rk[j];
             s_keys[lp] = k[j];
             s_vals[lp] = v[j];
@@ -595,7 +609,7 @@ k_sort_pass(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ v
     const uint32_t cnt_tile = (n - base < RS_TILE) ? (n - base) : RS_TILE;
     for (uint32_t i = tid; i < cnt_tile; i += RS_THREADS) {
         const uint32_t key = s_keys[i];
-        const uint32_t d = (key >> shift) & 0xffu;
+        const uint32_t d = (key >> shift) & (NB - 1u);
         const uint32_t g = s_gbase[d] + (i - s_lstart[d]);
         if (keys_out) keys_out[g] = key;
         vals_out[g] = s_vals[i];
@@ -1279,10 +1293,11 @@ hipError_t oi_launch_set_xf(hipStream_t st, float* dst, const float* mat4)
 // keys_hi: null for 30-bit keys, else the high words of 63-bit keys
 hipError_t oi_launch_morton(hipStream_t st, const oi_build_desc* d, const uint2* work, const oi_build_desc* one,
                             uint32_t nblocks, const float* partials, uint32_t* keys, uint32_t* keys_hi,
-                            uint32_t* ghist, uint32_t* fmesh)
+                            uint32_t* ghist, uint32_t* fmesh, uint32_t hbits)
 {
+    if ((hbits != 8 && hbits != 10) || (keys_hi && hbits != 8)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_morton, dim3(nblocks), dim3(OI_BLOCK), 0, st, make_btab(d, work, one), partials, keys,
-                       keys_hi, ghist, 0u, fmesh);
+                       keys_hi, ghist, 0u, fmesh, hbits);
     return hipGetLastError();
 }
 
@@ -1300,11 +1315,13 @@ hipError_t oi_launch_mesh_box_inline(hipStream_t st, const oi_build_desc* meshes
 }
 
 hipError_t oi_launch_morton_inline(hipStream_t st, const oi_build_desc* meshes, uint32_t n, const uint32_t* boff,
-                                   uint32_t* keys, uint32_t* keys_hi, uint32_t* ghist, uint32_t hstride)
+                                   uint32_t* keys, uint32_t* keys_hi, uint32_t* ghist, uint32_t hstride,
+                                   uint32_t hbits)
 {
     if (n == 0 || n > OI_INLINE_MESHES) return hipErrorInvalidValue;
+    if ((hbits != 8 && hbits != 10) || (keys_hi && hbits != 8)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_morton, dim3(boff[n]), dim3(OI_BLOCK), 0, st, make_btab_inline(meshes, n, boff), nullptr,
-                       keys, keys_hi, ghist, hstride, nullptr);
+                       keys, keys_hi, ghist, hstride, nullptr, hbits);
     return hipGetLastError();
 }
 
@@ -1320,16 +1337,16 @@ hipError_t oi_launch_sort_pass_segs(hipStream_t st, const uint32_t* keys_in, con
                                     uint32_t* keys_out, uint32_t* vals_out, uint32_t ns, const uint32_t* base,
                                     const uint32_t* n, uint32_t shift, const uint32_t* ghist, uint32_t hstride,
                                     uint32_t* lookback, uint32_t* lookback_next, uint32_t* tile_counter,
-                                    const uint32_t* key_of_val);
+                                    const uint32_t* key_of_val, uint32_t bits);
 
 hipError_t oi_launch_sort_pass(hipStream_t st, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
                                uint32_t* vals_out, uint32_t n, uint32_t shift, const uint32_t* ghist,
                                uint32_t* lookback, uint32_t* lookback_next, uint32_t* tile_counter,
-                               const uint32_t* key_of_val)
+                               const uint32_t* key_of_val, uint32_t bits)
 {
     const uint32_t base = 0;
     return oi_launch_sort_pass_segs(st, keys_in, vals_in, keys_out, vals_out, 1, &base, &n, shift, ghist, 0,
-                                    lookback, lookback_next, tile_counter, key_of_val);
+                                    lookback, lookback_next, tile_counter, key_of_val, bits);
 }
 
 uint32_t oi_sort_segs_max() { return OI_SORT_SEGS; }
@@ -1342,9 +1359,9 @@ hipError_t oi_launch_sort_pass_segs(hipStream_t st, const uint32_t* keys_in, con
                                     uint32_t* keys_out, uint32_t* vals_out, uint32_t ns, const uint32_t* base,
                                     const uint32_t* n, uint32_t shift, const uint32_t* ghist, uint32_t hstride,
                                     uint32_t* lookback, uint32_t* lookback_next, uint32_t* tile_counter,
-                                    const uint32_t* key_of_val)
+                                    const uint32_t* key_of_val, uint32_t bits)
 {
-    if (ns == 0 || ns > OI_SORT_SEGS) return hipErrorInvalidValue;
+    if (ns == 0 || ns > OI_SORT_SEGS || (bits != 8 && bits != 10)) return hipErrorInvalidValue;
     SortSegs S{};
     S.ns = ns;
     S.hstride = hstride;
@@ -1357,8 +1374,12 @@ hipError_t oi_launch_sort_pass_segs(hipStream_t st, const uint32_t* keys_in, con
         tiles += (n[k] + RS_TILE - 1) / RS_TILE;
     }
     S.tile0[ns] = tiles;
-    hipLaunchKernelGGL(k_sort_pass, dim3(tiles), dim3(RS_THREADS), 0, st, keys_in, vals_in, keys_out, vals_out, S,
-                       shift, ghist, lookback, lookback_next, tile_counter, key_of_val);
+    if (bits == 10)
+        hipLaunchKernelGGL(k_sort_pass<10>, dim3(tiles), dim3(RS_THREADS), 0, st, keys_in, vals_in, keys_out,
+                           vals_out, S, shift, ghist, lookback, lookback_next, tile_counter, key_of_val);
+    else
+        hipLaunchKernelGGL(k_sort_pass<8>, dim3(tiles), dim3(RS_THREADS), 0, st, keys_in, vals_in, keys_out,
+                           vals_out, S, shift, ghist, lookback, lookback_next, tile_counter, key_of_val);
     return hipGetLastError();
 }
 

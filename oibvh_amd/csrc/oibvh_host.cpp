@@ -139,9 +139,6 @@ struct oibvh_scene {
     uint32_t *b_keys = nullptr, *b_keys_alt = nullptr, *b_vals_tmp = nullptr, *b_vals = nullptr;
     uint32_t *b_keys_hi = nullptr, *b_sort = nullptr;
     uint64_t b_cap = 0, b_hi_cap = 0, b_sort_cap = 0;
-    // the last build's launch key and, once a key repeats, its graph
-    std::vector<uint8_t> b_key;
-    hipGraphExec_t b_graph = nullptr;
 };
 
 // ---------------------------------------------------------------- context
@@ -466,24 +463,31 @@ oibvh_status oi_enqueue_build(oibvh_ctx* c, const oi_build_desc& d, const oi_bui
     const bool m63 = s.keys_hi != nullptr;
     uint32_t* ghist = s.sort;
     uint32_t* tctr = s.sort + OI_SORT_HIST;
-    uint32_t* lb[2] = {s.sort + OI_SORT_HEAD, s.sort + OI_SORT_HEAD + 256ull * tiles};
+    uint32_t* lb[2] = {s.sort + OI_SORT_HEAD, s.sort + OI_SORT_HEAD + (uint64_t)OI_SORT_LB * tiles};
     hipStream_t st = stream ? stream : c->stream;
     // the mesh-box launch also clears the histograms, tile counters and the
     // first look-back buffer (a kernel, not a memset: capturable)
-    HIPCHK(c, oi_launch_mesh_box(st, nullptr, nullptr, &d, d.nparts, d.box + 6, s.sort, OI_SORT_HEAD + 256u * tiles));
+    HIPCHK(c, oi_launch_mesh_box(st, nullptr, nullptr, &d, d.nparts, d.box + 6, s.sort,
+                                 OI_SORT_HEAD + OI_SORT_LB * tiles));
     const uint32_t mblocks = (F + oi_morton_chunk() - 1) / oi_morton_chunk();
-    HIPCHK(c, oi_launch_morton(st, nullptr, nullptr, &d, mblocks, d.box + 6, s.keys, s.keys_hi, ghist, nullptr));
-    // LSD passes over the low word (keys move with the values), then, for
-    // 63-bit keys, over the high word looked up by face id; values end in perm
-    const int np = m63 ? 8 : 4;
+    // 30-bit keys: three passes of 10-bit digits; 63-bit keys: four 8-bit
+    // passes over the low word (keys move with the values), then four over
+    // the high word looked up by face id. Values end in perm.
+    const uint32_t bits = m63 ? 8u : OI_M30_BITS;
+    HIPCHK(c, oi_launch_morton(st, nullptr, nullptr, &d, mblocks, d.box + 6, s.keys, s.keys_hi, ghist, nullptr,
+                               bits));
+    const int np = m63 ? 8 : OI_M30_PASSES, nlo = m63 ? 4 : OI_M30_PASSES;
     for (int p = 0; p < np; p++) {
-        const bool lo = p < 4;
+        const bool lo = p < nlo;
         const uint32_t* kin = lo ? ((p & 1) ? s.keys_alt : s.keys) : nullptr;
-        uint32_t* kout = (lo && p < 3) ? ((p & 1) ? s.keys : s.keys_alt) : nullptr;
-        const uint32_t* vin = p == 0 ? nullptr : ((p & 1) ? s.vals_tmp : d.perm);
-        uint32_t* vout = (p & 1) ? d.perm : s.vals_tmp;
-        HIPCHK(c, oi_launch_sort_pass(st, kin, vin, kout, vout, F, 8u * (p & 3), ghist + 256 * p, lb[p & 1],
-                                      p + 1 < np ? lb[(p + 1) & 1] : nullptr, tctr + p, lo ? nullptr : s.keys_hi));
+        uint32_t* kout = (lo && p + 1 < nlo) ? ((p & 1) ? s.keys : s.keys_alt) : nullptr;
+        // the passes alternate between perm and vals_tmp, the last writing perm
+        uint32_t* vout = ((np - 1 - p) & 1) ? s.vals_tmp : d.perm;
+        const uint32_t* vin = p == 0 ? nullptr : (((np - p) & 1) ? s.vals_tmp : d.perm);
+        const uint32_t shift = bits * (uint32_t)(p % nlo);
+        HIPCHK(c, oi_launch_sort_pass(st, kin, vin, kout, vout, F, shift, ghist + (p << bits), lb[p & 1],
+                                      p + 1 < np ? lb[(p + 1) & 1] : nullptr, tctr + p, lo ? nullptr : s.keys_hi,
+                                      bits));
     }
     const uint32_t h = oi_reduce_height(M, d.L);
     HIPCHK(c, oi_launch_reduce(st, M, nullptr, nullptr, &d, oi_lvl_cnt(d.L - h, d.L, d.vl), d.perm, 0));
@@ -508,7 +512,7 @@ static oibvh_status build_scratch(oibvh_tree* t, oi_build_scratch* out)
         HIPCHK(c, dalloc(&t->keys_alt, F));
         HIPCHK(c, dalloc(&t->vals_tmp, F));
         t->tiles = tiles;
-        HIPCHK(c, dalloc(&t->sort_scratch, OI_SORT_HEAD + 512ull * tiles));
+        HIPCHK(c, dalloc(&t->sort_scratch, OI_SORT_HEAD + 2ull * OI_SORT_LB * tiles));
     }
     if (m63 && !t->keys_hi) HIPCHK(c, dalloc(&t->keys_hi, F));
     *out = oi_build_scratch{t->keys, t->keys_alt, m63 ? t->keys_hi : nullptr, t->vals_tmp, t->sort_scratch};
@@ -697,7 +701,6 @@ extern "C" void oibvh_scene_destroy(oibvh_scene* s)
     dfree(s->b_vals);
     dfree(s->b_keys_hi);
     dfree(s->b_sort);
-    if (s->b_graph) (void)hipGraphExecDestroy(s->b_graph);
     for (hipEvent_t e : s->ev_refit) (void)hipEventDestroy(e);
     for (hipEvent_t e : s->ev_detect) (void)hipEventDestroy(e);
     oi_det_free(&s->det);
@@ -974,7 +977,7 @@ static oibvh_status scene_refit(oibvh_scene* s, const oi_seed_args* seed, bool* 
 static oibvh_status scene_build_scratch(oibvh_scene* s, uint64_t F, uint32_t n, uint32_t tiles, bool m63)
 {
     oibvh_ctx* c = s->ctx;
-    const uint64_t words = (uint64_t)n * OI_SORT_HIST + OI_SCENE_CTRS + 512ull * tiles;
+    const uint64_t words = (uint64_t)n * OI_SORT_HIST + OI_SCENE_CTRS + 2ull * OI_SORT_LB * tiles;
     const bool grow = F > s->b_cap || words > s->b_sort_cap || (m63 && F > s->b_hi_cap);
     if (!grow) return OIBVH_OK;
     HIPCHK(c, oi_ctx_quiesce(c));
@@ -1055,25 +1058,30 @@ static oibvh_status enqueue_group(oibvh_scene* s, const SceneBuildGroup& g)
     hipStream_t st = c->stream;
     uint32_t* ghist = s->b_sort;
     uint32_t* ctrs = ghist + (uint64_t)n * OI_SORT_HIST;
-    uint32_t* lb[2] = {ctrs + OI_SCENE_CTRS, ctrs + OI_SCENE_CTRS + 256ull * tiles};
+    uint32_t* lb[2] = {ctrs + OI_SCENE_CTRS, ctrs + OI_SCENE_CTRS + (uint64_t)OI_SORT_LB * tiles};
     // the mesh-box launch also clears the histograms, counters and the first
     // look-back buffer (a kernel, not a memset: capturable)
-    HIPCHK(c, oi_launch_mesh_box_inline(st, g.d, n, g.box_off, ghist, n * OI_SORT_HIST + OI_SCENE_CTRS + 256u * tiles));
+    HIPCHK(c, oi_launch_mesh_box_inline(st, g.d, n, g.box_off, ghist,
+                                        n * OI_SORT_HIST + OI_SCENE_CTRS + OI_SORT_LB * tiles));
+    // 30-bit keys: three passes of 10-bit digits; 63-bit keys: four 8-bit
+    // passes over the low words, then four over the high words looked up by
+    // value. The sorted values (face ids over the concatenated faces) end in
+    // b_vals.
+    const uint32_t bits = g.m63 ? 8u : OI_M30_BITS;
     HIPCHK(c, oi_launch_morton_inline(st, g.d, n, g.mort_off, s->b_keys, g.m63 ? s->b_keys_hi : nullptr, ghist,
-                                      OI_SORT_HIST));
-    // LSD passes over the low words, then, for 63-bit keys, over the high
-    // words looked up by value; the sorted values (face ids over the
-    // concatenated faces) end in b_vals
-    const int np = g.m63 ? 8 : 4;
+                                      OI_SORT_HIST, bits));
+    const int np = g.m63 ? 8 : OI_M30_PASSES, nlo = g.m63 ? 4 : OI_M30_PASSES;
     for (int p = 0; p < np; p++) {
-        const bool lo = p < 4;
+        const bool lo = p < nlo;
         const uint32_t* kin = lo ? ((p & 1) ? s->b_keys_alt : s->b_keys) : nullptr;
-        uint32_t* kout = (lo && p < 3) ? ((p & 1) ? s->b_keys : s->b_keys_alt) : nullptr;
-        const uint32_t* vin = p == 0 ? nullptr : ((p & 1) ? s->b_vals_tmp : s->b_vals);
-        uint32_t* vout = (p & 1) ? s->b_vals : s->b_vals_tmp;
-        HIPCHK(c, oi_launch_sort_pass_segs(st, kin, vin, kout, vout, n, g.base, g.nf, 8u * (p & 3), ghist + 256 * p,
-                                           OI_SORT_HIST, lb[p & 1], p + 1 < np ? lb[(p + 1) & 1] : nullptr,
-                                           ctrs + p * OI_SCENE_BUILD_GROUP, lo ? nullptr : s->b_keys_hi));
+        uint32_t* kout = (lo && p + 1 < nlo) ? ((p & 1) ? s->b_keys : s->b_keys_alt) : nullptr;
+        // the passes alternate between b_vals and b_vals_tmp, the last writing b_vals
+        uint32_t* vout = ((np - 1 - p) & 1) ? s->b_vals_tmp : s->b_vals;
+        const uint32_t* vin = p == 0 ? nullptr : (((np - p) & 1) ? s->b_vals_tmp : s->b_vals);
+        HIPCHK(c, oi_launch_sort_pass_segs(st, kin, vin, kout, vout, n, g.base, g.nf, bits * (uint32_t)(p % nlo),
+                                           ghist + (p << bits), OI_SORT_HIST, lb[p & 1],
+                                           p + 1 < np ? lb[(p + 1) & 1] : nullptr, ctrs + p * OI_SCENE_BUILD_GROUP,
+                                           lo ? nullptr : s->b_keys_hi, bits));
     }
     uint32_t na = 0;
     oi_build_desc apex[OI_SCENE_BUILD_GROUP];
@@ -1087,9 +1095,11 @@ static oibvh_status enqueue_group(oibvh_scene* s, const SceneBuildGroup& g)
 // OibvhTree::build on every tree of the scene (oibvhTree.cu:237-388). A single
 // tree's build is a chain of small launches that leave most of the GPU idle,
 // so up to four consecutive trees of the same Morton width share each launch
-// of the chain. A build whose launches equal the previous call's (same trees,
-// buffers and tuning) replays them from a hipGraph captured on its second
-// occurrence. Results equal oibvh_tree_build on each tree.
+// of the chain. Plain launches: replaying them from a captured hipGraph was
+// measured slower (164 against 157 us for two 1M-face trees, 94 against 90 us
+// for two 20K-face trees), the graph's launch costing more than the host
+// enqueues, which overlap the GPU's work. Results equal oibvh_tree_build on
+// each tree.
 extern "C" oibvh_status oibvh_scene_build(oibvh_scene* s)
 {
     if (!s) return OIBVH_ERR_INVALID;
@@ -1117,51 +1127,8 @@ extern "C" oibvh_status oibvh_scene_build(oibvh_scene* s)
     if (groups.empty()) return OIBVH_OK;
     oibvh_status r = scene_build_scratch(s, Fmax, nmax, tmax, m63);
     if (r != OIBVH_OK) return r;
-    // what the launches depend on: every group's geometry and descriptors,
-    // the scratch buffers and the stream
-    std::vector<uint8_t> key(groups.size() * sizeof(SceneBuildGroup) + 8 * sizeof(void*));
-    std::memset(key.data(), 0, key.size());
-    for (size_t k = 0; k < groups.size(); k++) std::memcpy(&key[k * sizeof(SceneBuildGroup)], &groups[k], sizeof(SceneBuildGroup));
-    const void* ptrs[8] = {s->b_keys, s->b_keys_alt, s->b_vals_tmp, s->b_vals, s->b_keys_hi, s->b_sort, c->stream, nullptr};
-    std::memcpy(&key[groups.size() * sizeof(SceneBuildGroup)], ptrs, sizeof ptrs);
-    const bool same = key == s->b_key;
-    if (!same) {
-        if (s->b_graph) {
-            HIPCHK(c, oi_ctx_quiesce(c));
-            (void)hipGraphExecDestroy(s->b_graph);
-            s->b_graph = nullptr;
-        }
-        s->b_key = key;
-        // a first occurrence runs as plain launches
-        for (const SceneBuildGroup& g : groups)
-            if ((r = enqueue_group(s, g)) != OIBVH_OK) return r;
-    } else {
-        if (!s->b_graph) {
-            hipGraph_t graph = nullptr;
-            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            oibvh_status rr = OIBVH_OK;
-            for (const SceneBuildGroup& g : groups)
-                if ((rr = enqueue_group(s, g)) != OIBVH_OK) break;
-            hipError_t e = hipStreamEndCapture(c->stream, &graph);
-            if (rr != OIBVH_OK) {
-                if (graph) (void)hipGraphDestroy(graph);
-                s->b_key.clear();
-                return rr;
-            }
-            if (e != hipSuccess) {
-                s->b_key.clear();
-                return oi_set_err(c, OIBVH_ERR_HIP, "scene build capture: %s", hipGetErrorString(e));
-            }
-            e = hipGraphInstantiate(&s->b_graph, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            if (e != hipSuccess) {
-                s->b_graph = nullptr;
-                s->b_key.clear();
-                return oi_set_err(c, OIBVH_ERR_HIP, "scene build instantiate: %s", hipGetErrorString(e));
-            }
-        }
-        HIPCHK(c, hipGraphLaunch(s->b_graph, c->stream));
-    }
+    for (const SceneBuildGroup& g : groups)
+        if ((r = enqueue_group(s, g)) != OIBVH_OK) return r;
     for (oibvh_tree* t : s->trees) t->built = true;
     return OIBVH_OK;
 }

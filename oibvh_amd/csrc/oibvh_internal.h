@@ -20,23 +20,32 @@ hipError_t oi_launch_mesh_box(hipStream_t, const oi_build_desc*, const uint2*, c
 uint32_t oi_morton_chunk();
 hipError_t oi_launch_mesh_box_inline(hipStream_t, const oi_build_desc*, uint32_t, const uint32_t*, uint32_t*,
                                      uint32_t);
+// hbits: digit width of the histograms the Morton launch accumulates (8, or
+// 10 for 30-bit keys)
 hipError_t oi_launch_morton_inline(hipStream_t, const oi_build_desc*, uint32_t, const uint32_t*, uint32_t*,
-                                   uint32_t*, uint32_t*, uint32_t);
+                                   uint32_t*, uint32_t*, uint32_t, uint32_t hbits = 8);
 hipError_t oi_launch_morton(hipStream_t, const oi_build_desc*, const uint2*, const oi_build_desc*, uint32_t,
-                            const float*, uint32_t*, uint32_t*, uint32_t*, uint32_t*);
+                            const float*, uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t hbits = 8);
 
-// sort scratch: digit histograms of up to 8 key bytes | per-pass tile
-// counters | two look-back buffers of 256 words per tile
-#define OI_SORT_HIST 2048u
+// sort scratch: digit histograms (8 x 256 or 3 x 1024 bins) | per-pass tile
+// counters | two look-back buffers of OI_SORT_LB words per tile (256 per
+// tile suffice for 8-bit digits)
+#define OI_SORT_HIST 3072u
 #define OI_SORT_CTRS 16u
 #define OI_SORT_HEAD (OI_SORT_HIST + OI_SORT_CTRS)
+#define OI_SORT_LB 1024u
+// 30-bit Morton keys of a tree or scene build: three passes of 10-bit digits
+#define OI_M30_PASSES 3
+#define OI_M30_BITS 10u
 uint32_t oi_sort_tile();
 hipError_t oi_launch_sort_pass(hipStream_t, const uint32_t*, const uint32_t*, uint32_t*, uint32_t*, uint32_t,
-                               uint32_t, const uint32_t*, uint32_t*, uint32_t*, uint32_t*, const uint32_t*);
+                               uint32_t, const uint32_t*, uint32_t*, uint32_t*, uint32_t*, const uint32_t*,
+                               uint32_t bits = 8);
 uint32_t oi_sort_segs_max();
+// bits: digit width, 8 or 10 (look-back buffers of 2^bits words per tile)
 hipError_t oi_launch_sort_pass_segs(hipStream_t, const uint32_t*, const uint32_t*, uint32_t*, uint32_t*, uint32_t,
                                     const uint32_t*, const uint32_t*, uint32_t, const uint32_t*, uint32_t, uint32_t*,
-                                    uint32_t*, uint32_t*, const uint32_t*);
+                                    uint32_t*, uint32_t*, const uint32_t*, uint32_t bits = 8);
 uint32_t oi_reduce_height(int, uint32_t);
 hipError_t oi_launch_reduce(hipStream_t, int, const oi_build_desc*, const uint2*, const oi_build_desc*, uint32_t,
                             const uint32_t*, uint32_t, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr,

@@ -20,15 +20,15 @@
 
 // layout of the scratch (32-bit words), each part 256-byte aligned:
 // keys | keys_alt | vals_tmp (F each) | sort head + look-back (OI_SORT_HEAD +
-// 512 words per sort tile) | Morton box + mesh-box partials (6 + 6 * 256)
+// 2 OI_SORT_LB words per sort tile) | Morton box + mesh-box partials (6 + 6 * 256)
 #define OI_NA_ALIGN 64u
 #define OI_NA_ROUND(x) (((x) + OI_NA_ALIGN - 1u) & ~(OI_NA_ALIGN - 1u))
-// the sort's tile is 4096 keys; the run-time check below refuses a build
+// the sort's tile is 8192 keys; the run-time check below refuses a build
 // whose parts do not fit (should that tile ever shrink)
-#define OI_NA_TILE_MIN 4096u
+#define OI_NA_TILE_MIN 8192u
 #define OI_NA_WORDS                                                                                  \
     (3u * OI_NA_ROUND(OIBVH_NOALLOC_MAX_FACES) +                                                     \
-     OI_NA_ROUND(OI_SORT_HEAD + 512u * ((OIBVH_NOALLOC_MAX_FACES + OI_NA_TILE_MIN - 1u) / OI_NA_TILE_MIN)) + \
+     OI_NA_ROUND(OI_SORT_HEAD + 2u * OI_SORT_LB * ((OIBVH_NOALLOC_MAX_FACES + OI_NA_TILE_MIN - 1u) / OI_NA_TILE_MIN)) + \
      OI_NA_ROUND(6u + 6u * 256u))
 
 static __device__ __align__(256) uint32_t scratch[OI_NA_WORDS];
@@ -70,7 +70,8 @@ extern "C" oibvh_status oibvh_build_noalloc(oibvh_ctx* ctx, const float* xyz, ui
         return oi_set_err(ctx, OIBVH_ERR_CAPACITY, "oibvh_build_noalloc: %u faces exceed the maximum of %u", F,
                           OIBVH_NOALLOC_MAX_FACES);
     const uint32_t tiles = (F + oi_sort_tile() - 1) / oi_sort_tile();
-    const uint64_t need = 3ull * OI_NA_ROUND(F) + OI_NA_ROUND(OI_SORT_HEAD + 512ull * tiles) + OI_NA_ROUND(6u + 6u * 256u);
+    const uint64_t need = 3ull * OI_NA_ROUND(F) + OI_NA_ROUND(OI_SORT_HEAD + 2ull * OI_SORT_LB * tiles) +
+                          OI_NA_ROUND(6u + 6u * 256u);
     if (need > OI_NA_WORDS)
         return oi_set_err(ctx, OIBVH_ERR_CAPACITY, "oibvh_build_noalloc: %llu scratch words needed, %u held",
                           (unsigned long long)need, (unsigned)OI_NA_WORDS);
@@ -91,7 +92,7 @@ extern "C" oibvh_status oibvh_build_noalloc(oibvh_ctx* ctx, const float* xyz, ui
     s.vals_tmp = w;
     w += OI_NA_ROUND(F);
     s.sort = w;
-    w += OI_NA_ROUND(OI_SORT_HEAD + 512u * tiles);
+    w += OI_NA_ROUND(OI_SORT_HEAD + 2u * OI_SORT_LB * tiles);
     s.keys_hi = nullptr;  // 30-bit keys (the reference's morton3D)
     oi_build_desc d{};
     d.pos = xyz;
