@@ -263,7 +263,8 @@ def run_batched(args, ctx, out):
         b.set_transforms_device(W.xbuf.at((i % W.nf) * W.stride))
 
     progress(f"batched: built in {t_build:.2f} s; autotuning")
-    tuned = autotune.tune_batch(ctx, b, stage, flags={"fp64_fallback": True}, reps=args.tune_reps, narrow=(0, 1))
+    tuned = autotune.tune_batch(ctx, b, stage, entries=(0, 1, 2, 3, 4), flags={"fp64_fallback": True},
+                                reps=args.tune_reps, narrow=(0, 1))
     progress(f"batched: tuned {tuned}")
     entry, expand = tuned["entry_level"], tuned["expand_levels"]
     # frames one after another, then overlapped (refit of frame f beside the
@@ -281,8 +282,9 @@ def run_batched(args, ctx, out):
         # tuned again with overlapped frames: a detect that leaves room on the
         # GPU for the refit beside it can win there while losing alone
         depth = max((int(F) - 1).bit_length() for F in W.faces)
-        ov = autotune.tune_batch(ctx, b, stage, flags={"fp64_fallback": True, "overlap": True},
-                                 blocks=(0, 512, 2048, 4096, 8192), refit_blocks=(0, 512, 1024, 2048),
+        ov = autotune.tune_batch(ctx, b, stage, entries=(0, 1, 2, 3, 4),
+                                 flags={"fp64_fallback": True, "overlap": True},
+                                 blocks=(0, 2048, 4096), refit_blocks=(0, 512, 1024, 2048),
                                  reps=args.tune_reps, narrow=(0, 1), depth=depth)
         progress(f"batched: tuned for overlapped frames {ov}")
         e_o, x_o = ov["entry_level"], ov["expand_levels"]
@@ -576,7 +578,9 @@ def run_instanced(args, ctx, W):
 
     base = ctx.get_tuning()
     overlap = not args.no_overlap
-    tuned = autotune.tune_batch(ctx, b, stage, flags={"fp64_fallback": True, "overlap": overlap},
+    tuned = autotune.tune_batch(ctx, b, stage, entries=(0, 1, 2, 3, 4),
+                                flags={"fp64_fallback": True, "overlap": overlap},
+                                blocks=(0, 2048, 4096) if overlap else (0, 4096, 8192),
                                 refit_blocks=(0, 512, 1024, 2048) if overlap else (0,), reps=args.tune_reps,
                                 narrow=(0, 1))
     ms, tm, st, _ = _time_batched(args, api, ctx, b, W, tuned["entry_level"], tuned["expand_levels"],
@@ -883,7 +887,7 @@ def run_self(args, ctx):
                        traverse_expand=tuned["expand_levels"])
         tuned = {"fixed": True, **tuned}
     else:
-        tuned = autotune.tune_batch(ctx, b, stage, entries=(4, 6), expands=(2, 3), blocks=(0,), leaves=(0,),
+        tuned = autotune.tune_batch(ctx, b, stage, entries=(3, 4, 5, 6), expands=(2, 3), blocks=(0,), leaves=(0,),
                                     local=(), tail=((32, 1), (64, 2), (128, 2)),
                                     flags={"self_collide": True, "overlap": True}, refit_blocks=(0, 1024), reps=4,
                                     depth=depth, narrow=(0, 1))
