@@ -299,8 +299,8 @@ OIBVH_API oibvh_status oibvh_scene_refit(oibvh_scene* scene);
  * current positions), then oibvh_scene_refit, then oibvh_scene_detect with
  * entry_level, expand_levels and flags.
  *
- * With OIBVH_FRAME_OVERLAP (scenes of up to four trees; others ignore it),
- * frames whose positions are known ahead (a prescribed animation) overlap:
+ * With OIBVH_FRAME_OVERLAP (scenes of two or more trees, or one tree in self
+ * mode), frames whose positions are known ahead (a prescribed animation) overlap:
  * each tree keeps a second node buffer, the refit writes the one the
  * previous frame's detect is not reading, and the detect runs on a second
  * stream of the context, so frame f + 1's refit runs while frame f's detect

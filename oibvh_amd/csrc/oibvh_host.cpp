@@ -115,8 +115,11 @@ struct oibvh_tree {
 struct oibvh_scene {
     oibvh_ctx* ctx = nullptr;
     std::vector<oibvh_tree*> trees;
-    oi_mesh_desc* d_desc = nullptr;
-    uint32_t desc_cap = 0;
+    // mesh tables of scenes of more than four trees, one per detect counter
+    // set: an overlapped frame writes its own while the previous frame's
+    // detect may still read the other
+    oi_mesh_desc* d_desc[OI_FRAME_DEPTH] = {};
+    uint32_t desc_cap[OI_FRAME_DEPTH] = {};
     std::vector<oi_mesh_desc> h_desc;  // host copies stay alive until the next detect
     oi_build_desc* d_geom = nullptr;   // per-tree geometry for the vertex export
     uint32_t geom_cap = 0;
@@ -692,7 +695,7 @@ extern "C" void oibvh_scene_destroy(oibvh_scene* s)
     if (!s) return;
     (void)hipSetDevice(s->ctx->device);
     (void)oi_ctx_quiesce(s->ctx);
-    dfree(s->d_desc);
+    for (uint32_t k = 0; k < OI_FRAME_DEPTH; k++) dfree(s->d_desc[k]);
     dfree(s->d_geom);
     dfree(s->d_xf);
     dfree(s->b_keys);
@@ -784,14 +787,20 @@ static oibvh_status prepare_scene_detect(oibvh_scene* s)
     if (acc > OI_IMPLICIT_LIMIT)
         return oi_set_err(c, OIBVH_ERR_INVALID, "scene too large: %llu implicit BVTT indices (limit %u)",
                           (unsigned long long)acc, OI_IMPLICIT_LIMIT);
-    if (n > 4) {  // small scenes pass the table in the kernel arguments
-        if (s->desc_cap < n) {
-            dfree(s->d_desc);
-            s->desc_cap = 0;
-            HIPCHK(c, dalloc(&s->d_desc, n));
-            s->desc_cap = n;
+    // small scenes pass the table in the kernel arguments; larger ones use the
+    // table of the detect's counter set, which no queued detect reads any
+    // more (an overlapped frame has made the stream wait for the last one
+    // that used this set)
+    const uint32_t ks = d.set;
+    if (n > 4) {
+        if (s->desc_cap[ks] < n) {
+            HIPCHK(c, oi_ctx_quiesce(c));
+            dfree(s->d_desc[ks]);
+            s->desc_cap[ks] = 0;
+            HIPCHK(c, dalloc(&s->d_desc[ks], n));
+            s->desc_cap[ks] = n;
         }
-        HIPCHK(c, hipMemcpyAsync(s->d_desc, desc.data(), n * sizeof(oi_mesh_desc), hipMemcpyHostToDevice,
+        HIPCHK(c, hipMemcpyAsync(s->d_desc[ks], desc.data(), n * sizeof(oi_mesh_desc), hipMemcpyHostToDevice,
                                  c->stream));
     }
     const bool self = (s->last_flags & OIBVH_DETECT_SELF) != 0;
@@ -802,7 +811,7 @@ static oibvh_status prepare_scene_detect(oibvh_scene* s)
         if (r != OIBVH_OK) return r;
         std::memcpy(s->front_key, key, sizeof key);
     }
-    d.tab = oi_tab_args{n > 4 ? s->d_desc : nullptr, desc.data(), n, nullptr, 0};
+    d.tab = oi_tab_args{n > 4 ? s->d_desc[ks] : nullptr, desc.data(), n, nullptr, 0};
     d.depth = depth;
     d.expand = c->trav_expand ? c->trav_expand : std::min(3u, std::max(1u, s->last_expand));
     d.schedule = c->trav_schedule;
@@ -1235,8 +1244,7 @@ extern "C" oibvh_status oibvh_scene_frame(oibvh_scene* s, const float* const* po
     const bool trivial = s->trees.empty() || (s->trees.size() == 1 && !(flags & OIBVH_DETECT_SELF));
     for (oibvh_tree* t : s->trees)
         if (!t->built) return oi_set_err(c, OIBVH_ERR_NOT_BUILT, "frame: tree not built");
-    if ((flags & OIBVH_FRAME_OVERLAP) && !trivial && s->trees.size() <= oi_inline_meshes())
-        return overlapped_frame(s, entry, expand, flags);
+    if ((flags & OIBVH_FRAME_OVERLAP) && !trivial) return overlapped_frame(s, entry, expand, flags);
     HIPCHK(c, oi_ctx_join(c));
     if (trivial) {
         oibvh_status r = scene_refit(s, nullptr, nullptr);

@@ -850,6 +850,47 @@ def test_detect_step_schedule(api, ctx, orc, expand, schedule):
         ctx.set_tuning(traverse_expand=0, traverse_schedule=0)
 
 
+@pytest.mark.parametrize("fp64", [False, True])
+def test_scene_frame_overlap_many_trees(api, ctx, orc, fp64):
+    """Overlapped frames on a scene of six trees (three deforming sphere
+    pairs apart from each other), whose mesh table lives in device memory,
+    one per detect counter set: frames queued back to back and read at the
+    end, and frames read one by one, equal the oracle's pairs and nodes."""
+    say(f"scene frame overlap, six trees, fp64={fp64}")
+    meshes, waves, shift = [], [], []
+    for j in range(3):
+        ms, ws = M.deforming_uv_pair(96, 49, seed=700 + 10 * j)
+        meshes += ms
+        waves += ws
+        shift += [np.array([0.0, 3.0 * j, 0.0], np.float32)] * 2
+    ks = (2, 7, 19, 40)
+    frames = [[(waves[m].frame(k) + shift[m]).astype(np.float32) for m in range(6)] for k in ks]
+    bufs = [[api.DeviceBuffer.from_array(x) for x in fr] for fr in frames]
+    trees = [gpu_tree(api, ctx, frames[0][m], meshes[m][1]) for m in range(6)]
+    sc = api.Scene(ctx)
+    for t in trees:
+        sc.addOibvhTree(t)
+
+    def want(k):
+        return orc.detect([orc.Tree(frames[k][m], meshes[m][1], order=trees[m].perm()) for m in range(6)],
+                          fp64=fp64)
+
+    for k in range(len(ks)):  # all queued, then the last one read
+        sc.frame([b.at(0) for b in bufs[k]], 2, 2, fp64_fallback=fp64, asynchronous=True, overlap=True)
+    w = want(len(ks) - 1)
+    assert len(w) > 0 and len(np.unique(w[:, 0])) == 3
+    assert np.array_equal(sc.getIntTriPairs(), w)
+    for m in range(6):
+        o = orc.Tree(frames[-1][m], meshes[m][1], order=trees[m].perm())
+        assert np.array_equal(bits(trees[m].nodes()), bits(o.nodes())), m
+    for k in (1, 0, 2):  # read one by one
+        sc.frame([b.at(0) for b in bufs[k]], 2, 2, fp64_fallback=fp64, overlap=True)
+        assert np.array_equal(sc.getIntTriPairs(), want(k)), k
+    st = sc.stats()
+    assert st["retries"] == 0 and st["async_errors"] == 0
+    sc.close()
+
+
 def test_detect_many_meshes(api, ctx, orc):
     say("detect 6 meshes")
     r = M.Rng(99)
@@ -865,8 +906,8 @@ def test_detect_many_meshes(api, ctx, orc):
     assert len(want) > 0
     p, _, trees = gpu_pairs(api, ctx, meshes, 1, 2)
     assert np.array_equal(p, want)
-    # frames with OIBVH_FRAME_OVERLAP on a scene of more than four trees run
-    # one after another, with the same pairs
+    # frames with OIBVH_FRAME_OVERLAP on a scene of more than four trees (its
+    # mesh table in device memory), with the same pairs
     sc = api.Scene(ctx)
     for t in trees:
         sc.addOibvhTree(t)
