@@ -213,11 +213,16 @@ def _time_batched(args, api, ctx, b, W, entry, expand, K, warm, overlap, timing_
     b.stats()
     timer = api.Timer(ctx)
     ctx.synchronize()
-    b.set_timing(timing_every)
+    # event samples start at the second timed frame: the first follows the
+    # synchronisation above, so with overlap its refit runs alone
+    if K == 1:
+        b.set_timing(timing_every)
     timer.start()
     for k in range(K):
         stage(warm + k)
         b.frame(entry, expand, fp64_fallback=True, asynchronous=True, overlap=overlap)
+        if k == 0 and K > 1:
+            b.set_timing(timing_every)
     # the last frame's detect runs on the library's side stream: wait for it
     # before the stop event (the host round trip, microseconds, is counted)
     ctx.synchronize()
@@ -335,7 +340,8 @@ def run_batched(args, ctx, out):
                "detect": {"kernels": "BVTT walk (k_bvtt_local / k_bvtt_step) + k_narrow<FP64 fallback>, one graph "
                                      "on the side stream", "avg_ms": detect_ms, "launches": tm["detects"],
                           "alg_bytes": detect_bytes,
-                          "gbps": detect_bytes / (detect_ms * 1e-3) / 1e9 if detect_ms else None}}
+                          "gbps": detect_bytes / (detect_ms * 1e-3) / 1e9 if detect_ms else None,
+                          "frac": detect_bytes / (detect_ms * 1e-3) / HBM_PEAK if detect_ms else None}}
 
     # ------------------------------------------------------------ parity
     verify = {}
@@ -898,13 +904,19 @@ def run_self(args, ctx):
     timer = api.Timer(ctx)
     k = max(1, args.self_frames)
     ctx.synchronize()
+    if k == 1:
+        b.set_timing(8)
     timer.start()
     for j in range(k):
         stage(j)
         b.frame(entry, expand, self_collide=True, asynchronous=True, overlap=True)
+        if j == 0 and k > 1:
+            b.set_timing(8)  # samples from the second frame, as in _time_batched
     ctx.synchronize()
     timer.stop()
     ms = timer.ms() / k
+    tm = b.timing()
+    b.set_timing(0)
     st = b.stats()
     pairs = b.pairs()
     tri = int(sum(f.shape[0] for _, f in base))
@@ -939,6 +951,17 @@ def run_self(args, ctx):
            "adjacent_skipped": st["adjacent"], "candidates": st["candidates"], "bvtt_tests": st["bvtt_tests"],
            "async_errors": st["async_errors"], "retries_last_frame": st["retries"],
            "bvtt_levels_last_frame": b.level_counts().tolist(), "autotune": tuned, "host_gen_s": t_gen}
+    # the walk's roofline: each tree's nodes read once by the broad phase,
+    # positions and faces once by the narrow phase, 16 B per hit written
+    det_ms = tm["detect_ms"] / max(1, tm["detects"])
+    det_bytes = sum(24 * node_count(int(f.shape[0])) + 12 * int(x.shape[0]) + 12 * int(f.shape[0])
+                    for x, f in base) + 16 * len(pairs)
+    res["kernels"] = {"refit_stage1_ms": tm["refit_ms"] / max(1, tm["refits"]),
+                      "detect": {"avg_ms": det_ms, "launches": tm["detects"], "alg_bytes": det_bytes,
+                                 "alg_bytes_unit": "24 N per tree (broad) + 12 V + 12 F per tree (narrow) + "
+                                                   "16 B per hit",
+                                 "gbps": det_bytes / (det_ms * 1e-3) / 1e9 if det_ms else None,
+                                 "frac": det_bytes / (det_ms * 1e-3) / HBM_PEAK if det_ms else None}}
     if plain_pairs is not None:
         res["pair_diffs_batched_vs_plain_full_size"] = _pair_diff(pairs, plain_pairs)
     if not args.no_verify:
