@@ -646,6 +646,10 @@ def run_deforming(args, ctx):
         for i in range(n):
             scene.frame(pos_arr[(1 + i0 + i) % n_frames], entry, expand, asynchronous=True, overlap=True)
 
+    def overlapped_at(i0, n, e, x):
+        for i in range(n):
+            scene.frame(pos_arr[(1 + i0 + i) % n_frames], e, x, asynchronous=True, overlap=True)
+
     base = ctx.get_tuning()
     if args.deforming_tuning:
         # fixed settings (counter runs, whose passes must see the same kernels)
@@ -653,12 +657,20 @@ def run_deforming(args, ctx):
         entry, expand = tuned.pop("entry_level"), tuned.pop("expand_levels")
         ctx.set_tuning(**{**tuned, "traverse_expand": expand})
         tuned = {"fixed": True, "entry_level": entry, "expand_levels": expand, **tuned}
-        tuned_rb = None
+        tuned_rb = tuned_alone = None
         scene.detectCollision(entry, expand)
         scene.getIntTriPairCount()
     else:
         progress("deforming pair: autotuning")
-        tuned = autotune.tune(ctx, scene, tune_frame)
+        tuned = tuned_alone = autotune.tune(ctx, scene, tune_frame)
+        # then the walk's shape again on overlapped frames, around that pick
+        keys = ("traverse_blocks", "reduce_leaves_per_thread")
+        cands = [tuned] + [{"entry_level": tuned["entry_level"], "expand_levels": x,
+                            **{k: tuned[k] for k in keys}, "traverse_local": lc, "traverse_local_levels": ll,
+                            "traverse_tail": tc, "traverse_tail_steps": 0, "narrow_separate": ns}
+                           for x in (2, 3) for lc, ll in ((0, 0), (128, 3), (256, 3), (256, 2))
+                           for tc in (0, 32, 64) for ns in (0, 1)]
+        tuned = autotune.tune_overlapped(ctx, scene, overlapped_at, cands)
         entry, expand = tuned["entry_level"], tuned["expand_levels"]
         ctx.set_tuning(traverse_expand=expand)
         scene.detectCollision(entry, expand)
@@ -755,7 +767,8 @@ def run_deforming(args, ctx):
            "rebuild_ms_both_trees": build_ms, "rebuild_ms_one_after_another": build_seq_ms,
            "rebuild_reps": len(build_v), "rebuild_ms_range": [float(min(build_v)), float(max(build_v))],
            "frame_overlap": True,
-           "entry_level": entry, "expand_levels": expand, "autotune": tuned, "autotune_refit_blocks": tuned_rb,
+           "entry_level": entry, "expand_levels": expand, "autotune": tuned, "autotune_alone": tuned_alone,
+           "autotune_refit_blocks": tuned_rb,
            "pair_diffs_last_frame_vs_oracle": diffs, "pairs_last_frame": int(len(pairs_last)),
            "async_errors": st["async_errors"], "retries_last_frame": st["retries"],
            "kernels": {"refit_stage1": {"kernel": rk, "avg_ms": refit_ms, "launches": tm["refits"],
@@ -784,7 +797,7 @@ def run_deforming(args, ctx):
 def run_precision(args, ctx):
     """Coplanar / edge-touching scenes: FP32 filter + FP64 re-check; every
     scene checked against the FP64 oracle."""
-    from oibvh_amd import api
+    from oibvh_amd import api, autotune
     from oibvh_amd import meshgen as M
 
     progress("precision: generating scenes")
@@ -797,14 +810,22 @@ def run_precision(args, ctx):
             b.add_mesh(si, x, f)
     b.build()
     tri = int(sum(f.shape[0] for ms in scenes for _, f in ms))
-    entry, expand = 4, 2
-    b.frame(entry, expand, fp64_fallback=True)
+    # static geometry: every frame refits and detects the same positions
+    overlap = not args.no_overlap
+    base = ctx.get_tuning()
+    tuned = autotune.tune_batch(ctx, b, lambda i: None, entries=(0, 1, 2, 3, 4),
+                                flags={"fp64_fallback": True, "overlap": overlap},
+                                blocks=(0, 2048, 4096), refit_blocks=(0, 512, 1024) if overlap else (0,),
+                                reps=args.tune_reps, narrow=(0, 1))
+    entry, expand = tuned["entry_level"], tuned["expand_levels"]
+    b.frame(entry, expand, fp64_fallback=True, overlap=overlap)
     timer = api.Timer(ctx)
     k = max(1, args.precision_frames)
     ctx.synchronize()
     timer.start()
     for _ in range(k):
-        b.frame(entry, expand, fp64_fallback=True, asynchronous=True)
+        b.frame(entry, expand, fp64_fallback=True, asynchronous=True, overlap=overlap)
+    ctx.synchronize()  # an overlapped frame's detect runs on the side stream
     timer.stop()
     ms = timer.ms() / k
     st = b.stats()
@@ -818,7 +839,8 @@ def run_precision(args, ctx):
            "candidates": st["candidates"], "fp64_candidates": st["fp64_candidates"],
            "fp64_candidate_rate": st["fp64_candidates"] / max(1, st["candidates"]),
            "fp64_axis_evaluations": st["fp64_rechecks"],
-           "async_errors": st["async_errors"], "retries_last_frame": st["retries"]}
+           "async_errors": st["async_errors"], "retries_last_frame": st["retries"],
+           "frames_overlapped": overlap, "autotune": tuned}
     progress(f"precision: {1000.0 / ms:.1f} frames/s; FP64 oracle on every scene")
     if not args.no_verify:
         import oracle
@@ -845,6 +867,7 @@ def run_precision(args, ctx):
                     "scenes_where_fp32_and_fp64_differ": changed, "oracle_s": time.time() - t0})
     res["host_gen_s"] = t_gen
     b.close()
+    ctx.set_tuning(**base)
     return res
 
 

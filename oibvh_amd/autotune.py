@@ -216,6 +216,40 @@ def tune_batch(ctx: api.Context, batch: api.Batch, before_frame, *, entries=(0, 
     return best
 
 
+def tune_overlapped(ctx: api.Context, scene: api.Scene, run, candidates, *, frames: int = 40,
+                    reps: int = 2) -> dict:
+    """Pick the detect's shape by the rate of overlapped frames (a detect
+    that is fastest alone can lose beside the refit of the next frame).
+    run(i0, n, entry, expand) queues n overlapped frames from frame index
+    i0; `candidates` are dicts of 'entry_level', 'expand_levels' and
+    oibvh_tuning fields (the entry level and levels per step go to
+    traverse_expand). Sets the best on ctx and returns it with its 'ms'."""
+    timer = api.Timer(ctx)
+    base = ctx.get_tuning()
+    best = None
+    for cand in candidates:
+        knobs = {k: v for k, v in cand.items() if k not in ("entry_level", "expand_levels", "ms")}
+        e, x = cand["entry_level"], cand["expand_levels"]
+        ctx.set_tuning(**{**base, **knobs, "traverse_expand": x})
+        run(0, 4, e, x)
+        scene.getIntTriPairCount()
+        ms = None
+        for r in range(reps):
+            ctx.synchronize()
+            timer.start()
+            run(4 + r * frames, frames, e, x)
+            ctx.synchronize()
+            timer.stop()
+            t = timer.ms() / frames
+            ms = t if ms is None else min(ms, t)
+        scene.getIntTriPairCount()
+        if best is None or ms < best["ms"]:
+            best = {**cand, "ms": ms}
+    knobs = {k: v for k, v in best.items() if k not in ("entry_level", "expand_levels", "ms")}
+    ctx.set_tuning(**{**base, **knobs, "traverse_expand": best["expand_levels"]})
+    return best
+
+
 def tune_refit_blocks(ctx: api.Context, scene: api.Scene, run, *, blocks=(0, 384, 448, 512, 640),
                       frames: int = 40, reps: int = 2) -> dict:
     """Pick refit_blocks (the cap on the scene refit's k_reduce_w workgroups)
