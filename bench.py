@@ -916,7 +916,9 @@ def run_self(args, ctx):
                        traverse_expand=tuned["expand_levels"])
         tuned = {"fixed": True, **tuned}
     else:
-        tuned = autotune.tune_batch(ctx, b, stage, entries=(3, 4, 5, 6), expands=(2, 3), blocks=(0,), leaves=(0,),
+        tuned = autotune.tune_batch(ctx, b, stage, entries=(3, 4, 5, 6), expands=(2, 3),
+                                    blocks=(0, 2048, 4096, 8192),
+                                    leaves=(0,),
                                     local=(), tail=((32, 1), (64, 2), (128, 2)),
                                     flags={"self_collide": True, "overlap": True}, refit_blocks=(0, 1024), reps=4,
                                     depth=depth, narrow=(0, 1))
