@@ -73,8 +73,9 @@ def test_noalloc_build_equals_tree_build(ctx):
 
 
 def test_noalloc_build_odd_sizes(ctx):
-    """Face counts around powers of two and the sort's 4096-key tiles."""
-    for F in (1, 2, 3, 255, 4095, 4096, 4097, 70001):
+    """Face counts around powers of two, the sort's 8192-key tiles and the
+    maximum (the scratch's sizing)."""
+    for F in (1, 2, 3, 255, 4095, 4096, 4097, 8191, 8192, 8193, 70001, L.NOALLOC_MAX_FACES):
         r = M.Rng(0x0B1B0100 + F)
         xyz = r.uniform(3 * (F + 2), -1, 1).reshape(-1, 3).astype(np.float32)
         faces = (np.arange(3 * F, dtype=np.uint32).reshape(F, 3) % np.uint32(xyz.shape[0]))
