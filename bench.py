@@ -19,9 +19,13 @@ inside the region). Beside it the same line carries:
                    oibvh_scene per scene, each frame per scene its matrices
                    (read in place) and one oibvh_scene_frame, driven from C++
                    (libbench_plain.so); the ratio batched / plain
-  roofline         the refit's stage-1 kernel (the largest single kernel of a
+  roofline         the refit's stage-1 kernel (the largest single launch of a
                    frame): algorithmic bytes per launch over its event-timed
                    duration in the timed region
+  frame_roofline   refit and detect algorithmic bytes over the frame time
+  kernels          the refit's and the detect's event-timed durations, the
+                   detect's (the walk's) algorithmic bytes and roofline
+                   fraction
   cpu_baseline     the oracle (CPU restatement of the reference CPU path, one
                    thread) on a stated sample of scenes, extrapolated
   instanced        the same scenes as rigid instances of 250 base shapes
