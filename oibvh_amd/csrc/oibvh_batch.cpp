@@ -463,10 +463,7 @@ static oibvh_status ensure_slot(oibvh_batch* b, uint32_t k)
 static oibvh_status wait_slot_readers(oibvh_batch* b, uint32_t k)
 {
     oibvh_ctx* c = b->ctx;
-    const uint64_t q = b->slot[k].read_seq;
-    // the ring event of detect q - 1 is recorded again OI_SIDE_RING detects
-    // later; a wait then covers a later point, which is still safe
-    if (q && c->side) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_ring[(q - 1) % OI_SIDE_RING], 0));
+    if (c->side[0]) HIPCHK(c, oi_ctx_wait_tickets(c, &b->slot[k].read_seq, 1));
     return OIBVH_OK;
 }
 
@@ -932,14 +929,16 @@ static oibvh_status overlapped_frame(oibvh_batch* b, uint32_t entry, uint32_t ex
     oibvh_batch::Graph& gd = b->g_detect[b->cur];
     key[7] |= seeded ? (1ull << 40) : 0ull;
     if ((r = captured(b, gd, key, [&]() { return oi_det_enqueue(&d, false, seeded); })) != OIBVH_OK) return r;
-    HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_main, 0));
+    const uint64_t q = c->side_seq;
+    hipStream_t side = oi_side_stream(c, q);
+    HIPCHK(c, hipStreamWaitEvent(side, c->ev_main, 0));
     hipEvent_t e0, e1;
     if ((r = detect_timing(b, &e0, &e1)) != OIBVH_OK) return r;
-    if (e0) HIPCHK(c, hipEventRecord(e0, c->side));
-    HIPCHK(c, hipGraphLaunch(gd.exec, c->side));
-    if (e1) HIPCHK(c, hipEventRecord(e1, c->side));
-    const uint64_t q = c->side_seq++;
-    HIPCHK(c, hipEventRecord(c->ev_ring[q % OI_SIDE_RING], c->side));
+    if (e0) HIPCHK(c, hipEventRecord(e0, side));
+    HIPCHK(c, hipGraphLaunch(gd.exec, side));
+    if (e1) HIPCHK(c, hipEventRecord(e1, side));
+    c->side_seq++;
+    HIPCHK(c, hipEventRecord(c->ev_ring[q % OI_SIDE_RING], side));
     b->slot[b->cur].read_seq = q + 1;
     c->side_pending = true;
     b->ov_active = true;

@@ -10,7 +10,7 @@
 // rewritten
 static oibvh_status idle(oibvh_ctx* c)
 {
-    if (c->side) HIPCHK(c, oi_ctx_quiesce(c));
+    if (c->side[0]) HIPCHK(c, oi_ctx_quiesce(c));
     return OIBVH_OK;
 }
 
@@ -45,14 +45,41 @@ oibvh_status oi_det_init(oi_detector* d, oibvh_ctx* c)
     return OIBVH_OK;
 }
 
+// the current set's working buffers to / from their per-set slots
+static void save_view(oi_detector* d)
+{
+    const uint32_t k = d->set;
+    d->set_front[k][0] = d->front[0];
+    d->set_front[k][1] = d->front[1];
+    d->set_front_cap[k] = d->front_cap;
+    d->set_cands[k] = d->cands;
+    d->set_cand_cap[k] = d->cand_cap;
+    d->set_hits[k] = d->hits;
+    d->set_hits_cap[k] = d->hits_cap;
+}
+
+static void load_view(oi_detector* d)
+{
+    const uint32_t k = d->set;
+    d->front[0] = d->set_front[k][0];
+    d->front[1] = d->set_front[k][1];
+    d->front_cap = d->set_front_cap[k];
+    d->cands = d->set_cands[k];
+    d->cand_cap = d->set_cand_cap[k];
+    d->hits = d->set_hits[k];
+    d->hits_cap = d->set_hits_cap[k];
+}
+
 void oi_det_use_set(oi_detector* d, uint32_t k)
 {
     if (k == d->set) return;
+    save_view(d);
     // the set left behind keeps an unchecked result until a check reads it;
     // the set taken holds none any more: an asynchronous seed folds its
     // errors into its sticky bits, and a synchronous frame checks first
     if (d->unchecked) d->set_unchecked[d->set] = true;
     d->set = k % OI_FRAME_DEPTH;
+    load_view(d);
     d->set_unchecked[d->set] = false;
     d->ctl = d->ctl_base + d->set;
     d->counts = d->counts_base ? d->counts_base + 256u * d->set : nullptr;
@@ -63,12 +90,17 @@ void oi_det_flip_set(oi_detector* d) { oi_det_use_set(d, (d->set + 1u) % OI_FRAM
 void oi_det_free(oi_detector* d)
 {
     dfree(d->d_init);
-    dfree(d->front[0]);
-    dfree(d->front[1]);
+    save_view(d);
+    for (uint32_t k = 0; k < OI_FRAME_DEPTH; k++) {
+        dfree(d->set_front[k][0]);
+        dfree(d->set_front[k][1]);
+        dfree(d->set_cands[k]);
+        dfree(d->set_hits[k]);
+        d->set_front_cap[k] = d->set_cand_cap[k] = d->set_hits_cap[k] = 0;
+    }
+    load_view(d);
     dfree(d->counts_base);
     d->counts = nullptr;
-    dfree(d->cands);
-    dfree(d->hits);
     dfree(d->d_sorted);
     dfree(d->sort_words);
     dfree(d->sort_vals);
@@ -151,7 +183,7 @@ oibvh_status oi_det_prepare(oi_detector* d)
             d->tail_from = T;
         }
     }
-    const uint32_t fcap = std::max(c->queue_cap, oi_np2(std::max(1u, d->n_front)));
+    const uint32_t fcap = std::max(std::max(c->queue_cap, d->want_front), oi_np2(std::max(1u, d->n_front)));
     if (d->front_cap < fcap) {
         oibvh_status ri = idle(c);
         if (ri != OIBVH_OK) return ri;
@@ -166,8 +198,10 @@ oibvh_status oi_det_prepare(oi_detector* d)
     oibvh_status r;
     if ((r = grow(c, d->counts_base, d->counts_cap, 256u * OI_FRAME_DEPTH, d->generation)) != OIBVH_OK) return r;
     d->counts = d->counts_base + 256u * d->set;
-    if ((r = grow(c, d->cands, d->cand_cap, c->cand_cap, d->generation)) != OIBVH_OK) return r;
-    if ((r = grow(c, d->hits, d->hits_cap, c->hits_cap, d->generation)) != OIBVH_OK) return r;
+    if ((r = grow(c, d->cands, d->cand_cap, std::max(c->cand_cap, d->want_cands), d->generation)) != OIBVH_OK)
+        return r;
+    if ((r = grow(c, d->hits, d->hits_cap, std::max(c->hits_cap, d->want_hits), d->generation)) != OIBVH_OK)
+        return r;
     return OIBVH_OK;
 }
 
@@ -348,16 +382,19 @@ oibvh_status oi_det_check(oi_detector* d)
             HIPCHK(c, dalloc(&d->front[0], need));
             HIPCHK(c, dalloc(&d->front[1], need));
             d->front_cap = (uint32_t)need;
+            d->want_front = d->front_cap;
             rerun = true;
         }
         if ((k.error & OI_ERR_CAND_OVERFLOW) || (!d->fused && k.cands > d->cand_cap)) {
             r = grow(c, d->cands, d->cand_cap, std::max<uint64_t>(k.cands, 2ull * d->cand_cap), d->generation);
             if (r != OIBVH_OK) return r;
+            d->want_cands = d->cand_cap;
             rerun = true;
         }
         if ((k.error & OI_ERR_HITS_OVERFLOW) || k.n_hits > d->hits_cap) {
             const uint64_t need = std::max<uint64_t>((uint64_t)k.n_hits + (k.n_hits >> 3), 2ull * d->hits_cap);
             if ((r = grow(c, d->hits, d->hits_cap, need, d->generation)) != OIBVH_OK) return r;
+            d->want_hits = d->hits_cap;
             rerun = true;
         }
         if (!rerun) {

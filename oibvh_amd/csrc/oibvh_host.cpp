@@ -12,39 +12,61 @@ static void ctx_free(oibvh_ctx* ctx)
 {
     (void)hipSetDevice(ctx->device);
     (void)oi_ctx_quiesce(ctx);
-    if (ctx->side) {
+    if (ctx->side[0]) {
         (void)hipEventDestroy(ctx->ev_main);
         for (hipEvent_t e : ctx->ev_ring) (void)hipEventDestroy(e);
-        (void)hipStreamDestroy(ctx->side);
+        for (hipStream_t x : ctx->side) (void)hipStreamDestroy(x);
     }
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+}
+
+hipError_t oi_ctx_wait_tickets(oibvh_ctx* c, const uint64_t* tickets, uint32_t n)
+{
+    uint64_t last[OI_SIDE_STREAMS] = {};
+    for (uint32_t i = 0; i < n; i++)
+        if (tickets[i]) {
+            uint64_t& l = last[(tickets[i] - 1) % OI_SIDE_STREAMS];
+            l = std::max(l, tickets[i]);
+        }
+    for (uint64_t t : last)
+        if (t) {
+            hipError_t e = hipStreamWaitEvent(c->stream, c->ev_ring[(t - 1) % OI_SIDE_RING], 0);
+            if (e != hipSuccess) return e;
+        }
+    return hipSuccess;
 }
 
 hipError_t oi_ctx_join(oibvh_ctx* ctx)
 {
     if (!ctx->side_pending) return hipSuccess;
     ctx->side_pending = false;
-    return hipStreamWaitEvent(ctx->stream, ctx->ev_ring[(ctx->side_seq - 1) % OI_SIDE_RING], 0);
+    // the last detect of each side stream
+    uint64_t t[OI_SIDE_STREAMS];
+    for (uint32_t k = 0; k < OI_SIDE_STREAMS; k++) t[k] = ctx->side_seq > k ? ctx->side_seq - k : 0;
+    return oi_ctx_wait_tickets(ctx, t, OI_SIDE_STREAMS);
 }
 
 hipError_t oi_ctx_quiesce(oibvh_ctx* ctx)
 {
     hipError_t e = oi_ctx_join(ctx);
-    if (e == hipSuccess && ctx->side) e = hipStreamSynchronize(ctx->side);
+    for (hipStream_t x : ctx->side)
+        if (e == hipSuccess && x) e = hipStreamSynchronize(x);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     return e;
 }
 
-// the side stream of overlapped frames and its events (first use)
+// the side streams of overlapped frames and their events (first use)
 oibvh_status oi_ctx_side_init(oibvh_ctx* c)
 {
-    if (c->side) return OIBVH_OK;
-    hipStream_t s = nullptr;
-    HIPCHK(c, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    if (c->side[0]) return OIBVH_OK;
+    hipStream_t s[OI_SIDE_STREAMS] = {};
+    hipError_t e = hipSuccess;
+    for (uint32_t k = 0; k < OI_SIDE_STREAMS && e == hipSuccess; k++)
+        e = hipStreamCreateWithFlags(&s[k], hipStreamNonBlocking);
     // recorded by kernel launches (hipExtLaunchKernel's stop event), so
     // created as ordinary timing events
-    hipError_t e = hipEventCreate(&c->ev_main);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev_main);
     for (uint32_t k = 0; k < OI_SIDE_RING && e == hipSuccess; k++) e = hipEventCreate(&c->ev_ring[k]);
     if (e != hipSuccess) {
         if (c->ev_main) (void)hipEventDestroy(c->ev_main);
@@ -53,10 +75,11 @@ oibvh_status oi_ctx_side_init(oibvh_ctx* c)
             if (x) (void)hipEventDestroy(x);
             x = nullptr;
         }
-        (void)hipStreamDestroy(s);
-        return oi_set_err(c, OIBVH_ERR_HIP, "side stream: %s", hipGetErrorString(e));
+        for (hipStream_t x : s)
+            if (x) (void)hipStreamDestroy(x);
+        return oi_set_err(c, OIBVH_ERR_HIP, "side streams: %s", hipGetErrorString(e));
     }
-    c->side = s;
+    std::copy(s, s + OI_SIDE_STREAMS, c->side);
     return OIBVH_OK;
 }
 
@@ -1152,11 +1175,12 @@ extern "C" oibvh_status oibvh_scene_refit(oibvh_scene* s)
 }
 
 // An overlapped frame (OIBVH_FRAME_OVERLAP): every tree is refit on the
-// context's stream into its other node buffer, once the last detect that read
-// that buffer has finished, and the detect runs on the side stream after the
-// refit. The refit of the next frame so runs while this frame's detect does;
-// the detects stay in order on the side stream, and any other call first
-// orders the context's stream after them (oi_ctx_join).
+// context's stream into its next node buffer, once the last detect that read
+// that buffer has finished, and the detect runs on a side stream after the
+// refit. The refit of the next frame so runs while this frame's detect does,
+// and consecutive detects, on alternate side streams with their own counter
+// sets, may run at once; any other call first orders the context's stream
+// after them (oi_ctx_join).
 static oibvh_status overlapped_frame(oibvh_scene* s, uint32_t entry, uint32_t expand, uint32_t flags)
 {
     oibvh_ctx* c = s->ctx;
@@ -1165,26 +1189,24 @@ static oibvh_status overlapped_frame(oibvh_scene* s, uint32_t entry, uint32_t ex
     oibvh_status r = oi_ctx_side_init(c);
     if (r != OIBVH_OK) return r;
     if (d.unchecked && !async && (r = oi_det_check(&d)) != OIBVH_OK) return r;
-    // wait for the latest side detect that read a buffer about to be
-    // rewritten or used the counter set about to be reset (the side stream
-    // runs its detects in order, so that covers the earlier ones); once
-    // OI_SIDE_RING detects later its ring event has been recorded again, and
-    // the wait is for that later point
+    // wait for the side detects that read a buffer about to be rewritten or
+    // used the counter set (and its working buffers) about to be reset
     bool set_unchecked[OI_FRAME_DEPTH];
     std::copy(d.set_unchecked, d.set_unchecked + OI_FRAME_DEPTH, set_unchecked);
+    const uint32_t prev_set = d.set;
     oi_det_flip_set(&d);
-    uint64_t need = d.set_seq[d.set];
+    uint64_t need[2] = {d.set_seq[d.set], 0};
     for (oibvh_tree* t : s->trees) {
         const uint32_t k = (t->cur + 1u) % OI_FRAME_DEPTH;
         if (!t->nbuf[k]) {
-            HIPCHK(c, oi_ctx_quiesce(c));  // the allocation is the first of this tree's second buffer
+            HIPCHK(c, oi_ctx_quiesce(c));  // the allocation is the first of this tree's buffer k
             HIPCHK(c, dalloc(&t->nbuf[k], 6ull * t->N));
         }
-        need = std::max(need, t->read_seq[k]);
+        need[1] = std::max(need[1], t->read_seq[k]);
         t->cur = k;
         t->nodes = t->nbuf[k];
     }
-    if (need) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_ring[(need - 1) % OI_SIDE_RING], 0));
+    HIPCHK(c, oi_ctx_wait_tickets(c, need, 2));
     s->last_entry = entry;
     s->last_expand = expand;
     s->last_flags = flags;
@@ -1196,9 +1218,7 @@ static oibvh_status overlapped_frame(oibvh_scene* s, uint32_t entry, uint32_t ex
             t->cur = (t->cur + OI_FRAME_DEPTH - 1u) % OI_FRAME_DEPTH;
             t->nodes = t->nbuf[t->cur];
         }
-        d.set = (d.set + OI_FRAME_DEPTH - 1u) % OI_FRAME_DEPTH;
-        d.ctl = d.ctl_base + d.set;
-        d.counts = d.counts_base ? d.counts_base + 256u * d.set : nullptr;
+        oi_det_use_set(&d, prev_set);
         std::copy(set_unchecked, set_unchecked + OI_FRAME_DEPTH, d.set_unchecked);
         return r;
     }
@@ -1206,20 +1226,21 @@ static oibvh_status overlapped_frame(oibvh_scene* s, uint32_t entry, uint32_t ex
     const oi_seed_args seed = oi_det_seed_args(&d, !async);
     bool seeded = false;
     if ((r = scene_refit(s, &seed, &seeded, c->ev_main)) != OIBVH_OK) return r;
-    HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_main, 0));
+    const uint64_t q = c->side_seq;
+    hipStream_t side = oi_side_stream(c, q);
+    HIPCHK(c, hipStreamWaitEvent(side, c->ev_main, 0));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (s->timing && (s->seq_detect++ % s->timing) == 0) {
         if ((r = timing_pair(c, s->ev_detect, s->n_detect, &e0, &e1)) != OIBVH_OK) return r;
-        HIPCHK(c, hipEventRecord(e0, c->side));
+        HIPCHK(c, hipEventRecord(e0, side));
     }
     // the detect's last launch records this detect's ring event
-    const uint64_t q = c->side_seq;
-    d.stream = c->side;
+    d.stream = side;
     r = oi_det_enqueue(&d, !async, seeded, c->ev_ring[q % OI_SIDE_RING]);
     d.stream = nullptr;
     if (r != OIBVH_OK) return r;
     c->side_seq++;
-    if (e1) HIPCHK(c, hipEventRecord(e1, c->side));
+    if (e1) HIPCHK(c, hipEventRecord(e1, side));
     for (oibvh_tree* t : s->trees) t->read_seq[t->cur] = q + 1;
     d.set_seq[d.set] = q + 1;
     c->side_pending = true;

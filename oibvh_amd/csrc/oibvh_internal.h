@@ -98,12 +98,12 @@ struct oibvh_ctx {
     uint32_t queue_cap = 1u << 22;
     uint32_t hits_cap = 1u << 20;
     uint32_t cand_cap = 1u << 20;
-    // overlapped frames (OIBVH_FRAME_OVERLAP) run their detects on `side`,
-    // created at default priority on first use
-    hipStream_t side = nullptr;
+    // overlapped frames (OIBVH_FRAME_OVERLAP) run their detects on the side
+    // streams, created at default priority on first use: detect q (side_seq
+    // counts them) runs on side[q % OI_SIDE_STREAMS] and ends with
+    // ev_ring[q % OI_SIDE_RING], so that consecutive detects may run at once
+    hipStream_t side[2] = {nullptr, nullptr};
     hipEvent_t ev_main = nullptr;  // hand-over from the context's stream (a frame's refit done)
-    // the side stream after its last OI_SIDE_RING detects: detect q ends
-    // with ev_ring[q % OI_SIDE_RING]; side_seq counts them
     hipEvent_t ev_ring[4] = {nullptr, nullptr, nullptr, nullptr};
     uint64_t side_seq = 0;
     bool side_pending = false;     // side work the context's stream has not waited for
@@ -111,7 +111,10 @@ struct oibvh_ctx {
     bool destroy_pending = false; // oibvh_ctx_destroy called while objects were live
 };
 
+#define OI_SIDE_STREAMS 2u
 #define OI_SIDE_RING 4u
+static_assert(OI_SIDE_RING % OI_SIDE_STREAMS == 0,
+              "a ring event is recorded again only on the side stream of the detect it marked");
 // node buffers per tree and counter sets per detector of overlapped frames:
 // frame f's refit rewrites the buffer and resets the set that frame
 // f - OI_FRAME_DEPTH's detect used, so it waits for that detect only and may
@@ -135,8 +138,16 @@ hipError_t oi_ctx_join(oibvh_ctx* ctx);
 // wait until both streams are idle (before memory they may use is freed or
 // rewritten from the host)
 hipError_t oi_ctx_quiesce(oibvh_ctx* ctx);
-// create the side stream and its events of overlapped frames (first use)
+// create the side streams and their events of overlapped frames (first use)
 oibvh_status oi_ctx_side_init(oibvh_ctx* c);
+// the side stream of detect q
+inline hipStream_t oi_side_stream(const oibvh_ctx* c, uint64_t q) { return c->side[q % OI_SIDE_STREAMS]; }
+// order the context's stream after side detects: tickets[i] is 1 + a detect
+// (0: none). Per side stream only the latest ticket is waited for (a stream
+// runs its detects in order); once OI_SIDE_RING detects later its ring event
+// has been recorded again, on the same stream, and the wait is for that
+// later point
+hipError_t oi_ctx_wait_tickets(oibvh_ctx* c, const uint64_t* tickets, uint32_t n);
 
 oibvh_status oi_set_err(oibvh_ctx* ctx, oibvh_status st, const char* fmt, ...);
 
@@ -225,6 +236,19 @@ struct oi_detector {
     uint32_t cand_cap = 0;
     uint4* hits = nullptr;
     uint32_t hits_cap = 0;
+    // front, cands and hits above are the current set's: every counter set
+    // has working buffers of its own (the detects of consecutive overlapped
+    // frames may run at once, on different side streams), kept here while
+    // another set is current (oi_det_use_set)
+    uint2* set_front[OI_FRAME_DEPTH][2] = {};
+    uint32_t set_front_cap[OI_FRAME_DEPTH] = {};
+    uint2* set_cands[OI_FRAME_DEPTH] = {};
+    uint32_t set_cand_cap[OI_FRAME_DEPTH] = {};
+    uint4* set_hits[OI_FRAME_DEPTH] = {};
+    uint32_t set_hits_cap[OI_FRAME_DEPTH] = {};
+    // sizes every set's buffers grow to when it is next prepared (raised by
+    // overflows, so that a set does not overflow where another already did)
+    uint32_t want_front = 0, want_cands = 0, want_hits = 0;
     oi_detect_ctl* ctl = nullptr;    // counters of the current set
     oi_detect_ctl* h_ctl = nullptr;  // pinned, OI_FRAME_DEPTH (the current set's, then the others read)
     std::vector<uint32_t> h_counts;
