@@ -119,9 +119,11 @@ static_assert(OI_SIDE_RING % OI_SIDE_STREAMS == 0,
 // frame f's refit rewrites the buffer and resets the set that frame
 // f - OI_FRAME_DEPTH's detect used, so it waits for that detect only and may
 // run beside the detects of the frames in between. Measured on the deforming
-// UV pair (2 x 1M triangles): 3 gave 75.9-80.5 us per frame against 73.9 us
-// with 2 (more node buffers in flight, and a refit beside more detect work)
-#define OI_FRAME_DEPTH 2u
+// UV pair (2 x 1M triangles, the 200-frame sequence with rebuilds), with two
+// side streams: 3 gave 63.3 us per frame against 66.5 us with 2. With one
+// side stream 3 had been the slower (75.9-80.5 us against 73.9 us): the
+// refit ran ahead into detects that could not start before the one ahead
+#define OI_FRAME_DEPTH 3u
 static_assert(OI_SIDE_RING >= OI_FRAME_DEPTH, "the ring must still hold the event a refit waits for");
 
 // device address of oibvh_build_noalloc's scratch array on `device`, taken
