@@ -301,10 +301,12 @@ OIBVH_API oibvh_status oibvh_scene_refit(oibvh_scene* scene);
  *
  * With OIBVH_FRAME_OVERLAP (scenes of two or more trees, or one tree in self
  * mode), frames whose positions are known ahead (a prescribed animation) overlap:
- * each tree keeps a second node buffer, the refit writes the one the
- * previous frame's detect is not reading, and the detect runs on a second
- * stream of the context, so frame f + 1's refit runs while frame f's detect
- * does. Results and errors are those of the same frames without the flag.
+ * each tree keeps three node buffers, frame f's refit writes the one frame
+ * f - 3's detect read, and the detects run on two more streams of the context
+ * (alternating by frame), so frame f + 1's refit, and its detect, run while
+ * frame f's detect does. The call first waits on the host for frame f - 3's
+ * detect to end, so the host stays at most three frames ahead of the GPU.
+ * Results and errors are those of the same frames without the flag.
  * The caller keeps frame f's position memory (and model matrices) unchanged
  * until frame f's results are read or a later call that waits for it; every
  * other entry point first waits for the overlapped detects queued before it

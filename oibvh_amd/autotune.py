@@ -141,8 +141,8 @@ def tune_batch(ctx: api.Context, batch: api.Batch, before_frame, *, entries=(0, 
         for i in range(reps):
             before_frame(i + 1)
             batch.frame(entry, expand, asynchronous=True, **flags)
-        # an overlapped frame's detect runs on the library's side stream:
-        # wait for it before the stop event
+        # an overlapped frame's detect runs on one of the library's side
+        # streams: wait for it before the stop event
         ctx.synchronize()
         timer.stop()
         ms = timer.ms() / reps

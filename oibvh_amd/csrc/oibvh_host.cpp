@@ -37,6 +37,22 @@ hipError_t oi_ctx_wait_tickets(oibvh_ctx* c, const uint64_t* tickets, uint32_t n
     return hipSuccess;
 }
 
+hipError_t oi_ctx_host_wait_tickets(oibvh_ctx* c, const uint64_t* tickets, uint32_t n)
+{
+    uint64_t last[OI_SIDE_STREAMS] = {};
+    for (uint32_t i = 0; i < n; i++)
+        if (tickets[i]) {
+            uint64_t& l = last[(tickets[i] - 1) % OI_SIDE_STREAMS];
+            l = std::max(l, tickets[i]);
+        }
+    for (uint64_t t : last)
+        if (t) {
+            hipError_t e = hipEventSynchronize(c->ev_ring[(t - 1) % OI_SIDE_RING]);
+            if (e != hipSuccess) return e;
+        }
+    return hipSuccess;
+}
+
 hipError_t oi_ctx_join(oibvh_ctx* ctx)
 {
     if (!ctx->side_pending) return hipSuccess;
@@ -1176,11 +1192,11 @@ extern "C" oibvh_status oibvh_scene_refit(oibvh_scene* s)
 
 // An overlapped frame (OIBVH_FRAME_OVERLAP): every tree is refit on the
 // context's stream into its next node buffer, once the last detect that read
-// that buffer has finished, and the detect runs on a side stream after the
-// refit. The refit of the next frame so runs while this frame's detect does,
-// and consecutive detects, on alternate side streams with their own counter
-// sets, may run at once; any other call first orders the context's stream
-// after them (oi_ctx_join).
+// that buffer has finished (a wait on the host), and the detect runs on a
+// side stream after the refit. The refit of the next frame so runs while
+// this frame's detect does, and consecutive detects, on alternate side
+// streams with their own counter sets, may run at once; any other call first
+// orders the context's stream after them (oi_ctx_join).
 static oibvh_status overlapped_frame(oibvh_scene* s, uint32_t entry, uint32_t expand, uint32_t flags)
 {
     oibvh_ctx* c = s->ctx;
@@ -1206,7 +1222,10 @@ static oibvh_status overlapped_frame(oibvh_scene* s, uint32_t entry, uint32_t ex
         t->cur = k;
         t->nodes = t->nbuf[k];
     }
-    HIPCHK(c, oi_ctx_wait_tickets(c, need, 2));
+    // on the host: a stream wait's barrier packet held the context's stream
+    // about 10 us per frame (2 x 1M faces), and that detect, OI_FRAME_DEPTH
+    // frames back, has usually ended; the host stays at most that far ahead
+    HIPCHK(c, oi_ctx_host_wait_tickets(c, need, 2));
     s->last_entry = entry;
     s->last_expand = expand;
     s->last_flags = flags;

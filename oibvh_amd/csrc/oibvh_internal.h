@@ -150,6 +150,9 @@ inline hipStream_t oi_side_stream(const oibvh_ctx* c, uint64_t q) { return c->si
 // has been recorded again, on the same stream, and the wait is for that
 // later point
 hipError_t oi_ctx_wait_tickets(oibvh_ctx* c, const uint64_t* tickets, uint32_t n);
+// the same wait on the host: returns once those detects have ended, so that
+// the context's stream needs no wait of its own
+hipError_t oi_ctx_host_wait_tickets(oibvh_ctx* c, const uint64_t* tickets, uint32_t n);
 
 oibvh_status oi_set_err(oibvh_ctx* ctx, oibvh_status st, const char* fmt, ...);
 
