@@ -592,20 +592,26 @@ def test_scene_frame_overlap(api, ctx, orc, self_collide, fp64):
             b.free()
 
 
-@pytest.mark.parametrize("overlap", [False, True])
-def test_async_frames_report_earlier_errors(api, ctx, orc, overlap):
-    """Frames queued without a host wait whose hit lists overflow: reading
-    the last one re-runs it with a larger buffer (the oracle's pairs) and
-    reports the overflow of the unchecked frames before it in async_errors,
-    with overlapped frames (a counter set per frame in flight) as without.
+@pytest.mark.parametrize("overlap,cap", [(False, "hits"), (True, "hits"), (True, "front")])
+def test_async_frames_report_earlier_errors(api, ctx, orc, overlap, cap):
+    """Frames queued without a host wait whose hit lists (or BVTT fronts)
+    overflow: reading the last one re-runs it with a larger buffer (the
+    oracle's pairs) and reports the overflow of the unchecked frames before
+    it in async_errors, with overlapped frames (a counter set and working
+    buffers per frame in flight, detects on alternating streams) as without.
     Errors are reported once: clean asynchronous frames after that check
-    report none."""
-    say(f"async frame errors overlap={overlap}")
+    report none, so every set's buffers have grown to the re-run's size."""
+    say(f"async frame errors overlap={overlap} cap={cap}")
     meshes = M.interpenetrating_icospheres(subdiv=4, seed=27, offset=1.2)
     want = orc.detect([orc.Tree(x, f) for x, f in meshes])
     assert len(want) > 16
     ctx2 = api.Context(0)
-    ctx2.set_tuning(hits_capacity=16)
+    if cap == "hits":
+        ctx2.set_tuning(hits_capacity=16)
+        bit = 8  # hit-list overflow
+    else:
+        ctx2.set_tuning(front_capacity=64)
+        bit = 1  # BVTT front overflow
     try:
         trees = [gpu_tree(api, ctx2, x, f) for x, f in meshes]
         sc = api.Scene(ctx2)
@@ -616,7 +622,7 @@ def test_async_frames_report_earlier_errors(api, ctx, orc, overlap):
         assert np.array_equal(sc.getIntTriPairs(), want)
         st = sc.stats()
         assert st["retries"] >= 1
-        assert st["async_errors"] & 8  # hit-list overflow of the earlier frames
+        assert st["async_errors"] & bit  # the overflow of the earlier frames
         for _ in range(5):  # the grown buffer holds these frames' hits
             sc.frame(None, 2, 2, asynchronous=True, overlap=overlap)
         assert np.array_equal(sc.getIntTriPairs(), want)
