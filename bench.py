@@ -342,7 +342,7 @@ def run_batched(args, ctx, out):
     kernels = {"refit_stage1": {"kernel": rk, "avg_ms": refit_ms, "launches": tm["refits"],
                                 "alg_bytes": refit_bytes},
                "detect": {"kernels": "BVTT walk (k_bvtt_local / k_bvtt_step) + k_narrow<FP64 fallback>, one graph "
-                                     "on the side stream", "avg_ms": detect_ms, "launches": tm["detects"],
+                                     "on a side stream", "avg_ms": detect_ms, "launches": tm["detects"],
                           "alg_bytes": detect_bytes,
                           "gbps": detect_bytes / (detect_ms * 1e-3) / 1e9 if detect_ms else None,
                           "frac": detect_bytes / (detect_ms * 1e-3) / HBM_PEAK if detect_ms else None}}
@@ -838,7 +838,7 @@ def run_precision(args, ctx):
     timer.start()
     for _ in range(k):
         b.frame(entry, expand, fp64_fallback=True, asynchronous=True, overlap=overlap)
-    ctx.synchronize()  # an overlapped frame's detect runs on the side stream
+    ctx.synchronize()  # an overlapped frame's detect runs on a side stream
     timer.stop()
     ms = timer.ms() / k
     st = b.stats()

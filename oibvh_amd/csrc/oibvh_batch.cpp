@@ -16,7 +16,7 @@
 // Overlapped frames (OIBVH_FRAME_OVERLAP) keep two slots of per-frame state:
 // node buffer, model-matrix table and descriptor tables. Frame f refits into
 // one slot on the context's stream while frame f - 1's detect reads the other
-// on the side stream; staging calls after an overlapped frame write the slot
+// on a side stream (the context's two alternate by frame); staging calls after an overlapped frame write the slot
 // the next one will refit, once the detect that last read it has finished.
 #include <new>
 
@@ -925,7 +925,7 @@ static oibvh_status overlapped_frame(oibvh_batch* b, uint32_t entry, uint32_t ex
     }
     if (r != OIBVH_OK) return r;
     HIPCHK(c, hipEventRecord(c->ev_main, c->stream));
-    // detect half, captured on the context's stream, run on the side stream
+    // detect half, captured on the context's stream, run on this detect's side stream
     oibvh_batch::Graph& gd = b->g_detect[b->cur];
     key[7] |= seeded ? (1ull << 40) : 0ull;
     if ((r = captured(b, gd, key, [&]() { return oi_det_enqueue(&d, false, seeded); })) != OIBVH_OK) return r;

@@ -313,7 +313,7 @@ oibvh_status oi_det_check(oi_detector* d)
 {
     oibvh_ctx* c = d->ctx;
     if (!d->unchecked) return OIBVH_OK;
-    HIPCHK(c, oi_ctx_join(c));  // an overlapped frame's detect ran on the side stream
+    HIPCHK(c, oi_ctx_join(c));  // an overlapped frame's detect ran on a side stream
     uint32_t retries = 0, earlier = 0;
     for (;;) {
         d->h_counts.resize(d->n_levels + 1);
