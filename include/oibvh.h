@@ -404,8 +404,8 @@ OIBVH_API oibvh_status oibvh_batch_detect(oibvh_batch* batch, uint32_t entry_lev
  * With OIBVH_FRAME_OVERLAP, frames whose inputs are known ahead (prescribed
  * motion) overlap: the batch keeps a second node buffer, matrix table and
  * descriptor tables; frame f refits into one (a graph on the context's
- * stream) while frame f - 1's detect reads the other (a graph on the side
- * stream). After an overlapped frame, oibvh_batch_set_transforms and
+ * stream) while frame f - 1's detect reads the other (a graph on one of the
+ * context's two side streams, which alternate by frame). After an overlapped frame, oibvh_batch_set_transforms and
  * oibvh_batch_set_positions with OIBVH_PTR_DEVICE_BORROW stage the next
  * overlapped frame's inputs without waiting for this frame's detect;
  * set_positions with a copy, and every other call, first wait for the queued
