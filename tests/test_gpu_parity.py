@@ -897,6 +897,62 @@ def test_scene_frame_overlap_many_trees(api, ctx, orc, fp64):
     sc.close()
 
 
+def test_scene_frame_overlap_two_scenes(api, ctx, orc):
+    """Two scenes on one context with their overlapped frames interleaved:
+    the detects share the context's side streams and its ring of events
+    (ten detects wrap the ring of four), each scene waits only for its own
+    earlier detects, and both scenes' pairs and nodes equal the oracle's,
+    for frames queued back to back and for frames read one by one."""
+    say("scene frame overlap, two scenes on one context")
+    sets = []
+    for j in range(2):
+        meshes, waves = M.deforming_uv_pair(112, 57, seed=900 + 10 * j)
+        ks = (1, 6, 11, 17, 29)
+        frames = [[waves[m].frame(k) for m in range(2)] for k in ks]
+        bufs = [[api.DeviceBuffer.from_array(x) for x in fr] for fr in frames]
+        trees = [gpu_tree(api, ctx, frames[0][m], meshes[m][1]) for m in range(2)]
+        sc = api.Scene(ctx)
+        for t in trees:
+            sc.addOibvhTree(t)
+        sets.append((meshes, frames, bufs, trees, sc))
+
+    def want(j, k):
+        meshes, frames, _, trees, _ = sets[j]
+        return orc.detect([orc.Tree(frames[k][m], meshes[m][1], order=trees[m].perm()) for m in range(2)])
+
+    def check_nodes(j, k):
+        meshes, frames, _, trees, _ = sets[j]
+        for m in range(2):
+            o = orc.Tree(frames[k][m], meshes[m][1], order=trees[m].perm())
+            assert np.array_equal(bits(trees[m].nodes()), bits(o.nodes())), (j, k, m)
+
+    n = len(sets[0][1])
+    for k in range(n):  # interleaved, all queued
+        for j in range(2):
+            sets[j][4].frame([b.at(0) for b in sets[j][2][k]], 3, 2, asynchronous=True, overlap=True)
+    for j in range(2):
+        assert np.array_equal(sets[j][4].getIntTriPairs(), want(j, n - 1)), j
+        check_nodes(j, n - 1)
+    for k in (3, 0, 2):  # interleaved, scene 1 queued, scene 0 read each time
+        sets[1][4].frame([b.at(0) for b in sets[1][2][k]], 3, 2, asynchronous=True, overlap=True)
+        sets[0][4].frame([b.at(0) for b in sets[0][2][k]], 3, 2, overlap=True)
+        assert np.array_equal(sets[0][4].getIntTriPairs(), want(0, k)), k
+    assert np.array_equal(sets[1][4].getIntTriPairs(), want(1, 2))
+    check_nodes(1, 2)
+    for j in range(2):
+        st = sets[j][4].stats()
+        assert st["retries"] == 0 and st["async_errors"] == 0
+    for meshes, frames, bufs, trees, sc in sets:
+        for t, x in zip(trees, frames[0]):
+            t.refit(x)  # owned memory again before the buffers go away
+        sc.close()
+    ctx.synchronize()
+    for meshes, frames, bufs, trees, sc in sets:
+        for bb in bufs:
+            for b in bb:
+                b.free()
+
+
 def test_detect_many_meshes(api, ctx, orc):
     say("detect 6 meshes")
     r = M.Rng(99)
