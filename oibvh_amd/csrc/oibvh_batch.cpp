@@ -16,8 +16,9 @@
 // Overlapped frames (OIBVH_FRAME_OVERLAP) keep two slots of per-frame state:
 // node buffer, model-matrix table and descriptor tables. Frame f refits into
 // one slot on the context's stream while frame f - 1's detect reads the other
-// on a side stream (the context's two alternate by frame); staging calls after an overlapped frame write the slot
-// the next one will refit, once the detect that last read it has finished.
+// on a side stream (the context's two alternate by frame); staging calls
+// after an overlapped frame write the slot the next one will refit, once the
+// detect that last read it has finished.
 #include <new>
 
 #include "oibvh_internal.h"
