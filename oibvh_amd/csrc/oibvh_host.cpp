@@ -21,14 +21,21 @@ static void ctx_free(oibvh_ctx* ctx)
     delete ctx;
 }
 
-hipError_t oi_ctx_wait_tickets(oibvh_ctx* c, const uint64_t* tickets, uint32_t n)
+// the latest of `tickets` (1 + a detect, 0: none) on each side stream
+static void latest_per_stream(const uint64_t* tickets, uint32_t n, uint64_t* last)
 {
-    uint64_t last[OI_SIDE_STREAMS] = {};
+    for (uint32_t k = 0; k < OI_SIDE_STREAMS; k++) last[k] = 0;
     for (uint32_t i = 0; i < n; i++)
         if (tickets[i]) {
             uint64_t& l = last[(tickets[i] - 1) % OI_SIDE_STREAMS];
             l = std::max(l, tickets[i]);
         }
+}
+
+hipError_t oi_ctx_wait_tickets(oibvh_ctx* c, const uint64_t* tickets, uint32_t n)
+{
+    uint64_t last[OI_SIDE_STREAMS];
+    latest_per_stream(tickets, n, last);
     for (uint64_t t : last)
         if (t) {
             hipError_t e = hipStreamWaitEvent(c->stream, c->ev_ring[(t - 1) % OI_SIDE_RING], 0);
@@ -39,12 +46,8 @@ hipError_t oi_ctx_wait_tickets(oibvh_ctx* c, const uint64_t* tickets, uint32_t n
 
 hipError_t oi_ctx_host_wait_tickets(oibvh_ctx* c, const uint64_t* tickets, uint32_t n)
 {
-    uint64_t last[OI_SIDE_STREAMS] = {};
-    for (uint32_t i = 0; i < n; i++)
-        if (tickets[i]) {
-            uint64_t& l = last[(tickets[i] - 1) % OI_SIDE_STREAMS];
-            l = std::max(l, tickets[i]);
-        }
+    uint64_t last[OI_SIDE_STREAMS];
+    latest_per_stream(tickets, n, last);
     for (uint64_t t : last)
         if (t) {
             hipError_t e = hipEventSynchronize(c->ev_ring[(t - 1) % OI_SIDE_RING]);
