@@ -76,6 +76,28 @@ hipError_t oi_launch_hits_sort_small(hipStream_t, const uint4*, uint32_t, const 
 uint32_t oi_hits_sort_small_max();
 }
 
+// side streams of overlapped frames' detects, and the ring of their events
+#ifndef OI_SIDE_STREAMS
+#define OI_SIDE_STREAMS 2u
+#endif
+#ifndef OI_SIDE_RING
+#define OI_SIDE_RING 4u
+#endif
+static_assert(OI_SIDE_RING % OI_SIDE_STREAMS == 0,
+              "a ring event is recorded again only on the side stream of the detect it marked");
+// node buffers per tree and counter sets per detector of overlapped frames:
+// frame f's refit rewrites the buffer and resets the set that frame
+// f - OI_FRAME_DEPTH's detect used, so it waits for that detect only and may
+// run beside the detects of the frames in between. Measured on the deforming
+// UV pair (2 x 1M triangles, the 200-frame sequence with rebuilds), with two
+// side streams: 3 gave 63.3 us per frame against 66.5 us with 2. With one
+// side stream 3 had been the slower (75.9-80.5 us against 73.9 us): the
+// refit ran ahead into detects that could not start before the one ahead
+#ifndef OI_FRAME_DEPTH
+#define OI_FRAME_DEPTH 3u
+#endif
+static_assert(OI_SIDE_RING >= OI_FRAME_DEPTH, "the ring must still hold the event a refit waits for");
+
 // ---------------------------------------------------------------------------
 struct oibvh_ctx {
     int device = 0;
@@ -102,29 +124,15 @@ struct oibvh_ctx {
     // streams, created at default priority on first use: detect q (side_seq
     // counts them) runs on side[q % OI_SIDE_STREAMS] and ends with
     // ev_ring[q % OI_SIDE_RING], so that consecutive detects may run at once
-    hipStream_t side[2] = {nullptr, nullptr};
+    hipStream_t side[OI_SIDE_STREAMS] = {};
     hipEvent_t ev_main = nullptr;  // hand-over from the context's stream (a frame's refit done)
-    hipEvent_t ev_ring[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev_ring[OI_SIDE_RING] = {};
     uint64_t side_seq = 0;
     bool side_pending = false;     // side work the context's stream has not waited for
     int live = 0;                 // trees, scenes and batches of this context not yet destroyed
     bool destroy_pending = false; // oibvh_ctx_destroy called while objects were live
 };
 
-#define OI_SIDE_STREAMS 2u
-#define OI_SIDE_RING 4u
-static_assert(OI_SIDE_RING % OI_SIDE_STREAMS == 0,
-              "a ring event is recorded again only on the side stream of the detect it marked");
-// node buffers per tree and counter sets per detector of overlapped frames:
-// frame f's refit rewrites the buffer and resets the set that frame
-// f - OI_FRAME_DEPTH's detect used, so it waits for that detect only and may
-// run beside the detects of the frames in between. Measured on the deforming
-// UV pair (2 x 1M triangles, the 200-frame sequence with rebuilds), with two
-// side streams: 3 gave 63.3 us per frame against 66.5 us with 2. With one
-// side stream 3 had been the slower (75.9-80.5 us against 73.9 us): the
-// refit ran ahead into detects that could not start before the one ahead
-#define OI_FRAME_DEPTH 3u
-static_assert(OI_SIDE_RING >= OI_FRAME_DEPTH, "the ring must still hold the event a refit waits for");
 
 // device address of oibvh_build_noalloc's scratch array on `device`, taken
 // once per device with that device current (oibvh_noalloc.hip)
