@@ -301,11 +301,11 @@ OIBVH_API oibvh_status oibvh_scene_refit(oibvh_scene* scene);
  *
  * With OIBVH_FRAME_OVERLAP (scenes of two or more trees, or one tree in self
  * mode), frames whose positions are known ahead (a prescribed animation) overlap:
- * each tree keeps three node buffers, frame f's refit writes the one frame
- * f - 3's detect read, and the detects run on two more streams of the context
+ * each tree keeps four node buffers, frame f's refit writes the one frame
+ * f - 4's detect read, and the detects run on two more streams of the context
  * (alternating by frame), so frame f + 1's refit, and its detect, run while
- * frame f's detect does. The call first waits on the host for frame f - 3's
- * detect to end, so the host stays at most three frames ahead of the GPU.
+ * frame f's detect does. The call first waits on the host for frame f - 4's
+ * detect to end, so the host stays at most four frames ahead of the GPU.
  * Results and errors are those of the same frames without the flag.
  * The caller keeps frame f's position memory (and model matrices) unchanged
  * until frame f's results are read or a later call that waits for it; every
