@@ -93,7 +93,7 @@ static_assert(OI_SIDE_RING % OI_SIDE_STREAMS == 0,
 // side streams: 3 gave 63.3 us per frame against 66.5 us with 2, and 4
 // (the host up to four frames ahead, which absorbs the stalls of the
 // rebuild frames) 56.7-58.5 against 59.2-62.9 us with 3, in three A/B runs;
-// 5 and 6 gave 58.0-58.7 us against 57.3-57.7 us with 4.
+// 5 and 6 gave 57.7-58.7 us against 57.3-57.7 us with 4 (no gain).
 // With one side stream 3 had been the slower (75.9-80.5 us against 73.9 us):
 // the refit ran ahead into detects that could not start before the one
 // ahead. Three side streams measured 61.5-61.6 us (depth 3 or 4)
