@@ -206,6 +206,19 @@ def tune_batch(ctx: api.Context, batch: api.Batch, before_frame, *, entries=(0, 
         if ms < best["ms"]:
             best = {**best, "refit_blocks": rb, "reduce_leaves_per_thread": lpt, "ms": ms}
     ctx.set_tuning(refit_blocks=best["refit_blocks"], reduce_leaves_per_thread=best["reduce_leaves_per_thread"])
+    # the reduce's subtree size again, now with the detect's shape it runs
+    # beside (it was first picked with the shape of entries[0], expands[0])
+    if len(leaves) > 1 and not best["refit_blocks"]:
+        ms0 = run(best["entry_level"], best["expand_levels"])
+        best = {**best, "ms": min(best["ms"], ms0)}
+        for lpt in leaves:
+            if lpt == best["reduce_leaves_per_thread"]:
+                continue
+            ctx.set_tuning(reduce_leaves_per_thread=lpt)
+            ms = run(best["entry_level"], best["expand_levels"])
+            if ms < best["ms"]:
+                best = {**best, "reduce_leaves_per_thread": lpt, "ms": ms}
+        ctx.set_tuning(reduce_leaves_per_thread=best["reduce_leaves_per_thread"])
     ctx.set_tuning(**{**base, "reduce_leaves_per_thread": best["reduce_leaves_per_thread"],
                       "traverse_blocks": best["traverse_blocks"], "traverse_expand": best["expand_levels"],
                       "traverse_schedule": best["traverse_schedule"],
